@@ -1,0 +1,176 @@
+/*
+ * pray_hip.h -- C ABI of libpray_hip.so, the MI355X (gfx950) tracer of the
+ * Nostrum Pray drop-in.
+ *
+ * The reference crosses to the GPU through C++ entry points that allocate,
+ * copy, launch and free per call and terminate the process on any error:
+ *   bool isComputableOnGPU(const Scene&)            include/path_trace.cu.h:11
+ *   void pathTraceOnGPURecursive(Image&, const Scene&) include/path_trace.cu.h:12
+ *   KDTreeGPU buildGPUTreeFromKDTree(...)           include/gpu_util.h:10
+ *   class KDTreeGPU / struct KDTreeGPUNode          include/kd_tree_gpu.h:11-19,
+ *                                                   include/kd_tree_gpu_structs.h:7-22
+ *   SpatialDatabase::intersect (closest hit)        include/spatial.h:15, src/kd_tree.cpp:142-279
+ *   shadow test through intersect                   src/pray.cpp:39-44, :123-133
+ *   void render(Image&, const Scene&)               include/pray.h:50
+ * This library replaces them with plain-C calls: plain pointers and sizes,
+ * int status codes (never exit()), device memory owned by an opaque context,
+ * scenes uploaded once and reused.
+ *
+ * Conventions
+ *   - Every function returns PRAY_OK (0) or a negative PRAY_ERR_* code; the
+ *     text of the last error is available from pray_last_error().
+ *   - Host-buffer calls are synchronous: results are on the host when they
+ *     return.  *_device calls take device pointers and a hipStream_t (passed
+ *     as void*, NULL = the context's stream) and return after enqueueing.
+ *   - One context per host thread; a context is not re-entrant.
+ *   - Floating-point results are computed in FP32 without contraction and are
+ *     bit-identical to the CPU restatement of the reference's algorithm (see
+ *     oracle/).  Ray directions are taken as given (callers pass directions
+ *     that are already normalised, as the reference's Ray constructor does,
+ *     geometry.h:84-87).
+ */
+#ifndef PRAY_HIP_H
+#define PRAY_HIP_H
+
+#include <stdint.h>
+
+#ifndef PRAY_API
+#define PRAY_API __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRAY_OK 0
+#define PRAY_ERR_INVALID (-1)     /* bad argument */
+#define PRAY_ERR_HIP (-2)         /* HIP runtime error */
+#define PRAY_ERR_NOMEM (-3)       /* device allocation failed */
+#define PRAY_ERR_NOSCENE (-4)     /* unknown scene id */
+#define PRAY_ERR_UNSUPPORTED (-5) /* e.g. tree deeper than the traversal stack */
+#define PRAY_ERR_NODEVICE (-6)    /* no GPU visible */
+
+/* Trace modes. */
+#define PRAY_TRACE_CLOSEST 0        /* lexicographic minimum of (t, triangle index) over t > 0 */
+#define PRAY_TRACE_ANY 1            /* occluded iff some hit has t > 0 and t*t <  limit2 (pray.cpp:42) */
+#define PRAY_TRACE_ANY_INCLUSIVE 2  /* occluded iff some hit has t > 0 and t*t <= limit2 (pray.cpp:129) */
+
+/* Triangle data layouts of an uploaded scene (tuning; results are identical). */
+#define PRAY_LAYOUT_INDEXED 0  /* leaf index arrays + per-field triangle arrays a/ab/ac */
+#define PRAY_LAYOUT_PACKED 1   /* per-leaf packed 48-byte triangle records */
+
+typedef struct pray_ctx pray_ctx;
+
+/* Opens `device` and creates a context with its own stream.  Replaces the
+ * implicit device setup of path_trace.cu:501-504. */
+PRAY_API int pray_init(int device, pray_ctx **out);
+PRAY_API void pray_destroy(pray_ctx *ctx);
+PRAY_API const char *pray_last_error(const pray_ctx *ctx);
+
+/* Memory feasibility for the flat layout (replaces isComputableOnGPU,
+ * path_trace.cu:557-583): *ok = 1 when the device has room for the scene
+ * (nodes, leaf refs, triangles incl. shading data) and n_rays rays with
+ * their hit records and image.  Does not allocate. */
+PRAY_API int pray_mem_ok(pray_ctx *ctx, uint64_t n_tris, uint64_t n_refs, uint64_t n_nodes, uint64_t n_rays, int *ok);
+
+/* Uploads a flattened tree once (replaces copyKDTree, path_trace.cu:480-492).
+ *   nodes      n_nodes 8-byte nodes as 2 uint32 words each (layout in
+ *              nostrum_pray_amd/host/gpu_layout.hpp); node 1 is the root.
+ *   leaf_refs  n_refs uint32: triangle index | 0x80000000 on a leaf's last entry
+ *   tri_a, tri_ab, tri_ac   3*n_tris floats each: vertex a and edges b-a, c-a
+ *   tri_normal, tri_color   3*n_tris floats each, may be NULL for trace-only scenes
+ *   tri_emit   n_tris bytes (1 = emissive), may be NULL
+ *   root_box   6 floats: min xyz, max xyz of the root cell
+ *   max_depth  root-to-leaf depth of the tree (<= 24)
+ * n_tris == 0 is a valid empty scene (every ray misses). */
+PRAY_API int pray_upload_scene(pray_ctx *ctx, const uint32_t *nodes, uint32_t n_nodes, const uint32_t *leaf_refs,
+                      uint32_t n_refs, const float *tri_a, const float *tri_ab, const float *tri_ac,
+                      const float *tri_normal, const float *tri_color, const uint8_t *tri_emit, uint32_t n_tris,
+                      const float *root_box, uint32_t max_depth, int *scene_id);
+PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
+
+/* Tuning of a scene's traversal (results never depend on it):
+ *   layout     PRAY_LAYOUT_*
+ *   stack_lds  LDS ring entries per lane (4, 8 or 16); deeper entries spill
+ *              to a per-lane area in HBM
+ *   leaf_ilp   triangle records loaded together per leaf step (1 or 2)
+ *   fetch_min  a persistent wave refills its finished lanes when at least
+ *              this many are idle (1..64)
+ * Pass -1 to keep a value. */
+PRAY_API int pray_set_tuning(pray_ctx *ctx, int scene_id, int layout, int stack_lds, int leaf_ilp, int fetch_min);
+PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *layout, int *stack_lds, int *leaf_ilp, int *fetch_min);
+
+/* Closest-hit or any-hit query for n rays (replaces SpatialDatabase::intersect,
+ * spatial.h:15, and the shadow tests of pray.cpp:39-44 / :123-133).
+ *   rays     n x 6 floats: origin xyz, direction xyz (normalised)
+ *   limit2   n floats (squared distance limit), ANY modes only, else NULL
+ *   det_eps  determinant cull: a triangle is skipped when det < det_eps
+ *            (the reference uses 1e-4, geometry.h:12)
+ *   hit_idx  n int32: triangle index, -1 on a miss
+ *   hit_t    n floats: distance, FLT_MAX on a miss (may be NULL) */
+PRAY_API int pray_trace(pray_ctx *ctx, int scene_id, const float *rays, uint64_t n, int mode, float det_eps,
+               const float *limit2, int32_t *hit_idx, float *hit_t);
+PRAY_API int pray_trace_device(pray_ctx *ctx, int scene_id, const float *d_rays, uint64_t n, int mode, float det_eps,
+                      const float *d_limit2, int32_t *d_hit_idx, float *d_hit_t, void *stream);
+
+/* Same traversal, additionally writing per-ray work counts: d_counts[2*i] =
+ * nodes visited (8-byte nodes; every child-pair fetch counts 2, the root 1),
+ * d_counts[2*i+1] = leaf triangle references tested. */
+PRAY_API int pray_trace_counts_device(pray_ctx *ctx, int scene_id, const float *d_rays, uint64_t n, int mode,
+                             float det_eps, const float *d_limit2, int32_t *d_hit_idx, float *d_hit_t,
+                             uint32_t *d_counts, void *stream);
+
+/* Incoherent ray stream: ray i has origin lo + span * U and a direction
+ * uniform on the sphere (z = 1 - 2u, phi = 2 pi u'), normalised as the
+ * reference normalises; U and u come from Philox4x32-10 keyed by `seed`
+ * with counters (i, 0, 0, 0) and (i, 1, 0, 0).  box6 = lo xyz, span xyz. */
+PRAY_API int pray_gen_rays_device(pray_ctx *ctx, uint64_t seed, const float *box6, uint64_t n, float *d_rays, void *stream);
+PRAY_API int pray_gen_rays(pray_ctx *ctx, uint64_t seed, const float *box6, uint64_t n, float *rays);
+
+/* Camera of the reference (scene_preprocessor.cpp:98-136) as 16 floats:
+ * position xyz, inverse view matrix rows 0..2 x cols 0..2 (9), angle,
+ * 1/width, 1/height, width/height.  Writes width*height primary rays
+ * (n x 6 floats, row-major pixels, top row first). */
+PRAY_API int pray_camera_rays(pray_ctx *ctx, const float *camera16, uint32_t width, uint32_t height, float *rays);
+
+/* Whitted ray tracing of one image (replaces rayTrace, pray.cpp:380-590):
+ * primary rays from the camera, closest hit, per-light Lambert with an any-hit
+ * shadow ray (inclusive limit, pray.cpp:129), clamp and round to RGB8.
+ *   lights    n_lights x 6 floats: position xyz, colour xyz
+ *   background 3 floats
+ *   rgb       width*height*3 bytes (top row first)
+ *   hit_idx   width*height int32 primary hits, may be NULL */
+PRAY_API int pray_render_rt(pray_ctx *ctx, int scene_id, const float *camera16, uint32_t width, uint32_t height,
+                   const float *lights, uint32_t n_lights, const float *background, float det_eps, uint8_t *rgb,
+                   int32_t *hit_idx);
+
+/* Wavefront path tracing (replaces pathTraceOnGPURecursive,
+ * path_trace.cu:494-555, NORMAL mode estimator pray.cpp:342-378):
+ * per sample pass, generate -> (extend -> shade)* over HBM ray queues,
+ * Philox4x32-10 keyed by (seed, pixel, sample, bounce); Russian roulette
+ * from bounce `rr_depth` on (rr_depth >= max_depth disables it).
+ *   hdr   width*height*3 floats: mean radiance before clamping, may be NULL
+ *   rgb   width*height*3 bytes: clamp(mean)*255+0.5 truncated, may be NULL */
+PRAY_API int pray_render_pt(pray_ctx *ctx, int scene_id, const float *camera16, uint32_t width, uint32_t height,
+                   uint32_t spp, uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *background,
+                   float det_eps, float *hdr, uint8_t *rgb);
+
+/* Device-memory and timing utilities (no reference counterpart: the
+ * reference allocates and copies inside every call).  They let a caller keep
+ * rays and hit records resident in HBM between calls and time work on the
+ * context's stream with HIP events. */
+PRAY_API int pray_device_alloc(pray_ctx *ctx, uint64_t bytes, void **d_ptr);
+PRAY_API int pray_device_free(pray_ctx *ctx, void *d_ptr);
+PRAY_API int pray_copy_to_device(pray_ctx *ctx, void *d_dst, const void *src, uint64_t bytes);
+PRAY_API int pray_copy_to_host(pray_ctx *ctx, void *dst, const void *d_src, uint64_t bytes);
+PRAY_API int pray_sync(pray_ctx *ctx);
+/* Records an event on the context's stream; slot 0..15. */
+PRAY_API int pray_event_record(pray_ctx *ctx, int slot);
+/* Milliseconds between two recorded events (synchronises on the second). */
+PRAY_API int pray_event_elapsed(pray_ctx *ctx, int slot0, int slot1, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRAY_HIP_H */

@@ -1,0 +1,95 @@
+/*
+ * pray_host.h -- C ABI of libpray_host.so, the CPU side of the Nostrum Pray
+ * drop-in: scene loading and procedural scenes, the median and SAH KD-tree
+ * builders (which stay on the CPU) and the flattening into the 8-byte-node
+ * device layout.  Callers: the Pray CLI, the Python tests and bench.py.
+ *
+ *   ph_scene_load_json   reference readSceneDescription (include/io.h:9)
+ *   ph_build             reference preprocessScene tree build
+ *                        (src/scene_preprocessor.cpp:52-70), KDTree::build
+ *                        (src/kd_tree.cpp:25-46), KDTreeSAH::build
+ *                        (src/kd_tree_sah.cpp:32-90)
+ *   ph_flatten           reference buildGPUTreeFromKDTree (include/gpu_util.h:10)
+ *   ph_camera / ph_prim_rays   reference calculatePrimRays
+ *                        (src/scene_preprocessor.cpp:98-136)
+ *   ph_upload            flatten + pray_upload_scene in one call
+ *
+ * Handles are opaque pointers; all functions returning int use 0 for success.
+ */
+#ifndef PRAY_HOST_H
+#define PRAY_HOST_H
+
+#include <stdint.h>
+
+#ifndef PRAY_API
+#define PRAY_API __attribute__((visibility("default")))
+#endif
+
+#include "pray_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ph_scene ph_scene;
+typedef struct ph_tree ph_tree;
+
+/* Procedural scenes (see nostrum_pray_amd/host/scene.hpp). */
+PRAY_API ph_scene *ph_scene_terrain(uint32_t grid, uint64_t seed);
+PRAY_API ph_scene *ph_scene_spheres(uint64_t seed);
+PRAY_API ph_scene *ph_scene_cornell(uint64_t seed, uint32_t target_tris);
+PRAY_API ph_scene *ph_scene_random(uint64_t seed, uint32_t n_tris);
+/* Builds a scene from raw triangles: abc = n x 9 floats (a, b, c), color =
+ * n x 3 floats, emit = n bytes (may be NULL). */
+PRAY_API ph_scene *ph_scene_from_tris(const float *abc, const float *color, const uint8_t *emit, uint64_t n);
+/* Reference scene JSON + OBJ; NULL on failure with the message in err. */
+PRAY_API ph_scene *ph_scene_load_json(const char *path, char *err, uint64_t err_len);
+PRAY_API int ph_scene_write_json(const ph_scene *s, const char *json_path);
+PRAY_API void ph_scene_free(ph_scene *s);
+
+PRAY_API uint64_t ph_scene_num_tris(const ph_scene *s);
+/* abc: n x 9 raw vertices, color n x 3, emit n (any may be NULL). */
+PRAY_API void ph_scene_get_tris(const ph_scene *s, float *abc, float *color, uint8_t *emit);
+/* Precomputed fields after initialiseTriangles: ab, ac, normal (n x 3 each;
+ * any may be NULL). */
+PRAY_API void ph_scene_get_derived(const ph_scene *s, float *ab, float *ac, float *normal);
+/* info[16]: mode, width, height, fov, max_depth, num_samples, n_lights,
+ * bg r g b, det_eps, rr_depth (-1 = off), seed, tree kind (-1 auto), 0, 0 */
+PRAY_API void ph_scene_info(const ph_scene *s, double *info);
+/* Overrides width/height/spp/depth (0 keeps a value; rr_depth < 0 keeps). */
+PRAY_API void ph_scene_set_render(ph_scene *s, uint32_t width, uint32_t height, uint32_t spp, uint32_t max_depth, int rr_depth);
+/* lights: n_lights x 6 floats (position, colour). */
+PRAY_API void ph_scene_get_lights(const ph_scene *s, float *lights);
+/* camera16 as taken by pray_render_* / pray_camera_rays. */
+PRAY_API void ph_camera(const ph_scene *s, float *camera16);
+/* width*height*6 floats; host restatement of calculatePrimRays. */
+PRAY_API void ph_prim_rays(const ph_scene *s, float *rays);
+
+/* Tree build over the scene's triangles: kind 0 = median, 1 = SAH,
+ * -1 = the reference's policy. */
+PRAY_API ph_tree *ph_build(ph_scene *s, int kind);
+/* Same with an explicit depth cap (the reference's is 20; deeper trees have
+ * smaller leaves and produce identical hits). */
+PRAY_API ph_tree *ph_build_depth(ph_scene *s, int kind, int max_depth);
+PRAY_API void ph_tree_free(ph_tree *t);
+/* stats[8]: kind, nnodes, maxdepth, leafNodes, emptyLeafNodes, triangleCount,
+ * flat node count, flat leaf refs */
+PRAY_API void ph_tree_stats(const ph_tree *t, uint64_t *stats);
+/* Depth-first export (left before right).  Call with NULL buffers to get
+ * sizes[2] = {nodes, leaf refs}; then node_i32 = nodes x 4
+ * {is_leaf, axis, first, count}, node_f32 = nodes x 7 {split, box[6]},
+ * tris = leaf refs. */
+PRAY_API void ph_tree_export(const ph_tree *t, uint64_t *sizes, int32_t *node_i32, float *node_f32, uint32_t *tris);
+/* Flat layout: sizes[4] = {n_nodes, n_refs, n_tris, max_depth}; buffers may
+ * be NULL. */
+PRAY_API void ph_flat_get(const ph_tree *t, uint64_t *sizes, uint32_t *nodes, uint32_t *refs, float *a, float *ab, float *ac,
+                 float *normal, float *color, uint8_t *emit, float *root_box);
+/* Uploads the flat layout into a pray context; with_shading = 0 skips
+ * normals/colours/emission. */
+PRAY_API int ph_upload(const ph_tree *t, pray_ctx *ctx, int with_shading, int *scene_id);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRAY_HOST_H */

@@ -1,0 +1,464 @@
+"""Python bindings of the Nostrum Pray MI355X tracer (ctypes over the C ABI).
+
+The product is native: ``lib/libpray_hip.so`` (gfx950 kernels behind
+``include/pray_hip.h``) and ``lib/libpray_host.so`` (scene loading, KD-tree
+builders and flattening behind ``include/pray_host.h``).  This module only
+loads those in-tree libraries and wraps their entry points for tests and the
+benchmark; it raises ``NativeLibraryMissing`` when they have not been built.
+There is no Python or CPU fallback for tracing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "libpray_hip.so")
+HOST_LIB = os.path.join(LIB_DIR, "libpray_host.so")
+CLI = os.path.join(HERE, "bin", "Pray")
+
+PRAY_OK = 0
+TRACE_CLOSEST, TRACE_ANY, TRACE_ANY_INCLUSIVE = 0, 1, 2
+LAYOUT_INDEXED, LAYOUT_PACKED = 0, 1
+REFERENCE_DET_EPS = 1e-4  # geometry.h:12
+
+ERRORS = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "NOSCENE", -5: "UNSUPPORTED", -6: "NODEVICE"}
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+class PrayError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pray error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+_hip = None
+_host = None
+
+_vp = C.c_void_p
+_u32, _u64, _i32, _f32 = C.c_uint32, C.c_uint64, C.c_int32, C.c_float
+
+
+def _p(a):
+    """Pointer to a numpy array's data (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(_vp)
+
+
+def libraries():
+    """Loads (once) and returns the (hip, host) ctypes libraries."""
+    global _hip, _host
+    if _hip is not None:
+        return _hip, _host
+    for path in (HIP_LIB, HOST_LIB):
+        if not os.path.exists(path):
+            raise NativeLibraryMissing(f"{path} is not built; run __graft_entry__.build()")
+    hip = C.CDLL(HIP_LIB, mode=C.RTLD_LOCAL)
+    host = C.CDLL(HOST_LIB, mode=C.RTLD_LOCAL)
+    sig = {
+        "pray_init": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+        "pray_destroy": (None, [_vp]),
+        "pray_last_error": (C.c_char_p, [_vp]),
+        "pray_mem_ok": (C.c_int, [_vp, _u64, _u64, _u64, _u64, C.POINTER(C.c_int)]),
+        "pray_upload_scene": (C.c_int, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32,
+                                        C.POINTER(C.c_int)]),
+        "pray_free_scene": (C.c_int, [_vp, C.c_int]),
+        "pray_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "pray_get_tuning": (C.c_int, [_vp, C.c_int] + [C.POINTER(C.c_int)] * 4),
+        "pray_trace": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp]),
+        "pray_trace_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp]),
+        "pray_trace_counts_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp, _vp]),
+        "pray_gen_rays_device": (C.c_int, [_vp, _u64, _vp, _u64, _vp, _vp]),
+        "pray_gen_rays": (C.c_int, [_vp, _u64, _vp, _u64, _vp]),
+        "pray_camera_rays": (C.c_int, [_vp, _vp, _u32, _u32, _vp]),
+        "pray_render_rt": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _vp, _u32, _vp, _f32, _vp, _vp]),
+        "pray_render_pt": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u64, _vp, _f32, _vp, _vp]),
+        "pray_device_alloc": (C.c_int, [_vp, _u64, C.POINTER(_vp)]),
+        "pray_device_free": (C.c_int, [_vp, _vp]),
+        "pray_copy_to_device": (C.c_int, [_vp, _vp, _vp, _u64]),
+        "pray_copy_to_host": (C.c_int, [_vp, _vp, _vp, _u64]),
+        "pray_sync": (C.c_int, [_vp]),
+        "pray_event_record": (C.c_int, [_vp, C.c_int]),
+        "pray_event_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_f32)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(hip, name)
+        f.restype, f.argtypes = res, args
+    hsig = {
+        "ph_scene_terrain": (_vp, [_u32, _u64]),
+        "ph_scene_spheres": (_vp, [_u64]),
+        "ph_scene_cornell": (_vp, [_u64, _u32]),
+        "ph_scene_random": (_vp, [_u64, _u32]),
+        "ph_scene_from_tris": (_vp, [_vp, _vp, _vp, _u64]),
+        "ph_scene_load_json": (_vp, [C.c_char_p, C.c_char_p, _u64]),
+        "ph_scene_write_json": (C.c_int, [_vp, C.c_char_p]),
+        "ph_scene_free": (None, [_vp]),
+        "ph_scene_num_tris": (_u64, [_vp]),
+        "ph_scene_get_tris": (None, [_vp, _vp, _vp, _vp]),
+        "ph_scene_get_derived": (None, [_vp, _vp, _vp, _vp]),
+        "ph_scene_info": (None, [_vp, _vp]),
+        "ph_scene_set_render": (None, [_vp, _u32, _u32, _u32, _u32, C.c_int]),
+        "ph_scene_get_lights": (None, [_vp, _vp]),
+        "ph_camera": (None, [_vp, _vp]),
+        "ph_prim_rays": (None, [_vp, _vp]),
+        "ph_build": (_vp, [_vp, C.c_int]),
+        "ph_build_depth": (_vp, [_vp, C.c_int, C.c_int]),
+        "ph_tree_free": (None, [_vp]),
+        "ph_tree_stats": (None, [_vp, _vp]),
+        "ph_tree_export": (None, [_vp, _vp, _vp, _vp, _vp]),
+        "ph_flat_get": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "ph_upload": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_int)]),
+    }
+    for name, (res, args) in hsig.items():
+        f = getattr(host, name)
+        f.restype, f.argtypes = res, args
+    _hip, _host = hip, host
+    return hip, host
+
+
+def exported_symbols():
+    """Names of every function declared in include/*.h, for export checks."""
+    hip, host = libraries()
+    return hip, host
+
+
+# ----------------------------------------------------------------- host ---
+
+class Scene:
+    """A host scene (triangles, camera, lights, render settings)."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise ValueError("scene construction failed")
+        self._h = handle
+        _, self._lib = libraries()
+
+    @classmethod
+    def terrain(cls, grid, seed):
+        return cls(libraries()[1].ph_scene_terrain(grid, seed))
+
+    @classmethod
+    def spheres(cls, seed):
+        return cls(libraries()[1].ph_scene_spheres(seed))
+
+    @classmethod
+    def cornell(cls, seed, target_tris=100_000):
+        return cls(libraries()[1].ph_scene_cornell(seed, target_tris))
+
+    @classmethod
+    def random(cls, seed, n_tris):
+        return cls(libraries()[1].ph_scene_random(seed, n_tris))
+
+    @classmethod
+    def from_triangles(cls, abc, color=None, emit=None):
+        abc = np.ascontiguousarray(abc, dtype=np.float32).reshape(-1, 9)
+        n = abc.shape[0]
+        color = None if color is None else np.ascontiguousarray(color, dtype=np.float32).reshape(n, 3)
+        emit = None if emit is None else np.ascontiguousarray(emit, dtype=np.uint8).reshape(n)
+        return cls(libraries()[1].ph_scene_from_tris(_p(abc), _p(color), _p(emit), n))
+
+    @classmethod
+    def load_json(cls, path):
+        err = C.create_string_buffer(1024)
+        h = libraries()[1].ph_scene_load_json(os.fsencode(path), err, 1024)
+        if not h:
+            raise ValueError(err.value.decode())
+        return cls(h)
+
+    def write_json(self, path):
+        if self._lib.ph_scene_write_json(self._h, os.fsencode(path)) != 0:
+            raise OSError(f"cannot write {path}")
+
+    def close(self):
+        if self._h:
+            self._lib.ph_scene_free(self._h)
+            self._h = None
+
+    __del__ = close
+
+    @property
+    def num_tris(self):
+        return int(self._lib.ph_scene_num_tris(self._h))
+
+    def triangles(self):
+        n = self.num_tris
+        abc = np.empty((n, 9), np.float32)
+        col = np.empty((n, 3), np.float32)
+        emit = np.empty(n, np.uint8)
+        self._lib.ph_scene_get_tris(self._h, _p(abc), _p(col), _p(emit))
+        return abc, col, emit
+
+    def derived(self):
+        n = self.num_tris
+        ab, ac, nn = (np.empty((n, 3), np.float32) for _ in range(3))
+        self._lib.ph_scene_get_derived(self._h, _p(ab), _p(ac), _p(nn))
+        return ab, ac, nn
+
+    def info(self):
+        o = np.zeros(16, np.float64)
+        self._lib.ph_scene_info(self._h, _p(o))
+        return {
+            "mode": int(o[0]), "width": int(o[1]), "height": int(o[2]), "fov": int(o[3]),
+            "max_depth": int(o[4]), "spp": int(o[5]), "n_lights": int(o[6]),
+            "background": o[7:10].astype(np.float32), "det_eps": np.float32(o[10]),
+            "rr_depth": int(o[11]), "seed": int(o[12]), "tree": int(o[13]),
+        }
+
+    def set_render(self, width=0, height=0, spp=0, max_depth=0, rr_depth=-1):
+        self._lib.ph_scene_set_render(self._h, width, height, spp, max_depth, rr_depth)
+
+    def lights(self):
+        n = self.info()["n_lights"]
+        out = np.empty((n, 6), np.float32)
+        if n:
+            self._lib.ph_scene_get_lights(self._h, _p(out))
+        return out
+
+    def camera(self):
+        out = np.empty(16, np.float32)
+        self._lib.ph_camera(self._h, _p(out))
+        return out
+
+    def prim_rays(self):
+        inf = self.info()
+        out = np.empty((inf["width"] * inf["height"], 6), np.float32)
+        self._lib.ph_prim_rays(self._h, _p(out))
+        return out
+
+    def build(self, kind=-1, max_depth=20):
+        """KD tree over the scene: 0 median, 1 SAH, -1 the reference policy;
+        max_depth 20 is the reference's depth cap."""
+        return Tree(self._lib.ph_build_depth(self._h, kind, max_depth))
+
+
+class Tree:
+    """A built KD tree and its flat device layout (host memory)."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise ValueError("tree build failed")
+        self._h = handle
+        _, self._lib = libraries()
+
+    def close(self):
+        if self._h:
+            self._lib.ph_tree_free(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def stats(self):
+        o = np.zeros(8, np.uint64)
+        self._lib.ph_tree_stats(self._h, _p(o))
+        keys = ["kind", "nnodes", "maxdepth", "leafNodes", "emptyLeafNodes", "triangleCount", "flat_nodes",
+                "flat_refs"]
+        return {k: int(v) for k, v in zip(keys, o)}
+
+    def export(self):
+        """Preorder export: (node_i32 [n,4], node_f32 [n,7], leaf tris)."""
+        sz = np.zeros(2, np.uint64)
+        self._lib.ph_tree_export(self._h, _p(sz), None, None, None)
+        ni = np.empty((int(sz[0]), 4), np.int32)
+        nf = np.empty((int(sz[0]), 7), np.float32)
+        tr = np.empty(int(sz[1]), np.uint32)
+        self._lib.ph_tree_export(self._h, None, _p(ni), _p(nf), _p(tr))
+        return ni, nf, tr
+
+    def box(self):
+        """Root cell (min xyz, max xyz)."""
+        b = np.empty(6, np.float32)
+        self._lib.ph_flat_get(self._h, *([None] * 9), _p(b))
+        return b
+
+    def flat(self):
+        sz = np.zeros(4, np.uint64)
+        self._lib.ph_flat_get(self._h, _p(sz), *([None] * 9))
+        nn, nr, nt, md = (int(x) for x in sz)
+        d = {
+            "nodes": np.empty(2 * nn, np.uint32), "refs": np.empty(nr, np.uint32),
+            "a": np.empty((nt, 3), np.float32), "ab": np.empty((nt, 3), np.float32),
+            "ac": np.empty((nt, 3), np.float32), "normal": np.empty((nt, 3), np.float32),
+            "color": np.empty((nt, 3), np.float32), "emit": np.empty(nt, np.uint8),
+            "box": np.empty(6, np.float32), "max_depth": md,
+        }
+        self._lib.ph_flat_get(self._h, None, _p(d["nodes"]), _p(d["refs"]), _p(d["a"]), _p(d["ab"]), _p(d["ac"]),
+                              _p(d["normal"]), _p(d["color"]), _p(d["emit"]), _p(d["box"]))
+        return d
+
+
+# ------------------------------------------------------------------ GPU ---
+
+class Context:
+    """A pray_ctx on one GPU."""
+
+    def __init__(self, device=0):
+        hip, host = libraries()
+        self._lib, self._host = hip, host
+        h = _vp()
+        rc = hip.pray_init(device, C.byref(h))
+        if rc != PRAY_OK:
+            raise PrayError(rc, "pray_init failed")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pray_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _check(self, rc):
+        if rc != PRAY_OK:
+            raise PrayError(rc, self._lib.pray_last_error(self._h).decode())
+
+    def upload(self, tree, shading=True):
+        sid = C.c_int(-1)
+        self._check(self._host.ph_upload(tree._h, self._h, 1 if shading else 0, C.byref(sid)))
+        return sid.value
+
+    def upload_flat(self, f, shading=True):
+        sid = C.c_int(-1)
+        nt = f["a"].shape[0]
+        self._check(self._lib.pray_upload_scene(
+            self._h, _p(f["nodes"]), len(f["nodes"]) // 2, _p(f["refs"]), len(f["refs"]), _p(f["a"]), _p(f["ab"]),
+            _p(f["ac"]), _p(f["normal"]) if shading else None, _p(f["color"]) if shading else None,
+            _p(f["emit"]) if shading else None, nt, _p(f["box"]), f["max_depth"], C.byref(sid)))
+        return sid.value
+
+    def free_scene(self, sid):
+        self._check(self._lib.pray_free_scene(self._h, sid))
+
+    # device memory and timing on the context's stream
+    def alloc(self, shape, dtype):
+        return DeviceBuffer(self, shape, dtype)
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, arr.shape, arr.dtype)
+        b.upload(arr)
+        return b
+
+    def sync(self):
+        self._check(self._lib.pray_sync(self._h))
+
+    def record(self, slot):
+        self._check(self._lib.pray_event_record(self._h, slot))
+
+    def elapsed_ms(self, slot0, slot1):
+        ms = _f32()
+        self._check(self._lib.pray_event_elapsed(self._h, slot0, slot1, C.byref(ms)))
+        return float(ms.value)
+
+    def set_tuning(self, sid, layout=-1, stack_lds=-1, leaf_ilp=-1, fetch_min=-1):
+        self._check(self._lib.pray_set_tuning(self._h, sid, layout, stack_lds, leaf_ilp, fetch_min))
+
+    def get_tuning(self, sid):
+        v = [C.c_int() for _ in range(4)]
+        self._check(self._lib.pray_get_tuning(self._h, sid, *[C.byref(x) for x in v]))
+        return {"layout": v[0].value, "stack_lds": v[1].value, "leaf_ilp": v[2].value, "fetch_min": v[3].value}
+
+    def trace(self, sid, rays, mode=TRACE_CLOSEST, det_eps=REFERENCE_DET_EPS, limit2=None):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        idx = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        lim = None if limit2 is None else np.ascontiguousarray(limit2, np.float32).reshape(n)
+        self._check(self._lib.pray_trace(self._h, sid, _p(rays), n, mode, det_eps, _p(lim), _p(idx), _p(t)))
+        return idx, t
+
+    def trace_device(self, sid, d_rays, n, d_idx, d_t, mode=TRACE_CLOSEST, det_eps=REFERENCE_DET_EPS,
+                     d_limit2=None, stream=None):
+        self._check(self._lib.pray_trace_device(self._h, sid, d_rays, n, mode, det_eps, d_limit2, d_idx, d_t, stream))
+
+    def trace_counts_device(self, sid, d_rays, n, d_idx, d_t, d_counts, mode=TRACE_CLOSEST,
+                            det_eps=REFERENCE_DET_EPS, d_limit2=None, stream=None):
+        self._check(self._lib.pray_trace_counts_device(self._h, sid, d_rays, n, mode, det_eps, d_limit2, d_idx, d_t,
+                                                       d_counts, stream))
+
+    def gen_rays(self, seed, box6, n):
+        box6 = np.ascontiguousarray(box6, np.float32)
+        out = np.empty((n, 6), np.float32)
+        self._check(self._lib.pray_gen_rays(self._h, seed, _p(box6), n, _p(out)))
+        return out
+
+    def gen_rays_device(self, seed, box6, n, d_rays, stream=None):
+        box6 = np.ascontiguousarray(box6, np.float32)
+        self._check(self._lib.pray_gen_rays_device(self._h, seed, _p(box6), n, d_rays, stream))
+
+    def camera_rays(self, cam16, width, height):
+        cam16 = np.ascontiguousarray(cam16, np.float32)
+        out = np.empty((width * height, 6), np.float32)
+        self._check(self._lib.pray_camera_rays(self._h, _p(cam16), width, height, _p(out)))
+        return out
+
+    def render_rt(self, sid, cam16, width, height, lights, background, det_eps=REFERENCE_DET_EPS, want_hits=False):
+        cam16 = np.ascontiguousarray(cam16, np.float32)
+        lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 6)
+        bg = np.ascontiguousarray(background, np.float32).reshape(3)
+        rgb = np.empty((height, width, 3), np.uint8)
+        hits = np.empty(width * height, np.int32) if want_hits else None
+        self._check(self._lib.pray_render_rt(self._h, sid, _p(cam16), width, height, _p(lights), lights.shape[0],
+                                             _p(bg), det_eps, _p(rgb), _p(hits)))
+        return (rgb, hits) if want_hits else rgb
+
+    def render_pt(self, sid, cam16, width, height, spp, max_depth, rr_depth, seed, background, det_eps,
+                  want_rgb=True):
+        cam16 = np.ascontiguousarray(cam16, np.float32)
+        bg = np.ascontiguousarray(background, np.float32).reshape(3)
+        hdr = np.empty((height, width, 3), np.float32)
+        rgb = np.empty((height, width, 3), np.uint8) if want_rgb else None
+        self._check(self._lib.pray_render_pt(self._h, sid, _p(cam16), width, height, spp, max_depth, rr_depth, seed,
+                                             _p(bg), det_eps, _p(hdr), _p(rgb)))
+        return hdr, rgb
+
+
+class DeviceBuffer:
+    """A device allocation owned through the C ABI (pray_device_alloc)."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        ctx._check(ctx._lib.pray_device_alloc(ctx._h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, self.dtype)
+        assert arr.nbytes == self.nbytes
+        self.ctx._check(self.ctx._lib.pray_copy_to_device(self.ctx._h, self.ptr, _p(arr), self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(self.ctx._lib.pray_copy_to_host(self.ctx._h, _p(out), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx._h:
+            self.ctx._lib.pray_device_free(self.ctx._h, self.ptr)
+        self.ptr = None
+
+    __del__ = free
+
+
+def scene_box_expanded(box6, frac=0.1):
+    """Scene AABB expanded by `frac` of each extent (half per side) as the
+    (lo xyz, span xyz) pair the ray generator takes."""
+    lo = np.asarray(box6[:3], np.float64)
+    hi = np.asarray(box6[3:], np.float64)
+    ext = hi - lo
+    lo2 = lo - 0.5 * frac * ext
+    span = ext * (1 + frac)
+    return np.concatenate([lo2, span]).astype(np.float32)

@@ -1,0 +1,633 @@
+// libpray_hip.so: gfx950 tracer behind the C ABI of include/pray_hip.h.
+//
+// Device layout of a scene (uploaded once, resident in HBM):
+//   nodes    uint2[n_nodes]          8-byte nodes (format in host/gpu_layout.hpp)
+//   refs     uint32[n_refs + 2]      leaf triangle-index runs, bit 31 = last
+//   a,ab,ac  float[3*n_tris] each    triangle vertex a and edges (per-field arrays)
+//   recs     float4[3*(n_refs + 2)]  packed per-leaf records {a, idx|last}, {ab, 0}, {ac, 0}
+//   normal, color float[3*n_tris], emit uint8[n_tris]   shading data (optional)
+// The two trailing pad entries keep the leaf look-ahead loads in bounds.
+// Traversal kernels: traverse.hpp.  Renderers: render.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pray_hip.h"
+#include "device_math.hpp"
+#include "traverse.hpp"
+
+using prt::DevScene;
+using prt::TraceArgs;
+using prt::WAVE;
+
+namespace {
+
+constexpr uint32_t MAX_TREE_DEPTH = 32;
+
+// Packed per-leaf records from the per-field arrays.
+__global__ void k_pack_records(const uint32_t *refs, uint32_t n_refs, const float *a, const float *ab,
+                               const float *ac, float4 *recs) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_refs) return;
+    const uint32_t ref = refs[k];
+    const size_t t = ref & ~prt::REF_LAST;
+    recs[3 * (size_t)k] = make_float4(a[3 * t], a[3 * t + 1], a[3 * t + 2], __uint_as_float(ref));
+    recs[3 * (size_t)k + 1] = make_float4(ab[3 * t], ab[3 * t + 1], ab[3 * t + 2], 0.f);
+    recs[3 * (size_t)k + 2] = make_float4(ac[3 * t], ac[3 * t + 1], ac[3 * t + 2], 0.f);
+}
+
+__global__ void k_fill_miss(uint32_t n, int32_t *out_idx, float *out_t, uint32_t *counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out_idx[i] = -1;
+    if (out_t) out_t[i] = FLT_MAX;
+    if (counts) counts[2 * (size_t)i] = counts[2 * (size_t)i + 1] = 0;
+}
+
+// Incoherent ray stream (see pray_gen_rays_device).
+__global__ void k_gen_rays(uint32_t k0, uint32_t k1, float lx, float ly, float lz, float sx, float sy, float sz,
+                           uint32_t n, float *rays) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 r = prd::philox4x32_10(make_uint4(i, 0u, 0u, 0u), k0, k1);
+    const uint4 q = prd::philox4x32_10(make_uint4(i, 1u, 0u, 0u), k0, k1);
+    const float ox = lx + sx * prd::u01(r.x);
+    const float oy = ly + sy * prd::u01(r.y);
+    const float oz = lz + sz * prd::u01(r.z);
+    const float z = 1.0f - 2.0f * prd::u01(r.w);
+    const float rxy = sqrtf(fmaxf(0.f, 1.0f - z * z));
+    float sp, cp;
+    prd::sincos2pi(prd::u01(q.x), sp, cp);
+    float dx = rxy * cp, dy = rxy * sp, dz = z;
+    prd::normalize3(dx, dy, dz);
+    float *o = rays + 6 * (size_t)i;
+    o[0] = ox, o[1] = oy, o[2] = oz, o[3] = dx, o[4] = dy, o[5] = dz;
+}
+
+// Primary rays of the reference camera (scene_preprocessor.cpp:123-135).
+struct Cam {
+    float px, py, pz;
+    float m[9];
+    float angle, invW, invH, aspect;
+};
+
+__device__ __forceinline__ void camera_ray(const Cam &c, uint32_t xp, uint32_t yp, float &dx, float &dy, float &dz) {
+    const float x = (2.0f * (((float)xp + 0.5f) * c.invW) - 1.0f) * c.angle;
+    const float y = (1.0f - 2.0f * (((float)yp + 0.5f) * c.invH)) * c.angle / c.aspect;
+    const float z = 1.0f;
+    dx = x * c.m[0] + y * c.m[3] + z * c.m[6];
+    dy = x * c.m[1] + y * c.m[4] + z * c.m[7];
+    dz = x * c.m[2] + y * c.m[5] + z * c.m[8];
+    prd::normalize3(dx, dy, dz);
+}
+
+__global__ void k_camera_rays(Cam c, uint32_t w, uint32_t h, float *rays) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    float dx, dy, dz;
+    camera_ray(c, i % w, i / w, dx, dy, dz);
+    float *o = rays + 6 * (size_t)i;
+    o[0] = c.px, o[1] = c.py, o[2] = c.pz, o[3] = dx, o[4] = dy, o[5] = dz;
+}
+
+// ------------------------------------------------------------ host side --
+
+struct Tuning {
+    int layout = PRAY_LAYOUT_PACKED;
+    int stack_lds = 8;
+    int leaf_ilp = 2;
+    int fetch_min = 32;
+};
+
+struct SceneRec {
+    DevScene d;
+    std::vector<void *> allocs;
+    Tuning tune;
+};
+
+}  // namespace
+
+struct pray_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<int, std::unique_ptr<SceneRec>> scenes;
+    int next_id = 1;
+    hipEvent_t events[16] = {};
+    uint32_t *work = nullptr;   // ray counters of the persistent kernels
+    uint4 *spill = nullptr;     // per-lane stack spill area
+    size_t spill_bytes = 0;
+    std::map<const void *, int> occupancy;  // kernel -> resident blocks per CU
+};
+
+namespace {
+
+int fail(pray_ctx *c, int code, const char *fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPC(ctx, expr)                                                                                       \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? PRAY_ERR_NOMEM : PRAY_ERR_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                                    \
+    } while (0)
+
+inline hipStream_t pick(pray_ctx *c, void *stream) { return stream ? (hipStream_t)stream : c->stream; }
+
+inline unsigned nblocks(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+struct DBuf {
+    void *p = nullptr;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+typedef void (*TraceKernel)(DevScene, TraceArgs);
+
+template <bool ANY, int LAYOUT>
+TraceKernel pick_kernel_kl(int K, int ilp) {
+    if (K == 4) return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 4, 2> : prt::k_trace<ANY, LAYOUT, 4, 1>;
+    if (K == 16) return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 16, 2> : prt::k_trace<ANY, LAYOUT, 16, 1>;
+    return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 8, 2> : prt::k_trace<ANY, LAYOUT, 8, 1>;
+}
+
+TraceKernel pick_kernel(bool any, int layout, int K, int ilp) {
+    if (any)
+        return layout == PRAY_LAYOUT_PACKED ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED>(K, ilp)
+                                            : pick_kernel_kl<true, PRAY_LAYOUT_INDEXED>(K, ilp);
+    return layout == PRAY_LAYOUT_PACKED ? pick_kernel_kl<false, PRAY_LAYOUT_PACKED>(K, ilp)
+                                        : pick_kernel_kl<false, PRAY_LAYOUT_INDEXED>(K, ilp);
+}
+
+// Resident 64-lane blocks per CU for a kernel (cached).
+int resident_blocks(pray_ctx *c, TraceKernel k) {
+    auto it = c->occupancy.find((const void *)k);
+    if (it != c->occupancy.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, WAVE, 0) != hipSuccess || nb <= 0) nb = 8;
+    c->occupancy[(const void *)k] = nb;
+    return nb;
+}
+
+int ensure_spill(pray_ctx *c, size_t bytes) {
+    if (bytes <= c->spill_bytes) return PRAY_OK;
+    if (c->spill) {
+        HIPC(c, hipStreamSynchronize(c->stream));
+        HIPC(c, hipFree(c->spill));
+        c->spill = nullptr;
+        c->spill_bytes = 0;
+    }
+    HIPC(c, hipMalloc((void **)&c->spill, bytes));
+    c->spill_bytes = bytes;
+    return PRAY_OK;
+}
+
+// Enqueues one trace over n rays (or *n_dev rays when n_dev is set, with n as
+// the launch bound).
+int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n, const uint32_t *n_dev, int mode,
+                  float eps, const float *lim2, int32_t *idx, float *t, uint32_t *counts, hipStream_t st) {
+    if (n == 0) return PRAY_OK;
+    if (n > 0x7FFFFFFFull) return fail(c, PRAY_ERR_INVALID, "at most 2^31-1 rays per call");
+    if (mode != PRAY_TRACE_CLOSEST && mode != PRAY_TRACE_ANY && mode != PRAY_TRACE_ANY_INCLUSIVE)
+        return fail(c, PRAY_ERR_INVALID, "unknown trace mode %d", mode);
+    if (mode != PRAY_TRACE_CLOSEST && !lim2) return fail(c, PRAY_ERR_INVALID, "any-hit modes need limit2");
+    if (sc.d.n_tris == 0) {
+        if (n_dev) return fail(c, PRAY_ERR_INVALID, "device-counted trace on an empty scene");
+        hipLaunchKernelGGL(k_fill_miss, dim3(nblocks(n, 256)), dim3(256), 0, st, (uint32_t)n, idx, t, counts);
+        HIPC(c, hipGetLastError());
+        return PRAY_OK;
+    }
+    const Tuning &tu = sc.tune;
+    TraceKernel k = pick_kernel(mode != PRAY_TRACE_CLOSEST, tu.layout, tu.stack_lds, tu.leaf_ilp);
+    const uint64_t want = (n + WAVE - 1) / WAVE;
+    const uint64_t fit = (uint64_t)c->num_cus * (uint64_t)resident_blocks(c, k);
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min(want, fit));
+    const uint32_t depth = std::max<uint32_t>(1, sc.d.max_depth);
+    int rc = ensure_spill(c, (size_t)grid * WAVE * depth * sizeof(uint4));
+    if (rc) return rc;
+    TraceArgs a;
+    a.rays = rays;
+    a.n = (uint32_t)n;
+    a.n_dev = n_dev;
+    a.limit2 = lim2;
+    a.eps = eps;
+    a.inclusive = mode == PRAY_TRACE_ANY_INCLUSIVE;
+    a.out_idx = idx;
+    a.out_t = t;
+    a.counts = counts;
+    a.work = c->work;
+    a.spill = c->spill;
+    a.spill_depth = depth;
+    a.fetch_min = (uint32_t)tu.fetch_min;
+    hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, c->work, 1u);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(WAVE), 0, st, sc.d, a);
+    HIPC(c, hipGetLastError());
+    return PRAY_OK;
+}
+
+SceneRec *find_scene(pray_ctx *c, int id) {
+    auto it = c->scenes.find(id);
+    return it == c->scenes.end() ? nullptr : it->second.get();
+}
+
+int device_alloc(pray_ctx *c, SceneRec &sc, void **p, size_t bytes) {
+    if (bytes == 0) {
+        *p = nullptr;
+        return PRAY_OK;
+    }
+    HIPC(c, hipMalloc(p, bytes));
+    sc.allocs.push_back(*p);
+    return PRAY_OK;
+}
+
+void free_scene_rec(SceneRec &sc) {
+    for (void *p : sc.allocs) (void)hipFree(p);
+    sc.allocs.clear();
+}
+
+// Structural check of an uploaded tree so that no kernel can read out of
+// bounds or loop: every inner node's child pair lies after it (acyclic) and
+// inside the array, every leaf run lies inside leaf_refs and ends with a
+// last-flagged entry, every referenced triangle exists, and the depth bound
+// holds.
+int validate_tree(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const uint32_t *refs, uint32_t n_refs,
+                  uint32_t n_tris, uint32_t max_depth) {
+    if (n_nodes < 2) return fail(c, PRAY_ERR_INVALID, "tree has no root");
+    for (uint32_t k = 0; k < n_refs; k++)
+        if ((refs[k] & ~prt::REF_LAST) >= n_tris) return fail(c, PRAY_ERR_INVALID, "leaf ref %u out of range", k);
+    if (!(refs[n_refs - 1] & prt::REF_LAST)) return fail(c, PRAY_ERR_INVALID, "last leaf run is unterminated");
+    std::vector<std::pair<uint32_t, uint32_t>> stack{{1u, 0u}};
+    uint64_t visited = 0;
+    while (!stack.empty()) {
+        const auto [i, d] = stack.back();
+        stack.pop_back();
+        if (++visited > n_nodes) return fail(c, PRAY_ERR_INVALID, "tree is not a tree");
+        if (d > max_depth) return fail(c, PRAY_ERR_INVALID, "tree deeper than the declared max_depth %u", max_depth);
+        const uint32_t w = nodes[2 * i + 1];
+        if ((w & 3u) == prt::LEAF_TAG) {
+            const uint32_t k = w >> 2;
+            if (k == prt::LEAF_EMPTY) continue;
+            if (k >= n_refs) return fail(c, PRAY_ERR_INVALID, "leaf of node %u starts outside leaf_refs", i);
+        } else {
+            const uint32_t lc = w >> 2;
+            if ((lc & 1u) || lc <= i || (uint64_t)lc + 1 >= n_nodes)
+                return fail(c, PRAY_ERR_INVALID, "node %u has a bad child index %u", i, lc);
+            stack.push_back({lc, d + 1});
+            stack.push_back({lc + 1, d + 1});
+        }
+    }
+    return PRAY_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI --
+
+extern "C" {
+
+int pray_init(int device, pray_ctx **out) {
+    if (!out) return PRAY_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return PRAY_ERR_NODEVICE;
+    if (device < 0 || device >= n) return PRAY_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return PRAY_ERR_HIP;
+    auto *c = new pray_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void **)&c->work, 64 * sizeof(uint32_t)) != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return PRAY_ERR_HIP;
+    }
+    c->num_cus = prop.multiProcessorCount;
+    *out = c;
+    return PRAY_OK;
+}
+
+void pray_destroy(pray_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->scenes) free_scene_rec(*kv.second);
+    c->scenes.clear();
+    for (hipEvent_t &e : c->events)
+        if (e) (void)hipEventDestroy(e);
+    if (c->work) (void)hipFree(c->work);
+    if (c->spill) (void)hipFree(c->spill);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *pray_last_error(const pray_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int pray_mem_ok(pray_ctx *c, uint64_t n_tris, uint64_t n_refs, uint64_t n_nodes, uint64_t n_rays, int *ok) {
+    if (!c || !ok) return fail(c, PRAY_ERR_INVALID, "null argument");
+    HIPC(c, hipSetDevice(c->device));
+    size_t fr = 0, tot = 0;
+    HIPC(c, hipMemGetInfo(&fr, &tot));
+    const uint64_t need = n_nodes * 8 + n_refs * (4 + 48) + n_tris * (36 + 24 + 1) + n_rays * (24 + 8 + 3 + 12 * 4);
+    *ok = (uint64_t)fr >= need ? 1 : 0;
+    return PRAY_OK;
+}
+
+int pray_upload_scene(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const uint32_t *leaf_refs,
+                      uint32_t n_refs, const float *tri_a, const float *tri_ab, const float *tri_ac,
+                      const float *tri_normal, const float *tri_color, const uint8_t *tri_emit, uint32_t n_tris,
+                      const float *root_box, uint32_t max_depth, int *scene_id) {
+    if (!c || !scene_id) return fail(c, PRAY_ERR_INVALID, "null argument");
+    *scene_id = -1;
+    if (n_tris > 0 && (!nodes || n_nodes < 2 || !leaf_refs || n_refs == 0 || !tri_a || !tri_ab || !tri_ac || !root_box))
+        return fail(c, PRAY_ERR_INVALID, "incomplete scene description");
+    if (max_depth > MAX_TREE_DEPTH)
+        return fail(c, PRAY_ERR_UNSUPPORTED, "tree depth %u exceeds %u", max_depth, MAX_TREE_DEPTH);
+    if (n_tris > 0) {
+        const int v = validate_tree(c, nodes, n_nodes, leaf_refs, n_refs, n_tris, max_depth);
+        if (v) return v;
+    }
+    HIPC(c, hipSetDevice(c->device));
+    auto sc = std::make_unique<SceneRec>();
+    DevScene &d = sc->d;
+    d.n_tris = n_tris;
+    d.n_nodes = n_nodes;
+    d.n_refs = n_refs;
+    d.max_depth = max_depth;
+    if (root_box) std::memcpy(d.box, root_box, sizeof d.box);
+    int rc = PRAY_OK;
+    auto up = [&](const void *src, size_t bytes, size_t alloc_bytes, void **dst) -> int {
+        *dst = nullptr;
+        if (!src || bytes == 0) return PRAY_OK;
+        int r = device_alloc(c, *sc, dst, alloc_bytes);
+        if (r) return r;
+        HIPC(c, hipMemsetAsync(*dst, 0, alloc_bytes, c->stream));
+        HIPC(c, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+        return PRAY_OK;
+    };
+    void *p = nullptr;
+    if (n_tris > 0) {
+        if ((rc = up(nodes, (size_t)n_nodes * 8, (size_t)n_nodes * 8, &p))) goto bad;
+        d.nodes = (const uint2 *)p;
+        if ((rc = up(leaf_refs, (size_t)n_refs * 4, (size_t)(n_refs + 2) * 4, &p))) goto bad;
+        d.refs = (const uint32_t *)p;
+        if ((rc = up(tri_a, (size_t)n_tris * 12, (size_t)n_tris * 12, &p))) goto bad;
+        d.a = (const float *)p;
+        if ((rc = up(tri_ab, (size_t)n_tris * 12, (size_t)n_tris * 12, &p))) goto bad;
+        d.ab = (const float *)p;
+        if ((rc = up(tri_ac, (size_t)n_tris * 12, (size_t)n_tris * 12, &p))) goto bad;
+        d.ac = (const float *)p;
+        if ((rc = up(tri_normal, (size_t)n_tris * 12, (size_t)n_tris * 12, &p))) goto bad;
+        d.normal = (const float *)p;
+        if ((rc = up(tri_color, (size_t)n_tris * 12, (size_t)n_tris * 12, &p))) goto bad;
+        d.color = (const float *)p;
+        if ((rc = up(tri_emit, (size_t)n_tris, (size_t)n_tris, &p))) goto bad;
+        d.emit = (const uint8_t *)p;
+        if ((rc = device_alloc(c, *sc, &p, (size_t)(n_refs + 2) * 48))) goto bad;
+        if (hipMemsetAsync(p, 0, (size_t)(n_refs + 2) * 48, c->stream) != hipSuccess) {
+            rc = fail(c, PRAY_ERR_HIP, "record clear failed");
+            goto bad;
+        }
+        d.recs = (const float4 *)p;
+        hipLaunchKernelGGL(k_pack_records, dim3(nblocks(n_refs, 256)), dim3(256), 0, c->stream, d.refs, n_refs, d.a,
+                           d.ab, d.ac, (float4 *)p);
+        if (hipGetLastError() != hipSuccess) {
+            rc = fail(c, PRAY_ERR_HIP, "record packing launch failed");
+            goto bad;
+        }
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        rc = fail(c, PRAY_ERR_HIP, "scene upload failed");
+        goto bad;
+    }
+    *scene_id = c->next_id++;
+    c->scenes[*scene_id] = std::move(sc);
+    return PRAY_OK;
+bad:
+    (void)hipStreamSynchronize(c->stream);
+    free_scene_rec(*sc);
+    return rc;
+}
+
+int pray_free_scene(pray_ctx *c, int scene_id) {
+    if (!c) return PRAY_ERR_INVALID;
+    auto it = c->scenes.find(scene_id);
+    if (it == c->scenes.end()) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    free_scene_rec(*it->second);
+    c->scenes.erase(it);
+    return PRAY_OK;
+}
+
+int pray_set_tuning(pray_ctx *c, int scene_id, int layout, int stack_lds, int leaf_ilp, int fetch_min) {
+    if (!c) return PRAY_ERR_INVALID;
+    SceneRec *sc = find_scene(c, scene_id);
+    if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    if (layout >= 0 && layout != PRAY_LAYOUT_INDEXED && layout != PRAY_LAYOUT_PACKED)
+        return fail(c, PRAY_ERR_INVALID, "unknown layout %d", layout);
+    if (stack_lds >= 0 && stack_lds != 4 && stack_lds != 8 && stack_lds != 16)
+        return fail(c, PRAY_ERR_INVALID, "stack_lds must be 4, 8 or 16");
+    if (leaf_ilp >= 0 && leaf_ilp != 1 && leaf_ilp != 2) return fail(c, PRAY_ERR_INVALID, "leaf_ilp must be 1 or 2");
+    if (fetch_min >= 0 && (fetch_min < 1 || fetch_min > WAVE))
+        return fail(c, PRAY_ERR_INVALID, "fetch_min must be in 1..64");
+    if (layout >= 0) sc->tune.layout = layout;
+    if (stack_lds >= 0) sc->tune.stack_lds = stack_lds;
+    if (leaf_ilp >= 0) sc->tune.leaf_ilp = leaf_ilp;
+    if (fetch_min >= 0) sc->tune.fetch_min = fetch_min;
+    return PRAY_OK;
+}
+
+int pray_get_tuning(pray_ctx *c, int scene_id, int *layout, int *stack_lds, int *leaf_ilp, int *fetch_min) {
+    if (!c) return PRAY_ERR_INVALID;
+    SceneRec *sc = find_scene(c, scene_id);
+    if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    if (layout) *layout = sc->tune.layout;
+    if (stack_lds) *stack_lds = sc->tune.stack_lds;
+    if (leaf_ilp) *leaf_ilp = sc->tune.leaf_ilp;
+    if (fetch_min) *fetch_min = sc->tune.fetch_min;
+    return PRAY_OK;
+}
+
+int pray_trace_device(pray_ctx *c, int scene_id, const float *d_rays, uint64_t n, int mode, float det_eps,
+                      const float *d_limit2, int32_t *d_hit_idx, float *d_hit_t, void *stream) {
+    if (!c) return PRAY_ERR_INVALID;
+    SceneRec *sc = find_scene(c, scene_id);
+    if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    if (n && (!d_rays || !d_hit_idx)) return fail(c, PRAY_ERR_INVALID, "null buffer");
+    HIPC(c, hipSetDevice(c->device));
+    return enqueue_trace(c, *sc, d_rays, n, nullptr, mode, det_eps, d_limit2, d_hit_idx, d_hit_t, nullptr,
+                         pick(c, stream));
+}
+
+int pray_trace_counts_device(pray_ctx *c, int scene_id, const float *d_rays, uint64_t n, int mode, float det_eps,
+                             const float *d_limit2, int32_t *d_hit_idx, float *d_hit_t, uint32_t *d_counts,
+                             void *stream) {
+    if (!c) return PRAY_ERR_INVALID;
+    SceneRec *sc = find_scene(c, scene_id);
+    if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    if (n && (!d_rays || !d_hit_idx || !d_counts)) return fail(c, PRAY_ERR_INVALID, "null buffer");
+    HIPC(c, hipSetDevice(c->device));
+    return enqueue_trace(c, *sc, d_rays, n, nullptr, mode, det_eps, d_limit2, d_hit_idx, d_hit_t, d_counts,
+                         pick(c, stream));
+}
+
+int pray_trace(pray_ctx *c, int scene_id, const float *rays, uint64_t n, int mode, float det_eps,
+               const float *limit2, int32_t *hit_idx, float *hit_t) {
+    if (!c) return PRAY_ERR_INVALID;
+    SceneRec *sc = find_scene(c, scene_id);
+    if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    if (n == 0) return PRAY_OK;
+    if (!rays || !hit_idx) return fail(c, PRAY_ERR_INVALID, "null buffer");
+    if (mode != PRAY_TRACE_CLOSEST && !limit2) return fail(c, PRAY_ERR_INVALID, "any-hit modes need limit2");
+    HIPC(c, hipSetDevice(c->device));
+    DBuf dr, dl, di, dt;
+    HIPC(c, hipMalloc(&dr.p, n * 24));
+    HIPC(c, hipMalloc(&di.p, n * 4));
+    HIPC(c, hipMalloc(&dt.p, n * 4));
+    HIPC(c, hipMemcpyAsync(dr.p, rays, n * 24, hipMemcpyHostToDevice, c->stream));
+    if (mode != PRAY_TRACE_CLOSEST) {
+        HIPC(c, hipMalloc(&dl.p, n * 4));
+        HIPC(c, hipMemcpyAsync(dl.p, limit2, n * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    int rc = enqueue_trace(c, *sc, (const float *)dr.p, n, nullptr, mode, det_eps, (const float *)dl.p,
+                           (int32_t *)di.p, (float *)dt.p, nullptr, c->stream);
+    if (rc) return rc;
+    HIPC(c, hipMemcpyAsync(hit_idx, di.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (hit_t) HIPC(c, hipMemcpyAsync(hit_t, dt.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return PRAY_OK;
+}
+
+int pray_gen_rays_device(pray_ctx *c, uint64_t seed, const float *box6, uint64_t n, float *d_rays, void *stream) {
+    if (!c || !box6) return fail(c, PRAY_ERR_INVALID, "null argument");
+    if (n > 0xFFFFFFFFull) return fail(c, PRAY_ERR_INVALID, "at most 2^32-1 rays");
+    if (n == 0) return PRAY_OK;
+    HIPC(c, hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_gen_rays, dim3(nblocks(n, 256)), dim3(256), 0, pick(c, stream), (uint32_t)seed,
+                       (uint32_t)(seed >> 32), box6[0], box6[1], box6[2], box6[3], box6[4], box6[5], (uint32_t)n,
+                       d_rays);
+    HIPC(c, hipGetLastError());
+    return PRAY_OK;
+}
+
+int pray_gen_rays(pray_ctx *c, uint64_t seed, const float *box6, uint64_t n, float *rays) {
+    if (!c || !box6 || (n && !rays)) return fail(c, PRAY_ERR_INVALID, "null argument");
+    if (n == 0) return PRAY_OK;
+    HIPC(c, hipSetDevice(c->device));
+    DBuf dr;
+    HIPC(c, hipMalloc(&dr.p, n * 24));
+    int rc = pray_gen_rays_device(c, seed, box6, n, (float *)dr.p, c->stream);
+    if (rc) return rc;
+    HIPC(c, hipMemcpyAsync(rays, dr.p, n * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return PRAY_OK;
+}
+
+int pray_camera_rays(pray_ctx *c, const float *cam16, uint32_t width, uint32_t height, float *rays) {
+    if (!c || !cam16 || !rays) return fail(c, PRAY_ERR_INVALID, "null argument");
+    const uint64_t n = (uint64_t)width * height;
+    if (n == 0) return PRAY_OK;
+    if (n > 0xFFFFFFFFull) return fail(c, PRAY_ERR_INVALID, "image too large");
+    HIPC(c, hipSetDevice(c->device));
+    Cam cam;
+    std::memcpy(&cam, cam16, sizeof cam);
+    DBuf dr;
+    HIPC(c, hipMalloc(&dr.p, n * 24));
+    hipLaunchKernelGGL(k_camera_rays, dim3(nblocks(n, 256)), dim3(256), 0, c->stream, cam, width, height,
+                       (float *)dr.p);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipMemcpyAsync(rays, dr.p, n * 24, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return PRAY_OK;
+}
+
+int pray_render_rt(pray_ctx *c, int, const float *, uint32_t, uint32_t, const float *, uint32_t, const float *, float,
+                   uint8_t *, int32_t *) {
+    return fail(c, PRAY_ERR_UNSUPPORTED, "pray_render_rt not built yet");
+}
+
+int pray_render_pt(pray_ctx *c, int, const float *, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t,
+                   const float *, float, float *, uint8_t *) {
+    return fail(c, PRAY_ERR_UNSUPPORTED, "pray_render_pt not built yet");
+}
+
+int pray_device_alloc(pray_ctx *c, uint64_t bytes, void **p) {
+    if (!c || !p) return fail(c, PRAY_ERR_INVALID, "null argument");
+    *p = nullptr;
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipMalloc(p, bytes ? bytes : 1));
+    return PRAY_OK;
+}
+
+int pray_device_free(pray_ctx *c, void *p) {
+    if (!c) return PRAY_ERR_INVALID;
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    HIPC(c, hipFree(p));
+    return PRAY_OK;
+}
+
+int pray_copy_to_device(pray_ctx *c, void *dst, const void *src, uint64_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return fail(c, PRAY_ERR_INVALID, "null argument");
+    if (!bytes) return PRAY_OK;
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return PRAY_OK;
+}
+
+int pray_copy_to_host(pray_ctx *c, void *dst, const void *src, uint64_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return fail(c, PRAY_ERR_INVALID, "null argument");
+    if (!bytes) return PRAY_OK;
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return PRAY_OK;
+}
+
+int pray_sync(pray_ctx *c) {
+    if (!c) return PRAY_ERR_INVALID;
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return PRAY_OK;
+}
+
+int pray_event_record(pray_ctx *c, int slot) {
+    if (!c || slot < 0 || slot >= 16) return fail(c, PRAY_ERR_INVALID, "bad event slot");
+    HIPC(c, hipSetDevice(c->device));
+    if (!c->events[slot]) HIPC(c, hipEventCreate(&c->events[slot]));
+    HIPC(c, hipEventRecord(c->events[slot], c->stream));
+    return PRAY_OK;
+}
+
+int pray_event_elapsed(pray_ctx *c, int s0, int s1, float *ms) {
+    if (!c || !ms || s0 < 0 || s0 >= 16 || s1 < 0 || s1 >= 16 || !c->events[s0] || !c->events[s1])
+        return fail(c, PRAY_ERR_INVALID, "bad event slot");
+    HIPC(c, hipEventSynchronize(c->events[s1]));
+    HIPC(c, hipEventElapsedTime(ms, c->events[s0], c->events[s1]));
+    return PRAY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,313 @@
+// Closest-hit / any-hit traversal kernel for gfx950.
+//
+// One lane = one ray at a time; waves are persistent and refill lanes whose
+// ray finished from a global ray counter (one atomic per refill, shared by
+// the idle lanes of the wave through a ballot), so long rays never hold a wave
+// of finished lanes.  Every loop iteration each lane does ONE step: either an
+// inner-node step (one aligned 16-byte child-pair load) or a leaf step (ILP
+// triangle records loaded together; the record array is padded so the
+// look-ahead stays in bounds).  Pending far children go to a per-lane stack:
+// K entries in an LDS ring, the oldest entries spilled to a per-lane global
+// area when the ring is full.
+//
+// Interval arithmetic: planes are intersected front to back with the
+// direction's sign choosing near/far; every plane distance is widened by a
+// relative 1e-6 (conservative: a cell is never skipped by rounding), and a
+// pending cell is pruned only when its widened entry exceeds the best t
+// (strictly), so equal-t ties are always examined.  The closest hit is the
+// lexicographic minimum of (t, triangle index) over t > 0, which equals the
+// reference brute force (bf_search.cpp:26-40) on every ray.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_math.hpp"
+
+namespace prt {
+
+constexpr int WAVE = 64;
+constexpr uint32_t LEAF_TAG = 3u;
+constexpr uint32_t LEAF_EMPTY = 0x3FFFFFFFu;
+constexpr uint32_t REF_LAST = 0x80000000u;
+constexpr uint32_t NO_HIT = 0xFFFFFFFFu;
+constexpr float WIDEN = 1e-6f;
+
+struct DevScene {
+    const uint2 *nodes = nullptr;
+    const uint32_t *refs = nullptr;   // n_refs + 2 (padded)
+    const float *a = nullptr, *ab = nullptr, *ac = nullptr;
+    const float4 *recs = nullptr;     // 3 * (n_refs + 2) (padded)
+    const float *normal = nullptr, *color = nullptr;
+    const uint8_t *emit = nullptr;
+    float box[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t n_tris = 0, n_nodes = 0, n_refs = 0, max_depth = 0;
+};
+
+struct TraceArgs {
+    const float *rays;        // n x 6
+    uint32_t n;               // rays (or launch bound when n_dev is set)
+    const uint32_t *n_dev;    // optional device-side ray count
+    const float *limit2;      // any-hit squared limits
+    float eps;                // determinant cull
+    int inclusive;            // any-hit: t*t <= limit2 instead of <
+    int32_t *out_idx;
+    float *out_t;             // optional
+    uint32_t *counts;         // optional: 2 per ray (nodes, refs)
+    uint32_t *work;           // ray counter, zero at launch
+    uint4 *spill;             // per lane slot: spill_depth entries
+    uint32_t spill_depth;
+    uint32_t fetch_min;       // refill when at least this many lanes are idle
+};
+
+enum : uint32_t { ST_IDLE = 0, ST_NODE = 1, ST_LEAF = 2, ST_DEAD = 3 };
+
+__device__ __forceinline__ float whi(float x) { return x + fabsf(x) * WIDEN; }
+__device__ __forceinline__ float wlo(float x) { return x - fabsf(x) * WIDEN; }
+__device__ __forceinline__ bool empty_leaf(uint32_t w) { return w == ((LEAF_EMPTY << 2) | LEAF_TAG); }
+
+__device__ __forceinline__ void slab(float bmin, float bmax, float o, float inv, float &lo, float &hi) {
+    float t0 = (bmin - o) * inv, t1 = (bmax - o) * inv;
+    if (inv < 0.f) {
+        const float s = t0;
+        t0 = t1;
+        t1 = s;
+    }
+    lo = fmaxf(lo, wlo(t0));
+    hi = fminf(hi, whi(t1));
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ void load_tri(const DevScene &s, uint32_t k, uint32_t &ref, float4 &A, float4 &B,
+                                         float4 &Cc) {
+    if (LAYOUT == 1) {
+        A = s.recs[3 * (size_t)k];
+        B = s.recs[3 * (size_t)k + 1];
+        Cc = s.recs[3 * (size_t)k + 2];
+        ref = __float_as_uint(A.w);
+    } else {
+        ref = s.refs[k];
+        const size_t t = 3 * (size_t)(ref & ~REF_LAST);
+        A = make_float4(s.a[t], s.a[t + 1], s.a[t + 2], 0.f);
+        B = make_float4(s.ab[t], s.ab[t + 1], s.ab[t + 2], 0.f);
+        Cc = make_float4(s.ac[t], s.ac[t + 1], s.ac[t + 2], 0.f);
+    }
+}
+
+template <bool ANY, int LAYOUT, int K, int ILP>
+__global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) {
+    static_assert((K & (K - 1)) == 0, "K must be a power of two");
+    __shared__ uint32_t st_w[K * WAVE];
+    __shared__ float st_lo[K * WAVE];
+    __shared__ float st_hi[K * WAVE];
+    const uint32_t lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint4 *spill = A.spill + (size_t)(blockIdx.x * WAVE + lane) * A.spill_depth;
+    const uint32_t nn = A.n_dev ? *A.n_dev : A.n;
+    const uint2 *nodes = s.nodes;
+    const uint4 *pairs = reinterpret_cast<const uint4 *>(s.nodes);
+
+    uint32_t state = ST_IDLE, ray = 0, w = 0, k = 0, best_i = NO_HIT, cn = 0, cr = 0;
+    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, lim2 = 0;
+    float lo = 0, hi = 0, best_t = INFINITY;
+    int sp = 0, base = 0;
+
+    while (true) {
+        // ------------------------------------------------------- refill --
+        const uint64_t idle = __ballot(state == ST_IDLE);
+        const uint64_t dead = __ballot(state == ST_DEAD);
+        if (dead == ~0ull) break;
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (nidle && (nidle >= A.fetch_min || (idle | dead) == ~0ull)) {
+            const uint32_t first = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
+            uint32_t b = 0;
+            if (lane == first) b = atomicAdd(A.work, nidle);
+            b = __shfl(b, (int)first);
+            if (state == ST_IDLE) {
+                ray = b + (uint32_t)__popcll(idle & below);
+                if (ray >= nn) {
+                    state = ST_DEAD;
+                } else {
+                    const float2 r0 = reinterpret_cast<const float2 *>(A.rays)[3 * (size_t)ray];
+                    const float2 r1 = reinterpret_cast<const float2 *>(A.rays)[3 * (size_t)ray + 1];
+                    const float2 r2 = reinterpret_cast<const float2 *>(A.rays)[3 * (size_t)ray + 2];
+                    ox = r0.x, oy = r0.y, oz = r1.x, dx = r1.y, dy = r2.x, dz = r2.y;
+                    ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+                    lo = 0.f, hi = INFINITY;
+                    slab(s.box[0], s.box[3], ox, ix, lo, hi);
+                    slab(s.box[1], s.box[4], oy, iy, lo, hi);
+                    slab(s.box[2], s.box[5], oz, iz, lo, hi);
+                    best_t = INFINITY, best_i = NO_HIT, sp = 0, base = 0, cn = 0, cr = 0;
+                    bool live = true;
+                    if (ANY) {
+                        lim2 = A.limit2[ray];
+                        if (!(lim2 >= 0.f))
+                            live = false;
+                        else
+                            hi = fminf(hi, whi(sqrtf(lim2)));
+                    }
+                    if (live && lo <= hi) {
+                        cn = 1;
+                        w = nodes[1].y;
+                        if ((w & 3u) != LEAF_TAG) {
+                            state = ST_NODE;
+                        } else if (!empty_leaf(w)) {
+                            state = ST_LEAF;
+                            k = w >> 2;
+                        } else {
+                            live = false;
+                        }
+                    } else {
+                        live = false;
+                    }
+                    if (!live) {
+                        A.out_idx[ray] = -1;
+                        if (A.out_t) A.out_t[ray] = FLT_MAX;
+                        if (A.counts) A.counts[2 * (size_t)ray] = cn, A.counts[2 * (size_t)ray + 1] = 0;
+                        state = ST_IDLE;
+                    }
+                }
+            }
+        }
+
+        bool need_pop = false;
+        // ---------------------------------------------------- node step --
+        if (state == ST_NODE) {
+            const uint32_t axis = w & 3u;
+            const uint4 pr = pairs[w >> 3];  // pair index = (w >> 2) >> 1
+            cn += 2;
+            const float oa = axis == 0 ? ox : (axis == 1 ? oy : oz);
+            const float ia = axis == 0 ? ix : (axis == 1 ? iy : iz);
+            const float tL = (__uint_as_float(pr.x) - oa) * ia;
+            const float tR = (__uint_as_float(pr.z) - oa) * ia;
+            const bool neg = ia < 0.f;
+            const float tn = neg ? tR : tL, tf = neg ? tL : tR;
+            const uint32_t wn = neg ? pr.w : pr.y, wf = neg ? pr.y : pr.w;
+            const float cap = ANY ? hi : fminf(hi, best_t);
+            const float n_hi = fminf(cap, whi(tn));
+            const float f_lo = fmaxf(lo, wlo(tf));
+            const bool gn = lo <= n_hi && !empty_leaf(wn);
+            const bool gf = f_lo <= cap && !empty_leaf(wf);
+            uint32_t nw = wn;
+            if (gn && gf) {
+                if (sp - base == K) {  // ring full: spill the oldest entry
+                    const uint32_t at = (uint32_t)(base & (K - 1)) * WAVE + lane;
+                    spill[base] = make_uint4(st_w[at], __float_as_uint(st_lo[at]), __float_as_uint(st_hi[at]), 0u);
+                    ++base;
+                }
+                const uint32_t at = (uint32_t)(sp & (K - 1)) * WAVE + lane;
+                st_w[at] = wf;
+                st_lo[at] = f_lo;
+                st_hi[at] = hi;
+                ++sp;
+                hi = n_hi;
+            } else if (gn) {
+                hi = n_hi;
+            } else if (gf) {
+                nw = wf;
+                lo = f_lo;
+            } else {
+                need_pop = true;
+            }
+            if (!need_pop) {
+                w = nw;
+                if ((w & 3u) == LEAF_TAG) {
+                    state = ST_LEAF;
+                    k = w >> 2;
+                }
+            }
+        }
+        // ---------------------------------------------------- leaf step --
+        else if (state == ST_LEAF) {
+            uint32_t ref[ILP];
+            float4 ta[ILP], tb[ILP], tc[ILP];
+#pragma unroll
+            for (int j = 0; j < ILP; ++j) load_tri<LAYOUT>(s, k + j, ref[j], ta[j], tb[j], tc[j]);
+            bool leaf_done = false, found = false;
+#pragma unroll
+            for (int j = 0; j < ILP; ++j) {
+                if (!leaf_done && !found) {
+                    ++cr;
+                    const uint32_t tri = ref[j] & ~REF_LAST;
+                    float t;
+                    if (prd::mt_hit(ox, oy, oz, dx, dy, dz, ta[j].x, ta[j].y, ta[j].z, tb[j].x, tb[j].y, tb[j].z,
+                                    tc[j].x, tc[j].y, tc[j].z, A.eps, t)) {
+                        if (!ANY) {
+                            if (t > 0.f && (t < best_t || (t == best_t && tri < best_i))) {
+                                best_t = t;
+                                best_i = tri;
+                            }
+                        } else {
+                            const float t2 = t * t;
+                            if (t > 0.f && (A.inclusive ? (t2 <= lim2) : (t2 < lim2))) {
+                                best_t = t;
+                                best_i = tri;
+                                found = true;
+                            }
+                        }
+                    }
+                    if (ref[j] & REF_LAST) leaf_done = true;
+                }
+            }
+            if (found) {
+                sp = 0;
+                base = 0;
+                need_pop = true;  // stack is empty: the pop below finishes the ray
+            } else if (leaf_done) {
+                need_pop = true;
+            } else {
+                k += ILP;
+            }
+        }
+
+        // ----------------------------------------------------------- pop --
+        if (need_pop) {
+            bool got = false;
+            while (sp > 0) {
+                --sp;
+                uint32_t ew;
+                float elo, ehi;
+                if (sp >= base) {
+                    const uint32_t at = (uint32_t)(sp & (K - 1)) * WAVE + lane;
+                    ew = st_w[at];
+                    elo = st_lo[at];
+                    ehi = st_hi[at];
+                } else {
+                    const uint4 e = spill[sp];
+                    ew = e.x;
+                    elo = __uint_as_float(e.y);
+                    ehi = __uint_as_float(e.z);
+                    base = sp;
+                }
+                const float cap = ANY ? ehi : fminf(ehi, best_t);
+                if (elo <= cap) {
+                    w = ew;
+                    lo = elo;
+                    hi = ehi;
+                    got = true;
+                    break;
+                }
+            }
+            if (got) {
+                if ((w & 3u) == LEAF_TAG) {
+                    state = ST_LEAF;
+                    k = w >> 2;
+                } else {
+                    state = ST_NODE;
+                }
+            } else {
+                A.out_idx[ray] = (int32_t)best_i;
+                if (A.out_t) A.out_t[ray] = best_i == NO_HIT ? FLT_MAX : best_t;
+                if (A.counts) A.counts[2 * (size_t)ray] = cn, A.counts[2 * (size_t)ray + 1] = cr;
+                state = ST_IDLE;
+            }
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+}  // namespace prt

@@ -1,0 +1,274 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes wrappers of the CPU restatement of the reference (``liboracle.so``,
+built from ``oracle.cpp``) and of the reference's own sources compiled by
+``oracle/Makefile`` into ``oracle/_ref/``.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` use
+this package, as the checker; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+REF_LIB = os.path.join(REF_DIR, "libpray_ref.so")
+REF_LIB_FMA = os.path.join(REF_DIR, "libpray_ref_fma.so")
+REF_CLI = os.path.join(REF_DIR, "Pray")
+REF_CLI_FMA = os.path.join(REF_DIR, "Pray_fma")
+
+_vp, _u32, _u64, _f32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float
+_lib = None
+_refs = {}
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError(f"{ORACLE_LIB} not built (make -C oracle oracle)")
+        L = C.CDLL(ORACLE_LIB)
+        sig = {
+            "orc_philox": (None, [_vp, _u32, _u32, _vp]),
+            "orc_sincos2pi": (None, [_vp, _u64, _vp, _vp]),
+            "orc_sample_hemisphere": (None, [_vp, _vp, _vp, _u64, _vp]),
+            "orc_gen_rays": (None, [_u64, _vp, _u64, _vp]),
+            "orc_init_tris": (None, [_vp, _u64, _vp, _vp, _vp]),
+            "orc_mt": (C.c_int, [_vp, _vp, _vp, _vp, _f32, _vp]),
+            "orc_mt_avx": (None, [_vp, _u64, _vp, _vp, _vp, _f32, _vp, _vp]),
+            "orc_slab": (None, [_vp, _u64, _vp, _vp, _vp]),
+            "orc_bf_closest": (None, [_vp, _vp, _vp, _u64, _vp, _u64, _f32, _vp, _vp]),
+            "orc_ref_closest": (None, [_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _u64, _f32, _vp, _vp]),
+            "orc_ref_closest_avx": (C.c_double, [_vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _u64, _f32, C.c_int, _vp,
+                                                 _vp]),
+            "orc_flat_trace": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp,
+                                      _vp]),
+            "orc_camera": (None, [_vp, _vp, _u64, _u64, _u64, _vp]),
+            "orc_camera_rays": (None, [_vp, _u64, _u64, _vp]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype, f.argtypes = r, a
+        _lib = L
+    return _lib
+
+
+def have_ref(fma=False):
+    return os.path.exists(REF_LIB_FMA if fma else REF_LIB)
+
+
+def ref(fma=False):
+    path = REF_LIB_FMA if fma else REF_LIB
+    if path not in _refs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built (make -C oracle ref, needs /root/reference)")
+        L = C.CDLL(path)
+        sig = {
+            "ref_build": (_vp, [_vp, _u64, C.c_int]),
+            "ref_free": (None, [_vp]),
+            "ref_stats": (None, [_vp, _vp]),
+            "ref_intersect": (None, [_vp, _vp, _u64, _vp, _vp]),
+            "ref_intersect_avx": (C.c_double, [_vp, _vp, _u64, C.c_int, _vp, _vp]),
+            "ref_export": (None, [_vp, _vp, _vp, _vp, _vp]),
+            "ref_init_tris": (None, [_vp, _u64, _vp, _vp, _vp]),
+            "ref_prim_rays": (None, [_vp, _vp, _u64, _u64, _u64, _vp]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype, f.argtypes = r, a
+        _refs[path] = L
+    return _refs[path]
+
+
+# ------------------------------------------------------------ restatement --
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    out = np.empty(4, np.uint32)
+    lib().orc_philox(_p(c), int(key[0]), int(key[1]), _p(out))
+    return out
+
+
+def sincos2pi(u):
+    u = np.ascontiguousarray(u, np.float32)
+    s, c = np.empty_like(u), np.empty_like(u)
+    lib().orc_sincos2pi(_p(u), u.size, _p(s), _p(c))
+    return s, c
+
+
+def sample_hemisphere(normals, u1, u2):
+    n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    u1 = np.ascontiguousarray(u1, np.float32)
+    u2 = np.ascontiguousarray(u2, np.float32)
+    out = np.empty_like(n)
+    lib().orc_sample_hemisphere(_p(n), _p(u1), _p(u2), n.shape[0], _p(out))
+    return out
+
+
+def gen_rays(seed, box6, n):
+    box6 = np.ascontiguousarray(box6, np.float32)
+    out = np.empty((n, 6), np.float32)
+    lib().orc_gen_rays(seed, _p(box6), n, _p(out))
+    return out
+
+
+def init_tris(abc):
+    abc = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
+    n = abc.shape[0]
+    ab, ac, nn = (np.empty((n, 3), np.float32) for _ in range(3))
+    lib().orc_init_tris(_p(abc), n, _p(ab), _p(ac), _p(nn))
+    return ab, ac, nn
+
+
+def mt(ray6, a, ab, ac, eps):
+    t = np.zeros(1, np.float32)
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (ray6, a, ab, ac)]
+    h = lib().orc_mt(*[_p(x) for x in arrs], eps, _p(t))
+    return bool(h), float(t[0])
+
+
+def mt_avx(rays, a, ab, ac, eps):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    hit = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (a, ab, ac)]
+    lib().orc_mt_avx(_p(rays), n, *[_p(x) for x in arrs], eps, _p(hit), _p(t))
+    return hit.astype(bool), t
+
+
+def slab(rays, box6):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    s, a = np.empty(n, np.int32), np.empty(n, np.int32)
+    box6 = np.ascontiguousarray(box6, np.float32)
+    lib().orc_slab(_p(rays), n, _p(box6), _p(s), _p(a))
+    return s.astype(bool), a.astype(bool)
+
+
+def bf_closest(a, ab, ac, rays, eps):
+    a, ab, ac = (np.ascontiguousarray(x, np.float32).reshape(-1, 3) for x in (a, ab, ac))
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    idx, t = np.empty(n, np.int32), np.empty(n, np.float32)
+    lib().orc_bf_closest(_p(a), _p(ab), _p(ac), a.shape[0], _p(rays), n, eps, _p(idx), _p(t))
+    return idx, t
+
+
+def ref_closest(export, a, ab, ac, rays, eps):
+    """Exhaustive reference traversal (scalar) over a preorder tree export."""
+    ni, nf, tr = export
+    a, ab, ac = (np.ascontiguousarray(x, np.float32).reshape(-1, 3) for x in (a, ab, ac))
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    idx, t = np.empty(n, np.int32), np.empty(n, np.float32)
+    lib().orc_ref_closest(_p(ni), _p(nf), ni.shape[0], _p(tr), _p(a), _p(ab), _p(ac), _p(rays), n, eps, _p(idx),
+                          _p(t))
+    return idx, t
+
+
+def ref_closest_avx(export, a, ab, ac, rays, eps, threads):
+    """AVX packet reference traversal; returns (idx, t, seconds)."""
+    ni, nf, tr = export
+    a, ab, ac = (np.ascontiguousarray(x, np.float32).reshape(-1, 3) for x in (a, ab, ac))
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    idx, t = np.empty(n, np.int32), np.empty(n, np.float32)
+    sec = lib().orc_ref_closest_avx(_p(ni), _p(nf), ni.shape[0], _p(tr), _p(a), _p(ab), _p(ac), _p(rays), n, eps,
+                                    threads, _p(idx), _p(t))
+    return idx, t, sec
+
+
+def flat_trace(flat, rays, mode=0, eps=1e-4, limit2=None, counts=False):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    idx, t = np.empty(n, np.int32), np.empty(n, np.float32)
+    cnt = np.empty((n, 2), np.uint32) if counts else None
+    lim = None if limit2 is None else np.ascontiguousarray(limit2, np.float32)
+    lib().orc_flat_trace(_p(flat["nodes"]), _p(flat["refs"]), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]),
+                         _p(flat["box"]), flat["a"].shape[0], _p(rays), n, mode, eps, _p(lim), _p(idx), _p(t),
+                         _p(cnt))
+    return (idx, t, cnt) if counts else (idx, t)
+
+
+def camera(pos, look, fov, w, h):
+    out = np.empty(16, np.float32)
+    lib().orc_camera(_p(np.ascontiguousarray(pos, np.float32)), _p(np.ascontiguousarray(look, np.float32)), fov, w,
+                     h, _p(out))
+    return out
+
+
+def camera_rays(cam16, w, h):
+    cam16 = np.ascontiguousarray(cam16, np.float32)
+    out = np.empty((w * h, 6), np.float32)
+    lib().orc_camera_rays(_p(cam16), w, h, _p(out))
+    return out
+
+
+# -------------------------------------------------------------- reference --
+
+class RefDB:
+    """The reference's KDTree / KDTreeSAH / BFSearch compiled from source."""
+
+    def __init__(self, abc, kind, fma=False):
+        self._L = ref(fma)
+        abc = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
+        self._h = self._L.ref_build(_p(abc), abc.shape[0], kind)
+
+    def close(self):
+        if self._h:
+            self._L.ref_free(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def stats(self):
+        o = np.zeros(5, np.uint64)
+        self._L.ref_stats(self._h, _p(o))
+        return dict(zip(["nnodes", "maxdepth", "leafNodes", "emptyLeafNodes", "triangleCount"], (int(x) for x in o)))
+
+    def intersect(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        idx, t = np.empty(n, np.int32), np.empty(n, np.float32)
+        self._L.ref_intersect(self._h, _p(rays), n, _p(idx), _p(t))
+        return idx, t
+
+    def intersect_avx(self, rays, threads=8):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        idx, t = np.empty(n, np.int32), np.empty(n, np.float32)
+        sec = self._L.ref_intersect_avx(self._h, _p(rays), n, threads, _p(idx), _p(t))
+        return idx, t, sec
+
+    def export(self):
+        sz = np.zeros(2, np.uint64)
+        self._L.ref_export(self._h, _p(sz), None, None, None)
+        ni = np.empty((int(sz[0]), 4), np.int32)
+        nf = np.empty((int(sz[0]), 7), np.float32)
+        tr = np.empty(int(sz[1]), np.uint32)
+        self._L.ref_export(self._h, None, _p(ni), _p(nf), _p(tr))
+        return ni, nf, tr
+
+
+def ref_init_tris(abc, fma=False):
+    abc = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
+    n = abc.shape[0]
+    ab, ac, nn = (np.empty((n, 3), np.float32) for _ in range(3))
+    ref(fma).ref_init_tris(_p(abc), n, _p(ab), _p(ac), _p(nn))
+    return ab, ac, nn
+
+
+def ref_prim_rays(pos, look, fov, w, h, fma=False):
+    out = np.empty((w * h, 6), np.float32)
+    ref(fma).ref_prim_rays(_p(np.ascontiguousarray(pos, np.float32)), _p(np.ascontiguousarray(look, np.float32)),
+                           fov, w, h, _p(out))
+    return out

@@ -1,0 +1,699 @@
+// ORACLE -- test infrastructure only.
+//
+// CPU restatement of the reference's tracing path (nicmue/nostrum_pray), used
+// by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py as
+// the checker.  Nothing in the product (nostrum_pray_amd/, include/) links or
+// calls this file.  Each function names the reference lines it follows.
+// Compiled with -ffp-contract=off so expressions round exactly as written.
+//
+// Parity pins (see tests/test_oracle.py): scalar MT == AVX MT bitwise;
+// closed-form MT and slab cases; BF == reference BFSearch compiled from
+// source (oracle/_ref); exhaustive KD traversal == reference KDTree::intersect
+// compiled from source; camera rays == reference calculatePrimRays compiled
+// from source; Philox4x32-10 == the Random123 known-answer vectors; flat
+// traversal twin == BF on every ray.  Functions with no reference
+// counterpart (Philox, the polynomial sincos, the radiance-cache MLP) are
+// pinned by their published definitions (known-answer vectors, series
+// bounds, finite differences).
+#include <immintrin.h>
+#include <omp.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#define ORC_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+constexpr float INFTY = std::numeric_limits<float>::max();
+constexpr uint32_t NO_HIT = 0xFFFFFFFFu;
+
+// ---------------------------------------------------------------- basics --
+struct V {
+    float x, y, z;
+};
+inline V sub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V add(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V mul(V a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V mulv(V a, V b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+// geometry.h:36 / :38
+inline float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V cross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+// geometry.h:28-34
+inline void normalize(V &a) {
+    const float n2 = a.x * a.x + a.y * a.y + a.z * a.z;
+    if (n2 > 0) {
+        const float inv = 1 / std::sqrt(n2);
+        a.x *= inv, a.y *= inv, a.z *= inv;
+    }
+}
+inline V ld(const float *p) { return {p[0], p[1], p[2]}; }
+
+// Moeller-Trumbore, geometry.h:241-262 (scalar path).
+inline bool mt(V o, V d, V a, V ab, V ac, float eps, float &t) {
+    const V pvec = cross(d, ac);
+    const float det = dot(ab, pvec);
+    if (det < eps) return false;
+    const float invDet = 1 / det;
+    const V tvec = sub(o, a);
+    const float u = dot(tvec, pvec) * invDet;
+    if (u < 0.f || u > 1.f) return false;
+    const V qvec = cross(tvec, ab);
+    const float v = dot(d, qvec) * invDet;
+    if (v < 0.f || u + v > 1.f) return false;
+    t = dot(ac, qvec) * invDet;
+    return true;
+}
+
+// Slab "line" test, geometry.h:151-166 (multiplies by 1/dir, std::min/max).
+inline bool slabLine(const float *bmin, const float *bmax, V o, V inv) {
+    const float t1x = (bmin[0] - o.x) * inv.x, t1y = (bmin[1] - o.y) * inv.y, t1z = (bmin[2] - o.z) * inv.z;
+    const float t2x = (bmax[0] - o.x) * inv.x, t2y = (bmax[1] - o.y) * inv.y, t2z = (bmax[2] - o.z) * inv.z;
+    const float nx = std::min(t1x, t2x), ny = std::min(t1y, t2y), nz = std::min(t1z, t2z);
+    const float fx = std::max(t1x, t2x), fy = std::max(t1y, t2y), fz = std::max(t1z, t2z);
+    const float tNear = std::max(std::max(nx, ny), std::max(nz, std::numeric_limits<float>::lowest()));
+    const float tFar = std::min(std::min(fx, fy), std::min(fz, std::numeric_limits<float>::max()));
+    return tNear <= tFar;
+}
+
+// --------------------------------------------------------------- AVX MT --
+// geometry_simd.h:9-33 and geometry_simd.cpp:42-96.
+inline void cross8(__m256 ax, __m256 ay, __m256 az, __m256 bx, __m256 by, __m256 bz, __m256 &rx, __m256 &ry,
+                   __m256 &rz) {
+    rx = _mm256_sub_ps(_mm256_mul_ps(ay, bz), _mm256_mul_ps(az, by));
+    ry = _mm256_sub_ps(_mm256_mul_ps(az, bx), _mm256_mul_ps(ax, bz));
+    rz = _mm256_sub_ps(_mm256_mul_ps(ax, by), _mm256_mul_ps(ay, bx));
+}
+inline __m256 dot8(__m256 ax, __m256 ay, __m256 az, __m256 bx, __m256 by, __m256 bz) {
+    return _mm256_add_ps(_mm256_add_ps(_mm256_mul_ps(ax, bx), _mm256_mul_ps(ay, by)), _mm256_mul_ps(az, bz));
+}
+
+inline __m256 mt8(const float *a, const float *ab, const float *ac, const __m256 *o, const __m256 *d, __m256 mask,
+                  float eps, __m256 &dist) {
+    const __m256 abX = _mm256_set1_ps(ab[0]), abY = _mm256_set1_ps(ab[1]), abZ = _mm256_set1_ps(ab[2]);
+    const __m256 acX = _mm256_set1_ps(ac[0]), acY = _mm256_set1_ps(ac[1]), acZ = _mm256_set1_ps(ac[2]);
+    __m256 px, py, pz;
+    cross8(d[0], d[1], d[2], acX, acY, acZ, px, py, pz);
+    const __m256 det = dot8(abX, abY, abZ, px, py, pz);
+    __m256 res = _mm256_and_ps(mask, _mm256_cmp_ps(det, _mm256_set1_ps(eps), _CMP_GE_OQ));
+    if (_mm256_movemask_ps(res) == 0) return res;
+    const __m256 invDet = _mm256_div_ps(_mm256_set1_ps(1.f), det);
+    const __m256 tx = _mm256_sub_ps(o[0], _mm256_set1_ps(a[0]));
+    const __m256 ty = _mm256_sub_ps(o[1], _mm256_set1_ps(a[1]));
+    const __m256 tz = _mm256_sub_ps(o[2], _mm256_set1_ps(a[2]));
+    const __m256 u = _mm256_mul_ps(dot8(tx, ty, tz, px, py, pz), invDet);
+    res = _mm256_and_ps(res, _mm256_cmp_ps(u, _mm256_set1_ps(0.f), _CMP_GE_OQ));
+    res = _mm256_and_ps(res, _mm256_cmp_ps(u, _mm256_set1_ps(1.f), _CMP_LE_OQ));
+    if (_mm256_movemask_ps(res) == 0) return res;
+    __m256 qx, qy, qz;
+    cross8(tx, ty, tz, abX, abY, abZ, qx, qy, qz);
+    const __m256 v = _mm256_mul_ps(dot8(d[0], d[1], d[2], qx, qy, qz), invDet);
+    res = _mm256_and_ps(res, _mm256_cmp_ps(v, _mm256_set1_ps(0.f), _CMP_GE_OQ));
+    res = _mm256_and_ps(res, _mm256_cmp_ps(_mm256_add_ps(u, v), _mm256_set1_ps(1.f), _CMP_LE_OQ));
+    if (_mm256_movemask_ps(res) == 0) return res;
+    dist = _mm256_mul_ps(dot8(acX, acY, acZ, qx, qy, qz), invDet);
+    return res;
+}
+
+// Packet slab, geometry_simd.cpp:6-40 (divides by the direction).
+inline __m256 slab8(const float *bmin, const float *bmax, const __m256 *o, const __m256 *d) {
+    const __m256 t1x = _mm256_div_ps(_mm256_sub_ps(_mm256_set1_ps(bmin[0]), o[0]), d[0]);
+    const __m256 t1y = _mm256_div_ps(_mm256_sub_ps(_mm256_set1_ps(bmin[1]), o[1]), d[1]);
+    const __m256 t1z = _mm256_div_ps(_mm256_sub_ps(_mm256_set1_ps(bmin[2]), o[2]), d[2]);
+    const __m256 t2x = _mm256_div_ps(_mm256_sub_ps(_mm256_set1_ps(bmax[0]), o[0]), d[0]);
+    const __m256 t2y = _mm256_div_ps(_mm256_sub_ps(_mm256_set1_ps(bmax[1]), o[1]), d[1]);
+    const __m256 t2z = _mm256_div_ps(_mm256_sub_ps(_mm256_set1_ps(bmax[2]), o[2]), d[2]);
+    const __m256 tn = _mm256_max_ps(_mm256_max_ps(_mm256_min_ps(t1x, t2x), _mm256_min_ps(t1y, t2y)),
+                                    _mm256_max_ps(_mm256_min_ps(t1z, t2z),
+                                                  _mm256_set1_ps(std::numeric_limits<float>::lowest())));
+    const __m256 tf = _mm256_min_ps(_mm256_min_ps(_mm256_max_ps(t1x, t2x), _mm256_max_ps(t1y, t2y)),
+                                    _mm256_min_ps(_mm256_max_ps(t1z, t2z), _mm256_set1_ps(INFTY)));
+    return _mm256_cmp_ps(tn, tf, _CMP_LE_OQ);
+}
+
+// ----------------------------------------------------------- Philox etc. --
+// Philox4x32-10 (Salmon et al., SC 2011).
+inline void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        const uint32_t n3 = (uint32_t)p0;
+        c[0] = n0, c[1] = n1, c[2] = n2, c[3] = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+inline float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// sin/cos of 2*pi*u: quadrant from 4u, Taylor polynomials on [0, pi/2).
+inline void sincos2pi(float u, float &s, float &c) {
+    const float x = u * 4.0f;
+    const int q = (int)x;
+    const float r = x - (float)q;
+    const float a = r * 1.57079632679489661923f;
+    const float a2 = a * a;
+    float sp = -2.50521083854417188e-08f;
+    sp = sp * a2 + 2.75573192239858907e-06f;
+    sp = sp * a2 + -1.98412698412698413e-04f;
+    sp = sp * a2 + 8.33333333333333333e-03f;
+    sp = sp * a2 + -1.66666666666666667e-01f;
+    const float sa = a + a * (a2 * sp);
+    float cp = 2.08767569878680990e-09f;
+    cp = cp * a2 + -2.75573192239858907e-07f;
+    cp = cp * a2 + 2.48015873015873016e-05f;
+    cp = cp * a2 + -1.38888888888888889e-03f;
+    cp = cp * a2 + 4.16666666666666667e-02f;
+    cp = cp * a2 + -0.5f;
+    const float ca = 1.0f + a2 * cp;
+    switch (q & 3) {
+        case 0: s = sa, c = ca; break;
+        case 1: s = ca, c = -sa; break;
+        case 2: s = -sa, c = -ca; break;
+        default: s = -ca, c = sa; break;
+    }
+}
+
+// Cosine-weighted hemisphere sample, geometry.cpp:4-29, with
+// sin(acos(sqrt(1-u1))) written as sqrt(u1); normalised twice (the second
+// time by the Ray constructor, geometry.h:84-87).
+inline V sampleHemisphere(V n, float u1, float u2) {
+    V h = n;
+    if (std::fabs(h.x) <= std::fabs(h.y) && std::fabs(h.x) <= std::fabs(h.z))
+        h.x = 1.f;
+    else if (std::fabs(h.y) <= std::fabs(h.x) && std::fabs(h.y) <= std::fabs(h.z))
+        h.y = 1.f;
+    else
+        h.z = 1.f;
+    V x = cross(h, n);
+    normalize(x);
+    V z = cross(x, n);
+    normalize(z);
+    const float st = std::sqrt(u1), ct = std::sqrt(1.0f - u1);
+    float sp, cp;
+    sincos2pi(u2, sp, cp);
+    const V s = {st * cp, ct, st * sp};
+    V d = add(add(mul(x, s.x), mul(n, s.y)), mul(z, s.z));
+    normalize(d);
+    normalize(d);
+    return d;
+}
+
+// ---------------------------------------------------------- flat layout --
+constexpr uint32_t LEAF_TAG = 3u, LEAF_EMPTY = 0x3FFFFFFFu, REF_LAST = 0x80000000u;
+
+struct Flat {
+    const uint32_t *nodes;
+    const uint32_t *refs;
+    const float *a, *ab, *ac;
+    const float *box;
+};
+
+inline float whi(float x) { return x + std::fabs(x) * 1e-6f; }
+inline float wlo(float x) { return x - std::fabs(x) * 1e-6f; }
+
+inline void slabW(float bmin, float bmax, float o, float inv, float &lo, float &hi) {
+    float t0 = (bmin - o) * inv, t1 = (bmax - o) * inv;
+    if (inv < 0.f) std::swap(t0, t1);
+    lo = std::fmax(lo, wlo(t0));
+    hi = std::fmin(hi, whi(t1));
+}
+
+// CPU twin of the GPU traversal (nostrum_pray_amd/csrc/pray_hip.hip,
+// trace_one): same interval arithmetic, same visit order, same counters.
+// mode 0 closest, 1 any (t*t < lim2), 2 any inclusive (t*t <= lim2).
+uint32_t flatTrace(const Flat &f, V o, V d, int mode, float lim2, float eps, float &tout, uint32_t &cn, uint32_t &cr) {
+    const V inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    float lo = 0.f, hi = INFINITY;
+    slabW(f.box[0], f.box[3], o.x, inv.x, lo, hi);
+    slabW(f.box[1], f.box[4], o.y, inv.y, lo, hi);
+    slabW(f.box[2], f.box[5], o.z, inv.z, lo, hi);
+    if (mode != 0) {
+        if (!(lim2 >= 0.f)) return NO_HIT;
+        hi = std::fmin(hi, whi(std::sqrt(lim2)));
+    }
+    float best_t = INFINITY;
+    uint32_t best_i = NO_HIT;
+    if (!(lo <= hi)) return best_i;
+    struct E {
+        uint32_t w;
+        float lo, hi;
+    };
+    E st[64];
+    int sp = 0;
+    uint32_t w = f.nodes[3];
+    cn += 1;
+    const float oo[3] = {o.x, o.y, o.z}, ii[3] = {inv.x, inv.y, inv.z};
+    while (true) {
+        while ((w & 3u) != LEAF_TAG) {
+            const uint32_t axis = w & 3u, lc = w >> 2;
+            const uint32_t *pr = f.nodes + 2 * lc;
+            cn += 2;
+            float cl, cr_;
+            std::memcpy(&cl, pr, 4);
+            std::memcpy(&cr_, pr + 2, 4);
+            const float tL = (cl - oo[axis]) * ii[axis];
+            const float tR = (cr_ - oo[axis]) * ii[axis];
+            const bool neg = ii[axis] < 0.f;
+            const float tn = neg ? tR : tL, tf = neg ? tL : tR;
+            const uint32_t wn = neg ? pr[3] : pr[1], wf = neg ? pr[1] : pr[3];
+            const float cap = mode == 0 ? std::fmin(hi, best_t) : hi;
+            const float n_hi = std::fmin(cap, whi(tn));
+            const float f_lo = std::fmax(lo, wlo(tf));
+            const bool gn = lo <= n_hi, gf = f_lo <= cap;
+            if (gn && gf) {
+                st[sp++] = {wf, f_lo, hi};
+                w = wn;
+                hi = n_hi;
+            } else if (gn) {
+                w = wn;
+                hi = n_hi;
+            } else if (gf) {
+                w = wf;
+                lo = f_lo;
+            } else {
+                w = (LEAF_EMPTY << 2) | LEAF_TAG;
+            }
+        }
+        uint32_t k = w >> 2;
+        if (k != LEAF_EMPTY) {
+            while (true) {
+                const uint32_t ref = f.refs[k];
+                const uint32_t tri = ref & ~REF_LAST;
+                cr += 1;
+                float t;
+                if (mt(o, d, ld(f.a + 3 * (size_t)tri), ld(f.ab + 3 * (size_t)tri), ld(f.ac + 3 * (size_t)tri), eps,
+                       t)) {
+                    if (mode == 0) {
+                        if (t > 0.f && (t < best_t || (t == best_t && tri < best_i))) best_t = t, best_i = tri;
+                    } else {
+                        const bool occ = mode == 1 ? (t * t < lim2) : (t * t <= lim2);
+                        if (t > 0.f && occ) {
+                            tout = t;
+                            return tri;
+                        }
+                    }
+                }
+                if (ref & REF_LAST) break;
+                ++k;
+            }
+        }
+        bool got = false;
+        while (sp > 0) {
+            const E e = st[--sp];
+            const float cap = mode == 0 ? std::fmin(e.hi, best_t) : e.hi;
+            if (e.lo <= cap) {
+                w = e.w, lo = e.lo, hi = e.hi;
+                got = true;
+                break;
+            }
+        }
+        if (!got) break;
+    }
+    tout = best_t;
+    return best_i;
+}
+
+// ----------------------------------------------------- reference KD tree --
+// A tree in depth-first preorder (left subtree before right): per node
+// is_leaf, first/count into the leaf triangle list, box[6].
+struct PreTree {
+    const int32_t *ni;  // n x 4: is_leaf, axis, first, count
+    const float *nf;    // n x 7: split, box[6]
+    const uint32_t *tris;
+    std::vector<uint32_t> next;  // index after each node's subtree
+    uint64_t n;
+    void build() {
+        next.assign(n, 0);
+        // Postorder sizes via an explicit stack of (node, phase).
+        std::vector<std::pair<uint64_t, int>> st;
+        if (n) st.push_back({0, 0});
+        std::vector<uint64_t> rightOf(n, 0);
+        while (!st.empty()) {
+            auto &top = st.back();
+            const uint64_t i = top.first;
+            if (ni[4 * i] == 1) {
+                next[i] = i + 1;
+                st.pop_back();
+                continue;
+            }
+            if (top.second == 0) {
+                top.second = 1;
+                st.push_back({i + 1, 0});
+            } else if (top.second == 1) {
+                top.second = 2;
+                rightOf[i] = next[i + 1];
+                st.push_back({next[i + 1], 0});
+            } else {
+                next[i] = next[rightOf[i]];
+                st.pop_back();
+            }
+        }
+    }
+};
+
+// KDTree::intersectRecursive scalar (kd_tree.cpp:237-279): visit every node
+// whose box the line crosses, left then right; leaf-local strict minimum with
+// t > 0, accepted when below the running distance.
+bool refRec(const PreTree &T, uint64_t i, const float *A, const float *AB, const float *AC, V o, V d, V inv, float eps,
+            uint32_t &triIdx, float &dist) {
+    const float *box = T.nf + 7 * i + 1;
+    if (!slabLine(box, box + 3, o, inv)) return false;
+    if (T.ni[4 * i] == 0) {
+        const bool hl = refRec(T, i + 1, A, AB, AC, o, d, inv, eps, triIdx, dist);
+        const bool hr = refRec(T, T.next[i + 1], A, AB, AC, o, d, inv, eps, triIdx, dist);
+        return hl || hr;
+    }
+    bool hit = false;
+    float minDist = INFTY;
+    uint32_t minIdx = 0;
+    const uint32_t first = (uint32_t)T.ni[4 * i + 2], cnt = (uint32_t)T.ni[4 * i + 3];
+    for (uint32_t k = 0; k < cnt; k++) {
+        const uint32_t t = T.tris[first + k];
+        float tmp;
+        if (mt(o, d, ld(A + 3 * (size_t)t), ld(AB + 3 * (size_t)t), ld(AC + 3 * (size_t)t), eps, tmp)) {
+            if (tmp < minDist && tmp > 0) {
+                hit = true;
+                minDist = tmp;
+                minIdx = t;
+            }
+        }
+    }
+    if (hit && minDist < dist) {
+        triIdx = minIdx;
+        dist = minDist;
+        return true;
+    }
+    return false;
+}
+
+// Packet version (kd_tree.cpp:282-367).
+__m256 refRec8(const PreTree &T, uint64_t i, const float *A, const float *AB, const float *AC, const __m256 *o,
+               const __m256 *d, __m256 mask, float eps, uint32_t *trsIdx, __m256 &distance) {
+    const float *box = T.nf + 7 * i + 1;
+    const __m256 bb = slab8(box, box + 3, o, d);
+    if (_mm256_movemask_ps(bb) == 0) return bb;
+    mask = _mm256_and_ps(mask, bb);
+    if (T.ni[4 * i] == 0) {
+        const __m256 hl = refRec8(T, i + 1, A, AB, AC, o, d, mask, eps, trsIdx, distance);
+        const __m256 hr = refRec8(T, T.next[i + 1], A, AB, AC, o, d, mask, eps, trsIdx, distance);
+        return _mm256_or_ps(hl, hr);
+    }
+    __m256 hit = _mm256_setzero_ps(), minDist = _mm256_set1_ps(INFTY);
+    uint32_t minIdx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t first = (uint32_t)T.ni[4 * i + 2], cnt = (uint32_t)T.ni[4 * i + 3];
+    for (uint32_t k = 0; k < cnt; k++) {
+        const uint32_t t = T.tris[first + k];
+        __m256 tmp = _mm256_setzero_ps();
+        const __m256 r = mt8(A + 3 * (size_t)t, AB + 3 * (size_t)t, AC + 3 * (size_t)t, o, d, mask, eps, tmp);
+        const __m256 check = _mm256_and_ps(
+            r, _mm256_and_ps(_mm256_cmp_ps(tmp, minDist, _CMP_LT_OQ), _mm256_cmp_ps(tmp, _mm256_setzero_ps(), _CMP_GT_OQ)));
+        const int hm = _mm256_movemask_ps(check);
+        if (hm == 0) continue;
+        minDist = _mm256_blendv_ps(minDist, tmp, check);
+        for (int j = 0; j < 8; j++)
+            if ((hm >> j) & 1) minIdx[j] = t;
+        hit = _mm256_or_ps(hit, check);
+    }
+    const __m256 endCheck = _mm256_and_ps(hit, _mm256_cmp_ps(minDist, distance, _CMP_LT_OQ));
+    const int em = _mm256_movemask_ps(endCheck);
+    for (int j = 0; j < 8; j++)
+        if ((em >> j) & 1) trsIdx[j] = minIdx[j];
+    distance = _mm256_blendv_ps(distance, minDist, endCheck);
+    return endCheck;
+}
+
+void loadPacket(const float *rays, uint64_t n, uint64_t base, __m256 *o, __m256 *d) {
+    alignas(32) float c[6][8];
+    for (int j = 0; j < 8; j++) {
+        const uint64_t r = base + j < n ? base + j : n - 1;  // pad with the last ray
+        for (int k = 0; k < 6; k++) c[k][j] = rays[6 * r + k];
+    }
+    for (int k = 0; k < 3; k++) o[k] = _mm256_load_ps(c[k]), d[k] = _mm256_load_ps(c[3 + k]);
+}
+
+}  // namespace
+
+// ================================================================== C ABI ==
+
+ORC_API void orc_philox(const uint32_t *ctr, uint32_t k0, uint32_t k1, uint32_t *out) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox(c, k0, k1);
+    std::memcpy(out, c, 16);
+}
+
+ORC_API void orc_sincos2pi(const float *u, uint64_t n, float *s, float *c) {
+    for (uint64_t i = 0; i < n; i++) sincos2pi(u[i], s[i], c[i]);
+}
+
+ORC_API void orc_sample_hemisphere(const float *normals, const float *u1, const float *u2, uint64_t n, float *out) {
+    for (uint64_t i = 0; i < n; i++) {
+        const V d = sampleHemisphere(ld(normals + 3 * i), u1[i], u2[i]);
+        out[3 * i] = d.x, out[3 * i + 1] = d.y, out[3 * i + 2] = d.z;
+    }
+}
+
+// Ray stream of pray_gen_rays_device.
+ORC_API void orc_gen_rays(uint64_t seed, const float *box6, uint64_t n, float *rays) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t r[4] = {(uint32_t)i, 0u, 0u, 0u}, q[4] = {(uint32_t)i, 1u, 0u, 0u};
+        philox(r, k0, k1);
+        philox(q, k0, k1);
+        const float ox = box6[0] + box6[3] * u01(r[0]);
+        const float oy = box6[1] + box6[4] * u01(r[1]);
+        const float oz = box6[2] + box6[5] * u01(r[2]);
+        const float z = 1.0f - 2.0f * u01(r[3]);
+        const float rxy = std::sqrt(std::fmax(0.f, 1.0f - z * z));
+        float sp, cp;
+        sincos2pi(u01(q[0]), sp, cp);
+        V dd = {rxy * cp, rxy * sp, z};
+        normalize(dd);
+        float *o = rays + 6 * i;
+        o[0] = ox, o[1] = oy, o[2] = oz, o[3] = dd.x, o[4] = dd.y, o[5] = dd.z;
+    }
+}
+
+// initialiseTriangles (scene_preprocessor.cpp:79-94): ab, ac, normal.
+ORC_API void orc_init_tris(const float *abc, uint64_t n, float *ab, float *ac, float *normal) {
+    for (uint64_t i = 0; i < n; i++) {
+        const V a = ld(abc + 9 * i), b = ld(abc + 9 * i + 3), c = ld(abc + 9 * i + 6);
+        const V e1 = sub(b, a), e2 = sub(c, a);
+        V nn = cross(e1, e2);
+        normalize(nn);
+        std::memcpy(ab + 3 * i, &e1, 12);
+        std::memcpy(ac + 3 * i, &e2, 12);
+        std::memcpy(normal + 3 * i, &nn, 12);
+    }
+}
+
+// Single MT test (scalar and AVX paths) for unit tests.
+ORC_API int orc_mt(const float *ray6, const float *a, const float *ab, const float *ac, float eps, float *t) {
+    float tt = 0;
+    const bool h = mt(ld(ray6), ld(ray6 + 3), ld(a), ld(ab), ld(ac), eps, tt);
+    *t = h ? tt : NAN;
+    return h ? 1 : 0;
+}
+
+// AVX MT over rays (in packets of 8) against one triangle; hit mask + t.
+ORC_API void orc_mt_avx(const float *rays, uint64_t n, const float *a, const float *ab, const float *ac, float eps,
+                        int32_t *hit, float *t) {
+    for (uint64_t b = 0; b < n; b += 8) {
+        __m256 o[3], d[3];
+        loadPacket(rays, n, b, o, d);
+        __m256 dist = _mm256_set1_ps(NAN);
+        const __m256 m = mt8(a, ab, ac, o, d, _mm256_castsi256_ps(_mm256_set1_epi32(-1)), eps, dist);
+        alignas(32) float dv[8];
+        _mm256_store_ps(dv, dist);
+        const int mm = _mm256_movemask_ps(m);
+        for (int j = 0; j < 8 && b + j < n; j++) {
+            hit[b + j] = (mm >> j) & 1;
+            t[b + j] = ((mm >> j) & 1) ? dv[j] : NAN;
+        }
+    }
+}
+
+// Reference line-slab test, scalar (geometry.h:151-166) and AVX
+// (geometry_simd.cpp:6-40).
+ORC_API void orc_slab(const float *rays, uint64_t n, const float *box6, int32_t *scalar, int32_t *avx) {
+    for (uint64_t i = 0; i < n; i++) {
+        const V o = ld(rays + 6 * i), d = ld(rays + 6 * i + 3);
+        const V inv = {1 / d.x, 1 / d.y, 1 / d.z};
+        scalar[i] = slabLine(box6, box6 + 3, o, inv) ? 1 : 0;
+    }
+    for (uint64_t b = 0; b < n; b += 8) {
+        __m256 o[3], d[3];
+        loadPacket(rays, n, b, o, d);
+        const int m = _mm256_movemask_ps(slab8(box6, box6 + 3, o, d));
+        for (int j = 0; j < 8 && b + j < n; j++) avx[b + j] = (m >> j) & 1;
+    }
+}
+
+// Brute force (bf_search.cpp:26-40): global strict minimum over t > 0 in
+// index order, so the lowest index wins at equal t.
+ORC_API void orc_bf_closest(const float *A, const float *AB, const float *AC, uint64_t n_tris, const float *rays,
+                            uint64_t n, float eps, int32_t *out_idx, float *out_t) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint64_t i = 0; i < n; i++) {
+        const V o = ld(rays + 6 * i), d = ld(rays + 6 * i + 3);
+        float dist = INFTY;
+        uint32_t idx = NO_HIT;
+        for (uint64_t k = 0; k < n_tris; k++) {
+            float t;
+            if (mt(o, d, ld(A + 3 * k), ld(AB + 3 * k), ld(AC + 3 * k), eps, t) && t > 0 && t < dist) {
+                dist = t;
+                idx = (uint32_t)k;
+            }
+        }
+        out_idx[i] = (int32_t)idx;
+        out_t[i] = dist;
+    }
+}
+
+// Exhaustive reference traversal on a preorder tree export (scalar path).
+ORC_API void orc_ref_closest(const int32_t *ni, const float *nf, uint64_t n_nodes, const uint32_t *tris,
+                             const float *A, const float *AB, const float *AC, const float *rays, uint64_t n, float eps,
+                             int32_t *out_idx, float *out_t) {
+    PreTree T{ni, nf, tris, {}, n_nodes};
+    T.build();
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint64_t i = 0; i < n; i++) {
+        const V o = ld(rays + 6 * i), d = ld(rays + 6 * i + 3);
+        const V inv = {1 / d.x, 1 / d.y, 1 / d.z};
+        float dist = INFTY;
+        uint32_t idx = 0;
+        const bool hit = n_nodes && refRec(T, 0, A, AB, AC, o, d, inv, eps, idx, dist);
+        out_idx[i] = hit ? (int32_t)idx : -1;
+        out_t[i] = hit ? dist : INFTY;
+    }
+}
+
+// Packet traversal timed as the CPU baseline: OpenMP schedule(dynamic,10)
+// over 8-ray packets (pray.cpp:386) through the AVX recursion.  Returns the
+// wall time in seconds.
+ORC_API double orc_ref_closest_avx(const int32_t *ni, const float *nf, uint64_t n_nodes, const uint32_t *tris,
+                                   const float *A, const float *AB, const float *AC, const float *rays, uint64_t n,
+                                   float eps, int threads, int32_t *out_idx, float *out_t) {
+    PreTree T{ni, nf, tris, {}, n_nodes};
+    T.build();
+    const uint64_t packets = (n + 7) / 8;
+    const double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 10) num_threads(threads)
+    for (uint64_t p = 0; p < packets; p++) {
+        __m256 o[3], d[3];
+        loadPacket(rays, n, 8 * p, o, d);
+        __m256 dist = _mm256_set1_ps(INFTY);
+        uint32_t idx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const __m256 h = n_nodes ? refRec8(T, 0, A, AB, AC, o, d, _mm256_castsi256_ps(_mm256_set1_epi32(-1)), eps, idx,
+                                           dist)
+                                 : _mm256_setzero_ps();
+        alignas(32) float dv[8];
+        _mm256_store_ps(dv, dist);
+        const int m = _mm256_movemask_ps(h);
+        for (int j = 0; j < 8 && 8 * p + j < n; j++) {
+            out_idx[8 * p + j] = ((m >> j) & 1) ? (int32_t)idx[j] : -1;
+            out_t[8 * p + j] = ((m >> j) & 1) ? dv[j] : INFTY;
+        }
+    }
+    return omp_get_wtime() - t0;
+}
+
+// CPU twin of the GPU traversal over the flat layout.
+ORC_API void orc_flat_trace(const uint32_t *nodes, const uint32_t *refs, const float *A, const float *AB,
+                            const float *AC, const float *box6, uint64_t n_tris, const float *rays, uint64_t n,
+                            int mode, float eps, const float *lim2, int32_t *out_idx, float *out_t, uint32_t *counts) {
+    const Flat f{nodes, refs, A, AB, AC, box6};
+#pragma omp parallel for schedule(dynamic, 64)
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t cn = 0, cr = 0;
+        float t = INFINITY;
+        uint32_t h = NO_HIT;
+        if (n_tris) h = flatTrace(f, ld(rays + 6 * i), ld(rays + 6 * i + 3), mode, lim2 ? lim2[i] : 0.f, eps, t, cn, cr);
+        out_idx[i] = (int32_t)h;
+        out_t[i] = h == NO_HIT ? FLT_MAX : t;
+        if (counts) counts[2 * i] = cn, counts[2 * i + 1] = cr;
+    }
+}
+
+// Camera (scene_preprocessor.cpp:98-136): returns the 16 camera floats.
+ORC_API void orc_camera(const float *pos, const float *look, uint64_t fov, uint64_t w, uint64_t h, float *cam16) {
+    const V P = ld(pos), L = ld(look);
+    const V target = add(P, L);
+    const V up = {0, 1, 0};
+    V zaxis = sub(target, P);
+    normalize(zaxis);
+    V xaxis = cross(up, zaxis);
+    normalize(xaxis);
+    const V yaxis = cross(zaxis, xaxis);
+    float t[4][4] = {{xaxis.x, yaxis.x, zaxis.x, 0},
+                     {xaxis.y, yaxis.y, zaxis.y, 0},
+                     {xaxis.z, yaxis.z, zaxis.z, 0},
+                     {-dot(xaxis, P), -dot(yaxis, P), -dot(zaxis, P), 1}};
+    float s[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    // Matrix44::inverse (geometry.h:322-402).
+    bool singular = false;
+    for (int i = 0; i < 3 && !singular; i++) {
+        int pivot = i;
+        float ps = std::fabs(t[i][i]);
+        for (int j = i + 1; j < 4; j++) {
+            const float tmp = std::fabs(t[j][i]);
+            if (tmp > ps) pivot = j, ps = tmp;
+        }
+        if (ps == 0) {
+            singular = true;
+            break;
+        }
+        if (pivot != i)
+            for (int j = 0; j < 4; j++) std::swap(t[i][j], t[pivot][j]), std::swap(s[i][j], s[pivot][j]);
+        for (int j = i + 1; j < 4; j++) {
+            const float f = t[j][i] / t[i][i];
+            for (int k = 0; k < 4; k++) t[j][k] -= f * t[i][k], s[j][k] -= f * s[i][k];
+        }
+    }
+    for (int i = 3; i >= 0 && !singular; --i) {
+        float f = t[i][i];
+        if (f == 0) {
+            singular = true;
+            break;
+        }
+        for (int j = 0; j < 4; j++) t[i][j] /= f, s[i][j] /= f;
+        for (int j = 0; j < i; j++) {
+            f = t[j][i];
+            for (int k = 0; k < 4; k++) t[j][k] -= f * t[i][k], s[j][k] -= f * s[i][k];
+        }
+    }
+    if (singular)
+        for (int r = 0; r < 4; r++)
+            for (int q = 0; q < 4; q++) s[r][q] = r == q ? 1.f : 0.f;
+    const float PI = static_cast<float>(std::acos(-1));
+    cam16[0] = P.x, cam16[1] = P.y, cam16[2] = P.z;
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 3; q++) cam16[3 + 3 * r + q] = s[r][q];
+    cam16[12] = std::tan(PI * 0.5f * fov / 180.f);
+    cam16[13] = 1 / float(w);
+    cam16[14] = 1 / float(h);
+    cam16[15] = w / float(h);
+}
+
+// Primary rays from the 16 camera floats (scene_preprocessor.cpp:124-135).
+ORC_API void orc_camera_rays(const float *c, uint64_t w, uint64_t h, float *rays) {
+    for (uint64_t y = 0; y < h; y++) {
+        for (uint64_t x = 0; x < w; x++) {
+            const float px = (2 * ((x + 0.5f) * c[13]) - 1) * c[12];
+            const float py = (1 - 2 * ((y + 0.5f) * c[14])) * c[12] / c[15];
+            V d = {px * c[3] + py * c[6] + 1.f * c[9], px * c[4] + py * c[7] + 1.f * c[10],
+                   px * c[5] + py * c[8] + 1.f * c[11]};
+            normalize(d);
+            float *o = rays + 6 * (x + y * w);
+            o[0] = c[0], o[1] = c[1], o[2] = c[2], o[3] = d.x, o[4] = d.y, o[5] = d.z;
+        }
+    }
+}

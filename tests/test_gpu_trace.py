@@ -1,0 +1,184 @@
+"""GPU parity of the closest-hit / any-hit traversal (through the C ABI)
+against the oracle: bit-exact triangle indices and distances."""
+import numpy as np
+import pytest
+
+import nostrum_pray_amd as P
+import oracle as O
+from conftest import first_diff, say
+
+pytestmark = pytest.mark.gpu
+
+SCENES = {}
+
+
+def scene(name):
+    """(scene, tree, flat) built once per session."""
+    if name not in SCENES:
+        if name == "spheres-median":
+            s = P.Scene.spheres(1)
+            t = s.build(0)
+        elif name == "spheres-sah":
+            s = P.Scene.spheres(1)
+            t = s.build(1)
+        elif name == "terrain256-sah":
+            s = P.Scene.terrain(256, 2024)
+            t = s.build(1)
+        elif name == "terrain256-median":
+            s = P.Scene.terrain(256, 2024)
+            t = s.build(0)
+        elif name == "cornell20k-sah":
+            s = P.Scene.cornell(3, 20000)
+            t = s.build(1)
+        else:
+            raise KeyError(name)
+        SCENES[name] = (s, t, t.flat())
+    return SCENES[name]
+
+
+def check_closest(ctx, name, rays, eps, bf_rows=0):
+    s, t, flat = scene(name)
+    sid = ctx.upload(t)
+    try:
+        for layout in (P.LAYOUT_PACKED, P.LAYOUT_INDEXED):
+            ctx.set_tuning(sid, layout=layout)
+            say(f"closest {name} layout={layout} n={len(rays)} eps={eps}")
+            gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, eps)
+            oi, ot = O.flat_trace(flat, rays, 0, eps)
+            assert first_diff(gi, oi) is None, first_diff(gi, oi)
+            assert np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
+        if bf_rows:
+            sub = rays[:bf_rows]
+            bi, bt = O.bf_closest(flat["a"], flat["ab"], flat["ac"], sub, eps)
+            assert first_diff(gi[:bf_rows], bi) is None, first_diff(gi[:bf_rows], bi)
+            h = bi >= 0
+            assert np.array_equal(gt[:bf_rows][h], bt[h])
+        return gi, gt
+    finally:
+        ctx.free_scene(sid)
+
+
+@pytest.mark.parametrize("name", ["spheres-median", "spheres-sah", "cornell20k-sah"])
+def test_closest_camera_rays(ctx, name):
+    s, _, _ = scene(name)
+    rays = s.prim_rays()
+    gi, _ = check_closest(ctx, name, rays, 1e-4, bf_rows=4096)
+    assert (gi >= 0).mean() > 0.2
+
+
+@pytest.mark.parametrize("name,eps", [("spheres-median", 1e-4), ("spheres-sah", 1e-4), ("terrain256-sah", 1e-12),
+                                      ("terrain256-sah", 1e-4), ("terrain256-median", 1e-12),
+                                      ("cornell20k-sah", 1e-9)])
+def test_closest_incoherent(ctx, name, eps):
+    _, _, flat = scene(name)
+    rays = O.gen_rays(77, P.scene_box_expanded(flat["box"]), 1 << 16)
+    gi, gt = check_closest(ctx, name, rays, eps, bf_rows=2048)
+    if name.startswith("terrain") and eps == 1e-4:
+        # every terrain triangle is culled by the reference's absolute epsilon
+        assert (gi < 0).all()
+
+
+@pytest.mark.parametrize("mode", [P.TRACE_ANY, P.TRACE_ANY_INCLUSIVE])
+@pytest.mark.parametrize("name,eps", [("spheres-median", 1e-4), ("terrain256-sah", 1e-12), ("cornell20k-sah", 1e-9)])
+def test_any_hit(ctx, name, eps, mode):
+    s, t, flat = scene(name)
+    rays = O.gen_rays(91, P.scene_box_expanded(flat["box"]), 1 << 15)
+    ci, ct = O.flat_trace(flat, rays, 0, eps)
+    rng = np.random.default_rng(5)
+    np.seterr(over="ignore")  # t*t of misses (FLT_MAX) overflows; those lanes take the other branch
+    # limits around the closest hit distance, some exactly equal to t*t
+    lim = np.where(ci >= 0, (ct * ct) * rng.choice([0.5, 1.0, 2.0], size=len(ct)).astype(np.float32),
+                   rng.uniform(0, 4, len(ct)).astype(np.float32)).astype(np.float32)
+    sid = ctx.upload(t)
+    try:
+        say(f"any {name} mode={mode}")
+        gi, gt = ctx.trace(sid, rays, mode, eps, limit2=lim)
+    finally:
+        ctx.free_scene(sid)
+    sq = ct * ct
+    want = (ci >= 0) & ((sq < lim) if mode == P.TRACE_ANY else (sq <= lim))
+    assert np.array_equal(gi >= 0, want)
+    occ = np.flatnonzero(gi >= 0)
+    # every reported occluder is a genuine hit under the same predicate
+    for k in occ[:512]:
+        h, tt = O.mt(rays[k], flat["a"][gi[k]], flat["ab"][gi[k]], flat["ac"][gi[k]], eps)
+        assert h and tt > 0 and tt == gt[k]
+        assert (tt * np.float32(tt) < lim[k]) if mode == P.TRACE_ANY else (np.float32(tt) * np.float32(tt) <= lim[k])
+
+
+def test_work_counts_match_cpu_twin(ctx):
+    _, t, flat = scene("terrain256-sah")
+    rays = O.gen_rays(3, P.scene_box_expanded(flat["box"]), 1 << 14)
+    sid = ctx.upload(t)
+    try:
+        say("counts terrain256-sah")
+        n = rays.shape[0]
+        d_rays = ctx.to_device(rays)
+        d_idx = ctx.alloc(n, np.int32)
+        d_t = ctx.alloc(n, np.float32)
+        d_c = ctx.alloc((n, 2), np.uint32)
+        ctx.trace_counts_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, d_c.ptr, P.TRACE_CLOSEST, 1e-12)
+        ctx.sync()
+        gc = d_c.numpy()
+        gi = d_idx.numpy()
+    finally:
+        ctx.free_scene(sid)
+    oi, ot, oc = O.flat_trace(flat, rays, 0, 1e-12, counts=True)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(gc, oc)
+
+
+def test_gen_rays_bitwise(ctx):
+    box = np.array([-0.05, -0.01, -0.05, 1.1, 0.12, 1.1], np.float32)
+    say("gen rays")
+    g = ctx.gen_rays(123456789, box, 100_000)
+    o = O.gen_rays(123456789, box, 100_000)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+    d = g[:, 3:].astype(np.float64)
+    assert np.allclose((d * d).sum(1), 1.0, atol=1e-6)
+
+
+def test_camera_rays_bitwise(ctx):
+    s, _, _ = scene("spheres-median")
+    cam = s.camera()
+    say("camera rays")
+    g = ctx.camera_rays(cam, 512, 512)
+    assert np.array_equal(g.view(np.uint32), s.prim_rays().view(np.uint32))
+    if O.have_ref():
+        r = O.ref_prim_rays([0, 3, 8], [0, -0.35, -1], 60, 512, 512)
+        assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
+
+
+def test_empty_scene(ctx):
+    s = P.Scene.from_triangles(np.zeros((0, 9), np.float32))
+    t = s.build(0)
+    sid = ctx.upload(t)
+    try:
+        rays = O.gen_rays(1, np.array([0, 0, 0, 1, 1, 1], np.float32), 1000)
+        gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, 1e-4)
+        assert (gi == -1).all() and (gt == np.finfo(np.float32).max).all()
+    finally:
+        ctx.free_scene(sid)
+
+
+def test_axis_parallel_and_degenerate_rays(ctx):
+    """Rays with zero direction components and origins on split planes and
+    on the scene box faces (the slab/NaN cases of the interval arithmetic)."""
+    _, t, flat = scene("terrain256-sah")
+    rng = np.random.default_rng(8)
+    n = 20000
+    o = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    o[:, 1] = rng.choice([0.2, 0.05, 0.0, 0.1], n).astype(np.float32)
+    grid = (rng.integers(0, 257, n) / 256).astype(np.float32)
+    o[: n // 2, 0] = grid[: n // 2]  # exactly on cell boundaries (split candidates)
+    d = np.zeros((n, 3), np.float32)
+    axis = rng.integers(0, 3, n)
+    d[np.arange(n), axis] = rng.choice([-1.0, 1.0], n)
+    d[n // 3:, 1] = -1.0  # straight down
+    d[n // 4: n // 3] = rng.normal(size=(n // 3 - n // 4, 3)).astype(np.float32)
+    d[n // 4: n // 3, 0] = 0.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True).astype(np.float32)
+    d = d.astype(np.float32)
+    d[d == 0] = rng.choice([0.0, -0.0], int((d == 0).sum())).astype(np.float32)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    check_closest(ctx, "terrain256-sah", rays, 1e-12, bf_rows=3000)
