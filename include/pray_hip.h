@@ -155,6 +155,13 @@ PRAY_API int pray_render_pt(pray_ctx *ctx, int scene_id, const float *camera16, 
                    uint32_t spp, uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *background,
                    float det_eps, float *hdr, uint8_t *rgb);
 
+/* pray_render_pt with the sample pass either captured once into a hipGraph
+ * and replayed spp times (use_graph = 1, the default of pray_render_pt) or
+ * enqueued kernel by kernel (use_graph = 0); results are identical. */
+PRAY_API int pray_render_pt_ex(pray_ctx *ctx, int scene_id, const float *camera16, uint32_t width, uint32_t height,
+                               uint32_t spp, uint32_t max_depth, uint32_t rr_depth, uint64_t seed,
+                               const float *background, float det_eps, float *hdr, uint8_t *rgb, int use_graph);
+
 /* Device-memory and timing utilities (no reference counterpart: the
  * reference allocates and copies inside every call).  They let a caller keep
  * rays and hit records resident in HBM between calls and time work on the

@@ -23,6 +23,7 @@
 
 #include "../../include/pray_hip.h"
 #include "device_math.hpp"
+#include "render.hpp"
 #include "traverse.hpp"
 
 using prt::DevScene;
@@ -73,31 +74,7 @@ __global__ void k_gen_rays(uint32_t k0, uint32_t k1, float lx, float ly, float l
     o[0] = ox, o[1] = oy, o[2] = oz, o[3] = dx, o[4] = dy, o[5] = dz;
 }
 
-// Primary rays of the reference camera (scene_preprocessor.cpp:123-135).
-struct Cam {
-    float px, py, pz;
-    float m[9];
-    float angle, invW, invH, aspect;
-};
-
-__device__ __forceinline__ void camera_ray(const Cam &c, uint32_t xp, uint32_t yp, float &dx, float &dy, float &dz) {
-    const float x = (2.0f * (((float)xp + 0.5f) * c.invW) - 1.0f) * c.angle;
-    const float y = (1.0f - 2.0f * (((float)yp + 0.5f) * c.invH)) * c.angle / c.aspect;
-    const float z = 1.0f;
-    dx = x * c.m[0] + y * c.m[3] + z * c.m[6];
-    dy = x * c.m[1] + y * c.m[4] + z * c.m[7];
-    dz = x * c.m[2] + y * c.m[5] + z * c.m[8];
-    prd::normalize3(dx, dy, dz);
-}
-
-__global__ void k_camera_rays(Cam c, uint32_t w, uint32_t h, float *rays) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= w * h) return;
-    float dx, dy, dz;
-    camera_ray(c, i % w, i / w, dx, dy, dz);
-    float *o = rays + 6 * (size_t)i;
-    o[0] = c.px, o[1] = c.py, o[2] = c.pz, o[3] = dx, o[4] = dy, o[5] = dz;
-}
+using prr::Cam;
 
 // ------------------------------------------------------------ host side --
 
@@ -555,7 +532,7 @@ int pray_camera_rays(pray_ctx *c, const float *cam16, uint32_t width, uint32_t h
     std::memcpy(&cam, cam16, sizeof cam);
     DBuf dr;
     HIPC(c, hipMalloc(&dr.p, n * 24));
-    hipLaunchKernelGGL(k_camera_rays, dim3(nblocks(n, 256)), dim3(256), 0, c->stream, cam, width, height,
+    hipLaunchKernelGGL(prr::k_camera, dim3(nblocks(n, 256)), dim3(256), 0, c->stream, cam, width, height,
                        (float *)dr.p);
     HIPC(c, hipGetLastError());
     HIPC(c, hipMemcpyAsync(rays, dr.p, n * 24, hipMemcpyDeviceToHost, c->stream));
@@ -563,14 +540,198 @@ int pray_camera_rays(pray_ctx *c, const float *cam16, uint32_t width, uint32_t h
     return PRAY_OK;
 }
 
-int pray_render_rt(pray_ctx *c, int, const float *, uint32_t, uint32_t, const float *, uint32_t, const float *, float,
-                   uint8_t *, int32_t *) {
-    return fail(c, PRAY_ERR_UNSUPPORTED, "pray_render_rt not built yet");
+int pray_render_rt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_t h, const float *lights,
+                   uint32_t nl, const float *bg, float det_eps, uint8_t *rgb, int32_t *hit_idx) {
+    if (!c) return PRAY_ERR_INVALID;
+    SceneRec *sc = find_scene(c, scene_id);
+    if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    if (!cam16 || !bg || !rgb || (nl && !lights)) return fail(c, PRAY_ERR_INVALID, "null argument");
+    const uint64_t npix = (uint64_t)w * h;
+    if (npix == 0) return PRAY_OK;
+    if (npix > (1u << 30) || (uint64_t)nl * npix > (1ull << 31))
+        return fail(c, PRAY_ERR_INVALID, "image or light count too large");
+    if (sc->d.n_tris && (!sc->d.normal || !sc->d.color))
+        return fail(c, PRAY_ERR_INVALID, "scene was uploaded without shading data");
+    HIPC(c, hipSetDevice(c->device));
+    prr::RtGrid g{};
+    g.w = w, g.h = h;
+    if (w >= 4 && h >= 2) {
+        g.tx = (w - 4) / 4 + 1, g.ty = (h - 2) / 2 + 1, g.missing = w % 4, g.rows_sc = h - 1;
+        g.n_avx = 8 * g.tx * g.ty;
+        g.n_sc = g.rows_sc * (g.missing + 1) + ((h & 1u) ? w : 0u);
+    } else {  // too small for a packet: every pixel takes the scalar path
+        g.tx = g.ty = 0, g.missing = w - 1, g.rows_sc = h, g.n_avx = 0, g.n_sc = w * h;
+    }
+    const uint64_t lanes = (uint64_t)g.n_avx + g.n_sc;
+    const uint64_t nsh = lanes * nl;
+    Cam cam;
+    std::memcpy(&cam, cam16, sizeof cam);
+    DBuf rays, hit, tt, lt, sr, sl, so, out;
+    HIPC(c, hipMalloc(&rays.p, npix * 24));
+    HIPC(c, hipMalloc(&hit.p, npix * 4));
+    HIPC(c, hipMalloc(&tt.p, npix * 4));
+    HIPC(c, hipMalloc(&out.p, npix * 3));
+    if (nl) {
+        HIPC(c, hipMalloc(&lt.p, (size_t)nl * 24));
+        HIPC(c, hipMalloc(&sr.p, nsh * 24));
+        HIPC(c, hipMalloc(&sl.p, nsh * 4));
+        HIPC(c, hipMalloc(&so.p, nsh * 4));
+        HIPC(c, hipMemcpyAsync(lt.p, lights, (size_t)nl * 24, hipMemcpyHostToDevice, c->stream));
+    }
+    hipLaunchKernelGGL(prr::k_camera, dim3(nblocks(npix, 256)), dim3(256), 0, c->stream, cam, w, h, (float *)rays.p);
+    HIPC(c, hipGetLastError());
+    int rc = enqueue_trace(c, *sc, (const float *)rays.p, npix, nullptr, PRAY_TRACE_CLOSEST, det_eps, nullptr,
+                           (int32_t *)hit.p, (float *)tt.p, nullptr, c->stream);
+    if (rc) return rc;
+    prr::RtArgs a;
+    a.g = g;
+    a.rays = (const float *)rays.p;
+    a.hit = (const int32_t *)hit.p;
+    a.t = (const float *)tt.p;
+    a.lights = (const float *)lt.p;
+    a.nl = nl;
+    a.bg[0] = bg[0], a.bg[1] = bg[1], a.bg[2] = bg[2];
+    a.srays = (float *)sr.p;
+    a.slim = (float *)sl.p;
+    a.socc = (const int32_t *)so.p;
+    a.rgb = (uint8_t *)out.p;
+    if (nl && sc->d.n_tris) {
+        hipLaunchKernelGGL(prr::k_rt_shadow_setup, dim3(nblocks(lanes, 256)), dim3(256), 0, c->stream, sc->d, a);
+        HIPC(c, hipGetLastError());
+        rc = enqueue_trace(c, *sc, (const float *)sr.p, nsh, nullptr, PRAY_TRACE_ANY_INCLUSIVE, det_eps,
+                           (const float *)sl.p, (int32_t *)so.p, nullptr, nullptr, c->stream);
+        if (rc) return rc;
+    } else if (nl) {
+        hipLaunchKernelGGL(k_fill_miss, dim3(nblocks(nsh, 256)), dim3(256), 0, c->stream, (uint32_t)nsh,
+                           (int32_t *)so.p, (float *)nullptr, (uint32_t *)nullptr);
+    }
+    hipLaunchKernelGGL(prr::k_rt_shade, dim3(nblocks(lanes, WAVE)), dim3(WAVE), 0, c->stream, sc->d, a);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipMemcpyAsync(rgb, out.p, npix * 3, hipMemcpyDeviceToHost, c->stream));
+    if (hit_idx) HIPC(c, hipMemcpyAsync(hit_idx, hit.p, npix * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return PRAY_OK;
 }
 
-int pray_render_pt(pray_ctx *c, int, const float *, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t,
-                   const float *, float, float *, uint8_t *) {
-    return fail(c, PRAY_ERR_UNSUPPORTED, "pray_render_pt not built yet");
+int pray_render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_t h, uint32_t spp,
+                   uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *bg, float det_eps, float *hdr,
+                   uint8_t *rgb) {
+    return pray_render_pt_ex(c, scene_id, cam16, w, h, spp, max_depth, rr_depth, seed, bg, det_eps, hdr, rgb, 1);
+}
+
+int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_t h, uint32_t spp,
+                      uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *bg, float det_eps, float *hdr,
+                      uint8_t *rgb, int use_graph) {
+    if (!c) return PRAY_ERR_INVALID;
+    SceneRec *sc = find_scene(c, scene_id);
+    if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    if (!cam16 || !bg) return fail(c, PRAY_ERR_INVALID, "null argument");
+    const uint64_t npix = (uint64_t)w * h;
+    if (npix == 0 || spp == 0) return PRAY_OK;
+    if (npix > (1u << 28)) return fail(c, PRAY_ERR_INVALID, "image too large");
+    if (sc->d.n_tris && (!sc->d.normal || !sc->d.color || !sc->d.emit))
+        return fail(c, PRAY_ERR_INVALID, "scene was uploaded without shading data");
+    HIPC(c, hipSetDevice(c->device));
+    Cam cam;
+    std::memcpy(&cam, cam16, sizeof cam);
+    DBuf prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb;
+    HIPC(c, hipMalloc(&prim.p, npix * 24));
+    for (int q = 0; q < 2; q++) {
+        HIPC(c, hipMalloc(&rq[q].p, npix * 24));
+        HIPC(c, hipMalloc(&sq[q].p, npix * 16));
+    }
+    HIPC(c, hipMalloc(&cnt.p, 4 * sizeof(uint32_t)));  // count0, count1, sample
+    HIPC(c, hipMalloc(&hit.p, npix * 4));
+    HIPC(c, hipMalloc(&tt.p, npix * 4));
+    HIPC(c, hipMalloc(&contrib.p, npix * 12));
+    HIPC(c, hipMalloc(&sum.p, npix * 12));
+    HIPC(c, hipMalloc(&outh.p, npix * 12));
+    if (rgb) HIPC(c, hipMalloc(&outb.p, npix * 3));
+    uint32_t *count = (uint32_t *)cnt.p;
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(prr::k_camera, dim3(nblocks(npix, 256)), dim3(256), 0, st, cam, w, h, (float *)prim.p);
+    hipLaunchKernelGGL(prr::k_fill_f32, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (float *)sum.p,
+                       (uint32_t)(3 * npix), 0.f);
+    hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count, 4u);
+    HIPC(c, hipGetLastError());
+    prr::PtArgs P;
+    P.n_pix = (uint32_t)npix;
+    P.max_depth = max_depth;
+    P.rr_depth = rr_depth;
+    P.k0 = (uint32_t)seed, P.k1 = (uint32_t)(seed >> 32);
+    P.bg[0] = bg[0], P.bg[1] = bg[1], P.bg[2] = bg[2];
+    P.sample = count + 2;
+    // The persistent trace kernel needs its spill area before any capture.
+    {
+        TraceKernel k = pick_kernel(false, sc->tune.layout, sc->tune.stack_lds, sc->tune.leaf_ilp);
+        const uint64_t want = (npix + WAVE - 1) / WAVE;
+        const uint64_t fit = (uint64_t)c->num_cus * (uint64_t)resident_blocks(c, k);
+        const size_t need = (size_t)std::max<uint64_t>(1, std::min(want, fit)) * WAVE *
+                            std::max<uint32_t>(1, sc->d.max_depth) * sizeof(uint4);
+        int rc = ensure_spill(c, need);
+        if (rc) return rc;
+    }
+    // One sample pass: generate, (zero next count, extend, shade) per bounce,
+    // accumulate.
+    auto pass = [&]() -> int {
+        hipLaunchKernelGGL(prr::k_pt_generate, dim3(nblocks(npix, 256)), dim3(256), 0, st, (const float *)prim.p,
+                           (uint32_t)npix, (float *)rq[0].p, (float4 *)sq[0].p, count + 0);
+        for (uint32_t b = 0; b <= max_depth; b++) {
+            const int cur = b & 1, nxt = cur ^ 1;
+            hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + nxt, 1u);
+            if (sc->d.n_tris) {
+                int rc = enqueue_trace(c, *sc, (const float *)rq[cur].p, npix, count + cur, PRAY_TRACE_CLOSEST,
+                                       det_eps, nullptr, (int32_t *)hit.p, (float *)tt.p, nullptr, st);
+                if (rc) return rc;
+            } else {
+                hipLaunchKernelGGL(k_fill_miss, dim3(nblocks(npix, 256)), dim3(256), 0, st, (uint32_t)npix,
+                                   (int32_t *)hit.p, (float *)tt.p, (uint32_t *)nullptr);
+            }
+            hipLaunchKernelGGL(prr::k_pt_shade, dim3(nblocks(npix, 256)), dim3(256), 0, st, sc->d, P, b,
+                               (const float *)rq[cur].p, (const float4 *)sq[cur].p, (const uint32_t *)(count + cur),
+                               (const int32_t *)hit.p, (const float *)tt.p, (float *)rq[nxt].p, (float4 *)sq[nxt].p,
+                               count + nxt, (float *)contrib.p);
+        }
+        hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
+                           (const float *)contrib.p, (float *)sum.p, count + 2);
+        HIPC(c, hipGetLastError());
+        return PRAY_OK;
+    };
+    if (use_graph) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPC(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = pass();
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, PRAY_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(c, PRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+        }
+        for (uint32_t s = 0; s < spp && e == hipSuccess; s++) e = hipGraphLaunch(exec, st);
+        const hipError_t e2 = hipStreamSynchronize(st);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess || e2 != hipSuccess)
+            return fail(c, PRAY_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    } else {
+        for (uint32_t s = 0; s < spp; s++) {
+            int rc = pass();
+            if (rc) return rc;
+        }
+    }
+    hipLaunchKernelGGL(prr::k_pt_finalize, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
+                       (const float *)sum.p, (float)spp, (float *)outh.p, (uint8_t *)outb.p);
+    HIPC(c, hipGetLastError());
+    if (hdr) HIPC(c, hipMemcpyAsync(hdr, outh.p, npix * 12, hipMemcpyDeviceToHost, st));
+    if (rgb) HIPC(c, hipMemcpyAsync(rgb, outb.p, npix * 3, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipStreamSynchronize(st));
+    return PRAY_OK;
 }
 
 int pray_device_alloc(pray_ctx *c, uint64_t bytes, void **p) {

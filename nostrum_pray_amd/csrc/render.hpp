@@ -1,0 +1,342 @@
+// Renderers on top of the traversal kernel.
+//
+// Ray tracing (reference rayTrace, pray.cpp:380-494).  The reference shades
+// 2x4 pixel packets with AVX (pray.cpp:58-196) and then re-renders the last
+// column(s) and an odd last row with the scalar path (pray.cpp:17-55).  The
+// two paths differ in rounding order, in the shadow comparison (>= vs >), in
+// re-normalising the shadow direction, and in packet-wide behaviour: after
+// the shadow query the lambert mask is not applied again, so a back-facing
+// light adds a negative term whenever another lane of the packet faces it,
+// and a packet stops at the first light after which all eight lanes are white.
+// Here each packet is eight adjacent lanes of a wave (64 = 8 packets) and the
+// packet-wide decisions are ballots over those eight lanes; scalar-path
+// pixels get their own lanes after the packets.  Shadow rays of both paths go
+// through one inclusive any-hit launch (a strict limit L2 is passed as the
+// largest float below L2).
+//
+// Path tracing (NORMAL estimator, pray.cpp:342-378; path_trace.cu:395-464).
+// Wavefront: per sample pass, generate fills queue 0 with one path per pixel;
+// each bounce traces the queue (closest hit, device-side count) and a shade
+// kernel terminates paths (miss -> throughput * background, emitter ->
+// throughput * Le, depth limit or Russian roulette -> 0) or pushes the bounced
+// ray to the other queue (one atomic per wave).  Each path writes its
+// contribution exactly once per pass; an accumulate kernel adds the pass to
+// the sum in sample order, so results do not depend on queue order.  Random
+// numbers: Philox4x32-10 keyed by the 64-bit seed, counter (pixel, sample,
+// bounce, 0); the sample index lives in device memory so one captured graph
+// serves every pass.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_math.hpp"
+#include "traverse.hpp"
+
+namespace prr {
+
+using prt::DevScene;
+using prt::WAVE;
+
+struct Cam {
+    float px, py, pz;
+    float m[9];
+    float angle, invW, invH, aspect;
+};
+
+// Primary ray direction of the reference camera (scene_preprocessor.cpp:128-133).
+__device__ __forceinline__ void camera_dir(const Cam &c, uint32_t xp, uint32_t yp, float &dx, float &dy, float &dz) {
+    const float x = (2.0f * (((float)xp + 0.5f) * c.invW) - 1.0f) * c.angle;
+    const float y = (1.0f - 2.0f * (((float)yp + 0.5f) * c.invH)) * c.angle / c.aspect;
+    const float z = 1.0f;
+    dx = x * c.m[0] + y * c.m[3] + z * c.m[6];
+    dy = x * c.m[1] + y * c.m[4] + z * c.m[7];
+    dz = x * c.m[2] + y * c.m[5] + z * c.m[8];
+    prd::normalize3(dx, dy, dz);
+}
+
+// std::max(lo, std::min(hi, v)) (geometry.h:15): NaN maps to hi.
+__device__ __forceinline__ float clamp_std(float v) {
+    const float m = (v < 1.0f) ? v : 1.0f;
+    return (0.0f < m) ? m : 0.0f;
+}
+__device__ __forceinline__ uint8_t to_byte(float c) { return (uint8_t)(uint32_t)(c * 255.f + 0.5f); }
+
+// Pixel layout of the RT launch.  Lanes [0, n_avx) are packet lanes (8 per
+// packet, packets row-major), lanes [n_avx, n_avx + n_sc) scalar pixels.
+struct RtGrid {
+    uint32_t w, h, tx, ty, missing, n_avx, n_sc, rows_sc;
+    __device__ __forceinline__ bool packet_lane(uint32_t g, uint32_t &x, uint32_t &y) const {
+        if (g >= n_avx) return false;
+        const uint32_t tile = g >> 3, j = g & 7u;
+        x = 4 * (tile % tx) + (j & 3u);
+        y = 2 * (tile / tx) + (j >> 2);
+        return true;
+    }
+    __device__ __forceinline__ void scalar_pixel(uint32_t s, uint32_t &x, uint32_t &y) const {
+        const uint32_t per = missing + 1;
+        if (s < rows_sc * per) {
+            y = s / per;
+            x = w - 1 - missing + s % per;
+        } else {
+            y = h - 1;
+            x = s - rows_sc * per;
+        }
+    }
+    // Packet pixels later overwritten by the scalar pass (pray.cpp:466-478).
+    __device__ __forceinline__ bool overwritten(uint32_t x, uint32_t y) const {
+        return y + 1 < h && x + 1 + missing >= w;
+    }
+};
+
+struct RtArgs {
+    RtGrid g;
+    const float *rays;      // w*h primary rays
+    const int32_t *hit;     // w*h
+    const float *t;         // w*h
+    const float *lights;    // nl x 6
+    uint32_t nl;
+    float bg[3];
+    float *srays;           // nl x (n_avx + n_sc) shadow rays
+    float *slim;            // nl x (n_avx + n_sc) squared limits
+    const int32_t *socc;    // nl x (n_avx + n_sc) any-hit results
+    uint8_t *rgb;           // w*h*3
+};
+
+__device__ __forceinline__ void hit_point(const RtArgs &A, uint32_t p, float &x, float &y, float &z) {
+    const float *r = A.rays + 6 * (size_t)p;
+    const float t = A.t[p];
+    x = r[0] + r[3] * t;
+    y = r[1] + r[4] * t;
+    z = r[2] + r[5] * t;
+}
+
+// One shadow ray per (light, lane): direction and limit of the path the lane
+// follows; -1 marks lanes that shoot no shadow ray.
+__global__ void k_rt_shadow_setup(DevScene s, RtArgs A) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lanes = A.g.n_avx + A.g.n_sc;
+    if (g >= lanes) return;
+    uint32_t x, y;
+    const bool avx = A.g.packet_lane(g, x, y);
+    if (!avx) A.g.scalar_pixel(g - A.g.n_avx, x, y);
+    const uint32_t p = x + y * A.g.w;
+    const int32_t h = A.hit[p];
+    float px = 0, py = 0, pz = 0, nx = 0, ny = 0, nz = 0;
+    if (h >= 0) {
+        hit_point(A, p, px, py, pz);
+        nx = s.normal[3 * (size_t)h], ny = s.normal[3 * (size_t)h + 1], nz = s.normal[3 * (size_t)h + 2];
+    }
+    for (uint32_t l = 0; l < A.nl; l++) {
+        const size_t at = (size_t)l * lanes + g;
+        float *o = A.srays + 6 * at;
+        float lim = -1.0f;
+        if (h >= 0) {
+            float lx = A.lights[6 * l] - px, ly = A.lights[6 * l + 1] - py, lz = A.lights[6 * l + 2] - pz;
+            const float d2 = lx * lx + ly * ly + lz * lz;
+            if (avx) {
+                const float inv = 1.0f / sqrtf(d2);
+                lx *= inv, ly *= inv, lz *= inv;
+            } else {
+                prd::normalize3(lx, ly, lz);
+            }
+            const float lambert = lx * nx + ly * ny + lz * nz;
+            if (lambert >= 0.f) {
+                if (!avx) prd::normalize3(lx, ly, lz);  // Ray(intersectionPos, L)
+                o[0] = px, o[1] = py, o[2] = pz, o[3] = lx, o[4] = ly, o[5] = lz;
+                lim = avx ? d2 : nextafterf(d2, -INFINITY);
+            }
+        }
+        if (lim < 0.f) o[0] = o[1] = o[2] = o[3] = o[4] = o[5] = 0.f, o[5] = 1.f;
+        A.slim[at] = lim;
+    }
+}
+
+__global__ __launch_bounds__(WAVE) void k_rt_shade(DevScene s, RtArgs A) {
+    const uint32_t g = blockIdx.x * WAVE + threadIdx.x;
+    const uint32_t lanes = A.g.n_avx + A.g.n_sc;
+    const uint32_t lane = threadIdx.x, grp = lane & ~7u;
+    const bool valid = g < lanes;
+    uint32_t x = 0, y = 0;
+    const bool avx = valid && A.g.packet_lane(g, x, y);
+    if (valid && !avx) A.g.scalar_pixel(g - A.g.n_avx, x, y);
+    const uint32_t p = x + y * A.g.w;
+    const int32_t h = valid ? A.hit[p] : -1;
+    const bool hit = h >= 0;
+    float px = 0, py = 0, pz = 0, nx = 0, ny = 0, nz = 0, ar = 0, ag = 0, ab = 0;
+    if (hit) {
+        hit_point(A, p, px, py, pz);
+        nx = s.normal[3 * (size_t)h], ny = s.normal[3 * (size_t)h + 1], nz = s.normal[3 * (size_t)h + 2];
+        ar = s.color[3 * (size_t)h], ag = s.color[3 * (size_t)h + 1], ab = s.color[3 * (size_t)h + 2];
+    }
+    auto group = [&](bool pred) -> uint32_t { return (uint32_t)((__ballot(pred) >> grp) & 0xFFull); };
+    const bool packet_hit = group(avx && hit) != 0;
+    float cr = (avx && !hit) ? A.bg[0] : 0.f, cg = (avx && !hit) ? A.bg[1] : 0.f, cb = (avx && !hit) ? A.bg[2] : 0.f;
+    bool done = !valid || (avx && !packet_hit) || (!avx && !hit);
+    for (uint32_t l = 0; l < A.nl; l++) {
+        const float lr = A.lights[6 * l + 3], lg = A.lights[6 * l + 4], lb = A.lights[6 * l + 5];
+        float lx = A.lights[6 * l] - px, ly = A.lights[6 * l + 1] - py, lz = A.lights[6 * l + 2] - pz;
+        const float d2 = lx * lx + ly * ly + lz * lz;
+        if (avx) {
+            const float inv = 1.0f / sqrtf(d2);
+            lx *= inv, ly *= inv, lz *= inv;
+        } else {
+            prd::normalize3(lx, ly, lz);
+        }
+        const float lambert = lx * nx + ly * ny + lz * nz;
+        const bool occluded = valid && A.socc[(size_t)l * lanes + g] >= 0;
+        if (avx) {
+            // packet-wide: skip the light unless some lane faces it; then every
+            // hit lane that is not occluded adds its (possibly negative) term
+            const bool smask = hit && lambert >= 0.f;
+            const bool any_s = group(avx && !done && smask) != 0;
+            const bool lit = hit && !(smask && occluded);
+            const bool any_lit = group(avx && !done && any_s && lit) != 0;
+            if (!done && any_s && any_lit && lit) {
+                const float decay = lambert / d2;
+                cr = cr + ar * (lr * decay);
+                cg = cg + ag * (lg * decay);
+                cb = cb + ab * (lb * decay);
+            }
+            const bool white = cr >= 1.f && cg >= 1.f && cb >= 1.f;
+            const uint32_t wmask = group(avx && white);
+            if (avx && !done && any_s && any_lit && wmask == 0xFFu) done = true;
+        } else if (!done) {
+            if (lambert < 0.f || occluded) continue;
+            const float inv = 1.0f / d2;
+            cr = cr + ((ar * lr) * lambert) * inv;
+            cg = cg + ((ag * lg) * lambert) * inv;
+            cb = cb + ((ab * lb) * lambert) * inv;
+            if (cr >= 1.f && cg >= 1.f && cb >= 1.f) done = true;
+        }
+    }
+    if (!valid) return;
+    uint8_t *o = A.rgb + 3 * (size_t)p;
+    if (avx) {
+        if (A.g.overwritten(x, y)) return;
+        if (!packet_hit) {
+            o[0] = to_byte(A.bg[0]), o[1] = to_byte(A.bg[1]), o[2] = to_byte(A.bg[2]);
+        } else {
+            o[0] = to_byte(clamp_std(cr)), o[1] = to_byte(clamp_std(cg)), o[2] = to_byte(clamp_std(cb));
+        }
+    } else if (!hit) {
+        o[0] = to_byte(A.bg[0]), o[1] = to_byte(A.bg[1]), o[2] = to_byte(A.bg[2]);
+    } else {
+        o[0] = to_byte(clamp_std(cr)), o[1] = to_byte(clamp_std(cg)), o[2] = to_byte(clamp_std(cb));
+    }
+}
+
+__global__ void k_camera(Cam c, uint32_t w, uint32_t h, float *rays) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    float dx, dy, dz;
+    camera_dir(c, i % w, i / w, dx, dy, dz);
+    float *o = rays + 6 * (size_t)i;
+    o[0] = c.px, o[1] = c.py, o[2] = c.pz, o[3] = dx, o[4] = dy, o[5] = dz;
+}
+
+// ------------------------------------------------------------- path trace --
+
+struct PtArgs {
+    uint32_t n_pix, max_depth, rr_depth, k0, k1;
+    float bg[3];
+    const uint32_t *sample;   // current sample index (device)
+};
+
+// Queue entries: ray (6 floats) + state float4 {throughput rgb, pixel bits}.
+__global__ void k_pt_generate(const float *prim, uint32_t n_pix, float *rays, float4 *state, uint32_t *count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *count = n_pix;
+    if (i >= n_pix) return;
+    const float *r = prim + 6 * (size_t)i;
+    float *o = rays + 6 * (size_t)i;
+    o[0] = r[0], o[1] = r[1], o[2] = r[2], o[3] = r[3], o[4] = r[4], o[5] = r[5];
+    state[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(i));
+}
+
+__global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, uint32_t bounce, const float *rays_in,
+                                                  const float4 *state_in, const uint32_t *count_in,
+                                                  const int32_t *hit, const float *tt, float *rays_out,
+                                                  float4 *state_out, uint32_t *count_out, float *contrib) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *count_in;
+    const bool active = i < n;
+    bool push = false;
+    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, tr = 0, tg = 0, tb = 0;
+    uint32_t pix = 0;
+    if (active) {
+        const float4 st = state_in[i];
+        pix = __float_as_uint(st.w);
+        const int32_t h = hit[i];
+        float cr = 0.f, cg = 0.f, cb = 0.f;
+        bool term = true;
+        if (h < 0) {
+            cr = st.x * P.bg[0], cg = st.y * P.bg[1], cb = st.z * P.bg[2];
+        } else if (s.emit[h]) {
+            cr = st.x * s.color[3 * (size_t)h], cg = st.y * s.color[3 * (size_t)h + 1],
+            cb = st.z * s.color[3 * (size_t)h + 2];
+        } else if (bounce < P.max_depth) {
+            const float *r = rays_in + 6 * (size_t)i;
+            const float t = tt[i];
+            ox = r[0] + r[3] * t, oy = r[1] + r[4] * t, oz = r[2] + r[5] * t;
+            const float ar = s.color[3 * (size_t)h], ag = s.color[3 * (size_t)h + 1], ab = s.color[3 * (size_t)h + 2];
+            const uint4 u = prd::philox4x32_10(make_uint4(pix, *P.sample, bounce, 0u), P.k0, P.k1);
+            prd::sample_hemisphere(s.normal[3 * (size_t)h], s.normal[3 * (size_t)h + 1], s.normal[3 * (size_t)h + 2],
+                                   prd::u01(u.x), prd::u01(u.y), dx, dy, dz);
+            tr = st.x * ar, tg = st.y * ag, tb = st.z * ab;
+            bool live = true;
+            if (bounce >= P.rr_depth) {
+                const float q = fmaxf(fmaxf(ar, ag), ab);
+                if (q < 1.f) {
+                    if (!(prd::u01(u.z) < q))
+                        live = false;
+                    else
+                        tr = tr / q, tg = tg / q, tb = tb / q;
+                }
+            }
+            term = !live;
+            push = live;
+        }
+        if (term) {
+            float *c = contrib + 3 * (size_t)pix;
+            c[0] = cr, c[1] = cg, c[2] = cb;
+        }
+    }
+    // one atomic per wave for the surviving paths
+    const uint64_t m = __ballot(push);
+    if (m) {
+        const uint32_t lane = threadIdx.x & (WAVE - 1);
+        const uint32_t first = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+        uint32_t base = 0;
+        if (lane == first) base = atomicAdd(count_out, (uint32_t)__popcll(m));
+        base = __shfl(base, (int)first);
+        if (push) {
+            const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            float *o = rays_out + 6 * (size_t)slot;
+            o[0] = ox, o[1] = oy, o[2] = oz, o[3] = dx, o[4] = dy, o[5] = dz;
+            state_out[slot] = make_float4(tr, tg, tb, __uint_as_float(pix));
+        }
+    }
+}
+
+// sum += contribution of this pass (sample order), then advance the sample.
+__global__ void k_pt_accumulate(uint32_t n_pix, const float *contrib, float *sum, uint32_t *sample) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * n_pix) sum[i] = sum[i] + contrib[i];
+    if (i == 0) *sample = *sample + 1;
+}
+
+__global__ void k_pt_finalize(uint32_t n_pix, const float *sum, float spp, float *hdr, uint8_t *rgb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n_pix) return;
+    const float m = sum[i] / spp;
+    if (hdr) hdr[i] = m;
+    if (rgb) rgb[i] = to_byte(clamp_std(m));
+}
+
+__global__ void k_fill_f32(float *p, uint32_t n, float v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace prr

@@ -199,6 +199,45 @@ def flat_trace(flat, rays, mode=0, eps=1e-4, limit2=None, counts=False):
     return (idx, t, cnt) if counts else (idx, t)
 
 
+def render_rt(flat, cam16, w, h, lights, bg, eps):
+    """The reference rayTrace restated (AVX packets + scalar remainder)."""
+    L = lib()
+    L.orc_render_rt.restype = None
+    L.orc_render_rt.argtypes = [_vp] * 9 + [_u64, _u64, _vp, C.c_int, _vp, _f32, _vp]
+    lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 6)
+    rgb = np.empty((h, w, 3), np.uint8)
+    L.orc_render_rt(_p(flat["nodes"]), _p(flat["refs"]), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]),
+                    _p(flat["box"]), _p(flat["normal"]), _p(flat["color"]), _p(np.ascontiguousarray(cam16, np.float32)),
+                    w, h, _p(lights), lights.shape[0], _p(np.ascontiguousarray(bg, np.float32)), eps, _p(rgb))
+    return rgb
+
+
+def render_pt(flat, cam16, w, h, spp, max_depth, rr_depth, seed, bg, eps):
+    """Path tracing restated with the build's Philox streams; (hdr, rgb)."""
+    L = lib()
+    L.orc_render_pt.restype = None
+    L.orc_render_pt.argtypes = [_vp] * 10 + [_u64, _u64, _u32, _u32, _u32, _u64, _vp, _f32, _vp, _vp]
+    hdr = np.empty((h, w, 3), np.float32)
+    rgb = np.empty((h, w, 3), np.uint8)
+    L.orc_render_pt(_p(flat["nodes"]), _p(flat["refs"]), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]),
+                    _p(flat["box"]), _p(flat["normal"]), _p(flat["color"]), _p(flat["emit"]),
+                    _p(np.ascontiguousarray(cam16, np.float32)), w, h, spp, max_depth, rr_depth, seed,
+                    _p(np.ascontiguousarray(bg, np.float32)), eps, _p(hdr), _p(rgb))
+    return hdr, rgb
+
+
+def read_bmp(path):
+    """24-bit BMP (as stb_image_write writes it) -> (h, w, 3) RGB, top row first."""
+    data = open(path, "rb").read()
+    off = int.from_bytes(data[10:14], "little")
+    w = int.from_bytes(data[18:22], "little", signed=True)
+    h = int.from_bytes(data[22:26], "little", signed=True)
+    row = (w * 3 + 3) & ~3
+    px = np.frombuffer(data, np.uint8, count=row * abs(h), offset=off).reshape(abs(h), row)[:, : w * 3]
+    img = px.reshape(abs(h), w, 3)[:, :, ::-1]
+    return img[::-1].copy() if h > 0 else img.copy()
+
+
 def camera(pos, look, fov, w, h):
     out = np.empty(16, np.float32)
     lib().orc_camera(_p(np.ascontiguousarray(pos, np.float32)), _p(np.ascontiguousarray(look, np.float32)), fov, w,
@@ -272,3 +311,29 @@ def ref_prim_rays(pos, look, fov, w, h, fma=False):
     ref(fma).ref_prim_rays(_p(np.ascontiguousarray(pos, np.float32)), _p(np.ascontiguousarray(look, np.float32)),
                            fov, w, h, _p(out))
     return out
+
+
+def ref_load_scene(path, fma=False):
+    """The reference's readSceneDescription (io.cpp:27-137): dict with info,
+    triangles (n x 9), colours, emit flags, lights and background."""
+    L = ref(fma)
+    L.ref_load_scene.restype, L.ref_load_scene.argtypes = _vp, [C.c_char_p]
+    L.ref_scene_free.restype, L.ref_scene_free.argtypes = None, [_vp]
+    L.ref_scene_get.restype, L.ref_scene_get.argtypes = None, [_vp] * 7
+    h = L.ref_load_scene(os.fsencode(path))
+    try:
+        info = np.zeros(16, np.float64)
+        L.ref_scene_get(h, _p(info), None, None, None, None, None)
+        n, nl = int(info[7]), int(info[6])
+        tris = np.empty((n, 9), np.float32)
+        col = np.empty((n, 3), np.float32)
+        emit = np.empty(n, np.uint8)
+        lights = np.empty((nl, 6), np.float32)
+        bg = np.empty(3, np.float32)
+        L.ref_scene_get(h, None, _p(tris), _p(col), _p(emit), _p(lights), _p(bg))
+    finally:
+        L.ref_scene_free(h)
+    return {"mode": int(info[0]), "width": int(info[1]), "height": int(info[2]), "fov": int(info[3]),
+            "max_depth": int(info[4]), "spp": int(info[5]), "tris": tris, "color": col, "emit": emit,
+            "lights": lights, "background": bg, "camera_position": info[8:11].astype(np.float32),
+            "camera_look": info[11:14].astype(np.float32)}

@@ -623,6 +623,235 @@ ORC_API void orc_flat_trace(const uint32_t *nodes, const uint32_t *refs, const f
     }
 }
 
+// -------------------------------------------------------------- shading --
+ORC_API void orc_camera_rays(const float *c, uint64_t w, uint64_t h, float *rays);
+
+namespace {
+
+struct Shade {
+    Flat f;
+    const float *normal, *color;
+    const uint8_t *emit;
+    float eps;
+};
+
+// Closest hit through the flat twin (equal to brute force on every ray).
+inline uint32_t closest(const Shade &S, V o, V d, float &t) {
+    uint32_t cn = 0, cr = 0;
+    return flatTrace(S.f, o, d, 0, 0.f, S.eps, t, cn, cr);
+}
+
+// std::max(lo, std::min(hi, v)) with the standard library's argument order
+// (geometry.h:15): NaN becomes hi.
+inline float clampStd(float lo, float hi, float v) {
+    const float m = (v < hi) ? v : hi;
+    return (lo < m) ? m : lo;
+}
+inline uint8_t toByte(float c) { return static_cast<uint8_t>(c * 255.f + 0.5f); }
+
+// Scalar evaluateRay (pray.cpp:17-55).
+V evalScalar(const Shade &S, V o, V d, const float *lights, int nl, V bg) {
+    float t;
+    const uint32_t h = closest(S, o, d, t);
+    if (h == NO_HIT) return bg;
+    V color = {0.f, 0.f, 0.f};
+    const V pos = add(o, mul(d, t));
+    const V n = ld(S.normal + 3 * (size_t)h), alb = ld(S.color + 3 * (size_t)h);
+    for (int l = 0; l < nl; l++) {
+        const V lp = ld(lights + 6 * l), lc = ld(lights + 6 * l + 3);
+        V L = sub(lp, pos);
+        const float dist2 = L.x * L.x + L.y * L.y + L.z * L.z;
+        normalize(L);
+        const float lambert = dot(L, n);
+        if (lambert < 0.f) continue;
+        V Ld = L;
+        normalize(Ld);  // Ray(intersectionPos, L) normalises again
+        float st;
+        const uint32_t sh = closest(S, pos, Ld, st);
+        if (sh != NO_HIT && dist2 > st * st) continue;
+        const float inv = 1 / dist2;
+        color = add(color, mul(mul(mulv(alb, lc), lambert), inv));
+        if (color.x >= 1.f && color.y >= 1.f && color.z >= 1.f) break;
+    }
+    return {clampStd(0.f, 1.f, color.x), clampStd(0.f, 1.f, color.y), clampStd(0.f, 1.f, color.z)};
+}
+
+// AVX evaluateRay for one 2x4 packet (pray.cpp:58-196), lanes in the
+// reference's order (row y: x..x+3, row y+1: x..x+3).
+void evalPacket(const Shade &S, const float *rays, const float *lights, int nl, V bg, const uint64_t pix[8], V out[8],
+                bool &noHit) {
+    V o[8], d[8], pos[8], n[8], alb[8], col[8];
+    bool hit[8];
+    float t[8];
+    uint32_t tri[8];
+    noHit = true;
+    for (int j = 0; j < 8; j++) {
+        o[j] = ld(rays + 6 * pix[j]);
+        d[j] = ld(rays + 6 * pix[j] + 3);
+        tri[j] = closest(S, o[j], d[j], t[j]);
+        hit[j] = tri[j] != NO_HIT;
+        noHit = noHit && !hit[j];
+    }
+    if (noHit) {
+        for (int j = 0; j < 8; j++) out[j] = bg;
+        return;
+    }
+    for (int j = 0; j < 8; j++) {
+        col[j] = hit[j] ? V{0.f, 0.f, 0.f} : bg;
+        pos[j] = add(o[j], mul(d[j], t[j]));
+        n[j] = hit[j] ? ld(S.normal + 3 * (size_t)tri[j]) : V{0.f, 0.f, 0.f};
+        alb[j] = hit[j] ? ld(S.color + 3 * (size_t)tri[j]) : V{0.f, 0.f, 0.f};
+    }
+    for (int l = 0; l < nl; l++) {
+        const V lp = ld(lights + 6 * l), lc = ld(lights + 6 * l + 3);
+        V L[8];
+        float dist2[8], lambert[8];
+        bool smask[8], any = false;
+        for (int j = 0; j < 8; j++) {
+            L[j] = sub(lp, pos[j]);
+            dist2[j] = L[j].x * L[j].x + L[j].y * L[j].y + L[j].z * L[j].z;
+            const float inv = 1.f / std::sqrt(dist2[j]);  // normalize_avx: no zero guard
+            L[j] = mul(L[j], inv);
+            lambert[j] = dot(L[j], n[j]);
+            smask[j] = hit[j] && lambert[j] >= 0.f;
+            any = any || smask[j];
+        }
+        if (!any) continue;
+        bool lit[8], anyLit = false;
+        for (int j = 0; j < 8; j++) {
+            bool occluded = false;
+            if (smask[j]) {
+                float st;
+                const uint32_t sh = closest(S, pos[j], L[j], st);
+                occluded = sh != NO_HIT && dist2[j] >= st * st;
+            }
+            // the lambert mask is not re-applied here (pray.cpp:129)
+            lit[j] = hit[j] && !occluded;
+            anyLit = anyLit || lit[j];
+        }
+        if (!anyLit) continue;
+        bool white = true;
+        for (int j = 0; j < 8; j++) {
+            if (lit[j]) {
+                const float decay = lambert[j] / dist2[j];
+                col[j].x = col[j].x + alb[j].x * (lc.x * decay);
+                col[j].y = col[j].y + alb[j].y * (lc.y * decay);
+                col[j].z = col[j].z + alb[j].z * (lc.z * decay);
+            }
+            white = white && col[j].x >= 1.f && col[j].y >= 1.f && col[j].z >= 1.f;
+        }
+        if (white) break;
+    }
+    for (int j = 0; j < 8; j++)
+        out[j] = {clampStd(0.f, 1.f, col[j].x), clampStd(0.f, 1.f, col[j].y), clampStd(0.f, 1.f, col[j].z)};
+}
+
+}  // namespace
+
+// rayTrace (pray.cpp:380-494): 2x4 AVX packets, then the scalar remainder
+// columns (one extra column recomputed, quirk of pray.cpp:463-478) and an odd
+// last row.  rgb: w*h*3 bytes, top row first.
+ORC_API void orc_render_rt(const uint32_t *nodes, const uint32_t *refs, const float *A, const float *AB,
+                           const float *AC, const float *box6, const float *normal, const float *color,
+                           const float *cam16, uint64_t w, uint64_t h, const float *lights, int nl, const float *bg3,
+                           float eps, uint8_t *rgb) {
+    const Shade S{{nodes, refs, A, AB, AC, box6}, normal, color, nullptr, eps};
+    std::vector<float> rays(6 * w * h);
+    orc_camera_rays(cam16, w, h, rays.data());
+    const V bg = ld(bg3);
+    auto put = [&](uint64_t x, uint64_t y, V c) {
+        uint8_t *p = rgb + 3 * (x + y * w);
+        p[0] = toByte(c.x), p[1] = toByte(c.y), p[2] = toByte(c.z);
+    };
+    if (h >= 2 && w >= 4) {
+        const uint64_t ty = (h - 2) / 2 + 1, tx = (w - 4) / 4 + 1;
+#pragma omp parallel for schedule(dynamic, 10) collapse(2)
+        for (uint64_t yt = 0; yt < ty; yt++) {
+            for (uint64_t xt = 0; xt < tx; xt++) {
+                const uint64_t x = 4 * xt, y = 2 * yt;
+                uint64_t pix[8];
+                for (int j = 0; j < 8; j++) pix[j] = (x + (j & 3)) + (y + (j >> 2)) * w;
+                V out[8];
+                bool noHit;
+                evalPacket(S, rays.data(), lights, nl, bg, pix, out, noHit);
+                for (int j = 0; j < 8; j++) put(x + (j & 3), y + (j >> 2), out[j]);
+            }
+        }
+    }
+    const uint64_t missing = w % 4;
+    for (uint64_t y = 0; y + 1 < h; y++)
+        for (uint64_t x = w - 1 - missing; x < w; x++)
+            put(x, y, evalScalar(S, ld(&rays[6 * (x + y * w)]), ld(&rays[6 * (x + y * w) + 3]), lights, nl, bg));
+    if (h % 2 != 0)
+        for (uint64_t x = 0; x < w; x++)
+            put(x, h - 1,
+                evalScalar(S, ld(&rays[6 * (x + (h - 1) * w)]), ld(&rays[6 * (x + (h - 1) * w) + 3]), lights, nl, bg));
+}
+
+// Path tracing (NORMAL estimator, pray.cpp:342-378) with the build's
+// wavefront conventions: throughput multiplied forward, Philox4x32-10 keyed
+// by (seed_lo, seed_hi) with counter (pixel, sample, bounce, 0) giving
+// u1, u2 (hemisphere) and u3 (Russian roulette from bounce rr_depth on:
+// survive with q = max albedo component, throughput divided by q).
+// Samples are summed in order; hdr = sum / spp; rgb = clamp -> byte.
+ORC_API void orc_render_pt(const uint32_t *nodes, const uint32_t *refs, const float *A, const float *AB,
+                           const float *AC, const float *box6, const float *normal, const float *color,
+                           const uint8_t *emit, const float *cam16, uint64_t w, uint64_t h, uint32_t spp,
+                           uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *bg3, float eps,
+                           float *hdr, uint8_t *rgb) {
+    const Shade S{{nodes, refs, A, AB, AC, box6}, normal, color, emit, eps};
+    std::vector<float> rays(6 * w * h);
+    orc_camera_rays(cam16, w, h, rays.data());
+    const V bg = ld(bg3);
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint64_t p = 0; p < w * h; p++) {
+        V sum = {0.f, 0.f, 0.f};
+        for (uint32_t s = 0; s < spp; s++) {
+            V o = ld(&rays[6 * p]), d = ld(&rays[6 * p + 3]);
+            V T = {1.f, 1.f, 1.f};
+            V contrib = {0.f, 0.f, 0.f};
+            for (uint32_t b = 0;; b++) {
+                float t;
+                const uint32_t tri = closest(S, o, d, t);
+                if (tri == NO_HIT) {
+                    contrib = mulv(T, bg);
+                    break;
+                }
+                if (emit[tri]) {
+                    contrib = mulv(T, ld(color + 3 * (size_t)tri));
+                    break;
+                }
+                if (b >= max_depth) break;
+                const V alb = ld(color + 3 * (size_t)tri);
+                uint32_t c[4] = {(uint32_t)p, s, b, 0u};
+                philox(c, k0, k1);
+                const V pos = add(o, mul(d, t));
+                const V nd = sampleHemisphere(ld(normal + 3 * (size_t)tri), u01(c[0]), u01(c[1]));
+                T = mulv(T, alb);
+                if (b >= rr_depth) {
+                    const float q = std::max(std::max(alb.x, alb.y), alb.z);
+                    if (q < 1.f) {
+                        if (!(u01(c[2]) < q)) break;
+                        T = {T.x / q, T.y / q, T.z / q};
+                    }
+                }
+                o = pos;
+                d = nd;
+            }
+            sum = add(sum, contrib);
+        }
+        const float fs = (float)spp;
+        const V m = {sum.x / fs, sum.y / fs, sum.z / fs};
+        if (hdr) hdr[3 * p] = m.x, hdr[3 * p + 1] = m.y, hdr[3 * p + 2] = m.z;
+        if (rgb) {
+            rgb[3 * p] = toByte(clampStd(0.f, 1.f, m.x));
+            rgb[3 * p + 1] = toByte(clampStd(0.f, 1.f, m.y));
+            rgb[3 * p + 2] = toByte(clampStd(0.f, 1.f, m.z));
+        }
+    }
+}
+
 // Camera (scene_preprocessor.cpp:98-136): returns the 16 camera floats.
 ORC_API void orc_camera(const float *pos, const float *look, uint64_t fov, uint64_t w, uint64_t h, float *cam16) {
     const V P = ld(pos), L = ld(look);

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "bf_search.h"
+#include "io.h"
 #include "kd_tree.h"
 #include "kd_tree_sah.h"
 #include "pray.h"
@@ -161,6 +162,48 @@ REF_API void ref_init_tris(const float *abc, uint64_t n, float *ab, float *ac, f
         std::memcpy(ac + 3 * i, e + 3, 12);
         std::memcpy(normal + 3 * i, e + 6, 12);
     }
+}
+
+// readSceneDescription (io.cpp:27-137): returns a handle holding the scene.
+// info[12] = mode, width, height, fov, max_depth, num_samples, n_lights,
+// n_tris, camera pos (3) ... see ref_scene_get.
+REF_API void *ref_load_scene(const char *path) {
+    auto *s = new pray::Scene();
+    pray::readSceneDescription(path, *s);
+    return s;
+}
+
+REF_API void ref_scene_free(void *h) { delete static_cast<pray::Scene *>(h); }
+
+// info[16]: mode, width, height, fov, max_depth, num_samples, n_lights, n_tris,
+// cam pos xyz, cam look xyz, 0, 0; tris n x 9, colors n x 3, emit n,
+// lights n x 6, background 3 (any buffer may be NULL).
+REF_API void ref_scene_get(void *h, double *info, float *tris, float *colors, uint8_t *emit, float *lights,
+                           float *bg) {
+    auto *s = static_cast<pray::Scene *>(h);
+    if (info) {
+        const double v[16] = {(double)s->mode, (double)s->width, (double)s->height, (double)s->field_of_view,
+                              (double)s->maxDepth, (double)s->numSamples, (double)s->lights.size(),
+                              (double)s->triangles.size(), s->camera_position.x, s->camera_position.y,
+                              s->camera_position.z, s->camera_look.x, s->camera_look.y, s->camera_look.z, 0, 0};
+        std::memcpy(info, v, sizeof v);
+    }
+    for (size_t i = 0; i < s->triangles.size(); i++) {
+        const pray::Triangle &t = s->triangles[i];
+        if (tris) {
+            const float v[9] = {t.a.x, t.a.y, t.a.z, t.b.x, t.b.y, t.b.z, t.c.x, t.c.y, t.c.z};
+            std::memcpy(tris + 9 * i, v, sizeof v);
+        }
+        if (colors) colors[3 * i] = t.color.x, colors[3 * i + 1] = t.color.y, colors[3 * i + 2] = t.color.z;
+        if (emit) emit[i] = t.isEmitting ? 1 : 0;
+    }
+    if (lights)
+        for (size_t i = 0; i < s->lights.size(); i++) {
+            const pray::LightSource &l = s->lights[i];
+            const float v[6] = {l.position.x, l.position.y, l.position.z, l.color.x, l.color.y, l.color.z};
+            std::memcpy(lights + 6 * i, v, sizeof v);
+        }
+    if (bg) bg[0] = s->backgroundColor.x, bg[1] = s->backgroundColor.y, bg[2] = s->backgroundColor.z;
 }
 
 // calculatePrimRays: width*height rays, 6 floats each.

@@ -1,0 +1,174 @@
+"""Host surface: the KD-tree builders reproduce the reference's trees
+(compiled from source), the flat layout is well formed, scene I/O matches
+the reference loader, and the C ABI exports what include/*.h declares."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import nostrum_pray_amd as P
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+needs_ref = pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not available")
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"^PRAY_API[^;(]*?\b(\w+)\s*\(", text, re.MULTILINE)))
+
+
+def test_abi_exports_every_declared_symbol():
+    hip, host = P.libraries()
+    for name in declared("pray_hip.h"):
+        assert hasattr(hip, name), name
+    for name in declared("pray_host.h"):
+        assert hasattr(host, name), name
+    assert len(declared("pray_hip.h")) >= 20 and len(declared("pray_host.h")) >= 20
+
+
+def test_abi_only_exports_c_symbols():
+    import subprocess
+    for lib in (P.HIP_LIB, P.HOST_LIB):
+        out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True).stdout
+        syms = [l.split()[-1] for l in out.splitlines() if " T " in l]
+        assert syms and all(s.startswith(("pray_", "ph_")) for s in syms), syms
+
+
+def test_context_without_gpu_fails_cleanly():
+    """On a machine without a GPU, pray_init reports NODEVICE instead of
+    crashing; with a GPU it succeeds."""
+    try:
+        c = P.Context(0)
+        c.close()
+    except P.PrayError as e:
+        assert e.code == -6
+
+
+def compare_trees(scene, kind, depth=20):
+    abc, _, _ = scene.triangles()
+    t = scene.build(kind, max_depth=depth)
+    R = O.RefDB(abc, kind)
+    mine, ref = t.export(), R.export()
+    for m, r in zip(mine, ref):
+        assert m.shape == r.shape
+        assert np.array_equal(m.view(np.uint32), r.view(np.uint32))
+    st, rs = t.stats(), R.stats()
+    for k in ("nnodes", "maxdepth", "leafNodes", "emptyLeafNodes", "triangleCount"):
+        assert st[k] == rs[k], k
+    return t
+
+
+@needs_ref
+@pytest.mark.parametrize("maker", [lambda: P.Scene.spheres(1), lambda: P.Scene.terrain(96, 3),
+                                   lambda: P.Scene.cornell(3, 12000), lambda: P.Scene.random(4, 6000)])
+@pytest.mark.parametrize("kind", [0, 1])
+def test_builders_equal_reference(maker, kind):
+    compare_trees(maker(), kind)
+
+
+@needs_ref
+def test_sah_builder_equal_reference_on_example_obj():
+    s = P.Scene.load_json(os.path.join(ROOT, "tests", "golden", "scenes", "head_fov30.json"))
+    compare_trees(s, 1)
+    compare_trees(s, 0)
+
+
+def test_flat_layout_well_formed():
+    for kind in (0, 1):
+        s = P.Scene.cornell(2, 5000)
+        t = s.build(kind)
+        f = t.flat()
+        nodes = f["nodes"].reshape(-1, 2)
+        refs = f["refs"]
+        n_tris = f["a"].shape[0]
+        assert refs[-1] & 0x80000000
+        assert ((refs & 0x7FFFFFFF) < n_tris).all()
+        # every triangle is referenced by at least one leaf
+        assert len(np.unique(refs & 0x7FFFFFFF)) == n_tris
+        seen, stack, depth = 0, [(1, 0)], 0
+        while stack:
+            i, d = stack.pop()
+            seen += 1
+            depth = max(depth, d)
+            w = int(nodes[i, 1])
+            if w & 3 == 3:
+                continue
+            lc = w >> 2
+            assert lc % 2 == 0 and lc > i and lc + 1 < len(nodes)
+            stack += [(lc, d + 1), (lc + 1, d + 1)]
+        assert seen == len(nodes) - 1 and depth == f["max_depth"]
+        # SAH children share the split plane; median children clip inside the parent
+        if kind == 1:
+            inner = nodes[1:][(nodes[1:, 1] & 3) != 3]
+            lcs = inner[:, 1] >> 2
+            assert np.array_equal(nodes[lcs, 0], nodes[lcs + 1, 0])
+
+
+def test_depth_cap_extension_keeps_leaf_sets_correct():
+    s = P.Scene.terrain(128, 9)
+    t20 = s.build(1, max_depth=20)
+    t12 = s.build(1, max_depth=12)
+    assert t12.stats()["maxdepth"] <= 12
+    f = t12.flat()
+    rays = O.gen_rays(4, P.scene_box_expanded(f["box"]), 3000)
+    a = O.flat_trace(t20.flat(), rays, 0, 1e-12)
+    b = O.flat_trace(f, rays, 0, 1e-12)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_procedural_scenes_sizes_and_determinism():
+    assert P.Scene.spheres(1).num_tris == 25602
+    assert P.Scene.terrain(64, 1).num_tris == 2 * 64 * 64
+    c = P.Scene.cornell(1, 100_000)
+    assert abs(c.num_tris - 100_000) < 3000
+    a1, _, _ = P.Scene.terrain(32, 5).triangles()
+    a2, _, _ = P.Scene.terrain(32, 5).triangles()
+    a3, _, _ = P.Scene.terrain(32, 6).triangles()
+    assert np.array_equal(a1, a2) and not np.array_equal(a1, a3)
+    # terrain triangles face up (det > 0 for downward rays)
+    ab, ac, nn = P.Scene.terrain(32, 5).derived()
+    assert (nn[:, 1] > 0).all()
+    # heights within [0, 0.1]
+    assert a1[:, 1::3].min() >= 0 and a1[:, 1::3].max() <= 0.1
+
+
+@needs_ref
+@pytest.mark.parametrize("name", ["cube_200x150.json", "head_fov30.json", "plant_256.json", "pt_cornell.json"])
+def test_scene_loader_equals_reference(name):
+    path = os.path.join(ROOT, "tests", "golden", "scenes", name)
+    r = O.ref_load_scene(path)
+    s = P.Scene.load_json(path)
+    a, c, e = s.triangles()
+    assert np.array_equal(a.view(np.uint32), r["tris"].view(np.uint32))
+    assert np.array_equal(c.view(np.uint32), r["color"].view(np.uint32))
+    assert np.array_equal(e, r["emit"])
+    inf = s.info()
+    assert np.array_equal(s.lights(), r["lights"])
+    assert np.array_equal(inf["background"], r["background"])
+    for k in ("mode", "width", "height", "fov", "max_depth", "spp"):
+        assert inf[k] == r[k], k
+
+
+def test_scene_json_round_trip(tmp_path):
+    s = P.Scene.cornell(4, 2000)
+    path = str(tmp_path / "c.json")
+    s.write_json(path)
+    r = P.Scene.load_json(path)
+    a, ca, ea = s.triangles()
+    b, cb, eb = r.triangles()
+    assert np.array_equal(a, b) and np.array_equal(ca, cb) and np.array_equal(ea, eb)
+    ia, ib = s.info(), r.info()
+    for k in ("mode", "width", "height", "fov", "max_depth", "spp"):
+        assert ia[k] == ib[k]
+    assert np.array_equal(s.camera(), r.camera())
+
+
+def test_scene_load_errors_are_reported(tmp_path):
+    with pytest.raises(ValueError):
+        P.Scene.load_json(str(tmp_path / "missing.json"))
+    bad = tmp_path / "bad.json"
+    bad.write_text("{ not json")
+    with pytest.raises(ValueError):
+        P.Scene.load_json(str(bad))

@@ -1,0 +1,253 @@
+"""Pins the oracle (CPU restatement of the reference) before it is trusted:
+known-answer vectors, closed forms, invariants, and equality with the
+reference's own sources compiled by oracle/Makefile."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+needs_ref = pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not available")
+
+
+# ------------------------------------------------------------- Philox KATs --
+# Random123 known-answer vectors for philox4x32_10 (kat_vectors in the
+# Random123 distribution; Salmon et al., SC 2011).
+PHILOX_KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, want):
+    assert list(O.philox(ctr, key)) == want
+
+
+def test_sincos_accuracy_and_identity():
+    u = np.concatenate([np.linspace(0, 1, 100001, endpoint=False, dtype=np.float32),
+                        np.array([0.0, 0.25, 0.5, 0.75, 0.125], np.float32)])
+    s, c = O.sincos2pi(u)
+    ang = 2 * np.pi * u.astype(np.float64)
+    assert np.max(np.abs(s - np.sin(ang))) < 2e-7
+    assert np.max(np.abs(c - np.cos(ang))) < 2e-7
+    assert np.max(np.abs(s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2 - 1)) < 5e-7
+    # exact quadrant points
+    s4, c4 = O.sincos2pi(np.array([0.0, 0.25, 0.5, 0.75], np.float32))
+    assert list(s4) == [0.0, 1.0, -0.0, -1.0] and list(c4) == [1.0, -0.0, -1.0, 0.0]
+
+
+def test_gen_rays_distribution():
+    box = np.array([-1, -2, -3, 2, 4, 6], np.float32)
+    r = O.gen_rays(99, box, 200_000)
+    o, d = r[:, :3], r[:, 3:]
+    assert (o >= box[:3]).all() and (o <= box[:3] + box[3:]).all()
+    assert np.allclose(np.linalg.norm(d.astype(np.float64), axis=1), 1, atol=1e-6)
+    # uniform on the sphere: each component has mean 0 and variance 1/3
+    assert np.allclose(d.mean(0), 0, atol=0.01) and np.allclose(d.var(0), 1 / 3, atol=0.01)
+    # different seeds give different streams, same seed the same
+    assert not np.array_equal(r, O.gen_rays(100, box, 200_000))
+    assert np.array_equal(r, O.gen_rays(99, box, 200_000))
+
+
+# ------------------------------------------------------------------ MT ------
+def tri(a, b, c):
+    a, b, c = (np.array(v, np.float32) for v in (a, b, c))
+    return a, b - a, c - a
+
+
+def test_mt_closed_form_axis_aligned():
+    # z = 2 plane triangle facing -z, ray from origin along +z: exact t = 2.
+    a, ab, ac = tri([0, 0, 2], [0, 1, 2], [1, 0, 2])  # normal (ab x ac) points to -z
+    h, t = O.mt(np.array([0.25, 0.25, 0, 0, 0, 1], np.float32), a, ab, ac, 1e-4)
+    assert h and t == 2.0
+    # same triangle, reversed winding: culled (det < eps)
+    a2, ab2, ac2 = tri([0, 0, 2], [1, 0, 2], [0, 1, 2])
+    h, _ = O.mt(np.array([0.25, 0.25, 0, 0, 0, 1], np.float32), a2, ab2, ac2, 1e-4)
+    assert not h
+    # ray behind: t negative is still reported by MT (callers require t > 0)
+    h, t = O.mt(np.array([0.25, 0.25, 5, 0, 0, 1], np.float32), a, ab, ac, 1e-4)
+    assert h and t == -3.0
+
+
+def test_mt_edges_inclusive_and_outside():
+    a, ab, ac = tri([0, 0, 1], [0, 1, 1], [1, 0, 1])
+    for p, want in [((0, 0), True), ((1, 0), True), ((0, 1), True), ((0.5, 0.5), True),
+                    ((0.5, 0.5000001), False), ((-1e-6, 0.5), False), ((0.3, 0.3), True)]:
+        h, t = O.mt(np.array([p[0], p[1], 0, 0, 0, 1], np.float32), a, ab, ac, 1e-4)
+        assert h == want, (p, h)
+        if h:
+            assert t == 1.0
+
+
+def test_mt_det_cull_boundary():
+    # det = ab . (d x ac); for d = +z and a unit right triangle det = 1 * scale^2
+    s = np.float32(0.01)  # det = 1e-4 exactly representable? use comparison with eps
+    a, ab, ac = tri([0, 0, 1], [0, s, 1], [s, 0, 1])
+    ray = np.array([0.001, 0.001, 0, 0, 0, 1], np.float32)
+    det = float(np.dot(ab, np.cross(ray[3:], ac)))
+    h_lo, _ = O.mt(ray, a, ab, ac, np.float32(det))  # det < det is false -> kept
+    h_hi, _ = O.mt(ray, a, ab, ac, np.nextafter(np.float32(det), np.float32(1)))
+    assert h_lo and not h_hi
+
+
+def test_mt_scalar_equals_avx_bitwise():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        v = rng.normal(size=(3, 3)).astype(np.float32)
+        a, ab, ac = v[0], v[1] - v[0], v[2] - v[0]
+        rays = O.gen_rays(int(rng.integers(1 << 30)), np.array([-2, -2, -2, 4, 4, 4], np.float32), 4000)
+        hit_v, t_v = O.mt_avx(rays, a, ab, ac, 1e-4)
+        for k in range(0, 4000, 7):
+            h, t = O.mt(rays[k], a, ab, ac, 1e-4)
+            assert h == hit_v[k]
+            if h:
+                assert np.float32(t).view(np.uint32) == t_v[k].view(np.uint32)
+
+
+# ---------------------------------------------------------------- slab -----
+def test_slab_closed_forms():
+    box = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    rays = np.array([
+        [0.5, 0.5, -1, 0, 0, 1],     # through the box
+        [0.5, 0.5, 2, 0, 0, 1],      # behind the box: the LINE still crosses it
+        [2, 0.5, 0.5, 0, 1, 0],      # parallel, outside the x slab
+        [0.5, 2, 0.5, 1, 0, 0],      # parallel, outside the y slab
+        [-1, -1, -1, 0.57735026, 0.57735026, 0.57735026],
+        [1.5, 0.5, 0.5, 0, 0, 1],    # parallel to z, outside in x
+    ], np.float32)
+    s, a = O.slab(rays, box)
+    assert list(s) == [True, True, False, False, True, False]
+    assert list(a) == list(s)
+
+
+# ----------------------------------------------- equality with reference ---
+def scene_tris(n_side=24, seed=1):
+    import nostrum_pray_amd as P
+    s = P.Scene.terrain(n_side, seed)
+    return s
+
+
+@needs_ref
+def test_init_tris_equal_reference():
+    import nostrum_pray_amd as P
+    abc, _, _ = P.Scene.spheres(3).triangles()
+    mine = O.init_tris(abc)
+    ref = O.ref_init_tris(abc)
+    for m, r in zip(mine, ref):
+        assert np.array_equal(m.view(np.uint32), r.view(np.uint32))
+
+
+@needs_ref
+@pytest.mark.parametrize("pos,look,fov,w,h", [([0, 3, 8], [0, -0.35, -1], 60, 64, 48),
+                                               ([2, 3, 3], [0, 0, 1], 90, 40, 30),
+                                               ([0, 1, 0.9], [0, 0, -1], 130, 33, 17),
+                                               ([1.5, 5, 6], [-0.1, -0.5, -1], 30, 20, 20)])
+def test_camera_equals_reference(pos, look, fov, w, h):
+    cam = O.camera(pos, look, fov, w, h)
+    mine = O.camera_rays(cam, w, h)
+    ref = O.ref_prim_rays(pos, look, fov, w, h)
+    assert np.array_equal(mine.view(np.uint32), ref.view(np.uint32))
+
+
+@needs_ref
+@pytest.mark.parametrize("scene_kind", ["spheres", "cornell"])
+def test_bf_and_exhaustive_equal_reference(scene_kind):
+    import nostrum_pray_amd as P
+    s = P.Scene.spheres(1) if scene_kind == "spheres" else P.Scene.cornell(5, 6000)
+    abc, _, _ = s.triangles()
+    ab, ac, _ = s.derived()
+    a = abc[:, :3].copy()
+    t = s.build(0)
+    rays = np.concatenate([s.prim_rays()[::97], O.gen_rays(3, P.scene_box_expanded(t.box()), 3000)])
+    bi, bt = O.bf_closest(a, ab, ac, rays, 1e-4)
+    R = O.RefDB(abc, 2)
+    ri, rt = R.intersect(rays)
+    assert np.array_equal(bi, ri)
+    assert np.array_equal(bt[bi >= 0], rt[ri >= 0])
+    KD = O.RefDB(abc, 0)
+    ki, kt = KD.intersect(rays)
+    ei, et = O.ref_closest(t.export(), a, ab, ac, rays, 1e-4)
+    assert np.array_equal(ei, ki) and np.array_equal(et[ei >= 0], kt[ki >= 0])
+    kai, kat, _ = KD.intersect_avx(rays, 2)
+    oi, ot, _ = O.ref_closest_avx(t.export(), a, ab, ac, rays, 1e-4, 2)
+    assert np.array_equal(oi, kai) and np.array_equal(ot[oi >= 0], kat[kai >= 0])
+    # the exhaustive traversal equals brute force except at equal-t ties,
+    # where the reference's depth-first order may pick another triangle and
+    # brute force (strict <, index order) picks the lowest index
+    diff = np.flatnonzero(ei != bi)
+    assert np.array_equal(et.view(np.uint32), bt.view(np.uint32))
+    assert ((bi[diff] >= 0) & (ei[diff] >= 0) & (bi[diff] < ei[diff])).all()
+    assert diff.size <= rays.shape[0] // 100
+
+
+def test_flat_twin_equals_bruteforce():
+    import nostrum_pray_amd as P
+    for kind in (0, 1):
+        s = P.Scene.cornell(7, 8000)
+        t = s.build(kind)
+        f = t.flat()
+        rays = np.concatenate([s.prim_rays()[::61], O.gen_rays(11, P.scene_box_expanded(f["box"]), 4000)])
+        for eps in (1e-4, 1e-9):
+            fi, ft = O.flat_trace(f, rays, 0, eps)
+            bi, bt = O.bf_closest(f["a"], f["ab"], f["ac"], rays, eps)
+            assert np.array_equal(fi, bi)
+            assert np.array_equal(ft[fi >= 0], bt[bi >= 0])
+
+
+def test_flat_twin_any_hit_is_threshold_of_closest():
+    import nostrum_pray_amd as P
+    s = P.Scene.spheres(2)
+    t = s.build(1)
+    f = t.flat()
+    rays = O.gen_rays(5, P.scene_box_expanded(f["box"]), 20000)
+    ci, ct = O.flat_trace(f, rays, 0, 1e-4)
+    lim = np.where(ci >= 0, ct * ct, 1.0).astype(np.float32)
+    ai, _ = O.flat_trace(f, rays, 1, 1e-4, limit2=lim)
+    ii, _ = O.flat_trace(f, rays, 2, 1e-4, limit2=lim)
+    assert not (ai >= 0).any()  # strict: t*t < t*t never holds
+    assert np.array_equal(ii >= 0, ci >= 0)  # inclusive: exactly the hit rays
+
+
+def test_pt_oracle_closed_forms():
+    """Estimator invariants of the restated path tracer (pray.cpp:342-378):
+    inside an all-emissive room every pixel is Le; with an inner box of
+    albedo rho, pixels are rho*Le (box) or Le (wall); with depth 0 the box
+    pixels are 0 (depth limit) and with roulette the mean stays unbiased."""
+    import nostrum_pray_amd as P
+    from test_gpu_render import box_room
+    le, rho = np.float32(3.0), np.float32(0.5)
+    w = h = 24
+    cam = O.camera([0, 0, 1.8], [0, 0, -1], 90, w, h)
+    abc, col, emit = box_room(le)
+    f = P.Scene.from_triangles(abc, col, emit).build(1).flat()
+    hdr, rgb = O.render_pt(f, cam, w, h, 3, 2, 1 << 30, 1, [0, 0, 0], 1e-9)
+    assert (hdr == le).all() and (rgb == 255).all()
+    abc, col, emit = box_room(le, 0.5, rho)
+    f = P.Scene.from_triangles(abc, col, emit).build(1).flat()
+    hdr, _ = O.render_pt(f, cam, w, h, 3, 2, 1 << 30, 1, [0, 0, 0], 1e-9)
+    box = hdr[..., 0] == np.float32(le * rho)
+    assert box.any() and (box | (hdr[..., 0] == le)).all()
+    hdr0, _ = O.render_pt(f, cam, w, h, 3, 0, 1 << 30, 1, [0, 0, 0], 1e-9)
+    assert (hdr0[box] == 0).all() and (hdr0[~box] == le).all()
+    # Russian roulette from bounce 0 with q = rho: survivors carry 1/q, so the
+    # mean over many samples approaches rho * Le / rho * rho = rho * Le
+    hdr_rr, _ = O.render_pt(f, cam, w, h, 256, 2, 0, 1, [0, 0, 0], 1e-9)
+    assert abs(hdr_rr[box].mean() - le * rho) < 0.05 * le * rho
+
+
+def test_sample_hemisphere_properties():
+    rng = np.random.default_rng(1)
+    n = rng.normal(size=(50000, 3)).astype(np.float32)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    u1 = rng.random(50000, dtype=np.float32)
+    u2 = rng.random(50000, dtype=np.float32)
+    d = O.sample_hemisphere(n, u1, u2)
+    assert np.allclose(np.linalg.norm(d.astype(np.float64), axis=1), 1, atol=1e-6)
+    cos = (d.astype(np.float64) * n).sum(1)
+    assert (cos >= -1e-6).all()
+    # cosine-weighted: cos(theta) = sqrt(1 - u1) (up to rounding), so
+    # cos^2 is uniform with mean 1/2
+    assert np.allclose(cos, np.sqrt(1 - u1.astype(np.float64)), atol=2e-5)
+    assert abs((cos ** 2).mean() - 0.5) < 0.01
