@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--max-depth", type=int, default=20, help="KD-tree depth cap (20 = reference)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=1 << 24, help="CPU baseline sample (prefix of the stream)")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (reference for HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -83,6 +85,14 @@ def main():
     d_cnt = ctx.alloc((n, 2), np.uint32)
     d_lim = ctx.to_device(np.full(n, np.finfo(np.float32).max, np.float32))
     ctx.gen_rays_device(args.seed, box, n, d_rays.ptr)
+    if args.calibrate_copy:
+        nb = args.calibrate_copy << 20
+        src, dst = ctx.alloc(nb, np.uint8), ctx.alloc(nb, np.uint8)
+        ctx.copy_device(dst.ptr, src.ptr, nb)
+        ctx.sync()
+        log(f"calibration copy: {nb} bytes read, {nb} bytes written")
+        src.free()
+        dst.free()
 
     # Work counts (untimed): nodes and leaf references per ray.
     ctx.trace_counts_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, d_cnt.ptr, P.TRACE_CLOSEST, args.det_eps)
@@ -121,7 +131,9 @@ def main():
 
     achieved = bytes_per_ray * n / (ms * 1e-3) / 1e9
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "trace_closest_terrain2048_pmc.json")
+    # HBM traffic of this kernel/workload from the committed rocprofv3 PMC
+    # summary (FETCH_SIZE/WRITE_SIZE calibrated on a plain copy)
+    pmc = os.path.join(ROOT, "profiles", f"trace_closest_terrain{args.grid}_depth{args.max_depth}_pmc.json")
     if os.path.exists(pmc):
         try:
             with open(pmc) as fh:

@@ -171,6 +171,10 @@ PRAY_API int pray_device_free(pray_ctx *ctx, void *d_ptr);
 PRAY_API int pray_copy_to_device(pray_ctx *ctx, void *d_dst, const void *src, uint64_t bytes);
 PRAY_API int pray_copy_to_host(pray_ctx *ctx, void *dst, const void *d_src, uint64_t bytes);
 PRAY_API int pray_sync(pray_ctx *ctx);
+/* Device-to-device copy by a plain streaming kernel (16-byte loads/stores);
+ * bytes must be a multiple of 16.  Also the reference point for calibrating
+ * HBM traffic counters. */
+PRAY_API int pray_copy_device(pray_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes, void *stream);
 /* Records an event on the context's stream; slot 0..15. */
 PRAY_API int pray_event_record(pray_ctx *ctx, int slot);
 /* Milliseconds between two recorded events (synchronises on the second). */

@@ -85,6 +85,9 @@ def libraries():
         "pray_copy_to_device": (C.c_int, [_vp, _vp, _vp, _u64]),
         "pray_copy_to_host": (C.c_int, [_vp, _vp, _vp, _u64]),
         "pray_sync": (C.c_int, [_vp]),
+        "pray_copy_device": (C.c_int, [_vp, _vp, _vp, _u64, _vp]),
+        "pray_render_pt_ex": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u64, _vp, _f32, _vp,
+                                        _vp, C.c_int]),
         "pray_event_record": (C.c_int, [_vp, C.c_int]),
         "pray_event_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_f32)]),
     }
@@ -352,6 +355,9 @@ class Context:
     def sync(self):
         self._check(self._lib.pray_sync(self._h))
 
+    def copy_device(self, d_dst, d_src, nbytes, stream=None):
+        self._check(self._lib.pray_copy_device(self._h, d_dst, d_src, nbytes, stream))
+
     def record(self, slot):
         self._check(self._lib.pray_event_record(self._h, slot))
 
@@ -413,13 +419,13 @@ class Context:
         return (rgb, hits) if want_hits else rgb
 
     def render_pt(self, sid, cam16, width, height, spp, max_depth, rr_depth, seed, background, det_eps,
-                  want_rgb=True):
+                  want_rgb=True, use_graph=True):
         cam16 = np.ascontiguousarray(cam16, np.float32)
         bg = np.ascontiguousarray(background, np.float32).reshape(3)
         hdr = np.empty((height, width, 3), np.float32)
         rgb = np.empty((height, width, 3), np.uint8) if want_rgb else None
-        self._check(self._lib.pray_render_pt(self._h, sid, _p(cam16), width, height, spp, max_depth, rr_depth, seed,
-                                             _p(bg), det_eps, _p(hdr), _p(rgb)))
+        self._check(self._lib.pray_render_pt_ex(self._h, sid, _p(cam16), width, height, spp, max_depth, rr_depth,
+                                                seed, _p(bg), det_eps, _p(hdr), _p(rgb), 1 if use_graph else 0))
         return hdr, rgb
 
 

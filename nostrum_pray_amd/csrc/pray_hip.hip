@@ -76,6 +76,12 @@ __global__ void k_gen_rays(uint32_t k0, uint32_t k1, float lx, float ly, float l
 
 using prr::Cam;
 
+// Grid-stride streaming copy (one 16-byte load and store per element).
+__global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // ------------------------------------------------------------ host side --
 
 struct Tuning {
@@ -772,6 +778,17 @@ int pray_sync(pray_ctx *c) {
     if (!c) return PRAY_ERR_INVALID;
     HIPC(c, hipSetDevice(c->device));
     HIPC(c, hipStreamSynchronize(c->stream));
+    return PRAY_OK;
+}
+
+int pray_copy_device(pray_ctx *c, void *dst, const void *src, uint64_t bytes, void *stream) {
+    if (!c || (bytes && (!dst || !src)) || (bytes & 15u)) return fail(c, PRAY_ERR_INVALID, "bad copy arguments");
+    if (!bytes) return PRAY_OK;
+    HIPC(c, hipSetDevice(c->device));
+    const uint64_t n4 = bytes / 16;
+    const unsigned grid = (unsigned)std::min<uint64_t>((n4 + 255) / 256, (uint64_t)c->num_cus * 32);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, pick(c, stream), (uint4 *)dst, (const uint4 *)src, n4);
+    HIPC(c, hipGetLastError());
     return PRAY_OK;
 }
 
