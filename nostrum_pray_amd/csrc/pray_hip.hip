@@ -250,10 +250,10 @@ void free_scene_rec(SceneRec &sc) {
 }
 
 // Structural check of an uploaded tree so that no kernel can read out of
-// bounds or loop: every inner node's child pair lies after it (acyclic) and
-// inside the array, every leaf run lies inside leaf_refs and ends with a
-// last-flagged entry, every referenced triangle exists, and the depth bound
-// holds.
+// bounds or loop: every inner node's child pair is even-aligned and inside
+// the array, no path is longer than the declared depth (which also rules out
+// cycles), every leaf run lies inside leaf_refs and ends with a
+// last-flagged entry, and every referenced triangle exists.
 int validate_tree(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const uint32_t *refs, uint32_t n_refs,
                   uint32_t n_tris, uint32_t max_depth) {
     if (n_nodes < 2) return fail(c, PRAY_ERR_INVALID, "tree has no root");
@@ -274,7 +274,7 @@ int validate_tree(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const ui
             if (k >= n_refs) return fail(c, PRAY_ERR_INVALID, "leaf of node %u starts outside leaf_refs", i);
         } else {
             const uint32_t lc = w >> 2;
-            if ((lc & 1u) || lc <= i || (uint64_t)lc + 1 >= n_nodes)
+            if ((lc & 1u) || lc < 2 || (uint64_t)lc + 1 >= n_nodes)
                 return fail(c, PRAY_ERR_INVALID, "node %u has a bad child index %u", i, lc);
             stack.push_back({lc, d + 1});
             stack.push_back({lc + 1, d + 1});

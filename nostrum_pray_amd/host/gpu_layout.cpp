@@ -2,6 +2,7 @@
 
 #include <cstring>
 #include <stdexcept>
+#include <unordered_map>
 
 namespace pray {
 
@@ -33,32 +34,106 @@ struct Flattener {
         g.nodes[2 * slot + 1] = (first << 2) | LEAF_TAG;
     }
 
-    // Depth-first: a node's children pair is allocated when the node is
-    // visited, then the left subtree is laid out before the right one.
-    void visit(const KDTreeElement *n, uint32_t slot, uint32_t depth) {
-        if (depth > g.maxDepth) g.maxDepth = depth;
-        if (!n->left && !n->right) {
-            leaf(n, slot);
-            return;
-        }
+    // Writes inner node n (at `slot`) with its children pair at node index lc.
+    void placePair(const KDTreeElement *n, uint32_t slot, uint32_t lc) {
         const uint32_t axis = n->splitPlane.axis;
         if (axis > 2) throw std::runtime_error("inner node without split axis");
-        const uint32_t lc = static_cast<uint32_t>(g.nodes.size() / 2);
-        if (lc >= LEAF_EMPTY) throw std::runtime_error("node array exceeds 2^30 entries");
-        g.nodes.resize(g.nodes.size() + 4, 0u);
         g.nodes[2 * slot + 1] = (lc << 2) | axis;
-        KDTreeElement emptyLeaf;
-        const KDTreeElement *L = n->left ? n->left.get() : &emptyLeaf;
-        const KDTreeElement *R = n->right ? n->right.get() : &emptyLeaf;
         // Clip planes along this node's axis: the left child's upper bound and
         // the right child's lower bound.  A missing child is empty, so any
         // value works; the parent's bound is used.
-        const float clipL = n->left ? L->boundingBox.max[axis] : n->boundingBox.min[axis];
-        const float clipR = n->right ? R->boundingBox.min[axis] : n->boundingBox.max[axis];
+        const float clipL = n->left ? n->left->boundingBox.max[axis] : n->boundingBox.min[axis];
+        const float clipR = n->right ? n->right->boundingBox.min[axis] : n->boundingBox.max[axis];
         g.nodes[2 * lc] = fbits(clipL);
         g.nodes[2 * (lc + 1)] = fbits(clipR);
-        visit(L, lc, depth + 1);
-        visit(R, lc + 1, depth + 1);
+    }
+
+    // A fresh block of 16 nodes (8 children pairs, 128 bytes); returns its
+    // first node index.
+    uint32_t newBlock() {
+        const size_t base = g.nodes.size() / 2;
+        if (base + 16 >= LEAF_EMPTY) throw std::runtime_error("node array exceeds 2^30 entries");
+        g.nodes.resize(g.nodes.size() + 32, 0u);
+        return static_cast<uint32_t>(base);
+    }
+
+    struct Item {
+        const KDTreeElement *n;
+        uint32_t slot, depth;
+    };
+
+    std::unordered_map<const KDTreeElement *, uint32_t> innerCount;
+    uint32_t packBase = 0, packNext = 8;  // shared block for small subtrees
+
+    static bool isLeaf(const KDTreeElement *n) { return !n->left && !n->right; }
+
+    uint32_t countInner(const KDTreeElement *n) {
+        if (!n || isLeaf(n)) return 0;
+        const uint32_t c = 1 + countInner(n->left.get()) + countInner(n->right.get());
+        innerCount[n] = c;
+        return c;
+    }
+
+    // Places the children pairs of `root` and of its inner descendants
+    // breadth first into [base + 2*next, base + 16) until `limit` pairs are
+    // placed; returns the inner nodes left over (their pairs not placed).
+    std::vector<Item> fill(Item root, uint32_t base, uint32_t &next, uint32_t limit) {
+        KDTreeElement emptyLeaf;
+        std::vector<Item> queue{root}, rest;
+        for (size_t qi = 0; qi < queue.size(); qi++) {
+            const Item it = queue[qi];
+            if (next >= limit) {
+                rest.push_back(it);
+                continue;
+            }
+            const uint32_t lc = base + 2 * next++;
+            placePair(it.n, it.slot, lc);
+            const KDTreeElement *kids[2] = {it.n->left ? it.n->left.get() : &emptyLeaf,
+                                            it.n->right ? it.n->right.get() : &emptyLeaf};
+            for (uint32_t k = 0; k < 2; k++) {
+                const uint32_t d = it.depth + 1;
+                if (d > g.maxDepth) g.maxDepth = d;
+                if (isLeaf(kids[k]))
+                    leaf(kids[k], lc + k);
+                else
+                    queue.push_back({kids[k], lc + k, d});
+            }
+        }
+        return rest;
+    }
+
+    // Treelet layout: an inner node's children pair and, breadth first, the
+    // pairs of its descendants fill one 128-byte block of 8 pairs; inner
+    // nodes left over start blocks of their own.  A subtree of at most 8
+    // inner nodes is placed whole into a block shared with other small
+    // subtrees (first fit).  One cache line thus serves up to three levels of
+    // descent, and children pairs always lie after their parent.
+    void place(Item it) {
+        const uint32_t c = innerCount[it.n];
+        if (c <= 8) {
+            if (8 - packNext < c) {
+                packBase = newBlock();
+                packNext = 0;
+            }
+            const uint32_t limit = packNext + c;
+            fill(it, packBase, packNext, limit);
+            return;
+        }
+        uint32_t next = 0;
+        for (const Item &r : fill(it, newBlock(), next, 8)) place(r);
+    }
+
+    void layout(const KDTreeElement *root) {
+        // Block 0 holds the padding node 0 and the root (node 1) in its
+        // first pair slot.
+        const uint32_t base = newBlock();
+        if (isLeaf(root)) {
+            leaf(root, 1);
+            return;
+        }
+        countInner(root);
+        uint32_t next = 1;
+        for (const Item &r : fill({root, 1, 0}, base, next, 8)) place(r);
     }
 };
 
@@ -89,10 +164,10 @@ KDTreeGPU buildGPUTreeFromKDTree(const KDTree &tree) {
     }
     if (n == 0 || !tree.root) return g;  // empty scene: no nodes
 
-    g.nodes.assign(4, 0u);  // node 0 (padding) and node 1 (root)
+    g.nodes.clear();
     g.leafRefs.reserve(n + n / 2);
     Flattener f{g};
-    f.visit(tree.root.get(), 1, 0);
+    f.layout(tree.root.get());
     const BoundingBox &rb = tree.root->boundingBox;
     g.rootBox[0] = rb.min.x, g.rootBox[1] = rb.min.y, g.rootBox[2] = rb.min.z;
     g.rootBox[3] = rb.max.x, g.rootBox[4] = rb.max.y, g.rootBox[5] = rb.max.z;

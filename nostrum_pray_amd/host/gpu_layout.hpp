@@ -14,7 +14,10 @@
 //                 is one aligned 16-byte load.
 //          leaf:  bits[31:2] = index of the first entry of the leaf's run in
 //                 leaf_refs, or LEAF_EMPTY.
-// The root is node 1 (node 0 is padding).
+// The root is node 1 (node 0 is padding).  Nodes are stored in 128-byte
+// blocks of 8 children pairs: a node's children pair and, breadth first, the
+// pairs of its descendants share a block until it is full (a treelet), so one
+// cache line serves up to three levels of descent.  Unused slots are zero.
 //
 // leaf_refs: uint32 per (leaf, triangle) reference: triangle index in bits
 //   [30:0]; bit 31 marks the last entry of a leaf's run.
