@@ -50,7 +50,11 @@ def main():
     ap.add_argument("--det-eps", type=float, default=1e-12,
                     help="determinant cull; the reference's 1e-4 culls every terrain triangle")
     ap.add_argument("--layout", type=int, default=-1)
-    ap.add_argument("--max-depth", type=int, default=20, help="KD-tree depth cap (20 = reference)")
+    ap.add_argument("--max-depth", type=int, default=40, help="KD-tree depth cap (reference: 20)")
+    ap.add_argument("--leaf-size", type=int, default=8, help="nodes this small are not split (reference: 32)")
+    ap.add_argument("--traversal-cost", type=float, default=2.0, help="SAH traversal cost (reference: 5)")
+    ap.add_argument("--no-ref-tree", action="store_true",
+                    help="skip timing the same rays over the tree built with the reference's parameters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=1 << 24, help="CPU baseline sample (prefix of the stream)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -67,9 +71,10 @@ def main():
     t0 = time.time()
     scene = P.Scene.terrain(args.grid, args.seed)
     n_tris = scene.num_tris
-    tree = scene.build(1, max_depth=args.max_depth)
-    flat_sizes = tree.stats()
-    log(f"scene {n_tris} tris, SAH tree {flat_sizes} in {time.time() - t0:.1f}s")
+    tree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+    tstats = tree.stats()
+    tree_tag = f"sah-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}"
+    log(f"scene {n_tris} tris, SAH tree {tree_tag} {tstats} in {time.time() - t0:.1f}s")
 
     ctx = P.Context(0)
     sid = ctx.upload(tree, shading=False)
@@ -101,20 +106,22 @@ def main():
     hits = int((d_idx.numpy() >= 0).sum())
     d_cnt.free()
     tun = ctx.get_tuning(sid)
-    ref_bytes = 48 if tun["layout"] == P.LAYOUT_PACKED else 40
+    # Algorithmic bytes: ray 24 + result 8 + 8 per node visited + 40 per
+    # triangle reference tested (4-byte index + a, ab, ac); the packed layout
+    # actually moves 48 bytes per reference.
     nodes_per_ray = cnt[0] / n
     refs_per_ray = cnt[1] / n
-    bytes_per_ray = 24 + 8 + 8 * nodes_per_ray + ref_bytes * refs_per_ray
+    bytes_per_ray = 24 + 8 + 8 * nodes_per_ray + 40 * refs_per_ray
     log(f"hits {hits}/{n}  nodes/ray {nodes_per_ray:.1f}  refs/ray {refs_per_ray:.1f}  B/ray {bytes_per_ray:.0f}")
 
-    def timed(mode, steps, warmup, lim=None):
+    def timed(mode, steps, warmup, lim=None, scene_id=sid):
         for _ in range(warmup):
-            ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, mode, args.det_eps, d_limit2=lim)
+            ctx.trace_device(scene_id, d_rays.ptr, n, d_idx.ptr, d_t.ptr, mode, args.det_eps, d_limit2=lim)
         ctx.sync()
         t_wall = time.perf_counter()
         ctx.record(0)
         for _ in range(steps):
-            ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, mode, args.det_eps, d_limit2=lim)
+            ctx.trace_device(scene_id, d_rays.ptr, n, d_idx.ptr, d_t.ptr, mode, args.det_eps, d_limit2=lim)
         ctx.record(1)
         ms_ev = ctx.elapsed_ms(0, 1) / steps
         ctx.sync()
@@ -124,6 +131,7 @@ def main():
     ms, ms_wall = timed(P.TRACE_CLOSEST, args.steps, args.warmup)
     mrays = n / (ms * 1e-3) / 1e6
     log(f"closest-hit {ms:.3f} ms/step  {mrays:.1f} Mrays/s  (wall {ms_wall:.3f} ms/step)")
+    closest_idx, closest_t = d_idx.numpy(), d_t.numpy()
 
     # Any-hit over the same stream (unlimited distance), reported beside.
     ms_any, _ = timed(P.TRACE_ANY, args.steps, 2, lim=d_lim.ptr)
@@ -133,13 +141,27 @@ def main():
     traffic = None
     # HBM traffic of this kernel/workload from the committed rocprofv3 PMC
     # summary (FETCH_SIZE/WRITE_SIZE calibrated on a plain copy)
-    pmc = os.path.join(ROOT, "profiles", f"trace_closest_terrain{args.grid}_depth{args.max_depth}_pmc.json")
-    if os.path.exists(pmc):
-        try:
-            with open(pmc) as fh:
-                traffic = json.load(fh).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"trace_closest_terrain{args.grid}_{tree_tag}_pmc.json")
+    if os.path.exists(pmc) and args.rays == 1 << 24:
+        with open(pmc) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    # The same rays over the tree built with the reference's parameters
+    # (depth cap 20, leaf size 32, SAH costs 5 / 1.5): identical hits, more
+    # triangle tests per ray.
+    ref_tree = None
+    if not args.no_ref_tree:
+        t1 = time.time()
+        rtree = scene.build(1)
+        rsid = ctx.upload(rtree, shading=False)
+        log(f"reference-parameter tree {rtree.stats()} in {time.time() - t1:.1f}s")
+        ms_ref, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=rsid)
+        same = bool(np.array_equal(closest_idx, d_idx.numpy()) and
+                    np.array_equal(closest_t.view(np.uint32), d_t.numpy().view(np.uint32)))
+        ctx.free_scene(rsid)
+        ref_tree = {"ms_per_step": round(ms_ref, 4), "Mrays_per_s": round(n / (ms_ref * 1e-3) / 1e6, 2),
+                    "identical_hits": same, "tree": "SAH depth cap 20, leaf size 32, costs 5 / 1.5"}
+        log(f"reference-parameter tree: {ms_ref:.3f} ms/step, identical hits {same}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -161,9 +183,11 @@ def main():
         "config": {
             "workload": f"closest-hit, {n} incoherent rays (origins uniform in the scene box +10%, directions "
                         f"uniform on the sphere, Philox seed {args.seed}), heightfield terrain {args.grid}x{args.grid} "
-                        f"cells = {n_tris} FP32 triangles, SAH KD tree (depth cap {args.max_depth}), det_eps "
+                        f"cells = {n_tris} FP32 triangles, SAH KD tree (depth cap {args.max_depth}, leaf size "
+                        f"{args.leaf_size}, traversal cost {args.traversal_cost:g}), det_eps "
                         f"{args.det_eps:g}",
-            "rays": n, "triangles": n_tris, "tree": "SAH", "max_depth": args.max_depth,
+            "rays": n, "triangles": n_tris, "tree": tree_tag, "tree_depth": tstats["maxdepth"],
+            "tree_nodes": tstats["flat_nodes"], "leaf_refs": tstats["flat_refs"],
             "layout": "packed" if tun["layout"] == P.LAYOUT_PACKED else "indexed",
             "hit_fraction": round(hits / n, 4),
         },
@@ -179,6 +203,7 @@ def main():
             "refs_per_ray": round(float(refs_per_ray), 2),
         },
         "any_hit": {"ms_per_step": round(ms_any, 4), "Mrays_per_s": round(n / (ms_any * 1e-3) / 1e6, 2)},
+        "reference_parameter_tree": ref_tree,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

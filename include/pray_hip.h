@@ -81,7 +81,7 @@ PRAY_API int pray_mem_ok(pray_ctx *ctx, uint64_t n_tris, uint64_t n_refs, uint64
  *   tri_normal, tri_color   3*n_tris floats each, may be NULL for trace-only scenes
  *   tri_emit   n_tris bytes (1 = emissive), may be NULL
  *   root_box   6 floats: min xyz, max xyz of the root cell
- *   max_depth  root-to-leaf depth of the tree (<= 24)
+ *   max_depth  root-to-leaf depth of the tree (<= 64)
  * n_tris == 0 is a valid empty scene (every ray misses). */
 PRAY_API int pray_upload_scene(pray_ctx *ctx, const uint32_t *nodes, uint32_t n_nodes, const uint32_t *leaf_refs,
                       uint32_t n_refs, const float *tri_a, const float *tri_ab, const float *tri_ac,

@@ -71,6 +71,13 @@ PRAY_API ph_tree *ph_build(ph_scene *s, int kind);
 /* Same with an explicit depth cap (the reference's is 20; deeper trees have
  * smaller leaves and produce identical hits). */
 PRAY_API ph_tree *ph_build_depth(ph_scene *s, int kind, int max_depth);
+/* Same with the leaf size below which nodes are not split (reference 32,
+ * kd_tree.h:50) and, for the SAH tree, the traversal and intersection costs
+ * (reference 5 and 1.5, kd_tree_sah.h:49-50).  Values <= 0 keep the
+ * reference's.  Any tree produces the same closest hits; these only trade
+ * node visits against triangle tests. */
+PRAY_API ph_tree *ph_build_ex(ph_scene *s, int kind, int max_depth, int leaf_size, float traversal_cost,
+                              float intersection_cost);
 PRAY_API void ph_tree_free(ph_tree *t);
 /* stats[8]: kind, nnodes, maxdepth, leafNodes, emptyLeafNodes, triangleCount,
  * flat node count, flat leaf refs */

@@ -113,6 +113,7 @@ def libraries():
         "ph_prim_rays": (None, [_vp, _vp]),
         "ph_build": (_vp, [_vp, C.c_int]),
         "ph_build_depth": (_vp, [_vp, C.c_int, C.c_int]),
+        "ph_build_ex": (_vp, [_vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]),
         "ph_tree_free": (None, [_vp]),
         "ph_tree_stats": (None, [_vp, _vp]),
         "ph_tree_export": (None, [_vp, _vp, _vp, _vp, _vp]),
@@ -235,10 +236,11 @@ class Scene:
         self._lib.ph_prim_rays(self._h, _p(out))
         return out
 
-    def build(self, kind=-1, max_depth=20):
-        """KD tree over the scene: 0 median, 1 SAH, -1 the reference policy;
-        max_depth 20 is the reference's depth cap."""
-        return Tree(self._lib.ph_build_depth(self._h, kind, max_depth))
+    def build(self, kind=-1, max_depth=20, leaf_size=0, traversal_cost=0.0, intersection_cost=0.0):
+        """KD tree over the scene: 0 median, 1 SAH, -1 the reference policy.
+        max_depth 20, leaf_size 32 and the SAH costs 5 / 1.5 are the
+        reference's; 0 keeps them."""
+        return Tree(self._lib.ph_build_ex(self._h, kind, max_depth, leaf_size, traversal_cost, intersection_cost))
 
 
 class Tree:

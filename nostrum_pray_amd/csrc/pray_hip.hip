@@ -32,7 +32,7 @@ using prt::WAVE;
 
 namespace {
 
-constexpr uint32_t MAX_TREE_DEPTH = 32;
+constexpr uint32_t MAX_TREE_DEPTH = 64;
 
 // Packed per-leaf records from the per-field arrays.
 __global__ void k_pack_records(const uint32_t *refs, uint32_t n_refs, const float *a, const float *ab,

@@ -151,7 +151,10 @@ void ph_prim_rays(const ph_scene *s, float *rays) {
 
 ph_tree *ph_build(ph_scene *s, int kind) { return ph_build_depth(s, kind, (int)pray::KDTree::MAX_DEPTH); }
 
-ph_tree *ph_build_depth(ph_scene *s, int kind, int max_depth) {
+ph_tree *ph_build_depth(ph_scene *s, int kind, int max_depth) { return ph_build_ex(s, kind, max_depth, 0, 0.f, 0.f); }
+
+ph_tree *ph_build_ex(ph_scene *s, int kind, int max_depth, int leaf_size, float traversal_cost,
+                     float intersection_cost) {
     s->ensureDerived();
     auto *t = new ph_tree();
     pray::Scene &S = s->s;
@@ -164,6 +167,11 @@ ph_tree *ph_build_depth(ph_scene *s, int kind, int max_depth) {
     else
         t->tree = std::make_unique<pray::KDTree>();
     t->tree->depthLimit = max_depth > 0 ? (size_t)max_depth : pray::KDTree::MAX_DEPTH;
+    if (leaf_size > 0) t->tree->leafSize = (size_t)leaf_size;
+    if (auto *sah = dynamic_cast<pray::KDTreeSAH *>(t->tree.get())) {
+        if (traversal_cost > 0) sah->traversalCost = traversal_cost;
+        if (intersection_cost > 0) sah->intersectionCost = intersection_cost;
+    }
     t->tree->build(S.triangles);
     t->flat = pray::buildGPUTreeFromKDTree(*t->tree);
     return t;

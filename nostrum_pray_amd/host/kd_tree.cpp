@@ -58,7 +58,7 @@ std::unique_ptr<KDTreeElement> KDTree::buildRecursive(std::vector<uint32_t> &tri
     for (size_t i = 1; i < triIndices.size(); i++) nodeBB = nodeBB.expand(triangles[triIndices[i]].boundingBox);
     const uint8_t axis = nodeBB.getLongestAxis();
 
-    if (triIndices.size() <= THRESHOLD || depth >= depthLimit) {
+    if (triIndices.size() <= leafSize || depth >= depthLimit) {
         std::vector<uint32_t> copy(triIndices);
         return std::make_unique<KDTreeElement>(nodeBB, std::move(copy));
     }
@@ -127,13 +127,17 @@ inline float lambdaBias(size_t nl, size_t nr, float pl, float pr) {
     return 1.0f;
 }
 
-inline float costOf(float pl, float pr, size_t nl, size_t nr) {
-    return lambdaBias(nl, nr, pl, pr) *
-           (KDTreeSAH::traversalCost + KDTreeSAH::triangleIntersectionCost * (pl * nl + pr * nr));
+struct SahCost {
+    float traversal, intersection;
+};
+
+inline float costOf(const SahCost &c, float pl, float pr, size_t nl, size_t nr) {
+    return lambdaBias(nl, nr, pl, pr) * (c.traversal + c.intersection * (pl * nl + pr * nr));
 }
 
 // SAH cost of one plane (kd_tree_sah.cpp:141-166).
-void sah(const SplitPlane &p, const BoundingBox &b, size_t nl, size_t nr, size_t np, float &costP, PlaneSide &pside) {
+void sah(const SahCost &c, const SplitPlane &p, const BoundingBox &b, size_t nl, size_t nr, size_t np, float &costP,
+         PlaneSide &pside) {
     costP = INFTY;
     BoundingBox bl, br;
     b.split(p, bl, br);
@@ -141,8 +145,8 @@ void sah(const SplitPlane &p, const BoundingBox &b, size_t nl, size_t nr, size_t
     const float pr = br.surfaceArea() / b.surfaceArea();
     if (pl == 0 || pr == 0) return;
     if (b.max[p.axis] - b.min[p.axis] == 0) return;
-    const float costPl = costOf(pl, pr, nl + np, nr);
-    const float costPr = costOf(pl, pr, nl, np + nr);
+    const float costPl = costOf(c, pl, pr, nl + np, nr);
+    const float costPr = costOf(c, pl, pr, nl, np + nr);
     if (costPl < costPr) {
         costP = costPl;
         pside = LEFT;
@@ -157,8 +161,8 @@ void sah(const SplitPlane &p, const BoundingBox &b, size_t nl, size_t nr, size_t
 // increment then steps once more, so the first event after each plane group is
 // never counted.  That behaviour decides which planes and counts the reference
 // evaluates, so it is kept.
-void findPlane(size_t numTriangles, const std::vector<SahEvent> &ev, const BoundingBox &b, SplitPlane &pest,
-               float &cest, PlaneSide &psideest) {
+void findPlane(const SahCost &cost, size_t numTriangles, const std::vector<SahEvent> &ev, const BoundingBox &b,
+               SplitPlane &pest, float &cest, PlaneSide &psideest) {
     cest = INFTY;
     size_t Nl[3] = {0, 0, 0}, Np[3] = {0, 0, 0}, Nr[3] = {numTriangles, numTriangles, numTriangles};
     const size_t n = ev.size();
@@ -185,7 +189,7 @@ void findPlane(size_t numTriangles, const std::vector<SahEvent> &ev, const Bound
         float C;
         PlaneSide ps = UNKNOWN;
         const SplitPlane p(pa, pd);
-        sah(p, b, Nl[pa], Nr[pa], Np[pa], C, ps);
+        sah(cost, p, b, Nl[pa], Nr[pa], Np[pa], C, ps);
         if (C < cest) {
             cest = C;
             pest = p;
@@ -199,17 +203,17 @@ void findPlane(size_t numTriangles, const std::vector<SahEvent> &ev, const Bound
 
 struct SahBuilder {
     const std::vector<Triangle> &tris;
-    size_t depthLimit;
+    size_t depthLimit, leafSize;
+    SahCost cost;
 
     std::unique_ptr<KDTreeElement> rec(std::vector<uint32_t> &T, const BoundingBox &b, std::vector<SahEvent> &E,
                                        const SplitPlane &prevp, size_t depth) {
         SplitPlane p;
         float Cp;
         PlaneSide pside = UNKNOWN;
-        findPlane(T.size(), E, b, p, Cp, pside);
+        findPlane(cost, T.size(), E, b, p, Cp, pside);
 
-        if (T.size() <= KDTree::THRESHOLD || depth >= depthLimit || p == prevp ||
-            Cp > KDTreeSAH::triangleIntersectionCost * T.size()) {
+        if (T.size() <= leafSize || depth >= depthLimit || p == prevp || Cp > cost.intersection * T.size()) {
             std::vector<SahEvent>().swap(E);
             return std::make_unique<KDTreeElement>(b, std::move(T));
         }
@@ -342,7 +346,7 @@ void KDTreeSAH::buildSAH() {
     }
     __gnu_parallel::sort(events.begin(), events.end(), evLess);
 
-    SahBuilder builder{triangles, depthLimit};
+    SahBuilder builder{triangles, depthLimit, leafSize, {traversalCost, intersectionCost}};
 #pragma omp parallel
 #pragma omp single nowait
     root = builder.rec(T, initialBB, events, SplitPlane(), 0);

@@ -72,6 +72,9 @@ public:
     // Depth cap used by the build; MAX_DEPTH reproduces the reference tree.
     // A larger cap is an extension of this build (deeper trees, smaller leaves).
     size_t depthLimit = MAX_DEPTH;
+    // Leaf size below which a node is never split; THRESHOLD reproduces the
+    // reference tree, a smaller one is an extension (smaller leaves).
+    size_t leafSize = THRESHOLD;
     // Parallel task depth; results do not depend on it.
     static constexpr size_t TASK_DEPTH = 8;
 
@@ -88,9 +91,12 @@ public:
     void build(const std::vector<Triangle> &triangles) override;
     void build(std::vector<Triangle> &&triangles) override;
 
-    // SAH constants (kd_tree_sah.h:49-50).
-    static constexpr float traversalCost = 5.f;
-    static constexpr float triangleIntersectionCost = 1.5f;
+    // SAH constants (kd_tree_sah.h:49-50).  Reference values by default;
+    // other values are an extension for traversal hardware with other costs.
+    static constexpr float REF_TRAVERSAL_COST = 5.f;
+    static constexpr float REF_INTERSECTION_COST = 1.5f;
+    float traversalCost = REF_TRAVERSAL_COST;
+    float intersectionCost = REF_INTERSECTION_COST;
 
 private:
     void buildSAH();

@@ -122,6 +122,25 @@ def test_depth_cap_extension_keeps_leaf_sets_correct():
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("kind", [0, 1])
+def test_build_parameters_change_tree_not_hits(kind):
+    """Smaller leaves and a cheaper SAH traversal give a deeper tree with
+    fewer triangle tests per ray and exactly the same closest hits; explicit
+    reference values reproduce the default tree."""
+    s = P.Scene.terrain(128, 9)
+    ref = s.build(kind)
+    same = s.build(kind, max_depth=20, leaf_size=32, traversal_cost=5.0, intersection_cost=1.5)
+    for x, y in zip(ref.export(), same.export()):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    deep = s.build(kind, max_depth=40, leaf_size=8, traversal_cost=2.0)
+    assert deep.stats()["leafNodes"] > 2 * ref.stats()["leafNodes"]
+    rays = O.gen_rays(5, P.scene_box_expanded(ref.box()), 4000)
+    a = O.flat_trace(ref.flat(), rays, 0, 1e-12, counts=True)
+    b = O.flat_trace(deep.flat(), rays, 0, 1e-12, counts=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    assert b[2][:, 1].sum() < a[2][:, 1].sum()
+
+
 def test_procedural_scenes_sizes_and_determinism():
     assert P.Scene.spheres(1).num_tris == 25602
     assert P.Scene.terrain(64, 1).num_tris == 2 * 64 * 64
