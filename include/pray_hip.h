@@ -162,6 +162,62 @@ PRAY_API int pray_render_pt_ex(pray_ctx *ctx, int scene_id, const float *camera1
                                uint32_t spp, uint32_t max_depth, uint32_t rr_depth, uint64_t seed,
                                const float *background, float det_eps, float *hdr, uint8_t *rgb, int use_graph);
 
+/* Learned radiance cache (no reference counterpart; the reference's path
+ * tracer, path_trace.cu:369-464, always traces paths to the end).
+ *
+ * An MLP predicts the radiance arriving at a surface point, divided by the
+ * point's albedo, from 10 features: position in the scene box mapped to
+ * [0,1]^3, ray direction and surface normal as (theta/pi, phi/2pi + 1/2),
+ * albedo rgb.  Network: 12-frequency sin/cos encoding of the 7 geometric
+ * features + albedo + a constant 1 (172 of 192 inputs) -> 5 hidden layers of
+ * width 64 with ReLU -> 3 outputs; BF16 weights and activations with FP32
+ * accumulation on the matrix cores, FP32 master weights and Adam (beta1 0.9,
+ * beta2 0.99, eps 1e-8).  Loss: relative L2, mean over samples and channels
+ * of (y - t)^2 / (y^2 + 0.01) with the denominator held constant.
+ * Parameters are PRAY_CACHE_PARAMS floats, row-major [out][in] per layer:
+ * 64x192, 4 x 64x64, 32x64 (rows 3.. of the last layer and inputs 172.. of
+ * the first are padding and stay zero). */
+#define PRAY_CACHE_PARAMS 30720
+#define PRAY_CACHE_FEATURES 10
+PRAY_API int pray_cache_create(pray_ctx *ctx, uint64_t seed, float learning_rate, uint32_t max_batch, int *cache_id);
+PRAY_API int pray_cache_destroy(pray_ctx *ctx, int cache_id);
+PRAY_API int pray_cache_get_params(pray_ctx *ctx, int cache_id, float *params);
+/* Sets the master weights and resets the optimiser state. */
+PRAY_API int pray_cache_set_params(pray_ctx *ctx, int cache_id, const float *params);
+/* out: n x 3 predictions for n x 10 features. */
+PRAY_API int pray_cache_infer(pray_ctx *ctx, int cache_id, const float *features, uint64_t n, float *out);
+/* Gradient of the batch loss (n <= max_batch samples, targets n x 3) without
+ * an update; grad: PRAY_CACHE_PARAMS floats; loss optional. */
+PRAY_API int pray_cache_gradient(pray_ctx *ctx, int cache_id, const float *features, const float *targets, uint64_t n,
+                                 float *grad, float *loss);
+/* One Adam step on a batch; loss (before the step) optional. */
+PRAY_API int pray_cache_train_step(pray_ctx *ctx, int cache_id, const float *features, const float *targets,
+                                   uint64_t n, float *loss);
+PRAY_API int pray_cache_stats(pray_ctx *ctx, int cache_id, uint32_t *steps, float *loss_last, float *loss_running,
+                              int *cache_on);
+
+/* Path tracing with the radiance cache, `frames` samples per pixel.  Each
+ * frame (one hipGraph replay) traces one path per pixel as pray_render_pt
+ * does, except at a non-emissive hit of bounce `cache_bounce` (0 = the
+ * primary hit):
+ *   - while the cache is on, the path stops there and its pixel receives
+ *     throughput * albedo * max(cache prediction, 0);
+ *   - at `train_paths` training pixels per frame (every stride-th pixel from
+ *     an offset drawn per frame) the hit's features become a training sample
+ *     and one more ray leaves it (own random stream); the target is that
+ *     ray's emission or background, or albedo * the cache's prediction at its
+ *     hit.  One Adam step on the frame's samples follows.
+ * The cache is on once at least `min_steps` steps were taken and the running
+ * training loss is below `max_loss`; otherwise the frame is plain path tracing
+ * with pray_render_pt's random numbers, so max_loss = 0 gives exactly
+ * pray_render_pt's image.  frame_stats (optional): frames x {training loss,
+ * cache on}. */
+PRAY_API int pray_render_pt_cache(pray_ctx *ctx, int scene_id, int cache_id, const float *camera16, uint32_t width,
+                                  uint32_t height, uint32_t frames, uint32_t max_depth, uint32_t rr_depth,
+                                  uint64_t seed, const float *background, float det_eps, uint32_t train_paths,
+                                  uint32_t cache_bounce, float max_loss, uint32_t min_steps, float *hdr, uint8_t *rgb,
+                                  float *frame_stats);
+
 /* Device-memory and timing utilities (no reference counterpart: the
  * reference allocates and copies inside every call).  They let a caller keep
  * rays and hit records resident in HBM between calls and time work on the

@@ -88,6 +88,17 @@ def libraries():
         "pray_copy_device": (C.c_int, [_vp, _vp, _vp, _u64, _vp]),
         "pray_render_pt_ex": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u64, _vp, _f32, _vp,
                                         _vp, C.c_int]),
+        "pray_cache_create": (C.c_int, [_vp, _u64, _f32, _u32, C.POINTER(C.c_int)]),
+        "pray_cache_destroy": (C.c_int, [_vp, C.c_int]),
+        "pray_cache_get_params": (C.c_int, [_vp, C.c_int, _vp]),
+        "pray_cache_set_params": (C.c_int, [_vp, C.c_int, _vp]),
+        "pray_cache_infer": (C.c_int, [_vp, C.c_int, _vp, _u64, _vp]),
+        "pray_cache_gradient": (C.c_int, [_vp, C.c_int, _vp, _vp, _u64, _vp, C.POINTER(_f32)]),
+        "pray_cache_train_step": (C.c_int, [_vp, C.c_int, _vp, _vp, _u64, C.POINTER(_f32)]),
+        "pray_cache_stats": (C.c_int, [_vp, C.c_int, C.POINTER(_u32), C.POINTER(_f32), C.POINTER(_f32),
+                                       C.POINTER(C.c_int)]),
+        "pray_render_pt_cache": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u64, _vp, _f32,
+                                           _u32, _u32, _f32, _u32, _vp, _vp, _vp]),
         "pray_event_record": (C.c_int, [_vp, C.c_int]),
         "pray_event_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_f32)]),
     }
@@ -429,6 +440,78 @@ class Context:
         self._check(self._lib.pray_render_pt_ex(self._h, sid, _p(cam16), width, height, spp, max_depth, rr_depth,
                                                 seed, _p(bg), det_eps, _p(hdr), _p(rgb), 1 if use_graph else 0))
         return hdr, rgb
+
+
+CACHE_PARAMS = 30720
+CACHE_FEATURES = 10
+
+
+class Cache:
+    """The radiance-cache MLP of a context (pray_cache_*)."""
+
+    def __init__(self, ctx, seed=1, learning_rate=1e-2, max_batch=1 << 16):
+        self.ctx = ctx
+        cid = C.c_int(-1)
+        ctx._check(ctx._lib.pray_cache_create(ctx._h, seed, learning_rate, max_batch, C.byref(cid)))
+        self.id = cid.value
+
+    def close(self):
+        if self.id is not None and self.ctx._h:
+            self.ctx._check(self.ctx._lib.pray_cache_destroy(self.ctx._h, self.id))
+        self.id = None
+
+    def params(self):
+        out = np.empty(CACHE_PARAMS, np.float32)
+        self.ctx._check(self.ctx._lib.pray_cache_get_params(self.ctx._h, self.id, _p(out)))
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.shape == (CACHE_PARAMS,)
+        self.ctx._check(self.ctx._lib.pray_cache_set_params(self.ctx._h, self.id, _p(w)))
+
+    def infer(self, feats):
+        feats = np.ascontiguousarray(feats, np.float32).reshape(-1, CACHE_FEATURES)
+        out = np.empty((feats.shape[0], 3), np.float32)
+        self.ctx._check(self.ctx._lib.pray_cache_infer(self.ctx._h, self.id, _p(feats), feats.shape[0], _p(out)))
+        return out
+
+    def gradient(self, feats, targets):
+        feats = np.ascontiguousarray(feats, np.float32).reshape(-1, CACHE_FEATURES)
+        targets = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        g = np.empty(CACHE_PARAMS, np.float32)
+        loss = _f32()
+        self.ctx._check(self.ctx._lib.pray_cache_gradient(self.ctx._h, self.id, _p(feats), _p(targets),
+                                                          feats.shape[0], _p(g), C.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, feats, targets):
+        feats = np.ascontiguousarray(feats, np.float32).reshape(-1, CACHE_FEATURES)
+        targets = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        loss = _f32()
+        self.ctx._check(self.ctx._lib.pray_cache_train_step(self.ctx._h, self.id, _p(feats), _p(targets),
+                                                            feats.shape[0], C.byref(loss)))
+        return loss.value
+
+    def stats(self):
+        st, ll, lr, on = _u32(), _f32(), _f32(), C.c_int()
+        self.ctx._check(self.ctx._lib.pray_cache_stats(self.ctx._h, self.id, C.byref(st), C.byref(ll), C.byref(lr),
+                                                       C.byref(on)))
+        return {"steps": st.value, "loss": ll.value, "loss_running": lr.value, "cache_on": bool(on.value)}
+
+    def render_pt(self, sid, cam16, width, height, frames, max_depth, rr_depth, seed, background, det_eps,
+                  train_paths=1 << 16, cache_bounce=2, max_loss=1e30, min_steps=1, want_rgb=False):
+        """Path tracing with this cache (pray_render_pt_cache); returns
+        (hdr, rgb or None, per-frame {loss, cache on} array)."""
+        cam16 = np.ascontiguousarray(cam16, np.float32)
+        bg = np.ascontiguousarray(background, np.float32).reshape(3)
+        hdr = np.empty((height, width, 3), np.float32)
+        rgb = np.empty((height, width, 3), np.uint8) if want_rgb else None
+        stats = np.empty((frames, 2), np.float32)
+        self.ctx._check(self.ctx._lib.pray_render_pt_cache(
+            self.ctx._h, sid, self.id, _p(cam16), width, height, frames, max_depth, rr_depth, seed, _p(bg), det_eps,
+            train_paths, cache_bounce, max_loss, min_steps, _p(hdr), _p(rgb), _p(stats)))
+        return hdr, rgb, stats
 
 
 class DeviceBuffer:

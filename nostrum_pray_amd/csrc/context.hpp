@@ -1,0 +1,83 @@
+// Host-side state of a pray_ctx shared by the library's translation units
+// (tracer and radiance cache).  Internal: none of this is exported.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pray_hip.h"
+#include "traverse.hpp"
+
+namespace prc {
+
+struct Tuning {
+    int layout = PRAY_LAYOUT_PACKED;
+    int stack_lds = 8;
+    int leaf_ilp = 2;
+    int fetch_min = 32;
+};
+
+struct SceneRec {
+    prt::DevScene d;
+    std::vector<void *> allocs;
+    Tuning tune;
+};
+
+struct CacheRec;  // radiance cache (cache.hip)
+void free_cache_rec(CacheRec *r);
+
+}  // namespace prc
+
+struct pray_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<int, std::unique_ptr<prc::SceneRec>> scenes;
+    std::map<int, prc::CacheRec *> caches;
+    int next_id = 1;
+    hipEvent_t events[16] = {};
+    uint32_t *work = nullptr;   // ray counters of the persistent kernels
+    uint4 *spill = nullptr;     // per-lane stack spill area
+    size_t spill_bytes = 0;
+    std::map<const void *, int> occupancy;  // kernel -> resident blocks per CU
+};
+
+namespace prc {
+
+int fail(pray_ctx *c, int code, const char *fmt, ...);
+
+#define HIPC(ctx, expr)                                                                                       \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            return ::prc::fail((ctx), e_ == hipErrorOutOfMemory ? PRAY_ERR_NOMEM : PRAY_ERR_HIP,            \
+                               "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__);          \
+    } while (0)
+
+inline hipStream_t pick(pray_ctx *c, void *stream) { return stream ? (hipStream_t)stream : c->stream; }
+
+inline unsigned nblocks(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+struct DBuf {
+    void *p = nullptr;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+SceneRec *find_scene(pray_ctx *c, int id);
+int ensure_spill(pray_ctx *c, size_t bytes);
+// Enqueues one trace over n rays (or *n_dev rays when n_dev is set, with n as
+// the launch bound).
+int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n, const uint32_t *n_dev, int mode,
+                  float eps, const float *lim2, int32_t *idx, float *t, uint32_t *counts, hipStream_t st);
+// Makes sure the closest-hit kernel of a scene has its stack spill area for
+// launches of up to n rays (needed before a graph capture).
+int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n);
+
+}  // namespace prc

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/pray_hip.h"
+#include "context.hpp"
 #include "device_math.hpp"
 #include "render.hpp"
 #include "traverse.hpp"
@@ -82,38 +83,9 @@ __global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src,
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
-// ------------------------------------------------------------ host side --
-
-struct Tuning {
-    int layout = PRAY_LAYOUT_PACKED;
-    int stack_lds = 8;
-    int leaf_ilp = 2;
-    int fetch_min = 32;
-};
-
-struct SceneRec {
-    DevScene d;
-    std::vector<void *> allocs;
-    Tuning tune;
-};
-
 }  // namespace
 
-struct pray_ctx {
-    int device = 0;
-    int num_cus = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    std::map<int, std::unique_ptr<SceneRec>> scenes;
-    int next_id = 1;
-    hipEvent_t events[16] = {};
-    uint32_t *work = nullptr;   // ray counters of the persistent kernels
-    uint4 *spill = nullptr;     // per-lane stack spill area
-    size_t spill_bytes = 0;
-    std::map<const void *, int> occupancy;  // kernel -> resident blocks per CU
-};
-
-namespace {
+namespace prc {
 
 int fail(pray_ctx *c, int code, const char *fmt, ...) {
     if (c) {
@@ -126,25 +98,6 @@ int fail(pray_ctx *c, int code, const char *fmt, ...) {
     }
     return code;
 }
-
-#define HIPC(ctx, expr)                                                                                       \
-    do {                                                                                                      \
-        hipError_t e_ = (expr);                                                                               \
-        if (e_ != hipSuccess)                                                                                 \
-            return fail((ctx), e_ == hipErrorOutOfMemory ? PRAY_ERR_NOMEM : PRAY_ERR_HIP, "%s: %s (%s:%d)", \
-                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                                    \
-    } while (0)
-
-inline hipStream_t pick(pray_ctx *c, void *stream) { return stream ? (hipStream_t)stream : c->stream; }
-
-inline unsigned nblocks(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
-
-struct DBuf {
-    void *p = nullptr;
-    ~DBuf() {
-        if (p) (void)hipFree(p);
-    }
-};
 
 typedef void (*TraceKernel)(DevScene, TraceArgs);
 
@@ -223,7 +176,7 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
     a.spill = c->spill;
     a.spill_depth = depth;
     a.fetch_min = (uint32_t)tu.fetch_min;
-    hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, c->work, 1u);
+    hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, c->work, grid * (uint32_t)WAVE);
     hipLaunchKernelGGL(k, dim3(grid), dim3(WAVE), 0, st, sc.d, a);
     HIPC(c, hipGetLastError());
     return PRAY_OK;
@@ -283,7 +236,17 @@ int validate_tree(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const ui
     return PRAY_OK;
 }
 
-}  // namespace
+int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n) {
+    TraceKernel k = pick_kernel(false, sc.tune.layout, sc.tune.stack_lds, sc.tune.leaf_ilp);
+    const uint64_t want = (n + WAVE - 1) / WAVE;
+    const uint64_t fit = (uint64_t)c->num_cus * (uint64_t)resident_blocks(c, k);
+    return ensure_spill(c, (size_t)std::max<uint64_t>(1, std::min(want, fit)) * WAVE *
+                               std::max<uint32_t>(1, sc.d.max_depth) * sizeof(uint4));
+}
+
+}  // namespace prc
+
+using namespace prc;
 
 // ------------------------------------------------------------------ C ABI --
 
@@ -317,6 +280,8 @@ void pray_destroy(pray_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto &kv : c->scenes) free_scene_rec(*kv.second);
     c->scenes.clear();
+    for (auto &kv : c->caches) free_cache_rec(kv.second);
+    c->caches.clear();
     for (hipEvent_t &e : c->events)
         if (e) (void)hipEventDestroy(e);
     if (c->work) (void)hipFree(c->work);
@@ -669,12 +634,7 @@ int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w,
     P.sample = count + 2;
     // The persistent trace kernel needs its spill area before any capture.
     {
-        TraceKernel k = pick_kernel(false, sc->tune.layout, sc->tune.stack_lds, sc->tune.leaf_ilp);
-        const uint64_t want = (npix + WAVE - 1) / WAVE;
-        const uint64_t fit = (uint64_t)c->num_cus * (uint64_t)resident_blocks(c, k);
-        const size_t need = (size_t)std::max<uint64_t>(1, std::min(want, fit)) * WAVE *
-                            std::max<uint32_t>(1, sc->d.max_depth) * sizeof(uint4);
-        int rc = ensure_spill(c, need);
+        int rc = prepare_trace(c, *sc, npix);
         if (rc) return rc;
     }
     // One sample pass: generate, (zero next count, extend, shade) per bounce,
@@ -693,8 +653,8 @@ int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w,
                 hipLaunchKernelGGL(k_fill_miss, dim3(nblocks(npix, 256)), dim3(256), 0, st, (uint32_t)npix,
                                    (int32_t *)hit.p, (float *)tt.p, (uint32_t *)nullptr);
             }
-            hipLaunchKernelGGL(prr::k_pt_shade, dim3(nblocks(npix, 256)), dim3(256), 0, st, sc->d, P, b,
-                               (const float *)rq[cur].p, (const float4 *)sq[cur].p, (const uint32_t *)(count + cur),
+            hipLaunchKernelGGL(prr::k_pt_shade<prr::PT_PLAIN>, dim3(nblocks(npix, 256)), dim3(256), 0, st, sc->d, P,
+                               prr::CacheArgs{}, b, (const float *)rq[cur].p, (const float4 *)sq[cur].p, (const uint32_t *)(count + cur),
                                (const int32_t *)hit.p, (const float *)tt.p, (float *)rq[nxt].p, (float4 *)sq[nxt].p,
                                count + nxt, (float *)contrib.p);
         }

@@ -113,7 +113,7 @@ __device__ __forceinline__ void hit_point(const RtArgs &A, uint32_t p, float &x,
 
 // One shadow ray per (light, lane): direction and limit of the path the lane
 // follows; -1 marks lanes that shoot no shadow ray.
-__global__ void k_rt_shadow_setup(DevScene s, RtArgs A) {
+static __global__ void k_rt_shadow_setup(DevScene s, RtArgs A) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lanes = A.g.n_avx + A.g.n_sc;
     if (g >= lanes) return;
@@ -152,7 +152,7 @@ __global__ void k_rt_shadow_setup(DevScene s, RtArgs A) {
     }
 }
 
-__global__ __launch_bounds__(WAVE) void k_rt_shade(DevScene s, RtArgs A) {
+static __global__ __launch_bounds__(WAVE) void k_rt_shade(DevScene s, RtArgs A) {
     const uint32_t g = blockIdx.x * WAVE + threadIdx.x;
     const uint32_t lanes = A.g.n_avx + A.g.n_sc;
     const uint32_t lane = threadIdx.x, grp = lane & ~7u;
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(WAVE) void k_rt_shade(DevScene s, RtArgs A) {
     }
 }
 
-__global__ void k_camera(Cam c, uint32_t w, uint32_t h, float *rays) {
+static __global__ void k_camera(Cam c, uint32_t w, uint32_t h, float *rays) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= w * h) return;
     float dx, dy, dz;
@@ -244,7 +244,7 @@ struct PtArgs {
 };
 
 // Queue entries: ray (6 floats) + state float4 {throughput rgb, pixel bits}.
-__global__ void k_pt_generate(const float *prim, uint32_t n_pix, float *rays, float4 *state, uint32_t *count) {
+static __global__ void k_pt_generate(const float *prim, uint32_t n_pix, float *rays, float4 *state, uint32_t *count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *count = n_pix;
     if (i >= n_pix) return;
@@ -254,16 +254,83 @@ __global__ void k_pt_generate(const float *prim, uint32_t n_pix, float *rays, fl
     state[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(i));
 }
 
-__global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, uint32_t bounce, const float *rays_in,
-                                                  const float4 *state_in, const uint32_t *count_in,
-                                                  const int32_t *hit, const float *tt, float *rays_out,
-                                                  float4 *state_out, uint32_t *count_out, float *contrib) {
+// Paths that end in the radiance cache (PT_CACHE) share the plain shading;
+// at a non-emissive hit of bounce `cache_bounce`:
+//  - while the cache is on, the path stops there and leaves a query
+//    {features, weight = throughput * albedo, pixel}: the cache's prediction
+//    of the radiance arriving at that point (divided by its albedo)
+//    completes the pixel; otherwise the path continues as a plain path;
+//  - if the pixel is one of this frame's training pixels (pixels
+//    off + k * stride, k < n_train, off drawn per frame), the point's
+//    features become training sample k and a training ray leaves it in a
+//    direction of its own random stream with throughput 1.  The training
+//    ray's hit gives the sample's target (k_pt_train_targets): emission or
+//    background, or albedo times the cache's prediction there (the cache
+//    trains on its own predictions one bounce further, as in Mueller et al.
+//    2021, "Real-time Neural Radiance Caching for Path Tracing").
+enum : int { PT_PLAIN = 0, PT_CACHE = 1 };
+
+struct CacheArgs {
+    uint32_t cache_bounce;
+    const uint32_t *cache_on;   // device flag
+    float *queries;             // 16 floats per query
+    uint32_t *n_queries;
+    uint32_t n_train, stride;   // training pixels per frame, their spacing
+    float *feats;               // n_train x 10: training sample features
+    float *train_rays;          // training rays (6 floats) ...
+    float4 *train_state;        // ... and {1, 1, 1, sample index bits}
+    uint32_t *n_train_rays;
+    float lo[3], inv[3];        // position -> [0,1]^3
+};
+
+__device__ __forceinline__ void sph01(float x, float y, float z, float &th, float &ph) {
+    th = acosf(fminf(fmaxf(z, -1.f), 1.f)) * 0.318309886183790672f;
+    ph = atan2f(y, x) * 0.159154943091895336f + 0.5f;
+}
+
+__device__ __forceinline__ void cache_features(const CacheArgs &C, float px, float py, float pz, float dx, float dy,
+                                               float dz, float nx, float ny, float nz, float ar, float ag, float ab,
+                                               float *f) {
+    f[0] = (px - C.lo[0]) * C.inv[0], f[1] = (py - C.lo[1]) * C.inv[1], f[2] = (pz - C.lo[2]) * C.inv[2];
+    sph01(dx, dy, dz, f[3], f[4]);
+    sph01(nx, ny, nz, f[5], f[6]);
+    f[7] = ar, f[8] = ag, f[9] = ab;
+}
+
+// First training pixel of a frame.
+__device__ __forceinline__ uint32_t train_offset(uint32_t frame, uint32_t n_pix, uint32_t k0, uint32_t k1) {
+    const uint4 u = prd::philox4x32_10(make_uint4(frame, 0u, 0xFFFFFFFFu, 3u), k0, k1);
+    return (uint32_t)(((uint64_t)u.x * n_pix) >> 32);
+}
+
+// Wave-aggregated slot in a queue: one atomic per wave.
+__device__ __forceinline__ uint32_t wave_slot(bool want, uint32_t *counter) {
+    const uint64_t m = __ballot(want);
+    uint32_t base = 0;
+    if (m) {
+        const uint32_t lane = threadIdx.x & (WAVE - 1);
+        const uint32_t first = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+        if (lane == first) base = atomicAdd(counter, (uint32_t)__popcll(m));
+        base = __shfl(base, (int)first);
+        base += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    }
+    return base;
+}
+
+template <int MODE>
+static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, CacheArgs C, uint32_t bounce,
+                                                         const float *rays_in, const float4 *state_in,
+                                                         const uint32_t *count_in, const int32_t *hit,
+                                                         const float *tt, float *rays_out, float4 *state_out,
+                                                         uint32_t *count_out, float *contrib) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = *count_in;
     const bool active = i < n;
-    bool push = false;
+    bool push = false, query = false, train = false;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, tr = 0, tg = 0, tb = 0;
-    uint32_t pix = 0;
+    float tdx = 0, tdy = 0, tdz = 0;
+    float feat[10];
+    uint32_t pix = 0, tk = 0;
     if (active) {
         const float4 st = state_in[i];
         pix = __float_as_uint(st.w);
@@ -280,53 +347,117 @@ __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, uint32_t
             const float t = tt[i];
             ox = r[0] + r[3] * t, oy = r[1] + r[4] * t, oz = r[2] + r[5] * t;
             const float ar = s.color[3 * (size_t)h], ag = s.color[3 * (size_t)h + 1], ab = s.color[3 * (size_t)h + 2];
-            const uint4 u = prd::philox4x32_10(make_uint4(pix, *P.sample, bounce, 0u), P.k0, P.k1);
-            prd::sample_hemisphere(s.normal[3 * (size_t)h], s.normal[3 * (size_t)h + 1], s.normal[3 * (size_t)h + 2],
-                                   prd::u01(u.x), prd::u01(u.y), dx, dy, dz);
-            tr = st.x * ar, tg = st.y * ag, tb = st.z * ab;
-            bool live = true;
-            if (bounce >= P.rr_depth) {
-                const float q = fmaxf(fmaxf(ar, ag), ab);
-                if (q < 1.f) {
-                    if (!(prd::u01(u.z) < q))
-                        live = false;
-                    else
-                        tr = tr / q, tg = tg / q, tb = tb / q;
+            const float nx = s.normal[3 * (size_t)h], ny = s.normal[3 * (size_t)h + 1], nz = s.normal[3 * (size_t)h + 2];
+            const bool at_cache = MODE == PT_CACHE && bounce == C.cache_bounce;
+            if (at_cache) {
+                const uint32_t d = (pix + P.n_pix - train_offset(*P.sample, P.n_pix, P.k0, P.k1)) % P.n_pix;
+                if (d % C.stride == 0u && d / C.stride < C.n_train) {
+                    tk = d / C.stride;
+                    train = true;
+                    cache_features(C, ox, oy, oz, r[3], r[4], r[5], nx, ny, nz, ar, ag, ab, C.feats + 10 * (size_t)tk);
+                    const uint4 u = prd::philox4x32_10(make_uint4(pix, *P.sample, bounce, 1u), P.k0, P.k1);
+                    prd::sample_hemisphere(nx, ny, nz, prd::u01(u.x), prd::u01(u.y), tdx, tdy, tdz);
                 }
             }
-            term = !live;
-            push = live;
+            if (at_cache && *C.cache_on) {
+                cache_features(C, ox, oy, oz, r[3], r[4], r[5], nx, ny, nz, ar, ag, ab, feat);
+                tr = st.x * ar, tg = st.y * ag, tb = st.z * ab;
+                query = true;
+                term = false;
+            } else {
+                const uint4 u = prd::philox4x32_10(make_uint4(pix, *P.sample, bounce, 0u), P.k0, P.k1);
+                prd::sample_hemisphere(nx, ny, nz, prd::u01(u.x), prd::u01(u.y), dx, dy, dz);
+                tr = st.x * ar, tg = st.y * ag, tb = st.z * ab;
+                bool live = true;
+                if (bounce >= P.rr_depth) {
+                    const float q = fmaxf(fmaxf(ar, ag), ab);
+                    if (q < 1.f) {
+                        if (!(prd::u01(u.z) < q))
+                            live = false;
+                        else
+                            tr = tr / q, tg = tg / q, tb = tb / q;
+                    }
+                }
+                term = !live;
+                push = live;
+            }
         }
         if (term) {
             float *c = contrib + 3 * (size_t)pix;
             c[0] = cr, c[1] = cg, c[2] = cb;
         }
     }
-    // one atomic per wave for the surviving paths
-    const uint64_t m = __ballot(push);
-    if (m) {
-        const uint32_t lane = threadIdx.x & (WAVE - 1);
-        const uint32_t first = (uint32_t)(__ffsll((unsigned long long)m) - 1);
-        uint32_t base = 0;
-        if (lane == first) base = atomicAdd(count_out, (uint32_t)__popcll(m));
-        base = __shfl(base, (int)first);
-        if (push) {
-            const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            float *o = rays_out + 6 * (size_t)slot;
-            o[0] = ox, o[1] = oy, o[2] = oz, o[3] = dx, o[4] = dy, o[5] = dz;
-            state_out[slot] = make_float4(tr, tg, tb, __uint_as_float(pix));
+    if (MODE == PT_CACHE) {
+        const uint32_t q = wave_slot(query, C.n_queries);
+        if (query) {
+            float *o = C.queries + 16 * (size_t)q;
+#pragma unroll
+            for (int k = 0; k < 10; ++k) o[k] = feat[k];
+            o[10] = tr, o[11] = tg, o[12] = tb, o[13] = __uint_as_float(pix);
         }
+        const uint32_t tq = wave_slot(train, C.n_train_rays);
+        if (train) {
+            float *o = C.train_rays + 6 * (size_t)tq;
+            o[0] = ox, o[1] = oy, o[2] = oz, o[3] = tdx, o[4] = tdy, o[5] = tdz;
+            C.train_state[tq] = make_float4(1.f, 1.f, 1.f, __uint_as_float(tk));
+        }
+    }
+    // one atomic per wave for the surviving paths
+    const uint32_t slot = wave_slot(push, count_out);
+    if (push) {
+        float *o = rays_out + 6 * (size_t)slot;
+        o[0] = ox, o[1] = oy, o[2] = oz, o[3] = dx, o[4] = dy, o[5] = dz;
+        state_out[slot] = make_float4(tr, tg, tb, __uint_as_float(pix));
+    }
+}
+
+// Targets of the training samples from their training rays' hits: emission
+// or background directly; at a non-emissive hit a query {features, weight =
+// albedo, sample index} whose prediction the cache's inference writes.
+static __global__ __launch_bounds__(256) void k_pt_train_targets(DevScene s, PtArgs P, CacheArgs C,
+                                                                 const float *rays, const float4 *state,
+                                                                 const uint32_t *count, const int32_t *hit,
+                                                                 const float *tt, float4 *targets, float *queries,
+                                                                 uint32_t *n_queries) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < *count;
+    bool query = false;
+    float feat[10], wr = 0.f, wg = 0.f, wb = 0.f;
+    uint32_t k = 0;
+    if (active) {
+        k = __float_as_uint(state[i].w);
+        const int32_t h = hit[i];
+        if (h < 0) {
+            targets[k] = make_float4(P.bg[0], P.bg[1], P.bg[2], 1.f);
+        } else if (s.emit[h]) {
+            targets[k] = make_float4(s.color[3 * (size_t)h], s.color[3 * (size_t)h + 1], s.color[3 * (size_t)h + 2], 1.f);
+        } else {
+            const float *r = rays + 6 * (size_t)i;
+            const float t = tt[i];
+            wr = s.color[3 * (size_t)h], wg = s.color[3 * (size_t)h + 1], wb = s.color[3 * (size_t)h + 2];
+            cache_features(C, r[0] + r[3] * t, r[1] + r[4] * t, r[2] + r[5] * t, r[3], r[4], r[5],
+                           s.normal[3 * (size_t)h], s.normal[3 * (size_t)h + 1], s.normal[3 * (size_t)h + 2], wr, wg,
+                           wb, feat);
+            query = true;
+        }
+    }
+    const uint32_t q = wave_slot(query, n_queries);
+    if (query) {
+        float *o = queries + 16 * (size_t)q;
+#pragma unroll
+        for (int j = 0; j < 10; ++j) o[j] = feat[j];
+        o[10] = wr, o[11] = wg, o[12] = wb, o[13] = __uint_as_float(k);
     }
 }
 
 // sum += contribution of this pass (sample order), then advance the sample.
-__global__ void k_pt_accumulate(uint32_t n_pix, const float *contrib, float *sum, uint32_t *sample) {
+static __global__ void k_pt_accumulate(uint32_t n_pix, const float *contrib, float *sum, uint32_t *sample) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 3 * n_pix) sum[i] = sum[i] + contrib[i];
     if (i == 0) *sample = *sample + 1;
 }
 
-__global__ void k_pt_finalize(uint32_t n_pix, const float *sum, float spp, float *hdr, uint8_t *rgb) {
+static __global__ void k_pt_finalize(uint32_t n_pix, const float *sum, float spp, float *hdr, uint8_t *rgb) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 3 * n_pix) return;
     const float m = sum[i] / spp;
@@ -334,7 +465,7 @@ __global__ void k_pt_finalize(uint32_t n_pix, const float *sum, float spp, float
     if (rgb) rgb[i] = to_byte(clamp_std(m));
 }
 
-__global__ void k_fill_f32(float *p, uint32_t n, float v) {
+static __global__ void k_fill_f32(float *p, uint32_t n, float v) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cfloat>
+
 #include "device_math.hpp"
 
 namespace prt {
@@ -54,7 +56,7 @@ struct TraceArgs {
     int32_t *out_idx;
     float *out_t;             // optional
     uint32_t *counts;         // optional: 2 per ray (nodes, refs)
-    uint32_t *work;           // ray counter, zero at launch
+    uint32_t *work;           // ray counter, 64 x grid size at launch
     uint4 *spill;             // per lane slot: spill_depth entries
     uint32_t spill_depth;
     uint32_t fetch_min;       // refill when at least this many lanes are idle
@@ -95,7 +97,7 @@ __device__ __forceinline__ void load_tri(const DevScene &s, uint32_t k, uint32_t
 }
 
 template <bool ANY, int LAYOUT, int K, int ILP>
-__global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) {
+static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) {
     static_assert((K & (K - 1)) == 0, "K must be a power of two");
     __shared__ uint32_t st_w[K * WAVE];
     __shared__ float st_lo[K * WAVE];
@@ -111,6 +113,7 @@ __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) {
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, lim2 = 0;
     float lo = 0, hi = 0, best_t = INFINITY;
     int sp = 0, base = 0;
+    bool first_fetch = true;
 
     while (true) {
         // ------------------------------------------------------- refill --
@@ -120,9 +123,18 @@ __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) {
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (nidle && (nidle >= A.fetch_min || (idle | dead) == ~0ull)) {
             const uint32_t first = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
+            // The first batch of every wave is static (rays 64*block ..),
+            // the counter starts after them: a launch over few rays costs no
+            // atomics, where thousands of resident waves would otherwise
+            // queue on one address.
             uint32_t b = 0;
-            if (lane == first) b = atomicAdd(A.work, nidle);
-            b = __shfl(b, (int)first);
+            if (first_fetch) {
+                b = blockIdx.x * WAVE;
+                first_fetch = false;
+            } else {
+                if (lane == first) b = atomicAdd(A.work, nidle);
+                b = __shfl(b, (int)first);
+            }
             if (state == ST_IDLE) {
                 ray = b + (uint32_t)__popcll(idle & below);
                 if (ray >= nn) {
@@ -305,9 +317,13 @@ __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) {
     }
 }
 
-__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+static __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+static __global__ void k_set_u32(uint32_t *p, uint32_t v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
 
 }  // namespace prt

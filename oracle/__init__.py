@@ -53,6 +53,10 @@ def lib():
                                       _vp]),
             "orc_camera": (None, [_vp, _vp, _u64, _u64, _u64, _vp]),
             "orc_camera_rays": (None, [_vp, _u64, _u64, _vp]),
+            "orc_mlp_encode": (None, [_vp, _u64, _vp]),
+            "orc_mlp_forward": (None, [_vp, _vp, _u64, C.c_int, _vp]),
+            "orc_mlp_grad": (None, [_vp, _vp, _vp, _u64, C.c_int, _vp, _vp, _vp]),
+            "orc_adam": (None, [_vp, _vp, _vp, _vp, _u64, _u32, _f32]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -337,3 +341,39 @@ def ref_load_scene(path, fma=False):
             "max_depth": int(info[4]), "spp": int(info[5]), "tris": tris, "color": col, "emit": emit,
             "lights": lights, "background": bg, "camera_position": info[8:11].astype(np.float32),
             "camera_look": info[11:14].astype(np.float32)}
+
+
+# ------------------------------------------------------ radiance cache MLP --
+
+def mlp_encode(feats):
+    feats = np.ascontiguousarray(feats, np.float32).reshape(-1, 10)
+    out = np.empty((feats.shape[0], 192), np.float32)
+    lib().orc_mlp_encode(_p(feats), feats.shape[0], _p(out))
+    return out
+
+
+def mlp_forward(params, feats, emulate=True):
+    params = np.ascontiguousarray(params, np.float32)
+    feats = np.ascontiguousarray(feats, np.float32).reshape(-1, 10)
+    out = np.empty((feats.shape[0], 3), np.float64)
+    lib().orc_mlp_forward(_p(params), _p(feats), feats.shape[0], 1 if emulate else 0, _p(out))
+    return out
+
+
+def mlp_grad(params, feats, targets, emulate=True, denom=None):
+    """(gradient float64[30720], loss) of the relative-L2 loss; `denom`
+    (n x 3) freezes the loss denominators."""
+    params = np.ascontiguousarray(params, np.float32)
+    feats = np.ascontiguousarray(feats, np.float32).reshape(-1, 10)
+    targets = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+    den = None if denom is None else np.ascontiguousarray(denom, np.float64)
+    g = np.empty(params.shape[0], np.float64)
+    loss = np.zeros(1, np.float64)
+    lib().orc_mlp_grad(_p(params), _p(feats), _p(targets), feats.shape[0], 1 if emulate else 0, _p(den), _p(g),
+                       _p(loss))
+    return g, float(loss[0])
+
+
+def adam(params, m, v, grad, t, lr):
+    """One Adam step in place (float32 state, as the device keeps it)."""
+    lib().orc_adam(_p(params), _p(m), _p(v), _p(np.ascontiguousarray(grad, np.float64)), params.shape[0], t, lr)

@@ -926,3 +926,166 @@ ORC_API void orc_camera_rays(const float *c, uint64_t w, uint64_t h, float *rays
         }
     }
 }
+
+// ------------------------------------------------- radiance-cache MLP --
+// CPU model of the cache network (no reference counterpart; pinned by
+// central finite differences in double precision and by PyTorch autograd).
+// Layers row-major [out][in]: 64x192, 4 x 64x64, 32x64; inputs are the
+// encoding of 10 features (12 frequencies of sin/cos(2 pi 2^(k-1) x) for
+// the 7 geometric features, albedo, a constant 1, zero padding).  With
+// `emulate`, weights, encoded inputs, hidden activations and back-propagated
+// gradients are rounded to bfloat16 (nearest even) where the device rounds
+// them; sums are in double.  Without it everything is double.
+namespace {
+
+constexpr int MLP_IN = 192, MLP_W = 64, MLP_NP = 30720;
+constexpr int MLP_OUT[6] = {64, 64, 64, 64, 64, 32};
+constexpr int MLP_INS[6] = {192, 64, 64, 64, 64, 64};
+constexpr int MLP_OFS[6] = {0, 12288, 16384, 20480, 24576, 28672};
+
+inline float bf16r(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+float mlp_encode1(const float *f, int c) {
+    if (c < 168) {
+        const int fi = c / 24, r = c % 24, k = r >> 1;
+        const float x = std::ldexp(f[fi], k - 1);
+        const float u = x - std::floor(x);
+        float s, co;
+        sincos2pi(u, s, co);
+        return (r & 1) ? co : s;
+    }
+    if (c < 171) return f[7 + (c - 168)];
+    return c == 171 ? 1.f : 0.f;
+}
+
+struct MlpFwd {
+    double x[MLP_IN];
+    double h[5][MLP_W];  // post-ReLU (rounded when emulating)
+    double y[3];
+};
+
+void mlp_forward1(const std::vector<double> &W, const float *f, bool emulate, MlpFwd &F) {
+    for (int c = 0; c < MLP_IN; c++) {
+        const float e = mlp_encode1(f, c);
+        F.x[c] = emulate ? bf16r(e) : e;
+    }
+    const double *in = F.x;
+    for (int L = 0; L < 5; L++) {
+        for (int o = 0; o < MLP_W; o++) {
+            double z = 0;
+            const double *w = &W[MLP_OFS[L] + o * MLP_INS[L]];
+            for (int k = 0; k < MLP_INS[L]; k++) z += w[k] * in[k];
+            const double a = z > 0 ? z : 0;
+            F.h[L][o] = emulate ? bf16r((float)a) : a;
+        }
+        in = F.h[L];
+    }
+    for (int o = 0; o < 3; o++) {
+        double z = 0;
+        const double *w = &W[MLP_OFS[5] + o * MLP_W];
+        for (int k = 0; k < MLP_W; k++) z += w[k] * F.h[4][k];
+        F.y[o] = z;
+    }
+}
+
+std::vector<double> mlp_weights(const float *params, bool emulate) {
+    std::vector<double> W(MLP_NP);
+    for (int p = 0; p < MLP_NP; p++) W[p] = emulate ? bf16r(params[p]) : params[p];
+    return W;
+}
+
+}  // namespace
+
+ORC_API void orc_mlp_encode(const float *feats, uint64_t n, float *out) {
+    for (uint64_t i = 0; i < n; i++)
+        for (int c = 0; c < MLP_IN; c++) out[i * MLP_IN + c] = mlp_encode1(feats + 10 * i, c);
+}
+
+ORC_API void orc_mlp_forward(const float *params, const float *feats, uint64_t n, int emulate, double *out) {
+    const std::vector<double> W = mlp_weights(params, emulate != 0);
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; i++) {
+        MlpFwd F;
+        mlp_forward1(W, feats + 10 * i, emulate != 0, F);
+        for (int r = 0; r < 3; r++) out[3 * i + r] = F.y[r];
+    }
+}
+
+// Loss = mean over samples and channels of (y - t)^2 / D with D = y^2 + 0.01
+// taken from this evaluation, or from `denom` (n x 3) when given; the
+// gradient treats D as a constant.
+ORC_API void orc_mlp_grad(const float *params, const float *feats, const float *targets, uint64_t n, int emulate,
+                          const double *denom, double *grad, double *loss) {
+    const bool em = emulate != 0;
+    const std::vector<double> W = mlp_weights(params, em);
+    const int nt = omp_get_max_threads();
+    std::vector<std::vector<double>> G(nt, std::vector<double>(MLP_NP, 0.0));
+    std::vector<double> Ls(nt, 0.0);
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; i++) {
+        std::vector<double> &g = G[omp_get_thread_num()];
+        MlpFwd F;
+        mlp_forward1(W, feats + 10 * i, em, F);
+        double dz[MLP_W], dh[MLP_W];
+        double dy[32] = {0};
+        for (int r = 0; r < 3; r++) {
+            const double y = F.y[r], t = targets[3 * i + r];
+            const double D = denom ? denom[3 * i + r] : y * y + 0.01;
+            Ls[omp_get_thread_num()] += (y - t) * (y - t) / D;
+            const double d = 2.0 * (y - t) / D / (3.0 * (double)n);
+            dy[r] = em ? bf16r((float)d) : d;
+        }
+        // output layer
+        for (int o = 0; o < 3; o++)
+            for (int k = 0; k < MLP_W; k++) g[MLP_OFS[5] + o * MLP_W + k] += dy[o] * F.h[4][k];
+        for (int k = 0; k < MLP_W; k++) {
+            double s = 0;
+            for (int o = 0; o < 3; o++) s += W[MLP_OFS[5] + o * MLP_W + k] * dy[o];
+            dh[k] = s;
+        }
+        for (int L = 4; L >= 0; L--) {
+            for (int k = 0; k < MLP_W; k++) {
+                const double v = F.h[L][k] > 0 ? dh[k] : 0.0;
+                dz[k] = em ? bf16r((float)v) : v;
+            }
+            const double *in = L > 0 ? F.h[L - 1] : F.x;
+            for (int o = 0; o < MLP_W; o++)
+                for (int k = 0; k < MLP_INS[L]; k++) g[MLP_OFS[L] + o * MLP_INS[L] + k] += dz[o] * in[k];
+            if (L == 0) break;
+            for (int k = 0; k < MLP_W; k++) {
+                double s = 0;
+                for (int o = 0; o < MLP_W; o++) s += W[MLP_OFS[L] + o * MLP_W + k] * dz[o];
+                dh[k] = s;
+            }
+        }
+    }
+    for (int p = 0; p < MLP_NP; p++) {
+        double s = 0;
+        for (int t = 0; t < nt; t++) s += G[t][p];
+        grad[p] = s;
+    }
+    if (loss) {
+        double s = 0;
+        for (double x : Ls) s += x;
+        *loss = s / (3.0 * (double)n);
+    }
+}
+
+// One Adam step (beta1 0.9, beta2 0.99, eps 1e-8) on float master weights.
+ORC_API void orc_adam(float *params, float *m, float *v, const double *grad, uint64_t n, uint32_t t, float lr) {
+    for (uint64_t p = 0; p < n; p++) {
+        const float g = (float)grad[p];
+        m[p] = 0.9f * m[p] + (1.f - 0.9f) * g;
+        v[p] = 0.99f * v[p] + (1.f - 0.99f) * g * g;
+        const float mh = m[p] / (1.f - std::pow(0.9f, (float)t));
+        const float vh = v[p] / (1.f - std::pow(0.99f, (float)t));
+        params[p] = params[p] - lr * mh / (std::sqrt(vh) + 1e-8f);
+    }
+}
