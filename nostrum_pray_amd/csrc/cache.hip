@@ -207,17 +207,18 @@ int pray_cache_create(pray_ctx *c, uint64_t seed, float learning_rate, uint32_t 
         free_cache_rec(r);
         return rc;
     }
-    // Uniform(+-sqrt(6 / (fan_in + fan_out))) on the used entries; the padded
-    // input columns of layer 0 and output rows of layer 5 stay zero (they
-    // receive no gradient).
+    // Uniform(+-sqrt(6 / (fan_in + fan_out))) on the used entries; the 20
+    // always-zero input channels of layer 0 and output rows 3.. of layer 5
+    // stay zero (they receive no gradient).
     std::vector<float> w(prm::N_PARAMS, 0.f);
     SplitMix rng{seed};
     for (int L = 0; L < prm::N_LAYERS; L++) {
         const prm::Layer Y = prm::layer(L);
-        const int rows = L == 5 ? 3 : Y.out, cols = L == 0 ? 172 : Y.in;
-        const double a = std::sqrt(6.0 / (double)(cols + rows));
+        const int rows = L == 5 ? 3 : Y.out, used_in = L == 0 ? 172 : Y.in;
+        const double a = std::sqrt(6.0 / (double)(used_in + rows));
         for (int o = 0; o < rows; o++)
-            for (int k = 0; k < cols; k++) w[Y.pofs + o * Y.in + k] = (float)((2.0 * rng.u01() - 1.0) * a);
+            for (int k = 0; k < Y.in; k++)
+                if (L != 0 || prm::input_used(k)) w[Y.pofs + o * Y.in + k] = (float)((2.0 * rng.u01() - 1.0) * a);
     }
     hipError_t e = hipMemcpy(r->master, w.data(), prm::N_PARAMS * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(r->partial, 0, (size_t)r->max_chunks * prm::N_PARAMS * 4);

@@ -8,8 +8,13 @@
 //
 // Features per sample: position in the scene box mapped to [0,1]^3, ray
 // direction and surface normal as (theta/pi, phi/2pi + 1/2), albedo rgb.
-// Encoding: channel c < 168 is sin / cos(2 pi 2^(k-1) x_f) with f = c / 24,
-// k = (c % 24) / 2 and cos for odd c; 168..170 albedo; 171 = 1; rest 0.
+// Encoding (192 input channels; see enc_value): with channel c held in slot
+// s of lane group g of reduction block m (c = kchan(m, g, s)) and
+// q = 8m + s, e = 4(q % 6) + g:
+//   q < 42   sin (e even) or cos (e odd) of 2 pi 2^(e/2 - 1) x_f, f = q / 6
+//   q == 42  albedo r, g, b for g = 0, 1, 2 and the constant 1 for g = 3
+//   q > 42   0
+// so the feature read by a slot does not depend on the lane.
 //
 // All layers are computed "transposed": Z = W * H with output channels as
 // matrix rows and samples as columns, 16 samples per wave.  The accumulator
@@ -62,6 +67,13 @@ __host__ __device__ inline void kslot(int k, int &g, int &s) {
 // Channel held in slot s of lane group g for reduction block m.
 __host__ __device__ inline int kchan(int m, int g, int s) { return 32 * m + (s < 4 ? 4 * g + s : 16 + 4 * g + s - 4); }
 
+// Whether encoded input channel c can be nonzero (172 of the 192 can).
+__host__ __device__ inline bool input_used(int c) {
+    int g, s;
+    kslot(c & 31, g, s);
+    return 8 * (c >> 5) + s <= 42;
+}
+
 // Round to nearest even (finite inputs).
 __device__ __forceinline__ uint16_t bf16_bits(float x) {
     const uint32_t u = __float_as_uint(x);
@@ -82,19 +94,23 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f3
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Encoded input channel c of a feature vector.
-__device__ __forceinline__ float encode(const float *f, int c) {
-    if (c < 168) {
-        const int fi = c / 24, r = c % 24, k = r >> 1;
-        const float x = ldexpf(f[fi], k - 1);
+// Encoded input held in slot s of lane group g of reduction block m (m and s
+// are compile-time constants where this is inlined, so f is indexed
+// statically).
+__device__ __forceinline__ float enc_value(const float *f, int m, int g, int s) {
+    const int q = 8 * m + s;
+    if (q < 42) {
+        const int e = 4 * (q % 6) + g;
+        const float x = ldexpf(f[q / 6], (e >> 1) - 1);
         const float u = x - floorf(x);
-        float s, co;
-        prd::sincos2pi(u, s, co);
-        return (r & 1) ? co : s;
+        float sn, cs;
+        prd::sincos2pi(u, sn, cs);
+        return (e & 1) ? cs : sn;
     }
-    if (c < 171) return f[7 + (c - 168)];
-    return c == 171 ? 1.f : 0.f;
+    if (q == 42) return g == 0 ? f[7] : (g == 1 ? f[8] : (g == 2 ? f[9] : 1.f));
+    return 0.f;
 }
+
 
 // Fragment positions of master parameter p (forward and, for layers >= 1,
 // backward / transposed).  Returns the layer.
@@ -166,7 +182,7 @@ __device__ __forceinline__ void encode_frags(const float *f, uint32_t g, Frag (&
 #pragma unroll
     for (int m = 0; m < 6; ++m)
 #pragma unroll
-        for (int s = 0; s < 8; ++s) X[m].h[s] = bf16_bits(encode(f, kchan(m, (int)g, s)));
+        for (int s = 0; s < 8; ++s) X[m].h[s] = bf16_bits(enc_value(f, m, (int)g, s));
 }
 
 // Queries for inference: 16 floats each {features[10], weight rgb, slot bits,
@@ -182,33 +198,102 @@ struct InferArgs {
     float4 *targets;
 };
 
-static __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x8 *__restrict__ Wf, InferArgs A) {
+// Inference: the forward fragments are staged in LDS once per block (60 KB);
+// each wave evaluates INF_NB 16-sample blocks per iteration so that every
+// weight fragment read from LDS feeds INF_NB matrix operations.
+constexpr int INF_NB = 4;
+
+static __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x8 *__restrict__ Wg, InferArgs A) {
+    __shared__ bf16x8 Wf[N_FFRAG * WAVE];
+    for (int k = threadIdx.x; k < N_FFRAG * WAVE; k += blockDim.x) Wf[k] = Wg[k];
+    __syncthreads();
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t j = lane & 15, g = lane >> 4;
     const uint32_t n = A.n_dev ? *A.n_dev : A.n;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     const uint32_t n_waves = gridDim.x * blockDim.x / WAVE;
-    for (uint32_t b = wave; b * 16 < n; b += n_waves) {
-        const uint32_t i = b * 16 + j;
-        float f[N_FEAT];
-        const bool ok = i < n;
+    constexpr uint32_t PER = 16 * INF_NB;
+    for (uint32_t b = wave; b * PER < n; b += n_waves) {
+        float f[INF_NB][N_FEAT];
 #pragma unroll
-        for (int k = 0; k < N_FEAT; ++k) f[k] = ok ? A.queries[16 * (size_t)i + k] : 0.f;
-        Frag X[6], H[5][2];
-        encode_frags(f, g, X);
-        const f32x4 y = forward16(Wf, X, H, lane);
-        if (ok && g == 0) {
-            if (A.out) {
-                A.out[3 * (size_t)i] = y[0], A.out[3 * (size_t)i + 1] = y[1], A.out[3 * (size_t)i + 2] = y[2];
-            } else {
-                const float *q = A.queries + 16 * (size_t)i;
-                const uint32_t slot = __float_as_uint(q[13]);
-                const float r = q[10] * fmaxf(y[0], 0.f), gg = q[11] * fmaxf(y[1], 0.f), b2 = q[12] * fmaxf(y[2], 0.f);
+        for (int q = 0; q < INF_NB; ++q) {
+            const uint32_t i = b * PER + q * 16 + j;
+#pragma unroll
+            for (int k = 0; k < N_FEAT; ++k) f[q][k] = i < n ? A.queries[16 * (size_t)i + k] : 0.f;
+        }
+        f32x4 acc[4][INF_NB];
+#pragma unroll
+        for (int ob = 0; ob < 4; ++ob)
+#pragma unroll
+            for (int q = 0; q < INF_NB; ++q) acc[ob][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            Frag X[INF_NB];
+#pragma unroll
+            for (int q = 0; q < INF_NB; ++q)
+#pragma unroll
+                for (int s = 0; s < 8; ++s) X[q].h[s] = bf16_bits(enc_value(f[q], m, (int)g, s));
+#pragma unroll
+            for (int ob = 0; ob < 4; ++ob) {
+                const bf16x8 w = Wf[(ob * 6 + m) * WAVE + lane];
+#pragma unroll
+                for (int q = 0; q < INF_NB; ++q) acc[ob][q] = mfma(w, X[q].v, acc[ob][q]);
+            }
+        }
+        Frag H[INF_NB][2];
+#pragma unroll
+        for (int L = 0; L < 5; ++L) {
+            if (L > 0) {
+                const Layer Y = layer(L);
+#pragma unroll
+                for (int ob = 0; ob < 4; ++ob) {
+#pragma unroll
+                    for (int q = 0; q < INF_NB; ++q) acc[ob][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        const bf16x8 w = Wf[(Y.ffrag + ob * 2 + m) * WAVE + lane];
+#pragma unroll
+                        for (int q = 0; q < INF_NB; ++q) acc[ob][q] = mfma(w, H[q][m].v, acc[ob][q]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < INF_NB; ++q)
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        H[q][m].h[r] = bf16_bits(relu(acc[2 * m][q][r]));
+                        H[q][m].h[4 + r] = bf16_bits(relu(acc[2 * m + 1][q][r]));
+                    }
+        }
+        f32x4 y[INF_NB];
+#pragma unroll
+        for (int q = 0; q < INF_NB; ++q) y[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const bf16x8 w = Wf[(layer(5).ffrag + m) * WAVE + lane];
+#pragma unroll
+            for (int q = 0; q < INF_NB; ++q) y[q] = mfma(w, H[q][m].v, y[q]);
+        }
+        if (g == 0) {
+#pragma unroll
+            for (int q = 0; q < INF_NB; ++q) {
+                const uint32_t i = b * PER + q * 16 + j;
+                if (i >= n) continue;
+                if (A.out) {
+                    A.out[3 * (size_t)i] = y[q][0], A.out[3 * (size_t)i + 1] = y[q][1], A.out[3 * (size_t)i + 2] = y[q][2];
+                    continue;
+                }
+                const float *qq = A.queries + 16 * (size_t)i;
+                const uint32_t slot = __float_as_uint(qq[13]);
+                const float r = qq[10] * fmaxf(y[q][0], 0.f), gg = qq[11] * fmaxf(y[q][1], 0.f),
+                            bb = qq[12] * fmaxf(y[q][2], 0.f);
                 if (A.targets) {
-                    A.targets[slot] = make_float4(r, gg, b2, 1.f);
+                    A.targets[slot] = make_float4(r, gg, bb, 1.f);
                 } else {
                     float *c = A.contrib + 3 * (size_t)slot;
-                    c[0] = r, c[1] = gg, c[2] = b2;
+                    c[0] = r, c[1] = gg, c[2] = bb;
                 }
             }
         }

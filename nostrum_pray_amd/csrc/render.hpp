@@ -303,6 +303,28 @@ __device__ __forceinline__ uint32_t train_offset(uint32_t frame, uint32_t n_pix,
     return (uint32_t)(((uint64_t)u.x * n_pix) >> 32);
 }
 
+// Block-aggregated slot in a queue: one atomic per 256-thread block (every
+// thread of the block must call it; `lds` holds 5 words).
+__device__ __forceinline__ uint32_t block_slot(bool want, uint32_t *counter, uint32_t *lds) {
+    const uint64_t m = __ballot(want);
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    if (lane == 0) lds[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t k = 0; k < blockDim.x / WAVE; ++k) {
+            const uint32_t c = lds[k];
+            lds[k] = tot;
+            tot += c;
+        }
+        lds[4] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = lds[4] + lds[wv] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads();
+    return base;
+}
+
 // Wave-aggregated slot in a queue: one atomic per wave.
 __device__ __forceinline__ uint32_t wave_slot(bool want, uint32_t *counter) {
     const uint64_t m = __ballot(want);
@@ -387,23 +409,24 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
             c[0] = cr, c[1] = cg, c[2] = cb;
         }
     }
+    __shared__ uint32_t qs[5];
     if (MODE == PT_CACHE) {
-        const uint32_t q = wave_slot(query, C.n_queries);
+        const uint32_t q = block_slot(query, C.n_queries, qs);
         if (query) {
             float *o = C.queries + 16 * (size_t)q;
 #pragma unroll
             for (int k = 0; k < 10; ++k) o[k] = feat[k];
             o[10] = tr, o[11] = tg, o[12] = tb, o[13] = __uint_as_float(pix);
         }
-        const uint32_t tq = wave_slot(train, C.n_train_rays);
+        const uint32_t tq = block_slot(train, C.n_train_rays, qs);
         if (train) {
             float *o = C.train_rays + 6 * (size_t)tq;
             o[0] = ox, o[1] = oy, o[2] = oz, o[3] = tdx, o[4] = tdy, o[5] = tdz;
             C.train_state[tq] = make_float4(1.f, 1.f, 1.f, __uint_as_float(tk));
         }
     }
-    // one atomic per wave for the surviving paths
-    const uint32_t slot = wave_slot(push, count_out);
+    // one atomic per block for the surviving paths
+    const uint32_t slot = block_slot(push, count_out, qs);
     if (push) {
         float *o = rays_out + 6 * (size_t)slot;
         o[0] = ox, o[1] = oy, o[2] = oz, o[3] = dx, o[4] = dy, o[5] = dz;

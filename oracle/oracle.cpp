@@ -932,7 +932,8 @@ ORC_API void orc_camera_rays(const float *c, uint64_t w, uint64_t h, float *rays
 // central finite differences in double precision and by PyTorch autograd).
 // Layers row-major [out][in]: 64x192, 4 x 64x64, 32x64; inputs are the
 // encoding of 10 features (12 frequencies of sin/cos(2 pi 2^(k-1) x) for
-// the 7 geometric features, albedo, a constant 1, zero padding).  With
+// the 7 geometric features, albedo, a constant 1, zero padding) in the
+// device's channel order (mlp_encode1).  With
 // `emulate`, weights, encoded inputs, hidden activations and back-propagated
 // gradients are rounded to bfloat16 (nearest even) where the device rounds
 // them; sums are in double.  Without it everything is double.
@@ -952,17 +953,25 @@ inline float bf16r(float x) {
     return r;
 }
 
+// Channel c sits in slot s of lane group g of the 32-channel block m
+// (c = 32m + 4g + s for s < 4, 32m + 16 + 4g + s - 4 otherwise); with
+// q = 8m + s and e = 4(q % 6) + g it holds sin (e even) / cos (e odd) of
+// 2 pi 2^(e/2 - 1) x_f, f = q / 6, for q < 42; albedo r, g, b and the
+// constant 1 for q == 42; zero beyond.
 float mlp_encode1(const float *f, int c) {
-    if (c < 168) {
-        const int fi = c / 24, r = c % 24, k = r >> 1;
-        const float x = std::ldexp(f[fi], k - 1);
+    const int m = c >> 5, cc = c & 31;
+    const int g = (cc & 15) >> 2, s = (cc & 3) + (cc >= 16 ? 4 : 0);
+    const int q = 8 * m + s;
+    if (q < 42) {
+        const int e = 4 * (q % 6) + g;
+        const float x = std::ldexp(f[q / 6], (e >> 1) - 1);
         const float u = x - std::floor(x);
-        float s, co;
-        sincos2pi(u, s, co);
-        return (r & 1) ? co : s;
+        float sn, cs;
+        sincos2pi(u, sn, cs);
+        return (e & 1) ? cs : sn;
     }
-    if (c < 171) return f[7 + (c - 168)];
-    return c == 171 ? 1.f : 0.f;
+    if (q == 42) return g < 3 ? f[7 + g] : 1.f;
+    return 0.f;
 }
 
 struct MlpFwd {

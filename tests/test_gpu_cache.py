@@ -8,7 +8,7 @@ import pytest
 import nostrum_pray_amd as P
 import oracle as O
 from conftest import say
-from test_oracle_mlp import random_batch, random_params
+from test_oracle_mlp import USED, random_batch, random_params
 
 pytestmark = pytest.mark.gpu
 
@@ -23,7 +23,7 @@ def cache(ctx):
 def test_initial_params_layout(cache):
     w = cache.params()
     assert w.shape == (30720,)
-    assert np.all(w[:12288].reshape(64, 192)[:, 172:] == 0)
+    assert np.all(w[:12288].reshape(64, 192)[:, ~USED] == 0) and USED.sum() == 172
     assert np.all(w[28672:].reshape(32, 64)[3:] == 0)
     assert np.abs(w[:12288]).max() <= np.sqrt(6 / 236) and np.abs(w[:12288]).max() > 0.1
 
@@ -62,7 +62,7 @@ def test_gradient_matches_oracle(cache):
     for lo, hi in zip([0, 12288, 16384, 20480, 24576, 28672], [12288, 16384, 20480, 24576, 28672, 30720]):
         d = np.linalg.norm(g[lo:hi] - gr[lo:hi]) / np.linalg.norm(gr[lo:hi])
         assert d < 2e-2, (lo, d)
-    assert np.all(g[:12288].reshape(64, 192)[:, 172:] == 0) and np.all(g[28672:].reshape(32, 64)[3:] == 0)
+    assert np.all(g[:12288].reshape(64, 192)[:, ~USED] == 0) and np.all(g[28672:].reshape(32, 64)[3:] == 0)
 
 
 def test_adam_step_matches_oracle(cache):

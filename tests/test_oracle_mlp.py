@@ -16,16 +16,25 @@ OFS = [0, 12288, 16384, 20480, 24576, 28672, 30720]
 INS = [192, 64, 64, 64, 64, 64]
 
 
+def channel_slot(c):
+    """(block m, lane group g, slot s) of encoded input channel c."""
+    m, cc = c >> 5, c & 31
+    return m, (cc & 15) >> 2, (cc & 3) + (4 if cc >= 16 else 0)
+
+
+USED = np.array([8 * channel_slot(c)[0] + channel_slot(c)[2] <= 42 for c in range(192)])
+
+
 def random_params(seed):
     """Uniform(+-sqrt(6/(fan_in+fan_out))) on the used entries, zero padding."""
     rng = np.random.default_rng(seed)
     w = np.zeros(N_PARAMS, np.float32)
     for L in range(6):
         rows = 3 if L == 5 else 64
-        cols = 172 if L == 0 else 64
-        a = np.sqrt(6.0 / (rows + cols))
+        cols = USED if L == 0 else np.ones(64, bool)
+        a = np.sqrt(6.0 / (rows + cols.sum()))
         blk = w[OFS[L]:OFS[L + 1]].reshape(-1, INS[L])
-        blk[:rows, :cols] = rng.uniform(-a, a, (rows, cols))
+        blk[:rows, cols] = rng.uniform(-a, a, (rows, cols.sum()))
     return w
 
 
@@ -37,16 +46,32 @@ def random_batch(seed, n):
 
 
 def test_encoding_matches_double_precision_sin_cos():
+    """Every (feature, frequency, sin/cos) appears in exactly one channel with
+    the value of double-precision sin/cos; albedo and the constant 1 appear
+    once; the 20 remaining channels are zero."""
     f, _ = random_batch(1, 300)
     e = O.mlp_encode(f).astype(np.float64)
+    ang = 2 * np.pi * f[:, :7, None].astype(np.float64) * np.ldexp(1.0, np.arange(12) - 1)[None, None, :]
+    want = {}
     for fi in range(7):
         for k in range(12):
-            ang = 2 * np.pi * np.ldexp(f[:, fi].astype(np.float64), k - 1)
-            c = fi * 24 + 2 * k
-            assert np.max(np.abs(e[:, c] - np.sin(ang))) < 2e-6
-            assert np.max(np.abs(e[:, c + 1] - np.cos(ang))) < 2e-6
-    assert np.array_equal(e[:, 168:171], f[:, 7:10])
-    assert (e[:, 171] == 1).all() and (e[:, 172:] == 0).all()
+            want[("sin", fi, k)] = np.sin(ang[:, fi, k])
+            want[("cos", fi, k)] = np.cos(ang[:, fi, k])
+    for r in range(3):
+        want[("albedo", r)] = f[:, 7 + r].astype(np.float64)
+    want[("one",)] = np.ones(300)
+    found = {}
+    for c in range(192):
+        col = e[:, c]
+        if not col.any():
+            found.setdefault("zero", []).append(c)
+            continue
+        hits = [key for key, v in want.items() if np.max(np.abs(col - v)) < 2e-6]
+        assert len(hits) == 1, (c, hits)
+        assert hits[0] not in found, hits[0]
+        found[hits[0]] = c
+    assert len(found) == len(want) + 1 and len(found["zero"]) == 20
+    assert sorted(found["zero"]) == list(np.flatnonzero(~USED))
 
 
 def test_gradient_matches_central_differences():
@@ -62,9 +87,9 @@ def test_gradient_matches_central_differences():
     picks = []
     for L in range(6):
         rows = 3 if L == 5 else 64
-        cols = 172 if L == 0 else 64
+        cols = np.flatnonzero(USED) if L == 0 else np.arange(64)
         for _ in range(8):
-            picks.append(OFS[L] + rng.integers(rows) * INS[L] + rng.integers(cols))
+            picks.append(OFS[L] + rng.integers(rows) * INS[L] + rng.choice(cols))
     # A single weight enters the outputs linearly while no ReLU changes
     # state, so with frozen denominators the loss is quadratic in it and the
     # central difference is exact up to rounding; a small step keeps every
@@ -81,7 +106,7 @@ def test_gradient_matches_central_differences():
         fd = (lp - lm) / (dp + dm)
         assert abs(fd - g[p]) <= 1e-7 * np.max(np.abs(g)) + 1e-5 * abs(g[p]), (p, fd, g[p])
     # padding receives no gradient
-    assert np.all(g[OFS[0]:OFS[1]].reshape(64, 192)[:, 172:] == 0)
+    assert np.all(g[OFS[0]:OFS[1]].reshape(64, 192)[:, ~USED] == 0)
     assert np.all(g[OFS[5]:OFS[6]].reshape(32, 64)[3:] == 0)
 
 
