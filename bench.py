@@ -1,11 +1,13 @@
 """Benchmark: closest-hit Mrays/s on 2^24 incoherent rays over the 8.39M-triangle
 heightfield terrain (2048x2048 cells), SAH KD tree built on the CPU and
-uploaded once, one MI355X.
+uploaded once, one MI355X; and the equal-RMSE wall time of the radiance-cache
+path tracer against the plain path tracer on the 100k-triangle Cornell room.
 
 A step is one closest-hit launch over the whole ray stream (rays resident in
 HBM).  Prints ONE JSON line with the metric, a roofline object for the
-traversal kernel and a CPU baseline (the restated reference CPU path timed on
-this host).  Run:  python bench.py [--steps K] [--warmup W]
+traversal kernel, the radiance-cache comparison and a CPU baseline (the
+restated reference CPU path timed on this host).
+Run:  python bench.py [--steps K] [--warmup W]
 """
 import argparse
 import json
@@ -59,6 +61,9 @@ def main():
     ap.add_argument("--cpu-rays", type=int, default=1 << 24, help="CPU baseline sample (prefix of the stream)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run one plain device copy of this many MiB (reference for HBM counters)")
+    ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-RMSE measurement")
+    ap.add_argument("--cache-size", type=int, default=1024, help="image side of the radiance-cache measurement")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="samples per pixel of the plain reference image")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -163,6 +168,15 @@ def main():
                     "identical_hits": same, "tree": "SAH depth cap 20, leaf size 32, costs 5 / 1.5"}
         log(f"reference-parameter tree: {ms_ref:.3f} ms/step, identical hits {same}")
 
+    rc_out = None
+    if not args.no_cache:
+        for b in (d_rays, d_idx, d_t, d_lim):
+            b.free()
+        d_rays = d_idx = d_t = d_lim = None
+        ctx.free_scene(sid)
+        sid = None
+        rc_out = radiance_cache_leg(P, ctx, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, args, box)
@@ -204,13 +218,86 @@ def main():
         },
         "any_hit": {"ms_per_step": round(ms_any, 4), "Mrays_per_s": round(n / (ms_any * 1e-3) / 1e6, 2)},
         "reference_parameter_tree": ref_tree,
+        "radiance_cache": rc_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     for b in (d_rays, d_idx, d_t, d_lim):
-        b.free()
-    ctx.free_scene(sid)
+        if b is not None:
+            b.free()
+    if sid is not None:
+        ctx.free_scene(sid)
     ctx.close()
+
+
+def equal_rmse_time(points, target):
+    """Time at which a method whose runs gave (ms, rmse) points reaches
+    `target`: log-log interpolation between the two nearest points (RMSE of
+    an unbiased estimator falls as time^-1/2), extrapolated at the ends."""
+    pts = sorted(points)
+    lt = np.log([p[0] for p in pts])
+    lr = np.log([p[1] for p in pts])
+    k = int(np.searchsorted(-lr, -np.log(target)))
+    k = min(max(k, 1), len(pts) - 1)
+    slope = (lt[k] - lt[k - 1]) / (lr[k] - lr[k - 1])
+    return float(np.exp(lt[k - 1] + slope * (np.log(target) - lr[k - 1])))
+
+
+def radiance_cache_leg(P, ctx, args):
+    """Equal-RMSE wall time of the cache path tracer against the plain
+    wavefront path tracer on the Cornell scene (100k triangles, depth 8,
+    roulette from bounce 3), both measured against a plain reference with
+    --ref-spp samples per pixel, RMSE on linear HDR."""
+    w = h = args.cache_size
+    scene = P.Scene.cornell(1, 100_000)
+    tree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+    sid = ctx.upload(tree)
+    inf = scene.info()
+    cam, bg = scene.camera(), inf["background"]
+    depth, rr = 8, 3
+    t0 = time.perf_counter()
+    ref, _ = ctx.render_pt(sid, cam, w, h, args.ref_spp, depth, rr, 1000003, bg, 1e-9, want_rgb=False)
+    log(f"cache leg: reference {args.ref_spp} spp in {time.perf_counter() - t0:.1f}s")
+
+    def rmse(img):
+        return float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
+
+    plain = []
+    for spp in (16, 32, 64, 128):
+        t0 = time.perf_counter()
+        img, _ = ctx.render_pt(sid, cam, w, h, spp, depth, rr, 7, bg, 1e-9, want_rgb=False)
+        ms = (time.perf_counter() - t0) * 1e3
+        plain.append({"spp": spp, "ms": round(ms, 2), "rmse": round(rmse(img), 6)})
+        log(f"cache leg: plain {spp} spp {ms:.1f} ms rmse {plain[-1]['rmse']:.5f}")
+    cache = []
+    warm = 8
+    for frames in (16, 32, 64, 128):
+        c = P.Cache(ctx, seed=3, learning_rate=1e-2, max_batch=1 << 16)
+        t0 = time.perf_counter()
+        img, _, st = c.render_pt(sid, cam, w, h, frames, depth, rr, 7, bg, 1e-9, train_paths=1 << 16,
+                                 cache_bounce=1, min_steps=warm)
+        ms = (time.perf_counter() - t0) * 1e3
+        c.close()
+        cache.append({"frames": frames, "ms": round(ms, 2), "rmse": round(rmse(img), 6),
+                      "frames_with_cache": int(st[:, 1].sum()), "final_loss": round(float(st[-1, 0]), 4)})
+        log(f"cache leg: cache {frames} frames {ms:.1f} ms rmse {cache[-1]['rmse']:.5f}")
+    pp = [(p["ms"], p["rmse"]) for p in plain]
+    for cpt in cache:
+        cpt["plain_ms_at_equal_rmse"] = round(equal_rmse_time(pp, cpt["rmse"]), 2)
+        cpt["speedup"] = round(cpt["plain_ms_at_equal_rmse"] / cpt["ms"], 3)
+    ctx.free_scene(sid)
+    best = cache[-1]
+    return {
+        "scene": f"Cornell room, {scene.num_tris} triangles, {w}x{h}, depth {depth}, roulette from bounce {rr}, "
+                 f"SAH tree (leaf size {args.leaf_size})",
+        "reference_spp": args.ref_spp,
+        "cache": {"network": "192 (172 used) -> 5 x 64 ReLU -> 3, BF16 matrix cores, FP32 Adam lr 1e-2",
+                  "training_samples_per_frame": 1 << 16, "terminates_at_hit": 2, "warmup_frames": warm},
+        "plain_runs": plain,
+        "cache_runs": cache,
+        "equal_rmse_speedup": best["speedup"],
+        "at_rmse": best["rmse"],
+    }
 
 
 def cpu_baseline(scene, args, box):
