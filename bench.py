@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-RMSE measurement")
     ap.add_argument("--cache-size", type=int, default=1024, help="image side of the radiance-cache measurement")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples per pixel of the plain reference image")
+    ap.add_argument("--workload", choices=("terrain", "batch"), default="terrain",
+                    help="terrain: the closest-hit stream (default); batch: 64 scenes path traced in one wavefront")
+    ap.add_argument("--batch-scenes", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -72,6 +75,10 @@ def main():
         sys.exit(2)
 
     import nostrum_pray_amd as P
+
+    if args.workload == "batch":
+        batch_workload(P, args)
+        return
 
     t0 = time.time()
     scene = P.Scene.terrain(args.grid, args.seed)
@@ -228,6 +235,79 @@ def main():
     if sid is not None:
         ctx.free_scene(sid)
     ctx.close()
+
+
+def batch_workload(P, args):
+    """Random icosphere/box/terrain scenes with triangle counts log-uniform in
+    [1k, 2M] and images of 128^2 .. 1024^2, 16 spp, depth 6: autotuned per
+    size bucket, path traced together in one wavefront and each alone (the
+    two must agree bit for bit); prints one JSON line with the aggregate and
+    per-scene Mrays/s."""
+    rng = np.random.default_rng(args.seed)
+    n = args.batch_scenes
+    tris = np.exp(rng.uniform(np.log(1e3), np.log(2e6), n)).astype(np.int64)
+    sides = rng.choice([128, 256, 512, 1024], n)
+    ctx = P.Context(0)
+    t0 = time.time()
+    scenes, sids = [], []
+    for k in range(n):
+        s = P.Scene.random(int(args.seed * 1000 + k), int(tris[k]))
+        t = s.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+        sids.append(ctx.upload(t))
+        scenes.append(s)
+    log(f"batch: {n} scenes ({int(sum(s.num_tris for s in scenes))} triangles) built and uploaded in "
+        f"{time.time() - t0:.1f}s")
+    tuned = {}
+    for k in range(n):
+        tun = ctx.autotune(sids[k])
+        b = int(np.floor(np.log(scenes[k].num_tris) / np.log(4)))
+        tuned.setdefault(b, tun)
+    cams = [P_camera(P, s, int(sides[k])) for k, s in enumerate(scenes)]
+    sizes = [(int(v), int(v)) for v in sides]
+    seeds = [17 + k for k in range(n)]
+    bgs = [s.info()["background"] for s in scenes]
+    spp, depth, rr = 16, 6, 3
+    imgs, _ = ctx.render_pt_batch(sids, cams, sizes, 1, depth, rr, seeds, bgs, 1e-9)  # warm-up
+    imgs, ms = ctx.render_pt_batch(sids, cams, sizes, spp, depth, rr, seeds, bgs, 1e-9)
+    rays = ctx.last_render_rays()
+    log(f"batch: {ms:.1f} ms, {rays} rays, {rays / ms / 1e3:.1f} Mrays/s")
+    per, same, single_ms = [], True, 0.0
+    for k in range(n):
+        ctx.record(0)
+        hdr, _ = ctx.render_pt(sids[k], cams[k], sizes[k][0], sizes[k][1], spp, depth, rr, seeds[k], bgs[k], 1e-9,
+                               want_rgb=False)
+        ctx.record(1)
+        kms = ctx.elapsed_ms(0, 1)
+        kr = ctx.last_render_rays()
+        ok = bool(np.array_equal(hdr.view(np.uint32), imgs[k].view(np.uint32)))
+        same = same and ok
+        single_ms += kms
+        per.append({"triangles": scenes[k].num_tris, "image": sizes[k][0], "ms_alone": round(kms, 3),
+                    "rays": kr, "Mrays_per_s_alone": round(kr / kms / 1e3, 2), "equal_to_batch": ok})
+    for sid in sids:
+        ctx.free_scene(sid)
+    ctx.close()
+    out = {
+        "metric": "Mrays/s path tracing, batched scenes (16 spp, depth 6)",
+        "value": round(rays / ms / 1e3, 2), "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} random icosphere/box/terrain scenes, triangles log-uniform in [1k, 2M], images "
+                               f"uniform over 128^2..1024^2, {spp} spp, depth {depth}, roulette from bounce {rr}, "
+                               f"one wavefront for all scenes", "scenes": n},
+        "batch_ms": round(ms, 2), "rays": rays, "sum_of_single_scene_ms": round(single_ms, 2),
+        "batch_speedup_over_single_scene_renders": round(single_ms / ms, 3),
+        "all_scenes_equal_to_single_renders": same,
+        "autotuned_buckets": {str(k): v for k, v in sorted(tuned.items())},
+        "per_scene": per,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def P_camera(P, scene, side):
+    """The scene's camera at a square image of `side` pixels."""
+    inf = scene.info()
+    scene.set_render(side, side, inf["spp"], inf["max_depth"], inf["rr_depth"])
+    return scene.camera()
 
 
 def equal_rmse_time(points, target):

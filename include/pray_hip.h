@@ -95,10 +95,23 @@ PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
  *              to a per-lane area in HBM
  *   leaf_ilp   triangle records loaded together per leaf step (1 or 2)
  *   fetch_min  a persistent wave refills its finished lanes when at least
- *              this many are idle (1..64)
+ *              this many are idle (1..64): the queue chunk
+ *   occupancy  percent (1..100) of the waves that fit on the device that the
+ *              persistent grid launches: the workgroup count
  * Pass -1 to keep a value. */
-PRAY_API int pray_set_tuning(pray_ctx *ctx, int scene_id, int layout, int stack_lds, int leaf_ilp, int fetch_min);
-PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *layout, int *stack_lds, int *leaf_ilp, int *fetch_min);
+PRAY_API int pray_set_tuning(pray_ctx *ctx, int scene_id, int layout, int stack_lds, int leaf_ilp, int fetch_min,
+                             int occupancy);
+PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *layout, int *stack_lds, int *leaf_ilp, int *fetch_min,
+                             int *occupancy);
+
+/* Autotuner (no reference counterpart): times closest-hit traversal of
+ * n_rays incoherent rays through the scene (0 = 2^20) over stack depth, leaf
+ * ILP, queue chunk and occupancy (coordinate descent from the defaults) and
+ * keeps the fastest setting for the scene and for its size bucket (triangle
+ * counts within a factor of 4).  Later scenes of a tuned bucket take the
+ * bucket's setting without timing unless `force` is set.  tuning5
+ * (optional) receives {layout, stack_lds, leaf_ilp, fetch_min, occupancy}. */
+PRAY_API int pray_autotune(pray_ctx *ctx, int scene_id, uint32_t n_rays, int force, int *tuning5);
 
 /* Closest-hit or any-hit query for n rays (replaces SpatialDatabase::intersect,
  * spatial.h:15, and the shadow tests of pray.cpp:39-44 / :123-133).
@@ -161,6 +174,25 @@ PRAY_API int pray_render_pt(pray_ctx *ctx, int scene_id, const float *camera16, 
 PRAY_API int pray_render_pt_ex(pray_ctx *ctx, int scene_id, const float *camera16, uint32_t width, uint32_t height,
                                uint32_t spp, uint32_t max_depth, uint32_t rr_depth, uint64_t seed,
                                const float *background, float det_eps, float *hdr, uint8_t *rgb, int use_graph);
+
+/* Path tracing of several scenes in one wavefront (no reference
+ * counterpart): every bounce is one persistent closest-hit launch over the
+ * paths of all scenes (rays carry their scene; packed layout) and one
+ * shading launch; the sample pass is captured once into a hipGraph and
+ * replayed spp times.  Scene k renders widths[k] x heights[k] pixels with
+ * camera cameras16[16k..], seed seeds[k] and background backgrounds[3k..];
+ * its image equals pray_render_pt's for the same arguments bit for bit.  The
+ * launches use the autotuned setting of the size bucket holding most pixels
+ * (defaults if none is tuned).  hdr[k] / rgb[k] (arrays of per-scene output
+ * pointers, each may be NULL); gpu_ms (optional) receives the device time of
+ * the spp passes. */
+PRAY_API int pray_render_pt_batch(pray_ctx *ctx, uint32_t n_scenes, const int *scene_ids, const float *cameras16,
+                                  const uint32_t *widths, const uint32_t *heights, uint32_t spp, uint32_t max_depth,
+                                  uint32_t rr_depth, const uint64_t *seeds, const float *backgrounds, float det_eps,
+                                  float *const *hdr, uint8_t *const *rgb, float *gpu_ms);
+/* Rays traced (all bounces, all samples) by the last pray_render_pt* or
+ * pray_render_pt_batch call of the context. */
+PRAY_API int pray_last_render_rays(pray_ctx *ctx, uint64_t *rays);
 
 /* Learned radiance cache (no reference counterpart; the reference's path
  * tracer, path_trace.cu:369-464, always traces paths to the end).

@@ -70,8 +70,12 @@ def libraries():
         "pray_upload_scene": (C.c_int, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32,
                                         C.POINTER(C.c_int)]),
         "pray_free_scene": (C.c_int, [_vp, C.c_int]),
-        "pray_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
-        "pray_get_tuning": (C.c_int, [_vp, C.c_int] + [C.POINTER(C.c_int)] * 4),
+        "pray_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "pray_get_tuning": (C.c_int, [_vp, C.c_int] + [C.POINTER(C.c_int)] * 5),
+        "pray_autotune": (C.c_int, [_vp, C.c_int, _u32, C.c_int, _vp]),
+        "pray_render_pt_batch": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp, _f32, _vp, _vp,
+                                           C.POINTER(_f32)]),
+        "pray_last_render_rays": (C.c_int, [_vp, C.POINTER(_u64)]),
         "pray_trace": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp]),
         "pray_trace_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp]),
         "pray_trace_counts_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp, _vp]),
@@ -379,13 +383,42 @@ class Context:
         self._check(self._lib.pray_event_elapsed(self._h, slot0, slot1, C.byref(ms)))
         return float(ms.value)
 
-    def set_tuning(self, sid, layout=-1, stack_lds=-1, leaf_ilp=-1, fetch_min=-1):
-        self._check(self._lib.pray_set_tuning(self._h, sid, layout, stack_lds, leaf_ilp, fetch_min))
+    def set_tuning(self, sid, layout=-1, stack_lds=-1, leaf_ilp=-1, fetch_min=-1, occupancy=-1):
+        self._check(self._lib.pray_set_tuning(self._h, sid, layout, stack_lds, leaf_ilp, fetch_min, occupancy))
 
     def get_tuning(self, sid):
-        v = [C.c_int() for _ in range(4)]
+        v = [C.c_int() for _ in range(5)]
         self._check(self._lib.pray_get_tuning(self._h, sid, *[C.byref(x) for x in v]))
-        return {"layout": v[0].value, "stack_lds": v[1].value, "leaf_ilp": v[2].value, "fetch_min": v[3].value}
+        return {"layout": v[0].value, "stack_lds": v[1].value, "leaf_ilp": v[2].value, "fetch_min": v[3].value,
+                "occupancy": v[4].value}
+
+    def autotune(self, sid, n_rays=0, force=False):
+        """Autotune the scene's traversal (or take its size bucket's setting)."""
+        out = np.zeros(5, np.int32)
+        self._check(self._lib.pray_autotune(self._h, sid, n_rays, 1 if force else 0, _p(out)))
+        return dict(zip(("layout", "stack_lds", "leaf_ilp", "fetch_min", "occupancy"), (int(x) for x in out)))
+
+    def last_render_rays(self):
+        r = _u64()
+        self._check(self._lib.pray_last_render_rays(self._h, C.byref(r)))
+        return r.value
+
+    def render_pt_batch(self, sids, cams, sizes, spp, max_depth, rr_depth, seeds, backgrounds, det_eps):
+        """Path traces several scenes in one wavefront; returns (list of HDR
+        images, device milliseconds)."""
+        n = len(sids)
+        ids = np.ascontiguousarray(sids, np.int32)
+        cams = np.ascontiguousarray(np.stack([np.asarray(c, np.float32) for c in cams]), np.float32)
+        ws = np.ascontiguousarray([w for w, _ in sizes], np.uint32)
+        hs = np.ascontiguousarray([h for _, h in sizes], np.uint32)
+        sd = np.ascontiguousarray(seeds, np.uint64)
+        bgs = np.ascontiguousarray(np.stack([np.asarray(b, np.float32) for b in backgrounds]), np.float32)
+        outs = [np.empty((h, w, 3), np.float32) for w, h in sizes]
+        ptrs = (_vp * n)(*[o.ctypes.data for o in outs])
+        ms = _f32()
+        self._check(self._lib.pray_render_pt_batch(self._h, n, _p(ids), _p(cams), _p(ws), _p(hs), spp, max_depth,
+                                                   rr_depth, _p(sd), _p(bgs), det_eps, ptrs, None, C.byref(ms)))
+        return outs, ms.value
 
     def trace(self, sid, rays, mode=TRACE_CLOSEST, det_eps=REFERENCE_DET_EPS, limit2=None):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)

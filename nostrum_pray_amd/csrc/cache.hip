@@ -446,7 +446,7 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
             hipLaunchKernelGGL(prr::k_pt_shade<prr::PT_CACHE>, dim3(nblocks(npix, 256)), dim3(256), 0, st, sc->d, P,
                                C, b, (const float *)rq[cur].p, (const float4 *)sq[cur].p,
                                (const uint32_t *)(count + cur), (const int32_t *)hit.p, (const float *)tt.p,
-                               (float *)rq[nxt].p, (float4 *)sq[nxt].p, count + nxt, (float *)contrib.p);
+                               (float *)rq[nxt].p, (float4 *)sq[nxt].p, count + nxt, (float *)contrib.p, prr::BatchArgs{});
         }
         int e = enqueue_trace(c, *sc, (const float *)trays.p, n_train, r->ctr, PRAY_TRACE_CLOSEST, det_eps, nullptr,
                               (int32_t *)thit.p, (float *)ttt.p, nullptr, st);

@@ -19,6 +19,7 @@ struct Tuning {
     int stack_lds = 8;
     int leaf_ilp = 2;
     int fetch_min = 32;
+    int occupancy = 100;  // percent of the resident persistent waves launched
 };
 
 struct SceneRec {
@@ -40,11 +41,14 @@ struct pray_ctx {
     std::map<int, std::unique_ptr<prc::SceneRec>> scenes;
     std::map<int, prc::CacheRec *> caches;
     int next_id = 1;
-    hipEvent_t events[16] = {};
+    hipEvent_t events[16] = {};   // pray_event_record slots
+    hipEvent_t ievents[4] = {};   // internal timing (autotuner, batch)
+    uint64_t last_rays = 0;       // rays traced by the last render call
     uint32_t *work = nullptr;   // ray counters of the persistent kernels
     uint4 *spill = nullptr;     // per-lane stack spill area
     size_t spill_bytes = 0;
     std::map<const void *, int> occupancy;  // kernel -> resident blocks per CU
+    std::map<int, prc::Tuning> tuned;        // scene-size bucket -> autotuned tuning
 };
 
 namespace prc {
@@ -79,5 +83,13 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
 // Makes sure the closest-hit kernel of a scene has its stack spill area for
 // launches of up to n rays (needed before a graph capture).
 int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n);
+// Closest hits of rays from several scenes (packed layout): ray i belongs to
+// table[ray_scene[i]]; max_depth is the deepest tree of the table.
+int enqueue_trace_multi(pray_ctx *c, const prt::DevScene *table, const uint32_t *ray_scene, uint32_t max_depth,
+                        const Tuning &tu, const float *rays, uint64_t n, const uint32_t *n_dev, float eps,
+                        int32_t *idx, float *t, hipStream_t st);
+int prepare_trace_multi(pray_ctx *c, const Tuning &tu, uint32_t max_depth, uint64_t n);
+// Scene-size bucket of the autotuner (factors of 4 in triangle count).
+int size_bucket(uint32_t n_tris);
 
 }  // namespace prc

@@ -108,6 +108,12 @@ TraceKernel pick_kernel_kl(int K, int ilp) {
     return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 8, 2> : prt::k_trace<ANY, LAYOUT, 8, 1>;
 }
 
+TraceKernel pick_kernel_multi(int K, int ilp) {
+    if (K == 4) return ilp == 2 ? prt::k_trace<false, 1, 4, 2, true> : prt::k_trace<false, 1, 4, 1, true>;
+    if (K == 16) return ilp == 2 ? prt::k_trace<false, 1, 16, 2, true> : prt::k_trace<false, 1, 16, 1, true>;
+    return ilp == 2 ? prt::k_trace<false, 1, 8, 2, true> : prt::k_trace<false, 1, 8, 1, true>;
+}
+
 TraceKernel pick_kernel(bool any, int layout, int K, int ilp) {
     if (any)
         return layout == PRAY_LAYOUT_PACKED ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED>(K, ilp)
@@ -124,6 +130,15 @@ int resident_blocks(pray_ctx *c, TraceKernel k) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, WAVE, 0) != hipSuccess || nb <= 0) nb = 8;
     c->occupancy[(const void *)k] = nb;
     return nb;
+}
+
+// Persistent grid: one wave per 64 rays, at most the tuned share of the
+// waves that fit on the device at once.
+unsigned grid_size(pray_ctx *c, TraceKernel k, const Tuning &tu, uint64_t n) {
+    const uint64_t want = (n + WAVE - 1) / WAVE;
+    const uint64_t per_cu = std::max<uint64_t>(1, (uint64_t)resident_blocks(c, k) * (uint64_t)tu.occupancy / 100);
+    const uint64_t fit = (uint64_t)c->num_cus * per_cu;
+    return (unsigned)std::max<uint64_t>(1, std::min(want, fit));
 }
 
 int ensure_spill(pray_ctx *c, size_t bytes) {
@@ -156,13 +171,11 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
     }
     const Tuning &tu = sc.tune;
     TraceKernel k = pick_kernel(mode != PRAY_TRACE_CLOSEST, tu.layout, tu.stack_lds, tu.leaf_ilp);
-    const uint64_t want = (n + WAVE - 1) / WAVE;
-    const uint64_t fit = (uint64_t)c->num_cus * (uint64_t)resident_blocks(c, k);
-    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min(want, fit));
+    const unsigned grid = grid_size(c, k, tu, n);
     const uint32_t depth = std::max<uint32_t>(1, sc.d.max_depth);
     int rc = ensure_spill(c, (size_t)grid * WAVE * depth * sizeof(uint4));
     if (rc) return rc;
-    TraceArgs a;
+    TraceArgs a{};
     a.rays = rays;
     a.n = (uint32_t)n;
     a.n_dev = n_dev;
@@ -180,6 +193,46 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
     hipLaunchKernelGGL(k, dim3(grid), dim3(WAVE), 0, st, sc.d, a);
     HIPC(c, hipGetLastError());
     return PRAY_OK;
+}
+
+int enqueue_trace_multi(pray_ctx *c, const prt::DevScene *table, const uint32_t *ray_scene, uint32_t max_depth,
+                        const Tuning &tu, const float *rays, uint64_t n, const uint32_t *n_dev, float eps,
+                        int32_t *idx, float *t, hipStream_t st) {
+    if (n == 0) return PRAY_OK;
+    if (n > 0x7FFFFFFFull) return fail(c, PRAY_ERR_INVALID, "at most 2^31-1 rays per call");
+    TraceKernel k = pick_kernel_multi(tu.stack_lds, tu.leaf_ilp);
+    const unsigned grid = grid_size(c, k, tu, n);
+    const uint32_t depth = std::max<uint32_t>(1, max_depth);
+    int rc = ensure_spill(c, (size_t)grid * WAVE * depth * sizeof(uint4));
+    if (rc) return rc;
+    TraceArgs a{};
+    a.rays = rays;
+    a.n = (uint32_t)n;
+    a.n_dev = n_dev;
+    a.eps = eps;
+    a.out_idx = idx;
+    a.out_t = t;
+    a.work = c->work;
+    a.spill = c->spill;
+    a.spill_depth = depth;
+    a.fetch_min = (uint32_t)tu.fetch_min;
+    a.table = table;
+    a.ray_scene = ray_scene;
+    hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, c->work, grid * (uint32_t)WAVE);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(WAVE), 0, st, prt::DevScene{}, a);
+    HIPC(c, hipGetLastError());
+    return PRAY_OK;
+}
+
+int prepare_trace_multi(pray_ctx *c, const Tuning &tu, uint32_t max_depth, uint64_t n) {
+    TraceKernel k = pick_kernel_multi(tu.stack_lds, tu.leaf_ilp);
+    return ensure_spill(c, (size_t)grid_size(c, k, tu, n) * WAVE * std::max<uint32_t>(1, max_depth) * sizeof(uint4));
+}
+
+int size_bucket(uint32_t n_tris) {
+    int b = 0;
+    while (b < 15 && (n_tris >> (2 * b + 2)) >= 1u) ++b;  // 4^b <= n_tris < 4^(b+1)
+    return b;
 }
 
 SceneRec *find_scene(pray_ctx *c, int id) {
@@ -238,10 +291,8 @@ int validate_tree(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const ui
 
 int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n) {
     TraceKernel k = pick_kernel(false, sc.tune.layout, sc.tune.stack_lds, sc.tune.leaf_ilp);
-    const uint64_t want = (n + WAVE - 1) / WAVE;
-    const uint64_t fit = (uint64_t)c->num_cus * (uint64_t)resident_blocks(c, k);
-    return ensure_spill(c, (size_t)std::max<uint64_t>(1, std::min(want, fit)) * WAVE *
-                               std::max<uint32_t>(1, sc.d.max_depth) * sizeof(uint4));
+    return ensure_spill(c, (size_t)grid_size(c, k, sc.tune, n) * WAVE * std::max<uint32_t>(1, sc.d.max_depth) *
+                               sizeof(uint4));
 }
 
 }  // namespace prc
@@ -283,6 +334,8 @@ void pray_destroy(pray_ctx *c) {
     for (auto &kv : c->caches) free_cache_rec(kv.second);
     c->caches.clear();
     for (hipEvent_t &e : c->events)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t &e : c->ievents)
         if (e) (void)hipEventDestroy(e);
     if (c->work) (void)hipFree(c->work);
     if (c->spill) (void)hipFree(c->spill);
@@ -389,7 +442,7 @@ int pray_free_scene(pray_ctx *c, int scene_id) {
     return PRAY_OK;
 }
 
-int pray_set_tuning(pray_ctx *c, int scene_id, int layout, int stack_lds, int leaf_ilp, int fetch_min) {
+int pray_set_tuning(pray_ctx *c, int scene_id, int layout, int stack_lds, int leaf_ilp, int fetch_min, int occupancy) {
     if (!c) return PRAY_ERR_INVALID;
     SceneRec *sc = find_scene(c, scene_id);
     if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
@@ -400,6 +453,9 @@ int pray_set_tuning(pray_ctx *c, int scene_id, int layout, int stack_lds, int le
     if (leaf_ilp >= 0 && leaf_ilp != 1 && leaf_ilp != 2) return fail(c, PRAY_ERR_INVALID, "leaf_ilp must be 1 or 2");
     if (fetch_min >= 0 && (fetch_min < 1 || fetch_min > WAVE))
         return fail(c, PRAY_ERR_INVALID, "fetch_min must be in 1..64");
+    if (occupancy >= 0 && (occupancy < 1 || occupancy > 100))
+        return fail(c, PRAY_ERR_INVALID, "occupancy must be in 1..100");
+    if (occupancy >= 0) sc->tune.occupancy = occupancy;
     if (layout >= 0) sc->tune.layout = layout;
     if (stack_lds >= 0) sc->tune.stack_lds = stack_lds;
     if (leaf_ilp >= 0) sc->tune.leaf_ilp = leaf_ilp;
@@ -407,7 +463,8 @@ int pray_set_tuning(pray_ctx *c, int scene_id, int layout, int stack_lds, int le
     return PRAY_OK;
 }
 
-int pray_get_tuning(pray_ctx *c, int scene_id, int *layout, int *stack_lds, int *leaf_ilp, int *fetch_min) {
+int pray_get_tuning(pray_ctx *c, int scene_id, int *layout, int *stack_lds, int *leaf_ilp, int *fetch_min,
+                    int *occupancy) {
     if (!c) return PRAY_ERR_INVALID;
     SceneRec *sc = find_scene(c, scene_id);
     if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
@@ -415,6 +472,7 @@ int pray_get_tuning(pray_ctx *c, int scene_id, int *layout, int *stack_lds, int 
     if (stack_lds) *stack_lds = sc->tune.stack_lds;
     if (leaf_ilp) *leaf_ilp = sc->tune.leaf_ilp;
     if (fetch_min) *fetch_min = sc->tune.fetch_min;
+    if (occupancy) *occupancy = sc->tune.occupancy;
     return PRAY_OK;
 }
 
@@ -611,7 +669,7 @@ int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w,
         HIPC(c, hipMalloc(&rq[q].p, npix * 24));
         HIPC(c, hipMalloc(&sq[q].p, npix * 16));
     }
-    HIPC(c, hipMalloc(&cnt.p, 4 * sizeof(uint32_t)));  // count0, count1, sample
+    HIPC(c, hipMalloc(&cnt.p, 6 * sizeof(uint32_t)));  // count0, count1, sample, pad, traced rays (u64)
     HIPC(c, hipMalloc(&hit.p, npix * 4));
     HIPC(c, hipMalloc(&tt.p, npix * 4));
     HIPC(c, hipMalloc(&contrib.p, npix * 12));
@@ -623,8 +681,9 @@ int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w,
     hipLaunchKernelGGL(prr::k_camera, dim3(nblocks(npix, 256)), dim3(256), 0, st, cam, w, h, (float *)prim.p);
     hipLaunchKernelGGL(prr::k_fill_f32, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (float *)sum.p,
                        (uint32_t)(3 * npix), 0.f);
-    hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count, 4u);
+    hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count, 6u);
     HIPC(c, hipGetLastError());
+    unsigned long long *traced = (unsigned long long *)(count + 4);
     prr::PtArgs P;
     P.n_pix = (uint32_t)npix;
     P.max_depth = max_depth;
@@ -645,6 +704,7 @@ int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w,
         for (uint32_t b = 0; b <= max_depth; b++) {
             const int cur = b & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + nxt, 1u);
+            hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, traced, (const uint32_t *)(count + cur));
             if (sc->d.n_tris) {
                 int rc = enqueue_trace(c, *sc, (const float *)rq[cur].p, npix, count + cur, PRAY_TRACE_CLOSEST,
                                        det_eps, nullptr, (int32_t *)hit.p, (float *)tt.p, nullptr, st);
@@ -656,7 +716,7 @@ int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w,
             hipLaunchKernelGGL(prr::k_pt_shade<prr::PT_PLAIN>, dim3(nblocks(npix, 256)), dim3(256), 0, st, sc->d, P,
                                prr::CacheArgs{}, b, (const float *)rq[cur].p, (const float4 *)sq[cur].p, (const uint32_t *)(count + cur),
                                (const int32_t *)hit.p, (const float *)tt.p, (float *)rq[nxt].p, (float4 *)sq[nxt].p,
-                               count + nxt, (float *)contrib.p);
+                               count + nxt, (float *)contrib.p, prr::BatchArgs{});
         }
         hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
                            (const float *)contrib.p, (float *)sum.p, count + 2);
@@ -696,6 +756,7 @@ int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w,
     HIPC(c, hipGetLastError());
     if (hdr) HIPC(c, hipMemcpyAsync(hdr, outh.p, npix * 12, hipMemcpyDeviceToHost, st));
     if (rgb) HIPC(c, hipMemcpyAsync(rgb, outb.p, npix * 3, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipMemcpyAsync(&c->last_rays, traced, 8, hipMemcpyDeviceToHost, st));
     HIPC(c, hipStreamSynchronize(st));
     return PRAY_OK;
 }

@@ -339,12 +339,28 @@ __device__ __forceinline__ uint32_t wave_slot(bool want, uint32_t *counter) {
     return base;
 }
 
-template <int MODE>
+// Batched rendering of several scenes in one wavefront: queue entry i
+// belongs to scene scene_in[i]; pixel indices in the state are global
+// (scene's offset + pixel), random numbers use the scene's own key and pixel
+// index, so every scene's image equals its single-scene render.
+struct BatchScene {
+    float bg[3];
+    uint32_t k0, k1, pix_off, n_pix, pad;
+};
+
+struct BatchArgs {
+    const DevScene *table;
+    const BatchScene *scenes;
+    const uint32_t *scene_in;
+    uint32_t *scene_out;
+};
+
+template <int MODE, bool MULTI = false>
 static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, CacheArgs C, uint32_t bounce,
                                                          const float *rays_in, const float4 *state_in,
                                                          const uint32_t *count_in, const int32_t *hit,
                                                          const float *tt, float *rays_out, float4 *state_out,
-                                                         uint32_t *count_out, float *contrib) {
+                                                         uint32_t *count_out, float *contrib, BatchArgs B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = *count_in;
     const bool active = i < n;
@@ -352,24 +368,35 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, tr = 0, tg = 0, tb = 0;
     float tdx = 0, tdy = 0, tdz = 0;
     float feat[10];
-    uint32_t pix = 0, tk = 0;
+    uint32_t pix = 0, tk = 0, scn = 0;
     if (active) {
         const float4 st = state_in[i];
         pix = __float_as_uint(st.w);
+        const uint8_t *emit = s.emit;
+        const float *color = s.color, *normal = s.normal;
+        float bg0 = P.bg[0], bg1 = P.bg[1], bg2 = P.bg[2];
+        uint32_t k0 = P.k0, k1 = P.k1, lpix = pix;
+        if (MULTI) {
+            scn = B.scene_in[i];
+            const DevScene &S = B.table[scn];
+            const BatchScene &Q = B.scenes[scn];
+            emit = S.emit, color = S.color, normal = S.normal;
+            bg0 = Q.bg[0], bg1 = Q.bg[1], bg2 = Q.bg[2];
+            k0 = Q.k0, k1 = Q.k1, lpix = pix - Q.pix_off;
+        }
         const int32_t h = hit[i];
         float cr = 0.f, cg = 0.f, cb = 0.f;
         bool term = true;
         if (h < 0) {
-            cr = st.x * P.bg[0], cg = st.y * P.bg[1], cb = st.z * P.bg[2];
-        } else if (s.emit[h]) {
-            cr = st.x * s.color[3 * (size_t)h], cg = st.y * s.color[3 * (size_t)h + 1],
-            cb = st.z * s.color[3 * (size_t)h + 2];
+            cr = st.x * bg0, cg = st.y * bg1, cb = st.z * bg2;
+        } else if (emit[h]) {
+            cr = st.x * color[3 * (size_t)h], cg = st.y * color[3 * (size_t)h + 1], cb = st.z * color[3 * (size_t)h + 2];
         } else if (bounce < P.max_depth) {
             const float *r = rays_in + 6 * (size_t)i;
             const float t = tt[i];
             ox = r[0] + r[3] * t, oy = r[1] + r[4] * t, oz = r[2] + r[5] * t;
-            const float ar = s.color[3 * (size_t)h], ag = s.color[3 * (size_t)h + 1], ab = s.color[3 * (size_t)h + 2];
-            const float nx = s.normal[3 * (size_t)h], ny = s.normal[3 * (size_t)h + 1], nz = s.normal[3 * (size_t)h + 2];
+            const float ar = color[3 * (size_t)h], ag = color[3 * (size_t)h + 1], ab = color[3 * (size_t)h + 2];
+            const float nx = normal[3 * (size_t)h], ny = normal[3 * (size_t)h + 1], nz = normal[3 * (size_t)h + 2];
             const bool at_cache = MODE == PT_CACHE && bounce == C.cache_bounce;
             if (at_cache) {
                 const uint32_t d = (pix + P.n_pix - train_offset(*P.sample, P.n_pix, P.k0, P.k1)) % P.n_pix;
@@ -387,7 +414,7 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
                 query = true;
                 term = false;
             } else {
-                const uint4 u = prd::philox4x32_10(make_uint4(pix, *P.sample, bounce, 0u), P.k0, P.k1);
+                const uint4 u = prd::philox4x32_10(make_uint4(lpix, *P.sample, bounce, 0u), k0, k1);
                 prd::sample_hemisphere(nx, ny, nz, prd::u01(u.x), prd::u01(u.y), dx, dy, dz);
                 tr = st.x * ar, tg = st.y * ag, tb = st.z * ab;
                 bool live = true;
@@ -431,7 +458,31 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
         float *o = rays_out + 6 * (size_t)slot;
         o[0] = ox, o[1] = oy, o[2] = oz, o[3] = dx, o[4] = dy, o[5] = dz;
         state_out[slot] = make_float4(tr, tg, tb, __uint_as_float(pix));
+        if (MULTI) B.scene_out[slot] = scn;
     }
+}
+
+// Camera rays of all scenes of a batch, from their concatenated primary rays:
+// entry g belongs to the scene whose pixel range holds g.
+static __global__ void k_pt_generate_batch(const float *prim, uint32_t n_total, const BatchScene *scenes,
+                                           uint32_t n_scenes, float *rays, float4 *state, uint32_t *scene,
+                                           uint32_t *count) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) *count = n_total;
+    if (g >= n_total) return;
+    uint32_t lo = 0, hi = n_scenes - 1;  // last scene with pix_off <= g
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (scenes[mid].pix_off <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const float *r = prim + 6 * (size_t)g;
+    float *o = rays + 6 * (size_t)g;
+    o[0] = r[0], o[1] = r[1], o[2] = r[2], o[3] = r[3], o[4] = r[4], o[5] = r[5];
+    state[g] = make_float4(1.f, 1.f, 1.f, __uint_as_float(g));
+    scene[g] = lo;
 }
 
 // Targets of the training samples from their training rays' hits: emission
@@ -478,6 +529,12 @@ static __global__ void k_pt_accumulate(uint32_t n_pix, const float *contrib, flo
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 3 * n_pix) sum[i] = sum[i] + contrib[i];
     if (i == 0) *sample = *sample + 1;
+}
+
+// Adds a queue count to a traced-ray counter (inside graphs: a kernel, not
+// a memset or host call).
+static __global__ void k_acc_rays(unsigned long long *acc, const uint32_t *count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *acc += *count;
 }
 
 static __global__ void k_pt_finalize(uint32_t n_pix, const float *sum, float spp, float *hdr, uint8_t *rgb) {

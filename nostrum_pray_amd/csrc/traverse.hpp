@@ -60,6 +60,8 @@ struct TraceArgs {
     uint4 *spill;             // per lane slot: spill_depth entries
     uint32_t spill_depth;
     uint32_t fetch_min;       // refill when at least this many lanes are idle
+    const DevScene *table;    // multi-scene launches: scene of ray i is
+    const uint32_t *ray_scene;  // table[ray_scene[i]]
 };
 
 enum : uint32_t { ST_IDLE = 0, ST_NODE = 1, ST_LEAF = 2, ST_DEAD = 3 };
@@ -80,12 +82,12 @@ __device__ __forceinline__ void slab(float bmin, float bmax, float o, float inv,
 }
 
 template <int LAYOUT>
-__device__ __forceinline__ void load_tri(const DevScene &s, uint32_t k, uint32_t &ref, float4 &A, float4 &B,
-                                         float4 &Cc) {
+__device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, uint32_t k, uint32_t &ref, float4 &A,
+                                         float4 &B, float4 &Cc) {
     if (LAYOUT == 1) {
-        A = s.recs[3 * (size_t)k];
-        B = s.recs[3 * (size_t)k + 1];
-        Cc = s.recs[3 * (size_t)k + 2];
+        A = recs[3 * (size_t)k];
+        B = recs[3 * (size_t)k + 1];
+        Cc = recs[3 * (size_t)k + 2];
         ref = __float_as_uint(A.w);
     } else {
         ref = s.refs[k];
@@ -96,9 +98,13 @@ __device__ __forceinline__ void load_tri(const DevScene &s, uint32_t k, uint32_t
     }
 }
 
-template <bool ANY, int LAYOUT, int K, int ILP>
+// MULTI: rays of several scenes in one launch (packed layout only); each lane
+// takes its ray's node and record arrays and root cell from the scene table
+// when it picks the ray up.
+template <bool ANY, int LAYOUT, int K, int ILP, bool MULTI = false>
 static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) {
     static_assert((K & (K - 1)) == 0, "K must be a power of two");
+    static_assert(!MULTI || LAYOUT == 1, "multi-scene launches use the packed layout");
     __shared__ uint32_t st_w[K * WAVE];
     __shared__ float st_lo[K * WAVE];
     __shared__ float st_hi[K * WAVE];
@@ -106,8 +112,8 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
     const uint64_t below = (1ull << lane) - 1ull;
     uint4 *spill = A.spill + (size_t)(blockIdx.x * WAVE + lane) * A.spill_depth;
     const uint32_t nn = A.n_dev ? *A.n_dev : A.n;
-    const uint2 *nodes = s.nodes;
     const uint4 *pairs = reinterpret_cast<const uint4 *>(s.nodes);
+    const float4 *recs = s.recs;
 
     uint32_t state = ST_IDLE, ray = 0, w = 0, k = 0, best_i = NO_HIT, cn = 0, cr = 0;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, lim2 = 0;
@@ -146,9 +152,18 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
                     ox = r0.x, oy = r0.y, oz = r1.x, dx = r1.y, dy = r2.x, dz = r2.y;
                     ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
                     lo = 0.f, hi = INFINITY;
-                    slab(s.box[0], s.box[3], ox, ix, lo, hi);
-                    slab(s.box[1], s.box[4], oy, iy, lo, hi);
-                    slab(s.box[2], s.box[5], oz, iz, lo, hi);
+                    if (MULTI) {
+                        const DevScene &S = A.table[A.ray_scene[ray]];
+                        pairs = reinterpret_cast<const uint4 *>(S.nodes);
+                        recs = S.recs;
+                        slab(S.box[0], S.box[3], ox, ix, lo, hi);
+                        slab(S.box[1], S.box[4], oy, iy, lo, hi);
+                        slab(S.box[2], S.box[5], oz, iz, lo, hi);
+                    } else {
+                        slab(s.box[0], s.box[3], ox, ix, lo, hi);
+                        slab(s.box[1], s.box[4], oy, iy, lo, hi);
+                        slab(s.box[2], s.box[5], oz, iz, lo, hi);
+                    }
                     best_t = INFINITY, best_i = NO_HIT, sp = 0, base = 0, cn = 0, cr = 0;
                     bool live = true;
                     if (ANY) {
@@ -160,7 +175,7 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
                     }
                     if (live && lo <= hi) {
                         cn = 1;
-                        w = nodes[1].y;
+                        w = pairs[0].w;  // node 1 = second word pair of the first 16 bytes
                         if ((w & 3u) != LEAF_TAG) {
                             state = ST_NODE;
                         } else if (!empty_leaf(w)) {
@@ -234,7 +249,7 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
             uint32_t ref[ILP];
             float4 ta[ILP], tb[ILP], tc[ILP];
 #pragma unroll
-            for (int j = 0; j < ILP; ++j) load_tri<LAYOUT>(s, k + j, ref[j], ta[j], tb[j], tc[j]);
+            for (int j = 0; j < ILP; ++j) load_tri<LAYOUT>(s, recs, k + j, ref[j], ta[j], tb[j], tc[j]);
             bool leaf_done = false, found = false;
 #pragma unroll
             for (int j = 0; j < ILP; ++j) {

@@ -1,0 +1,75 @@
+"""Batched multi-scene path tracing and the autotuner on the GPU: every scene
+of a batch renders bit-identically to its single-scene render, and no tuning
+changes a traced result."""
+import numpy as np
+import pytest
+
+import nostrum_pray_amd as P
+import oracle as O
+from conftest import first_diff, say
+
+pytestmark = pytest.mark.gpu
+
+
+def scene_set():
+    """Random icosphere/box/terrain mixes of different sizes and cameras."""
+    out = []
+    for seed, tris, (w, h) in ((11, 3000, (64, 48)), (12, 20000, (96, 64)), (13, 1500, (32, 32)),
+                               (14, 60000, (80, 80)), (15, 9000, (48, 96))):
+        s = P.Scene.random(seed, tris)
+        out.append((s, s.build(1), O.camera([0, 4, 9], [0, -0.45, -1], 60, w, h), (w, h), 1000 + seed))
+    return out
+
+
+def test_batch_equals_single_scene_renders(ctx):
+    scenes = scene_set()
+    sids = [ctx.upload(t) for _, t, _, _, _ in scenes]
+    try:
+        say("batch of 5 scenes, 4 spp")
+        imgs, ms = ctx.render_pt_batch(sids, [c for _, _, c, _, _ in scenes], [wh for _, _, _, wh, _ in scenes],
+                                       4, 6, 3, [sd for *_, sd in scenes],
+                                       [s.info()["background"] for s, *_ in scenes], 1e-9)
+        batch_rays = ctx.last_render_rays()
+        assert ms > 0
+        single_rays = 0
+        for k, (s, _, cam, (w, h), seed) in enumerate(scenes):
+            say(f"single scene {k}")
+            hdr, _ = ctx.render_pt(sids[k], cam, w, h, 4, 6, 3, seed, s.info()["background"], 1e-9, want_rgb=False)
+            single_rays += ctx.last_render_rays()
+            assert np.array_equal(imgs[k].view(np.uint32), hdr.view(np.uint32)), (k, first_diff(imgs[k].ravel(),
+                                                                                              hdr.ravel()))
+            assert hdr.mean() > 0
+        assert batch_rays == single_rays
+    finally:
+        for sid in sids:
+            ctx.free_scene(sid)
+
+
+def test_autotune_never_changes_hits(ctx):
+    s = P.Scene.random(21, 40000)
+    t = s.build(1)
+    sid = ctx.upload(t)
+    try:
+        rays = O.gen_rays(5, P.scene_box_expanded(t.box()), 1 << 16)
+        say("autotune: reference trace")
+        ref_idx, ref_t = ctx.trace(sid, rays, P.TRACE_CLOSEST, 1e-9)
+        say("autotune: tuning")
+        tun = ctx.autotune(sid, n_rays=1 << 18, force=True)
+        assert tun["stack_lds"] in (4, 8, 16) and tun["leaf_ilp"] in (1, 2) and 1 <= tun["occupancy"] <= 100
+        assert ctx.get_tuning(sid) == tun
+        settings = [dict(), dict(stack_lds=4), dict(stack_lds=16, leaf_ilp=1), dict(fetch_min=16, occupancy=50),
+                    dict(fetch_min=64, occupancy=75), dict(layout=P.LAYOUT_INDEXED)]
+        for st in settings:
+            say(f"autotune: trace with {st}")
+            ctx.set_tuning(sid, **st)
+            idx, tt = ctx.trace(sid, rays, P.TRACE_CLOSEST, 1e-9)
+            assert np.array_equal(idx, ref_idx) and np.array_equal(tt.view(np.uint32), ref_t.view(np.uint32)), st
+        # a second scene of the same size bucket takes the bucket's setting
+        s2 = P.Scene.random(22, 45000)
+        sid2 = ctx.upload(s2.build(1))
+        try:
+            assert ctx.autotune(sid2) == tun
+        finally:
+            ctx.free_scene(sid2)
+    finally:
+        ctx.free_scene(sid)
