@@ -12,7 +12,8 @@
 //
 // Interval arithmetic: planes are intersected front to back with the
 // direction's sign choosing near/far; every plane distance is widened by a
-// relative 1e-6 (conservative: a cell is never skipped by rounding), and a
+// relative 1e-6 (conservative: a cell is never skipped by rounding; infinite
+// distances stay infinite), and a
 // pending cell is pruned only when its widened entry exceeds the best t
 // (strictly), so equal-t ties are always examined.  The closest hit is the
 // lexicographic minimum of (t, triangle index) over t > 0, which equals the
@@ -66,8 +67,12 @@ struct TraceArgs {
 
 enum : uint32_t { ST_IDLE = 0, ST_NODE = 1, ST_LEAF = 2, ST_DEAD = 3 };
 
-__device__ __forceinline__ float whi(float x) { return x + fabsf(x) * WIDEN; }
-__device__ __forceinline__ float wlo(float x) { return x - fabsf(x) * WIDEN; }
+// Widening by a relative WIDEN, multiplicatively so that +-infinity (a zero
+// direction component) stays infinite: x +- |x| * WIDEN would turn +inf into
+// inf - inf = NaN, which fminf/fmaxf ignore, and such a ray would lose all
+// pruning along that axis and walk most of the tree.
+__device__ __forceinline__ float whi(float x) { return x * (x > 0.f ? 1.f + WIDEN : 1.f - WIDEN); }
+__device__ __forceinline__ float wlo(float x) { return x * (x > 0.f ? 1.f - WIDEN : 1.f + WIDEN); }
 __device__ __forceinline__ bool empty_leaf(uint32_t w) { return w == ((LEAF_EMPTY << 2) | LEAF_TAG); }
 
 __device__ __forceinline__ void slab(float bmin, float bmax, float o, float inv, float &lo, float &hi) {

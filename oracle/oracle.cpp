@@ -216,8 +216,9 @@ struct Flat {
     const float *box;
 };
 
-inline float whi(float x) { return x + std::fabs(x) * 1e-6f; }
-inline float wlo(float x) { return x - std::fabs(x) * 1e-6f; }
+// Multiplicative widening: +-inf stays infinite (device twin).
+inline float whi(float x) { return x * (x > 0.f ? 1.f + 1e-6f : 1.f - 1e-6f); }
+inline float wlo(float x) { return x * (x > 0.f ? 1.f - 1e-6f : 1.f + 1e-6f); }
 
 inline void slabW(float bmin, float bmax, float o, float inv, float &lo, float &hi) {
     float t0 = (bmin - o) * inv, t1 = (bmax - o) * inv;

@@ -195,3 +195,22 @@ def test_scene_load_errors_are_reported(tmp_path):
     bad.write_text("{ not json")
     with pytest.raises(ValueError):
         P.Scene.load_json(str(bad))
+
+
+def test_axis_parallel_rays_keep_pruning():
+    """A direction with zero components gives infinite plane distances; the
+    widened interval must stay infinite (not NaN), or the ray loses all
+    pruning along that axis and walks the whole tree.  Regression: a
+    first-bounce path-tracing ray leaving the back wall along +z visited
+    every node of the Cornell tree."""
+    s = P.Scene.cornell(1, 100_000)
+    f = s.build(1, max_depth=40, leaf_size=8, traversal_cost=2.0).flat()
+    n_nodes = len(f["nodes"]) // 2
+    rays = np.array([[0.3668138, 1.8581737, -1.0000007, 0.0, 0.0, 1.0],
+                     [0.2, 0.7, 0.3, 0.0, 1.0, 0.0],
+                     [0.2, 0.7, 0.3, -1.0, 0.0, 0.0],
+                     [0.0, 1.0, 0.0, 0.0, -0.0, -1.0]], np.float32)
+    idx, t, cnt = O.flat_trace(f, rays, 0, 1e-9, counts=True)
+    bf_idx, bf_t = O.bf_closest(f["a"], f["ab"], f["ac"], rays, 1e-9)
+    assert np.array_equal(idx, bf_idx) and np.array_equal(t.view(np.uint32), bf_t.view(np.uint32))
+    assert cnt[:, 0].max() < 0.02 * n_nodes, cnt
