@@ -55,6 +55,7 @@ def main():
     ap.add_argument("--max-depth", type=int, default=40, help="KD-tree depth cap (reference: 20)")
     ap.add_argument("--leaf-size", type=int, default=8, help="nodes this small are not split (reference: 32)")
     ap.add_argument("--traversal-cost", type=float, default=2.0, help="SAH traversal cost (reference: 5)")
+    ap.add_argument("--no-autotune", action="store_true", help="trace with the default tuning")
     ap.add_argument("--no-ref-tree", action="store_true",
                     help="skip timing the same rays over the tree built with the reference's parameters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -90,8 +91,13 @@ def main():
 
     ctx = P.Context(0)
     sid = ctx.upload(tree, shading=False)
+    tuning = None
     if args.layout >= 0:
         ctx.set_tuning(sid, layout=args.layout)
+    elif not args.no_autotune:
+        t1 = time.time()
+        tuning = ctx.autotune(sid, n_rays=n_rays_for_tuning(args), force=True)
+        log(f"autotuned {tuning} in {time.time() - t1:.1f}s")
     box = P.scene_box_expanded(tree.box())
 
     # Inputs and outputs resident in HBM for the whole run.
@@ -166,6 +172,8 @@ def main():
         t1 = time.time()
         rtree = scene.build(1)
         rsid = ctx.upload(rtree, shading=False)
+        if tuning is not None:
+            ctx.set_tuning(rsid, **tuning)
         log(f"reference-parameter tree {rtree.stats()} in {time.time() - t1:.1f}s")
         ms_ref, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=rsid)
         same = bool(np.array_equal(closest_idx, d_idx.numpy()) and
@@ -210,6 +218,7 @@ def main():
             "rays": n, "triangles": n_tris, "tree": tree_tag, "tree_depth": tstats["maxdepth"],
             "tree_nodes": tstats["flat_nodes"], "leaf_refs": tstats["flat_refs"],
             "layout": "packed" if tun["layout"] == P.LAYOUT_PACKED else "indexed",
+            "tuning": tun, "autotuned": tuning is not None,
             "hit_fraction": round(hits / n, 4),
         },
         "roofline": {
@@ -310,6 +319,11 @@ def P_camera(P, scene, side):
     return scene.camera()
 
 
+def n_rays_for_tuning(args):
+    """The autotuner times the workload's own stream size (capped at 2^24)."""
+    return min(args.rays, 1 << 24)
+
+
 def equal_rmse_time(points, target):
     """Time at which a method whose runs gave (ms, rmse) points reaches
     `target`: log-log interpolation between the two nearest points (RMSE of
@@ -332,6 +346,8 @@ def radiance_cache_leg(P, ctx, args):
     scene = P.Scene.cornell(1, 100_000)
     tree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
     sid = ctx.upload(tree)
+    if not args.no_autotune:
+        ctx.autotune(sid, n_rays=1 << 22, force=True)
     inf = scene.info()
     cam, bg = scene.camera(), inf["background"]
     depth, rr = 8, 3

@@ -105,8 +105,8 @@ PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *layout, int *stac
                              int *occupancy);
 
 /* Autotuner (no reference counterpart): times closest-hit traversal of
- * n_rays incoherent rays through the scene (0 = 2^20) over stack depth, leaf
- * ILP, queue chunk and occupancy (coordinate descent from the defaults) and
+ * n_rays incoherent rays through the scene (0 = 2^20) over stack depth x
+ * leaf ILP x queue chunk, then occupancy for the fastest of those, and
  * keeps the fastest setting for the scene and for its size bucket (triangle
  * counts within a factor of 4).  Later scenes of a tuned bucket take the
  * bucket's setting without timing unless `force` is set.  tuning5

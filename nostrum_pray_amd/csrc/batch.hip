@@ -70,31 +70,22 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
         HIPC(c, hipMalloc(&tt.p, (size_t)n_rays * 4));
         int rc = sample_rays(c, *sc, n_rays, 0x5EEDull + (uint64_t)scene_id, (float *)rays.p);
         if (rc) return rc;
-        // Coordinate descent over stack depth, leaf ILP, refill chunk and
-        // grid occupancy from the defaults (packed layout).
+        // Grid over stack depth x leaf ILP x queue chunk at full occupancy,
+        // then the occupancy of the winner (packed layout).
         Tuning best;
-        float best_ms = time_trace(c, *sc, best, (const float *)rays.p, n_rays, 1e-9f, (int32_t *)idx.p,
-                                   (float *)tt.p, 2);
+        float best_ms = 1e30f;
         auto consider = [&](Tuning cand) {
             const float ms = time_trace(c, *sc, cand, (const float *)rays.p, n_rays, 1e-9f, (int32_t *)idx.p,
                                         (float *)tt.p, 2);
             if (ms < best_ms) best_ms = ms, best = cand;
         };
-        for (int K : {4, 16}) {
-            Tuning t = best;
-            t.stack_lds = K;
-            consider(t);
-        }
-        {
-            Tuning t = best;
-            t.leaf_ilp = 3 - best.leaf_ilp;
-            consider(t);
-        }
-        for (int fm : {16, 48, 64}) {
-            Tuning t = best;
-            t.fetch_min = fm;
-            consider(t);
-        }
+        for (int K : {4, 8, 16})
+            for (int ilp : {1, 2})
+                for (int fm : {32, 48, 64}) {
+                    Tuning t;
+                    t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm;
+                    consider(t);
+                }
         for (int occ : {75, 50}) {
             Tuning t = best;
             t.occupancy = occ;

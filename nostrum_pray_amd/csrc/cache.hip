@@ -423,6 +423,10 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
     }
     int rc = prepare_trace(c, *sc, npix);
     if (rc) return rc;
+    if (!c->side) HIPC(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    if (!c->fork_ev) HIPC(c, hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    if (!c->join_ev) HIPC(c, hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
+    hipStream_t side = c->side;
     const uint32_t step_base = r->steps;
     const unsigned infer_grid = (unsigned)std::min<uint64_t>((npix + 63) / 64, c->num_cus * 8ull);
 
@@ -448,25 +452,31 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
                                (const uint32_t *)(count + cur), (const int32_t *)hit.p, (const float *)tt.p,
                                (float *)rq[nxt].p, (float4 *)sq[nxt].p, count + nxt, (float *)contrib.p, prr::BatchArgs{});
         }
+        // The training rays (few, latency bound) and their targets run on the
+        // side stream while the pixels' inference runs on the main one.
+        HIPC(c, hipEventRecord(c->fork_ev, st));
+        HIPC(c, hipStreamWaitEvent(side, c->fork_ev, 0));
         int e = enqueue_trace(c, *sc, (const float *)trays.p, n_train, r->ctr, PRAY_TRACE_CLOSEST, det_eps, nullptr,
-                              (int32_t *)thit.p, (float *)ttt.p, nullptr, st);
+                              (int32_t *)thit.p, (float *)ttt.p, nullptr, side);
         if (e) return e;
-        hipLaunchKernelGGL(prr::k_pt_train_targets, dim3(nblocks(n_train, 256)), dim3(256), 0, st, sc->d, P, C,
+        hipLaunchKernelGGL(prr::k_pt_train_targets, dim3(nblocks(n_train, 256)), dim3(256), 0, side, sc->d, P, C,
                            (const float *)trays.p, (const float4 *)tstate.p, (const uint32_t *)r->ctr,
                            (const int32_t *)thit.p, (const float *)ttt.p, r->targets, (float *)tq.p, count + 4);
-        prm::InferArgs ia{};
-        ia.queries = (const float *)queries.p;
-        ia.n = (uint32_t)npix;
-        ia.n_dev = count + 3;
-        ia.contrib = (float *)contrib.p;
-        hipLaunchKernelGGL(prm::k_mlp_infer, dim3(infer_grid), dim3(256), 0, st, (const prm::bf16x8 *)r->ffrag, ia);
         prm::InferArgs it{};
         it.queries = (const float *)tq.p;
         it.n = n_train;
         it.n_dev = count + 4;
         it.targets = r->targets;
         hipLaunchKernelGGL(prm::k_mlp_infer, dim3((unsigned)std::min<uint64_t>((n_train + 63) / 64, infer_grid)),
-                           dim3(256), 0, st, (const prm::bf16x8 *)r->ffrag, it);
+                           dim3(256), 0, side, (const prm::bf16x8 *)r->ffrag, it);
+        HIPC(c, hipEventRecord(c->join_ev, side));
+        prm::InferArgs ia{};
+        ia.queries = (const float *)queries.p;
+        ia.n = (uint32_t)npix;
+        ia.n_dev = count + 3;
+        ia.contrib = (float *)contrib.p;
+        hipLaunchKernelGGL(prm::k_mlp_infer, dim3(infer_grid), dim3(256), 0, st, (const prm::bf16x8 *)r->ffrag, ia);
+        HIPC(c, hipStreamWaitEvent(st, c->join_ev, 0));
         e = enqueue_train(c, *r, n_train, true, max_loss, min_steps, (float *)hist.p, step_base, frames, st);
         if (e) return e;
         hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,

@@ -43,6 +43,8 @@ struct pray_ctx {
     int next_id = 1;
     hipEvent_t events[16] = {};   // pray_event_record slots
     hipEvent_t ievents[4] = {};   // internal timing (autotuner, batch)
+    hipStream_t side = nullptr;   // second stream for work overlapped inside graphs
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     uint64_t last_rays = 0;       // rays traced by the last render call
     uint32_t *work = nullptr;   // ray counters of the persistent kernels
     uint4 *spill = nullptr;     // per-lane stack spill area

@@ -337,6 +337,9 @@ void pray_destroy(pray_ctx *c) {
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t &e : c->ievents)
         if (e) (void)hipEventDestroy(e);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->work) (void)hipFree(c->work);
     if (c->spill) (void)hipFree(c->spill);
     (void)hipStreamDestroy(c->stream);
