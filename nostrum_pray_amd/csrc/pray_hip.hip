@@ -392,7 +392,8 @@ int pray_upload_scene(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, cons
     };
     void *p = nullptr;
     if (n_tris > 0) {
-        if ((rc = up(nodes, (size_t)n_nodes * 8, (size_t)n_nodes * 8, &p))) goto bad;
+        // padded to whole 128-byte blocks of 16 nodes
+        if ((rc = up(nodes, (size_t)n_nodes * 8, (((size_t)n_nodes + 15) & ~(size_t)15) * 8, &p))) goto bad;
         d.nodes = (const uint2 *)p;
         if ((rc = up(leaf_refs, (size_t)n_refs * 4, (size_t)(n_refs + 2) * 4, &p))) goto bad;
         d.refs = (const uint32_t *)p;

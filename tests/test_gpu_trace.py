@@ -91,13 +91,18 @@ def test_any_hit(ctx, name, eps, mode):
                    rng.uniform(0, 4, len(ct)).astype(np.float32)).astype(np.float32)
     sid = ctx.upload(t)
     try:
-        say(f"any {name} mode={mode}")
-        gi, gt = ctx.trace(sid, rays, mode, eps, limit2=lim)
+        res = []
+        for layout in (P.LAYOUT_PACKED, P.LAYOUT_INDEXED):
+            ctx.set_tuning(sid, layout=layout)
+            say(f"any {name} mode={mode} layout={layout}")
+            res.append(ctx.trace(sid, rays, mode, eps, limit2=lim))
     finally:
         ctx.free_scene(sid)
     sq = ct * ct
     want = (ci >= 0) & ((sq < lim) if mode == P.TRACE_ANY else (sq <= lim))
-    assert np.array_equal(gi >= 0, want)
+    for gi, _ in res:
+        assert np.array_equal(gi >= 0, want)
+    gi, gt = res[0]
     occ = np.flatnonzero(gi >= 0)
     # every reported occluder is a genuine hit under the same predicate
     for k in occ[:512]:
