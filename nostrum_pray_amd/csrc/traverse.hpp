@@ -3,10 +3,10 @@
 // One lane = one ray at a time; waves are persistent and refill lanes whose
 // ray finished from a global ray counter (one atomic per refill, shared by
 // the idle lanes of the wave through a ballot), so long rays never hold a wave
-// of finished lanes.  Every loop iteration each lane does ONE step: either an
-// inner-node step (one aligned 16-byte child-pair load) or a leaf step (ILP
-// triangle records loaded together; the record array is padded so the
-// look-ahead stays in bounds).  Pending far children go to a per-lane stack:
+// of finished lanes.  Every loop iteration each lane does an inner-node step
+// (one aligned 16-byte child-pair load), a leaf step (ILP triangle records
+// loaded together; the record array is padded so the look-ahead stays in
+// bounds), or both when the node step lands in a leaf.  Pending far children go to a per-lane stack:
 // K entries in an LDS ring, the oldest entries spilled to a per-lane global
 // area when the ring is full.
 //
@@ -250,7 +250,9 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
             }
         }
         // ---------------------------------------------------- leaf step --
-        else if (state == ST_LEAF) {
+        // (also for lanes that reached their leaf in this iteration's node
+        // step: the leaf branch runs anyway while any lane is in a leaf)
+        if (!need_pop && state == ST_LEAF) {
             uint32_t ref[ILP];
             float4 ta[ILP], tb[ILP], tc[ILP];
 #pragma unroll
