@@ -79,6 +79,13 @@ PRAY_API ph_tree *ph_build_depth(ph_scene *s, int kind, int max_depth);
 PRAY_API ph_tree *ph_build_ex(ph_scene *s, int kind, int max_depth, int leaf_size, float traversal_cost,
                               float intersection_cost);
 PRAY_API void ph_tree_free(ph_tree *t);
+/* Flat-tree file (no reference counterpart): the device layout, counts and
+ * root box, checksummed, so that a large scene is built once and later
+ * uploaded without a rebuild.  A loaded tree has no CPU tree (ph_tree_export
+ * reports 0 nodes); its stats come from the flat layout.  Return 0 / NULL on
+ * failure with the reason in err. */
+PRAY_API int ph_tree_save(const ph_tree *t, const char *path, char *err, uint64_t err_len);
+PRAY_API ph_tree *ph_tree_load(const char *path, char *err, uint64_t err_len);
 /* stats[8]: kind, nnodes, maxdepth, leafNodes, emptyLeafNodes, triangleCount,
  * flat node count, flat leaf refs */
 PRAY_API void ph_tree_stats(const ph_tree *t, uint64_t *stats);

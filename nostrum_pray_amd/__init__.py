@@ -129,6 +129,8 @@ def libraries():
         "ph_build": (_vp, [_vp, C.c_int]),
         "ph_build_depth": (_vp, [_vp, C.c_int, C.c_int]),
         "ph_build_ex": (_vp, [_vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]),
+        "ph_tree_save": (C.c_int, [_vp, C.c_char_p, C.c_char_p, _u64]),
+        "ph_tree_load": (_vp, [C.c_char_p, C.c_char_p, _u64]),
         "ph_tree_free": (None, [_vp]),
         "ph_tree_stats": (None, [_vp, _vp]),
         "ph_tree_export": (None, [_vp, _vp, _vp, _vp, _vp]),
@@ -260,6 +262,21 @@ class Scene:
 
 class Tree:
     """A built KD tree and its flat device layout (host memory)."""
+
+    def save(self, path):
+        """Writes the flat layout to a file (ph_tree_save)."""
+        err = C.create_string_buffer(512)
+        if self._lib.ph_tree_save(self._h, str(path).encode(), err, 512) != 0:
+            raise ValueError(err.value.decode())
+
+    @classmethod
+    def load(cls, path):
+        """A tree read from a flat-layout file (no CPU tree)."""
+        err = C.create_string_buffer(512)
+        h = libraries()[1].ph_tree_load(str(path).encode(), err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        return cls(h)
 
     def __init__(self, handle):
         if not handle:

@@ -29,6 +29,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "kd_tree.hpp"
@@ -56,5 +57,11 @@ struct KDTreeGPU {
 // Flattens a built tree.  Unlike the reference (gpu_util.cpp:13), the CPU tree
 // keeps its triangles; they are copied into the per-field arrays.
 KDTreeGPU buildGPUTreeFromKDTree(const KDTree &tree);
+
+// A flattened tree as a file (flat_io.cpp): written once, uploaded later
+// without rebuilding.  The load checks the header, the length and a
+// checksum; the upload validates the structure.
+bool saveFlat(const KDTreeGPU &g, int kind, const std::string &path, std::string &err);
+bool loadFlat(const std::string &path, KDTreeGPU &g, int &kind, std::string &err);
 
 }  // namespace pray

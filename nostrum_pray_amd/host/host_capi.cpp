@@ -1,4 +1,5 @@
 // extern "C" surface of libpray_host.so (include/pray_host.h).
+#include <cstdio>
 #include <cstring>
 #include <string>
 
@@ -181,18 +182,45 @@ void ph_tree_free(ph_tree *t) { delete t; }
 
 void ph_tree_stats(const ph_tree *t, uint64_t *o) {
     o[0] = (uint64_t)t->kind;
-    o[1] = t->tree->nnodes;
-    o[2] = t->tree->maxdepth;
-    o[3] = t->tree->leafNodes;
-    o[4] = t->tree->emptyLeafNodes;
-    o[5] = t->tree->triangleCount;
+    if (t->tree) {
+        o[1] = t->tree->nnodes;
+        o[2] = t->tree->maxdepth;
+        o[3] = t->tree->leafNodes;
+        o[4] = t->tree->emptyLeafNodes;
+        o[5] = t->tree->triangleCount;
+    } else {  // loaded from a file: the flat layout's own counts
+        o[1] = 2 * (uint64_t)t->flat.numLeaves - 1;
+        o[2] = t->flat.maxDepth;
+        o[3] = t->flat.numLeaves;
+        o[4] = t->flat.numEmptyLeaves;
+        o[5] = t->flat.numTris();
+    }
     o[6] = t->flat.numNodes();
     o[7] = t->flat.leafRefs.size();
 }
 
+int ph_tree_save(const ph_tree *t, const char *path, char *err, uint64_t err_len) {
+    std::string e;
+    if (!t || !path) return -1;
+    if (pray::saveFlat(t->flat, t->kind, path, e)) return 0;
+    if (err && err_len) std::snprintf(err, err_len, "%s", e.c_str());
+    return -1;
+}
+
+ph_tree *ph_tree_load(const char *path, char *err, uint64_t err_len) {
+    auto *t = new ph_tree();
+    std::string e;
+    if (!path || !pray::loadFlat(path, t->flat, t->kind, e)) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", path ? e.c_str() : "null path");
+        delete t;
+        return nullptr;
+    }
+    return t;
+}
+
 void ph_tree_export(const ph_tree *t, uint64_t *sizes, int32_t *ni, float *nf, uint32_t *tris) {
     pray::TreeExport ex;
-    pray::exportTree(t->tree->root.get(), ex);
+    if (t->tree) pray::exportTree(t->tree->root.get(), ex);
     const size_t n = ex.is_leaf.size();
     if (sizes) sizes[0] = n, sizes[1] = ex.tris.size();
     if (ni)
@@ -230,6 +258,7 @@ void ph_flat_get(const ph_tree *t, uint64_t *sizes, uint32_t *nodes, uint32_t *r
 
 int ph_upload(const ph_tree *t, pray_ctx *ctx, int with_shading, int *scene_id) {
     const pray::KDTreeGPU &g = t->flat;
+    if (with_shading && g.triNormal.size() != 3 * (size_t)g.numTris()) with_shading = 0;  // file without shading
     return pray_upload_scene(ctx, g.nodes.data(), g.numNodes(), g.leafRefs.data(), (uint32_t)g.leafRefs.size(),
                              g.triA.data(), g.triAB.data(), g.triAC.data(), with_shading ? g.triNormal.data() : nullptr,
                              with_shading ? g.triColor.data() : nullptr, with_shading ? g.triEmit.data() : nullptr,

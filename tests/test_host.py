@@ -214,3 +214,34 @@ def test_axis_parallel_rays_keep_pruning():
     bf_idx, bf_t = O.bf_closest(f["a"], f["ab"], f["ac"], rays, 1e-9)
     assert np.array_equal(idx, bf_idx) and np.array_equal(t.view(np.uint32), bf_t.view(np.uint32))
     assert cnt[:, 0].max() < 0.02 * n_nodes, cnt
+
+
+def test_flat_tree_file_round_trip(tmp_path):
+    """A saved flat tree loads back bit for bit and traces identically; a
+    damaged or truncated file is refused."""
+    s = P.Scene.cornell(5, 8000)
+    t = s.build(1, max_depth=40, leaf_size=8, traversal_cost=2.0)
+    path = tmp_path / "c.pflat"
+    t.save(path)
+    u = P.Tree.load(path)
+    a, b = t.flat(), u.flat()
+    for k in ("nodes", "refs", "a", "ab", "ac", "normal", "color", "emit", "box"):
+        assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
+    assert a["max_depth"] == b["max_depth"]
+    st, su = t.stats(), u.stats()
+    for k in ("kind", "maxdepth", "leafNodes", "emptyLeafNodes", "triangleCount", "flat_nodes", "flat_refs"):
+        assert st[k] == su[k], k
+    rays = O.gen_rays(8, P.scene_box_expanded(b["box"]), 2000)
+    assert np.array_equal(O.flat_trace(a, rays, 0, 1e-9)[0], O.flat_trace(b, rays, 0, 1e-9)[0])
+    raw = bytearray(path.read_bytes())
+    raw[200] ^= 0x40
+    bad = tmp_path / "bad.pflat"
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(ValueError, match="checksum"):
+        P.Tree.load(bad)
+    short = tmp_path / "short.pflat"
+    short.write_bytes(path.read_bytes()[:-9])
+    with pytest.raises(ValueError, match="length"):
+        P.Tree.load(short)
+    with pytest.raises(ValueError):
+        P.Tree.load(tmp_path / "missing.pflat")
