@@ -175,6 +175,21 @@ PRAY_API int pray_render_pt_ex(pray_ctx *ctx, int scene_id, const float *camera1
                                uint32_t spp, uint32_t max_depth, uint32_t rr_depth, uint64_t seed,
                                const float *background, float det_eps, float *hdr, uint8_t *rgb, int use_graph);
 
+/* Progressive path tracing (the reference renders all samples in one call,
+ * path_trace.cu:494-555, and keeps only the clamped 8-bit image,
+ * pray.cpp:377 / main.cpp:93-98): samples first_sample .. first_sample+spp-1
+ * of the image pray_render_pt would render with the same arguments.
+ *   accum  width*height*3 floats, in/out: the running sum of per-sample
+ *          radiance over samples 0 .. first_sample-1 (not read when
+ *          first_sample = 0); on return the sum over 0 .. first_sample+spp-1
+ *   hdr / rgb  as pray_render_pt, the mean over all first_sample+spp samples
+ * Renders split over several calls (passing accum on) give the image of one
+ * pray_render_pt call bit for bit.  At most 2^24 samples in total. */
+PRAY_API int pray_render_pt_progressive(pray_ctx *ctx, int scene_id, const float *camera16, uint32_t width,
+                                        uint32_t height, uint32_t first_sample, uint32_t spp, uint32_t max_depth,
+                                        uint32_t rr_depth, uint64_t seed, const float *background, float det_eps,
+                                        float *accum, float *hdr, uint8_t *rgb);
+
 /* Path tracing of several scenes in one wavefront (no reference
  * counterpart): every bounce is one persistent closest-hit launch over the
  * paths of all scenes (rays carry their scene; packed layout) and one

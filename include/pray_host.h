@@ -98,6 +98,25 @@ PRAY_API void ph_tree_export(const ph_tree *t, uint64_t *sizes, int32_t *node_i3
  * be NULL. */
 PRAY_API void ph_flat_get(const ph_tree *t, uint64_t *sizes, uint32_t *nodes, uint32_t *refs, float *a, float *ab, float *ac,
                  float *normal, float *color, uint8_t *emit, float *root_box);
+/* Float images and progressive-render state (no reference counterpart: the
+ * reference writes only the clamped 8-bit BMP, main.cpp:93-98).
+ * ph_write_pfm / ph_read_pfm: RGB Portable Float Map, w*h*3 floats top row
+ * first (ph_read_pfm with rgb = NULL only reports wh[2] = {w, h}).
+ * ph_scene_fingerprint: hash of everything that decides a path-traced pixel
+ * except the sample count (triangles, materials, camera, size, depth limits,
+ * seed, background, det_eps).
+ * ph_accum_save / ph_accum_load: the running radiance sum of
+ * pray_render_pt_progressive (w*h*3 floats) after `samples` samples, with the
+ * fingerprint and a checksum; info3 = {w, h, samples}; ph_accum_load
+ * checks the whole file and with sum = NULL only reports info3 and the
+ * fingerprint.  Return 0, or -1 with the reason in err. */
+PRAY_API uint64_t ph_scene_fingerprint(const ph_scene *s);
+PRAY_API int ph_write_pfm(const char *path, uint32_t w, uint32_t h, const float *rgb);
+PRAY_API int ph_read_pfm(const char *path, uint32_t *wh, float *rgb, char *err, uint64_t err_len);
+PRAY_API int ph_accum_save(const char *path, uint32_t w, uint32_t h, uint32_t samples, uint64_t fingerprint,
+                           const float *sum, char *err, uint64_t err_len);
+PRAY_API int ph_accum_load(const char *path, uint32_t *info3, uint64_t *fingerprint, float *sum, char *err,
+                           uint64_t err_len);
 /* Uploads the flat layout into a pray context; with_shading = 0 skips
  * normals/colours/emission. */
 PRAY_API int ph_upload(const ph_tree *t, pray_ctx *ctx, int with_shading, int *scene_id);

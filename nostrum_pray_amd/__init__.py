@@ -103,6 +103,8 @@ def libraries():
                                        C.POINTER(C.c_int)]),
         "pray_render_pt_cache": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u64, _vp, _f32,
                                            _u32, _u32, _f32, _u32, _vp, _vp, _vp]),
+        "pray_render_pt_progressive": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u64, _vp,
+                                                 _f32, _vp, _vp, _vp]),
         "pray_event_record": (C.c_int, [_vp, C.c_int]),
         "pray_event_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_f32)]),
     }
@@ -136,6 +138,11 @@ def libraries():
         "ph_tree_export": (None, [_vp, _vp, _vp, _vp, _vp]),
         "ph_flat_get": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "ph_upload": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_int)]),
+        "ph_scene_fingerprint": (_u64, [_vp]),
+        "ph_write_pfm": (C.c_int, [C.c_char_p, _u32, _u32, _vp]),
+        "ph_read_pfm": (C.c_int, [C.c_char_p, _vp, _vp, C.c_char_p, _u64]),
+        "ph_accum_save": (C.c_int, [C.c_char_p, _u32, _u32, _u32, _u64, _vp, C.c_char_p, _u64]),
+        "ph_accum_load": (C.c_int, [C.c_char_p, _vp, C.POINTER(_u64), _vp, C.c_char_p, _u64]),
     }
     for name, (res, args) in hsig.items():
         f = getattr(host, name)
@@ -246,6 +253,11 @@ class Scene:
         out = np.empty(16, np.float32)
         self._lib.ph_camera(self._h, _p(out))
         return out
+
+    def fingerprint(self):
+        """Hash of everything that decides a path-traced pixel except the
+        sample count (ph_scene_fingerprint)."""
+        return int(self._lib.ph_scene_fingerprint(self._h))
 
     def prim_rays(self):
         inf = self.info()
@@ -490,6 +502,71 @@ class Context:
         self._check(self._lib.pray_render_pt_ex(self._h, sid, _p(cam16), width, height, spp, max_depth, rr_depth,
                                                 seed, _p(bg), det_eps, _p(hdr), _p(rgb), 1 if use_graph else 0))
         return hdr, rgb
+
+    def render_pt_progressive(self, sid, cam16, width, height, first_sample, spp, max_depth, rr_depth, seed,
+                              background, det_eps, accum=None, want_rgb=True):
+        """Samples first_sample .. first_sample+spp-1 (pray_render_pt_progressive).
+        accum: the running sum after first_sample samples (None when
+        first_sample = 0).  Returns (hdr, rgb, new running sum)."""
+        cam16 = np.ascontiguousarray(cam16, np.float32)
+        bg = np.ascontiguousarray(background, np.float32).reshape(3)
+        acc = np.zeros((height, width, 3), np.float32) if accum is None else \
+            np.array(accum, np.float32, copy=True).reshape(height, width, 3)
+        hdr = np.empty((height, width, 3), np.float32)
+        rgb = np.empty((height, width, 3), np.uint8) if want_rgb else None
+        self._check(self._lib.pray_render_pt_progressive(self._h, sid, _p(cam16), width, height, first_sample, spp,
+                                                         max_depth, rr_depth, seed, _p(bg), det_eps, _p(acc),
+                                                         _p(hdr), _p(rgb)))
+        return hdr, rgb, acc
+
+
+def _host_error(err):
+    return ValueError(err.value.decode(errors="replace"))
+
+
+def write_pfm(path, hdr):
+    """Writes an (h, w, 3) float image as a Portable Float Map (ph_write_pfm)."""
+    hdr = np.ascontiguousarray(hdr, np.float32)
+    h, w, _ = hdr.shape
+    if libraries()[1].ph_write_pfm(str(path).encode(), w, h, _p(hdr)) != 0:
+        raise ValueError(f"cannot write {path}")
+
+
+def read_pfm(path):
+    """Reads a Portable Float Map as an (h, w, 3) float32 array (ph_read_pfm)."""
+    lib = libraries()[1]
+    err = C.create_string_buffer(512)
+    wh = np.zeros(2, np.uint32)
+    if lib.ph_read_pfm(str(path).encode(), _p(wh), None, err, 512) != 0:
+        raise _host_error(err)
+    out = np.empty((int(wh[1]), int(wh[0]), 3), np.float32)
+    if lib.ph_read_pfm(str(path).encode(), _p(wh), _p(out), err, 512) != 0:
+        raise _host_error(err)
+    return out
+
+
+def save_accum(path, accum, samples, fingerprint):
+    """Saves a progressive render's running sum (ph_accum_save)."""
+    accum = np.ascontiguousarray(accum, np.float32)
+    h, w, _ = accum.shape
+    err = C.create_string_buffer(512)
+    if libraries()[1].ph_accum_save(str(path).encode(), w, h, samples, fingerprint, _p(accum), err, 512) != 0:
+        raise _host_error(err)
+
+
+def load_accum(path):
+    """Loads a running sum saved by save_accum or the CLI's --accum:
+    (sum (h, w, 3), samples, fingerprint)."""
+    lib = libraries()[1]
+    err = C.create_string_buffer(512)
+    info = np.zeros(3, np.uint32)
+    fp = _u64(0)
+    if lib.ph_accum_load(str(path).encode(), _p(info), C.byref(fp), None, err, 512) != 0:
+        raise _host_error(err)
+    out = np.empty((int(info[1]), int(info[0]), 3), np.float32)
+    if lib.ph_accum_load(str(path).encode(), _p(info), C.byref(fp), _p(out), err, 512) != 0:
+        raise _host_error(err)
+    return out, int(info[2]), fp.value
 
 
 CACHE_PARAMS = 30720

@@ -652,9 +652,16 @@ int pray_render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, ui
     return pray_render_pt_ex(c, scene_id, cam16, w, h, spp, max_depth, rr_depth, seed, bg, det_eps, hdr, rgb, 1);
 }
 
-int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_t h, uint32_t spp,
-                      uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *bg, float det_eps, float *hdr,
-                      uint8_t *rgb, int use_graph) {
+namespace {
+
+// Samples first .. first+spp-1 of a path-traced image.  The running sum of
+// per-sample radiance (3 floats per pixel) starts from accum_in (zeros when
+// first == 0) and is returned in accum_out; hdr / rgb receive the mean over
+// all first+spp samples.  Samples are added in order into the same fp32 sum,
+// so splitting a render into several calls gives the image of one call.
+int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_t h, uint32_t first, uint32_t spp,
+              uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *bg, float det_eps,
+              const float *accum_in, float *accum_out, float *hdr, uint8_t *rgb, int use_graph) {
     if (!c) return PRAY_ERR_INVALID;
     SceneRec *sc = find_scene(c, scene_id);
     if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
@@ -662,6 +669,9 @@ int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w,
     const uint64_t npix = (uint64_t)w * h;
     if (npix == 0 || spp == 0) return PRAY_OK;
     if (npix > (1u << 28)) return fail(c, PRAY_ERR_INVALID, "image too large");
+    if (first && !accum_in) return fail(c, PRAY_ERR_INVALID, "resuming at sample %u needs the running sum", first);
+    // The mean divides by the sample count as a float: keep it exact.
+    if ((uint64_t)first + spp > (1u << 24)) return fail(c, PRAY_ERR_INVALID, "more than 2^24 samples");
     if (sc->d.n_tris && (!sc->d.normal || !sc->d.color || !sc->d.emit))
         return fail(c, PRAY_ERR_INVALID, "scene was uploaded without shading data");
     HIPC(c, hipSetDevice(c->device));
@@ -683,9 +693,13 @@ int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w,
     uint32_t *count = (uint32_t *)cnt.p;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(prr::k_camera, dim3(nblocks(npix, 256)), dim3(256), 0, st, cam, w, h, (float *)prim.p);
-    hipLaunchKernelGGL(prr::k_fill_f32, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (float *)sum.p,
-                       (uint32_t)(3 * npix), 0.f);
+    if (first)
+        HIPC(c, hipMemcpyAsync(sum.p, accum_in, npix * 12, hipMemcpyHostToDevice, st));
+    else
+        hipLaunchKernelGGL(prr::k_fill_f32, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (float *)sum.p,
+                           (uint32_t)(3 * npix), 0.f);
     hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count, 6u);
+    hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, count + 2, first);  // sample index
     HIPC(c, hipGetLastError());
     unsigned long long *traced = (unsigned long long *)(count + 4);
     prr::PtArgs P;
@@ -756,13 +770,31 @@ int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w,
         }
     }
     hipLaunchKernelGGL(prr::k_pt_finalize, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
-                       (const float *)sum.p, (float)spp, (float *)outh.p, (uint8_t *)outb.p);
+                       (const float *)sum.p, (float)(first + spp), (float *)outh.p, (uint8_t *)outb.p);
     HIPC(c, hipGetLastError());
     if (hdr) HIPC(c, hipMemcpyAsync(hdr, outh.p, npix * 12, hipMemcpyDeviceToHost, st));
     if (rgb) HIPC(c, hipMemcpyAsync(rgb, outb.p, npix * 3, hipMemcpyDeviceToHost, st));
+    if (accum_out) HIPC(c, hipMemcpyAsync(accum_out, sum.p, npix * 12, hipMemcpyDeviceToHost, st));
     HIPC(c, hipMemcpyAsync(&c->last_rays, traced, 8, hipMemcpyDeviceToHost, st));
     HIPC(c, hipStreamSynchronize(st));
     return PRAY_OK;
+}
+
+}  // namespace
+
+int pray_render_pt_ex(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_t h, uint32_t spp,
+                      uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *bg, float det_eps, float *hdr,
+                      uint8_t *rgb, int use_graph) {
+    return render_pt(c, scene_id, cam16, w, h, 0, spp, max_depth, rr_depth, seed, bg, det_eps, nullptr, nullptr, hdr,
+                     rgb, use_graph);
+}
+
+int pray_render_pt_progressive(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_t h,
+                               uint32_t first_sample, uint32_t spp, uint32_t max_depth, uint32_t rr_depth,
+                               uint64_t seed, const float *bg, float det_eps, float *accum, float *hdr, uint8_t *rgb) {
+    if (c && !accum) return fail(c, PRAY_ERR_INVALID, "null accumulation buffer");
+    return render_pt(c, scene_id, cam16, w, h, first_sample, spp, max_depth, rr_depth, seed, bg, det_eps, accum,
+                     accum, hdr, rgb, 1);
 }
 
 int pray_device_alloc(pray_ctx *c, uint64_t bytes, void **p) {

@@ -2,6 +2,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/pray_host.h"
 #include "gpu_layout.hpp"
@@ -216,6 +217,53 @@ ph_tree *ph_tree_load(const char *path, char *err, uint64_t err_len) {
         return nullptr;
     }
     return t;
+}
+
+uint64_t ph_scene_fingerprint(const ph_scene *s) { return s ? pray::renderFingerprint(s->s) : 0; }
+
+int ph_write_pfm(const char *path, uint32_t w, uint32_t h, const float *rgb) {
+    if (!path || !rgb || !w || !h) return -1;
+    return pray::writePFM(path, w, h, rgb) ? 0 : -1;
+}
+
+int ph_read_pfm(const char *path, uint32_t *wh, float *rgb, char *err, uint64_t err_len) {
+    std::string e;
+    uint32_t w = 0, h = 0;
+    std::vector<float> v;
+    if (!path || !wh || !pray::readPFM(path, w, h, v, e)) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", path && wh ? e.c_str() : "null argument");
+        return -1;
+    }
+    wh[0] = w, wh[1] = h;
+    if (rgb) std::memcpy(rgb, v.data(), v.size() * 4);
+    return 0;
+}
+
+int ph_accum_save(const char *path, uint32_t w, uint32_t h, uint32_t samples, uint64_t fingerprint, const float *sum,
+                  char *err, uint64_t err_len) {
+    std::string e;
+    if (!path || (!sum && (uint64_t)w * h != 0)) {
+        if (err && err_len) std::snprintf(err, err_len, "null argument");
+        return -1;
+    }
+    if (pray::saveAccum(path, w, h, samples, fingerprint, sum, e)) return 0;
+    if (err && err_len) std::snprintf(err, err_len, "%s", e.c_str());
+    return -1;
+}
+
+int ph_accum_load(const char *path, uint32_t *info3, uint64_t *fingerprint, float *sum, char *err, uint64_t err_len) {
+    std::string e;
+    uint32_t w = 0, h = 0, n = 0;
+    uint64_t fp = 0;
+    std::vector<float> v;
+    if (!path || !info3 || !pray::loadAccum(path, w, h, n, fp, v, e)) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", path && info3 ? e.c_str() : "null argument");
+        return -1;
+    }
+    info3[0] = w, info3[1] = h, info3[2] = n;
+    if (fingerprint) *fingerprint = fp;
+    if (sum) std::memcpy(sum, v.data(), v.size() * 4);
+    return 0;
 }
 
 void ph_tree_export(const ph_tree *t, uint64_t *sizes, int32_t *ni, float *nf, uint32_t *tris) {

@@ -1,11 +1,19 @@
 // Pray CLI: `Pray scene.json out.bmp` (reference src/main.cpp:23-109), plus
 // `Pray --gen <terrain|spheres|cornell|random> <seed> <out.json> [n]` to write
 // the procedural scenes as JSON + OBJ + MTL.
+//
+// Path-tracing scenes take three more options after the two file names:
+//   --hdr out.pfm     also write the linear mean radiance (before the clamp)
+//   --accum state     progressive rendering: continue the running sum kept
+//                     in `state` (created when absent) and save it again
+//   --spp N           samples to add in this run (default: the scene's)
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "pray_render.hpp"
 
@@ -40,10 +48,29 @@ int main(int argc, char *argv[]) {
         }
         return 0;
     }
-    if (argc != 3) {
-        std::cout << "usage: " << argv[0] << " <input.json> <output.bmp>\n";
+    std::string hdrPath, accumPath;
+    long long sppOverride = 0;
+    bool badArgs = argc < 3;
+    for (int i = 3; i < argc && !badArgs; i++) {
+        const std::string a = argv[i];
+        if (i + 1 >= argc) {
+            badArgs = true;
+        } else if (a == "--hdr") {
+            hdrPath = argv[++i];
+        } else if (a == "--accum") {
+            accumPath = argv[++i];
+        } else if (a == "--spp") {
+            sppOverride = std::strtoll(argv[++i], nullptr, 10);
+            badArgs = sppOverride <= 0 || sppOverride >= (1ll << 24);
+        } else {
+            badArgs = true;
+        }
+    }
+    if (badArgs) {
+        std::cout << "usage: " << argv[0] << " <input.json> <output.bmp> [--hdr out.pfm] [--accum state] [--spp N]\n";
         return 1;
     }
+    const bool progressive = !hdrPath.empty() || !accumPath.empty() || sppOverride > 0;
     const auto total0 = clk::now();
     std::cout << "--------------------------------------------------------\nINPUT FILE: " << argv[1] << "\n\n";
     std::cout << "Reading scene description..." << std::endl;
@@ -56,6 +83,11 @@ int main(int argc, char *argv[]) {
     }
     auto t1 = clk::now();
     std::cout << "Finished reading scene description. (" << ms(t0, t1) << " ms)" << std::endl;
+    if (progressive && scene.mode != pray::PATHTRACING) {
+        std::cerr << "--hdr, --accum and --spp apply to path-tracing scenes\n";
+        return 1;
+    }
+    if (sppOverride > 0) scene.numSamples = (size_t)sppOverride;
     std::cout << "\nScene information: Resolution: " << scene.width << "x" << scene.height
               << ", # Lights: " << scene.lights.size() << ", # Triangles: " << scene.triangles.size();
     if (scene.mode == pray::PATHTRACING)
@@ -77,7 +109,36 @@ int main(int argc, char *argv[]) {
         std::cout << "\nRendering scene with " << (scene.mode == pray::RAYTRACING ? "ray" : "path")
                   << " tracing on the GPU..." << std::endl;
         t0 = clk::now();
-        pray::render(image, scene);
+        std::vector<float> hdr;
+        if (progressive) {
+            // Resume from the saved running sum when it belongs to this scene.
+            std::vector<float> accum;
+            uint32_t first = 0;
+            const uint64_t fingerprint = pray::renderFingerprint(scene);
+            if (!accumPath.empty()) {
+                if (FILE *probe = std::fopen(accumPath.c_str(), "rb")) {
+                    std::fclose(probe);
+                    uint32_t w = 0, h = 0;
+                    uint64_t fp = 0;
+                    if (!pray::loadAccum(accumPath, w, h, first, fp, accum, err)) throw std::runtime_error(err);
+                    if (w != scene.width || h != scene.height || fp != fingerprint)
+                        throw std::runtime_error(accumPath + " belongs to another scene or render setting");
+                    if ((uint64_t)first + scene.numSamples >= (1ull << 24))
+                        throw std::runtime_error("more than 2^24 samples in total");
+                    std::cout << "Resuming after " << first << " samples" << std::endl;
+                }
+            }
+            pray::pathTraceProgressive(image, scene, first, (uint32_t)scene.numSamples, accum,
+                                       hdrPath.empty() ? nullptr : &hdr);
+            const uint32_t total = first + (uint32_t)scene.numSamples;
+            if (!accumPath.empty() &&
+                !pray::saveAccum(accumPath, (uint32_t)scene.width, (uint32_t)scene.height, total, fingerprint,
+                                 accum.data(), err))
+                throw std::runtime_error(err);
+            std::cout << "Accumulated " << total << " samples" << std::endl;
+        } else {
+            pray::render(image, scene);
+        }
         t1 = clk::now();
         std::cout << "Finished rendering. (" << ms(t0, t1) << " ms)" << std::endl;
 
@@ -85,6 +146,10 @@ int main(int argc, char *argv[]) {
         t0 = clk::now();
         if (!pray::writeBMP(argv[2], (uint32_t)scene.width, (uint32_t)scene.height, image.data())) {
             std::cerr << "writing " << argv[2] << " failed\n";
+            return 1;
+        }
+        if (!hdrPath.empty() && !pray::writePFM(hdrPath, (uint32_t)scene.width, (uint32_t)scene.height, hdr.data())) {
+            std::cerr << "writing " << hdrPath << " failed\n";
             return 1;
         }
         t1 = clk::now();

@@ -80,6 +80,44 @@ void pathTraceOnGPURecursive(Image &image, const Scene &scene) {
 
 void pathTrace(Image &image, const Scene &scene) { pathTraceOnGPURecursive(image, scene); }
 
+void pathTraceProgressive(Image &image, const Scene &scene, uint32_t first, uint32_t spp, std::vector<float> &accum,
+                          std::vector<float> *hdr) {
+    const size_t n3 = 3 * scene.width * scene.height;
+    image.assign(n3, 0);
+    if (hdr) hdr->assign(n3, 0.f);
+    if (first == 0) accum.assign(n3, 0.f);
+    if (accum.size() != n3) throw std::runtime_error("accumulation buffer does not match the image size");
+    if (!scene.triangleDB || scene.triangleDB->isEmpty()) {
+        // Every sample of every pixel is the background: the same fp32 sums
+        // the GPU would form, per channel.
+        const float bg[3] = {scene.backgroundColor.x, scene.backgroundColor.y, scene.backgroundColor.z};
+        for (int ch = 0; ch < 3; ch++) {
+            float acc = accum[ch];
+            for (uint32_t k = 0; k < spp; k++) acc = acc + bg[ch];
+            const float m = acc / (float)(first + spp);
+            const float lo = m < 1.f ? m : 1.f;
+            const float cl = 0.f < lo ? lo : 0.f;
+            for (size_t i = ch; i < n3; i += 3) {
+                accum[i] = acc;
+                image[i] = static_cast<uint8_t>(cl * 255.f + 0.5f);
+                if (hdr) (*hdr)[i] = m;
+            }
+        }
+        return;
+    }
+    const KDTreeGPU g = buildGPUTreeFromKDTree(*scene.triangleDB);
+    const int id = upload(scene, g);
+    float cam[16];
+    cameraParams(scene, cam);
+    const float bg[3] = {scene.backgroundColor.x, scene.backgroundColor.y, scene.backgroundColor.z};
+    const uint32_t rr = scene.rrDepth == SIZE_MAX ? 0xFFFFFFFFu : (uint32_t)scene.rrDepth;
+    const int rc = pray_render_pt_progressive(ctx(), id, cam, (uint32_t)scene.width, (uint32_t)scene.height, first,
+                                              spp, (uint32_t)scene.maxDepth, rr, scene.seed, bg, scene.detEps,
+                                              accum.data(), hdr ? hdr->data() : nullptr, image.data());
+    pray_free_scene(ctx(), id);
+    check(rc, "path tracing");
+}
+
 // render (pray.cpp:617-637): an empty scene is filled with the rounded
 // background colour.
 void render(Image &image, const Scene &scene) {

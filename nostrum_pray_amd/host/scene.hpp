@@ -99,4 +99,18 @@ bool writeSceneFiles(const Scene &scene, const std::string &json_path, std::stri
 // 24-bit BMP writer (bottom-up rows, BGR), input RGB top row first.
 bool writeBMP(const std::string &path, uint32_t w, uint32_t h, const uint8_t *rgb);
 
+// Float RGB images as Portable Float Maps (input / output top row first).
+bool writePFM(const std::string &path, uint32_t w, uint32_t h, const float *rgb);
+bool readPFM(const std::string &path, uint32_t &w, uint32_t &h, std::vector<float> &rgb, std::string &err);
+
+// Progressive rendering state: the running sum of per-sample radiance of a
+// path-traced image after `samples` samples, tagged with the fingerprint of
+// everything else that decides the pixels (renderFingerprint), so that a
+// resumed render refuses a state from another scene or setting.
+uint64_t renderFingerprint(const Scene &scene);
+bool saveAccum(const std::string &path, uint32_t w, uint32_t h, uint32_t samples, uint64_t fingerprint,
+               const float *sum, std::string &err);
+bool loadAccum(const std::string &path, uint32_t &w, uint32_t &h, uint32_t &samples, uint64_t &fingerprint,
+               std::vector<float> &sum, std::string &err);
+
 }  // namespace pray

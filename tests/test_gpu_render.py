@@ -215,3 +215,90 @@ def test_cli_drop_in_renders_reference_image(tmp_path):
         subprocess.run([O.REF_CLI, os.path.join(GOLDEN, "scenes", name), str(ref_out)], check=True,
                        capture_output=True)
         assert out.read_bytes() == ref_out.read_bytes()
+
+
+def test_pt_progressive_resume_equals_one_call(ctx):
+    """Samples split over several progressive calls (the running sum passed
+    on) give the one-call image bit for bit, which equals the oracle's."""
+    s = P.Scene.cornell(3, 20000)
+    t = s.build(1)
+    w, h, spp, depth, rr = 64, 48, 8, 6, 2
+    cam = O.camera([0, 1, 3.8], [0, 0, -1], 40, w, h)
+    bg = np.asarray(s.info()["background"], np.float32)
+    sid = ctx.upload(t)
+    try:
+        say("pt progressive: one call")
+        hdr, rgb = ctx.render_pt(sid, cam, w, h, spp, depth, rr, 77, bg, 1e-9)
+        rays_one = ctx.last_render_rays()
+        for split in ([3, 5], [1, 1, 6], [7, 1]):
+            say(f"pt progressive: split {split}")
+            acc, first, rays = None, 0, 0
+            for n in split:
+                ph, prgb, acc = ctx.render_pt_progressive(sid, cam, w, h, first, n, depth, rr, 77, bg, 1e-9, accum=acc)
+                rays += ctx.last_render_rays()
+                first += n
+                # every intermediate image is the mean of the samples so far
+                assert np.array_equal(ph.view(np.uint32), (acc / np.float32(first)).view(np.uint32))
+            assert np.array_equal(ph.view(np.uint32), hdr.view(np.uint32)), split
+            assert np.array_equal(prgb, rgb)
+            assert rays == rays_one
+        # resuming needs the running sum
+        with pytest.raises(P.PrayError):
+            ctx._check(ctx._lib.pray_render_pt_progressive(ctx.handle, sid, P._p(cam), w, h, 3, 1, depth, rr, 77,
+                                                           P._p(bg), 1e-9, None, None, None))
+    finally:
+        ctx.free_scene(sid)
+    oh, orgb = O.render_pt(t.flat(), cam, w, h, spp, depth, rr, 77, bg, 1e-9)
+    assert np.array_equal(hdr.view(np.uint32), oh.view(np.uint32)) and np.array_equal(rgb, orgb)
+
+
+def run_cli(*args):
+    r = subprocess.run([P.CLI, *map(str, args)], capture_output=True, text=True, timeout=120)
+    return r.returncode, r.stdout + r.stderr
+
+
+def test_cli_progressive_accumulation_and_hdr(ctx, tmp_path):
+    """`Pray scene.json out.bmp --spp N --accum state --hdr out.pfm`: two
+    resumed runs (2 + 3 samples) write the same BMP and PFM as one 5-sample
+    run, and the same image as the library call; a state from another
+    setting is refused."""
+    scene = os.path.join(GOLDEN, "scenes", "pt_cornell.json")
+    st = tmp_path / "state.pacc"
+    say("cli progressive 2 + 3")
+    rc, out = run_cli(scene, tmp_path / "a.bmp", "--spp", 2, "--accum", st, "--hdr", tmp_path / "a.pfm")
+    assert rc == 0 and "Accumulated 2 samples" in out, out
+    rc, out = run_cli(scene, tmp_path / "b.bmp", "--spp", 3, "--accum", st, "--hdr", tmp_path / "b.pfm")
+    assert rc == 0 and "Resuming after 2 samples" in out and "Accumulated 5 samples" in out, out
+    say("cli one run of 5")
+    rc, out = run_cli(scene, tmp_path / "c.bmp", "--spp", 5, "--hdr", tmp_path / "c.pfm")
+    assert rc == 0, out
+    assert (tmp_path / "b.bmp").read_bytes() == (tmp_path / "c.bmp").read_bytes()
+    assert (tmp_path / "b.pfm").read_bytes() == (tmp_path / "c.pfm").read_bytes()
+    acc, n, fp = P.load_accum(st)
+    s = P.Scene.load_json(scene)
+    assert n == 5 and fp == s.fingerprint()
+    hdr = P.read_pfm(tmp_path / "c.pfm")
+    assert np.array_equal(hdr.view(np.uint32), (acc / np.float32(5)).view(np.uint32))
+    assert np.array_equal(O.read_bmp(str(tmp_path / "c.bmp")), np.asarray(
+        np.clip(hdr, 0, 1) * np.float32(255) + np.float32(0.5), np.float32).astype(np.uint8))
+    # the library call with the scene's own settings renders the same image
+    inf = s.info()
+    sid = ctx.upload(s.build(-1))
+    try:
+        say("library render of the same scene")
+        lh, _ = ctx.render_pt(sid, s.camera(), inf["width"], inf["height"], 5, inf["max_depth"],
+                              inf["rr_depth"] if inf["rr_depth"] >= 0 else 0xFFFFFFFF, inf["seed"],
+                              inf["background"], inf["det_eps"])
+    finally:
+        ctx.free_scene(sid)
+    assert np.array_equal(lh.view(np.uint32), hdr.view(np.uint32))
+    # another render setting (max depth) is refused and leaves the state as it was
+    import shutil
+    for ext in ("obj", "mtl"):
+        shutil.copy(os.path.join(GOLDEN, "scenes", f"CornellBox-Original.{ext}"), tmp_path)
+    other = tmp_path / "other.json"
+    other.write_text(open(scene).read().replace('"max_depth": 1', '"max_depth": 2'))
+    before = st.read_bytes()
+    rc, out = run_cli(other, tmp_path / "d.bmp", "--spp", 1, "--accum", st)
+    assert rc != 0 and "another scene" in out, out
+    assert st.read_bytes() == before

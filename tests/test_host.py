@@ -245,3 +245,86 @@ def test_flat_tree_file_round_trip(tmp_path):
         P.Tree.load(short)
     with pytest.raises(ValueError):
         P.Tree.load(tmp_path / "missing.pflat")
+
+
+def parse_pfm(raw):
+    """Independent PFM reader: header lines, then rows bottom first in the
+    byte order given by the sign of the scale."""
+    parts = raw.split(b"\n", 3)
+    assert parts[0] == b"PF"
+    w, h = map(int, parts[1].split())
+    scale = float(parts[2])
+    data = np.frombuffer(parts[3], dtype="<f4" if scale < 0 else ">f4")
+    assert data.size == w * h * 3
+    return data.reshape(h, w, 3)[::-1].astype(np.float32)
+
+
+def test_pfm_files(tmp_path):
+    rng = np.random.default_rng(3)
+    img = rng.normal(size=(5, 7, 3)).astype(np.float32)
+    img[0, 0] = [np.inf, -0.0, 1e-40]
+    path = tmp_path / "a.pfm"
+    P.write_pfm(path, img)
+    raw = path.read_bytes()
+    assert raw.startswith(b"PF\n7 5\n-1.0\n") and len(raw) == 12 + 5 * 7 * 12
+    assert np.array_equal(parse_pfm(raw).view(np.uint32), img.view(np.uint32))
+    assert np.array_equal(P.read_pfm(path).view(np.uint32), img.view(np.uint32))
+    # big-endian files (positive scale) read too
+    big = tmp_path / "big.pfm"
+    big.write_bytes(b"PF\n7 5\n1.0\n" + img[::-1].astype(">f4").tobytes())
+    assert np.array_equal(P.read_pfm(big).view(np.uint32), img.view(np.uint32))
+    short = tmp_path / "short.pfm"
+    short.write_bytes(raw[:-4])
+    with pytest.raises(ValueError, match="truncated"):
+        P.read_pfm(short)
+    gray = tmp_path / "gray.pfm"
+    gray.write_bytes(b"Pf\n7 5\n-1.0\n" + bytes(140))
+    with pytest.raises(ValueError, match="not an RGB PFM"):
+        P.read_pfm(gray)
+
+
+def test_accumulation_files_and_fingerprint(tmp_path):
+    s = P.Scene.load_json(os.path.join(ROOT, "tests", "golden", "scenes", "pt_cornell.json"))
+    fp = s.fingerprint()
+    # the sample count is not part of the fingerprint; every other setting is
+    s.set_render(spp=17)
+    assert s.fingerprint() == fp
+    s.set_render(max_depth=3)
+    assert s.fingerprint() != fp
+    s2 = P.Scene.load_json(os.path.join(ROOT, "tests", "golden", "scenes", "pt_cornell.json"))
+    s2.set_render(width=801)
+    assert s2.fingerprint() != fp
+    assert P.Scene.cornell(1, 2000).fingerprint() != P.Scene.cornell(2, 2000).fingerprint()
+    acc = np.random.default_rng(4).uniform(0, 9, (6, 4, 3)).astype(np.float32)
+    path = tmp_path / "s.pacc"
+    P.save_accum(path, acc, 12, fp)
+    got, n, gfp = P.load_accum(path)
+    assert np.array_equal(got, acc) and n == 12 and gfp == fp
+    raw = path.read_bytes()
+    assert len(raw) == 32 + acc.nbytes + 8 and raw[:8] == b"PRAYACC1"
+    assert not (tmp_path / "s.pacc.tmp").exists()
+    bad = bytearray(raw)
+    bad[40] ^= 1
+    (tmp_path / "bad.pacc").write_bytes(bytes(bad))
+    with pytest.raises(ValueError, match="checksum"):
+        P.load_accum(tmp_path / "bad.pacc")
+    (tmp_path / "short.pacc").write_bytes(raw[:-1])
+    with pytest.raises(ValueError, match="truncated"):
+        P.load_accum(tmp_path / "short.pacc")
+    (tmp_path / "long.pacc").write_bytes(raw + b"x")
+    with pytest.raises(ValueError, match="trailing"):
+        P.load_accum(tmp_path / "long.pacc")
+    with pytest.raises(ValueError):
+        P.load_accum(tmp_path / "missing.pacc")
+
+
+def test_cli_rejects_bad_options(tmp_path):
+    import subprocess
+    scene = os.path.join(ROOT, "tests", "golden", "scenes", "head_fov30.json")
+    for extra in (["--spp", "0"], ["--bogus", "1"], ["--hdr"]):
+        r = subprocess.run([P.CLI, scene, str(tmp_path / "x.bmp"), *extra], capture_output=True, text=True)
+        assert r.returncode != 0 and "usage" in r.stdout, extra
+    # ray-tracing scenes have no running sum to keep
+    r = subprocess.run([P.CLI, scene, str(tmp_path / "x.bmp"), "--hdr", str(tmp_path / "x.pfm")], capture_output=True,
+                       text=True)
+    assert r.returncode != 0 and "path-tracing scenes" in r.stderr
