@@ -56,6 +56,9 @@ def main():
     ap.add_argument("--leaf-size", type=int, default=8, help="nodes this small are not split (reference: 32)")
     ap.add_argument("--traversal-cost", type=float, default=2.0, help="SAH traversal cost (reference: 5)")
     ap.add_argument("--no-autotune", action="store_true", help="trace with the default tuning")
+    ap.add_argument("--tuning", default="",
+                    help="explicit traversal tuning instead of the autotuner: "
+                         "layout,stack_lds,leaf_ilp,fetch_min,occupancy")
     ap.add_argument("--no-ref-tree", action="store_true",
                     help="skip timing the same rays over the tree built with the reference's parameters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -92,7 +95,10 @@ def main():
     ctx = P.Context(0)
     sid = ctx.upload(tree, shading=False)
     tuning = None
-    if args.layout >= 0:
+    if args.tuning:
+        vals = [int(x) for x in args.tuning.split(",")]
+        ctx.set_tuning(sid, **dict(zip(P.TUNING_FIELDS, vals)))
+    elif args.layout >= 0:
         ctx.set_tuning(sid, layout=args.layout)
     elif not args.no_autotune:
         t1 = time.time()
@@ -124,9 +130,9 @@ def main():
     hits = int((d_idx.numpy() >= 0).sum())
     d_cnt.free()
     tun = ctx.get_tuning(sid)
-    # Algorithmic bytes: ray 24 + result 8 + 8 per node visited + 40 per
-    # triangle reference tested (4-byte index + a, ab, ac); the packed layout
-    # actually moves 48 bytes per reference.
+    # Algorithmic bytes of the traversal kernel: ray 24 + result 8 + 8 per
+    # node visited + 40 per triangle reference tested (4-byte index + a, ab,
+    # ac); the packed layout actually moves 48 bytes per reference.
     nodes_per_ray = cnt[0] / n
     refs_per_ray = cnt[1] / n
     bytes_per_ray = 24 + 8 + 8 * nodes_per_ray + 40 * refs_per_ray
@@ -155,7 +161,17 @@ def main():
     ms_any, _ = timed(P.TRACE_ANY, args.steps, 2, lim=d_lim.ptr)
     log(f"any-hit {ms_any:.3f} ms/step  {n / (ms_any * 1e-3) / 1e6:.1f} Mrays/s")
 
-    achieved = bytes_per_ray * n / (ms * 1e-3) / 1e9
+    # The traversal kernel alone (HIP events around its launch, on the stream
+    # it runs on), without the counter reset in front of it.
+    ctx.kernel_timing(True)
+    kms = []
+    for _ in range(args.steps):
+        ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, P.TRACE_CLOSEST, args.det_eps)
+        kms.append(ctx.last_kernel_ms())
+    ctx.kernel_timing(False)
+    kernel_ms = float(np.mean(kms))
+    log(f"traversal kernel {kernel_ms:.3f} ms/launch (step {ms:.3f} ms)")
+    achieved = bytes_per_ray * n / (kernel_ms * 1e-3) / 1e9
     traffic = None
     # HBM traffic of this kernel/workload from the committed rocprofv3 PMC
     # summary (FETCH_SIZE/WRITE_SIZE calibrated on a plain copy)
@@ -228,6 +244,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "kernel": "k_trace (closest hit)",
+            "kernel_ms": round(kernel_ms, 4),
             "bytes_per_ray": round(float(bytes_per_ray), 1),
             "nodes_per_ray": round(float(nodes_per_ray), 2),
             "refs_per_ray": round(float(refs_per_ray), 2),

@@ -89,29 +89,30 @@ PRAY_API int pray_upload_scene(pray_ctx *ctx, const uint32_t *nodes, uint32_t n_
                       const float *root_box, uint32_t max_depth, int *scene_id);
 PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
 
-/* Tuning of a scene's traversal (results never depend on it):
- *   layout     PRAY_LAYOUT_*
- *   stack_lds  LDS ring entries per lane (4, 8 or 16); deeper entries spill
- *              to a per-lane area in HBM
- *   leaf_ilp   triangle records loaded together per leaf step (1 or 2)
- *   fetch_min  a persistent wave refills its finished lanes when at least
- *              this many are idle (1..64): the queue chunk
- *   occupancy  percent (1..100) of the waves that fit on the device that the
- *              persistent grid launches: the workgroup count
- * Pass -1 to keep a value. */
-PRAY_API int pray_set_tuning(pray_ctx *ctx, int scene_id, int layout, int stack_lds, int leaf_ilp, int fetch_min,
-                             int occupancy);
-PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *layout, int *stack_lds, int *leaf_ilp, int *fetch_min,
-                             int *occupancy);
+/* Tuning of a scene's traversal (results never depend on it), as an array
+ * tuning[PRAY_TUNING_FIELDS]:
+ *   [0] layout     PRAY_LAYOUT_*
+ *   [1] stack_lds  LDS ring entries per lane (4, 5, 6, 8 or 16); deeper entries
+ *                  spill to a per-lane area in HBM
+ *   [2] leaf_ilp   triangle records loaded together per leaf step (1 or 2)
+ *   [3] fetch_min  a persistent wave refills its finished lanes when at least
+ *                  this many are idle (1..64): the queue chunk
+ *   [4] occupancy  percent (1..100) of the waves that fit on the device that
+ *                  the persistent grid launches: the workgroup count
+ * pray_set_tuning: -1 keeps a value.  Calls of one context must not overlap
+ * (traces share its ray counter and stack spill area). */
+#define PRAY_TUNING_FIELDS 5
+PRAY_API int pray_set_tuning(pray_ctx *ctx, int scene_id, const int *tuning);
+PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *tuning);
 
 /* Autotuner (no reference counterpart): times closest-hit traversal of
  * n_rays incoherent rays through the scene (0 = 2^20) over stack depth x
  * leaf ILP x queue chunk, then occupancy for the fastest of those, and
  * keeps the fastest setting for the scene and for its size bucket (triangle
- * counts within a factor of 4).  Later scenes of a tuned bucket take the
- * bucket's setting without timing unless `force` is set.  tuning5
- * (optional) receives {layout, stack_lds, leaf_ilp, fetch_min, occupancy}. */
-PRAY_API int pray_autotune(pray_ctx *ctx, int scene_id, uint32_t n_rays, int force, int *tuning5);
+ * counts within a factor of 4).  Later scenes of a
+ * tuned bucket take the bucket's setting without timing unless `force` is
+ * set.  tuning (optional) receives the setting, PRAY_TUNING_FIELDS ints. */
+PRAY_API int pray_autotune(pray_ctx *ctx, int scene_id, uint32_t n_rays, int force, int *tuning);
 
 /* Closest-hit or any-hit query for n rays (replaces SpatialDatabase::intersect,
  * spatial.h:15, and the shadow tests of pray.cpp:39-44 / :123-133).
@@ -284,6 +285,12 @@ PRAY_API int pray_copy_device(pray_ctx *ctx, void *d_dst, const void *d_src, uin
 PRAY_API int pray_event_record(pray_ctx *ctx, int slot);
 /* Milliseconds between two recorded events (synchronises on the second). */
 PRAY_API int pray_event_elapsed(pray_ctx *ctx, int slot0, int slot1, float *ms);
+/* Kernel timing: with enable = 1, every later trace that is not being
+ * captured into a graph records events around its traversal kernel alone
+ * (not the counter reset before it); pray_last_kernel_ms synchronises and
+ * returns the last such kernel's duration. */
+PRAY_API int pray_kernel_timing(pray_ctx *ctx, int enable);
+PRAY_API int pray_last_kernel_ms(pray_ctx *ctx, float *ms);
 
 #ifdef __cplusplus
 }

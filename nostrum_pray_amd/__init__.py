@@ -23,6 +23,7 @@ CLI = os.path.join(HERE, "bin", "Pray")
 PRAY_OK = 0
 TRACE_CLOSEST, TRACE_ANY, TRACE_ANY_INCLUSIVE = 0, 1, 2
 LAYOUT_INDEXED, LAYOUT_PACKED = 0, 1
+TUNING_FIELDS = ("layout", "stack_lds", "leaf_ilp", "fetch_min", "occupancy")
 REFERENCE_DET_EPS = 1e-4  # geometry.h:12
 
 ERRORS = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "NOSCENE", -5: "UNSUPPORTED", -6: "NODEVICE"}
@@ -70,8 +71,8 @@ def libraries():
         "pray_upload_scene": (C.c_int, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32,
                                         C.POINTER(C.c_int)]),
         "pray_free_scene": (C.c_int, [_vp, C.c_int]),
-        "pray_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
-        "pray_get_tuning": (C.c_int, [_vp, C.c_int] + [C.POINTER(C.c_int)] * 5),
+        "pray_set_tuning": (C.c_int, [_vp, C.c_int, _vp]),
+        "pray_get_tuning": (C.c_int, [_vp, C.c_int, _vp]),
         "pray_autotune": (C.c_int, [_vp, C.c_int, _u32, C.c_int, _vp]),
         "pray_render_pt_batch": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp, _f32, _vp, _vp,
                                            C.POINTER(_f32)]),
@@ -105,6 +106,8 @@ def libraries():
                                            _u32, _u32, _f32, _u32, _vp, _vp, _vp]),
         "pray_render_pt_progressive": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u64, _vp,
                                                  _f32, _vp, _vp, _vp]),
+        "pray_kernel_timing": (C.c_int, [_vp, C.c_int]),
+        "pray_last_kernel_ms": (C.c_int, [_vp, C.POINTER(_f32)]),
         "pray_event_record": (C.c_int, [_vp, C.c_int]),
         "pray_event_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_f32)]),
     }
@@ -413,19 +416,28 @@ class Context:
         return float(ms.value)
 
     def set_tuning(self, sid, layout=-1, stack_lds=-1, leaf_ilp=-1, fetch_min=-1, occupancy=-1):
-        self._check(self._lib.pray_set_tuning(self._h, sid, layout, stack_lds, leaf_ilp, fetch_min, occupancy))
+        v = np.array([layout, stack_lds, leaf_ilp, fetch_min, occupancy], np.int32)
+        self._check(self._lib.pray_set_tuning(self._h, sid, _p(v)))
 
     def get_tuning(self, sid):
-        v = [C.c_int() for _ in range(5)]
-        self._check(self._lib.pray_get_tuning(self._h, sid, *[C.byref(x) for x in v]))
-        return {"layout": v[0].value, "stack_lds": v[1].value, "leaf_ilp": v[2].value, "fetch_min": v[3].value,
-                "occupancy": v[4].value}
+        v = np.zeros(len(TUNING_FIELDS), np.int32)
+        self._check(self._lib.pray_get_tuning(self._h, sid, _p(v)))
+        return dict(zip(TUNING_FIELDS, (int(x) for x in v)))
 
     def autotune(self, sid, n_rays=0, force=False):
         """Autotune the scene's traversal (or take its size bucket's setting)."""
-        out = np.zeros(5, np.int32)
+        out = np.zeros(len(TUNING_FIELDS), np.int32)
         self._check(self._lib.pray_autotune(self._h, sid, n_rays, 1 if force else 0, _p(out)))
-        return dict(zip(("layout", "stack_lds", "leaf_ilp", "fetch_min", "occupancy"), (int(x) for x in out)))
+        return dict(zip(TUNING_FIELDS, (int(x) for x in out)))
+
+    def kernel_timing(self, enable=True):
+        """Time the traversal kernel of later traces (pray_kernel_timing)."""
+        self._check(self._lib.pray_kernel_timing(self._h, 1 if enable else 0))
+
+    def last_kernel_ms(self):
+        ms = _f32()
+        self._check(self._lib.pray_last_kernel_ms(self._h, C.byref(ms)))
+        return ms.value
 
     def last_render_rays(self):
         r = _u64()

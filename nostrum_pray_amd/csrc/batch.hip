@@ -50,11 +50,14 @@ using namespace prc;
 
 extern "C" {
 
-int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tuning5) {
+int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tuning) {
     if (!c) return PRAY_ERR_INVALID;
     SceneRec *sc = find_scene(c, scene_id);
     if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
-    if (sc->d.n_tris == 0) return PRAY_OK;
+    if (sc->d.n_tris == 0) {
+        if (tuning) tuning_values(sc->tune, tuning);
+        return PRAY_OK;
+    }
     HIPC(c, hipSetDevice(c->device));
     const int bucket = size_bucket(sc->d.n_tris);
     auto it = c->tuned.find(bucket);
@@ -91,13 +94,13 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
             t.occupancy = occ;
             consider(t);
         }
+
         HIPC(c, hipGetLastError());
         c->tuned[bucket] = best;
         sc->tune = best;
     }
-    if (tuning5) {
-        tuning5[0] = sc->tune.layout, tuning5[1] = sc->tune.stack_lds, tuning5[2] = sc->tune.leaf_ilp;
-        tuning5[3] = sc->tune.fetch_min, tuning5[4] = sc->tune.occupancy;
+    if (tuning) {
+        tuning_values(sc->tune, tuning);
     }
     return PRAY_OK;
 }

@@ -421,7 +421,7 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
         C.lo[k] = lo;
         C.inv[k] = ext > 0.f ? 1.f / ext : 0.f;
     }
-    int rc = prepare_trace(c, *sc, npix);
+    int rc = prepare_trace(c, *sc, std::max<uint64_t>(npix, n_train));
     if (rc) return rc;
     if (!c->side) HIPC(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     if (!c->fork_ev) HIPC(c, hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));

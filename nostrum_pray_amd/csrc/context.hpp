@@ -43,6 +43,9 @@ struct pray_ctx {
     int next_id = 1;
     hipEvent_t events[16] = {};   // pray_event_record slots
     hipEvent_t ievents[4] = {};   // internal timing (autotuner, batch)
+    hipEvent_t kevents[2] = {};   // around the last traversal kernel (pray_kernel_timing)
+    bool kernel_timing = false;
+    bool kernel_timed = false;    // kevents hold a measurement
     hipStream_t side = nullptr;   // second stream for work overlapped inside graphs
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     uint64_t last_rays = 0;       // rays traced by the last render call
@@ -78,6 +81,8 @@ struct DBuf {
 
 SceneRec *find_scene(pray_ctx *c, int id);
 int ensure_spill(pray_ctx *c, size_t bytes);
+// {layout, stack_lds, leaf_ilp, fetch_min, occupancy}
+void tuning_values(const Tuning &t, int *v);
 // Enqueues one trace over n rays (or *n_dev rays when n_dev is set, with n as
 // the launch bound).
 int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n, const uint32_t *n_dev, int mode,

@@ -101,17 +101,29 @@ int fail(pray_ctx *c, int code, const char *fmt, ...) {
 
 typedef void (*TraceKernel)(DevScene, TraceArgs);
 
+// LDS ring sizes: 12 bytes per entry and lane, so a 64-lane block takes
+// 768*K bytes; with 160 KB of LDS per CU, K = 6 is the largest ring that
+// leaves room for 32 resident waves (the VGPR limit of the ILP-2 kernels)
+// and K = 5 for 36 (the ILP-1 kernels).
 template <bool ANY, int LAYOUT>
 TraceKernel pick_kernel_kl(int K, int ilp) {
-    if (K == 4) return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 4, 2> : prt::k_trace<ANY, LAYOUT, 4, 1>;
-    if (K == 16) return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 16, 2> : prt::k_trace<ANY, LAYOUT, 16, 1>;
-    return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 8, 2> : prt::k_trace<ANY, LAYOUT, 8, 1>;
+    switch (K) {
+    case 4: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 4, 2> : prt::k_trace<ANY, LAYOUT, 4, 1>;
+    case 5: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 5, 2> : prt::k_trace<ANY, LAYOUT, 5, 1>;
+    case 6: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 6, 2> : prt::k_trace<ANY, LAYOUT, 6, 1>;
+    case 16: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 16, 2> : prt::k_trace<ANY, LAYOUT, 16, 1>;
+    default: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 8, 2> : prt::k_trace<ANY, LAYOUT, 8, 1>;
+    }
 }
 
 TraceKernel pick_kernel_multi(int K, int ilp) {
-    if (K == 4) return ilp == 2 ? prt::k_trace<false, 1, 4, 2, true> : prt::k_trace<false, 1, 4, 1, true>;
-    if (K == 16) return ilp == 2 ? prt::k_trace<false, 1, 16, 2, true> : prt::k_trace<false, 1, 16, 1, true>;
-    return ilp == 2 ? prt::k_trace<false, 1, 8, 2, true> : prt::k_trace<false, 1, 8, 1, true>;
+    switch (K) {
+    case 4: return ilp == 2 ? prt::k_trace<false, 1, 4, 2, true> : prt::k_trace<false, 1, 4, 1, true>;
+    case 5: return ilp == 2 ? prt::k_trace<false, 1, 5, 2, true> : prt::k_trace<false, 1, 5, 1, true>;
+    case 6: return ilp == 2 ? prt::k_trace<false, 1, 6, 2, true> : prt::k_trace<false, 1, 6, 1, true>;
+    case 16: return ilp == 2 ? prt::k_trace<false, 1, 16, 2, true> : prt::k_trace<false, 1, 16, 1, true>;
+    default: return ilp == 2 ? prt::k_trace<false, 1, 8, 2, true> : prt::k_trace<false, 1, 8, 1, true>;
+    }
 }
 
 TraceKernel pick_kernel(bool any, int layout, int K, int ilp) {
@@ -190,7 +202,18 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
     a.spill_depth = depth;
     a.fetch_min = (uint32_t)tu.fetch_min;
     hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, c->work, grid * (uint32_t)WAVE);
+    bool timed = false;
+    if (c->kernel_timing) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPC(c, hipStreamIsCapturing(st, &cs));
+        timed = cs == hipStreamCaptureStatusNone;
+    }
+    if (timed) HIPC(c, hipEventRecord(c->kevents[0], st));
     hipLaunchKernelGGL(k, dim3(grid), dim3(WAVE), 0, st, sc.d, a);
+    if (timed) {
+        HIPC(c, hipEventRecord(c->kevents[1], st));
+        c->kernel_timed = true;
+    }
     HIPC(c, hipGetLastError());
     return PRAY_OK;
 }
@@ -295,6 +318,10 @@ int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n) {
                                sizeof(uint4));
 }
 
+void tuning_values(const Tuning &t, int *v) {
+    v[0] = t.layout, v[1] = t.stack_lds, v[2] = t.leaf_ilp, v[3] = t.fetch_min, v[4] = t.occupancy;
+}
+
 }  // namespace prc
 
 using namespace prc;
@@ -341,6 +368,8 @@ void pray_destroy(pray_ctx *c) {
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->work) (void)hipFree(c->work);
+    for (hipEvent_t e : c->kevents)
+        if (e) (void)hipEventDestroy(e);
     if (c->spill) (void)hipFree(c->spill);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -446,37 +475,36 @@ int pray_free_scene(pray_ctx *c, int scene_id) {
     return PRAY_OK;
 }
 
-int pray_set_tuning(pray_ctx *c, int scene_id, int layout, int stack_lds, int leaf_ilp, int fetch_min, int occupancy) {
+int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
     if (!c) return PRAY_ERR_INVALID;
     SceneRec *sc = find_scene(c, scene_id);
     if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    if (!v) return fail(c, PRAY_ERR_INVALID, "null tuning");
+    const int layout = v[0], stack_lds = v[1], leaf_ilp = v[2], fetch_min = v[3], occupancy = v[4];
     if (layout >= 0 && layout != PRAY_LAYOUT_INDEXED && layout != PRAY_LAYOUT_PACKED)
         return fail(c, PRAY_ERR_INVALID, "unknown layout %d", layout);
-    if (stack_lds >= 0 && stack_lds != 4 && stack_lds != 8 && stack_lds != 16)
-        return fail(c, PRAY_ERR_INVALID, "stack_lds must be 4, 8 or 16");
+    if (stack_lds >= 0 && stack_lds != 4 && stack_lds != 5 && stack_lds != 6 && stack_lds != 8 && stack_lds != 16)
+        return fail(c, PRAY_ERR_INVALID, "stack_lds must be 4, 5, 6, 8 or 16");
     if (leaf_ilp >= 0 && leaf_ilp != 1 && leaf_ilp != 2) return fail(c, PRAY_ERR_INVALID, "leaf_ilp must be 1 or 2");
     if (fetch_min >= 0 && (fetch_min < 1 || fetch_min > WAVE))
         return fail(c, PRAY_ERR_INVALID, "fetch_min must be in 1..64");
     if (occupancy >= 0 && (occupancy < 1 || occupancy > 100))
         return fail(c, PRAY_ERR_INVALID, "occupancy must be in 1..100");
-    if (occupancy >= 0) sc->tune.occupancy = occupancy;
-    if (layout >= 0) sc->tune.layout = layout;
-    if (stack_lds >= 0) sc->tune.stack_lds = stack_lds;
-    if (leaf_ilp >= 0) sc->tune.leaf_ilp = leaf_ilp;
-    if (fetch_min >= 0) sc->tune.fetch_min = fetch_min;
+    Tuning &t = sc->tune;
+    if (layout >= 0) t.layout = layout;
+    if (stack_lds >= 0) t.stack_lds = stack_lds;
+    if (leaf_ilp >= 0) t.leaf_ilp = leaf_ilp;
+    if (fetch_min >= 0) t.fetch_min = fetch_min;
+    if (occupancy >= 0) t.occupancy = occupancy;
     return PRAY_OK;
 }
 
-int pray_get_tuning(pray_ctx *c, int scene_id, int *layout, int *stack_lds, int *leaf_ilp, int *fetch_min,
-                    int *occupancy) {
+int pray_get_tuning(pray_ctx *c, int scene_id, int *v) {
     if (!c) return PRAY_ERR_INVALID;
     SceneRec *sc = find_scene(c, scene_id);
     if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
-    if (layout) *layout = sc->tune.layout;
-    if (stack_lds) *stack_lds = sc->tune.stack_lds;
-    if (leaf_ilp) *leaf_ilp = sc->tune.leaf_ilp;
-    if (fetch_min) *fetch_min = sc->tune.fetch_min;
-    if (occupancy) *occupancy = sc->tune.occupancy;
+    if (!v) return fail(c, PRAY_ERR_INVALID, "null tuning");
+    tuning_values(sc->tune, v);
     return PRAY_OK;
 }
 
@@ -854,6 +882,24 @@ int pray_event_record(pray_ctx *c, int slot) {
     HIPC(c, hipSetDevice(c->device));
     if (!c->events[slot]) HIPC(c, hipEventCreate(&c->events[slot]));
     HIPC(c, hipEventRecord(c->events[slot], c->stream));
+    return PRAY_OK;
+}
+
+int pray_kernel_timing(pray_ctx *c, int enable) {
+    if (!c) return PRAY_ERR_INVALID;
+    HIPC(c, hipSetDevice(c->device));
+    for (int k = 0; k < 2; k++)
+        if (!c->kevents[k]) HIPC(c, hipEventCreate(&c->kevents[k]));
+    c->kernel_timing = enable != 0;
+    return PRAY_OK;
+}
+
+int pray_last_kernel_ms(pray_ctx *c, float *ms) {
+    if (!c || !ms) return fail(c, PRAY_ERR_INVALID, "null argument");
+    if (!c->kernel_timed) return fail(c, PRAY_ERR_INVALID, "no traversal kernel has been timed");
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipEventSynchronize(c->kevents[1]));
+    HIPC(c, hipEventElapsedTime(ms, c->kevents[0], c->kevents[1]));
     return PRAY_OK;
 }
 

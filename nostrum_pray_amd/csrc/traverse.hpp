@@ -108,7 +108,7 @@ __device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, 
 // when it picks the ray up.
 template <bool ANY, int LAYOUT, int K, int ILP, bool MULTI = false>
 static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) {
-    static_assert((K & (K - 1)) == 0, "K must be a power of two");
+    static_assert(K >= 2 && K <= 16, "LDS ring of 2..16 entries");
     static_assert(!MULTI || LAYOUT == 1, "multi-scene launches use the packed layout");
     __shared__ uint32_t st_w[K * WAVE];
     __shared__ float st_lo[K * WAVE];
@@ -223,11 +223,11 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
             uint32_t nw = wn;
             if (gn && gf) {
                 if (sp - base == K) {  // ring full: spill the oldest entry
-                    const uint32_t at = (uint32_t)(base & (K - 1)) * WAVE + lane;
+                    const uint32_t at = ((uint32_t)base % K) * WAVE + lane;
                     spill[base] = make_uint4(st_w[at], __float_as_uint(st_lo[at]), __float_as_uint(st_hi[at]), 0u);
                     ++base;
                 }
-                const uint32_t at = (uint32_t)(sp & (K - 1)) * WAVE + lane;
+                const uint32_t at = ((uint32_t)sp % K) * WAVE + lane;
                 st_w[at] = wf;
                 st_lo[at] = f_lo;
                 st_hi[at] = hi;
@@ -302,7 +302,7 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
                 uint32_t ew;
                 float elo, ehi;
                 if (sp >= base) {
-                    const uint32_t at = (uint32_t)(sp & (K - 1)) * WAVE + lane;
+                    const uint32_t at = ((uint32_t)sp % K) * WAVE + lane;
                     ew = st_w[at];
                     elo = st_lo[at];
                     ehi = st_hi[at];

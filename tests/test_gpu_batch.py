@@ -58,7 +58,8 @@ def test_autotune_never_changes_hits(ctx):
         assert tun["stack_lds"] in (4, 8, 16) and tun["leaf_ilp"] in (1, 2) and 1 <= tun["occupancy"] <= 100
         assert ctx.get_tuning(sid) == tun
         settings = [dict(), dict(stack_lds=4), dict(stack_lds=16, leaf_ilp=1), dict(fetch_min=16, occupancy=50),
-                    dict(fetch_min=64, occupancy=75), dict(layout=P.LAYOUT_INDEXED)]
+                    dict(fetch_min=64, occupancy=75), dict(layout=P.LAYOUT_INDEXED), dict(stack_lds=6),
+                    dict(stack_lds=5, occupancy=10)]
         for st in settings:
             say(f"autotune: trace with {st}")
             ctx.set_tuning(sid, **st)

@@ -36,13 +36,17 @@ def scene(name):
     return SCENES[name]
 
 
+# (layout, LDS ring entries): every traversal setting gives the same hits
+SETTINGS = [(P.LAYOUT_PACKED, 8), (P.LAYOUT_PACKED, 5), (P.LAYOUT_INDEXED, 8), (P.LAYOUT_INDEXED, 6)]
+
+
 def check_closest(ctx, name, rays, eps, bf_rows=0):
     s, t, flat = scene(name)
     sid = ctx.upload(t)
     try:
-        for layout in (P.LAYOUT_PACKED, P.LAYOUT_INDEXED):
-            ctx.set_tuning(sid, layout=layout)
-            say(f"closest {name} layout={layout} n={len(rays)} eps={eps}")
+        for layout, ring in SETTINGS:
+            ctx.set_tuning(sid, layout=layout, stack_lds=ring)
+            say(f"closest {name} layout={layout} stack_lds={ring} n={len(rays)} eps={eps}")
             gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, eps)
             oi, ot = O.flat_trace(flat, rays, 0, eps)
             assert first_diff(gi, oi) is None, first_diff(gi, oi)
@@ -92,9 +96,9 @@ def test_any_hit(ctx, name, eps, mode):
     sid = ctx.upload(t)
     try:
         res = []
-        for layout in (P.LAYOUT_PACKED, P.LAYOUT_INDEXED):
-            ctx.set_tuning(sid, layout=layout)
-            say(f"any {name} mode={mode} layout={layout}")
+        for layout, ring in SETTINGS:
+            ctx.set_tuning(sid, layout=layout, stack_lds=ring)
+            say(f"any {name} mode={mode} layout={layout} stack_lds={ring}")
             res.append(ctx.trace(sid, rays, mode, eps, limit2=lim))
     finally:
         ctx.free_scene(sid)
@@ -112,6 +116,7 @@ def test_any_hit(ctx, name, eps, mode):
 
 
 def test_work_counts_match_cpu_twin(ctx):
+    """Per-ray node and triangle counts equal the CPU twin's."""
     _, t, flat = scene("terrain256-sah")
     rays = O.gen_rays(3, P.scene_box_expanded(flat["box"]), 1 << 14)
     sid = ctx.upload(t)
@@ -131,6 +136,32 @@ def test_work_counts_match_cpu_twin(ctx):
     oi, ot, oc = O.flat_trace(flat, rays, 0, 1e-12, counts=True)
     assert np.array_equal(gi, oi)
     assert np.array_equal(gc, oc)
+
+
+@pytest.mark.parametrize("ring", [5, 16])
+def test_launch_sizes(ctx, ring):
+    """Traces over ray counts around the wave and the first static batch,
+    one after another through the same counter and spill area, including a
+    ray without direction (its hit is not compared) and one far outside the
+    scene box."""
+    _, t, flat = scene("spheres-sah")
+    sid = ctx.upload(t)
+    try:
+        ctx.set_tuning(sid, stack_lds=ring, fetch_min=1 if ring == 5 else 64)
+        base = O.gen_rays(12, P.scene_box_expanded(flat["box"]), 9000)
+        base[5, 3:] = 0.0
+        base[7, :3] = [1e30, -1e30, 0.0]
+        keep = np.arange(len(base)) != 5
+        for n in (1, 63, 64, 65, 4095, 4096, 4097, 9000, 100):
+            rays = np.ascontiguousarray(base[:n])
+            say(f"launch ring={ring} n={n}")
+            gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, 1e-4)
+            oi, ot = O.flat_trace(flat, rays, 0, 1e-4)
+            k = keep[:n]
+            assert first_diff(gi[k], oi[k]) is None, (n, first_diff(gi[k], oi[k]))
+            assert np.array_equal(gt[k].view(np.uint32), ot[k].view(np.uint32))
+    finally:
+        ctx.free_scene(sid)
 
 
 def test_gen_rays_bitwise(ctx):
