@@ -61,6 +61,8 @@ def main():
                          "layout,stack_lds,leaf_ilp,fetch_min,occupancy")
     ap.add_argument("--no-ref-tree", action="store_true",
                     help="skip timing the same rays over the tree built with the reference's parameters")
+    ap.add_argument("--no-gpu-build", action="store_true",
+                    help="skip the GPU tree build and the trace over its tree")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=1 << 24, help="CPU baseline sample (prefix of the stream)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -87,7 +89,9 @@ def main():
     t0 = time.time()
     scene = P.Scene.terrain(args.grid, args.seed)
     n_tris = scene.num_tris
+    t_build = time.time()
     tree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+    cpu_build_s = time.time() - t_build
     tstats = tree.stats()
     tree_tag = f"sah-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}"
     log(f"scene {n_tris} tris, SAH tree {tree_tag} {tstats} in {time.time() - t0:.1f}s")
@@ -180,6 +184,38 @@ def main():
         with open(pmc) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
+    # The tree built on the GPU (Morton-ordered bounding interval hierarchy):
+    # build time, and the same rays over it (identical hits).
+    gpu_build = None
+    if not args.no_gpu_build:
+        abc, _, _ = scene.triangles()
+        builds = []
+        for _ in range(3):
+            t1 = time.time()
+            gid, bms = ctx.build_scene(abc, leaf_size=args.leaf_size)
+            wall = time.time() - t1
+            builds.append((bms, wall))
+            if len(builds) < 3:
+                ctx.free_scene(gid)
+        del abc
+        lay = ctx.scene_layout(gid)
+        ctx.set_tuning(gid, **tun)
+        ms_g, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=gid)
+        same_g = bool(np.array_equal(closest_idx, d_idx.numpy()) and
+                      np.array_equal(closest_t.view(np.uint32), d_t.numpy().view(np.uint32)))
+        ctx.free_scene(gid)
+        bms = sorted(b[0] for b in builds)[1]
+        gpu_build = {
+            "tree": f"Morton-ordered bounding interval hierarchy, leaf size {args.leaf_size}, depth "
+                    f"{lay['max_depth']}",
+            "build_ms": round(bms, 3), "build_wall_s_with_upload": round(sorted(b[1] for b in builds)[1], 3),
+            "cpu_sah_build_s": round(cpu_build_s, 2),
+            "ms_per_step": round(ms_g, 4), "Mrays_per_s": round(n / (ms_g * 1e-3) / 1e6, 2),
+            "identical_hits": same_g,
+        }
+        log(f"GPU build {bms:.2f} ms (CPU SAH build {cpu_build_s:.2f} s); trace over it {ms_g:.3f} ms, "
+            f"identical hits {same_g}")
+
     # The same rays over the tree built with the reference's parameters
     # (depth cap 20, leaf size 32, SAH costs 5 / 1.5): identical hits, more
     # triangle tests per ray.
@@ -252,6 +288,7 @@ def main():
         },
         "any_hit": {"ms_per_step": round(ms_any, 4), "Mrays_per_s": round(n / (ms_any * 1e-3) / 1e6, 2)},
         "reference_parameter_tree": ref_tree,
+        "gpu_tree_build": gpu_build,
         "radiance_cache": rc_out,
         "cpu_baseline": cpu,
     }

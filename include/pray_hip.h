@@ -89,6 +89,25 @@ PRAY_API int pray_upload_scene(pray_ctx *ctx, const uint32_t *nodes, uint32_t n_
                       const float *root_box, uint32_t max_depth, int *scene_id);
 PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
 
+/* Tree build on the GPU (the reference builds its trees on the CPU,
+ * kd_tree.cpp:25-84, kd_tree_sah.cpp:32-296): a bounding interval hierarchy
+ * over the triangles in the order of the 30-bit Morton code of their box
+ * centres (binary radix tree, Karras 2012), in the traversal's device layout,
+ * built and kept on the device.  Any tree gives the same closest hits.
+ *   tris       n x 9 floats a, b, c (host memory)
+ *   color      n x 3 floats, NULL for a scene without shading data
+ *   emit       n bytes, may be NULL (no emitters)
+ *   leaf_size  subtrees of at most this many triangles become leaves (1..64)
+ *   build_ms   optional: device time of the build (events around its kernels)
+ * The new scene takes the default tuning. */
+PRAY_API int pray_build_scene(pray_ctx *ctx, const float *tris, const float *color, const uint8_t *emit, uint32_t n,
+                              uint32_t leaf_size, int *scene_id, float *build_ms);
+/* A scene's device layout: sizes[4] = {n_nodes, n_refs, n_tris, max_depth};
+ * nodes = n_nodes x 2 words, refs = n_refs words, box = 6 floats (any of the
+ * buffers may be NULL). */
+PRAY_API int pray_scene_layout(pray_ctx *ctx, int scene_id, uint64_t *sizes, uint32_t *nodes, uint32_t *refs,
+                               float *box);
+
 /* Tuning of a scene's traversal (results never depend on it), as an array
  * tuning[PRAY_TUNING_FIELDS]:
  *   [0] layout     PRAY_LAYOUT_*

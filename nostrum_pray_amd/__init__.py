@@ -106,6 +106,8 @@ def libraries():
                                            _u32, _u32, _f32, _u32, _vp, _vp, _vp]),
         "pray_render_pt_progressive": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u64, _vp,
                                                  _f32, _vp, _vp, _vp]),
+        "pray_build_scene": (C.c_int, [_vp, _vp, _vp, _vp, _u32, _u32, C.POINTER(C.c_int), C.POINTER(_f32)]),
+        "pray_scene_layout": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
         "pray_kernel_timing": (C.c_int, [_vp, C.c_int]),
         "pray_last_kernel_ms": (C.c_int, [_vp, C.POINTER(_f32)]),
         "pray_event_record": (C.c_int, [_vp, C.c_int]),
@@ -390,6 +392,30 @@ class Context:
 
     def free_scene(self, sid):
         self._check(self._lib.pray_free_scene(self._h, sid))
+
+    def build_scene(self, abc, color=None, emit=None, leaf_size=8):
+        """Builds the tree of raw triangles on the GPU (pray_build_scene);
+        returns (scene id, device build milliseconds).  With color (n x 3)
+        the scene carries shading data."""
+        abc = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
+        n = abc.shape[0]
+        col = None if color is None else np.ascontiguousarray(color, np.float32).reshape(n, 3)
+        em = None if emit is None else np.ascontiguousarray(emit, np.uint8).reshape(n)
+        sid, ms = C.c_int(-1), _f32()
+        self._check(self._lib.pray_build_scene(self._h, _p(abc), _p(col), _p(em), n, leaf_size, C.byref(sid),
+                                               C.byref(ms)))
+        return sid.value, ms.value
+
+    def scene_layout(self, sid):
+        """A scene's device layout (pray_scene_layout) as {nodes, refs, box,
+        max_depth, n_tris}."""
+        sizes = np.zeros(4, np.uint64)
+        self._check(self._lib.pray_scene_layout(self._h, sid, _p(sizes), None, None, None))
+        nodes = np.empty(2 * int(sizes[0]), np.uint32)
+        refs = np.empty(int(sizes[1]), np.uint32)
+        box = np.empty(6, np.float32)
+        self._check(self._lib.pray_scene_layout(self._h, sid, _p(sizes), _p(nodes), _p(refs), _p(box)))
+        return {"nodes": nodes, "refs": refs, "box": box, "max_depth": int(sizes[3]), "n_tris": int(sizes[2])}
 
     # device memory and timing on the context's stream
     def alloc(self, shape, dtype):

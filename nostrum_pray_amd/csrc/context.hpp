@@ -79,7 +79,13 @@ struct DBuf {
     }
 };
 
+// Deepest tree the traversal's spill area is sized for.
+constexpr uint32_t MAX_TREE_DEPTH = 64;
+
 SceneRec *find_scene(pray_ctx *c, int id);
+// Device allocation owned by a scene record (freed by free_scene_rec).
+int device_alloc(pray_ctx *c, SceneRec &sc, void **p, size_t bytes);
+void free_scene_rec(SceneRec &sc);
 int ensure_spill(pray_ctx *c, size_t bytes);
 // {layout, stack_lds, leaf_ilp, fetch_min, occupancy}
 void tuning_values(const Tuning &t, int *v);

@@ -33,8 +33,6 @@ using prt::WAVE;
 
 namespace {
 
-constexpr uint32_t MAX_TREE_DEPTH = 64;
-
 // Packed per-leaf records from the per-field arrays.
 __global__ void k_pack_records(const uint32_t *refs, uint32_t n_refs, const float *a, const float *ab,
                                const float *ac, float4 *recs) {

@@ -57,6 +57,7 @@ def lib():
             "orc_mlp_forward": (None, [_vp, _vp, _u64, C.c_int, _vp]),
             "orc_mlp_grad": (None, [_vp, _vp, _vp, _u64, C.c_int, _vp, _vp, _vp]),
             "orc_adam": (None, [_vp, _vp, _vp, _vp, _u64, _u32, _f32]),
+            "orc_lbvh_build": (_u32, [_vp, _u64, _u32, _vp, _vp, _vp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -130,6 +131,20 @@ def init_tris(abc):
     ab, ac, nn = (np.empty((n, 3), np.float32) for _ in range(3))
     lib().orc_init_tris(_p(abc), n, _p(ab), _p(ac), _p(nn))
     return ab, ac, nn
+
+
+def lbvh_build(abc, leaf_size):
+    """Twin of pray_build_scene: the flat layout of the GPU-built tree
+    ({nodes, refs, a, ab, ac, box, max_depth}, as Tree.flat())."""
+    abc = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
+    n = abc.shape[0]
+    nodes = np.zeros(4 * max(n, 1), np.uint32)
+    refs = np.zeros(n, np.uint32)
+    box = np.zeros(6, np.float32)
+    depth = lib().orc_lbvh_build(_p(abc), n, leaf_size, _p(nodes), _p(refs), _p(box))
+    ab, ac, _ = init_tris(abc)
+    return {"nodes": nodes if n else nodes[:0], "refs": refs, "a": np.ascontiguousarray(abc[:, :3]), "ab": ab,
+            "ac": ac, "box": box, "max_depth": int(depth)}
 
 
 def mt(ray6, a, ab, ac, eps):

@@ -1099,3 +1099,174 @@ ORC_API void orc_adam(float *params, float *m, float *v, const double *grad, uin
         params[p] = params[p] - lr * mh / (std::sqrt(vh) + 1e-8f);
     }
 }
+
+// ---------------------------------------------------------------------------
+// GPU tree build twin (no reference counterpart: the reference builds on the
+// CPU).  The bounding interval hierarchy over Morton-ordered triangles that
+// pray_build_scene builds, restated top down: the range [f, l] of sorted
+// positions splits after the last position that shares more than the range's
+// common prefix of (code, position) with f.  The radix tree this recursion
+// produces is the one Karras (2012) computes node by node, with the same
+// numbering (a left child covering [f, g] is inner node g, a right child
+// covering [g+1, l] is inner node g+1), so the device layout can be compared
+// word for word.  Codes, boxes and splits follow the GPU kernel's arithmetic
+// (30-bit codes of ((centre - lo) * inv) * 1024 per axis, x in bits 3k+2).
+namespace lbvh {
+
+struct Build {
+    uint64_t n;
+    uint32_t leaf;
+    std::vector<float> lo, hi;        // triangle boxes
+    std::vector<uint32_t> key, val;   // sorted codes and triangle indices
+    std::vector<float> nlo, nhi;      // inner node boxes
+    std::vector<uint32_t> cnt, first, last, height;
+    std::vector<uint32_t> *nodes;     // 2 words per node
+    std::vector<uint8_t> last_flag;
+};
+
+static int delta(const Build &B, int64_t i, int64_t j) {
+    if (j < 0 || j >= (int64_t)B.n) return -1;
+    const uint32_t a = B.key[i], b = B.key[j];
+    if (a != b) return __builtin_clz(a ^ b);
+    return 32 + __builtin_clz((uint32_t)i ^ (uint32_t)j);
+}
+
+static uint32_t spread3(uint32_t v) {
+    uint32_t r = 0;
+    for (int b = 0; b < 10; b++) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
+}
+
+// Box, count and height (with collapsed leaves) of the node covering
+// [f, l]; `id` is its inner-node index when f < l.
+static void rec(Build &B, uint64_t f, uint64_t l, uint64_t id, float blo[3], float bhi[3], uint32_t &count,
+                uint32_t &h) {
+    if (f == l) {
+        const uint32_t t = B.val[f];
+        for (int k = 0; k < 3; k++) blo[k] = B.lo[3 * t + k], bhi[k] = B.hi[3 * t + k];
+        count = 1, h = 0;
+        return;
+    }
+    const int common = delta(B, (int64_t)f, (int64_t)l);
+    uint64_t g = f;
+    while (g + 1 < l && delta(B, (int64_t)f, (int64_t)(g + 1)) > common) ++g;
+    float llo[3], lhi[3], rlo[3], rhi[3];
+    uint32_t lc, lh, rc, rh;
+    rec(B, f, g, g, llo, lhi, lc, lh);
+    rec(B, g + 1, l, g + 1, rlo, rhi, rc, rh);
+    for (int k = 0; k < 3; k++) blo[k] = std::min(llo[k], rlo[k]), bhi[k] = std::max(lhi[k], rhi[k]);
+    count = lc + rc;
+    const uint32_t hl = (f == g || lc <= B.leaf) ? 0u : lh, hr = (g + 1 == l || rc <= B.leaf) ? 0u : rh;
+    h = count <= B.leaf ? 0u : 1u + std::max(hl, hr);
+    for (int k = 0; k < 3; k++) B.nlo[3 * id + k] = blo[k], B.nhi[3 * id + k] = bhi[k];
+    B.cnt[id] = count, B.first[id] = (uint32_t)f, B.last[id] = (uint32_t)l, B.height[id] = h;
+}
+
+static uint32_t axis_of(const Build &B, uint64_t id) {
+    const uint32_t x = B.key[B.first[id]] ^ B.key[B.last[id]];
+    if (x) {
+        const uint32_t m = (31u - (uint32_t)__builtin_clz(x)) % 3u;
+        return m == 2u ? 0u : (m == 1u ? 1u : 2u);
+    }
+    const float ex = B.nhi[3 * id] - B.nlo[3 * id], ey = B.nhi[3 * id + 1] - B.nlo[3 * id + 1],
+                ez = B.nhi[3 * id + 2] - B.nlo[3 * id + 2];
+    return (ex >= ey && ex >= ez) ? 0u : (ey >= ez ? 1u : 2u);
+}
+
+// Node entries of the range [f, l] (inner node `id` when f < l) as a child
+// of a node split along `axis`: {bound, word}.
+static void entry(Build &B, uint64_t f, uint64_t l, uint64_t id, uint32_t axis, bool upper, uint32_t *out) {
+    float b;
+    if (f == l) {
+        const uint32_t t = B.val[f];
+        b = upper ? B.hi[3 * t + axis] : B.lo[3 * t + axis];
+    } else {
+        b = upper ? B.nhi[3 * id + axis] : B.nlo[3 * id + axis];
+    }
+    std::memcpy(&out[0], &b, 4);
+    if (f == l || B.cnt[id] <= B.leaf) {
+        B.last_flag[l] = 1;
+        out[1] = ((uint32_t)f << 2) | 3u;
+    } else {
+        out[1] = ((uint32_t)(2 * (id + 1)) << 2) | axis_of(B, id);
+    }
+}
+
+static void write(Build &B, uint64_t f, uint64_t l, uint64_t id) {
+    if (f == l || B.cnt[id] <= B.leaf) return;  // a leaf: written by its parent
+    const int common = delta(B, (int64_t)f, (int64_t)l);
+    uint64_t g = f;
+    while (g + 1 < l && delta(B, (int64_t)f, (int64_t)(g + 1)) > common) ++g;
+    const uint32_t axis = axis_of(B, id);
+    uint32_t *pair = B.nodes->data() + 4 * (id + 1);
+    entry(B, f, g, g, axis, true, pair);
+    entry(B, g + 1, l, g + 1, axis, false, pair + 2);
+    write(B, f, g, g);
+    write(B, g + 1, l, g + 1);
+}
+
+}  // namespace lbvh
+
+// nodes: 2n entries of 2 words (4n words, zero where unused); refs: n;
+// box: 6; returns the tree depth.
+ORC_API uint32_t orc_lbvh_build(const float *abc, uint64_t n, uint32_t leaf_size, uint32_t *nodes, uint32_t *refs,
+                                float *box) {
+    lbvh::Build B;
+    B.n = n, B.leaf = leaf_size;
+    std::vector<uint32_t> words(4 * std::max<uint64_t>(n, 1), 0u);
+    B.nodes = &words;
+    B.lo.resize(3 * n), B.hi.resize(3 * n);
+    for (uint64_t i = 0; i < n; i++) {
+        const float *p = abc + 9 * i;
+        for (int k = 0; k < 3; k++) {
+            const float x = p[k], y = p[3 + k], z = p[6 + k];
+            B.lo[3 * i + k] = std::min(x, std::min(y, z));
+            B.hi[3 * i + k] = std::max(x, std::max(y, z));
+        }
+    }
+    float bl[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, bh[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint64_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) bl[k] = std::min(bl[k], B.lo[3 * i + k]), bh[k] = std::max(bh[k], B.hi[3 * i + k]);
+    for (int k = 0; k < 3; k++) box[k] = bl[k], box[3 + k] = bh[k];
+    std::vector<uint32_t> code(n);
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t q[3];
+        for (int k = 0; k < 3; k++) {
+            const float ext = bh[k] - bl[k];
+            const float inv = ext > 0.f ? 1.0f / ext : 0.f;
+            const float c = (B.lo[3 * i + k] + B.hi[3 * i + k]) * 0.5f;
+            const float u = ((c - bl[k]) * inv) * 1024.0f;
+            q[k] = (uint32_t)std::fmax(0.f, std::fmin(1023.f, u));
+        }
+        code[i] = (lbvh::spread3(q[0]) << 2) | (lbvh::spread3(q[1]) << 1) | lbvh::spread3(q[2]);
+    }
+    B.val.resize(n);
+    for (uint64_t i = 0; i < n; i++) B.val[i] = (uint32_t)i;
+    std::stable_sort(B.val.begin(), B.val.end(), [&](uint32_t x, uint32_t y) { return code[x] < code[y]; });
+    B.key.resize(n);
+    for (uint64_t i = 0; i < n; i++) B.key[i] = code[B.val[i]];
+    const uint64_t inner = n > 1 ? n - 1 : 1;
+    B.nlo.assign(3 * inner, 0.f), B.nhi.assign(3 * inner, 0.f);
+    B.cnt.assign(inner, 0), B.first.assign(inner, 0), B.last.assign(inner, 0), B.height.assign(inner, 0);
+    B.last_flag.assign(std::max<uint64_t>(n, 1), 0);
+    uint32_t depth = 0;
+    if (n >= 2) {
+        float rlo[3], rhi[3];
+        uint32_t rc, rh;
+        lbvh::rec(B, 0, n - 1, 0, rlo, rhi, rc, rh);
+        depth = B.cnt[0] <= leaf_size ? 0u : B.height[0];
+        if (B.cnt[0] <= leaf_size) {
+            words[3] = 3u;
+            B.last_flag[n - 1] = 1;
+        } else {
+            words[3] = (2u << 2) | lbvh::axis_of(B, 0);
+            lbvh::write(B, 0, n - 1, 0);
+        }
+    } else if (n == 1) {
+        words[3] = 3u;
+        B.last_flag[0] = 1;
+    }
+    std::memcpy(nodes, words.data(), 4 * std::max<uint64_t>(n, 1) * 4);
+    for (uint64_t k = 0; k < n; k++) refs[k] = B.val[k] | (B.last_flag[k] ? 0x80000000u : 0u);
+    return depth;
+}

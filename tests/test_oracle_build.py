@@ -1,0 +1,150 @@
+"""The oracle's twin of the GPU tree build (a bounding interval hierarchy
+over Morton-ordered triangles), checked on the CPU against what the
+construction fixes: codes recomputed independently in numpy, a stable sort,
+radix-tree splits at the highest differing code bit, child clip planes that
+bound their subtrees, every triangle in exactly one leaf, and closest hits
+equal to brute force."""
+import numpy as np
+import pytest
+
+import nostrum_pray_amd as P
+import oracle as O
+
+
+def spread3(v):
+    r = np.zeros_like(v)
+    for b in range(10):
+        r |= ((v >> b) & 1) << (3 * b)
+    return r
+
+
+def morton_codes(abc):
+    """Independent restatement in numpy float32 arithmetic."""
+    t = abc.reshape(-1, 3, 3)
+    lo, hi = t.min(1), t.max(1)
+    blo, bhi = lo.min(0), hi.max(0)
+    ext = (bhi - blo).astype(np.float32)
+    inv = np.where(ext > 0, np.float32(1) / np.where(ext > 0, ext, 1), 0).astype(np.float32)
+    c = ((lo + hi) * np.float32(0.5)).astype(np.float32)
+    u = (((c - blo).astype(np.float32) * inv).astype(np.float32) * np.float32(1024)).astype(np.float32)
+    q = np.clip(u, 0, 1023).astype(np.uint32)
+    return (spread3(q[:, 0]) << 2) | (spread3(q[:, 1]) << 1) | spread3(q[:, 2]), lo, hi
+
+
+def walk(flat):
+    """(leaves as (first, count), inner entries as (pair node index, axis,
+    child entries)) reached from the root."""
+    nodes = flat["nodes"].reshape(-1, 2)
+    refs = flat["refs"]
+    leaves, inner, depth = [], [], 0
+    stack = [(int(nodes[1, 1]), 0)]
+    while stack:
+        w, d = stack.pop()
+        depth = max(depth, d)
+        if w & 3 == 3:
+            first = w >> 2
+            k = first
+            while not refs[k] & 0x80000000:
+                k += 1
+            leaves.append((first, k - first + 1))
+            continue
+        lc, axis = w >> 2, w & 3
+        assert lc % 2 == 0 and lc + 1 < len(nodes)
+        inner.append((lc, axis))
+        stack += [(int(nodes[lc, 1]), d + 1), (int(nodes[lc + 1, 1]), d + 1)]
+    return leaves, inner, depth
+
+
+def subtree_tris(flat, w):
+    nodes = flat["nodes"].reshape(-1, 2)
+    refs = flat["refs"]
+    out, stack = [], [w]
+    while stack:
+        w = stack.pop()
+        if w & 3 == 3:
+            k = w >> 2
+            while True:
+                out.append(int(refs[k] & 0x7FFFFFFF))
+                if refs[k] & 0x80000000:
+                    break
+                k += 1
+        else:
+            lc = w >> 2
+            stack += [int(nodes[lc, 1]), int(nodes[lc + 1, 1])]
+    return np.array(out, np.int64)
+
+
+def duplicates_scene(n):
+    """n copies of one triangle: every code is equal, so every split falls
+    on the sorted positions."""
+    tri = np.array([0.1, 0.2, 0.3, 0.5, 0.2, 0.3, 0.1, 0.6, 0.35], np.float32)
+    return np.tile(tri, (n, 1))
+
+
+SCENES = {
+    "spheres": lambda: P.Scene.spheres(1).triangles()[0],
+    "terrain96": lambda: P.Scene.terrain(96, 3).triangles()[0],
+    "cornell": lambda: P.Scene.cornell(3, 12000).triangles()[0],
+    "random": lambda: P.Scene.random(4, 6000).triangles()[0],
+    "duplicates": lambda: duplicates_scene(1000),
+    "one": lambda: duplicates_scene(1),
+    "two": lambda: P.Scene.random(5, 2000).triangles()[0][:2],
+    "nine": lambda: P.Scene.random(6, 2000).triangles()[0][:9],
+}
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+@pytest.mark.parametrize("leaf_size", [1, 4, 8])
+def test_lbvh_structure(name, leaf_size):
+    abc = np.ascontiguousarray(SCENES[name](), np.float32).reshape(-1, 9)
+    n = abc.shape[0]
+    f = O.lbvh_build(abc, leaf_size)
+    codes, lo, hi = morton_codes(abc)
+    refs = f["refs"]
+    order = refs & 0x7FFFFFFF
+    # the leaf order is the stable sort of the codes
+    assert np.array_equal(order, np.argsort(codes, kind="stable"))
+    assert np.array_equal(f["box"], np.concatenate([lo.min(0), hi.max(0)]))
+    leaves, inner, depth = walk(f)
+    assert depth == f["max_depth"]
+    # every sorted position in exactly one leaf, leaves at most leaf_size
+    cover = np.zeros(n, np.int64)
+    for first, cnt in leaves:
+        cover[first:first + cnt] += 1
+        assert 1 <= cnt and (cnt <= leaf_size or cnt == 1)
+    assert (cover == 1).all()
+    nodes = f["nodes"].reshape(-1, 2)
+    skey = codes[order]
+    for lc, axis in inner:
+        left, right = subtree_tris(f, int(nodes[lc, 1])), subtree_tris(f, int(nodes[lc + 1, 1]))
+        lb = nodes[lc, 0:1].view(np.float32)[0]
+        rb = nodes[lc + 1, 0:1].view(np.float32)[0]
+        # clip planes are the subtrees' exact bounds along the node's axis
+        assert hi[left, axis].max() == lb and lo[right, axis].min() == rb
+        # radix split: left keys all below right keys in sorted order
+        lk, rk = codes[left], codes[right]
+        assert lk.max() <= rk.min()
+        # the split axis is that of the highest differing bit (if codes differ)
+        both = np.concatenate([lk, rk])
+        x = int(both.min()) ^ int(both.max())
+        if x:
+            bit = x.bit_length() - 1
+            assert axis == {2: 0, 1: 1, 0: 2}[bit % 3]
+            assert ((lk >> bit) & 1 == 0).all() and ((rk >> bit) & 1 == 1).all()
+    assert skey.size == n
+
+
+@pytest.mark.parametrize("name", ["spheres", "terrain96", "cornell", "random", "duplicates"])
+def test_lbvh_closest_hits_equal_brute_force(name):
+    abc = np.ascontiguousarray(SCENES[name](), np.float32).reshape(-1, 9)
+    f = O.lbvh_build(abc, 8)
+    box = f["box"]
+    rays = O.gen_rays(17, P.scene_box_expanded(box), 4000)
+    eps = 1e-9 if name != "spheres" else 1e-4
+    gi, gt = O.flat_trace(f, rays, 0, eps)
+    bi, bt = O.bf_closest(f["a"], f["ab"], f["ac"], rays, eps)
+    assert np.array_equal(gi, bi)
+    h = bi >= 0
+    assert np.array_equal(gt[h].view(np.uint32), bt[h].view(np.uint32))
+    if name != "duplicates":
+        assert h.mean() > 0.01
