@@ -192,7 +192,8 @@ def main():
         builds = []
         for _ in range(3):
             t1 = time.time()
-            gid, bms = ctx.build_scene(abc, leaf_size=args.leaf_size)
+            gid, bms = ctx.build_scene(abc, method=P.BUILD_SAH, leaf_size=args.leaf_size, max_depth=args.max_depth,
+                                       traversal_cost=args.traversal_cost)
             wall = time.time() - t1
             builds.append((bms, wall))
             if len(builds) < 3:
@@ -206,8 +207,9 @@ def main():
         ctx.free_scene(gid)
         bms = sorted(b[0] for b in builds)[1]
         gpu_build = {
-            "tree": f"Morton-ordered bounding interval hierarchy, leaf size {args.leaf_size}, depth "
-                    f"{lay['max_depth']}",
+            "tree": f"binned-SAH KD tree built on the GPU (32 bins, leaf size {args.leaf_size}, traversal cost "
+                    f"{args.traversal_cost:g}, depth cap {args.max_depth} -> depth {lay['max_depth']}, "
+                    f"{len(lay['nodes']) // 2} nodes, {len(lay['refs'])} references)",
             "build_ms": round(bms, 3), "build_wall_s_with_upload": round(sorted(b[1] for b in builds)[1], 3),
             "cpu_sah_build_s": round(cpu_build_s, 2),
             "ms_per_step": round(ms_g, 4), "Mrays_per_s": round(n / (ms_g * 1e-3) / 1e6, 2),

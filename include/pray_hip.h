@@ -90,18 +90,29 @@ PRAY_API int pray_upload_scene(pray_ctx *ctx, const uint32_t *nodes, uint32_t n_
 PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
 
 /* Tree build on the GPU (the reference builds its trees on the CPU,
- * kd_tree.cpp:25-84, kd_tree_sah.cpp:32-296): a bounding interval hierarchy
- * over the triangles in the order of the 30-bit Morton code of their box
- * centres (binary radix tree, Karras 2012), in the traversal's device layout,
- * built and kept on the device.  Any tree gives the same closest hits.
+ * kd_tree.cpp:25-84, kd_tree_sah.cpp:32-296), built and kept on the device
+ * in the traversal's layout.  Any tree gives the same closest hits.
+ *   method     PRAY_BUILD_SAH: binned-SAH KD tree built level by level (32
+ *              bins per axis, the reference's SAH cost with its empty-side
+ *              factor; references of straddling triangles are duplicated);
+ *              PRAY_BUILD_MORTON: bounding interval hierarchy over the
+ *              triangles in the order of the 30-bit Morton code of their box
+ *              centres (binary radix tree, Karras 2012): the fastest build,
+ *              a slower tree
  *   tris       n x 9 floats a, b, c (host memory)
  *   color      n x 3 floats, NULL for a scene without shading data
  *   emit       n bytes, may be NULL (no emitters)
- *   leaf_size  subtrees of at most this many triangles become leaves (1..64)
- *   build_ms   optional: device time of the build (events around its kernels)
+ *   leaf_size  nodes of at most this many triangles are not split (1..64)
+ *   max_depth  SAH depth cap (0 = 40, at most 64)
+ *   traversal_cost, intersection_cost  SAH costs (<= 0: 2 and 1)
+ *   build_ms   optional: device time from the uploaded triangles to the
+ *              finished layout
  * The new scene takes the default tuning. */
+#define PRAY_BUILD_MORTON 0
+#define PRAY_BUILD_SAH 1
 PRAY_API int pray_build_scene(pray_ctx *ctx, const float *tris, const float *color, const uint8_t *emit, uint32_t n,
-                              uint32_t leaf_size, int *scene_id, float *build_ms);
+                              int method, uint32_t leaf_size, uint32_t max_depth, float traversal_cost,
+                              float intersection_cost, int *scene_id, float *build_ms);
 /* A scene's device layout: sizes[4] = {n_nodes, n_refs, n_tris, max_depth};
  * nodes = n_nodes x 2 words, refs = n_refs words, box = 6 floats (any of the
  * buffers may be NULL). */

@@ -23,6 +23,7 @@ CLI = os.path.join(HERE, "bin", "Pray")
 PRAY_OK = 0
 TRACE_CLOSEST, TRACE_ANY, TRACE_ANY_INCLUSIVE = 0, 1, 2
 LAYOUT_INDEXED, LAYOUT_PACKED = 0, 1
+BUILD_MORTON, BUILD_SAH = 0, 1
 TUNING_FIELDS = ("layout", "stack_lds", "leaf_ilp", "fetch_min", "occupancy")
 REFERENCE_DET_EPS = 1e-4  # geometry.h:12
 
@@ -106,7 +107,8 @@ def libraries():
                                            _u32, _u32, _f32, _u32, _vp, _vp, _vp]),
         "pray_render_pt_progressive": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u64, _vp,
                                                  _f32, _vp, _vp, _vp]),
-        "pray_build_scene": (C.c_int, [_vp, _vp, _vp, _vp, _u32, _u32, C.POINTER(C.c_int), C.POINTER(_f32)]),
+        "pray_build_scene": (C.c_int, [_vp, _vp, _vp, _vp, _u32, C.c_int, _u32, _u32, _f32, _f32,
+                                       C.POINTER(C.c_int), C.POINTER(_f32)]),
         "pray_scene_layout": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
         "pray_kernel_timing": (C.c_int, [_vp, C.c_int]),
         "pray_last_kernel_ms": (C.c_int, [_vp, C.POINTER(_f32)]),
@@ -393,17 +395,19 @@ class Context:
     def free_scene(self, sid):
         self._check(self._lib.pray_free_scene(self._h, sid))
 
-    def build_scene(self, abc, color=None, emit=None, leaf_size=8):
-        """Builds the tree of raw triangles on the GPU (pray_build_scene);
-        returns (scene id, device build milliseconds).  With color (n x 3)
-        the scene carries shading data."""
+    def build_scene(self, abc, color=None, emit=None, method=None, leaf_size=8, max_depth=0, traversal_cost=0.0,
+                    intersection_cost=0.0):
+        """Builds the tree of raw triangles on the GPU (pray_build_scene,
+        BUILD_SAH by default); returns (scene id, device build
+        milliseconds).  With color (n x 3) the scene carries shading data."""
+        method = BUILD_SAH if method is None else method
         abc = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
         n = abc.shape[0]
         col = None if color is None else np.ascontiguousarray(color, np.float32).reshape(n, 3)
         em = None if emit is None else np.ascontiguousarray(emit, np.uint8).reshape(n)
         sid, ms = C.c_int(-1), _f32()
-        self._check(self._lib.pray_build_scene(self._h, _p(abc), _p(col), _p(em), n, leaf_size, C.byref(sid),
-                                               C.byref(ms)))
+        self._check(self._lib.pray_build_scene(self._h, _p(abc), _p(col), _p(em), n, method, leaf_size, max_depth,
+                                               traversal_cost, intersection_cost, C.byref(sid), C.byref(ms)))
         return sid.value, ms.value
 
     def scene_layout(self, sid):

@@ -30,6 +30,7 @@
 #include <cstring>
 #include <memory>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "../../include/pray_hip.h"
 #include "context.hpp"
@@ -320,18 +321,254 @@ static __global__ void k_write_nodes(Writer W, Up U) {
         make_uint2(__float_as_uint(child_bound(W, U, r, axis, false)), child_word(W, r));
 }
 
-// Leaf references and packed records in sorted order.
-static __global__ void k_records(uint32_t n, const uint32_t *vals, const uint8_t *last_flag, const float *a,
-                                 const float *ab, const float *ac, uint32_t *refs, float4 *recs) {
+// Leaf references in sorted order (triangle | last flag).
+static __global__ void k_morton_refs(uint32_t n, const uint32_t *vals, const uint8_t *last_flag, uint32_t *refs) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const uint32_t t = vals[k];
-    const uint32_t ref = t | (last_flag[k] ? REF_LAST : 0u);
-    refs[k] = ref;
-    const size_t o = 3 * (size_t)t;
+    refs[k] = vals[k] | (last_flag[k] ? REF_LAST : 0u);
+}
+
+// ---------------------------------------------------------------------------
+// Binned-SAH KD tree, built breadth first: one pass of kernels per tree level
+// over all triangle references of that level (references of one node are
+// contiguous, in increasing triangle order).
+//   k_sah_bins      per node and axis, counts of clipped reference boxes
+//                   starting and ending in each of SAH_BINS bins
+//   k_sah_split     per node: the cheapest plane on a bin boundary against
+//                   the leaf cost (the reference's SAH cost with its empty
+//                   -side factor 0.8, kd_tree_sah.cpp:141-166), or a leaf
+//   k_sah_classify  per reference: left if its clipped box starts below the
+//                   plane (or lies in it), right if it ends above it
+//   k_sah_counts    per node: child counts from the scanned flags
+//   k_sah_emit      per node: its own entry in the device layout (leaf run,
+//                   empty leaf, or children pair in BFS order) and the next
+//                   level's nodes
+//   k_sah_scatter   per reference: stable partition into the next level, or
+//                   into the final leaf runs
+// Both children of a split share the plane as their clip (the KD form of the
+// node step).  Everything is deterministic; the oracle's twin reproduces it
+// word for word.
+constexpr int SAH_BINS = 32;
+constexpr int SAH_HIST = 3 * 2 * SAH_BINS;  // per node: axis x {start, end} x bin
+constexpr int SAH_CHUNK = 2048;             // references per block of k_sah_bins
+
+struct LNode {
+    float lo[3], hi[3];     // cell
+    uint32_t begin, count;  // references
+    uint32_t entry;         // index of this node's entry in the device layout
+    float plane;            // clip plane written with the entry (the parent's split)
+};
+
+struct SahParams {
+    uint32_t leaf_size, max_depth;
+    float ct, ci;
+};
+
+__device__ __forceinline__ int sah_bin(float x, float lo, float scale) {
+    return (int)fmaxf(0.f, fminf((float)(SAH_BINS - 1), floorf((x - lo) * scale)));
+}
+
+__device__ __forceinline__ float half_area(float dx, float dy, float dz) { return dx * dy + dy * dz + dz * dx; }
+
+static __global__ __launch_bounds__(256) void k_sah_bins(uint32_t n_refs, const uint32_t *refs,
+                                                         const uint32_t *ref_node, const float *tlo,
+                                                         const float *thi, const LNode *nodes, uint32_t *hist) {
+    __shared__ uint32_t h[SAH_HIST];
+    const uint32_t first = blockIdx.x * SAH_CHUNK;
+    if (first >= n_refs) return;
+    const uint32_t last = min(first + SAH_CHUNK, n_refs) - 1;
+    const bool one_node = ref_node[first] == ref_node[last];  // block-uniform
+    if (one_node) {
+        for (int j = threadIdx.x; j < SAH_HIST; j += 256) h[j] = 0u;
+        __syncthreads();
+    }
+    for (uint32_t i = first + threadIdx.x; i <= last; i += 256) {
+        const uint32_t k = ref_node[i], t = refs[i];
+        const LNode &N = nodes[k];
+        uint32_t *dst = one_node ? h : hist + (size_t)k * SAH_HIST;
+        for (int a = 0; a < 3; a++) {
+            const float ext = N.hi[a] - N.lo[a];
+            const float scale = ext > 0.f ? (float)SAH_BINS / ext : 0.f;
+            const float clo = fmaxf(tlo[3 * (size_t)t + a], N.lo[a]);
+            const float chi = fminf(thi[3 * (size_t)t + a], N.hi[a]);
+            atomicAdd(&dst[a * 2 * SAH_BINS + sah_bin(clo, N.lo[a], scale)], 1u);
+            atomicAdd(&dst[a * 2 * SAH_BINS + SAH_BINS + sah_bin(chi, N.lo[a], scale)], 1u);
+        }
+    }
+    if (one_node) {
+        __syncthreads();
+        uint32_t *g = hist + (size_t)ref_node[first] * SAH_HIST;
+        for (int j = threadIdx.x; j < SAH_HIST; j += 256)
+            if (h[j]) atomicAdd(&g[j], h[j]);
+    }
+}
+
+// axis[k] = split axis or -1 for a leaf; pos[k] = split plane.
+static __global__ void k_sah_split(uint32_t n_nodes, const LNode *nodes, const uint32_t *hist, SahParams P,
+                                   uint32_t level, int *axis, float *pos) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_nodes) return;
+    const LNode N = nodes[k];
+    int best_a = -1;
+    float best_p = 0.f;
+    const float dx = N.hi[0] - N.lo[0], dy = N.hi[1] - N.lo[1], dz = N.hi[2] - N.lo[2];
+    const float sa = half_area(dx, dy, dz);
+    if (N.count > P.leaf_size && level < P.max_depth && sa > 0.f) {
+        const float leaf_cost = P.ci * (float)N.count;
+        float best = leaf_cost;
+        const uint32_t *H = hist + (size_t)k * SAH_HIST;
+        for (int a = 0; a < 3; a++) {
+            const float ext = N.hi[a] - N.lo[a];
+            if (!(ext > 0.f)) continue;
+            uint32_t n_left = 0, ends = 0;
+            for (int b = 1; b < SAH_BINS; b++) {
+                n_left += H[a * 2 * SAH_BINS + b - 1];
+                ends += H[a * 2 * SAH_BINS + SAH_BINS + b - 1];
+                const uint32_t n_right = N.count - ends;
+                const float p = N.lo[a] + ext * ((float)b * (1.0f / (float)SAH_BINS));
+                float l[3] = {dx, dy, dz}, r[3] = {dx, dy, dz};
+                l[a] = p - N.lo[a];
+                r[a] = N.hi[a] - p;
+                const float pl = half_area(l[0], l[1], l[2]) / sa, pr = half_area(r[0], r[1], r[2]) / sa;
+                float cost = P.ct + P.ci * (pl * (float)n_left + pr * (float)n_right);
+                if ((n_left == 0 || n_right == 0) && !(pl == 1.f || pr == 1.f)) cost *= 0.8f;
+                if (cost < best) best = cost, best_a = a, best_p = p;
+            }
+        }
+    }
+    axis[k] = best_a;
+    pos[k] = best_p;
+}
+
+// fl / fr[i] = reference i goes left / right (0 for references of leaves);
+// both arrays have one extra zero entry at n_refs for the scans.
+static __global__ void k_sah_classify(uint32_t n_refs, const uint32_t *refs, const uint32_t *ref_node,
+                                      const float *tlo, const float *thi, const LNode *nodes, const int *axis,
+                                      const float *pos, uint32_t *fl, uint32_t *fr) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == n_refs) fl[i] = fr[i] = 0u;
+    if (i >= n_refs) return;
+    const uint32_t k = ref_node[i], t = refs[i];
+    const int a = axis[k];
+    if (a < 0) {
+        fl[i] = fr[i] = 0u;
+        return;
+    }
+    const float p = pos[k];
+    const float clo = fmaxf(tlo[3 * (size_t)t + a], nodes[k].lo[a]);
+    const float chi = fminf(thi[3 * (size_t)t + a], nodes[k].hi[a]);
+    fl[i] = (clo < p || (clo == p && chi == p)) ? 1u : 0u;
+    fr[i] = chi > p ? 1u : 0u;
+}
+
+// Per-node values for the node scans: splits (1 per split node), references
+// of the children, references of leaves.
+static __global__ void k_sah_counts(uint32_t n_nodes, const LNode *nodes, const int *axis, const uint32_t *sl,
+                                    const uint32_t *sr, uint32_t *v_split, uint32_t *v_refs, uint32_t *v_leaf) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == n_nodes) v_split[k] = v_refs[k] = v_leaf[k] = 0u;
+    if (k >= n_nodes) return;
+    const LNode N = nodes[k];
+    if (axis[k] < 0) {
+        v_split[k] = 0u, v_refs[k] = 0u, v_leaf[k] = N.count;
+    } else {
+        const uint32_t nl = sl[N.begin + N.count] - sl[N.begin], nr = sr[N.begin + N.count] - sr[N.begin];
+        v_split[k] = 1u, v_refs[k] = nl + nr, v_leaf[k] = 0u;
+    }
+}
+
+struct Emit {
+    uint32_t n_nodes;
+    const LNode *nodes;
+    const int *axis;
+    const float *pos;
+    const uint32_t *sl, *sr;                  // scanned reference flags
+    const uint32_t *s_split, *s_refs, *s_leaf;  // scanned node values
+    uint32_t pair_base, leaf_base;
+    uint2 *out;          // device layout entries
+    LNode *next;         // next level's nodes
+};
+
+static __global__ void k_sah_emit(Emit E) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= E.n_nodes) return;
+    const LNode N = E.nodes[k];
+    const int a = E.axis[k];
+    if (a < 0) {
+        const uint32_t w = N.count ? (((E.leaf_base + E.s_leaf[k]) << 2) | LEAF_TAG) : ((prt::LEAF_EMPTY << 2) | LEAF_TAG);
+        E.out[N.entry] = make_uint2(__float_as_uint(N.plane), w);
+        return;
+    }
+    const uint32_t q = E.pair_base + E.s_split[k];
+    E.out[N.entry] = make_uint2(__float_as_uint(N.plane), ((2u * q) << 2) | (uint32_t)a);
+    const uint32_t nl = E.sl[N.begin + N.count] - E.sl[N.begin];
+    const float p = E.pos[k];
+    LNode L = N, R = N;
+    L.hi[a] = p, R.lo[a] = p;
+    L.begin = E.s_refs[k], L.count = nl, L.entry = 2 * q, L.plane = p;
+    R.begin = E.s_refs[k] + nl;
+    R.count = E.sr[N.begin + N.count] - E.sr[N.begin];
+    R.entry = 2 * q + 1, R.plane = p;
+    E.next[2 * E.s_split[k]] = L;
+    E.next[2 * E.s_split[k] + 1] = R;
+}
+
+struct Scatter {
+    uint32_t n_refs;
+    const uint32_t *refs, *ref_node;
+    const LNode *nodes;
+    const int *axis;
+    const uint32_t *fl, *fr, *sl, *sr;
+    const uint32_t *s_split, *s_refs, *s_leaf;
+    uint32_t leaf_base;
+    uint32_t *next_refs, *next_node;  // next level
+    uint32_t *leaf_refs;              // final leaf runs (triangle | last flag)
+};
+
+static __global__ void k_sah_scatter(Scatter S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n_refs) return;
+    const uint32_t k = S.ref_node[i], t = S.refs[i];
+    const LNode N = S.nodes[k];
+    if (S.axis[k] < 0) {
+        const uint32_t r = i - N.begin;
+        S.leaf_refs[S.leaf_base + S.s_leaf[k] + r] = t | (r + 1 == N.count ? REF_LAST : 0u);
+        return;
+    }
+    const uint32_t nl = S.sl[N.begin + N.count] - S.sl[N.begin];
+    if (S.fl[i]) {
+        const uint32_t j = S.s_refs[k] + (S.sl[i] - S.sl[N.begin]);
+        S.next_refs[j] = t;
+        S.next_node[j] = 2 * S.s_split[k];
+    }
+    if (S.fr[i]) {
+        const uint32_t j = S.s_refs[k] + nl + (S.sr[i] - S.sr[N.begin]);
+        S.next_refs[j] = t;
+        S.next_node[j] = 2 * S.s_split[k] + 1;
+    }
+}
+
+// Packed records of the final leaf runs.
+static __global__ void k_pack_leaf_records(uint32_t n_refs, const uint32_t *refs, const float *a, const float *ab,
+                                           const float *ac, float4 *recs) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_refs) return;
+    const uint32_t ref = refs[k];
+    const size_t o = 3 * (size_t)(ref & ~REF_LAST);
     recs[3 * (size_t)k] = make_float4(a[o], a[o + 1], a[o + 2], __uint_as_float(ref));
     recs[3 * (size_t)k + 1] = make_float4(ab[o], ab[o + 1], ab[o + 2], 0.f);
     recs[3 * (size_t)k + 2] = make_float4(ac[o], ac[o + 1], ac[o + 2], 0.f);
+}
+
+static __global__ void k_sah_root(const float *box, uint32_t n, LNode *nodes, uint32_t *refs, uint32_t *ref_node) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        LNode r;
+        for (int k = 0; k < 3; k++) r.lo[k] = box[k], r.hi[k] = box[3 + k];
+        r.begin = 0, r.count = n, r.entry = 1, r.plane = 0.f;
+        nodes[0] = r;
+    }
+    if (i < n) refs[i] = i, ref_node[i] = 0u;
 }
 
 }  // namespace prb
@@ -354,6 +591,193 @@ struct Scratch {
     }
 };
 
+// A device buffer that grows (by at least half) and optionally keeps its
+// contents; freed on every exit.
+struct Grow {
+    void *p = nullptr;
+    size_t cap = 0;
+    ~Grow() {
+        if (p) (void)hipFree(p);
+    }
+    int reserve(pray_ctx *c, size_t bytes, bool keep) {
+        if (bytes <= cap) return PRAY_OK;
+        const size_t nc = std::max(bytes, cap + cap / 2);
+        void *q = nullptr;
+        HIPC(c, hipMalloc(&q, nc));
+        if (keep && p && cap) HIPC(c, hipMemcpyAsync(q, p, cap, hipMemcpyDeviceToDevice, c->stream));
+        if (p) {
+            HIPC(c, hipStreamSynchronize(c->stream));
+            HIPC(c, hipFree(p));
+        }
+        p = q, cap = nc;
+        return PRAY_OK;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Built {
+    uint32_t n_entries = 0, n_refs = 0, depth = 0;
+};
+
+// Morton-ordered bounding interval hierarchy: entries = 2n node entries,
+// refs = n leaf references.
+int build_morton(pray_ctx *c, hipStream_t st, uint32_t n, uint32_t leaf_size, const float *tlo, const float *thi,
+                 const float *box, Grow &entries, Grow &refs, Built &out) {
+    Scratch S;
+    int rc = PRAY_OK;
+    uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr, *left = nullptr,
+             *right = nullptr, *parent = nullptr, *leaf_parent = nullptr, *first = nullptr, *last = nullptr,
+             *arrivals = nullptr, *count = nullptr, *height = nullptr;
+    float *nlo = nullptr, *nhi = nullptr;
+    uint8_t *last_flag = nullptr;
+    const size_t inner = n > 1 ? n - 1 : 1;
+    if ((rc = S.get(c, &keys, n)) || (rc = S.get(c, &vals, n)) || (rc = S.get(c, &keys2, n)) ||
+        (rc = S.get(c, &vals2, n)) || (rc = S.get(c, &left, inner)) || (rc = S.get(c, &right, inner)) ||
+        (rc = S.get(c, &parent, inner)) || (rc = S.get(c, &leaf_parent, n)) || (rc = S.get(c, &first, inner)) ||
+        (rc = S.get(c, &last, inner)) || (rc = S.get(c, &arrivals, inner)) || (rc = S.get(c, &count, inner)) ||
+        (rc = S.get(c, &height, inner)) || (rc = S.get(c, &nlo, 3 * inner)) || (rc = S.get(c, &nhi, 3 * inner)) ||
+        (rc = S.get(c, &last_flag, n)))
+        return rc;
+    if ((rc = entries.reserve(c, 2 * (size_t)n * 8, false)) || (rc = refs.reserve(c, (size_t)n * 4, false)))
+        return rc;
+    uint2 *nodes = entries.as<uint2>();
+    HIPC(c, hipMemsetAsync(nodes, 0, 2 * (size_t)n * 8, st));
+    HIPC(c, hipMemsetAsync(arrivals, 0, inner * 4, st));
+    HIPC(c, hipMemsetAsync(last_flag, 0, (size_t)n, st));
+    size_t sort_bytes = 0;
+    HIPC(c, rocprim::radix_sort_pairs(nullptr, sort_bytes, keys, keys2, vals, vals2, n, 0u, prb::MORTON_BITS, st));
+    uint8_t *sort_tmp = nullptr;
+    if ((rc = S.get(c, &sort_tmp, sort_bytes))) return rc;
+    const unsigned g = nblocks(n, 256);
+    hipLaunchKernelGGL(prb::k_morton, dim3(g), dim3(256), 0, st, n, tlo, thi, box, keys, vals);
+    HIPC(c, hipGetLastError());
+    HIPC(c, rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys, keys2, vals, vals2, n, 0u, prb::MORTON_BITS, st));
+    prb::Radix R{keys2, n, left, right, parent, leaf_parent, first, last};
+    prb::Up U{n, leaf_size, vals2, tlo, thi, left, right, parent, leaf_parent, arrivals, nlo, nhi, count, height};
+    prb::Writer W{n, leaf_size, keys2, left, right, first, last, count, nlo, nhi, nodes, last_flag};
+    if (n > 1) {
+        hipLaunchKernelGGL(prb::k_radix_tree, dim3(nblocks(n - 1, 256)), dim3(256), 0, st, R);
+        hipLaunchKernelGGL(prb::k_bottom_up, dim3(g), dim3(256), 0, st, U);
+    }
+    hipLaunchKernelGGL(prb::k_write_nodes, dim3(nblocks(inner, 256)), dim3(256), 0, st, W, U);
+    hipLaunchKernelGGL(prb::k_morton_refs, dim3(g), dim3(256), 0, st, n, (const uint32_t *)vals2,
+                       (const uint8_t *)last_flag, refs.as<uint32_t>());
+    HIPC(c, hipGetLastError());
+    uint32_t depth = 0;
+    if (n > 1) HIPC(c, hipMemcpyAsync(&depth, height, 4, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipStreamSynchronize(st));
+    out.n_entries = 2 * n, out.n_refs = n, out.depth = depth;
+    return PRAY_OK;
+}
+
+// Exclusive sum of in[0 .. count) into out with rocPRIM's device scan.
+int scan_u32(pray_ctx *c, hipStream_t st, Grow &tmp, const uint32_t *in, uint32_t *out, size_t count) {
+    size_t bytes = 0;
+    HIPC(c, rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, count, rocprim::plus<uint32_t>(), st));
+    int rc = tmp.reserve(c, std::max<size_t>(bytes, 1), false);
+    if (rc) return rc;
+    HIPC(c, rocprim::exclusive_scan(tmp.p, bytes, in, out, 0u, count, rocprim::plus<uint32_t>(), st));
+    return PRAY_OK;
+}
+
+// Binned-SAH KD tree, one level at a time (kernels in prb).
+int build_sah(pray_ctx *c, hipStream_t st, uint32_t n, const prb::SahParams &P, const float *tlo, const float *thi,
+              const float *box, Grow &entries, Grow &leaf_refs, Built &out) {
+    using prb::LNode;
+    Grow nodes[2], refs[2], rnode[2], fl, fr, sl, sr, axis, pos, hist, vs, vr, vl, ss, sr2, sleaf, tmp;
+    int rc = PRAY_OK;
+    int cur = 0;
+    if ((rc = nodes[0].reserve(c, sizeof(LNode), false)) || (rc = refs[0].reserve(c, (size_t)n * 4, false)) ||
+        (rc = rnode[0].reserve(c, (size_t)n * 4, false)) || (rc = entries.reserve(c, 2 * 8, false)))
+        return rc;
+    HIPC(c, hipMemsetAsync(entries.p, 0, 2 * 8, st));
+    hipLaunchKernelGGL(prb::k_sah_root, dim3(nblocks(n, 256)), dim3(256), 0, st, box, n, nodes[0].as<LNode>(),
+                       refs[0].as<uint32_t>(), rnode[0].as<uint32_t>());
+    uint32_t n_nodes = 1, n_refs = n, pair_base = 1, leaf_base = 0;
+    const uint64_t ref_cap = 16ull * n + 4096;  // duplication beyond this is refused
+    for (uint32_t level = 0;; level++) {
+        if ((rc = hist.reserve(c, (size_t)n_nodes * prb::SAH_HIST * 4, false)) ||
+            (rc = axis.reserve(c, (size_t)n_nodes * 4, false)) || (rc = pos.reserve(c, (size_t)n_nodes * 4, false)) ||
+            (rc = fl.reserve(c, ((size_t)n_refs + 1) * 4, false)) ||
+            (rc = fr.reserve(c, ((size_t)n_refs + 1) * 4, false)) ||
+            (rc = sl.reserve(c, ((size_t)n_refs + 1) * 4, false)) ||
+            (rc = sr.reserve(c, ((size_t)n_refs + 1) * 4, false)) ||
+            (rc = vs.reserve(c, ((size_t)n_nodes + 1) * 4, false)) ||
+            (rc = vr.reserve(c, ((size_t)n_nodes + 1) * 4, false)) ||
+            (rc = vl.reserve(c, ((size_t)n_nodes + 1) * 4, false)) ||
+            (rc = ss.reserve(c, ((size_t)n_nodes + 1) * 4, false)) ||
+            (rc = sr2.reserve(c, ((size_t)n_nodes + 1) * 4, false)) ||
+            (rc = sleaf.reserve(c, ((size_t)n_nodes + 1) * 4, false)))
+            return rc;
+        const LNode *N = nodes[cur].as<LNode>();
+        const uint32_t *R = refs[cur].as<uint32_t>(), *RN = rnode[cur].as<uint32_t>();
+        hipLaunchKernelGGL(prt::k_zero_u32, dim3(nblocks((uint64_t)n_nodes * prb::SAH_HIST, 256)), dim3(256), 0, st,
+                           hist.as<uint32_t>(), n_nodes * (uint32_t)prb::SAH_HIST);
+        if (n_refs)
+            hipLaunchKernelGGL(prb::k_sah_bins, dim3((n_refs + prb::SAH_CHUNK - 1) / prb::SAH_CHUNK), dim3(256), 0,
+                               st, n_refs, R, RN, tlo, thi, N, hist.as<uint32_t>());
+        hipLaunchKernelGGL(prb::k_sah_split, dim3(nblocks(n_nodes, 256)), dim3(256), 0, st, n_nodes, N,
+                           (const uint32_t *)hist.as<uint32_t>(), P, level, axis.as<int>(), pos.as<float>());
+        hipLaunchKernelGGL(prb::k_sah_classify, dim3(nblocks((uint64_t)n_refs + 1, 256)), dim3(256), 0, st, n_refs, R,
+                           RN, tlo, thi, N, (const int *)axis.as<int>(), (const float *)pos.as<float>(),
+                           fl.as<uint32_t>(), fr.as<uint32_t>());
+        HIPC(c, hipGetLastError());
+        if ((rc = scan_u32(c, st, tmp, fl.as<uint32_t>(), sl.as<uint32_t>(), (size_t)n_refs + 1)) ||
+            (rc = scan_u32(c, st, tmp, fr.as<uint32_t>(), sr.as<uint32_t>(), (size_t)n_refs + 1)))
+            return rc;
+        hipLaunchKernelGGL(prb::k_sah_counts, dim3(nblocks((uint64_t)n_nodes + 1, 256)), dim3(256), 0, st, n_nodes, N,
+                           (const int *)axis.as<int>(), (const uint32_t *)sl.as<uint32_t>(),
+                           (const uint32_t *)sr.as<uint32_t>(), vs.as<uint32_t>(), vr.as<uint32_t>(),
+                           vl.as<uint32_t>());
+        HIPC(c, hipGetLastError());
+        if ((rc = scan_u32(c, st, tmp, vs.as<uint32_t>(), ss.as<uint32_t>(), (size_t)n_nodes + 1)) ||
+            (rc = scan_u32(c, st, tmp, vr.as<uint32_t>(), sr2.as<uint32_t>(), (size_t)n_nodes + 1)) ||
+            (rc = scan_u32(c, st, tmp, vl.as<uint32_t>(), sleaf.as<uint32_t>(), (size_t)n_nodes + 1)))
+            return rc;
+        uint32_t tot[3];
+        HIPC(c, hipMemcpyAsync(&tot[0], ss.as<uint32_t>() + n_nodes, 4, hipMemcpyDeviceToHost, st));
+        HIPC(c, hipMemcpyAsync(&tot[1], sr2.as<uint32_t>() + n_nodes, 4, hipMemcpyDeviceToHost, st));
+        HIPC(c, hipMemcpyAsync(&tot[2], sleaf.as<uint32_t>() + n_nodes, 4, hipMemcpyDeviceToHost, st));
+        HIPC(c, hipStreamSynchronize(st));
+        const uint32_t n_split = tot[0], next_refs = tot[1], level_leaf = tot[2];
+        if ((uint64_t)next_refs + leaf_base + level_leaf > ref_cap)
+            return fail(c, PRAY_ERR_UNSUPPORTED, "SAH build: triangle references exceed 16 per triangle");
+        const int nxt = cur ^ 1;
+        if ((rc = entries.reserve(c, 2 * ((size_t)pair_base + n_split) * 8, true)) ||
+            (rc = leaf_refs.reserve(c, ((size_t)leaf_base + level_leaf) * 4, true)) ||
+            (rc = nodes[nxt].reserve(c, std::max<size_t>(2 * (size_t)n_split, 1) * sizeof(LNode), false)) ||
+            (rc = refs[nxt].reserve(c, std::max<size_t>(next_refs, 1) * 4, false)) ||
+            (rc = rnode[nxt].reserve(c, std::max<size_t>(next_refs, 1) * 4, false)))
+            return rc;
+        prb::Emit E{n_nodes, N, (const int *)axis.as<int>(), (const float *)pos.as<float>(),
+                    (const uint32_t *)sl.as<uint32_t>(), (const uint32_t *)sr.as<uint32_t>(),
+                    (const uint32_t *)ss.as<uint32_t>(), (const uint32_t *)sr2.as<uint32_t>(),
+                    (const uint32_t *)sleaf.as<uint32_t>(), pair_base, leaf_base, entries.as<uint2>(),
+                    nodes[nxt].as<LNode>()};
+        hipLaunchKernelGGL(prb::k_sah_emit, dim3(nblocks(n_nodes, 256)), dim3(256), 0, st, E);
+        prb::Scatter Sc{n_refs, R, RN, N, (const int *)axis.as<int>(), (const uint32_t *)fl.as<uint32_t>(),
+                        (const uint32_t *)fr.as<uint32_t>(), (const uint32_t *)sl.as<uint32_t>(),
+                        (const uint32_t *)sr.as<uint32_t>(), (const uint32_t *)ss.as<uint32_t>(),
+                        (const uint32_t *)sr2.as<uint32_t>(), (const uint32_t *)sleaf.as<uint32_t>(), leaf_base,
+                        refs[nxt].as<uint32_t>(), rnode[nxt].as<uint32_t>(), leaf_refs.as<uint32_t>()};
+        if (n_refs) hipLaunchKernelGGL(prb::k_sah_scatter, dim3(nblocks(n_refs, 256)), dim3(256), 0, st, Sc);
+        HIPC(c, hipGetLastError());
+        leaf_base += level_leaf;
+        pair_base += n_split;
+        if (n_split == 0) {
+            out.depth = level;
+            break;
+        }
+        cur = nxt;
+        n_nodes = 2 * n_split;
+        n_refs = next_refs;
+    }
+    HIPC(c, hipStreamSynchronize(st));
+    out.n_entries = 2 * pair_base;
+    out.n_refs = leaf_base;
+    return PRAY_OK;
+}
+
 }  // namespace
 }  // namespace prc
 
@@ -361,13 +785,20 @@ using namespace prc;
 
 extern "C" {
 
-int pray_build_scene(pray_ctx *c, const float *tris, const float *color, const uint8_t *emit, uint32_t n,
-                     uint32_t leaf_size, int *scene_id, float *build_ms) {
+int pray_build_scene(pray_ctx *c, const float *tris, const float *color, const uint8_t *emit, uint32_t n, int method,
+                     uint32_t leaf_size, uint32_t max_depth, float traversal_cost, float intersection_cost,
+                     int *scene_id, float *build_ms) {
     if (!c || !scene_id) return fail(c, PRAY_ERR_INVALID, "null argument");
     *scene_id = -1;
     if (n && !tris) return fail(c, PRAY_ERR_INVALID, "null triangles");
+    if (method != PRAY_BUILD_MORTON && method != PRAY_BUILD_SAH)
+        return fail(c, PRAY_ERR_INVALID, "unknown build method %d", method);
     if (leaf_size < 1 || leaf_size > 64) return fail(c, PRAY_ERR_INVALID, "leaf_size must be in 1..64");
+    if (max_depth == 0) max_depth = 40;
+    if (max_depth > MAX_TREE_DEPTH) return fail(c, PRAY_ERR_INVALID, "max_depth must be at most %u", MAX_TREE_DEPTH);
     if (n > (1u << 28)) return fail(c, PRAY_ERR_INVALID, "at most 2^28 triangles");
+    const prb::SahParams P{leaf_size, max_depth, traversal_cost > 0.f ? traversal_cost : 2.0f,
+                           intersection_cost > 0.f ? intersection_cost : 1.0f};
     HIPC(c, hipSetDevice(c->device));
     hipStream_t st = c->stream;
     auto sc = std::make_unique<SceneRec>();
@@ -390,31 +821,16 @@ int pray_build_scene(pray_ctx *c, const float *tris, const float *color, const u
         if (!c->ievents[k]) HIPC(c, hipEventCreate(&c->ievents[k]));
     int rc = PRAY_OK;
     Scratch S;
-    float *abc = nullptr, *tlo = nullptr, *thi = nullptr, *partial = nullptr, *box = nullptr, *nlo = nullptr,
-          *nhi = nullptr;
-    uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr, *left = nullptr,
-             *right = nullptr, *parent = nullptr, *leaf_parent = nullptr, *first = nullptr, *last = nullptr,
-             *arrivals = nullptr, *count = nullptr, *height = nullptr;
-    uint8_t *last_flag = nullptr;
-    const size_t inner = n > 1 ? n - 1 : 1;
+    float *abc = nullptr, *tlo = nullptr, *thi = nullptr, *partial = nullptr, *box = nullptr;
     if ((rc = S.get(c, &abc, 9 * (size_t)n)) || (rc = S.get(c, &tlo, 3 * (size_t)n)) ||
         (rc = S.get(c, &thi, 3 * (size_t)n)) || (rc = S.get(c, &partial, 6 * (size_t)prb::BOX_PARTIALS)) ||
-        (rc = S.get(c, &box, 6)) || (rc = S.get(c, &keys, n)) || (rc = S.get(c, &vals, n)) ||
-        (rc = S.get(c, &keys2, n)) || (rc = S.get(c, &vals2, n)) || (rc = S.get(c, &left, inner)) ||
-        (rc = S.get(c, &right, inner)) || (rc = S.get(c, &parent, inner)) || (rc = S.get(c, &leaf_parent, n)) ||
-        (rc = S.get(c, &first, inner)) || (rc = S.get(c, &last, inner)) || (rc = S.get(c, &arrivals, inner)) ||
-        (rc = S.get(c, &count, inner)) || (rc = S.get(c, &height, inner)) || (rc = S.get(c, &nlo, 3 * inner)) ||
-        (rc = S.get(c, &nhi, 3 * inner)) || (rc = S.get(c, &last_flag, n)))
+        (rc = S.get(c, &box, 6)))
         return rc;
     // Scene arrays (owned by the scene record).
     void *p = nullptr;
-    const size_t n_nodes = 2 * (size_t)n, n_nodes_pad = (n_nodes + 15) & ~(size_t)15;
     auto own = [&](size_t bytes, void **dst) -> int { return device_alloc(c, *sc, dst, bytes); };
     float *a = nullptr, *ab = nullptr, *ac = nullptr, *normal = nullptr, *col = nullptr;
     uint8_t *em = nullptr;
-    uint2 *nodes = nullptr;
-    uint32_t *refs = nullptr;
-    float4 *recs = nullptr;
     if ((rc = own(12 * (size_t)n, &p))) return rc;
     a = (float *)p;
     if ((rc = own(12 * (size_t)n, &p))) return rc;
@@ -429,13 +845,6 @@ int pray_build_scene(pray_ctx *c, const float *tris, const float *color, const u
         if ((rc = own((size_t)n, &p))) return rc;
         em = (uint8_t *)p;
     }
-    if ((rc = own(n_nodes_pad * 8, &p))) return rc;
-    nodes = (uint2 *)p;
-    if ((rc = own(((size_t)n + 2) * 4, &p))) return rc;
-    refs = (uint32_t *)p;
-    if ((rc = own(((size_t)n + 2) * 48, &p))) return rc;
-    recs = (float4 *)p;
-
     HIPC(c, hipMemcpyAsync(abc, tris, 36 * (size_t)n, hipMemcpyHostToDevice, st));
     if (color) {
         HIPC(c, hipMemcpyAsync(col, color, 12 * (size_t)n, hipMemcpyHostToDevice, st));
@@ -444,18 +853,8 @@ int pray_build_scene(pray_ctx *c, const float *tris, const float *color, const u
         else
             HIPC(c, hipMemsetAsync(em, 0, (size_t)n, st));
     }
-    HIPC(c, hipMemsetAsync(nodes, 0, n_nodes_pad * 8, st));
-    HIPC(c, hipMemsetAsync(refs, 0, ((size_t)n + 2) * 4, st));
-    HIPC(c, hipMemsetAsync(recs, 0, ((size_t)n + 2) * 48, st));
-    HIPC(c, hipMemsetAsync(arrivals, 0, inner * 4, st));
-    HIPC(c, hipMemsetAsync(last_flag, 0, (size_t)n, st));
-    // rocPRIM's scratch size for this count
-    size_t sort_bytes = 0;
-    HIPC(c, rocprim::radix_sort_pairs(nullptr, sort_bytes, keys, keys2, vals, vals2, n, 0u, prb::MORTON_BITS, st));
-    uint8_t *sort_tmp = nullptr;
-    if ((rc = S.get(c, &sort_tmp, sort_bytes))) return rc;
 
-    // ---- timed build ----
+    // ---- timed build: from the uploaded triangles to the device layout ----
     HIPC(c, hipEventRecord(c->ievents[0], st));
     const unsigned g = nblocks(n, 256);
     prb::Tri T{abc, a, ab, ac, normal, tlo, thi};
@@ -464,41 +863,47 @@ int pray_build_scene(pray_ctx *c, const float *tris, const float *color, const u
     hipLaunchKernelGGL(prb::k_box_partial, dim3(nb), dim3(prb::BOX_BLOCK), 0, st, n, (const float *)tlo,
                        (const float *)thi, partial);
     hipLaunchKernelGGL(prb::k_box_final, dim3(1), dim3(prb::BOX_BLOCK), 0, st, nb, (const float *)partial, box);
-    hipLaunchKernelGGL(prb::k_morton, dim3(g), dim3(256), 0, st, n, (const float *)tlo, (const float *)thi,
-                       (const float *)box, keys, vals);
     HIPC(c, hipGetLastError());
-    HIPC(c, rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys, keys2, vals, vals2, n, 0u, prb::MORTON_BITS, st));
-    prb::Radix R{keys2, n, left, right, parent, leaf_parent, first, last};
-    prb::Up U{n, leaf_size, vals2, tlo, thi, left, right, parent, leaf_parent, arrivals, nlo, nhi, count, height};
-    prb::Writer W{n, leaf_size, keys2, left, right, first, last, count, nlo, nhi, nodes, last_flag};
-    if (n > 1) {
-        hipLaunchKernelGGL(prb::k_radix_tree, dim3(nblocks(n - 1, 256)), dim3(256), 0, st, R);
-        hipLaunchKernelGGL(prb::k_bottom_up, dim3(g), dim3(256), 0, st, U);
-    }
-    hipLaunchKernelGGL(prb::k_write_nodes, dim3(nblocks(inner, 256)), dim3(256), 0, st, W, U);
-    hipLaunchKernelGGL(prb::k_records, dim3(g), dim3(256), 0, st, n, (const uint32_t *)vals2,
-                       (const uint8_t *)last_flag, (const float *)a, (const float *)ab, (const float *)ac, refs, recs);
+    Grow entries, refs;
+    Built B;
+    rc = method == PRAY_BUILD_MORTON ? build_morton(c, st, n, leaf_size, tlo, thi, box, entries, refs, B)
+                                     : build_sah(c, st, n, P, tlo, thi, box, entries, refs, B);
+    if (rc) return rc;
+    if (B.depth > MAX_TREE_DEPTH)
+        return fail(c, PRAY_ERR_UNSUPPORTED, "tree depth %u exceeds %u", B.depth, MAX_TREE_DEPTH);
+    // Final arrays: nodes padded to whole 128-byte blocks, references and
+    // records padded by two entries for the leaf look-ahead.
+    const size_t n_pad = ((size_t)B.n_entries + 15) & ~(size_t)15;
+    if ((rc = own(n_pad * 8, &p))) return rc;
+    uint2 *nodes = (uint2 *)p;
+    if ((rc = own(((size_t)B.n_refs + 2) * 4, &p))) return rc;
+    uint32_t *fr = (uint32_t *)p;
+    if ((rc = own(((size_t)B.n_refs + 2) * 48, &p))) return rc;
+    float4 *recs = (float4 *)p;
+    HIPC(c, hipMemsetAsync(nodes, 0, n_pad * 8, st));
+    HIPC(c, hipMemsetAsync(fr, 0, ((size_t)B.n_refs + 2) * 4, st));
+    HIPC(c, hipMemsetAsync(recs, 0, ((size_t)B.n_refs + 2) * 48, st));
+    HIPC(c, hipMemcpyAsync(nodes, entries.p, (size_t)B.n_entries * 8, hipMemcpyDeviceToDevice, st));
+    HIPC(c, hipMemcpyAsync(fr, refs.p, (size_t)B.n_refs * 4, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(prb::k_pack_leaf_records, dim3(nblocks(B.n_refs, 256)), dim3(256), 0, st, B.n_refs,
+                       (const uint32_t *)fr, (const float *)a, (const float *)ab, (const float *)ac, recs);
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(c->ievents[1], st));
     // ---- end of the timed build ----
 
-    uint32_t depth = 0;
     float box_h[6];
-    if (n > 1) HIPC(c, hipMemcpyAsync(&depth, height, 4, hipMemcpyDeviceToHost, st));
     HIPC(c, hipMemcpyAsync(box_h, box, 24, hipMemcpyDeviceToHost, st));
     HIPC(c, hipStreamSynchronize(st));
     if (build_ms) HIPC(c, hipEventElapsedTime(build_ms, c->ievents[0], c->ievents[1]));
-    if (depth > MAX_TREE_DEPTH)
-        return fail(c, PRAY_ERR_UNSUPPORTED, "tree depth %u exceeds %u", depth, MAX_TREE_DEPTH);
     d.nodes = nodes;
-    d.refs = refs;
+    d.refs = fr;
     d.a = a, d.ab = ab, d.ac = ac;
     d.normal = normal, d.color = col, d.emit = em;
     d.recs = recs;
     std::memcpy(d.box, box_h, sizeof d.box);
-    d.n_nodes = (uint32_t)n_nodes;
-    d.n_refs = n;
-    d.max_depth = depth;
+    d.n_nodes = B.n_entries;
+    d.n_refs = B.n_refs;
+    d.max_depth = B.depth;
     *scene_id = c->next_id++;
     c->scenes[*scene_id] = std::move(sc);
     return PRAY_OK;

@@ -58,6 +58,7 @@ def lib():
             "orc_mlp_grad": (None, [_vp, _vp, _vp, _u64, C.c_int, _vp, _vp, _vp]),
             "orc_adam": (None, [_vp, _vp, _vp, _vp, _u64, _u32, _f32]),
             "orc_lbvh_build": (_u32, [_vp, _u64, _u32, _vp, _vp, _vp]),
+            "orc_sah_build": (_u64, [_vp, _u64, _u32, _u32, _f32, _f32, _vp, _u64, _vp, _u64, _vp, _vp, _vp]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -145,6 +146,24 @@ def lbvh_build(abc, leaf_size):
     ab, ac, _ = init_tris(abc)
     return {"nodes": nodes if n else nodes[:0], "refs": refs, "a": np.ascontiguousarray(abc[:, :3]), "ab": ab,
             "ac": ac, "box": box, "max_depth": int(depth)}
+
+
+def sah_build(abc, leaf_size=8, max_depth=40, ct=2.0, ci=1.0):
+    """Twin of pray_build_scene(PRAY_BUILD_SAH): the flat layout
+    ({nodes, refs, a, ab, ac, box, max_depth})."""
+    abc = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
+    n = abc.shape[0]
+    nr, depth, box = _u64(0), _u32(0), np.zeros(6, np.float32)
+    L = lib()
+    ne = L.orc_sah_build(_p(abc), n, leaf_size, max_depth, ct, ci, None, 0, None, 0, C.byref(nr), C.byref(depth),
+                         _p(box))
+    nodes = np.zeros(2 * ne, np.uint32)
+    refs = np.zeros(max(nr.value, 1), np.uint32)
+    L.orc_sah_build(_p(abc), n, leaf_size, max_depth, ct, ci, _p(nodes), nodes.size, _p(refs), refs.size,
+                    C.byref(nr), C.byref(depth), _p(box))
+    ab, ac, _ = init_tris(abc)
+    return {"nodes": nodes, "refs": refs[:nr.value], "a": np.ascontiguousarray(abc[:, :3]), "ab": ab, "ac": ac,
+            "box": box, "max_depth": int(depth.value)}
 
 
 def mt(ray6, a, ab, ac, eps):

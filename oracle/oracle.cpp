@@ -1270,3 +1270,143 @@ ORC_API uint32_t orc_lbvh_build(const float *abc, uint64_t n, uint32_t leaf_size
     for (uint64_t k = 0; k < n; k++) refs[k] = B.val[k] | (B.last_flag[k] ? 0x80000000u : 0u);
     return depth;
 }
+
+// Twin of the GPU binned-SAH KD build (PRAY_BUILD_SAH): level by level, the
+// same bins (32 per axis over the node's cell, clipped reference boxes), the
+// same cost (traversal + intersection x area-weighted counts, 0.8 for an
+// empty side, kd_tree_sah.cpp:141-166) evaluated in the same order, the same
+// classification and stable partitions, and the same breadth-first numbering
+// of children pairs and leaf runs.
+namespace sahbuild {
+
+struct Node {
+    float lo[3], hi[3];
+    uint32_t begin, count, entry;
+    float plane;
+};
+
+static int bin(float x, float lo, float scale) {
+    return (int)std::fmax(0.f, std::fmin(31.f, std::floor((x - lo) * scale)));
+}
+
+static float half_area(float dx, float dy, float dz) { return dx * dy + dy * dz + dz * dx; }
+
+}  // namespace sahbuild
+
+// nodes: entry words (2 per entry), grown as needed; returns the number of
+// entries; refs: leaf runs; depth and box out.
+ORC_API uint64_t orc_sah_build(const float *abc, uint64_t n, uint32_t leaf_size, uint32_t max_depth, float ct,
+                               float ci, uint32_t *nodes_out, uint64_t nodes_cap, uint32_t *refs_out,
+                               uint64_t refs_cap, uint64_t *n_refs, uint32_t *depth, float *box) {
+    using sahbuild::Node;
+    std::vector<float> lo(3 * n), hi(3 * n);
+    for (uint64_t i = 0; i < n; i++) {
+        const float *p = abc + 9 * i;
+        for (int k = 0; k < 3; k++) {
+            const float x = p[k], y = p[3 + k], z = p[6 + k];
+            lo[3 * i + k] = std::min(x, std::min(y, z));
+            hi[3 * i + k] = std::max(x, std::max(y, z));
+        }
+    }
+    float bl[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, bh[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint64_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) bl[k] = std::min(bl[k], lo[3 * i + k]), bh[k] = std::max(bh[k], hi[3 * i + k]);
+    for (int k = 0; k < 3; k++) box[k] = bl[k], box[3 + k] = bh[k];
+    std::vector<uint32_t> entries(4, 0u), leaf;
+    std::vector<Node> cur(1);
+    for (int k = 0; k < 3; k++) cur[0].lo[k] = bl[k], cur[0].hi[k] = bh[k];
+    cur[0].begin = 0, cur[0].count = (uint32_t)n, cur[0].entry = 1, cur[0].plane = 0.f;
+    std::vector<uint32_t> refs(n);
+    for (uint64_t i = 0; i < n; i++) refs[i] = (uint32_t)i;
+    uint32_t pair_base = 1;
+    *depth = 0;
+    for (uint32_t level = 0;; level++) {
+        std::vector<Node> next;
+        std::vector<uint32_t> next_refs;
+        uint32_t n_split = 0;
+        for (const Node &N : cur) {
+            int best_a = -1;
+            float best_p = 0.f;
+            const float dx = N.hi[0] - N.lo[0], dy = N.hi[1] - N.lo[1], dz = N.hi[2] - N.lo[2];
+            const float sa = sahbuild::half_area(dx, dy, dz);
+            if (N.count > leaf_size && level < max_depth && sa > 0.f) {
+                uint32_t H[3][2][32] = {};
+                for (uint32_t i = N.begin; i < N.begin + N.count; i++) {
+                    const uint32_t t = refs[i];
+                    for (int a = 0; a < 3; a++) {
+                        const float ext = N.hi[a] - N.lo[a];
+                        const float scale = ext > 0.f ? 32.0f / ext : 0.f;
+                        const float clo = std::fmax(lo[3 * t + a], N.lo[a]);
+                        const float chi = std::fmin(hi[3 * t + a], N.hi[a]);
+                        H[a][0][sahbuild::bin(clo, N.lo[a], scale)]++;
+                        H[a][1][sahbuild::bin(chi, N.lo[a], scale)]++;
+                    }
+                }
+                float best = ci * (float)N.count;
+                for (int a = 0; a < 3; a++) {
+                    const float ext = N.hi[a] - N.lo[a];
+                    if (!(ext > 0.f)) continue;
+                    uint32_t nl = 0, ends = 0;
+                    for (int b = 1; b < 32; b++) {
+                        nl += H[a][0][b - 1];
+                        ends += H[a][1][b - 1];
+                        const uint32_t nr = N.count - ends;
+                        const float p = N.lo[a] + ext * ((float)b * (1.0f / 32.0f));
+                        float l[3] = {dx, dy, dz}, r[3] = {dx, dy, dz};
+                        l[a] = p - N.lo[a];
+                        r[a] = N.hi[a] - p;
+                        const float pl = sahbuild::half_area(l[0], l[1], l[2]) / sa;
+                        const float pr = sahbuild::half_area(r[0], r[1], r[2]) / sa;
+                        float cost = ct + ci * (pl * (float)nl + pr * (float)nr);
+                        if ((nl == 0 || nr == 0) && !(pl == 1.f || pr == 1.f)) cost *= 0.8f;
+                        if (cost < best) best = cost, best_a = a, best_p = p;
+                    }
+                }
+            }
+            uint32_t *e = entries.data();
+            (void)e;
+            if (best_a < 0) {
+                const uint32_t w = N.count ? (((uint32_t)leaf.size() << 2) | 3u) : ((0x3FFFFFFFu << 2) | 3u);
+                entries[2 * N.entry] = 0, entries[2 * N.entry + 1] = w;
+                std::memcpy(&entries[2 * N.entry], &N.plane, 4);
+                for (uint32_t i = 0; i < N.count; i++)
+                    leaf.push_back(refs[N.begin + i] | (i + 1 == N.count ? 0x80000000u : 0u));
+                continue;
+            }
+            const uint32_t q = pair_base + n_split++;
+            if (entries.size() < 4 * ((size_t)q + 1)) entries.resize(4 * ((size_t)q + 1), 0u);
+            std::memcpy(&entries[2 * N.entry], &N.plane, 4);
+            entries[2 * N.entry + 1] = ((2u * q) << 2) | (uint32_t)best_a;
+            Node L = N, R = N;
+            L.hi[best_a] = best_p, R.lo[best_a] = best_p;
+            L.entry = 2 * q, R.entry = 2 * q + 1, L.plane = R.plane = best_p;
+            std::vector<uint32_t> lr, rr;
+            for (uint32_t i = N.begin; i < N.begin + N.count; i++) {
+                const uint32_t t = refs[i];
+                const float clo = std::fmax(lo[3 * t + best_a], N.lo[best_a]);
+                const float chi = std::fmin(hi[3 * t + best_a], N.hi[best_a]);
+                if (clo < best_p || (clo == best_p && chi == best_p)) lr.push_back(t);
+                if (chi > best_p) rr.push_back(t);
+            }
+            L.begin = (uint32_t)next_refs.size(), L.count = (uint32_t)lr.size();
+            next_refs.insert(next_refs.end(), lr.begin(), lr.end());
+            R.begin = (uint32_t)next_refs.size(), R.count = (uint32_t)rr.size();
+            next_refs.insert(next_refs.end(), rr.begin(), rr.end());
+            next.push_back(L);
+            next.push_back(R);
+        }
+        pair_base += n_split;
+        if (n_split == 0) {
+            *depth = level;
+            break;
+        }
+        cur.swap(next);
+        refs.swap(next_refs);
+    }
+    const uint64_t n_entries = 2ull * pair_base;
+    entries.resize(2 * n_entries, 0u);
+    if (nodes_out && nodes_cap >= 2 * n_entries) std::memcpy(nodes_out, entries.data(), 2 * n_entries * 4);
+    if (refs_out && refs_cap >= leaf.size()) std::memcpy(refs_out, leaf.data(), leaf.size() * 4);
+    *n_refs = leaf.size();
+    return n_entries;
+}

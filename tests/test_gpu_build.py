@@ -15,14 +15,18 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("leaf_size", [1, 8])
 @pytest.mark.parametrize("name", sorted(SCENES))
-def test_gpu_build_equals_oracle(ctx, name, leaf_size):
+@pytest.mark.parametrize("method", [P.BUILD_MORTON, P.BUILD_SAH])
+def test_gpu_build_equals_oracle(ctx, method, name, leaf_size):
     abc = np.ascontiguousarray(SCENES[name](), np.float32).reshape(-1, 9)
     n = abc.shape[0]
-    say(f"gpu build {name} n={n} leaf={leaf_size}")
-    sid, ms = ctx.build_scene(abc, leaf_size=leaf_size)
+    say(f"gpu build method={method} {name} n={n} leaf={leaf_size}")
+    sid, ms = ctx.build_scene(abc, method=method, leaf_size=leaf_size, max_depth=24 if leaf_size == 1 else 40)
     try:
         g = ctx.scene_layout(sid)
-        o = O.lbvh_build(abc, leaf_size)
+        if method == P.BUILD_MORTON:
+            o = O.lbvh_build(abc, leaf_size)
+        else:
+            o = O.sah_build(abc, leaf_size, 24 if leaf_size == 1 else 40, 2.0, 1.0)
         assert g["n_tris"] == n and g["max_depth"] == o["max_depth"]
         assert np.array_equal(g["box"].view(np.uint32), o["box"].view(np.uint32))
         assert np.array_equal(g["refs"], o["refs"])
@@ -38,8 +42,9 @@ def test_gpu_build_equals_oracle(ctx, name, leaf_size):
         ctx.free_scene(sid)
 
 
-def test_gpu_build_empty_scene(ctx):
-    sid, _ = ctx.build_scene(np.zeros((0, 9), np.float32))
+@pytest.mark.parametrize("method", [P.BUILD_MORTON, P.BUILD_SAH])
+def test_gpu_build_empty_scene(ctx, method):
+    sid, _ = ctx.build_scene(np.zeros((0, 9), np.float32), method=method)
     try:
         rays = O.gen_rays(1, np.array([0, 0, 0, 1, 1, 1], np.float32), 100)
         gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, 1e-4)
@@ -53,6 +58,10 @@ def test_gpu_build_rejects_bad_leaf_size(ctx):
         ctx.build_scene(np.zeros((4, 9), np.float32), leaf_size=0)
     with pytest.raises(P.PrayError):
         ctx.build_scene(np.zeros((4, 9), np.float32), leaf_size=65)
+    with pytest.raises(P.PrayError):
+        ctx.build_scene(np.zeros((4, 9), np.float32), max_depth=65)
+    with pytest.raises(P.PrayError):
+        ctx.build_scene(np.zeros((4, 9), np.float32), method=7)
 
 
 def test_gpu_built_scene_path_traces_like_cpu_tree(ctx):
@@ -70,35 +79,43 @@ def test_gpu_built_scene_path_traces_like_cpu_tree(ctx):
         ref_hdr, ref_rgb = ctx.render_pt(sid, cam, w, h, spp, depth, rr, 9, bg, 1e-9)
     finally:
         ctx.free_scene(sid)
-    say("pt on the gpu-built tree")
-    gid, _ = ctx.build_scene(abc, col, emit, leaf_size=4)
-    try:
-        hdr, rgb = ctx.render_pt(gid, cam, w, h, spp, depth, rr, 9, bg, 1e-9)
-    finally:
-        ctx.free_scene(gid)
-    assert np.array_equal(hdr.view(np.uint32), ref_hdr.view(np.uint32)) and np.array_equal(rgb, ref_rgb)
+    for method in (P.BUILD_MORTON, P.BUILD_SAH):
+        say(f"pt on the gpu-built tree, method {method}")
+        gid, _ = ctx.build_scene(abc, col, emit, method=method, leaf_size=4)
+        try:
+            hdr, rgb = ctx.render_pt(gid, cam, w, h, spp, depth, rr, 9, bg, 1e-9)
+        finally:
+            ctx.free_scene(gid)
+        assert np.array_equal(hdr.view(np.uint32), ref_hdr.view(np.uint32)) and np.array_equal(rgb, ref_rgb)
     oh, _ = O.render_pt(t.flat(), cam, w, h, spp, depth, rr, 9, bg, 1e-9)
     assert np.array_equal(hdr.view(np.uint32), oh.view(np.uint32))
     assert hdr.mean() > 0.01
 
 
 def test_gpu_build_terrain_full_size(ctx):
-    """The 2048 x 2048-cell terrain (8,388,608 triangles): every triangle in
-    one leaf, and 2^20 incoherent rays hit exactly as over the CPU SAH tree."""
+    """The 2048 x 2048-cell terrain (8,388,608 triangles), both methods:
+    every triangle in a leaf, and 2^20 incoherent rays hit exactly as over
+    the CPU SAH tree."""
     s = P.Scene.terrain(2048, 2024)
     abc, _, _ = s.triangles()
-    say("gpu build terrain2048")
-    gid, ms = ctx.build_scene(abc, leaf_size=8)
-    print(f"GPU build of {abc.shape[0]} triangles: {ms:.2f} ms", flush=True)
-    try:
-        g = ctx.scene_layout(gid)
-        assert g["max_depth"] <= 64
-        assert np.array_equal(np.sort(g["refs"] & 0x7FFFFFFF), np.arange(abc.shape[0], dtype=np.uint32))
-        rays = O.gen_rays(2024, P.scene_box_expanded(g["box"]), 1 << 20)
-        say("trace terrain2048 gpu-built")
-        gi, gt = ctx.trace(gid, rays, P.TRACE_CLOSEST, 1e-12)
-    finally:
-        ctx.free_scene(gid)
+    box = None
+    res = {}
+    for method in (P.BUILD_MORTON, P.BUILD_SAH):
+        say(f"gpu build terrain2048 method {method}")
+        gid, ms = ctx.build_scene(abc, method=method, leaf_size=8)
+        print(f"GPU build (method {method}) of {abc.shape[0]} triangles: {ms:.2f} ms", flush=True)
+        try:
+            g = ctx.scene_layout(gid)
+            assert g["max_depth"] <= 64
+            tri = np.unique(g["refs"] & 0x7FFFFFFF)
+            assert np.array_equal(tri, np.arange(abc.shape[0], dtype=np.uint32))
+            if box is None:
+                box = g["box"]
+                rays = O.gen_rays(2024, P.scene_box_expanded(box), 1 << 20)
+            say(f"trace terrain2048 gpu-built method {method}")
+            res[method] = ctx.trace(gid, rays, P.TRACE_CLOSEST, 1e-12)
+        finally:
+            ctx.free_scene(gid)
     t = s.build(1, max_depth=40, leaf_size=8, traversal_cost=2.0)
     sid = ctx.upload(t, shading=False)
     try:
@@ -106,5 +123,6 @@ def test_gpu_build_terrain_full_size(ctx):
         ci, ct = ctx.trace(sid, rays, P.TRACE_CLOSEST, 1e-12)
     finally:
         ctx.free_scene(sid)
-    assert np.array_equal(gi, ci) and np.array_equal(gt.view(np.uint32), ct.view(np.uint32))
-    assert (gi >= 0).mean() > 0.1
+    for gi, gt in res.values():
+        assert np.array_equal(gi, ci) and np.array_equal(gt.view(np.uint32), ct.view(np.uint32))
+    assert (ci >= 0).mean() > 0.1

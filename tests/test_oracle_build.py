@@ -148,3 +148,117 @@ def test_lbvh_closest_hits_equal_brute_force(name):
     assert np.array_equal(gt[h].view(np.uint32), bt[h].view(np.uint32))
     if name != "duplicates":
         assert h.mean() > 0.01
+
+
+# ---------------------------------------------------------------- SAH twin --
+
+def walk_sah(flat):
+    """Leaves (first, count, cell) and inner nodes (axis, plane, cell, depth)
+    reached from the root, with cells from the splits."""
+    nodes = flat["nodes"].reshape(-1, 2)
+    refs = flat["refs"]
+    box = flat["box"].astype(np.float32)
+    leaves, inner = [], []
+    stack = [(int(nodes[1, 1]), box[:3].copy(), box[3:].copy(), 0)]
+    while stack:
+        w, lo, hi, d = stack.pop()
+        if w & 3 == 3:
+            first = w >> 2
+            if first == 0x3FFFFFFF:
+                leaves.append((None, 0, lo, hi, d))
+                continue
+            k = first
+            while not refs[k] & 0x80000000:
+                k += 1
+            leaves.append((first, k - first + 1, lo, hi, d))
+            continue
+        lc, a = w >> 2, w & 3
+        p = nodes[lc, 0:1].view(np.float32)[0]
+        assert nodes[lc + 1, 0:1].view(np.float32)[0] == p  # both children clip at the split plane
+        inner.append((a, p, lo, hi, d))
+        llo, lhi, rlo, rhi = lo.copy(), hi.copy(), lo.copy(), hi.copy()
+        lhi[a] = p
+        rlo[a] = p
+        stack += [(int(nodes[lc, 1]), llo, lhi, d + 1), (int(nodes[lc + 1, 1]), rlo, rhi, d + 1)]
+    return leaves, inner
+
+
+def root_split(abc, leaf_size, ct, ci):
+    """The root's binned-SAH plane, restated in numpy float32."""
+    t = abc.reshape(-1, 3, 3)
+    lo, hi = t.min(1), t.max(1)
+    blo, bhi = lo.min(0), hi.max(0)
+    f32 = np.float32
+    d = (bhi - blo).astype(np.float32)
+
+    def ha(x, y, z):
+        return f32(f32(f32(x * y) + f32(y * z)) + f32(z * x))
+
+    sa = ha(d[0], d[1], d[2])
+    n = lo.shape[0]
+    best, best_a, best_p = f32(ci * f32(n)), -1, f32(0)
+    for a in range(3):
+        ext = f32(bhi[a] - blo[a])
+        if not ext > 0:
+            continue
+        scale = f32(f32(32) / ext)
+        bl = np.clip(np.floor(((lo[:, a] - blo[a]).astype(np.float32) * scale).astype(np.float32)), 0, 31).astype(int)
+        bh = np.clip(np.floor(((hi[:, a] - blo[a]).astype(np.float32) * scale).astype(np.float32)), 0, 31).astype(int)
+        starts, ends = np.bincount(bl, minlength=32), np.bincount(bh, minlength=32)
+        for b in range(1, 32):
+            nl, nr = int(starts[:b].sum()), n - int(ends[:b].sum())
+            p = f32(blo[a] + f32(ext * f32(f32(b) * f32(1 / 32))))
+            l, r = d.copy(), d.copy()
+            l[a], r[a] = f32(p - blo[a]), f32(bhi[a] - p)
+            pl, pr = f32(ha(*l) / sa), f32(ha(*r) / sa)
+            cost = f32(f32(ct) + f32(f32(ci) * f32(f32(pl * f32(nl)) + f32(pr * f32(nr)))))
+            if (nl == 0 or nr == 0) and not (pl == 1 or pr == 1):
+                cost = f32(cost * f32(0.8))
+            if cost < best:
+                best, best_a, best_p = cost, a, p
+    return best_a, best_p
+
+
+@pytest.mark.parametrize("name", ["spheres", "terrain96", "cornell", "random", "duplicates", "one", "two", "nine"])
+def test_sah_twin_structure(name):
+    abc = np.ascontiguousarray(SCENES[name](), np.float32).reshape(-1, 9)
+    n = abc.shape[0]
+    f = O.sah_build(abc, leaf_size=4, max_depth=30, ct=2.0, ci=1.0)
+    leaves, inner = walk_sah(f)
+    t = abc.reshape(-1, 3, 3)
+    lo, hi = t.min(1), t.max(1)
+    assert max([lv[4] for lv in leaves]) == f["max_depth"] <= 30
+    seen = np.zeros(n, np.int64)
+    for first, cnt, clo, chi, d in leaves:
+        if first is None:
+            continue
+        tri = f["refs"][first:first + cnt] & 0x7FFFFFFF
+        assert (np.diff(tri.astype(np.int64)) > 0).all()  # stable partitions keep triangle order
+        seen[tri] += 1
+        # a leaf holds only triangles whose box meets its cell
+        assert (lo[tri] <= chi).all() and (hi[tri] >= clo).all()
+    assert (seen >= 1).all()
+    # every triangle whose box meets a leaf's cell interior is in that leaf
+    for first, cnt, clo, chi, d in leaves[:200]:
+        inside = ((lo < chi) & (hi > clo)).all(1)
+        got = set() if first is None else set((f["refs"][first:first + cnt] & 0x7FFFFFFF).tolist())
+        assert set(np.flatnonzero(inside).tolist()) <= got
+    # the root's plane is the binned-SAH minimum restated in numpy
+    a, p = root_split(abc, 4, 2.0, 1.0)
+    if a < 0 or n <= 4:
+        assert not inner
+    else:
+        assert inner[0][4] == 0 and inner[0][0] == a and inner[0][1] == p
+
+
+@pytest.mark.parametrize("name", ["spheres", "terrain96", "cornell", "random", "duplicates"])
+def test_sah_twin_closest_hits_equal_brute_force(name):
+    abc = np.ascontiguousarray(SCENES[name](), np.float32).reshape(-1, 9)
+    f = O.sah_build(abc)
+    rays = O.gen_rays(19, P.scene_box_expanded(f["box"]), 4000)
+    eps = 1e-9 if name != "spheres" else 1e-4
+    gi, gt = O.flat_trace(f, rays, 0, eps)
+    bi, bt = O.bf_closest(f["a"], f["ab"], f["ac"], rays, eps)
+    assert np.array_equal(gi, bi)
+    h = bi >= 0
+    assert np.array_equal(gt[h].view(np.uint32), bt[h].view(np.uint32))
