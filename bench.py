@@ -61,8 +61,10 @@ def main():
                          "layout,stack_lds,leaf_ilp,fetch_min,occupancy")
     ap.add_argument("--no-ref-tree", action="store_true",
                     help="skip timing the same rays over the tree built with the reference's parameters")
-    ap.add_argument("--no-gpu-build", action="store_true",
-                    help="skip the GPU tree build and the trace over its tree")
+    ap.add_argument("--tree", choices=("gpu-sah", "cpu-sah"), default="gpu-sah",
+                    help="the timed tree: SAH KD tree built on the GPU (binned) or on the CPU (exact events)")
+    ap.add_argument("--no-cpu-tree", action="store_true",
+                    help="skip timing the same rays over the CPU-built SAH tree")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=1 << 24, help="CPU baseline sample (prefix of the stream)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -89,15 +91,45 @@ def main():
     t0 = time.time()
     scene = P.Scene.terrain(args.grid, args.seed)
     n_tris = scene.num_tris
-    t_build = time.time()
-    tree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
-    cpu_build_s = time.time() - t_build
-    tstats = tree.stats()
-    tree_tag = f"sah-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}"
-    log(f"scene {n_tris} tris, SAH tree {tree_tag} {tstats} in {time.time() - t0:.1f}s")
-
     ctx = P.Context(0)
-    sid = ctx.upload(tree, shading=False)
+    cpu_tree = None
+    if args.tree == "gpu-sah":
+        # SAH KD tree built on the GPU (median of three builds timed).
+        abc, _, _ = scene.triangles()
+        builds = []
+        for _ in range(3):
+            t1 = time.time()
+            sid, bms = ctx.build_scene(abc, method=P.BUILD_SAH, leaf_size=args.leaf_size, max_depth=args.max_depth,
+                                       traversal_cost=args.traversal_cost)
+            builds.append((bms, time.time() - t1))
+            if len(builds) < 3:
+                ctx.free_scene(sid)
+        del abc
+        lay = ctx.scene_layout(sid)
+        tree_tag = f"gpusah-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}"
+        tree_depth, tree_nodes, tree_refs = lay["max_depth"], len(lay["nodes"]) // 2, len(lay["refs"])
+        raw_box = lay["box"]
+        tree_build = {"where": "GPU", "method": "binned SAH, 32 bins per axis, level by level",
+                      "build_ms": round(sorted(b[0] for b in builds)[1], 3),
+                      "build_wall_s_with_upload": round(sorted(b[1] for b in builds)[1], 3)}
+        tree_desc = (f"SAH KD tree built on the GPU (32 bins; depth cap {args.max_depth}, leaf size {args.leaf_size}, "
+                     f"traversal cost {args.traversal_cost:g})")
+    else:
+        t1 = time.time()
+        cpu_tree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size,
+                               traversal_cost=args.traversal_cost)
+        cpu_build_s = time.time() - t1
+        st = cpu_tree.stats()
+        sid = ctx.upload(cpu_tree, shading=False)
+        tree_tag = f"sah-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}"
+        tree_depth, tree_nodes, tree_refs = st["maxdepth"], st["flat_nodes"], st["flat_refs"]
+        raw_box = cpu_tree.box()
+        tree_build = {"where": "CPU", "method": "exact-event SAH", "build_s": round(cpu_build_s, 2)}
+        tree_desc = (f"SAH KD tree built on the CPU (exact events; depth cap {args.max_depth}, leaf size "
+                     f"{args.leaf_size}, traversal cost {args.traversal_cost:g})")
+    log(f"scene {n_tris} tris, tree {tree_tag}: depth {tree_depth}, {tree_nodes} nodes, {tree_refs} references, "
+        f"{tree_build} ({time.time() - t0:.1f}s)")
+
     tuning = None
     if args.tuning:
         vals = [int(x) for x in args.tuning.split(",")]
@@ -108,7 +140,7 @@ def main():
         t1 = time.time()
         tuning = ctx.autotune(sid, n_rays=n_rays_for_tuning(args), force=True)
         log(f"autotuned {tuning} in {time.time() - t1:.1f}s")
-    box = P.scene_box_expanded(tree.box())
+    box = P.scene_box_expanded(raw_box)
 
     # Inputs and outputs resident in HBM for the whole run.
     n = args.rays
@@ -184,39 +216,30 @@ def main():
         with open(pmc) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
-    # The tree built on the GPU (Morton-ordered bounding interval hierarchy):
-    # build time, and the same rays over it (identical hits).
-    gpu_build = None
-    if not args.no_gpu_build:
-        abc, _, _ = scene.triangles()
-        builds = []
-        for _ in range(3):
-            t1 = time.time()
-            gid, bms = ctx.build_scene(abc, method=P.BUILD_SAH, leaf_size=args.leaf_size, max_depth=args.max_depth,
-                                       traversal_cost=args.traversal_cost)
-            wall = time.time() - t1
-            builds.append((bms, wall))
-            if len(builds) < 3:
-                ctx.free_scene(gid)
-        del abc
-        lay = ctx.scene_layout(gid)
-        ctx.set_tuning(gid, **tun)
-        ms_g, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=gid)
-        same_g = bool(np.array_equal(closest_idx, d_idx.numpy()) and
+    # The same rays over the SAH tree built on the CPU with exact events
+    # (when the timed tree is the GPU-built one): identical hits.
+    cpu_sah = None
+    if args.tree == "gpu-sah" and not args.no_cpu_tree:
+        t1 = time.time()
+        ctree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+        cbuild = time.time() - t1
+        cst = ctree.stats()
+        csid = ctx.upload(ctree, shading=False)
+        del ctree
+        if tuning is not None:  # its own autotuned setting
+            ctx.autotune(csid, n_rays=n_rays_for_tuning(args), force=True)
+        else:
+            ctx.set_tuning(csid, **tun)
+        ctun = ctx.get_tuning(csid)
+        ms_c, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=csid)
+        same_c = bool(np.array_equal(closest_idx, d_idx.numpy()) and
                       np.array_equal(closest_t.view(np.uint32), d_t.numpy().view(np.uint32)))
-        ctx.free_scene(gid)
-        bms = sorted(b[0] for b in builds)[1]
-        gpu_build = {
-            "tree": f"binned-SAH KD tree built on the GPU (32 bins, leaf size {args.leaf_size}, traversal cost "
-                    f"{args.traversal_cost:g}, depth cap {args.max_depth} -> depth {lay['max_depth']}, "
-                    f"{len(lay['nodes']) // 2} nodes, {len(lay['refs'])} references)",
-            "build_ms": round(bms, 3), "build_wall_s_with_upload": round(sorted(b[1] for b in builds)[1], 3),
-            "cpu_sah_build_s": round(cpu_build_s, 2),
-            "ms_per_step": round(ms_g, 4), "Mrays_per_s": round(n / (ms_g * 1e-3) / 1e6, 2),
-            "identical_hits": same_g,
-        }
-        log(f"GPU build {bms:.2f} ms (CPU SAH build {cpu_build_s:.2f} s); trace over it {ms_g:.3f} ms, "
-            f"identical hits {same_g}")
+        ctx.free_scene(csid)
+        cpu_sah = {"tree": f"SAH KD tree built on the CPU (exact events, same parameters): depth {cst['maxdepth']}, "
+                           f"{cst['flat_nodes']} nodes, {cst['flat_refs']} references",
+                   "build_s": round(cbuild, 2), "tuning": ctun, "ms_per_step": round(ms_c, 4),
+                   "Mrays_per_s": round(n / (ms_c * 1e-3) / 1e6, 2), "identical_hits": same_c}
+        log(f"CPU SAH tree (built in {cbuild:.2f} s): {ms_c:.3f} ms/step, identical hits {same_c}")
 
     # The same rays over the tree built with the reference's parameters
     # (depth cap 20, leaf size 32, SAH costs 5 / 1.5): identical hits, more
@@ -266,11 +289,9 @@ def main():
         "config": {
             "workload": f"closest-hit, {n} incoherent rays (origins uniform in the scene box +10%, directions "
                         f"uniform on the sphere, Philox seed {args.seed}), heightfield terrain {args.grid}x{args.grid} "
-                        f"cells = {n_tris} FP32 triangles, SAH KD tree (depth cap {args.max_depth}, leaf size "
-                        f"{args.leaf_size}, traversal cost {args.traversal_cost:g}), det_eps "
-                        f"{args.det_eps:g}",
-            "rays": n, "triangles": n_tris, "tree": tree_tag, "tree_depth": tstats["maxdepth"],
-            "tree_nodes": tstats["flat_nodes"], "leaf_refs": tstats["flat_refs"],
+                        f"cells = {n_tris} FP32 triangles, {tree_desc}, det_eps {args.det_eps:g}",
+            "rays": n, "triangles": n_tris, "tree": tree_tag, "tree_depth": tree_depth,
+            "tree_nodes": tree_nodes, "leaf_refs": tree_refs, "tree_build": tree_build,
             "layout": "packed" if tun["layout"] == P.LAYOUT_PACKED else "indexed",
             "tuning": tun, "autotuned": tuning is not None,
             "hit_fraction": round(hits / n, 4),
@@ -290,7 +311,7 @@ def main():
         },
         "any_hit": {"ms_per_step": round(ms_any, 4), "Mrays_per_s": round(n / (ms_any * 1e-3) / 1e6, 2)},
         "reference_parameter_tree": ref_tree,
-        "gpu_tree_build": gpu_build,
+        "cpu_sah_tree": cpu_sah,
         "radiance_cache": rc_out,
         "cpu_baseline": cpu,
     }
