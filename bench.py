@@ -339,8 +339,7 @@ def batch_workload(P, args):
     scenes, sids = [], []
     for k in range(n):
         s = P.Scene.random(int(args.seed * 1000 + k), int(tris[k]))
-        t = s.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
-        sids.append(ctx.upload(t))
+        sids.append(gpu_sah_scene(P, ctx, s, args))
         scenes.append(s)
     log(f"batch: {n} scenes ({int(sum(s.num_tris for s in scenes))} triangles) built and uploaded in "
         f"{time.time() - t0:.1f}s")
@@ -380,7 +379,7 @@ def batch_workload(P, args):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{n} random icosphere/box/terrain scenes, triangles log-uniform in [1k, 2M], images "
                                f"uniform over 128^2..1024^2, {spp} spp, depth {depth}, roulette from bounce {rr}, "
-                               f"one wavefront for all scenes", "scenes": n},
+                               f"one wavefront for all scenes, SAH KD trees built on the GPU", "scenes": n},
         "batch_ms": round(ms, 2), "rays": rays, "sum_of_single_scene_ms": round(single_ms, 2),
         "batch_speedup_over_single_scene_renders": round(single_ms / ms, 3),
         "all_scenes_equal_to_single_renders": same,
@@ -388,6 +387,15 @@ def batch_workload(P, args):
         "per_scene": per,
     }
     print(json.dumps(out), flush=True)
+
+
+def gpu_sah_scene(P, ctx, scene, args):
+    """Uploads a scene with its SAH KD tree built on the GPU (with shading
+    data)."""
+    abc, col, emit = scene.triangles()
+    sid, _ = ctx.build_scene(abc, col, emit, method=P.BUILD_SAH, leaf_size=args.leaf_size, max_depth=args.max_depth,
+                             traversal_cost=args.traversal_cost)
+    return sid
 
 
 def P_camera(P, scene, side):
@@ -422,8 +430,7 @@ def radiance_cache_leg(P, ctx, args):
     --ref-spp samples per pixel, RMSE on linear HDR."""
     w = h = args.cache_size
     scene = P.Scene.cornell(1, 100_000)
-    tree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
-    sid = ctx.upload(tree)
+    sid = gpu_sah_scene(P, ctx, scene, args)
     if not args.no_autotune:
         ctx.autotune(sid, n_rays=1 << 22, force=True)
     inf = scene.info()
@@ -463,7 +470,7 @@ def radiance_cache_leg(P, ctx, args):
     best = cache[-1]
     return {
         "scene": f"Cornell room, {scene.num_tris} triangles, {w}x{h}, depth {depth}, roulette from bounce {rr}, "
-                 f"SAH tree (leaf size {args.leaf_size})",
+                 f"SAH KD tree built on the GPU (leaf size {args.leaf_size})",
         "reference_spp": args.ref_spp,
         "cache": {"network": "192 (172 used) -> 5 x 64 ReLU -> 3, BF16 matrix cores, FP32 Adam lr 1e-2",
                   "training_samples_per_frame": 1 << 16, "terminates_at_hit": 2, "warmup_frames": warm},

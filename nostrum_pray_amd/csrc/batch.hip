@@ -82,9 +82,9 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
                                         (float *)tt.p, 2);
             if (ms < best_ms) best_ms = ms, best = cand;
         };
-        for (int K : {4, 8, 16})
+        for (int K : {4, 6, 8, 16})
             for (int ilp : {1, 2})
-                for (int fm : {32, 48, 64}) {
+                for (int fm : {32, 40, 48, 64}) {
                     Tuning t;
                     t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm;
                     consider(t);
