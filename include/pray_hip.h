@@ -239,6 +239,11 @@ PRAY_API int pray_render_pt_batch(pray_ctx *ctx, uint32_t n_scenes, const int *s
 /* Rays traced (all bounces, all samples) by the last pray_render_pt* or
  * pray_render_pt_batch call of the context. */
 PRAY_API int pray_last_render_rays(pray_ctx *ctx, uint64_t *rays);
+/* Per bounce b (0 = camera rays), the paths traced at that bounce summed
+ * over all samples of the last pray_render_pt / _ex / _progressive call:
+ * the sizes of the wavefront queues.  Fills min(n, *n_bounces) counts;
+ * *n_bounces = max_depth + 1 (0 after other calls). */
+PRAY_API int pray_last_render_queues(pray_ctx *ctx, uint64_t *counts, uint32_t n, uint32_t *n_bounces);
 
 /* Learned radiance cache (no reference counterpart; the reference's path
  * tracer, path_trace.cu:369-464, always traces paths to the end).

@@ -78,6 +78,7 @@ def libraries():
         "pray_render_pt_batch": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp, _f32, _vp, _vp,
                                            C.POINTER(_f32)]),
         "pray_last_render_rays": (C.c_int, [_vp, C.POINTER(_u64)]),
+        "pray_last_render_queues": (C.c_int, [_vp, _vp, _u32, C.POINTER(_u32)]),
         "pray_trace": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp]),
         "pray_trace_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp]),
         "pray_trace_counts_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp, _vp]),
@@ -459,6 +460,14 @@ class Context:
         out = np.zeros(len(TUNING_FIELDS), np.int32)
         self._check(self._lib.pray_autotune(self._h, sid, n_rays, 1 if force else 0, _p(out)))
         return dict(zip(TUNING_FIELDS, (int(x) for x in out)))
+
+    def last_render_queues(self):
+        """Paths traced per bounce by the last path-tracing call."""
+        nb = _u32()
+        self._check(self._lib.pray_last_render_queues(self._h, None, 0, C.byref(nb)))
+        out = np.zeros(nb.value, np.uint64)
+        self._check(self._lib.pray_last_render_queues(self._h, _p(out), nb.value, C.byref(nb)))
+        return out
 
     def kernel_timing(self, enable=True):
         """Time the traversal kernel of later traces (pray_kernel_timing)."""

@@ -239,6 +239,7 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
         return fail(c, PRAY_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
     if (gpu_ms) HIPC(c, hipEventElapsedTime(gpu_ms, c->ievents[2], c->ievents[3]));
     HIPC(c, hipMemcpy(&c->last_rays, rays_acc.p, 8, hipMemcpyDeviceToHost));
+    c->last_queues.clear();
     hipLaunchKernelGGL(prr::k_pt_finalize, dim3(nblocks(3 * total, 256)), dim3(256), 0, st, (uint32_t)total,
                        (const float *)sum.p, (float)spp, (float *)outh.p, (uint8_t *)outb.p);
     HIPC(c, hipGetLastError());
@@ -256,6 +257,13 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
 int pray_last_render_rays(pray_ctx *c, uint64_t *rays) {
     if (!c || !rays) return PRAY_ERR_INVALID;
     *rays = c->last_rays;
+    return PRAY_OK;
+}
+
+int pray_last_render_queues(pray_ctx *c, uint64_t *counts, uint32_t n, uint32_t *n_bounces) {
+    if (!c) return PRAY_ERR_INVALID;
+    if (n_bounces) *n_bounces = (uint32_t)c->last_queues.size();
+    for (uint32_t b = 0; counts && b < n && b < c->last_queues.size(); b++) counts[b] = c->last_queues[b];
     return PRAY_OK;
 }
 

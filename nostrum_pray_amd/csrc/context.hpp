@@ -49,6 +49,7 @@ struct pray_ctx {
     hipStream_t side = nullptr;   // second stream for work overlapped inside graphs
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     uint64_t last_rays = 0;       // rays traced by the last render call
+    std::vector<uint64_t> last_queues;  // per bounce: paths traced (pray_render_pt*)
     uint32_t *work = nullptr;   // ray counters of the persistent kernels
     uint4 *spill = nullptr;     // per-lane stack spill area
     size_t spill_bytes = 0;

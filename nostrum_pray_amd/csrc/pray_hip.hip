@@ -703,13 +703,15 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
     HIPC(c, hipSetDevice(c->device));
     Cam cam;
     std::memcpy(&cam, cam16, sizeof cam);
-    DBuf prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb;
+    DBuf prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, qsum;
     HIPC(c, hipMalloc(&prim.p, npix * 24));
     for (int q = 0; q < 2; q++) {
         HIPC(c, hipMalloc(&rq[q].p, npix * 24));
         HIPC(c, hipMalloc(&sq[q].p, npix * 16));
     }
     HIPC(c, hipMalloc(&cnt.p, 6 * sizeof(uint32_t)));  // count0, count1, sample, pad, traced rays (u64)
+    if (max_depth > 64) return fail(c, PRAY_ERR_INVALID, "max_depth must be at most 64");
+    HIPC(c, hipMalloc(&qsum.p, 65 * sizeof(uint64_t)));  // paths traced per bounce, all samples
     HIPC(c, hipMalloc(&hit.p, npix * 4));
     HIPC(c, hipMalloc(&tt.p, npix * 4));
     HIPC(c, hipMalloc(&contrib.p, npix * 12));
@@ -725,6 +727,7 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
         hipLaunchKernelGGL(prr::k_fill_f32, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (float *)sum.p,
                            (uint32_t)(3 * npix), 0.f);
     hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count, 6u);
+    hipLaunchKernelGGL(prt::k_zero_u32, dim3(3), dim3(64), 0, st, (uint32_t *)qsum.p, 130u);
     hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, count + 2, first);  // sample index
     HIPC(c, hipGetLastError());
     unsigned long long *traced = (unsigned long long *)(count + 4);
@@ -749,6 +752,8 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
             const int cur = b & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + nxt, 1u);
             hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, traced, (const uint32_t *)(count + cur));
+            hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, (unsigned long long *)qsum.p + b,
+                               (const uint32_t *)(count + cur));
             if (sc->d.n_tris) {
                 int rc = enqueue_trace(c, *sc, (const float *)rq[cur].p, npix, count + cur, PRAY_TRACE_CLOSEST,
                                        det_eps, nullptr, (int32_t *)hit.p, (float *)tt.p, nullptr, st);
@@ -802,6 +807,8 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
     if (rgb) HIPC(c, hipMemcpyAsync(rgb, outb.p, npix * 3, hipMemcpyDeviceToHost, st));
     if (accum_out) HIPC(c, hipMemcpyAsync(accum_out, sum.p, npix * 12, hipMemcpyDeviceToHost, st));
     HIPC(c, hipMemcpyAsync(&c->last_rays, traced, 8, hipMemcpyDeviceToHost, st));
+    c->last_queues.assign(max_depth + 1, 0);
+    HIPC(c, hipMemcpyAsync(c->last_queues.data(), qsum.p, 8 * ((size_t)max_depth + 1), hipMemcpyDeviceToHost, st));
     HIPC(c, hipStreamSynchronize(st));
     return PRAY_OK;
 }

@@ -250,18 +250,20 @@ def render_rt(flat, cam16, w, h, lights, bg, eps):
     return rgb
 
 
-def render_pt(flat, cam16, w, h, spp, max_depth, rr_depth, seed, bg, eps):
-    """Path tracing restated with the build's Philox streams; (hdr, rgb)."""
+def render_pt(flat, cam16, w, h, spp, max_depth, rr_depth, seed, bg, eps, queues=False):
+    """Path tracing restated with the build's Philox streams; (hdr, rgb), and
+    with queues=True also the paths traced per bounce."""
     L = lib()
     L.orc_render_pt.restype = None
-    L.orc_render_pt.argtypes = [_vp] * 10 + [_u64, _u64, _u32, _u32, _u32, _u64, _vp, _f32, _vp, _vp]
+    L.orc_render_pt.argtypes = [_vp] * 10 + [_u64, _u64, _u32, _u32, _u32, _u64, _vp, _f32, _vp, _vp, _vp]
     hdr = np.empty((h, w, 3), np.float32)
     rgb = np.empty((h, w, 3), np.uint8)
+    q = np.zeros(max_depth + 1, np.uint64) if queues else None
     L.orc_render_pt(_p(flat["nodes"]), _p(flat["refs"]), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]),
                     _p(flat["box"]), _p(flat["normal"]), _p(flat["color"]), _p(flat["emit"]),
                     _p(np.ascontiguousarray(cam16, np.float32)), w, h, spp, max_depth, rr_depth, seed,
-                    _p(np.ascontiguousarray(bg, np.float32)), eps, _p(hdr), _p(rgb))
-    return hdr, rgb
+                    _p(np.ascontiguousarray(bg, np.float32)), eps, _p(hdr), _p(rgb), _p(q))
+    return (hdr, rgb, q) if queues else (hdr, rgb)
 
 
 def read_bmp(path):

@@ -799,8 +799,12 @@ ORC_API void orc_render_pt(const uint32_t *nodes, const uint32_t *refs, const fl
                            const float *AC, const float *box6, const float *normal, const float *color,
                            const uint8_t *emit, const float *cam16, uint64_t w, uint64_t h, uint32_t spp,
                            uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *bg3, float eps,
-                           float *hdr, uint8_t *rgb) {
+                           float *hdr, uint8_t *rgb, uint64_t *queues) {
+    // queues (optional, max_depth + 1 entries): paths traced per bounce over
+    // all pixels and samples, the sizes of a wavefront tracer's queues.
     const Shade S{{nodes, refs, A, AB, AC, box6}, normal, color, emit, eps};
+    if (queues)
+        for (uint32_t b = 0; b <= max_depth; b++) queues[b] = 0;
     std::vector<float> rays(6 * w * h);
     orc_camera_rays(cam16, w, h, rays.data());
     const V bg = ld(bg3);
@@ -813,6 +817,10 @@ ORC_API void orc_render_pt(const uint32_t *nodes, const uint32_t *refs, const fl
             V T = {1.f, 1.f, 1.f};
             V contrib = {0.f, 0.f, 0.f};
             for (uint32_t b = 0;; b++) {
+                if (queues) {
+#pragma omp atomic
+                    queues[b]++;
+                }
                 float t;
                 const uint32_t tri = closest(S, o, d, t);
                 if (tri == NO_HIT) {

@@ -114,10 +114,15 @@ def test_pt_cornell_equal_oracle(ctx, use_graph):
                                          P._p(hdr), P._p(rgb), use_graph))
     finally:
         ctx.free_scene(sid)
-    oh, orgb = O.render_pt(f, cam, w, h, spp, depth, rr, 77, bg, 1e-9)
+    queues = ctx.last_render_queues()
+    oh, orgb, oq = O.render_pt(f, cam, w, h, spp, depth, rr, 77, bg, 1e-9, queues=True)
     assert np.array_equal(hdr.view(np.uint32), oh.view(np.uint32))
     assert np.array_equal(rgb, orgb)
     assert hdr.mean() > 0.01
+    # the wavefront queues hold exactly the paths the per-pixel restatement
+    # traces at each bounce (all samples), and they shrink
+    assert np.array_equal(queues, oq), (queues, oq)
+    assert queues[0] == w * h * spp and (np.diff(queues.astype(np.int64)) <= 0).all() and queues[-1] > 0
 
 
 def test_pt_reference_cornell_example_equal_oracle(ctx):

@@ -235,6 +235,13 @@ def test_pt_oracle_closed_forms():
     # mean over many samples approaches rho * Le / rho * rho = rho * Le
     hdr_rr, _ = O.render_pt(f, cam, w, h, 256, 2, 0, 1, [0, 0, 0], 1e-9)
     assert abs(hdr_rr[box].mean() - le * rho) < 0.05 * le * rho
+    # queue sizes: every camera path is traced at bounce 0; exactly the box
+    # pixels' paths continue to bounce 1 (the walls emit), and all of those
+    # reach bounce 2 (the box sees only emitting walls, no roulette)
+    _, _, q = O.render_pt(f, cam, w, h, 3, 2, 1 << 30, 1, [0, 0, 0], 1e-9, queues=True)
+    assert q[0] == w * h * 3 and q[1] == 3 * box.sum() and q[2] == 0
+    _, _, q0 = O.render_pt(f, cam, w, h, 3, 0, 1 << 30, 1, [0, 0, 0], 1e-9, queues=True)
+    assert list(q0) == [w * h * 3]
 
 
 def test_sample_hemisphere_properties():
