@@ -109,10 +109,10 @@ def main():
         tree_tag = f"gpusah-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}"
         tree_depth, tree_nodes, tree_refs = lay["max_depth"], len(lay["nodes"]) // 2, len(lay["refs"])
         raw_box = lay["box"]
-        tree_build = {"where": "GPU", "method": "binned SAH, 32 bins per axis, level by level",
+        tree_build = {"where": "GPU", "method": "binned SAH, 64 bins per axis, level by level",
                       "build_ms": round(sorted(b[0] for b in builds)[1], 3),
                       "build_wall_s_with_upload": round(sorted(b[1] for b in builds)[1], 3)}
-        tree_desc = (f"SAH KD tree built on the GPU (32 bins; depth cap {args.max_depth}, leaf size {args.leaf_size}, "
+        tree_desc = (f"SAH KD tree built on the GPU (64 bins; depth cap {args.max_depth}, leaf size {args.leaf_size}, "
                      f"traversal cost {args.traversal_cost:g})")
     else:
         t1 = time.time()

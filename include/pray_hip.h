@@ -92,7 +92,7 @@ PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
 /* Tree build on the GPU (the reference builds its trees on the CPU,
  * kd_tree.cpp:25-84, kd_tree_sah.cpp:32-296), built and kept on the device
  * in the traversal's layout.  Any tree gives the same closest hits.
- *   method     PRAY_BUILD_SAH: binned-SAH KD tree built level by level (32
+ *   method     PRAY_BUILD_SAH: binned-SAH KD tree built level by level (64
  *              bins per axis, the reference's SAH cost with its empty-side
  *              factor; references of straddling triangles are duplicated);
  *              PRAY_BUILD_MORTON: bounding interval hierarchy over the

@@ -333,7 +333,7 @@ static __global__ void k_morton_refs(uint32_t n, const uint32_t *vals, const uin
 // over all triangle references of that level (references of one node are
 // contiguous, in increasing triangle order).
 //   k_sah_bins      per node and axis, counts of clipped reference boxes
-//                   starting and ending in each of SAH_BINS bins
+//                   starting and ending in each of SAH_BINS (64) bins
 //   k_sah_split     per node: the cheapest plane on a bin boundary against
 //                   the leaf cost (the reference's SAH cost with its empty
 //                   -side factor 0.8, kd_tree_sah.cpp:141-166), or a leaf
@@ -348,7 +348,7 @@ static __global__ void k_morton_refs(uint32_t n, const uint32_t *vals, const uin
 // Both children of a split share the plane as their clip (the KD form of the
 // node step).  Everything is deterministic; the oracle's twin reproduces it
 // word for word.
-constexpr int SAH_BINS = 32;
+constexpr int SAH_BINS = 64;
 constexpr int SAH_HIST = 3 * 2 * SAH_BINS;  // per node: axis x {start, end} x bin
 constexpr int SAH_CHUNK = 2048;             // references per block of k_sah_bins
 

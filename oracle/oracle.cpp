@@ -1280,7 +1280,7 @@ ORC_API uint32_t orc_lbvh_build(const float *abc, uint64_t n, uint32_t leaf_size
 }
 
 // Twin of the GPU binned-SAH KD build (PRAY_BUILD_SAH): level by level, the
-// same bins (32 per axis over the node's cell, clipped reference boxes), the
+// same bins (64 per axis over the node's cell, clipped reference boxes), the
 // same cost (traversal + intersection x area-weighted counts, 0.8 for an
 // empty side, kd_tree_sah.cpp:141-166) evaluated in the same order, the same
 // classification and stable partitions, and the same breadth-first numbering
@@ -1294,7 +1294,7 @@ struct Node {
 };
 
 static int bin(float x, float lo, float scale) {
-    return (int)std::fmax(0.f, std::fmin(31.f, std::floor((x - lo) * scale)));
+    return (int)std::fmax(0.f, std::fmin(63.f, std::floor((x - lo) * scale)));
 }
 
 static float half_area(float dx, float dy, float dz) { return dx * dy + dy * dz + dz * dx; }
@@ -1338,12 +1338,12 @@ ORC_API uint64_t orc_sah_build(const float *abc, uint64_t n, uint32_t leaf_size,
             const float dx = N.hi[0] - N.lo[0], dy = N.hi[1] - N.lo[1], dz = N.hi[2] - N.lo[2];
             const float sa = sahbuild::half_area(dx, dy, dz);
             if (N.count > leaf_size && level < max_depth && sa > 0.f) {
-                uint32_t H[3][2][32] = {};
+                uint32_t H[3][2][64] = {};
                 for (uint32_t i = N.begin; i < N.begin + N.count; i++) {
                     const uint32_t t = refs[i];
                     for (int a = 0; a < 3; a++) {
                         const float ext = N.hi[a] - N.lo[a];
-                        const float scale = ext > 0.f ? 32.0f / ext : 0.f;
+                        const float scale = ext > 0.f ? 64.0f / ext : 0.f;
                         const float clo = std::fmax(lo[3 * t + a], N.lo[a]);
                         const float chi = std::fmin(hi[3 * t + a], N.hi[a]);
                         H[a][0][sahbuild::bin(clo, N.lo[a], scale)]++;
@@ -1355,11 +1355,11 @@ ORC_API uint64_t orc_sah_build(const float *abc, uint64_t n, uint32_t leaf_size,
                     const float ext = N.hi[a] - N.lo[a];
                     if (!(ext > 0.f)) continue;
                     uint32_t nl = 0, ends = 0;
-                    for (int b = 1; b < 32; b++) {
+                    for (int b = 1; b < 64; b++) {
                         nl += H[a][0][b - 1];
                         ends += H[a][1][b - 1];
                         const uint32_t nr = N.count - ends;
-                        const float p = N.lo[a] + ext * ((float)b * (1.0f / 32.0f));
+                        const float p = N.lo[a] + ext * ((float)b * (1.0f / 64.0f));
                         float l[3] = {dx, dy, dz}, r[3] = {dx, dy, dz};
                         l[a] = p - N.lo[a];
                         r[a] = N.hi[a] - p;

@@ -184,7 +184,7 @@ def walk_sah(flat):
 
 
 def root_split(abc, leaf_size, ct, ci):
-    """The root's binned-SAH plane, restated in numpy float32."""
+    """The root's binned-SAH plane (64 bins), restated in numpy float32."""
     t = abc.reshape(-1, 3, 3)
     lo, hi = t.min(1), t.max(1)
     blo, bhi = lo.min(0), hi.max(0)
@@ -201,13 +201,13 @@ def root_split(abc, leaf_size, ct, ci):
         ext = f32(bhi[a] - blo[a])
         if not ext > 0:
             continue
-        scale = f32(f32(32) / ext)
-        bl = np.clip(np.floor(((lo[:, a] - blo[a]).astype(np.float32) * scale).astype(np.float32)), 0, 31).astype(int)
-        bh = np.clip(np.floor(((hi[:, a] - blo[a]).astype(np.float32) * scale).astype(np.float32)), 0, 31).astype(int)
-        starts, ends = np.bincount(bl, minlength=32), np.bincount(bh, minlength=32)
-        for b in range(1, 32):
+        scale = f32(f32(64) / ext)
+        bl = np.clip(np.floor(((lo[:, a] - blo[a]).astype(np.float32) * scale).astype(np.float32)), 0, 63).astype(int)
+        bh = np.clip(np.floor(((hi[:, a] - blo[a]).astype(np.float32) * scale).astype(np.float32)), 0, 63).astype(int)
+        starts, ends = np.bincount(bl, minlength=64), np.bincount(bh, minlength=64)
+        for b in range(1, 64):
             nl, nr = int(starts[:b].sum()), n - int(ends[:b].sum())
-            p = f32(blo[a] + f32(ext * f32(f32(b) * f32(1 / 32))))
+            p = f32(blo[a] + f32(ext * f32(f32(b) * f32(1 / 64))))
             l, r = d.copy(), d.copy()
             l[a], r[a] = f32(p - blo[a]), f32(bhi[a] - p)
             pl, pr = f32(ha(*l) / sa), f32(ha(*r) / sa)
