@@ -78,8 +78,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the ray stream would shard by ray index across "
-                                   "GPUs (see DESIGN.md), but --gpus > 1 is not supported"}))
+        print(json.dumps({"error": "one GPU only: the ray stream would shard by ray index across GPUs (each "
+                                   "holding the whole scene), but --gpus > 1 is not supported"}))
         sys.exit(2)
 
     import nostrum_pray_amd as P
