@@ -61,10 +61,11 @@ def main():
                          "layout,stack_lds,leaf_ilp,fetch_min,occupancy")
     ap.add_argument("--no-ref-tree", action="store_true",
                     help="skip timing the same rays over the tree built with the reference's parameters")
-    ap.add_argument("--tree", choices=("gpu-sah", "cpu-sah"), default="gpu-sah",
-                    help="the timed tree: SAH KD tree built on the GPU (binned) or on the CPU (exact events)")
-    ap.add_argument("--no-cpu-tree", action="store_true",
-                    help="skip timing the same rays over the CPU-built SAH tree")
+    ap.add_argument("--tree", choices=("cpu-sah", "gpu-sah"), default="cpu-sah",
+                    help="the timed tree: SAH KD tree built on the CPU and uploaded (the workload's definition) or "
+                         "built on the GPU (binned)")
+    ap.add_argument("--no-other-tree", action="store_true",
+                    help="skip timing the same rays over the SAH tree built the other way")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=1 << 24, help="CPU baseline sample (prefix of the stream)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -164,7 +165,7 @@ def main():
     ctx.sync()
     cnt = d_cnt.numpy().astype(np.int64).sum(0)
     hits = int((d_idx.numpy() >= 0).sum())
-    d_cnt.free()
+    d_cnt2 = d_cnt  # reused for the other tree's counts
     tun = ctx.get_tuning(sid)
     # Algorithmic bytes of the traversal kernel: ray 24 + result 8 + 8 per
     # node visited + 40 per triangle reference tested (4-byte index + a, ab,
@@ -216,30 +217,46 @@ def main():
         with open(pmc) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
-    # The same rays over the SAH tree built on the CPU with exact events
-    # (when the timed tree is the GPU-built one): identical hits.
-    cpu_sah = None
-    if args.tree == "gpu-sah" and not args.no_cpu_tree:
+    # The same rays over the SAH tree built the other way (GPU-built when the
+    # timed tree is the CPU one, and the reverse), with its own autotuned
+    # setting: identical hits.
+    other = None
+    if not args.no_other_tree:
         t1 = time.time()
-        ctree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
-        cbuild = time.time() - t1
-        cst = ctree.stats()
-        csid = ctx.upload(ctree, shading=False)
-        del ctree
-        if tuning is not None:  # its own autotuned setting
-            ctx.autotune(csid, n_rays=n_rays_for_tuning(args), force=True)
+        if args.tree == "cpu-sah":
+            abc, _, _ = scene.triangles()
+            osid, obms = ctx.build_scene(abc, method=P.BUILD_SAH, leaf_size=args.leaf_size, max_depth=args.max_depth,
+                                         traversal_cost=args.traversal_cost)
+            del abc
+            olay = ctx.scene_layout(osid)
+            odesc = (f"SAH KD tree built on the GPU (64 bins, same parameters): depth {olay['max_depth']}, "
+                     f"{len(olay['nodes']) // 2} nodes, {len(olay['refs'])} references")
+            obuild = {"build_ms": round(obms, 3), "build_wall_s_with_upload": round(time.time() - t1, 3)}
         else:
-            ctx.set_tuning(csid, **tun)
-        ctun = ctx.get_tuning(csid)
-        ms_c, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=csid)
-        same_c = bool(np.array_equal(closest_idx, d_idx.numpy()) and
+            ctree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size,
+                                traversal_cost=args.traversal_cost)
+            obuild = {"build_s": round(time.time() - t1, 2)}
+            cst = ctree.stats()
+            osid = ctx.upload(ctree, shading=False)
+            del ctree
+            odesc = (f"SAH KD tree built on the CPU (exact events, same parameters): depth {cst['maxdepth']}, "
+                     f"{cst['flat_nodes']} nodes, {cst['flat_refs']} references")
+        if tuning is not None:  # its own autotuned setting
+            ctx.autotune(osid, n_rays=n_rays_for_tuning(args), force=True)
+        else:
+            ctx.set_tuning(osid, **tun)
+        otun = ctx.get_tuning(osid)
+        ctx.trace_counts_device(osid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, d_cnt2.ptr, P.TRACE_CLOSEST, args.det_eps)
+        ocnt = d_cnt2.numpy().astype(np.int64).sum(0) / n
+        ms_o, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=osid)
+        same_o = bool(np.array_equal(closest_idx, d_idx.numpy()) and
                       np.array_equal(closest_t.view(np.uint32), d_t.numpy().view(np.uint32)))
-        ctx.free_scene(csid)
-        cpu_sah = {"tree": f"SAH KD tree built on the CPU (exact events, same parameters): depth {cst['maxdepth']}, "
-                           f"{cst['flat_nodes']} nodes, {cst['flat_refs']} references",
-                   "build_s": round(cbuild, 2), "tuning": ctun, "ms_per_step": round(ms_c, 4),
-                   "Mrays_per_s": round(n / (ms_c * 1e-3) / 1e6, 2), "identical_hits": same_c}
-        log(f"CPU SAH tree (built in {cbuild:.2f} s): {ms_c:.3f} ms/step, identical hits {same_c}")
+        ctx.free_scene(osid)
+        other = {"tree": odesc, **obuild, "tuning": otun, "nodes_per_ray": round(float(ocnt[0]), 2),
+                 "refs_per_ray": round(float(ocnt[1]), 2), "ms_per_step": round(ms_o, 4),
+                 "Mrays_per_s": round(n / (ms_o * 1e-3) / 1e6, 2), "identical_hits": same_o}
+        log(f"other tree ({odesc}; {obuild}): {ms_o:.3f} ms/step = {n / (ms_o * 1e-3) / 1e6:.1f} Mrays/s, "
+            f"identical hits {same_o}")
 
     # The same rays over the tree built with the reference's parameters
     # (depth cap 20, leaf size 32, SAH costs 5 / 1.5): identical hits, more
@@ -262,9 +279,9 @@ def main():
 
     rc_out = None
     if not args.no_cache:
-        for b in (d_rays, d_idx, d_t, d_lim):
+        for b in (d_rays, d_idx, d_t, d_lim, d_cnt2):
             b.free()
-        d_rays = d_idx = d_t = d_lim = None
+        d_rays = d_idx = d_t = d_lim = d_cnt2 = None
         ctx.free_scene(sid)
         sid = None
         rc_out = radiance_cache_leg(P, ctx, args)
@@ -311,12 +328,12 @@ def main():
         },
         "any_hit": {"ms_per_step": round(ms_any, 4), "Mrays_per_s": round(n / (ms_any * 1e-3) / 1e6, 2)},
         "reference_parameter_tree": ref_tree,
-        "cpu_sah_tree": cpu_sah,
+        "gpu_built_sah_tree" if args.tree == "cpu-sah" else "cpu_built_sah_tree": other,
         "radiance_cache": rc_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    for b in (d_rays, d_idx, d_t, d_lim):
+    for b in (d_rays, d_idx, d_t, d_lim, d_cnt2):
         if b is not None:
             b.free()
     if sid is not None:
@@ -430,7 +447,8 @@ def radiance_cache_leg(P, ctx, args):
     --ref-spp samples per pixel, RMSE on linear HDR."""
     w = h = args.cache_size
     scene = P.Scene.cornell(1, 100_000)
-    sid = gpu_sah_scene(P, ctx, scene, args)
+    tree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+    sid = ctx.upload(tree)
     if not args.no_autotune:
         ctx.autotune(sid, n_rays=1 << 22, force=True)
     inf = scene.info()
@@ -470,7 +488,7 @@ def radiance_cache_leg(P, ctx, args):
     best = cache[-1]
     return {
         "scene": f"Cornell room, {scene.num_tris} triangles, {w}x{h}, depth {depth}, roulette from bounce {rr}, "
-                 f"SAH KD tree built on the GPU (leaf size {args.leaf_size})",
+                 f"SAH KD tree built on the CPU (leaf size {args.leaf_size})",
         "reference_spp": args.ref_spp,
         "cache": {"network": "192 (172 used) -> 5 x 64 ReLU -> 3, BF16 matrix cores, FP32 Adam lr 1e-2",
                   "training_samples_per_frame": 1 << 16, "terminates_at_hit": 2, "warmup_frames": warm},
