@@ -61,11 +61,11 @@ def main():
                          "layout,stack_lds,leaf_ilp,fetch_min,occupancy")
     ap.add_argument("--no-ref-tree", action="store_true",
                     help="skip timing the same rays over the tree built with the reference's parameters")
-    ap.add_argument("--tree", choices=("cpu-sah", "gpu-sah"), default="cpu-sah",
-                    help="the timed tree: SAH KD tree built on the CPU and uploaded (the workload's definition) or "
-                         "built on the GPU (binned)")
+    ap.add_argument("--tree", choices=("cpu-sah", "cpu-sah-events", "gpu-sah"), default="cpu-sah",
+                    help="the timed tree: SAH KD tree built on the CPU with 64 bins per axis (default) or with the "
+                         "reference's event sweep, and uploaded (the workload's definition), or built on the GPU")
     ap.add_argument("--no-other-tree", action="store_true",
-                    help="skip timing the same rays over the SAH tree built the other way")
+                    help="skip timing the same rays over the trees built the other ways")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rays", type=int, default=1 << 24, help="CPU baseline sample (prefix of the stream)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -93,41 +93,8 @@ def main():
     scene = P.Scene.terrain(args.grid, args.seed)
     n_tris = scene.num_tris
     ctx = P.Context(0)
-    cpu_tree = None
-    if args.tree == "gpu-sah":
-        # SAH KD tree built on the GPU (median of three builds timed).
-        abc, _, _ = scene.triangles()
-        builds = []
-        for _ in range(3):
-            t1 = time.time()
-            sid, bms = ctx.build_scene(abc, method=P.BUILD_SAH, leaf_size=args.leaf_size, max_depth=args.max_depth,
-                                       traversal_cost=args.traversal_cost)
-            builds.append((bms, time.time() - t1))
-            if len(builds) < 3:
-                ctx.free_scene(sid)
-        del abc
-        lay = ctx.scene_layout(sid)
-        tree_tag = f"gpusah-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}"
-        tree_depth, tree_nodes, tree_refs = lay["max_depth"], len(lay["nodes"]) // 2, len(lay["refs"])
-        raw_box = lay["box"]
-        tree_build = {"where": "GPU", "method": "binned SAH, 64 bins per axis, level by level",
-                      "build_ms": round(sorted(b[0] for b in builds)[1], 3),
-                      "build_wall_s_with_upload": round(sorted(b[1] for b in builds)[1], 3)}
-        tree_desc = (f"SAH KD tree built on the GPU (64 bins; depth cap {args.max_depth}, leaf size {args.leaf_size}, "
-                     f"traversal cost {args.traversal_cost:g})")
-    else:
-        t1 = time.time()
-        cpu_tree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size,
-                               traversal_cost=args.traversal_cost)
-        cpu_build_s = time.time() - t1
-        st = cpu_tree.stats()
-        sid = ctx.upload(cpu_tree, shading=False)
-        tree_tag = f"sah-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}"
-        tree_depth, tree_nodes, tree_refs = st["maxdepth"], st["flat_nodes"], st["flat_refs"]
-        raw_box = cpu_tree.box()
-        tree_build = {"where": "CPU", "method": "exact-event SAH", "build_s": round(cpu_build_s, 2)}
-        tree_desc = (f"SAH KD tree built on the CPU (exact events; depth cap {args.max_depth}, leaf size "
-                     f"{args.leaf_size}, traversal cost {args.traversal_cost:g})")
+    sid, tree_tag, tree_desc, tree_build, (tree_depth, tree_nodes, tree_refs), raw_box = \
+        build_tree(P, ctx, scene, args, args.tree)
     log(f"scene {n_tris} tris, tree {tree_tag}: depth {tree_depth}, {tree_nodes} nodes, {tree_refs} references, "
         f"{tree_build} ({time.time() - t0:.1f}s)")
 
@@ -217,46 +184,32 @@ def main():
         with open(pmc) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
-    # The same rays over the SAH tree built the other way (GPU-built when the
-    # timed tree is the CPU one, and the reverse), with its own autotuned
-    # setting: identical hits.
-    other = None
+    # The same rays over the SAH trees built the other ways (each with its
+    # own autotuned setting): identical hits.
+    others = {}
     if not args.no_other_tree:
-        t1 = time.time()
-        if args.tree == "cpu-sah":
-            abc, _, _ = scene.triangles()
-            osid, obms = ctx.build_scene(abc, method=P.BUILD_SAH, leaf_size=args.leaf_size, max_depth=args.max_depth,
-                                         traversal_cost=args.traversal_cost)
-            del abc
-            olay = ctx.scene_layout(osid)
-            odesc = (f"SAH KD tree built on the GPU (64 bins, same parameters): depth {olay['max_depth']}, "
-                     f"{len(olay['nodes']) // 2} nodes, {len(olay['refs'])} references")
-            obuild = {"build_ms": round(obms, 3), "build_wall_s_with_upload": round(time.time() - t1, 3)}
-        else:
-            ctree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size,
-                                traversal_cost=args.traversal_cost)
-            obuild = {"build_s": round(time.time() - t1, 2)}
-            cst = ctree.stats()
-            osid = ctx.upload(ctree, shading=False)
-            del ctree
-            odesc = (f"SAH KD tree built on the CPU (exact events, same parameters): depth {cst['maxdepth']}, "
-                     f"{cst['flat_nodes']} nodes, {cst['flat_refs']} references")
-        if tuning is not None:  # its own autotuned setting
-            ctx.autotune(osid, n_rays=n_rays_for_tuning(args), force=True)
-        else:
-            ctx.set_tuning(osid, **tun)
-        otun = ctx.get_tuning(osid)
-        ctx.trace_counts_device(osid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, d_cnt2.ptr, P.TRACE_CLOSEST, args.det_eps)
-        ocnt = d_cnt2.numpy().astype(np.int64).sum(0) / n
-        ms_o, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=osid)
-        same_o = bool(np.array_equal(closest_idx, d_idx.numpy()) and
-                      np.array_equal(closest_t.view(np.uint32), d_t.numpy().view(np.uint32)))
-        ctx.free_scene(osid)
-        other = {"tree": odesc, **obuild, "tuning": otun, "nodes_per_ray": round(float(ocnt[0]), 2),
-                 "refs_per_ray": round(float(ocnt[1]), 2), "ms_per_step": round(ms_o, 4),
-                 "Mrays_per_s": round(n / (ms_o * 1e-3) / 1e6, 2), "identical_hits": same_o}
-        log(f"other tree ({odesc}; {obuild}): {ms_o:.3f} ms/step = {n / (ms_o * 1e-3) / 1e6:.1f} Mrays/s, "
-            f"identical hits {same_o}")
+        for which in TREES:
+            if which == args.tree:
+                continue
+            osid, _, odesc, obuild, (od, on, orf), _ = build_tree(P, ctx, scene, args, which)
+            if tuning is not None:
+                ctx.autotune(osid, n_rays=n_rays_for_tuning(args), force=True)
+            else:
+                ctx.set_tuning(osid, **tun)
+            otun = ctx.get_tuning(osid)
+            ctx.trace_counts_device(osid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, d_cnt2.ptr, P.TRACE_CLOSEST,
+                                    args.det_eps)
+            ocnt = d_cnt2.numpy().astype(np.int64).sum(0) / n
+            ms_o, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=osid)
+            same_o = bool(np.array_equal(closest_idx, d_idx.numpy()) and
+                          np.array_equal(closest_t.view(np.uint32), d_t.numpy().view(np.uint32)))
+            ctx.free_scene(osid)
+            others[which] = {"tree": f"{odesc}: depth {od}, {on} nodes, {orf} references", **obuild,
+                             "tuning": otun, "nodes_per_ray": round(float(ocnt[0]), 2),
+                             "refs_per_ray": round(float(ocnt[1]), 2), "ms_per_step": round(ms_o, 4),
+                             "Mrays_per_s": round(n / (ms_o * 1e-3) / 1e6, 2), "identical_hits": same_o}
+            log(f"{which}: {odesc}, {obuild}: {ms_o:.3f} ms/step = {n / (ms_o * 1e-3) / 1e6:.1f} Mrays/s, "
+                f"identical hits {same_o}")
 
     # The same rays over the tree built with the reference's parameters
     # (depth cap 20, leaf size 32, SAH costs 5 / 1.5): identical hits, more
@@ -328,7 +281,7 @@ def main():
         },
         "any_hit": {"ms_per_step": round(ms_any, 4), "Mrays_per_s": round(n / (ms_any * 1e-3) / 1e6, 2)},
         "reference_parameter_tree": ref_tree,
-        "gpu_built_sah_tree" if args.tree == "cpu-sah" else "cpu_built_sah_tree": other,
+        "other_trees": others,
         "radiance_cache": rc_out,
         "cpu_baseline": cpu,
     }
@@ -406,6 +359,48 @@ def batch_workload(P, args):
     print(json.dumps(out), flush=True)
 
 
+TREES = {
+    "cpu-sah": "SAH KD tree built on the CPU (64 bins per axis)",
+    "cpu-sah-events": "SAH KD tree built on the CPU (the reference's event sweep)",
+    "gpu-sah": "SAH KD tree built on the GPU (64 bins per axis)",
+}
+
+
+def build_tree(P, ctx, scene, args, which):
+    """Builds and uploads the terrain's tree: (scene id, tag, description,
+    build info, (depth, nodes, references), raw box)."""
+    params = (f"depth cap {args.max_depth}, leaf size {args.leaf_size}, traversal cost "
+              f"{args.traversal_cost:g}")
+    if which == "gpu-sah":
+        abc, _, _ = scene.triangles()
+        builds = []
+        for _ in range(3):  # the median of three builds
+            t1 = time.time()
+            sid, bms = ctx.build_scene(abc, method=P.BUILD_SAH, leaf_size=args.leaf_size, max_depth=args.max_depth,
+                                       traversal_cost=args.traversal_cost)
+            builds.append((bms, time.time() - t1))
+            if len(builds) < 3:
+                ctx.free_scene(sid)
+        del abc
+        lay = ctx.scene_layout(sid)
+        info = {"where": "GPU", "method": "binned SAH, 64 bins per axis, level by level",
+                "build_ms": round(sorted(b[0] for b in builds)[1], 3),
+                "build_wall_s_with_upload": round(sorted(b[1] for b in builds)[1], 3)}
+        return (sid, f"gpusah-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}",
+                f"{TREES[which]}, {params}", info,
+                (lay["max_depth"], len(lay["nodes"]) // 2, len(lay["refs"])), lay["box"])
+    kind, tag, method = (2, "binsah", "binned SAH, 64 bins per axis") if which == "cpu-sah" else \
+        (1, "sah", "exact-event SAH (the reference's sweep)")
+    t1 = time.time()
+    tree = scene.build(kind, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+    build_s = time.time() - t1
+    st = tree.stats()
+    sid = ctx.upload(tree, shading=False)
+    return (sid, f"{tag}-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}",
+            f"{TREES[which]}, {params}", {"where": "CPU", "method": method, "build_s": round(build_s, 2)},
+            (st["maxdepth"], st["flat_nodes"], st["flat_refs"]), tree.box())
+
+
 def gpu_sah_scene(P, ctx, scene, args):
     """Uploads a scene with its SAH KD tree built on the GPU (with shading
     data)."""
@@ -447,7 +442,7 @@ def radiance_cache_leg(P, ctx, args):
     --ref-spp samples per pixel, RMSE on linear HDR."""
     w = h = args.cache_size
     scene = P.Scene.cornell(1, 100_000)
-    tree = scene.build(1, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+    tree = scene.build(2, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
     sid = ctx.upload(tree)
     if not args.no_autotune:
         ctx.autotune(sid, n_rays=1 << 22, force=True)
@@ -488,7 +483,7 @@ def radiance_cache_leg(P, ctx, args):
     best = cache[-1]
     return {
         "scene": f"Cornell room, {scene.num_tris} triangles, {w}x{h}, depth {depth}, roulette from bounce {rr}, "
-                 f"SAH KD tree built on the CPU (leaf size {args.leaf_size})",
+                 f"SAH KD tree built on the CPU (64 bins per axis, leaf size {args.leaf_size})",
         "reference_spp": args.ref_spp,
         "cache": {"network": "192 (172 used) -> 5 x 64 ReLU -> 3, BF16 matrix cores, FP32 Adam lr 1e-2",
                   "training_samples_per_frame": 1 << 16, "terminates_at_hit": 2, "warmup_frames": warm},

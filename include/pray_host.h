@@ -65,16 +65,18 @@ PRAY_API void ph_camera(const ph_scene *s, float *camera16);
 /* width*height*6 floats; host restatement of calculatePrimRays. */
 PRAY_API void ph_prim_rays(const ph_scene *s, float *rays);
 
-/* Tree build over the scene's triangles: kind 0 = median, 1 = SAH,
- * -1 = the reference's policy. */
+/* Tree build over the scene's triangles: kind 0 = median, 1 = SAH (the
+ * reference's event sweep), 2 = binned SAH (64 bins per axis, this build's
+ * extension: the CPU twin of pray_build_scene's PRAY_BUILD_SAH, flattened
+ * with the host's treelet layout), -1 = the reference's policy. */
 PRAY_API ph_tree *ph_build(ph_scene *s, int kind);
 /* Same with an explicit depth cap (the reference's is 20; deeper trees have
  * smaller leaves and produce identical hits). */
 PRAY_API ph_tree *ph_build_depth(ph_scene *s, int kind, int max_depth);
 /* Same with the leaf size below which nodes are not split (reference 32,
- * kd_tree.h:50) and, for the SAH tree, the traversal and intersection costs
- * (reference 5 and 1.5, kd_tree_sah.h:49-50).  Values <= 0 keep the
- * reference's.  Any tree produces the same closest hits; these only trade
+ * kd_tree.h:50) and, for the SAH trees, the traversal and intersection costs
+ * (reference 5 and 1.5, kd_tree_sah.h:49-50; binned SAH 2 and 1).  Values
+ * <= 0 keep the defaults.  Any tree produces the same closest hits; these only trade
  * node visits against triangle tests. */
 PRAY_API ph_tree *ph_build_ex(ph_scene *s, int kind, int max_depth, int leaf_size, float traversal_cost,
                               float intersection_cost);

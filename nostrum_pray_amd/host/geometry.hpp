@@ -55,6 +55,7 @@ struct Vec3 {
     Vec3 &operator+=(const Vec3 &v) { x += v.x, y += v.y, z += v.z; return *this; }
     bool operator==(const Vec3 &v) const { return x == v.x && y == v.y && z == v.z; }
     float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    float &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
     float &at(int i) { return i == 0 ? x : (i == 1 ? y : z); }
     Vec3 clamp(float lo, float hi) const { return Vec3(clampf(lo, hi, x), clampf(lo, hi, y), clampf(lo, hi, z)); }
 };

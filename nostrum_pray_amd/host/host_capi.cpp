@@ -164,7 +164,9 @@ ph_tree *ph_build_ex(ph_scene *s, int kind, int max_depth, int leaf_size, float 
     if (kind >= 0) S.treeKind = kind;
     t->kind = pray::chooseTree(S);
     S.treeKind = saved;
-    if (t->kind == 1)
+    if (t->kind == 2)
+        t->tree = std::make_unique<pray::KDTreeBinnedSAH>();
+    else if (t->kind == 1)
         t->tree = std::make_unique<pray::KDTreeSAH>();
     else
         t->tree = std::make_unique<pray::KDTree>();
@@ -173,6 +175,10 @@ ph_tree *ph_build_ex(ph_scene *s, int kind, int max_depth, int leaf_size, float 
     if (auto *sah = dynamic_cast<pray::KDTreeSAH *>(t->tree.get())) {
         if (traversal_cost > 0) sah->traversalCost = traversal_cost;
         if (intersection_cost > 0) sah->intersectionCost = intersection_cost;
+    }
+    if (auto *bin = dynamic_cast<pray::KDTreeBinnedSAH *>(t->tree.get())) {
+        if (traversal_cost > 0) bin->traversalCost = traversal_cost;
+        if (intersection_cost > 0) bin->intersectionCost = intersection_cost;
     }
     t->tree->build(S.triangles);
     t->flat = pray::buildGPUTreeFromKDTree(*t->tree);

@@ -2,6 +2,7 @@
 #include "kd_tree.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <parallel/algorithm>
 
@@ -376,6 +377,112 @@ void exportTree(const KDTreeElement *root, TreeExport &out) {
             if (n->left) stack.push_back(n->left.get());
         }
     }
+}
+
+
+// ---------------------------------------------------------- binned SAH ---
+
+void KDTreeBinnedSAH::build(const std::vector<Triangle> &tris) {
+    triangles = tris;
+    triangleCount = triangles.size();
+    buildBinned();
+}
+
+void KDTreeBinnedSAH::build(std::vector<Triangle> &&tris) {
+    triangles = std::move(tris);
+    triangleCount = triangles.size();
+    buildBinned();
+}
+
+void KDTreeBinnedSAH::buildBinned() {
+    if (triangles.empty()) {
+        root = std::make_unique<KDTreeElement>();
+        return;
+    }
+    std::vector<uint32_t> T(triangles.size());
+    std::iota(T.begin(), T.end(), 0u);
+    BoundingBox cell = triangles[0].boundingBox;
+    for (size_t i = 1; i < triangles.size(); i++) cell = cell.expand(triangles[i].boundingBox);
+#pragma omp parallel
+#pragma omp single nowait
+    root = rec(std::move(T), cell, 0);
+    finishStats();
+}
+
+namespace {
+inline float halfArea(float dx, float dy, float dz) { return dx * dy + dy * dz + dz * dx; }
+inline int binOf(float x, float lo, float scale) {
+    return (int)std::fmax(0.f, std::fmin((float)(KDTreeBinnedSAH::BINS - 1), std::floor((x - lo) * scale)));
+}
+}  // namespace
+
+std::unique_ptr<KDTreeElement> KDTreeBinnedSAH::rec(std::vector<uint32_t> &&refs, const BoundingBox &cell,
+                                                    size_t depth) {
+    constexpr int B = BINS;
+    const float d[3] = {cell.max.x - cell.min.x, cell.max.y - cell.min.y, cell.max.z - cell.min.z};
+    const float sa = halfArea(d[0], d[1], d[2]);
+    const size_t n = refs.size();
+    int bestA = -1;
+    float bestP = 0.f;
+    if (n > leafSize && depth < depthLimit && sa > 0.f) {
+        uint32_t H[3][2][B] = {};
+        for (uint32_t t : refs) {
+            const BoundingBox &tb = triangles[t].boundingBox;
+            for (int a = 0; a < 3; a++) {
+                const float ext = cell.max[a] - cell.min[a];
+                const float scale = ext > 0.f ? (float)B / ext : 0.f;
+                const float clo = std::fmax(tb.min[a], cell.min[a]), chi = std::fmin(tb.max[a], cell.max[a]);
+                H[a][0][binOf(clo, cell.min[a], scale)]++;
+                H[a][1][binOf(chi, cell.min[a], scale)]++;
+            }
+        }
+        float best = intersectionCost * (float)n;
+        for (int a = 0; a < 3; a++) {
+            const float ext = cell.max[a] - cell.min[a];
+            if (!(ext > 0.f)) continue;
+            uint32_t nl = 0, ends = 0;
+            for (int b = 1; b < B; b++) {
+                nl += H[a][0][b - 1];
+                ends += H[a][1][b - 1];
+                const uint32_t nr = (uint32_t)n - ends;
+                const float p = cell.min[a] + ext * ((float)b * (1.0f / (float)B));
+                float l[3] = {d[0], d[1], d[2]}, r[3] = {d[0], d[1], d[2]};
+                l[a] = p - cell.min[a];
+                r[a] = cell.max[a] - p;
+                const float pl = halfArea(l[0], l[1], l[2]) / sa, pr = halfArea(r[0], r[1], r[2]) / sa;
+                float cost = traversalCost + intersectionCost * (pl * (float)nl + pr * (float)nr);
+                if ((nl == 0 || nr == 0) && !(pl == 1.f || pr == 1.f)) cost *= 0.8f;
+                if (cost < best) best = cost, bestA = a, bestP = p;
+            }
+        }
+    }
+    if (bestA < 0) return std::make_unique<KDTreeElement>(cell, std::move(refs));
+    std::vector<uint32_t> L, R;
+    for (uint32_t t : refs) {
+        const BoundingBox &tb = triangles[t].boundingBox;
+        const float clo = std::fmax(tb.min[bestA], cell.min[bestA]), chi = std::fmin(tb.max[bestA], cell.max[bestA]);
+        if (clo < bestP || (clo == bestP && chi == bestP)) L.push_back(t);
+        if (chi > bestP) R.push_back(t);
+    }
+    std::vector<uint32_t>().swap(refs);
+    BoundingBox lc = cell, rc = cell;
+    lc.max[bestA] = bestP;
+    rc.min[bestA] = bestP;
+    std::unique_ptr<KDTreeElement> l, r;
+    const bool task = L.size() + R.size() > 4096;
+#pragma omp task shared(l, L, lc) if (task)
+    l = rec(std::move(L), lc, depth + 1);
+#pragma omp task shared(r, R, rc) if (task)
+    r = rec(std::move(R), rc, depth + 1);
+#pragma omp taskwait
+    auto node = std::make_unique<KDTreeElement>();
+    node->boundingBox = cell;
+    node->left = std::move(l);
+    node->right = std::move(r);
+    node->splitPlane.axis = (uint8_t)bestA;
+    node->splitPlane.distance = bestP;
+    node->log();
+    return node;
 }
 
 }  // namespace pray

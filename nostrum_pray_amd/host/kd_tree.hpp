@@ -6,6 +6,8 @@
 // KDTreeSAH  -- Wald-Havran event-sweep SAH over disjoint spatial cells
 //               (reference src/kd_tree_sah.cpp:32-296,
 //               include/kd_tree_sah.h:14-91).
+// KDTreeBinnedSAH -- the SAH cost on 64 bins per axis (this build's
+//               extension, the CPU twin of the GPU build).
 //
 // Both produce the same pointer tree type (KDTreeElement, mirroring
 // include/kd_tree.h:80-134).  Leaf triangle lists, split planes and node
@@ -100,6 +102,29 @@ public:
 
 private:
     void buildSAH();
+};
+
+// Binned-SAH KD tree (no reference counterpart; an extension of this build):
+// the reference's SAH cost (traversal + intersection x area-weighted counts,
+// 0.8 for an empty side, kd_tree_sah.cpp:141-166) evaluated on the 63
+// boundaries of 64 equal bins per axis of the node's cell instead of on every
+// triangle event.  A reference goes left when its box clipped to the cell
+// starts below the plane (or lies in it) and right when it ends above it.
+// The decisions and their floating-point steps are those of the GPU build
+// (pray_build_scene with PRAY_BUILD_SAH), so both give the same tree; this
+// one is flattened with the host's treelet layout.
+class KDTreeBinnedSAH : public KDTree {
+public:
+    void build(const std::vector<Triangle> &triangles) override;
+    void build(std::vector<Triangle> &&triangles) override;
+
+    static constexpr int BINS = 64;
+    float traversalCost = 2.f;
+    float intersectionCost = 1.f;
+
+private:
+    void buildBinned();
+    std::unique_ptr<KDTreeElement> rec(std::vector<uint32_t> &&refs, const BoundingBox &cell, size_t depth);
 };
 
 // Export of a built tree in depth-first, left-before-right order, used to
