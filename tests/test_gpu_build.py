@@ -126,3 +126,19 @@ def test_gpu_build_terrain_full_size(ctx):
     for gi, gt in res.values():
         assert np.array_equal(gi, ci) and np.array_equal(gt.view(np.uint32), ct.view(np.uint32))
     assert (ci >= 0).mean() > 0.1
+
+
+@pytest.mark.parametrize("name", ["spheres", "terrain96", "cornell", "random"])
+def test_gpu_sah_build_equals_cpu_binned_tree(ctx, name):
+    """The GPU SAH build and the host's binned-SAH builder make the same tree
+    (same splits and leaf lists, different node order)."""
+    from test_oracle_build import canonical
+    abc = np.ascontiguousarray(SCENES[name](), np.float32).reshape(-1, 9)
+    say(f"gpu sah vs cpu binned {name}")
+    sid, _ = ctx.build_scene(abc, method=P.BUILD_SAH, leaf_size=8, max_depth=40)
+    try:
+        g = ctx.scene_layout(sid)
+    finally:
+        ctx.free_scene(sid)
+    f = P.Scene.from_triangles(abc).build(2, max_depth=40, leaf_size=8, traversal_cost=2.0).flat()
+    assert canonical(g) == canonical(f)

@@ -262,3 +262,45 @@ def test_sah_twin_closest_hits_equal_brute_force(name):
     assert np.array_equal(gi, bi)
     h = bi >= 0
     assert np.array_equal(gt[h].view(np.uint32), bt[h].view(np.uint32))
+
+
+def canonical(flat):
+    """Depth-first description of a flat tree, independent of its node
+    order: inner nodes (axis, clip planes) and leaf triangle lists."""
+    nodes = flat["nodes"].reshape(-1, 2)
+    refs = flat["refs"]
+    out, stack = [], [int(nodes[1, 1])]
+    while stack:
+        w = stack.pop()
+        if w & 3 == 3:
+            first = w >> 2
+            if first == 0x3FFFFFFF:
+                out.append(("E",))
+                continue
+            k, lst = first, []
+            while True:
+                lst.append(int(refs[k] & 0x7FFFFFFF))
+                if refs[k] & 0x80000000:
+                    break
+                k += 1
+            out.append(("L", tuple(lst)))
+            continue
+        lc = w >> 2
+        out.append(("N", w & 3, int(nodes[lc, 0]), int(nodes[lc + 1, 0])))
+        stack += [int(nodes[lc + 1, 1]), int(nodes[lc, 1])]
+    return out
+
+
+@pytest.mark.parametrize("name", ["spheres", "terrain96", "cornell", "random", "duplicates", "two", "nine"])
+@pytest.mark.parametrize("leaf_size,depth", [(8, 40), (1, 24)])
+def test_cpu_binned_builder_equals_twin(name, leaf_size, depth):
+    """The product's CPU binned-SAH builder (ph_build_ex kind 2) makes the
+    twin's tree node for node; only the node order (treelet layout)
+    differs."""
+    abc = np.ascontiguousarray(SCENES[name](), np.float32).reshape(-1, 9)
+    s = P.Scene.from_triangles(abc)
+    t = s.build(2, max_depth=depth, leaf_size=leaf_size, traversal_cost=2.0)
+    f = t.flat()
+    o = O.sah_build(abc, leaf_size, depth, 2.0, 1.0)
+    assert canonical(f) == canonical(o)
+    assert f["max_depth"] == o["max_depth"] and t.stats()["kind"] == 2
