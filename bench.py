@@ -142,14 +142,14 @@ def main():
     bytes_per_ray = 24 + 8 + 8 * nodes_per_ray + 40 * refs_per_ray
     log(f"hits {hits}/{n}  nodes/ray {nodes_per_ray:.1f}  refs/ray {refs_per_ray:.1f}  B/ray {bytes_per_ray:.0f}")
 
-    def timed(mode, steps, warmup, lim=None, scene_id=sid):
+    def timed(mode, steps, warmup, lim=None, scene_id=sid, eps=args.det_eps):
         for _ in range(warmup):
-            ctx.trace_device(scene_id, d_rays.ptr, n, d_idx.ptr, d_t.ptr, mode, args.det_eps, d_limit2=lim)
+            ctx.trace_device(scene_id, d_rays.ptr, n, d_idx.ptr, d_t.ptr, mode, eps, d_limit2=lim)
         ctx.sync()
         t_wall = time.perf_counter()
         ctx.record(0)
         for _ in range(steps):
-            ctx.trace_device(scene_id, d_rays.ptr, n, d_idx.ptr, d_t.ptr, mode, args.det_eps, d_limit2=lim)
+            ctx.trace_device(scene_id, d_rays.ptr, n, d_idx.ptr, d_t.ptr, mode, eps, d_limit2=lim)
         ctx.record(1)
         ms_ev = ctx.elapsed_ms(0, 1) / steps
         ctx.sync()
@@ -161,21 +161,40 @@ def main():
     log(f"closest-hit {ms:.3f} ms/step  {mrays:.1f} Mrays/s  (wall {ms_wall:.3f} ms/step)")
     closest_idx, closest_t = d_idx.numpy(), d_t.numpy()
 
-    # Any-hit over the same stream (unlimited distance), reported beside.
-    ms_any, _ = timed(P.TRACE_ANY, args.steps, 2, lim=d_lim.ptr)
-    log(f"any-hit {ms_any:.3f} ms/step  {n / (ms_any * 1e-3) / 1e6:.1f} Mrays/s")
-
     # The traversal kernel alone (HIP events around its launch, on the stream
     # it runs on), without the counter reset in front of it.
-    ctx.kernel_timing(True)
-    kms = []
-    for _ in range(args.steps):
-        ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, P.TRACE_CLOSEST, args.det_eps)
-        kms.append(ctx.last_kernel_ms())
-    ctx.kernel_timing(False)
-    kernel_ms = float(np.mean(kms))
+    def kernel_time(mode, lim=None):
+        ctx.kernel_timing(True)
+        kms = []
+        for _ in range(args.steps):
+            ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, mode, args.det_eps, d_limit2=lim)
+            kms.append(ctx.last_kernel_ms())
+        ctx.kernel_timing(False)
+        return float(np.mean(kms))
+
+    kernel_ms = kernel_time(P.TRACE_CLOSEST)
     log(f"traversal kernel {kernel_ms:.3f} ms/launch (step {ms:.3f} ms)")
     achieved = bytes_per_ray * n / (kernel_ms * 1e-3) / 1e9
+
+    # Any-hit over the same stream (unlimited distance), reported beside with
+    # its own work counts and kernel time.
+    ms_any, _ = timed(P.TRACE_ANY, args.steps, 2, lim=d_lim.ptr)
+    ctx.trace_counts_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, d_cnt.ptr, P.TRACE_ANY, args.det_eps,
+                            d_limit2=d_lim.ptr)
+    ctx.sync()
+    acnt = d_cnt.numpy().astype(np.int64).sum(0) / n
+    any_bytes = 24 + 8 + 8 * acnt[0] + 40 * acnt[1]
+    any_kernel_ms = kernel_time(P.TRACE_ANY, d_lim.ptr)
+    any_achieved = any_bytes * n / (any_kernel_ms * 1e-3) / 1e9
+    log(f"any-hit {ms_any:.3f} ms/step  {n / (ms_any * 1e-3) / 1e6:.1f} Mrays/s  kernel {any_kernel_ms:.3f} ms  "
+        f"nodes/ray {acnt[0]:.1f}  refs/ray {acnt[1]:.1f}  {any_achieved:.0f} GB/s")
+
+    # The reference's determinant cull (1e-4) over the same stream: below the
+    # terrain triangles' scale, so nothing is hit.
+    ms_ref_eps, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, eps=1e-4)
+    ref_eps = {"det_eps": 1e-4, "hit_fraction": round(float((d_idx.numpy() >= 0).mean()), 4),
+               "ms_per_step": round(ms_ref_eps, 4), "Mrays_per_s": round(n / (ms_ref_eps * 1e-3) / 1e6, 2)}
+    log(f"det_eps 1e-4: {ref_eps}")
     traffic = None
     # HBM traffic of this kernel/workload from the committed rocprofv3 PMC
     # summary (FETCH_SIZE/WRITE_SIZE calibrated on a plain copy)
@@ -241,7 +260,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene, args, box)
+        cpu = cpu_baseline(scene, args, box, closest_idx)
 
     out = {
         "metric": METRIC,
@@ -279,7 +298,11 @@ def main():
             "nodes_per_ray": round(float(nodes_per_ray), 2),
             "refs_per_ray": round(float(refs_per_ray), 2),
         },
-        "any_hit": {"ms_per_step": round(ms_any, 4), "Mrays_per_s": round(n / (ms_any * 1e-3) / 1e6, 2)},
+        "any_hit": {"ms_per_step": round(ms_any, 4), "Mrays_per_s": round(n / (ms_any * 1e-3) / 1e6, 2),
+                    "kernel_ms": round(any_kernel_ms, 4), "bytes_per_ray": round(float(any_bytes), 1),
+                    "nodes_per_ray": round(float(acnt[0]), 2), "refs_per_ray": round(float(acnt[1]), 2),
+                    "achieved_GBs": round(any_achieved, 1), "frac": round(any_achieved / HBM_PEAK_GBS, 4)},
+        "reference_det_eps": ref_eps,
         "reference_parameter_tree": ref_tree,
         "other_trees": others,
         "radiance_cache": rc_out,
@@ -494,14 +517,17 @@ def radiance_cache_leg(P, ctx, args):
     }
 
 
-def cpu_baseline(scene, args, box):
+def cpu_baseline(scene, args, box, gpu_idx):
     """The reference CPU path restated (oracle): OpenMP schedule(dynamic,10)
     over 8-ray AVX packets, exhaustive recursive traversal of the median KD tree
-    (the reference's choice at >= 1M triangles), AVX Moeller-Trumbore."""
+    (the reference's choice at >= 1M triangles), AVX Moeller-Trumbore.  Brute
+    force over every triangle (bf_search.cpp:26-40) is timed beside it on the
+    first 4096 rays."""
     import oracle as O
 
     threads = os.cpu_count() or 1
     os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     tree = scene.build(0)
     ex = tree.export()
     fl = tree.flat()
@@ -509,6 +535,12 @@ def cpu_baseline(scene, args, box):
     rays = O.gen_rays(args.seed, box, n)
     idx, t, sec = O.ref_closest_avx(ex, fl["a"], fl["ab"], fl["ac"], rays, args.det_eps, threads)
     log(f"cpu baseline: {n} rays in {sec:.2f}s on {threads} threads ({cpu_model()})")
+    nb = min(4096, n)
+    t0 = time.perf_counter()
+    bidx, _ = O.bf_closest(fl["a"], fl["ab"], fl["ac"], rays[:nb], args.det_eps)
+    bsec = time.perf_counter() - t0
+    same = int((bidx == gpu_idx[:nb]).sum())
+    log(f"brute force: {nb} rays in {bsec:.2f}s, {same}/{nb} hit indices equal to the GPU's")
     return {
         "value": round(n / sec / 1e6, 4),
         "unit": "Mrays/s",
@@ -517,6 +549,10 @@ def cpu_baseline(scene, args, box):
         "sample": f"first {n} rays of the same stream, same det_eps, restated reference AVX packet traversal over "
                   f"the reference's median KD tree (identical to the reference build), {threads} OpenMP threads on "
                   f"{cpu_model()}",
+        "brute_force": {"rays": nb, "seconds": round(bsec, 2), "Mrays_per_s": round(nb / bsec / 1e6, 6),
+                        "same_hit_index_as_gpu": same,
+                        "what": "scalar Moeller-Trumbore over every triangle, lowest index wins ties, "
+                                f"{threads} OpenMP threads"},
     }
 
 
