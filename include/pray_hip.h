@@ -125,8 +125,9 @@ PRAY_API int pray_scene_layout(pray_ctx *ctx, int scene_id, uint64_t *sizes, uin
  *   [1] stack_lds  LDS ring entries per lane (4, 5, 6, 8 or 16); deeper entries
  *                  spill to a per-lane area in HBM
  *   [2] leaf_ilp   triangle records loaded together per leaf step (1 or 2)
- *   [3] fetch_min  a persistent wave refills its finished lanes when at least
- *                  this many are idle (1..64): the queue chunk
+ *   [3] fetch_min  a persistent wave refills its finished lanes (from its
+ *                  pool of ray indices) when at least this many are idle
+ *                  (1..64)
  *   [4] occupancy  percent (1..100) of the waves that fit on the device that
  *                  the persistent grid launches: the workgroup count
  * pray_set_tuning: -1 keeps a value.  Calls of one context must not overlap
@@ -137,7 +138,7 @@ PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *tuning);
 
 /* Autotuner (no reference counterpart): times closest-hit traversal of
  * n_rays incoherent rays through the scene (0 = 2^20) over stack depth x
- * leaf ILP x queue chunk, then occupancy for the fastest of those, and
+ * leaf ILP x refill threshold, then occupancy for the fastest of those, and
  * keeps the fastest setting for the scene and for its size bucket (triangle
  * counts within a factor of 4).  Later scenes of a
  * tuned bucket take the bucket's setting without timing unless `force` is

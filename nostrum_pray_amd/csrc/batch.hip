@@ -84,7 +84,7 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
         };
         for (int K : {4, 6, 8, 16})
             for (int ilp : {1, 2})
-                for (int fm : {32, 40, 48, 64}) {
+                for (int fm : {16, 24, 32, 48}) {
                     Tuning t;
                     t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm;
                     consider(t);

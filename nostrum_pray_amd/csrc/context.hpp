@@ -16,9 +16,9 @@ namespace prc {
 
 struct Tuning {
     int layout = PRAY_LAYOUT_PACKED;
-    int stack_lds = 8;
+    int stack_lds = 6;
     int leaf_ilp = 2;
-    int fetch_min = 32;
+    int fetch_min = 16;
     int occupancy = 100;  // percent of the resident persistent waves launched
 };
 

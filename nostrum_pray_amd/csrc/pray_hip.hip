@@ -199,7 +199,7 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
     a.spill = c->spill;
     a.spill_depth = depth;
     a.fetch_min = (uint32_t)tu.fetch_min;
-    hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, c->work, grid * (uint32_t)WAVE);
+    hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, c->work, 0u);
     bool timed = false;
     if (c->kernel_timing) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -239,7 +239,7 @@ int enqueue_trace_multi(pray_ctx *c, const prt::DevScene *table, const uint32_t 
     a.fetch_min = (uint32_t)tu.fetch_min;
     a.table = table;
     a.ray_scene = ray_scene;
-    hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, c->work, grid * (uint32_t)WAVE);
+    hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, c->work, 0u);
     hipLaunchKernelGGL(k, dim3(grid), dim3(WAVE), 0, st, prt::DevScene{}, a);
     HIPC(c, hipGetLastError());
     return PRAY_OK;

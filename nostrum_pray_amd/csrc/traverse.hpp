@@ -1,9 +1,13 @@
 // Closest-hit / any-hit traversal kernel for gfx950.
 //
 // One lane = one ray at a time; waves are persistent and refill lanes whose
-// ray finished from a global ray counter (one atomic per refill, shared by
-// the idle lanes of the wave through a ballot), so long rays never hold a wave
-// of finished lanes.  Every loop iteration each lane does an inner-node step
+// ray finished from the wave's pool of ray indices, so long rays never hold a
+// wave of finished lanes.  The pool takes rays in chunks of 64: the first 7/8
+// of the chunks go to the waves round robin (chunk w, w + grid, ...), the rest
+// one at a time from a global counter (one atomic per chunk), which balances
+// the tail.  Thousands of resident waves taking every ray from one counter
+// serialise on its address; with the static share a launch issues an atomic
+// per 512 rays instead of one per refill.  Every loop iteration each lane does an inner-node step
 // (one aligned 16-byte child-pair load), a leaf step (ILP triangle records
 // loaded together; the record array is padded so the look-ahead stays in
 // bounds), or both when the node step lands in a leaf.  Pending far children go to a per-lane stack:
@@ -57,7 +61,7 @@ struct TraceArgs {
     int32_t *out_idx;
     float *out_t;             // optional
     uint32_t *counts;         // optional: 2 per ray (nodes, refs)
-    uint32_t *work;           // ray counter, 64 x grid size at launch
+    uint32_t *work;           // chunks taken past the static ones (0 at launch)
     uint4 *spill;             // per lane slot: spill_depth entries
     uint32_t spill_depth;
     uint32_t fetch_min;       // refill when at least this many lanes are idle
@@ -124,7 +128,12 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, lim2 = 0;
     float lo = 0, hi = 0, best_t = INFINITY;
     int sp = 0, base = 0;
-    bool first_fetch = true;
+    // Wave-uniform pool of ray indices [pool_next, pool_end) and the next
+    // static chunk; chunks from static_end on come from the counter.
+    const uint32_t nchunks = (nn + WAVE - 1) / WAVE;
+    const uint32_t static_end =
+        max(1u, (uint32_t)((uint64_t)nchunks * 7u / 8u / gridDim.x)) * gridDim.x;
+    uint32_t pool_next = 0, pool_end = 0, next_static = blockIdx.x;
 
     while (true) {
         // ------------------------------------------------------- refill --
@@ -133,21 +142,26 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
         if (dead == ~0ull) break;
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (nidle && (nidle >= A.fetch_min || (idle | dead) == ~0ull)) {
-            const uint32_t first = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
-            // The first batch of every wave is static (rays 64*block ..),
-            // the counter starts after them: a launch over few rays costs no
-            // atomics, where thousands of resident waves would otherwise
-            // queue on one address.
-            uint32_t b = 0;
-            if (first_fetch) {
-                b = blockIdx.x * WAVE;
-                first_fetch = false;
-            } else {
-                if (lane == first) b = atomicAdd(A.work, nidle);
-                b = __shfl(b, (int)first);
+            // The pool holds fewer rays than idle lanes: the next chunk
+            // (one chunk covers any shortfall, nidle <= 64).
+            const uint32_t avail = pool_end - pool_next;
+            uint32_t nb = 0;
+            if (avail < nidle) {
+                uint32_t c;
+                if (next_static < static_end) {
+                    c = next_static;
+                    next_static += gridDim.x;
+                } else {
+                    const uint32_t first = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
+                    uint32_t d = 0;
+                    if (lane == first) d = atomicAdd(A.work, 1u);
+                    c = static_end + __shfl(d, (int)first);
+                }
+                nb = c * WAVE;
             }
+            const uint32_t r = (uint32_t)__popcll(idle & below);
             if (state == ST_IDLE) {
-                ray = b + (uint32_t)__popcll(idle & below);
+                ray = r < avail ? pool_next + r : nb + (r - avail);
                 if (ray >= nn) {
                     state = ST_DEAD;
                 } else {
@@ -199,6 +213,12 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
                         state = ST_IDLE;
                     }
                 }
+            }
+            if (avail < nidle) {
+                pool_next = nb + (nidle - avail);
+                pool_end = nb + WAVE;
+            } else {
+                pool_next += nidle;
             }
         }
 

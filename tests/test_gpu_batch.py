@@ -60,7 +60,7 @@ def test_autotune_never_changes_hits(ctx):
         say("autotune: tuning")
         tun = ctx.autotune(sid, n_rays=1 << 18, force=True)
         assert tun["stack_lds"] in (4, 6, 8, 16) and tun["leaf_ilp"] in (1, 2) and 1 <= tun["occupancy"] <= 100
-        assert tun["fetch_min"] in (32, 40, 48, 64)
+        assert tun["fetch_min"] in (16, 24, 32, 48)
         assert ctx.get_tuning(sid) == tun
         settings = [dict(), dict(stack_lds=4), dict(stack_lds=16, leaf_ilp=1), dict(fetch_min=16, occupancy=50),
                     dict(fetch_min=64, occupancy=75), dict(layout=P.LAYOUT_INDEXED), dict(stack_lds=6),
