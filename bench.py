@@ -242,15 +242,18 @@ def main():
         t1 = time.time()
         rtree = scene.build(1)
         rsid = ctx.upload(rtree, shading=False)
-        if tuning is not None:
-            ctx.set_tuning(rsid, **tuning)
+        if tuning is not None:  # its own setting: its 32-triangle leaves favour other refill thresholds
+            ctx.autotune(rsid, n_rays=n_rays_for_tuning(args), force=True)
+        else:
+            ctx.set_tuning(rsid, **tun)
         log(f"reference-parameter tree {rtree.stats()} in {time.time() - t1:.1f}s")
         ms_ref, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=rsid)
         same = bool(np.array_equal(closest_idx, d_idx.numpy()) and
                     np.array_equal(closest_t.view(np.uint32), d_t.numpy().view(np.uint32)))
         ctx.free_scene(rsid)
         ref_tree = {"ms_per_step": round(ms_ref, 4), "Mrays_per_s": round(n / (ms_ref * 1e-3) / 1e6, 2),
-                    "identical_hits": same, "tree": "SAH depth cap 20, leaf size 32, costs 5 / 1.5"}
+                    "identical_hits": same, "tree": "SAH depth cap 20, leaf size 32, costs 5 / 1.5",
+                    "tuning": ctx.get_tuning(rsid)}
         log(f"reference-parameter tree: {ms_ref:.3f} ms/step, identical hits {same}")
 
     rc_out = None

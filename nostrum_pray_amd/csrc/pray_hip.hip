@@ -199,7 +199,7 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
     a.spill = c->spill;
     a.spill_depth = depth;
     a.fetch_min = (uint32_t)tu.fetch_min;
-    hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, c->work, 0u);
+    hipLaunchKernelGGL(prt::k_reset_counters, dim3(1), dim3(64), 0, st, c->work);
     bool timed = false;
     if (c->kernel_timing) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -239,7 +239,7 @@ int enqueue_trace_multi(pray_ctx *c, const prt::DevScene *table, const uint32_t 
     a.fetch_min = (uint32_t)tu.fetch_min;
     a.table = table;
     a.ray_scene = ray_scene;
-    hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, c->work, 0u);
+    hipLaunchKernelGGL(prt::k_reset_counters, dim3(1), dim3(64), 0, st, c->work);
     hipLaunchKernelGGL(k, dim3(grid), dim3(WAVE), 0, st, prt::DevScene{}, a);
     HIPC(c, hipGetLastError());
     return PRAY_OK;
@@ -340,7 +340,7 @@ int pray_init(int device, pray_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc((void **)&c->work, 64 * sizeof(uint32_t)) != hipSuccess) {
+        hipMalloc((void **)&c->work, (size_t)prt::NCTR * prt::CTR_STRIDE * sizeof(uint32_t)) != hipSuccess) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return PRAY_ERR_HIP;

@@ -2,12 +2,12 @@
 //
 // One lane = one ray at a time; waves are persistent and refill lanes whose
 // ray finished from the wave's pool of ray indices, so long rays never hold a
-// wave of finished lanes.  The pool takes rays in chunks of 64: the first 7/8
-// of the chunks go to the waves round robin (chunk w, w + grid, ...), the rest
-// one at a time from a global counter (one atomic per chunk), which balances
-// the tail.  Thousands of resident waves taking every ray from one counter
-// serialise on its address; with the static share a launch issues an atomic
-// per 512 rays instead of one per refill.  Every loop iteration each lane does an inner-node step
+// wave of finished lanes.  The pool takes rays in chunks of 64 from NCTR
+// counters 256 bytes apart: counter j hands out chunks j, j + NCTR, ...; a wave
+// starts at counter (block % NCTR) and moves on to the next when one runs dry.
+// Thousands of resident waves taking every refill from one counter serialise
+// on its address (an atomic per refill); here each address sees 1/NCTR of one
+// atomic per 64 rays, and no wave is assumed to be resident from the start.  Every loop iteration each lane does an inner-node step
 // (one aligned 16-byte child-pair load), a leaf step (ILP triangle records
 // loaded together; the record array is padded so the look-ahead stays in
 // bounds), or both when the node step lands in a leaf.  Pending far children go to a per-lane stack:
@@ -39,6 +39,8 @@ constexpr uint32_t LEAF_EMPTY = 0x3FFFFFFFu;
 constexpr uint32_t REF_LAST = 0x80000000u;
 constexpr uint32_t NO_HIT = 0xFFFFFFFFu;
 constexpr float WIDEN = 1e-6f;
+constexpr uint32_t NCTR = 64;        // ray-chunk counters
+constexpr uint32_t CTR_STRIDE = 64;  // uint32 words between counters (256 bytes)
 
 struct DevScene {
     const uint2 *nodes = nullptr;
@@ -61,7 +63,7 @@ struct TraceArgs {
     int32_t *out_idx;
     float *out_t;             // optional
     uint32_t *counts;         // optional: 2 per ray (nodes, refs)
-    uint32_t *work;           // chunks taken past the static ones (0 at launch)
+    uint32_t *work;           // NCTR chunk counters, CTR_STRIDE apart (0 at launch)
     uint4 *spill;             // per lane slot: spill_depth entries
     uint32_t spill_depth;
     uint32_t fetch_min;       // refill when at least this many lanes are idle
@@ -128,12 +130,10 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, lim2 = 0;
     float lo = 0, hi = 0, best_t = INFINITY;
     int sp = 0, base = 0;
-    // Wave-uniform pool of ray indices [pool_next, pool_end) and the next
-    // static chunk; chunks from static_end on come from the counter.
+    // Wave-uniform pool of ray indices [pool_next, pool_end) and the counter
+    // the wave takes its next chunk from.
     const uint32_t nchunks = (nn + WAVE - 1) / WAVE;
-    const uint32_t static_end =
-        max(1u, (uint32_t)((uint64_t)nchunks * 7u / 8u / gridDim.x)) * gridDim.x;
-    uint32_t pool_next = 0, pool_end = 0, next_static = blockIdx.x;
+    uint32_t pool_next = 0, pool_end = 0, ctr = blockIdx.x % NCTR;
 
     while (true) {
         // ------------------------------------------------------- refill --
@@ -147,16 +147,28 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
             const uint32_t avail = pool_end - pool_next;
             uint32_t nb = 0;
             if (avail < nidle) {
-                uint32_t c;
-                if (next_static < static_end) {
-                    c = next_static;
-                    next_static += gridDim.x;
-                } else {
-                    const uint32_t first = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
-                    uint32_t d = 0;
-                    if (lane == first) d = atomicAdd(A.work, 1u);
-                    c = static_end + __shfl(d, (int)first);
+                // One lane walks the counters from the current one until a
+                // chunk is left (a relaxed read skips drained counters without
+                // an atomic); none left: nb points past the last ray.
+                const uint32_t first = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
+                uint32_t c = nchunks;
+                if (lane == first) {
+                    for (uint32_t tries = 0; tries < NCTR; ++tries) {
+                        const uint32_t own = ctr < nchunks ? (nchunks - ctr + NCTR - 1) / NCTR : 0u;
+                        uint32_t *p = A.work + ctr * CTR_STRIDE;
+                        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < own) {
+                            const uint32_t i = atomicAdd(p, 1u);
+                            if (i < own) {
+                                c = ctr + i * NCTR;
+                                break;
+                            }
+                        }
+                        ctr = ctr + 1 == NCTR ? 0u : ctr + 1;
+                    }
                 }
+                // wave-uniform from here: scalar registers
+                c = (uint32_t)__builtin_amdgcn_readlane((int)c, (int)first);
+                ctr = (uint32_t)__builtin_amdgcn_readlane((int)ctr, (int)first);
                 nb = c * WAVE;
             }
             const uint32_t r = (uint32_t)__popcll(idle & below);
@@ -220,6 +232,8 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
             } else {
                 pool_next += nidle;
             }
+            pool_next = (uint32_t)__builtin_amdgcn_readfirstlane((int)pool_next);
+            pool_end = (uint32_t)__builtin_amdgcn_readfirstlane((int)pool_end);
         }
 
         bool need_pop = false;
@@ -366,6 +380,11 @@ static __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 
 static __global__ void k_set_u32(uint32_t *p, uint32_t v) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+// Clears the NCTR chunk counters (one 64-thread block).
+static __global__ void k_reset_counters(uint32_t *work) {
+    if (threadIdx.x < NCTR) work[threadIdx.x * CTR_STRIDE] = 0u;
 }
 
 }  // namespace prt
