@@ -250,10 +250,11 @@ def main():
         ms_ref, _ = timed(P.TRACE_CLOSEST, max(3, args.steps // 4), 1, scene_id=rsid)
         same = bool(np.array_equal(closest_idx, d_idx.numpy()) and
                     np.array_equal(closest_t.view(np.uint32), d_t.numpy().view(np.uint32)))
+        rtun = ctx.get_tuning(rsid)
         ctx.free_scene(rsid)
         ref_tree = {"ms_per_step": round(ms_ref, 4), "Mrays_per_s": round(n / (ms_ref * 1e-3) / 1e6, 2),
                     "identical_hits": same, "tree": "SAH depth cap 20, leaf size 32, costs 5 / 1.5",
-                    "tuning": ctx.get_tuning(rsid)}
+                    "tuning": rtun}
         log(f"reference-parameter tree: {ms_ref:.3f} ms/step, identical hits {same}")
 
     rc_out = None
