@@ -140,19 +140,23 @@ def test_work_counts_match_cpu_twin(ctx):
 
 @pytest.mark.parametrize("ring", [5, 16])
 def test_launch_sizes(ctx, ring):
-    """Traces over ray counts around the wave and the first static batch,
-    one after another through the same counter and spill area, including a
-    ray without direction (its hit is not compared) and one far outside the
-    scene box."""
+    """Traces over ray counts around the wave, the 64-ray chunks and the 64
+    chunk counters (partial last chunks, fewer waves than counters), one after
+    another through the same counters and spill area, including a ray without
+    direction (its hit is not compared) and one far outside the scene box.
+    The last launch has a grid of a few hundred waves for 3125 chunks, so the
+    waves walk the counters as they run dry."""
     _, t, flat = scene("spheres-sah")
     sid = ctx.upload(t)
     try:
         ctx.set_tuning(sid, stack_lds=ring, fetch_min=1 if ring == 5 else 64)
-        base = O.gen_rays(12, P.scene_box_expanded(flat["box"]), 9000)
+        base = O.gen_rays(12, P.scene_box_expanded(flat["box"]), 200_000)
         base[5, 3:] = 0.0
         base[7, :3] = [1e30, -1e30, 0.0]
         keep = np.arange(len(base)) != 5
-        for n in (1, 63, 64, 65, 4095, 4096, 4097, 9000, 100):
+        for n in (1, 63, 64, 65, 4095, 4096, 4097, 9000, 100, 12293, 200_000):
+            if n == 200_000:
+                ctx.set_tuning(sid, occupancy=1)
             rays = np.ascontiguousarray(base[:n])
             say(f"launch ring={ring} n={n}")
             gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, 1e-4)
