@@ -75,8 +75,9 @@ def main():
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-RMSE measurement")
     ap.add_argument("--cache-size", type=int, default=1024, help="image side of the radiance-cache measurement")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples per pixel of the plain reference image")
-    ap.add_argument("--workload", choices=("terrain", "batch"), default="terrain",
-                    help="terrain: the closest-hit stream (default); batch: 64 scenes path traced in one wavefront")
+    ap.add_argument("--workload", choices=("terrain", "batch", "io"), default="terrain",
+                    help="terrain: the closest-hit stream (default); batch: 64 scenes path traced in one wavefront; "
+                         "io: flat tree files and progressive path tracing")
     ap.add_argument("--batch-scenes", type=int, default=64)
     args = ap.parse_args()
 
@@ -89,6 +90,9 @@ def main():
 
     if args.workload == "batch":
         batch_workload(P, args)
+        return
+    if args.workload == "io":
+        io_workload(P, args)
         return
 
     t0 = time.time()
@@ -395,6 +399,101 @@ TREES = {
     "cpu-sah-events": "SAH KD tree built on the CPU (the reference's event sweep)",
     "gpu-sah": "SAH KD tree built on the GPU (64 bins per axis)",
 }
+
+
+def io_workload(P, args):
+    """The flat tree file and progressive path tracing, timed: the terrain's
+    binned SAH tree built on the CPU, written to a flat tree file, read back
+    and uploaded (traces over both uploads must agree); then the Cornell room
+    at --cache-size^2 path traced 16 spp in one call and in 16 progressive
+    calls of one sample each (the images must agree bit for bit), with the
+    accumulation file and the PFM written and read back.  One JSON line."""
+    import shutil
+    import tempfile
+
+    ctx = P.Context(0)
+    tmp = tempfile.mkdtemp(prefix="pray_io_")
+    try:
+        scene = P.Scene.terrain(args.grid, args.seed)
+        t0 = time.perf_counter()
+        tree = scene.build(2, max_depth=args.max_depth, leaf_size=args.leaf_size,
+                           traversal_cost=args.traversal_cost)
+        build_s = time.perf_counter() - t0
+        path = os.path.join(tmp, "terrain.pflat")
+        t0 = time.perf_counter()
+        tree.save(path)
+        save_s = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        loaded = P.Tree.load(path)
+        load_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sid_l = ctx.upload(loaded, shading=False)
+        ctx.sync()
+        upload_s = time.perf_counter() - t0
+        sid_b = ctx.upload(tree, shading=False)
+        n = 1 << 20
+        rays = ctx.gen_rays(args.seed, P.scene_box_expanded(tree.box()), n)
+        ib, tb = ctx.trace(sid_b, rays, P.TRACE_CLOSEST, args.det_eps)
+        il, tl = ctx.trace(sid_l, rays, P.TRACE_CLOSEST, args.det_eps)
+        same_tree = bool(np.array_equal(ib, il) and np.array_equal(tb.view(np.uint32), tl.view(np.uint32)))
+        ctx.free_scene(sid_b)
+        ctx.free_scene(sid_l)
+        del tree, loaded
+        log(f"io: build {build_s:.2f}s, save {save_s:.2f}s ({size / 1e6:.0f} MB), load {load_s:.2f}s, "
+            f"upload {upload_s:.2f}s, identical traces {same_tree}")
+
+        w = h = args.cache_size
+        room = P.Scene.cornell(1, 100_000)
+        rtree = room.build(2, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+        sid = ctx.upload(rtree)
+        inf = room.info()
+        cam, bg = room.camera(), inf["background"]
+        spp, depth, rr, seed = 16, 8, 3, 7
+        ctx.render_pt(sid, cam, w, h, 1, depth, rr, seed, bg, 1e-9, want_rgb=False)  # warm-up
+        t0 = time.perf_counter()
+        one, _ = ctx.render_pt(sid, cam, w, h, spp, depth, rr, seed, bg, 1e-9, want_rgb=False)
+        one_ms = (time.perf_counter() - t0) * 1e3
+        acc = None
+        t0 = time.perf_counter()
+        for s in range(spp):
+            hdr, _, acc = ctx.render_pt_progressive(sid, cam, w, h, s, 1, depth, rr, seed, bg, 1e-9, accum=acc,
+                                                    want_rgb=False)
+        prog_ms = (time.perf_counter() - t0) * 1e3
+        same_img = bool(np.array_equal(one.view(np.uint32), hdr.view(np.uint32)))
+        fp = room.fingerprint()
+        apath, ppath = os.path.join(tmp, "state.acc"), os.path.join(tmp, "image.pfm")
+        t0 = time.perf_counter()
+        P.save_accum(apath, acc, spp, fp)
+        acc_save_ms = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        acc2, n2, fp2 = P.load_accum(apath)
+        acc_load_ms = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        P.write_pfm(ppath, hdr)
+        pfm_ms = (time.perf_counter() - t0) * 1e3
+        same_files = bool(np.array_equal(np.asarray(acc2).view(np.uint32), np.asarray(acc).view(np.uint32)) and
+                          n2 == spp and fp2 == fp and
+                          np.array_equal(P.read_pfm(ppath).view(np.uint32), hdr.view(np.uint32)))
+        ctx.free_scene(sid)
+        log(f"io: {spp} spp in one call {one_ms:.1f} ms, in {spp} progressive calls {prog_ms:.1f} ms, identical "
+            f"{same_img}; accumulation file save {acc_save_ms:.1f} ms, load {acc_load_ms:.1f} ms; PFM {pfm_ms:.1f} ms")
+        print(json.dumps({
+            "metric": "flat tree file and progressive path tracing", "n_gpus": 1, "data": "synthetic",
+            "flat_tree_file": {"tree": f"terrain {args.grid}x{args.grid}, binned SAH built on the CPU",
+                               "build_s": round(build_s, 3), "save_s": round(save_s, 3), "bytes": size,
+                               "load_s": round(load_s, 3), "upload_s": round(upload_s, 3),
+                               "identical_traces": same_tree},
+            "progressive": {"scene": f"Cornell room, {room.num_tris} triangles, {w}x{h}, depth {depth}",
+                            "spp": spp, "one_call_ms": round(one_ms, 2), "progressive_ms": round(prog_ms, 2),
+                            "per_call_overhead_ms": round((prog_ms - one_ms) / spp, 3),
+                            "identical_image": same_img, "accum_save_ms": round(acc_save_ms, 2),
+                            "accum_load_ms": round(acc_load_ms, 2), "pfm_write_ms": round(pfm_ms, 2),
+                            "files_round_trip": same_files},
+        }), flush=True)
+    finally:
+        ctx.close()
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def build_tree(P, ctx, scene, args, which):
