@@ -478,6 +478,23 @@ def io_workload(P, args):
         ctx.free_scene(sid)
         log(f"io: {spp} spp in one call {one_ms:.1f} ms, in {spp} progressive calls {prog_ms:.1f} ms, identical "
             f"{same_img}; accumulation file save {acc_save_ms:.1f} ms, load {acc_load_ms:.1f} ms; PFM {pfm_ms:.1f} ms")
+
+        # The drop-in CLI on the example scenes (a process each: scene load,
+        # tree build, upload, render, BMP), against the golden images.
+        gold = np.load(os.path.join(ROOT, "tests", "golden", "ref_images.npz"))
+        sdir = os.path.join(ROOT, "tests", "golden", "scenes")
+        cli = []
+        for name in sorted(f for f in os.listdir(sdir) if f.endswith(".json")):
+            out = os.path.join(tmp, name + ".bmp")
+            t0 = time.perf_counter()
+            r = subprocess.run([P.CLI, os.path.join(sdir, name), out], capture_output=True, text=True, timeout=300)
+            wall = (time.perf_counter() - t0) * 1e3
+            key = f"{name}:exact"
+            same = bool(np.array_equal(bmp_rgb(out), gold[key])) if r.returncode == 0 and key in gold.files else None
+            cli.append({"scene": name, "exit": r.returncode, "wall_ms": round(wall, 1), **cli_stage_ms(r.stdout),
+                        "equals_golden": same})
+            log(f"io: Pray {name}: exit {r.returncode}, {wall:.0f} ms {cli_stage_ms(r.stdout)}, equals golden {same}")
+        cpu = None if args.no_cpu_baseline else cli_cpu_baseline(sdir, [c["scene"] for c in cli], gold, tmp)
         print(json.dumps({
             "metric": "flat tree file and progressive path tracing", "n_gpus": 1, "data": "synthetic",
             "flat_tree_file": {"tree": f"terrain {args.grid}x{args.grid}, binned SAH built on the CPU",
@@ -490,10 +507,61 @@ def io_workload(P, args):
                             "identical_image": same_img, "accum_save_ms": round(acc_save_ms, 2),
                             "accum_load_ms": round(acc_load_ms, 2), "pfm_write_ms": round(pfm_ms, 2),
                             "files_round_trip": same_files},
+            "cli": cli,
+            "cpu_baseline": cpu,
         }), flush=True)
     finally:
         ctx.close()
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+def cli_stage_ms(text):
+    """Stage times a CLI run prints (preprocessing, rendering, total), in ms."""
+    import re
+
+    out = {}
+    for key, pat in (("preprocess_ms", r"Preprocessing time: (\d+) ms"), ("render_ms", r"Finished rendering\. \((\d+) ms\)"),
+                     ("total_ms", r"Total time: (\d+) ms")):
+        m = re.search(pat, text)
+        out[key] = int(m.group(1)) if m else None
+    return out
+
+
+def cli_cpu_baseline(sdir, names, gold, tmp):
+    """The reference's own CLI, compiled from its sources into the oracle's
+    build folder, on the ray-traced example scenes (the path-traced one takes
+    minutes on the CPU): wall time per scene and whether its image equals the
+    golden one.  None when that binary was not built."""
+    import oracle as O
+
+    if not os.path.exists(O.REF_CLI):
+        return None
+    runs = []
+    for name in names:
+        if f"{name}:exact" not in gold.files:
+            continue
+        out = os.path.join(tmp, name + ".ref.bmp")
+        t0 = time.perf_counter()
+        r = subprocess.run([O.REF_CLI, os.path.join(sdir, name), out], capture_output=True, text=True, timeout=300)
+        wall = (time.perf_counter() - t0) * 1e3
+        same = bool(np.array_equal(bmp_rgb(out), gold[f"{name}:exact"])) if r.returncode == 0 else None
+        runs.append({"scene": name, "exit": r.returncode, "wall_ms": round(wall, 1), **cli_stage_ms(r.stdout),
+                     "equals_golden": same})
+        log(f"io: reference Pray (CPU) {name}: exit {r.returncode}, {wall:.0f} ms {cli_stage_ms(r.stdout)}")
+    return {"kind": "reference", "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "sample": "the reference CLI built here without FMA contraction, ray-traced example scenes, "
+                      f"OpenMP on {cpu_model()}", "runs": runs}
+
+
+def bmp_rgb(path):
+    """A 24-bit BMP as (h, w, 3) RGB, top row first."""
+    data = open(path, "rb").read()
+    off = int.from_bytes(data[10:14], "little")
+    w, h = (int.from_bytes(data[a:a + 4], "little", signed=True) for a in (18, 22))
+    stride = (3 * w + 3) // 4 * 4
+    rows = np.frombuffer(data, np.uint8, stride * abs(h), off).reshape(abs(h), stride)[:, :3 * w]
+    rgb = rows.reshape(abs(h), w, 3)[:, :, [2, 1, 0]]
+    return np.ascontiguousarray(rgb[::-1] if h > 0 else rgb)
 
 
 def build_tree(P, ctx, scene, args, which):
