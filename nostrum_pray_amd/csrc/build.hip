@@ -332,9 +332,13 @@ static __global__ void k_morton_refs(uint32_t n, const uint32_t *vals, const uin
 // Binned-SAH KD tree, built breadth first: one pass of kernels per tree level
 // over all triangle references of that level (references of one node are
 // contiguous, in increasing triangle order).
-//   k_sah_bins      per node and axis, counts of clipped reference boxes
-//                   starting and ending in each of SAH_BINS (64) bins
-//   k_sah_split     per node: the cheapest plane on a bin boundary against
+//   k_sah_small     per node of at most SAH_SMALL references: one wave bins
+//                   them and picks the plane itself, with no histogram in
+//                   global memory (the same counts, costs and choice as the
+//                   two kernels below)
+//   k_sah_bins      per larger node and axis, counts of clipped reference
+//                   boxes starting and ending in each of SAH_BINS (64) bins
+//   k_sah_split     per larger node: the cheapest plane on a bin boundary against
 //                   the leaf cost (the reference's SAH cost with its empty
 //                   -side factor 0.8, kd_tree_sah.cpp:141-166), or a leaf
 //   k_sah_classify  per reference: left if its clipped box starts below the
@@ -351,6 +355,7 @@ static __global__ void k_morton_refs(uint32_t n, const uint32_t *vals, const uin
 constexpr int SAH_BINS = 64;
 constexpr int SAH_HIST = 3 * 2 * SAH_BINS;  // per node: axis x {start, end} x bin
 constexpr int SAH_CHUNK = 2048;             // references per block of k_sah_bins
+constexpr uint32_t SAH_SMALL = 64;          // nodes up to this many references: k_sah_small
 
 struct LNode {
     float lo[3], hi[3];     // cell
@@ -378,6 +383,7 @@ static __global__ __launch_bounds__(256) void k_sah_bins(uint32_t n_refs, const 
     if (first >= n_refs) return;
     const uint32_t last = min(first + SAH_CHUNK, n_refs) - 1;
     const bool one_node = ref_node[first] == ref_node[last];  // block-uniform
+    if (one_node && nodes[ref_node[first]].count <= SAH_SMALL) return;
     if (one_node) {
         for (int j = threadIdx.x; j < SAH_HIST; j += 256) h[j] = 0u;
         __syncthreads();
@@ -385,6 +391,7 @@ static __global__ __launch_bounds__(256) void k_sah_bins(uint32_t n_refs, const 
     for (uint32_t i = first + threadIdx.x; i <= last; i += 256) {
         const uint32_t k = ref_node[i], t = refs[i];
         const LNode &N = nodes[k];
+        if (N.count <= SAH_SMALL) continue;  // binned by k_sah_small
         uint32_t *dst = one_node ? h : hist + (size_t)k * SAH_HIST;
         for (int a = 0; a < 3; a++) {
             const float ext = N.hi[a] - N.lo[a];
@@ -403,12 +410,95 @@ static __global__ __launch_bounds__(256) void k_sah_bins(uint32_t n_refs, const 
     }
 }
 
+// One wave per node of at most SAH_SMALL references (blocks of 4 waves):
+// lane i bins reference i, lane b (1..63) counts the references starting /
+// ending below bin boundary b from the wave's bins and evaluates the plane at
+// b on each axis with k_sah_split's arithmetic; the first minimum in
+// (axis, bin) order below the leaf cost wins, as in k_sah_split's scan.
+static __global__ __launch_bounds__(256) void k_sah_small(uint32_t n_nodes, const LNode *nodes, const uint32_t *refs,
+                                                          const float *tlo, const float *thi, SahParams P,
+                                                          uint32_t level, int *axis, float *pos) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (k >= n_nodes) return;  // wave-uniform
+    const LNode N = nodes[k];
+    if (N.count > SAH_SMALL) return;  // wave-uniform: k_sah_bins / k_sah_split
+    const float dx = N.hi[0] - N.lo[0], dy = N.hi[1] - N.lo[1], dz = N.hi[2] - N.lo[2];
+    const float sa = half_area(dx, dy, dz);
+    if (!(N.count > P.leaf_size && level < P.max_depth && sa > 0.f)) {
+        if (lane == 0) axis[k] = -1, pos[k] = 0.f;
+        return;
+    }
+    // bins of reference `lane`: start bins in bytes 0..2, end bins in 0..2 of e
+    uint32_t s = 0, e = 0;
+    float scale[3];
+    for (int a = 0; a < 3; a++) {
+        const float ext = N.hi[a] - N.lo[a];
+        scale[a] = ext > 0.f ? (float)SAH_BINS / ext : 0.f;
+    }
+    if (lane < N.count) {
+        const uint32_t t = refs[N.begin + lane];
+        for (int a = 0; a < 3; a++) {
+            const float clo = fmaxf(tlo[3 * (size_t)t + a], N.lo[a]);
+            const float chi = fminf(thi[3 * (size_t)t + a], N.hi[a]);
+            s |= (uint32_t)sah_bin(clo, N.lo[a], scale[a]) << (8 * a);
+            e |= (uint32_t)sah_bin(chi, N.lo[a], scale[a]) << (8 * a);
+        }
+    }
+    uint32_t nl[3] = {0, 0, 0}, ne[3] = {0, 0, 0};  // references starting / ending below boundary `lane`
+    for (uint32_t i = 0; i < N.count; i++) {
+        const uint32_t si = (uint32_t)__shfl((int)s, (int)i), ei = (uint32_t)__shfl((int)e, (int)i);
+        for (int a = 0; a < 3; a++) {
+            nl[a] += ((si >> (8 * a)) & 0xFFu) < lane ? 1u : 0u;
+            ne[a] += ((ei >> (8 * a)) & 0xFFu) < lane ? 1u : 0u;
+        }
+    }
+    // this lane's best plane: (cost, index a * 64 + b); a later axis only
+    // replaces an earlier one when strictly cheaper
+    float best = INFINITY;
+    uint32_t best_i = 0xFFFFFFFFu;
+    if (lane >= 1) {
+        for (int a = 0; a < 3; a++) {
+            const float ext = N.hi[a] - N.lo[a];
+            if (!(ext > 0.f)) continue;
+            const uint32_t n_left = nl[a], n_right = N.count - ne[a];
+            const float p = N.lo[a] + ext * ((float)lane * (1.0f / (float)SAH_BINS));
+            float l[3] = {dx, dy, dz}, r[3] = {dx, dy, dz};
+            l[a] = p - N.lo[a];
+            r[a] = N.hi[a] - p;
+            const float pl = half_area(l[0], l[1], l[2]) / sa, pr = half_area(r[0], r[1], r[2]) / sa;
+            float cost = P.ct + P.ci * (pl * (float)n_left + pr * (float)n_right);
+            if ((n_left == 0 || n_right == 0) && !(pl == 1.f || pr == 1.f)) cost *= 0.8f;
+            if (cost < best) best = cost, best_i = (uint32_t)a * SAH_BINS + lane;
+        }
+    }
+    // wave minimum of (cost, index): the first minimum of the sequential scan
+    for (int off = 32; off > 0; off >>= 1) {
+        const float oc = __shfl_xor(best, off);
+        const uint32_t oi = (uint32_t)__shfl_xor((int)best_i, off);
+        if (oc < best || (oc == best && oi < best_i)) best = oc, best_i = oi;
+    }
+    if (lane == 0) {
+        const float leaf_cost = P.ci * (float)N.count;
+        if (best < leaf_cost) {
+            const int a = (int)(best_i / SAH_BINS);
+            const uint32_t b = best_i % SAH_BINS;
+            axis[k] = a;
+            pos[k] = N.lo[a] + (N.hi[a] - N.lo[a]) * ((float)b * (1.0f / (float)SAH_BINS));
+        } else {
+            axis[k] = -1;
+            pos[k] = 0.f;
+        }
+    }
+}
+
 // axis[k] = split axis or -1 for a leaf; pos[k] = split plane.
 static __global__ void k_sah_split(uint32_t n_nodes, const LNode *nodes, const uint32_t *hist, SahParams P,
                                    uint32_t level, int *axis, float *pos) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_nodes) return;
     const LNode N = nodes[k];
+    if (N.count <= SAH_SMALL) return;  // decided by k_sah_small
     int best_a = -1;
     float best_p = 0.f;
     const float dx = N.hi[0] - N.lo[0], dy = N.hi[1] - N.lo[1], dz = N.hi[2] - N.lo[2];
@@ -716,6 +806,8 @@ int build_sah(pray_ctx *c, hipStream_t st, uint32_t n, const prb::SahParams &P, 
         if (n_refs)
             hipLaunchKernelGGL(prb::k_sah_bins, dim3((n_refs + prb::SAH_CHUNK - 1) / prb::SAH_CHUNK), dim3(256), 0,
                                st, n_refs, R, RN, tlo, thi, N, hist.as<uint32_t>());
+        hipLaunchKernelGGL(prb::k_sah_small, dim3(nblocks(n_nodes, 4)), dim3(256), 0, st, n_nodes, N, R, tlo, thi, P,
+                           level, axis.as<int>(), pos.as<float>());
         hipLaunchKernelGGL(prb::k_sah_split, dim3(nblocks(n_nodes, 256)), dim3(256), 0, st, n_nodes, N,
                            (const uint32_t *)hist.as<uint32_t>(), P, level, axis.as<int>(), pos.as<float>());
         hipLaunchKernelGGL(prb::k_sah_classify, dim3(nblocks((uint64_t)n_refs + 1, 256)), dim3(256), 0, st, n_refs, R,
