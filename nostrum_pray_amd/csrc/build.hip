@@ -354,7 +354,8 @@ static __global__ void k_morton_refs(uint32_t n, const uint32_t *vals, const uin
 // word for word.
 constexpr int SAH_BINS = 64;
 constexpr int SAH_HIST = 3 * 2 * SAH_BINS;  // per node: axis x {start, end} x bin
-constexpr int SAH_CHUNK = 2048;             // references per block of k_sah_bins
+constexpr int SAH_CHUNK = 1024;             // references per block of k_sah_bins
+constexpr int SAH_SLOTS = 16;               // LDS histograms per block of k_sah_bins
 constexpr uint32_t SAH_SMALL = 64;          // nodes up to this many references: k_sah_small
 
 struct LNode {
@@ -375,38 +376,46 @@ __device__ __forceinline__ int sah_bin(float x, float lo, float scale) {
 
 __device__ __forceinline__ float half_area(float dx, float dy, float dz) { return dx * dy + dy * dz + dz * dx; }
 
+// A block bins SAH_CHUNK consecutive references (sorted by node) into LDS
+// histograms for the first SAH_SLOTS nodes of its range, and the rest
+// straight into global memory; then it adds the nonzero LDS counts to the
+// global histograms (a node may span several blocks).
 static __global__ __launch_bounds__(256) void k_sah_bins(uint32_t n_refs, const uint32_t *refs,
                                                          const uint32_t *ref_node, const float *tlo,
                                                          const float *thi, const LNode *nodes, uint32_t *hist) {
-    __shared__ uint32_t h[SAH_HIST];
+    __shared__ uint32_t h[SAH_SLOTS * SAH_HIST];
     const uint32_t first = blockIdx.x * SAH_CHUNK;
     if (first >= n_refs) return;
-    const uint32_t last = min(first + SAH_CHUNK, n_refs) - 1;
-    const bool one_node = ref_node[first] == ref_node[last];  // block-uniform
-    if (one_node && nodes[ref_node[first]].count <= SAH_SMALL) return;
-    if (one_node) {
-        for (int j = threadIdx.x; j < SAH_HIST; j += 256) h[j] = 0u;
-        __syncthreads();
-    }
+    const uint32_t last = min(first + (uint32_t)SAH_CHUNK, n_refs) - 1;
+    const uint32_t k0 = ref_node[first];
+    const uint32_t nslots = min(ref_node[last] - k0 + 1, (uint32_t)SAH_SLOTS);
+    for (uint32_t j = threadIdx.x; j < nslots * SAH_HIST; j += 256) h[j] = 0u;
+    __syncthreads();
     for (uint32_t i = first + threadIdx.x; i <= last; i += 256) {
         const uint32_t k = ref_node[i], t = refs[i];
         const LNode &N = nodes[k];
         if (N.count <= SAH_SMALL) continue;  // binned by k_sah_small
-        uint32_t *dst = one_node ? h : hist + (size_t)k * SAH_HIST;
+        const uint32_t slot = k - k0;
         for (int a = 0; a < 3; a++) {
             const float ext = N.hi[a] - N.lo[a];
             const float scale = ext > 0.f ? (float)SAH_BINS / ext : 0.f;
             const float clo = fmaxf(tlo[3 * (size_t)t + a], N.lo[a]);
             const float chi = fminf(thi[3 * (size_t)t + a], N.hi[a]);
-            atomicAdd(&dst[a * 2 * SAH_BINS + sah_bin(clo, N.lo[a], scale)], 1u);
-            atomicAdd(&dst[a * 2 * SAH_BINS + SAH_BINS + sah_bin(chi, N.lo[a], scale)], 1u);
+            const int bs = a * 2 * SAH_BINS + sah_bin(clo, N.lo[a], scale);
+            const int be = a * 2 * SAH_BINS + SAH_BINS + sah_bin(chi, N.lo[a], scale);
+            if (slot < (uint32_t)SAH_SLOTS) {
+                atomicAdd(&h[slot * SAH_HIST + bs], 1u);
+                atomicAdd(&h[slot * SAH_HIST + be], 1u);
+            } else {
+                atomicAdd(&hist[(size_t)k * SAH_HIST + bs], 1u);
+                atomicAdd(&hist[(size_t)k * SAH_HIST + be], 1u);
+            }
         }
     }
-    if (one_node) {
-        __syncthreads();
-        uint32_t *g = hist + (size_t)ref_node[first] * SAH_HIST;
-        for (int j = threadIdx.x; j < SAH_HIST; j += 256)
-            if (h[j]) atomicAdd(&g[j], h[j]);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < nslots * SAH_HIST; j += 256) {
+        const uint32_t v = h[j];
+        if (v) atomicAdd(&hist[(size_t)(k0 + j / SAH_HIST) * SAH_HIST + j % SAH_HIST], v);
     }
 }
 
@@ -691,7 +700,7 @@ struct Grow {
     }
     int reserve(pray_ctx *c, size_t bytes, bool keep) {
         if (bytes <= cap) return PRAY_OK;
-        const size_t nc = std::max(bytes, cap + cap / 2);
+        const size_t nc = std::max(bytes, 2 * cap);
         void *q = nullptr;
         HIPC(c, hipMalloc(&q, nc));
         if (keep && p && cap) HIPC(c, hipMemcpyAsync(q, p, cap, hipMemcpyDeviceToDevice, c->stream));
@@ -777,8 +786,19 @@ int build_sah(pray_ctx *c, hipStream_t st, uint32_t n, const prb::SahParams &P, 
     Grow nodes[2], refs[2], rnode[2], fl, fr, sl, sr, axis, pos, hist, vs, vr, vl, ss, sr2, sleaf, tmp;
     int rc = PRAY_OK;
     int cur = 0;
-    if ((rc = nodes[0].reserve(c, sizeof(LNode), false)) || (rc = refs[0].reserve(c, (size_t)n * 4, false)) ||
-        (rc = rnode[0].reserve(c, (size_t)n * 4, false)) || (rc = entries.reserve(c, 2 * 8, false)))
+    // Up-front sizes from the triangle count (grown by doubling when a level
+    // needs more): a level's nodes, its references with some duplication,
+    // and the outputs.  Growing from small sizes reallocates (with a stream
+    // synchronisation) at almost every level.
+    const size_t est_nodes = std::max<size_t>(1024, (size_t)n / 2 + 1);
+    const size_t est_refs = (size_t)n + (size_t)n / 4 + 1;
+    for (Grow *g : {&nodes[0], &nodes[1]})
+        if ((rc = g->reserve(c, est_nodes * sizeof(LNode), false))) return rc;
+    for (Grow *g : {&refs[0], &refs[1], &rnode[0], &rnode[1], &fl, &fr, &sl, &sr})
+        if ((rc = g->reserve(c, est_refs * 4, false))) return rc;
+    for (Grow *g : {&axis, &pos, &vs, &vr, &vl, &ss, &sr2, &sleaf})
+        if ((rc = g->reserve(c, (est_nodes + 1) * 4, false))) return rc;
+    if ((rc = entries.reserve(c, 2 * est_nodes * 8, false)) || (rc = leaf_refs.reserve(c, est_refs * 4, false)))
         return rc;
     HIPC(c, hipMemsetAsync(entries.p, 0, 2 * 8, st));
     hipLaunchKernelGGL(prb::k_sah_root, dim3(nblocks(n, 256)), dim3(256), 0, st, box, n, nodes[0].as<LNode>(),
