@@ -81,8 +81,20 @@ def duplicates_scene(n):
     return np.tile(tri, (n, 1))
 
 
+def flat_scene(n):
+    """n random triangles in the plane y = 0.25: the scene box has no extent
+    in y, so no node has a plane to bin on that axis."""
+    rng = np.random.default_rng(7)
+    centre = rng.uniform(0.0, 1.0, (n, 1, 2))
+    xz = (centre + rng.uniform(-0.01, 0.01, (n, 3, 2))).astype(np.float32)
+    out = np.empty((n, 3, 3), np.float32)
+    out[:, :, 0], out[:, :, 1], out[:, :, 2] = xz[:, :, 0], np.float32(0.25), xz[:, :, 1]
+    return out.reshape(n, 9)
+
+
 SCENES = {
     "spheres": lambda: P.Scene.spheres(1).triangles()[0],
+    "flat": lambda: flat_scene(3000),
     "terrain96": lambda: P.Scene.terrain(96, 3).triangles()[0],
     "cornell": lambda: P.Scene.cornell(3, 12000).triangles()[0],
     "random": lambda: P.Scene.random(4, 6000).triangles()[0],
