@@ -75,9 +75,10 @@ def main():
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-RMSE measurement")
     ap.add_argument("--cache-size", type=int, default=1024, help="image side of the radiance-cache measurement")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples per pixel of the plain reference image")
-    ap.add_argument("--workload", choices=("terrain", "batch", "io"), default="terrain",
+    ap.add_argument("--workload", choices=("terrain", "batch", "io", "cache"), default="terrain",
                     help="terrain: the closest-hit stream (default); batch: 64 scenes path traced in one wavefront; "
-                         "io: flat tree files and progressive path tracing")
+                         "io: flat tree files, progressive path tracing and the CLI; cache: the radiance-cache "
+                         "equal-RMSE measurement alone")
     ap.add_argument("--batch-scenes", type=int, default=64)
     args = ap.parse_args()
 
@@ -93,6 +94,15 @@ def main():
         return
     if args.workload == "io":
         io_workload(P, args)
+        return
+    if args.workload == "cache":
+        ctx = P.Context(0)
+        try:
+            out = radiance_cache_leg(P, ctx, args)
+        finally:
+            ctx.close()
+        print(json.dumps({"metric": "equal-RMSE time, radiance cache vs plain path tracing", "n_gpus": 1,
+                          "data": "synthetic", "radiance_cache": out}), flush=True)
         return
 
     t0 = time.time()

@@ -154,7 +154,8 @@ unsigned grid_size(pray_ctx *c, TraceKernel k, const Tuning &tu, uint64_t n) {
 int ensure_spill(pray_ctx *c, size_t bytes) {
     if (bytes <= c->spill_bytes) return PRAY_OK;
     if (c->spill) {
-        HIPC(c, hipStreamSynchronize(c->stream));
+        // traces may run on callers' streams: wait for all of them
+        HIPC(c, hipDeviceSynchronize());
         HIPC(c, hipFree(c->spill));
         c->spill = nullptr;
         c->spill_bytes = 0;
