@@ -665,18 +665,18 @@ def radiance_cache_leg(P, ctx, args):
         return float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
 
     plain = []
-    for spp in (16, 32, 64, 128):
+    for spp in (16, 32, 64, 128, 256, 512):
         t0 = time.perf_counter()
         img, _ = ctx.render_pt(sid, cam, w, h, spp, depth, rr, 7, bg, 1e-9, want_rgb=False)
         ms = (time.perf_counter() - t0) * 1e3
         plain.append({"spp": spp, "ms": round(ms, 2), "rmse": round(rmse(img), 6)})
         log(f"cache leg: plain {spp} spp {ms:.1f} ms rmse {plain[-1]['rmse']:.5f}")
     cache = []
-    warm = 8
-    for frames in (16, 32, 64, 128):
+    warm, train = 4, 1 << 14  # plain frames before the cache takes over; training paths per frame
+    for frames in (16, 32, 64, 128, 256):
         c = P.Cache(ctx, seed=3, learning_rate=1e-2, max_batch=1 << 16)
         t0 = time.perf_counter()
-        img, _, st = c.render_pt(sid, cam, w, h, frames, depth, rr, 7, bg, 1e-9, train_paths=1 << 16,
+        img, _, st = c.render_pt(sid, cam, w, h, frames, depth, rr, 7, bg, 1e-9, train_paths=train,
                                  cache_bounce=1, min_steps=warm)
         ms = (time.perf_counter() - t0) * 1e3
         c.close()
@@ -694,7 +694,7 @@ def radiance_cache_leg(P, ctx, args):
                  f"SAH KD tree built on the CPU (64 bins per axis, leaf size {args.leaf_size})",
         "reference_spp": args.ref_spp,
         "cache": {"network": "192 (172 used) -> 5 x 64 ReLU -> 3, BF16 matrix cores, FP32 Adam lr 1e-2",
-                  "training_samples_per_frame": 1 << 16, "terminates_at_hit": 2, "warmup_frames": warm},
+                  "training_samples_per_frame": train, "terminates_at_hit": 2, "warmup_frames": warm},
         "plain_runs": plain,
         "cache_runs": cache,
         "equal_rmse_speedup": best["speedup"],
