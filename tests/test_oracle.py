@@ -203,7 +203,9 @@ def test_flat_twin_any_hit_is_threshold_of_closest():
     f = t.flat()
     rays = O.gen_rays(5, P.scene_box_expanded(f["box"]), 20000)
     ci, ct = O.flat_trace(f, rays, 0, 1e-4)
-    lim = np.where(ci >= 0, ct * ct, 1.0).astype(np.float32)
+    lim = np.ones_like(ct)
+    hit = ci >= 0
+    lim[hit] = ct[hit] * ct[hit]  # misses keep 1 (their t is FLT_MAX)
     ai, _ = O.flat_trace(f, rays, 1, 1e-4, limit2=lim)
     ii, _ = O.flat_trace(f, rays, 2, 1e-4, limit2=lim)
     assert not (ai >= 0).any()  # strict: t*t < t*t never holds
