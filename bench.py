@@ -652,11 +652,13 @@ def radiance_cache_leg(P, ctx, args):
     scene = P.Scene.cornell(1, 100_000)
     tree = scene.build(2, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
     sid = ctx.upload(tree)
-    if not args.no_autotune:
-        ctx.autotune(sid, n_rays=1 << 22, force=True)
     inf = scene.info()
     cam, bg = scene.camera(), inf["background"]
     depth, rr = 8, 3
+    tun = None
+    if not args.no_autotune:  # tuned on this path-tracing workload, for both tracers
+        tun = ctx.autotune_pt(sid, cam, w, h, depth, rr, 7, bg, 1e-9)
+        log(f"cache leg: tuning {tun}")
     t0 = time.perf_counter()
     ref, _ = ctx.render_pt(sid, cam, w, h, args.ref_spp, depth, rr, 1000003, bg, 1e-9, want_rgb=False)
     log(f"cache leg: reference {args.ref_spp} spp in {time.perf_counter() - t0:.1f}s")
@@ -692,7 +694,7 @@ def radiance_cache_leg(P, ctx, args):
     return {
         "scene": f"Cornell room, {scene.num_tris} triangles, {w}x{h}, depth {depth}, roulette from bounce {rr}, "
                  f"SAH KD tree built on the CPU (64 bins per axis, leaf size {args.leaf_size})",
-        "reference_spp": args.ref_spp,
+        "reference_spp": args.ref_spp, "tuning": tun,
         "cache": {"network": "192 (172 used) -> 5 x 64 ReLU -> 3, BF16 matrix cores, FP32 Adam lr 1e-2",
                   "training_samples_per_frame": train, "terminates_at_hit": 2, "warmup_frames": warm},
         "plain_runs": plain,

@@ -145,6 +145,15 @@ PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *tuning);
  * set.  tuning (optional) receives the setting, PRAY_TUNING_FIELDS ints. */
 PRAY_API int pray_autotune(pray_ctx *ctx, int scene_id, uint32_t n_rays, int force, int *tuning);
 
+/* Autotuner for path tracing (no reference counterpart): the same search as
+ * pray_autotune, timed on two-sample pray_render_pt renders of the caller's
+ * image and settings, whose bounce queues trace more coherent rays than the
+ * incoherent stream pray_autotune times.  Keeps the fastest setting for this
+ * scene only (not for its size bucket).  tuning (optional) receives it. */
+PRAY_API int pray_autotune_pt(pray_ctx *ctx, int scene_id, const float *camera16, uint32_t width, uint32_t height,
+                              uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *background,
+                              float det_eps, int *tuning);
+
 /* Closest-hit or any-hit query for n rays (replaces SpatialDatabase::intersect,
  * spatial.h:15, and the shadow tests of pray.cpp:39-44 / :123-133).
  *   rays     n x 6 floats: origin xyz, direction xyz (normalised)

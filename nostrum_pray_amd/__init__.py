@@ -75,6 +75,7 @@ def libraries():
         "pray_set_tuning": (C.c_int, [_vp, C.c_int, _vp]),
         "pray_get_tuning": (C.c_int, [_vp, C.c_int, _vp]),
         "pray_autotune": (C.c_int, [_vp, C.c_int, _u32, C.c_int, _vp]),
+        "pray_autotune_pt": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _u32, _u32, _u64, _vp, _f32, _vp]),
         "pray_render_pt_batch": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp, _f32, _vp, _vp,
                                            C.POINTER(_f32)]),
         "pray_last_render_rays": (C.c_int, [_vp, C.POINTER(_u64)]),
@@ -459,6 +460,16 @@ class Context:
         """Autotune the scene's traversal (or take its size bucket's setting)."""
         out = np.zeros(len(TUNING_FIELDS), np.int32)
         self._check(self._lib.pray_autotune(self._h, sid, n_rays, 1 if force else 0, _p(out)))
+        return dict(zip(TUNING_FIELDS, (int(x) for x in out)))
+
+    def autotune_pt(self, sid, cam16, width, height, max_depth, rr_depth, seed, background, det_eps):
+        """Autotune the scene's traversal on path-tracing renders of this
+        image and setting (pray_autotune_pt); returns the setting."""
+        cam16 = np.ascontiguousarray(cam16, np.float32)
+        bg = np.ascontiguousarray(background, np.float32).reshape(3)
+        out = np.zeros(len(TUNING_FIELDS), np.int32)
+        self._check(self._lib.pray_autotune_pt(self._h, sid, _p(cam16), width, height, max_depth, rr_depth, seed,
+                                               _p(bg), det_eps, _p(out)))
         return dict(zip(TUNING_FIELDS, (int(x) for x in out)))
 
     def last_render_queues(self):

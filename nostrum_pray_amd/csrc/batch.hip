@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -102,6 +103,58 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
     if (tuning) {
         tuning_values(sc->tune, tuning);
     }
+    return PRAY_OK;
+}
+
+int pray_autotune_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_t h, uint32_t max_depth,
+                     uint32_t rr_depth, uint64_t seed, const float *bg, float det_eps, int *tuning) {
+    if (!c) return PRAY_ERR_INVALID;
+    SceneRec *sc = find_scene(c, scene_id);
+    if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    if (!cam16 || !bg) return fail(c, PRAY_ERR_INVALID, "null argument");
+    if ((uint64_t)w * h == 0) return fail(c, PRAY_ERR_INVALID, "empty image");
+    if (sc->d.n_tris) {
+        // Same grid as pray_autotune, timed on two-sample renders of the
+        // caller's image (host clock around the whole call, which allocates
+        // its buffers; the fastest of two runs after one warm-up).
+        const Tuning saved = sc->tune;
+        Tuning best = saved;
+        double best_ms = 1e300;
+        auto consider = [&](Tuning cand) -> int {
+            sc->tune = cand;
+            double fastest = 1e300;
+            for (int r = 0; r < 3; r++) {
+                const auto t0 = std::chrono::steady_clock::now();
+                const int rc = pray_render_pt_ex(c, scene_id, cam16, w, h, 2, max_depth, rr_depth, seed, bg, det_eps,
+                                                 nullptr, nullptr, 1);
+                if (rc) return rc;
+                const double ms =
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (r > 0) fastest = std::min(fastest, ms);
+            }
+            if (fastest < best_ms) best_ms = fastest, best = cand;
+            return PRAY_OK;
+        };
+        int rc = PRAY_OK;
+        for (int K : {4, 6, 8, 16})
+            for (int ilp : {1, 2})
+                for (int fm : {16, 24, 32, 48}) {
+                    Tuning t;
+                    t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm;
+                    if (!rc) rc = consider(t);
+                }
+        for (int occ : {75, 50}) {
+            Tuning t = best;
+            t.occupancy = occ;
+            if (!rc) rc = consider(t);
+        }
+        if (rc) {
+            sc->tune = saved;
+            return rc;
+        }
+        sc->tune = best;
+    }
+    if (tuning) tuning_values(sc->tune, tuning);
     return PRAY_OK;
 }
 

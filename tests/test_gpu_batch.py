@@ -79,3 +79,35 @@ def test_autotune_never_changes_hits(ctx):
             ctx.free_scene(sid2)
     finally:
         ctx.free_scene(sid)
+
+
+def test_autotune_pt_keeps_images(ctx):
+    """The path-tracing autotuner picks a valid setting for the scene alone
+    and no image changes by a bit."""
+    s = P.Scene.cornell(3, 20000)
+    t = s.build(1)
+    w, h, depth, rr = 64, 48, 6, 2
+    cam = O.camera([0, 1, 3.8], [0, 0, -1], 40, w, h)
+    bg = np.asarray(s.info()["background"], np.float32)
+    sid = ctx.upload(t)
+    sid2 = ctx.upload(t)
+    try:
+        ctx.set_tuning(sid, stack_lds=16, leaf_ilp=1, fetch_min=64, occupancy=50)
+        say("autotune_pt: reference render")
+        ref, _ = ctx.render_pt(sid, cam, w, h, 4, depth, rr, 5, bg, 1e-9, want_rgb=False)
+        before2 = ctx.get_tuning(sid2)
+        say("autotune_pt: tuning")
+        tun = ctx.autotune_pt(sid, cam, w, h, depth, rr, 5, bg, 1e-9)
+        assert tun["stack_lds"] in (4, 6, 8, 16) and tun["leaf_ilp"] in (1, 2)
+        assert tun["fetch_min"] in (16, 24, 32, 48) and tun["occupancy"] in (50, 75, 100)
+        assert ctx.get_tuning(sid) == tun
+        assert ctx.get_tuning(sid2) == before2  # other scenes keep theirs
+        hdr, _ = ctx.render_pt(sid, cam, w, h, 4, depth, rr, 5, bg, 1e-9, want_rgb=False)
+        assert np.array_equal(hdr.view(np.uint32), ref.view(np.uint32))
+        with pytest.raises(P.PrayError):
+            ctx.autotune_pt(sid, cam, 0, h, depth, rr, 5, bg, 1e-9)
+        with pytest.raises(P.PrayError):
+            ctx.autotune_pt(12345, cam, w, h, depth, rr, 5, bg, 1e-9)
+    finally:
+        ctx.free_scene(sid)
+        ctx.free_scene(sid2)
