@@ -238,8 +238,9 @@ PRAY_API int pray_render_pt_progressive(pray_ctx *ctx, int scene_id, const float
  * replayed spp times.  Scene k renders widths[k] x heights[k] pixels with
  * camera cameras16[16k..], seed seeds[k] and background backgrounds[3k..];
  * its image equals pray_render_pt's for the same arguments bit for bit.  The
- * launches use the autotuned setting of the size bucket holding most pixels
- * (defaults if none is tuned).  hdr[k] / rgb[k] (arrays of per-scene output
+ * launches use the setting of the scene with the largest image in the size
+ * bucket holding most pixels: its bucket's after pray_autotune, its own after
+ * pray_autotune_pt or pray_set_tuning, the defaults otherwise.  hdr[k] / rgb[k] (arrays of per-scene output
  * pointers, each may be NULL); gpu_ms (optional) receives the device time of
  * the spp passes. */
 PRAY_API int pray_render_pt_batch(pray_ctx *ctx, uint32_t n_scenes, const int *scene_ids, const float *cameras16,

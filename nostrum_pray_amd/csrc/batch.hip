@@ -172,6 +172,8 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
     uint64_t total = 0;
     uint32_t tree_depth = 1;
     std::vector<uint64_t> bucket_pixels(16, 0);
+    std::vector<uint64_t> bucket_best(16, 0);          // pixels of each bucket's largest image
+    std::vector<const SceneRec *> bucket_scene(16, nullptr);  // and its scene
     for (uint32_t k = 0; k < n_scenes; k++) {
         SceneRec *sc = find_scene(c, scene_ids[k]);
         if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_ids[k]);
@@ -190,13 +192,16 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
         total += np;
         if (total > (1u << 28)) return fail(c, PRAY_ERR_INVALID, "batch too large (more than 2^28 pixels)");
         tree_depth = std::max(tree_depth, sc->d.max_depth);
-        bucket_pixels[size_bucket(sc->d.n_tris)] += np;
+        const int sb = size_bucket(sc->d.n_tris);
+        bucket_pixels[sb] += np;
+        if (np > bucket_best[sb]) bucket_best[sb] = np, bucket_scene[sb] = sc;
     }
-    // One launch per bounce serves every scene: it runs with the autotuned
-    // setting of the size bucket that holds most of the pixels.
+    // One launch per bounce serves every scene: it runs with the setting of
+    // the scene with the largest image in the size bucket that holds most of
+    // the pixels (its autotuned bucket setting after pray_autotune, or its
+    // own after pray_autotune_pt / pray_set_tuning).
     const int main_bucket = (int)(std::max_element(bucket_pixels.begin(), bucket_pixels.end()) - bucket_pixels.begin());
-    Tuning tu;
-    if (c->tuned.count(main_bucket)) tu = c->tuned[main_bucket];
+    Tuning tu = bucket_scene[main_bucket]->tune;
     tu.layout = PRAY_LAYOUT_PACKED;
 
     DBuf dtable, dbs, prim, rq[2], sq[2], qs[2], cnt, hit, tt, contrib, sum, outh, outb, rays_acc;
