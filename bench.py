@@ -686,7 +686,8 @@ def radiance_cache_leg(P, ctx, args):
         plain.append({"spp": spp, "ms": round(ms, 2), "rmse": round(rmse(img), 6)})
         log(f"cache leg: plain {spp} spp {ms:.1f} ms rmse {plain[-1]['rmse']:.5f}")
     cache = []
-    warm, train = 4, 1 << 14  # plain frames before the cache takes over; training paths per frame
+    # plain frames before the cache takes over; training samples per frame (2^16, the workload's)
+    warm, train = 4, 1 << 16
     for frames in (16, 32, 64, 128, 256):
         c = P.Cache(ctx, seed=3, learning_rate=1e-2, max_batch=1 << 16)
         t0 = time.perf_counter()
