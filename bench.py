@@ -8,6 +8,11 @@ HBM).  Prints ONE JSON line with the metric, a roofline object for the
 traversal kernel, the radiance-cache comparison and a CPU baseline (the
 restated reference CPU path timed on this host).
 Run:  python bench.py [--steps K] [--warmup W]
+
+Other workloads, one JSON line each: --workload batch (64 scenes path traced
+in one wavefront), --workload io (flat tree files, progressive path tracing,
+the drop-in CLI on the example scenes), --workload cache (the radiance-cache
+comparison alone).
 """
 import argparse
 import json
