@@ -130,15 +130,18 @@ PRAY_API int pray_scene_layout(pray_ctx *ctx, int scene_id, uint64_t *sizes, uin
  *                  (1..64)
  *   [4] occupancy  percent (1..100) of the waves that fit on the device that
  *                  the persistent grid launches: the workgroup count
+ *   [5] wg_waves   waves per workgroup (1, 2 or 4): the workgroup size; the
+ *                  waves of a workgroup trace independently
  * pray_set_tuning: -1 keeps a value.  Calls of one context must not overlap
  * (traces share its ray counter and stack spill area). */
-#define PRAY_TUNING_FIELDS 5
+#define PRAY_TUNING_FIELDS 6
 PRAY_API int pray_set_tuning(pray_ctx *ctx, int scene_id, const int *tuning);
 PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *tuning);
 
 /* Autotuner (no reference counterpart): times closest-hit traversal of
  * n_rays incoherent rays through the scene (0 = 2^20) over stack depth x
- * leaf ILP x refill threshold, then occupancy for the fastest of those, and
+ * leaf ILP x refill threshold, then workgroup size and occupancy for the
+ * fastest of those, and
  * keeps the fastest setting for the scene and for its size bucket (triangle
  * counts within a factor of 4).  Later scenes of a
  * tuned bucket take the bucket's setting without timing unless `force` is

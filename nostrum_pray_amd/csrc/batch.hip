@@ -75,7 +75,8 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
         int rc = sample_rays(c, *sc, n_rays, 0x5EEDull + (uint64_t)scene_id, (float *)rays.p);
         if (rc) return rc;
         // Grid over stack depth x leaf ILP x queue chunk at full occupancy,
-        // then the occupancy of the winner (packed layout).
+        // then the workgroup size and the occupancy of the winner (packed
+        // layout).
         Tuning best;
         float best_ms = 1e30f;
         auto consider = [&](Tuning cand) {
@@ -90,6 +91,12 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
                     t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm;
                     consider(t);
                 }
+        const Tuning grid_best = best;
+        for (int wg : {2, 4}) {
+            Tuning t = grid_best;
+            t.wg_waves = wg;
+            consider(t);
+        }
         for (int occ : {75, 50}) {
             Tuning t = best;
             t.occupancy = occ;
@@ -143,6 +150,12 @@ int pray_autotune_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, 
                     t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm;
                     if (!rc) rc = consider(t);
                 }
+        const Tuning grid_best = best;
+        for (int wg : {2, 4}) {
+            Tuning t = grid_best;
+            t.wg_waves = wg;
+            if (!rc) rc = consider(t);
+        }
         for (int occ : {75, 50}) {
             Tuning t = best;
             t.occupancy = occ;

@@ -20,6 +20,7 @@ struct Tuning {
     int leaf_ilp = 2;
     int fetch_min = 16;
     int occupancy = 100;  // percent of the resident persistent waves launched
+    int wg_waves = 1;     // waves per workgroup (1, 2 or 4)
 };
 
 struct SceneRec {
@@ -53,7 +54,7 @@ struct pray_ctx {
     uint32_t *work = nullptr;   // ray counters of the persistent kernels
     uint4 *spill = nullptr;     // per-lane stack spill area
     size_t spill_bytes = 0;
-    std::map<const void *, int> occupancy;  // kernel -> resident blocks per CU
+    std::map<std::pair<const void *, int>, int> occupancy;  // (kernel, waves per block) -> resident blocks per CU
     std::map<int, prc::Tuning> tuned;        // scene-size bucket -> autotuned tuning
 };
 
@@ -88,7 +89,7 @@ SceneRec *find_scene(pray_ctx *c, int id);
 int device_alloc(pray_ctx *c, SceneRec &sc, void **p, size_t bytes);
 void free_scene_rec(SceneRec &sc);
 int ensure_spill(pray_ctx *c, size_t bytes);
-// {layout, stack_lds, leaf_ilp, fetch_min, occupancy}
+// {layout, stack_lds, leaf_ilp, fetch_min, occupancy, wg_waves}
 void tuning_values(const Tuning &t, int *v);
 // Enqueues one trace over n rays (or *n_dev rays when n_dev is set, with n as
 // the launch bound).

@@ -132,23 +132,39 @@ TraceKernel pick_kernel(bool any, int layout, int K, int ilp) {
                                         : pick_kernel_kl<false, PRAY_LAYOUT_INDEXED>(K, ilp);
 }
 
-// Resident 64-lane blocks per CU for a kernel (cached).
-int resident_blocks(pray_ctx *c, TraceKernel k) {
-    auto it = c->occupancy.find((const void *)k);
+// Resident blocks per CU of a trace kernel with `wpw` waves per block and
+// `lds` bytes of dynamic LDS per block (cached).
+int resident_blocks(pray_ctx *c, TraceKernel k, int wpw, size_t lds) {
+    const auto key = std::make_pair((const void *)k, wpw);
+    auto it = c->occupancy.find(key);
     if (it != c->occupancy.end()) return it->second;
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, WAVE, 0) != hipSuccess || nb <= 0) nb = 8;
-    c->occupancy[(const void *)k] = nb;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, wpw * WAVE, lds) != hipSuccess || nb <= 0)
+        nb = std::max(1, 8 / wpw);
+    c->occupancy[key] = nb;
     return nb;
 }
 
 // Persistent grid: one wave per 64 rays, at most the tuned share of the
-// waves that fit on the device at once.
-unsigned grid_size(pray_ctx *c, TraceKernel k, const Tuning &tu, uint64_t n) {
+// waves that fit on the device at once, in blocks of tu.wg_waves waves.
+struct Launch {
+    unsigned blocks = 1, threads = WAVE;
+    size_t lds = 0;         // dynamic LDS per block: the waves' stack rings
+    uint64_t lanes = WAVE;  // lanes of the grid (stack spill slots)
+};
+
+Launch launch_shape(pray_ctx *c, TraceKernel k, const Tuning &tu, uint64_t n) {
+    const int wpw = tu.wg_waves;
+    Launch L;
+    L.threads = (unsigned)(wpw * WAVE);
+    L.lds = (size_t)wpw * prt::ring_lds_bytes(tu.stack_lds);
     const uint64_t want = (n + WAVE - 1) / WAVE;
-    const uint64_t per_cu = std::max<uint64_t>(1, (uint64_t)resident_blocks(c, k) * (uint64_t)tu.occupancy / 100);
-    const uint64_t fit = (uint64_t)c->num_cus * per_cu;
-    return (unsigned)std::max<uint64_t>(1, std::min(want, fit));
+    const uint64_t per_cu = std::max<uint64_t>(
+        1, (uint64_t)resident_blocks(c, k, wpw, L.lds) * (uint64_t)wpw * (uint64_t)tu.occupancy / 100);
+    const uint64_t waves = std::max<uint64_t>(1, std::min(want, (uint64_t)c->num_cus * per_cu));
+    L.blocks = (unsigned)((waves + wpw - 1) / wpw);
+    L.lanes = (uint64_t)L.blocks * wpw * WAVE;
+    return L;
 }
 
 int ensure_spill(pray_ctx *c, size_t bytes) {
@@ -182,9 +198,9 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
     }
     const Tuning &tu = sc.tune;
     TraceKernel k = pick_kernel(mode != PRAY_TRACE_CLOSEST, tu.layout, tu.stack_lds, tu.leaf_ilp);
-    const unsigned grid = grid_size(c, k, tu, n);
+    const Launch L = launch_shape(c, k, tu, n);
     const uint32_t depth = std::max<uint32_t>(1, sc.d.max_depth);
-    int rc = ensure_spill(c, (size_t)grid * WAVE * depth * sizeof(uint4));
+    int rc = ensure_spill(c, (size_t)L.lanes * depth * sizeof(uint4));
     if (rc) return rc;
     TraceArgs a{};
     a.rays = rays;
@@ -208,7 +224,7 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
         timed = cs == hipStreamCaptureStatusNone;
     }
     if (timed) HIPC(c, hipEventRecord(c->kevents[0], st));
-    hipLaunchKernelGGL(k, dim3(grid), dim3(WAVE), 0, st, sc.d, a);
+    hipLaunchKernelGGL(k, dim3(L.blocks), dim3(L.threads), L.lds, st, sc.d, a);
     if (timed) {
         HIPC(c, hipEventRecord(c->kevents[1], st));
         c->kernel_timed = true;
@@ -223,9 +239,9 @@ int enqueue_trace_multi(pray_ctx *c, const prt::DevScene *table, const uint32_t 
     if (n == 0) return PRAY_OK;
     if (n > 0x7FFFFFFFull) return fail(c, PRAY_ERR_INVALID, "at most 2^31-1 rays per call");
     TraceKernel k = pick_kernel_multi(tu.stack_lds, tu.leaf_ilp);
-    const unsigned grid = grid_size(c, k, tu, n);
+    const Launch L = launch_shape(c, k, tu, n);
     const uint32_t depth = std::max<uint32_t>(1, max_depth);
-    int rc = ensure_spill(c, (size_t)grid * WAVE * depth * sizeof(uint4));
+    int rc = ensure_spill(c, (size_t)L.lanes * depth * sizeof(uint4));
     if (rc) return rc;
     TraceArgs a{};
     a.rays = rays;
@@ -241,14 +257,14 @@ int enqueue_trace_multi(pray_ctx *c, const prt::DevScene *table, const uint32_t 
     a.table = table;
     a.ray_scene = ray_scene;
     hipLaunchKernelGGL(prt::k_reset_counters, dim3(1), dim3(64), 0, st, c->work);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(WAVE), 0, st, prt::DevScene{}, a);
+    hipLaunchKernelGGL(k, dim3(L.blocks), dim3(L.threads), L.lds, st, prt::DevScene{}, a);
     HIPC(c, hipGetLastError());
     return PRAY_OK;
 }
 
 int prepare_trace_multi(pray_ctx *c, const Tuning &tu, uint32_t max_depth, uint64_t n) {
     TraceKernel k = pick_kernel_multi(tu.stack_lds, tu.leaf_ilp);
-    return ensure_spill(c, (size_t)grid_size(c, k, tu, n) * WAVE * std::max<uint32_t>(1, max_depth) * sizeof(uint4));
+    return ensure_spill(c, (size_t)launch_shape(c, k, tu, n).lanes * std::max<uint32_t>(1, max_depth) * sizeof(uint4));
 }
 
 int size_bucket(uint32_t n_tris) {
@@ -313,12 +329,13 @@ int validate_tree(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const ui
 
 int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n) {
     TraceKernel k = pick_kernel(false, sc.tune.layout, sc.tune.stack_lds, sc.tune.leaf_ilp);
-    return ensure_spill(c, (size_t)grid_size(c, k, sc.tune, n) * WAVE * std::max<uint32_t>(1, sc.d.max_depth) *
+    return ensure_spill(c, (size_t)launch_shape(c, k, sc.tune, n).lanes * std::max<uint32_t>(1, sc.d.max_depth) *
                                sizeof(uint4));
 }
 
 void tuning_values(const Tuning &t, int *v) {
     v[0] = t.layout, v[1] = t.stack_lds, v[2] = t.leaf_ilp, v[3] = t.fetch_min, v[4] = t.occupancy;
+    v[5] = t.wg_waves;
 }
 
 }  // namespace prc
@@ -479,7 +496,7 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
     SceneRec *sc = find_scene(c, scene_id);
     if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
     if (!v) return fail(c, PRAY_ERR_INVALID, "null tuning");
-    const int layout = v[0], stack_lds = v[1], leaf_ilp = v[2], fetch_min = v[3], occupancy = v[4];
+    const int layout = v[0], stack_lds = v[1], leaf_ilp = v[2], fetch_min = v[3], occupancy = v[4], wg_waves = v[5];
     if (layout >= 0 && layout != PRAY_LAYOUT_INDEXED && layout != PRAY_LAYOUT_PACKED)
         return fail(c, PRAY_ERR_INVALID, "unknown layout %d", layout);
     if (stack_lds >= 0 && stack_lds != 4 && stack_lds != 5 && stack_lds != 6 && stack_lds != 8 && stack_lds != 16)
@@ -489,12 +506,15 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
         return fail(c, PRAY_ERR_INVALID, "fetch_min must be in 1..64");
     if (occupancy >= 0 && (occupancy < 1 || occupancy > 100))
         return fail(c, PRAY_ERR_INVALID, "occupancy must be in 1..100");
+    if (wg_waves >= 0 && wg_waves != 1 && wg_waves != 2 && wg_waves != 4)
+        return fail(c, PRAY_ERR_INVALID, "wg_waves must be 1, 2 or 4");
     Tuning &t = sc->tune;
     if (layout >= 0) t.layout = layout;
     if (stack_lds >= 0) t.stack_lds = stack_lds;
     if (leaf_ilp >= 0) t.leaf_ilp = leaf_ilp;
     if (fetch_min >= 0) t.fetch_min = fetch_min;
     if (occupancy >= 0) t.occupancy = occupancy;
+    if (wg_waves >= 0) t.wg_waves = wg_waves;
     return PRAY_OK;
 }
 

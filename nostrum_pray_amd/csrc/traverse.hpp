@@ -39,6 +39,7 @@ constexpr uint32_t LEAF_EMPTY = 0x3FFFFFFFu;
 constexpr uint32_t REF_LAST = 0x80000000u;
 constexpr uint32_t NO_HIT = 0xFFFFFFFFu;
 constexpr float WIDEN = 1e-6f;
+constexpr int MAX_WG_WAVES = 4;      // waves per workgroup (each an independent persistent wave)
 constexpr uint32_t NCTR = 64;        // ray-chunk counters
 constexpr uint32_t CTR_STRIDE = 64;  // uint32 words between counters (256 bytes)
 
@@ -112,16 +113,23 @@ __device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, 
 // MULTI: rays of several scenes in one launch (packed layout only); each lane
 // takes its ray's node and record arrays and root cell from the scene table
 // when it picks the ray up.
+//
+// A workgroup holds 1..MAX_WG_WAVES waves that never synchronise with each
+// other; each has its own LDS ring in the dynamic shared memory the launch
+// sizes (ring_lds_bytes per wave).
 template <bool ANY, int LAYOUT, int K, int ILP, bool MULTI = false>
-static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) {
+static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s, TraceArgs A) {
     static_assert(K >= 2 && K <= 16, "LDS ring of 2..16 entries");
     static_assert(!MULTI || LAYOUT == 1, "multi-scene launches use the packed layout");
-    __shared__ uint32_t st_w[K * WAVE];
-    __shared__ float st_lo[K * WAVE];
-    __shared__ float st_hi[K * WAVE];
-    const uint32_t lane = threadIdx.x;
+    extern __shared__ uint32_t ring_lds[];
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t wave_in_block = threadIdx.x / WAVE;
+    const uint32_t gwave = blockIdx.x * (blockDim.x / WAVE) + wave_in_block;  // global wave index
+    uint32_t *st_w = ring_lds + wave_in_block * (3 * K * WAVE);
+    float *st_lo = reinterpret_cast<float *>(st_w + K * WAVE);
+    float *st_hi = reinterpret_cast<float *>(st_w + 2 * K * WAVE);
     const uint64_t below = (1ull << lane) - 1ull;
-    uint4 *spill = A.spill + (size_t)(blockIdx.x * WAVE + lane) * A.spill_depth;
+    uint4 *spill = A.spill + ((size_t)gwave * WAVE + lane) * A.spill_depth;
     const uint32_t nn = A.n_dev ? *A.n_dev : A.n;
     const uint4 *pairs = reinterpret_cast<const uint4 *>(s.nodes);
     const float4 *recs = s.recs;
@@ -133,7 +141,7 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
     // Wave-uniform pool of ray indices [pool_next, pool_end) and the counter
     // the wave takes its next chunk from.
     const uint32_t nchunks = (nn + WAVE - 1) / WAVE;
-    uint32_t pool_next = 0, pool_end = 0, ctr = blockIdx.x % NCTR;
+    uint32_t pool_next = 0, pool_end = 0, ctr = gwave % NCTR;
 
     while (true) {
         // ------------------------------------------------------- refill --
@@ -372,6 +380,9 @@ static __global__ __launch_bounds__(WAVE) void k_trace(DevScene s, TraceArgs A) 
         }
     }
 }
+
+// LDS of one wave's ring: K entries of (word, lo, hi) per lane.
+constexpr size_t ring_lds_bytes(int K) { return (size_t)3 * K * WAVE * sizeof(uint32_t); }
 
 static __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -37,15 +37,16 @@ def scene(name):
 
 
 # (layout, LDS ring entries): every traversal setting gives the same hits
-SETTINGS = [(P.LAYOUT_PACKED, 8), (P.LAYOUT_PACKED, 5), (P.LAYOUT_INDEXED, 8), (P.LAYOUT_INDEXED, 6)]
+# (layout, LDS ring entries, waves per workgroup)
+SETTINGS = [(P.LAYOUT_PACKED, 8, 1), (P.LAYOUT_PACKED, 5, 2), (P.LAYOUT_INDEXED, 8, 4), (P.LAYOUT_INDEXED, 6, 1)]
 
 
 def check_closest(ctx, name, rays, eps, bf_rows=0):
     s, t, flat = scene(name)
     sid = ctx.upload(t)
     try:
-        for layout, ring in SETTINGS:
-            ctx.set_tuning(sid, layout=layout, stack_lds=ring)
+        for layout, ring, wg in SETTINGS:
+            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg)
             say(f"closest {name} layout={layout} stack_lds={ring} n={len(rays)} eps={eps}")
             gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, eps)
             oi, ot = O.flat_trace(flat, rays, 0, eps)
@@ -96,8 +97,8 @@ def test_any_hit(ctx, name, eps, mode):
     sid = ctx.upload(t)
     try:
         res = []
-        for layout, ring in SETTINGS:
-            ctx.set_tuning(sid, layout=layout, stack_lds=ring)
+        for layout, ring, wg in SETTINGS:
+            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg)
             say(f"any {name} mode={mode} layout={layout} stack_lds={ring}")
             res.append(ctx.trace(sid, rays, mode, eps, limit2=lim))
     finally:
@@ -149,7 +150,7 @@ def test_launch_sizes(ctx, ring):
     _, t, flat = scene("spheres-sah")
     sid = ctx.upload(t)
     try:
-        ctx.set_tuning(sid, stack_lds=ring, fetch_min=1 if ring == 5 else 64)
+        ctx.set_tuning(sid, stack_lds=ring, fetch_min=1 if ring == 5 else 64, wg_waves=1 if ring == 5 else 4)
         base = O.gen_rays(12, P.scene_box_expanded(flat["box"]), 200_000)
         base[5, 3:] = 0.0
         base[7, :3] = [1e30, -1e30, 0.0]
