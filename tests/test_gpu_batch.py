@@ -29,21 +29,26 @@ def test_batch_equals_single_scene_renders(ctx):
     ctx.free_scene(sids[1])
     sids[1], _ = ctx.build_scene(abc, col, emit, method=P.BUILD_SAH, leaf_size=8)
     try:
-        say("batch of 5 scenes, 4 spp")
-        imgs, ms = ctx.render_pt_batch(sids, [c for _, _, c, _, _ in scenes], [wh for _, _, _, wh, _ in scenes],
-                                       4, 6, 3, [sd for *_, sd in scenes],
-                                       [s.info()["background"] for s, *_ in scenes], 1e-9)
-        batch_rays = ctx.last_render_rays()
-        assert ms > 0
-        single_rays = 0
+        single, single_rays = [], 0
         for k, (s, _, cam, (w, h), seed) in enumerate(scenes):
             say(f"single scene {k}")
             hdr, _ = ctx.render_pt(sids[k], cam, w, h, 4, 6, 3, seed, s.info()["background"], 1e-9, want_rgb=False)
             single_rays += ctx.last_render_rays()
-            assert np.array_equal(imgs[k].view(np.uint32), hdr.view(np.uint32)), (k, first_diff(imgs[k].ravel(),
-                                                                                              hdr.ravel()))
             assert hdr.mean() > 0
-        assert batch_rays == single_rays
+            single.append(hdr)
+        # The batch runs with the setting of scene 3 (the largest image of the
+        # size bucket holding most pixels): the defaults, then 4-wave
+        # workgroups with a 16-entry ring and two triangles per leaf step.
+        for tun in (dict(), dict(wg_waves=4, stack_lds=16, leaf_ilp=2, fetch_min=24)):
+            ctx.set_tuning(sids[3], **tun)
+            say(f"batch of 5 scenes, 4 spp, scene 3 tuning {tun}")
+            imgs, ms = ctx.render_pt_batch(sids, [c for _, _, c, _, _ in scenes],
+                                           [wh for _, _, _, wh, _ in scenes], 4, 6, 3, [sd for *_, sd in scenes],
+                                           [s.info()["background"] for s, *_ in scenes], 1e-9)
+            assert ms > 0 and ctx.last_render_rays() == single_rays
+            for k in range(len(scenes)):
+                assert np.array_equal(imgs[k].view(np.uint32), single[k].view(np.uint32)), \
+                    (k, tun, first_diff(imgs[k].ravel(), single[k].ravel()))
     finally:
         for sid in sids:
             ctx.free_scene(sid)
