@@ -4,22 +4,24 @@
 // ray finished from the wave's pool of ray indices, so long rays never hold a
 // wave of finished lanes.  The pool takes rays in chunks of 64 from NCTR
 // counters 256 bytes apart: counter j hands out chunks j, j + NCTR, ...; a wave
-// starts at counter (block % NCTR) and moves on to the next when one runs dry.
-// Thousands of resident waves taking every refill from one counter serialise
-// on its address (an atomic per refill); here each address sees 1/NCTR of one
-// atomic per 64 rays, and no wave is assumed to be resident from the start.  Every loop iteration each lane does an inner-node step
-// (one aligned 16-byte child-pair load), a leaf step (ILP triangle records
-// loaded together; the record array is padded so the look-ahead stays in
-// bounds), or both when the node step lands in a leaf.  Pending far children go to a per-lane stack:
-// K entries in an LDS ring, the oldest entries spilled to a per-lane global
-// area when the ring is full.
+// starts at counter (global wave index % NCTR) and moves on to the next when
+// one runs dry.  Thousands of resident waves taking every refill from one
+// counter serialise on its address (an atomic per refill); here each address
+// sees 1/NCTR of one atomic per 64 rays, and no wave is assumed to be resident
+// from the start.
+//
+// Every loop iteration each lane does an inner-node step (one aligned 16-byte
+// child-pair load), a leaf step (ILP triangle records loaded together; the
+// record array is padded so the look-ahead stays in bounds), or both when the
+// node step lands in a leaf.  Pending far children go to a per-lane stack: K
+// entries in an LDS ring, the oldest entries spilled to a per-lane global area
+// when the ring is full.
 //
 // Interval arithmetic: planes are intersected front to back with the
 // direction's sign choosing near/far; every plane distance is widened by a
 // relative 1e-6 (conservative: a cell is never skipped by rounding; infinite
-// distances stay infinite), and a
-// pending cell is pruned only when its widened entry exceeds the best t
-// (strictly), so equal-t ties are always examined.  The closest hit is the
+// distances stay infinite), and a pending cell is pruned only when its widened
+// entry exceeds the best t (strictly), so equal-t ties are always examined.  The closest hit is the
 // lexicographic minimum of (t, triangle index) over t > 0, which equals the
 // reference brute force (bf_search.cpp:26-40) on every ray.
 #pragma once
