@@ -28,6 +28,17 @@ def test_abi_exports_every_declared_symbol():
     assert len(declared("pray_hip.h")) >= 20 and len(declared("pray_host.h")) >= 20
 
 
+def test_tuning_array_length_matches_the_header():
+    """The Python wrapper passes PRAY_TUNING_FIELDS ints to pray_set_tuning /
+    pray_get_tuning: its field list must have the header's length."""
+    text = open(os.path.join(ROOT, "include", "pray_hip.h")).read()
+    n = int(re.search(r"^#define PRAY_TUNING_FIELDS (\d+)", text, re.MULTILINE).group(1))
+    assert len(P.TUNING_FIELDS) == n
+    # every field is documented in the header's list, in order
+    doc = re.findall(r"^ \*   \[(\d)\] (\w+)", text, re.MULTILINE)
+    assert [(int(i), f) for i, f in doc] == list(enumerate(P.TUNING_FIELDS))
+
+
 def test_abi_only_exports_c_symbols():
     import subprocess
     for lib in (P.HIP_LIB, P.HOST_LIB):
