@@ -335,7 +335,8 @@ static __global__ void k_morton_refs(uint32_t n, const uint32_t *vals, const uin
 //   k_sah_small     per node of at most SAH_SMALL references: one wave bins
 //                   them and picks the plane itself, with no histogram in
 //                   global memory (the same counts, costs and choice as the
-//                   two kernels below)
+//                   two kernels below); for each larger node it clears that
+//                   node's histogram
 //   k_sah_bins      per larger node and axis, counts of clipped reference
 //                   boxes starting and ending in each of SAH_BINS (64) bins
 //   k_sah_split     per larger node: the cheapest plane on a bin boundary against
@@ -426,12 +427,15 @@ static __global__ __launch_bounds__(256) void k_sah_bins(uint32_t n_refs, const 
 // (axis, bin) order below the leaf cost wins, as in k_sah_split's scan.
 static __global__ __launch_bounds__(256) void k_sah_small(uint32_t n_nodes, const LNode *nodes, const uint32_t *refs,
                                                           const float *tlo, const float *thi, SahParams P,
-                                                          uint32_t level, int *axis, float *pos) {
+                                                          uint32_t level, int *axis, float *pos, uint32_t *hist) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t k = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (k >= n_nodes) return;  // wave-uniform
     const LNode N = nodes[k];
-    if (N.count > SAH_SMALL) return;  // wave-uniform: k_sah_bins / k_sah_split
+    if (N.count > SAH_SMALL) {  // wave-uniform: clears its histogram for k_sah_bins / k_sah_split
+        for (uint32_t j = lane; j < (uint32_t)SAH_HIST; j += 64) hist[(size_t)k * SAH_HIST + j] = 0u;
+        return;
+    }
     const float dx = N.hi[0] - N.lo[0], dy = N.hi[1] - N.lo[1], dz = N.hi[2] - N.lo[2];
     const float sa = half_area(dx, dy, dz);
     if (!(N.count > P.leaf_size && level < P.max_depth && sa > 0.f)) {
@@ -539,40 +543,40 @@ static __global__ void k_sah_split(uint32_t n_nodes, const LNode *nodes, const u
     pos[k] = best_p;
 }
 
-// fl / fr[i] = reference i goes left / right (0 for references of leaves);
-// both arrays have one extra zero entry at n_refs for the scans.
+// f[i] = (reference i goes left, goes right), (0, 0) for references of
+// leaves; one extra zero entry at n_refs for the scan.
 static __global__ void k_sah_classify(uint32_t n_refs, const uint32_t *refs, const uint32_t *ref_node,
                                       const float *tlo, const float *thi, const LNode *nodes, const int *axis,
-                                      const float *pos, uint32_t *fl, uint32_t *fr) {
+                                      const float *pos, uint2 *f) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == n_refs) fl[i] = fr[i] = 0u;
+    if (i == n_refs) f[i] = make_uint2(0u, 0u);
     if (i >= n_refs) return;
     const uint32_t k = ref_node[i], t = refs[i];
     const int a = axis[k];
     if (a < 0) {
-        fl[i] = fr[i] = 0u;
+        f[i] = make_uint2(0u, 0u);
         return;
     }
     const float p = pos[k];
     const float clo = fmaxf(tlo[3 * (size_t)t + a], nodes[k].lo[a]);
     const float chi = fminf(thi[3 * (size_t)t + a], nodes[k].hi[a]);
-    fl[i] = (clo < p || (clo == p && chi == p)) ? 1u : 0u;
-    fr[i] = chi > p ? 1u : 0u;
+    f[i] = make_uint2((clo < p || (clo == p && chi == p)) ? 1u : 0u, chi > p ? 1u : 0u);
 }
 
-// Per-node values for the node scans: splits (1 per split node), references
-// of the children, references of leaves.
-static __global__ void k_sah_counts(uint32_t n_nodes, const LNode *nodes, const int *axis, const uint32_t *sl,
-                                    const uint32_t *sr, uint32_t *v_split, uint32_t *v_refs, uint32_t *v_leaf) {
+// Per-node values for the node scan, from the scanned reference flags s:
+// (splits (1 per split node), references of the children, references of
+// leaves, 0); one extra zero entry at n_nodes.
+static __global__ void k_sah_counts(uint32_t n_nodes, const LNode *nodes, const int *axis, const uint2 *s,
+                                    uint4 *v) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == n_nodes) v_split[k] = v_refs[k] = v_leaf[k] = 0u;
+    if (k == n_nodes) v[k] = make_uint4(0u, 0u, 0u, 0u);
     if (k >= n_nodes) return;
     const LNode N = nodes[k];
     if (axis[k] < 0) {
-        v_split[k] = 0u, v_refs[k] = 0u, v_leaf[k] = N.count;
+        v[k] = make_uint4(0u, 0u, N.count, 0u);
     } else {
-        const uint32_t nl = sl[N.begin + N.count] - sl[N.begin], nr = sr[N.begin + N.count] - sr[N.begin];
-        v_split[k] = 1u, v_refs[k] = nl + nr, v_leaf[k] = 0u;
+        const uint2 b = s[N.begin], e = s[N.begin + N.count];
+        v[k] = make_uint4(1u, (e.x - b.x) + (e.y - b.y), 0u, 0u);
     }
 }
 
@@ -581,8 +585,8 @@ struct Emit {
     const LNode *nodes;
     const int *axis;
     const float *pos;
-    const uint32_t *sl, *sr;                  // scanned reference flags
-    const uint32_t *s_split, *s_refs, *s_leaf;  // scanned node values
+    const uint2 *s;   // scanned reference flags (left, right)
+    const uint4 *sn;  // scanned node values (splits, child references, leaf references, 0)
     uint32_t pair_base, leaf_base;
     uint2 *out;          // device layout entries
     LNode *next;         // next level's nodes
@@ -593,23 +597,25 @@ static __global__ void k_sah_emit(Emit E) {
     if (k >= E.n_nodes) return;
     const LNode N = E.nodes[k];
     const int a = E.axis[k];
+    const uint4 sn = E.sn[k];
     if (a < 0) {
-        const uint32_t w = N.count ? (((E.leaf_base + E.s_leaf[k]) << 2) | LEAF_TAG) : ((prt::LEAF_EMPTY << 2) | LEAF_TAG);
+        const uint32_t w = N.count ? (((E.leaf_base + sn.z) << 2) | LEAF_TAG) : ((prt::LEAF_EMPTY << 2) | LEAF_TAG);
         E.out[N.entry] = make_uint2(__float_as_uint(N.plane), w);
         return;
     }
-    const uint32_t q = E.pair_base + E.s_split[k];
+    const uint32_t q = E.pair_base + sn.x;
     E.out[N.entry] = make_uint2(__float_as_uint(N.plane), ((2u * q) << 2) | (uint32_t)a);
-    const uint32_t nl = E.sl[N.begin + N.count] - E.sl[N.begin];
+    const uint2 b = E.s[N.begin], e = E.s[N.begin + N.count];
+    const uint32_t nl = e.x - b.x;
     const float p = E.pos[k];
     LNode L = N, R = N;
     L.hi[a] = p, R.lo[a] = p;
-    L.begin = E.s_refs[k], L.count = nl, L.entry = 2 * q, L.plane = p;
-    R.begin = E.s_refs[k] + nl;
-    R.count = E.sr[N.begin + N.count] - E.sr[N.begin];
+    L.begin = sn.y, L.count = nl, L.entry = 2 * q, L.plane = p;
+    R.begin = sn.y + nl;
+    R.count = e.y - b.y;
     R.entry = 2 * q + 1, R.plane = p;
-    E.next[2 * E.s_split[k]] = L;
-    E.next[2 * E.s_split[k] + 1] = R;
+    E.next[2 * sn.x] = L;
+    E.next[2 * sn.x + 1] = R;
 }
 
 struct Scatter {
@@ -617,8 +623,8 @@ struct Scatter {
     const uint32_t *refs, *ref_node;
     const LNode *nodes;
     const int *axis;
-    const uint32_t *fl, *fr, *sl, *sr;
-    const uint32_t *s_split, *s_refs, *s_leaf;
+    const uint2 *f, *s;  // reference flags (left, right) and their scan
+    const uint4 *sn;     // scanned node values, as in Emit
     uint32_t leaf_base;
     uint32_t *next_refs, *next_node;  // next level
     uint32_t *leaf_refs;              // final leaf runs (triangle | last flag)
@@ -629,21 +635,23 @@ static __global__ void k_sah_scatter(Scatter S) {
     if (i >= S.n_refs) return;
     const uint32_t k = S.ref_node[i], t = S.refs[i];
     const LNode N = S.nodes[k];
+    const uint4 sn = S.sn[k];
     if (S.axis[k] < 0) {
         const uint32_t r = i - N.begin;
-        S.leaf_refs[S.leaf_base + S.s_leaf[k] + r] = t | (r + 1 == N.count ? REF_LAST : 0u);
+        S.leaf_refs[S.leaf_base + sn.z + r] = t | (r + 1 == N.count ? REF_LAST : 0u);
         return;
     }
-    const uint32_t nl = S.sl[N.begin + N.count] - S.sl[N.begin];
-    if (S.fl[i]) {
-        const uint32_t j = S.s_refs[k] + (S.sl[i] - S.sl[N.begin]);
+    const uint2 b = S.s[N.begin], e = S.s[N.begin + N.count], f = S.f[i], si = S.s[i];
+    const uint32_t nl = e.x - b.x;
+    if (f.x) {
+        const uint32_t j = sn.y + (si.x - b.x);
         S.next_refs[j] = t;
-        S.next_node[j] = 2 * S.s_split[k];
+        S.next_node[j] = 2 * sn.x;
     }
-    if (S.fr[i]) {
-        const uint32_t j = S.s_refs[k] + nl + (S.sr[i] - S.sr[N.begin]);
+    if (f.y) {
+        const uint32_t j = sn.y + nl + (si.y - b.y);
         S.next_refs[j] = t;
-        S.next_node[j] = 2 * S.s_split[k] + 1;
+        S.next_node[j] = 2 * sn.x + 1;
     }
 }
 
@@ -769,13 +777,27 @@ int build_morton(pray_ctx *c, hipStream_t st, uint32_t n, uint32_t leaf_size, co
     return PRAY_OK;
 }
 
-// Exclusive sum of in[0 .. count) into out with rocPRIM's device scan.
-int scan_u32(pray_ctx *c, hipStream_t st, Grow &tmp, const uint32_t *in, uint32_t *out, size_t count) {
+// Lane-wise sums of the scanned vectors (each lane is its own count).
+struct AddU2 {
+    __host__ __device__ uint2 operator()(const uint2 &a, const uint2 &b) const {
+        return make_uint2(a.x + b.x, a.y + b.y);
+    }
+};
+struct AddU4 {
+    __host__ __device__ uint4 operator()(const uint4 &a, const uint4 &b) const {
+        return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+    }
+};
+
+// Exclusive lane-wise sum of in[0 .. count) into out with rocPRIM's device
+// scan: one scan carries several counts at once.
+template <class T, class Op>
+int scan_sum(pray_ctx *c, hipStream_t st, Grow &tmp, const T *in, T *out, size_t count, T zero, Op op) {
     size_t bytes = 0;
-    HIPC(c, rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, count, rocprim::plus<uint32_t>(), st));
+    HIPC(c, rocprim::exclusive_scan(nullptr, bytes, in, out, zero, count, op, st));
     int rc = tmp.reserve(c, std::max<size_t>(bytes, 1), false);
     if (rc) return rc;
-    HIPC(c, rocprim::exclusive_scan(tmp.p, bytes, in, out, 0u, count, rocprim::plus<uint32_t>(), st));
+    HIPC(c, rocprim::exclusive_scan(tmp.p, bytes, in, out, zero, count, op, st));
     return PRAY_OK;
 }
 
@@ -783,7 +805,9 @@ int scan_u32(pray_ctx *c, hipStream_t st, Grow &tmp, const uint32_t *in, uint32_
 int build_sah(pray_ctx *c, hipStream_t st, uint32_t n, const prb::SahParams &P, const float *tlo, const float *thi,
               const float *box, Grow &entries, Grow &leaf_refs, Built &out) {
     using prb::LNode;
-    Grow nodes[2], refs[2], rnode[2], fl, fr, sl, sr, axis, pos, hist, vs, vr, vl, ss, sr2, sleaf, tmp;
+    // f / s: reference flags (left, right) and their scan; v / sn: node values
+    // (splits, child references, leaf references, 0) and their scan
+    Grow nodes[2], refs[2], rnode[2], f, s, axis, pos, hist, v, sn, tmp;
     int rc = PRAY_OK;
     int cur = 0;
     // Up-front sizes from the triangle count (grown by doubling when a level
@@ -794,10 +818,14 @@ int build_sah(pray_ctx *c, hipStream_t st, uint32_t n, const prb::SahParams &P, 
     const size_t est_refs = (size_t)n + (size_t)n / 4 + 1;
     for (Grow *g : {&nodes[0], &nodes[1]})
         if ((rc = g->reserve(c, est_nodes * sizeof(LNode), false))) return rc;
-    for (Grow *g : {&refs[0], &refs[1], &rnode[0], &rnode[1], &fl, &fr, &sl, &sr})
+    for (Grow *g : {&refs[0], &refs[1], &rnode[0], &rnode[1]})
         if ((rc = g->reserve(c, est_refs * 4, false))) return rc;
-    for (Grow *g : {&axis, &pos, &vs, &vr, &vl, &ss, &sr2, &sleaf})
+    for (Grow *g : {&f, &s})
+        if ((rc = g->reserve(c, (est_refs + 1) * sizeof(uint2), false))) return rc;
+    for (Grow *g : {&axis, &pos})
         if ((rc = g->reserve(c, (est_nodes + 1) * 4, false))) return rc;
+    for (Grow *g : {&v, &sn})
+        if ((rc = g->reserve(c, (est_nodes + 1) * sizeof(uint4), false))) return rc;
     if ((rc = entries.reserve(c, 2 * est_nodes * 8, false)) || (rc = leaf_refs.reserve(c, est_refs * 4, false)))
         return rc;
     HIPC(c, hipMemsetAsync(entries.p, 0, 2 * 8, st));
@@ -808,50 +836,38 @@ int build_sah(pray_ctx *c, hipStream_t st, uint32_t n, const prb::SahParams &P, 
     for (uint32_t level = 0;; level++) {
         if ((rc = hist.reserve(c, (size_t)n_nodes * prb::SAH_HIST * 4, false)) ||
             (rc = axis.reserve(c, (size_t)n_nodes * 4, false)) || (rc = pos.reserve(c, (size_t)n_nodes * 4, false)) ||
-            (rc = fl.reserve(c, ((size_t)n_refs + 1) * 4, false)) ||
-            (rc = fr.reserve(c, ((size_t)n_refs + 1) * 4, false)) ||
-            (rc = sl.reserve(c, ((size_t)n_refs + 1) * 4, false)) ||
-            (rc = sr.reserve(c, ((size_t)n_refs + 1) * 4, false)) ||
-            (rc = vs.reserve(c, ((size_t)n_nodes + 1) * 4, false)) ||
-            (rc = vr.reserve(c, ((size_t)n_nodes + 1) * 4, false)) ||
-            (rc = vl.reserve(c, ((size_t)n_nodes + 1) * 4, false)) ||
-            (rc = ss.reserve(c, ((size_t)n_nodes + 1) * 4, false)) ||
-            (rc = sr2.reserve(c, ((size_t)n_nodes + 1) * 4, false)) ||
-            (rc = sleaf.reserve(c, ((size_t)n_nodes + 1) * 4, false)))
+            (rc = f.reserve(c, ((size_t)n_refs + 1) * sizeof(uint2), false)) ||
+            (rc = s.reserve(c, ((size_t)n_refs + 1) * sizeof(uint2), false)) ||
+            (rc = v.reserve(c, ((size_t)n_nodes + 1) * sizeof(uint4), false)) ||
+            (rc = sn.reserve(c, ((size_t)n_nodes + 1) * sizeof(uint4), false)))
             return rc;
         const LNode *N = nodes[cur].as<LNode>();
         const uint32_t *R = refs[cur].as<uint32_t>(), *RN = rnode[cur].as<uint32_t>();
-        hipLaunchKernelGGL(prt::k_zero_u32, dim3(nblocks((uint64_t)n_nodes * prb::SAH_HIST, 256)), dim3(256), 0, st,
-                           hist.as<uint32_t>(), n_nodes * (uint32_t)prb::SAH_HIST);
+        // small nodes are decided here; larger ones get a cleared histogram
+        hipLaunchKernelGGL(prb::k_sah_small, dim3(nblocks(n_nodes, 4)), dim3(256), 0, st, n_nodes, N, R, tlo, thi, P,
+                           level, axis.as<int>(), pos.as<float>(), hist.as<uint32_t>());
         if (n_refs)
             hipLaunchKernelGGL(prb::k_sah_bins, dim3((n_refs + prb::SAH_CHUNK - 1) / prb::SAH_CHUNK), dim3(256), 0,
                                st, n_refs, R, RN, tlo, thi, N, hist.as<uint32_t>());
-        hipLaunchKernelGGL(prb::k_sah_small, dim3(nblocks(n_nodes, 4)), dim3(256), 0, st, n_nodes, N, R, tlo, thi, P,
-                           level, axis.as<int>(), pos.as<float>());
         hipLaunchKernelGGL(prb::k_sah_split, dim3(nblocks(n_nodes, 256)), dim3(256), 0, st, n_nodes, N,
                            (const uint32_t *)hist.as<uint32_t>(), P, level, axis.as<int>(), pos.as<float>());
         hipLaunchKernelGGL(prb::k_sah_classify, dim3(nblocks((uint64_t)n_refs + 1, 256)), dim3(256), 0, st, n_refs, R,
                            RN, tlo, thi, N, (const int *)axis.as<int>(), (const float *)pos.as<float>(),
-                           fl.as<uint32_t>(), fr.as<uint32_t>());
+                           f.as<uint2>());
         HIPC(c, hipGetLastError());
-        if ((rc = scan_u32(c, st, tmp, fl.as<uint32_t>(), sl.as<uint32_t>(), (size_t)n_refs + 1)) ||
-            (rc = scan_u32(c, st, tmp, fr.as<uint32_t>(), sr.as<uint32_t>(), (size_t)n_refs + 1)))
+        if ((rc = scan_sum(c, st, tmp, (const uint2 *)f.as<uint2>(), s.as<uint2>(), (size_t)n_refs + 1,
+                           make_uint2(0u, 0u), AddU2{})))
             return rc;
         hipLaunchKernelGGL(prb::k_sah_counts, dim3(nblocks((uint64_t)n_nodes + 1, 256)), dim3(256), 0, st, n_nodes, N,
-                           (const int *)axis.as<int>(), (const uint32_t *)sl.as<uint32_t>(),
-                           (const uint32_t *)sr.as<uint32_t>(), vs.as<uint32_t>(), vr.as<uint32_t>(),
-                           vl.as<uint32_t>());
+                           (const int *)axis.as<int>(), (const uint2 *)s.as<uint2>(), v.as<uint4>());
         HIPC(c, hipGetLastError());
-        if ((rc = scan_u32(c, st, tmp, vs.as<uint32_t>(), ss.as<uint32_t>(), (size_t)n_nodes + 1)) ||
-            (rc = scan_u32(c, st, tmp, vr.as<uint32_t>(), sr2.as<uint32_t>(), (size_t)n_nodes + 1)) ||
-            (rc = scan_u32(c, st, tmp, vl.as<uint32_t>(), sleaf.as<uint32_t>(), (size_t)n_nodes + 1)))
+        if ((rc = scan_sum(c, st, tmp, (const uint4 *)v.as<uint4>(), sn.as<uint4>(), (size_t)n_nodes + 1,
+                           make_uint4(0u, 0u, 0u, 0u), AddU4{})))
             return rc;
-        uint32_t tot[3];
-        HIPC(c, hipMemcpyAsync(&tot[0], ss.as<uint32_t>() + n_nodes, 4, hipMemcpyDeviceToHost, st));
-        HIPC(c, hipMemcpyAsync(&tot[1], sr2.as<uint32_t>() + n_nodes, 4, hipMemcpyDeviceToHost, st));
-        HIPC(c, hipMemcpyAsync(&tot[2], sleaf.as<uint32_t>() + n_nodes, 4, hipMemcpyDeviceToHost, st));
+        uint4 tot;  // the totals of the level: the scan's entry at n_nodes
+        HIPC(c, hipMemcpyAsync(&tot, sn.as<uint4>() + n_nodes, sizeof(uint4), hipMemcpyDeviceToHost, st));
         HIPC(c, hipStreamSynchronize(st));
-        const uint32_t n_split = tot[0], next_refs = tot[1], level_leaf = tot[2];
+        const uint32_t n_split = tot.x, next_refs = tot.y, level_leaf = tot.z;
         if ((uint64_t)next_refs + leaf_base + level_leaf > ref_cap)
             return fail(c, PRAY_ERR_UNSUPPORTED, "SAH build: triangle references exceed 16 per triangle");
         const int nxt = cur ^ 1;
@@ -862,15 +878,11 @@ int build_sah(pray_ctx *c, hipStream_t st, uint32_t n, const prb::SahParams &P, 
             (rc = rnode[nxt].reserve(c, std::max<size_t>(next_refs, 1) * 4, false)))
             return rc;
         prb::Emit E{n_nodes, N, (const int *)axis.as<int>(), (const float *)pos.as<float>(),
-                    (const uint32_t *)sl.as<uint32_t>(), (const uint32_t *)sr.as<uint32_t>(),
-                    (const uint32_t *)ss.as<uint32_t>(), (const uint32_t *)sr2.as<uint32_t>(),
-                    (const uint32_t *)sleaf.as<uint32_t>(), pair_base, leaf_base, entries.as<uint2>(),
-                    nodes[nxt].as<LNode>()};
+                    (const uint2 *)s.as<uint2>(), (const uint4 *)sn.as<uint4>(), pair_base, leaf_base,
+                    entries.as<uint2>(), nodes[nxt].as<LNode>()};
         hipLaunchKernelGGL(prb::k_sah_emit, dim3(nblocks(n_nodes, 256)), dim3(256), 0, st, E);
-        prb::Scatter Sc{n_refs, R, RN, N, (const int *)axis.as<int>(), (const uint32_t *)fl.as<uint32_t>(),
-                        (const uint32_t *)fr.as<uint32_t>(), (const uint32_t *)sl.as<uint32_t>(),
-                        (const uint32_t *)sr.as<uint32_t>(), (const uint32_t *)ss.as<uint32_t>(),
-                        (const uint32_t *)sr2.as<uint32_t>(), (const uint32_t *)sleaf.as<uint32_t>(), leaf_base,
+        prb::Scatter Sc{n_refs, R, RN, N, (const int *)axis.as<int>(), (const uint2 *)f.as<uint2>(),
+                        (const uint2 *)s.as<uint2>(), (const uint4 *)sn.as<uint4>(), leaf_base,
                         refs[nxt].as<uint32_t>(), rnode[nxt].as<uint32_t>(), leaf_refs.as<uint32_t>()};
         if (n_refs) hipLaunchKernelGGL(prb::k_sah_scatter, dim3(nblocks(n_refs, 256)), dim3(256), 0, st, Sc);
         HIPC(c, hipGetLastError());
