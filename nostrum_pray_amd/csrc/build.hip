@@ -21,8 +21,9 @@
 //
 // Kernels, in order: k_prep (edges, normals, boxes), k_box_partial /
 // k_box_final (scene box), k_morton, a stable radix sort of (code, index),
-// k_radix_tree, k_bottom_up (boxes, counts, collapsed heights), k_write_nodes
-// and k_records.
+// k_radix_tree, k_levels_init / k_levels_down (inner nodes in level order),
+// k_levels_up (boxes, counts, collapsed heights, deepest level first),
+// k_write_nodes and k_records.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -173,8 +174,6 @@ struct Radix {
     const uint32_t *keys;
     uint32_t n;
     uint32_t *left, *right;   // child: inner node index, or LEAF_BIT | sorted position
-    uint32_t *parent;         // of inner nodes (n-1 entries; the root's is unused)
-    uint32_t *leaf_parent;    // of sorted positions
     uint32_t *first, *last;   // sorted range of each inner node
 };
 
@@ -203,16 +202,13 @@ static __global__ void k_radix_tree(Radix R) {
     const uint32_t rc = hi == g + 1 ? (LEAF_BIT | (uint32_t)(g + 1)) : (uint32_t)(g + 1);
     R.left[i] = lc, R.right[i] = rc;
     R.first[i] = (uint32_t)lo, R.last[i] = (uint32_t)hi;
-    if (lc & LEAF_BIT) R.leaf_parent[g] = (uint32_t)i; else R.parent[g] = (uint32_t)i;
-    if (rc & LEAF_BIT) R.leaf_parent[g + 1] = (uint32_t)i; else R.parent[g + 1] = (uint32_t)i;
 }
 
 struct Up {
     uint32_t n, leaf_size;
     const uint32_t *vals;       // sorted position -> triangle
     const float *tlo, *thi;     // triangle boxes
-    const uint32_t *left, *right, *parent, *leaf_parent;
-    uint32_t *arrivals;         // n-1 counters, zero at launch
+    const uint32_t *left, *right;
     float *nlo, *nhi;           // inner node boxes, (n-1) x 3
     uint32_t *count, *height;   // triangles under an inner node; height with collapsed leaves
 };
@@ -230,30 +226,76 @@ __device__ __forceinline__ void child_info(const Up &U, uint32_t c, float lo[3],
     }
 }
 
-// One thread per sorted position walks up; the second thread to reach an
-// inner node combines its children and goes on.
-static __global__ void k_bottom_up(Up U) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= U.n || U.n < 2) return;
-    uint32_t node = U.leaf_parent[k];
-    while (true) {
-        __threadfence();
-        if (atomicAdd(&U.arrivals[node], 1u) == 0u) return;  // the sibling is not done yet
-        __threadfence();
-        float llo[3], lhi[3], rlo[3], rhi[3];
-        uint32_t lc, lh, rc, rh;
-        child_info(U, U.left[node], llo, lhi, lc, lh);
-        child_info(U, U.right[node], rlo, rhi, rc, rh);
-        for (int q = 0; q < 3; q++) {
-            U.nlo[3 * (size_t)node + q] = fminf(llo[q], rlo[q]);
-            U.nhi[3 * (size_t)node + q] = fmaxf(lhi[q], rhi[q]);
-        }
-        const uint32_t cnt = lc + rc;
-        U.count[node] = cnt;
-        U.height[node] = cnt <= U.leaf_size ? 0u : 1u + max(lh, rh);
-        if (node == 0) return;
-        node = U.parent[node];
+// The inner nodes in level order, so that boxes are combined one level per
+// launch, deepest first: a bottom-up walk in one launch would need a
+// device-scope fence per step, and on a part whose L2s are not coherent with
+// each other each such fence writes back and invalidates the L2.  Level d+1
+// lists the inner children of level d's nodes.  The radix tree is less than
+// 65 levels deep: each level lengthens the common prefix of the 64-bit
+// (key, position) pairs.
+constexpr uint32_t UP_LEVELS = 72;
+
+struct Levels {
+    const uint32_t *left, *right;
+    uint32_t n_inner;
+    uint32_t *order;  // inner nodes, level after level (n_inner entries)
+    uint32_t *count;  // nodes per level (UP_LEVELS + 1)
+    uint32_t *start;  // first entry of each level in order (UP_LEVELS + 1)
+};
+
+static __global__ void k_levels_init(Levels L) {
+    for (uint32_t j = threadIdx.x; j <= UP_LEVELS; j += blockDim.x) {
+        L.count[j] = j == 0 ? 1u : 0u;
+        L.start[j] = 0u;
     }
+    if (threadIdx.x == 0) L.order[0] = 0u;  // the root
+}
+
+// Appends the inner children of level d's nodes to level d+1, in any order
+// (a node's values depend on its children only); one atomic per wave.
+static __global__ __launch_bounds__(256) void k_levels_down(Levels L, uint32_t d) {
+    const uint32_t s = L.start[d], cnt = L.count[d], s1 = s + cnt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) L.start[d + 1] = s1;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < cnt; i0 += stride) {  // wave-uniform
+        const uint32_t i = i0 + lane;
+        uint32_t l = LEAF_BIT, r = LEAF_BIT;
+        if (i < cnt && s + i < L.n_inner) {
+            const uint32_t node = L.order[s + i];
+            l = L.left[node], r = L.right[node];
+        }
+        const bool hl = !(l & LEAF_BIT), hr = !(r & LEAF_BIT);
+        const uint64_t bl = __ballot(hl), br = __ballot(hr);
+        const uint32_t off = (uint32_t)(__popcll(bl & below) + __popcll(br & below));
+        const uint32_t total = (uint32_t)(__popcll(bl) + __popcll(br));
+        uint32_t base = 0u;
+        if (lane == 0 && total) base = atomicAdd(&L.count[d + 1], total);
+        base = (uint32_t)__shfl((int)base, 0);
+        const uint32_t j = s1 + base + off;
+        if (hl && j < L.n_inner) L.order[j] = l;
+        if (hr && j + (hl ? 1u : 0u) < L.n_inner) L.order[j + (hl ? 1u : 0u)] = r;
+    }
+}
+
+// Boxes, counts and collapsed heights of one level's nodes (their inner
+// children are on the next level, combined by the previous launch).
+static __global__ void k_levels_up(Up U, const uint32_t *order, uint32_t s, uint32_t cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const uint32_t node = order[s + i];
+    float llo[3], lhi[3], rlo[3], rhi[3];
+    uint32_t lc, lh, rc, rh;
+    child_info(U, U.left[node], llo, lhi, lc, lh);
+    child_info(U, U.right[node], rlo, rhi, rc, rh);
+    for (int q = 0; q < 3; q++) {
+        U.nlo[3 * (size_t)node + q] = fminf(llo[q], rlo[q]);
+        U.nhi[3 * (size_t)node + q] = fmaxf(lhi[q], rhi[q]);
+    }
+    const uint32_t c = lc + rc;
+    U.count[node] = c;
+    U.height[node] = c <= U.leaf_size ? 0u : 1u + max(lh, rh);
 }
 
 struct Writer {
@@ -733,15 +775,16 @@ int build_morton(pray_ctx *c, hipStream_t st, uint32_t n, uint32_t leaf_size, co
     Scratch S;
     int rc = PRAY_OK;
     uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr, *left = nullptr,
-             *right = nullptr, *parent = nullptr, *leaf_parent = nullptr, *first = nullptr, *last = nullptr,
-             *arrivals = nullptr, *count = nullptr, *height = nullptr;
+             *right = nullptr, *first = nullptr, *last = nullptr, *order = nullptr, *lcount = nullptr,
+             *lstart = nullptr, *count = nullptr, *height = nullptr;
     float *nlo = nullptr, *nhi = nullptr;
     uint8_t *last_flag = nullptr;
     const size_t inner = n > 1 ? n - 1 : 1;
     if ((rc = S.get(c, &keys, n)) || (rc = S.get(c, &vals, n)) || (rc = S.get(c, &keys2, n)) ||
         (rc = S.get(c, &vals2, n)) || (rc = S.get(c, &left, inner)) || (rc = S.get(c, &right, inner)) ||
-        (rc = S.get(c, &parent, inner)) || (rc = S.get(c, &leaf_parent, n)) || (rc = S.get(c, &first, inner)) ||
-        (rc = S.get(c, &last, inner)) || (rc = S.get(c, &arrivals, inner)) || (rc = S.get(c, &count, inner)) ||
+        (rc = S.get(c, &first, inner)) || (rc = S.get(c, &last, inner)) || (rc = S.get(c, &order, inner)) ||
+        (rc = S.get(c, &lcount, prb::UP_LEVELS + 1)) || (rc = S.get(c, &lstart, prb::UP_LEVELS + 1)) ||
+        (rc = S.get(c, &count, inner)) ||
         (rc = S.get(c, &height, inner)) || (rc = S.get(c, &nlo, 3 * inner)) || (rc = S.get(c, &nhi, 3 * inner)) ||
         (rc = S.get(c, &last_flag, n)))
         return rc;
@@ -749,7 +792,6 @@ int build_morton(pray_ctx *c, hipStream_t st, uint32_t n, uint32_t leaf_size, co
         return rc;
     uint2 *nodes = entries.as<uint2>();
     HIPC(c, hipMemsetAsync(nodes, 0, 2 * (size_t)n * 8, st));
-    HIPC(c, hipMemsetAsync(arrivals, 0, inner * 4, st));
     HIPC(c, hipMemsetAsync(last_flag, 0, (size_t)n, st));
     size_t sort_bytes = 0;
     HIPC(c, rocprim::radix_sort_pairs(nullptr, sort_bytes, keys, keys2, vals, vals2, n, 0u, prb::MORTON_BITS, st));
@@ -759,12 +801,33 @@ int build_morton(pray_ctx *c, hipStream_t st, uint32_t n, uint32_t leaf_size, co
     hipLaunchKernelGGL(prb::k_morton, dim3(g), dim3(256), 0, st, n, tlo, thi, box, keys, vals);
     HIPC(c, hipGetLastError());
     HIPC(c, rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys, keys2, vals, vals2, n, 0u, prb::MORTON_BITS, st));
-    prb::Radix R{keys2, n, left, right, parent, leaf_parent, first, last};
-    prb::Up U{n, leaf_size, vals2, tlo, thi, left, right, parent, leaf_parent, arrivals, nlo, nhi, count, height};
+    prb::Radix R{keys2, n, left, right, first, last};
+    prb::Up U{n, leaf_size, vals2, tlo, thi, left, right, nlo, nhi, count, height};
     prb::Writer W{n, leaf_size, keys2, left, right, first, last, count, nlo, nhi, nodes, last_flag};
     if (n > 1) {
         hipLaunchKernelGGL(prb::k_radix_tree, dim3(nblocks(n - 1, 256)), dim3(256), 0, st, R);
-        hipLaunchKernelGGL(prb::k_bottom_up, dim3(g), dim3(256), 0, st, U);
+        // inner nodes in level order (one launch per possible level: the
+        // sizes are on the device), then one launch per level from the deepest
+        const prb::Levels L{left, right, n - 1, order, lcount, lstart};
+        hipLaunchKernelGGL(prb::k_levels_init, dim3(1), dim3(128), 0, st, L);
+        const unsigned down_blocks = std::min<unsigned>(nblocks(n - 1, 256), 1024u);
+        for (uint32_t d = 0; d < prb::UP_LEVELS; d++)
+            hipLaunchKernelGGL(prb::k_levels_down, dim3(down_blocks), dim3(256), 0, st, L, d);
+        HIPC(c, hipGetLastError());
+        uint32_t lc[prb::UP_LEVELS + 1];
+        HIPC(c, hipMemcpyAsync(lc, lcount, sizeof lc, hipMemcpyDeviceToHost, st));
+        HIPC(c, hipStreamSynchronize(st));
+        uint32_t ls[prb::UP_LEVELS + 1], total = 0, levels = 0;
+        for (uint32_t d = 0; d <= prb::UP_LEVELS; d++) {
+            ls[d] = total, total += lc[d];
+            if (lc[d]) levels = d + 1;
+        }
+        if (lc[prb::UP_LEVELS] || total != n - 1)
+            return fail(c, PRAY_ERR_UNSUPPORTED, "Morton build: %u inner nodes in %u levels (expected %u)", total,
+                        levels, n - 1);
+        for (uint32_t d = levels; d-- > 0;)
+            hipLaunchKernelGGL(prb::k_levels_up, dim3(nblocks(lc[d], 256)), dim3(256), 0, st, U,
+                               (const uint32_t *)order, ls[d], lc[d]);
     }
     hipLaunchKernelGGL(prb::k_write_nodes, dim3(nblocks(inner, 256)), dim3(256), 0, st, W, U);
     hipLaunchKernelGGL(prb::k_morton_refs, dim3(g), dim3(256), 0, st, n, (const uint32_t *)vals2,
