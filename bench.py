@@ -12,7 +12,7 @@ Run:  python bench.py [--steps K] [--warmup W]
 Other workloads, one JSON line each: --workload batch (64 scenes path traced
 in one wavefront), --workload io (flat tree files, progressive path tracing,
 the drop-in CLI on the example scenes), --workload cache (the radiance-cache
-comparison alone).
+comparison alone), --workload build (the KD-tree builds on the GPU).
 """
 import argparse
 import json
@@ -80,10 +80,10 @@ def main():
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-RMSE measurement")
     ap.add_argument("--cache-size", type=int, default=1024, help="image side of the radiance-cache measurement")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples per pixel of the plain reference image")
-    ap.add_argument("--workload", choices=("terrain", "batch", "io", "cache"), default="terrain",
+    ap.add_argument("--workload", choices=("terrain", "batch", "io", "cache", "build"), default="terrain",
                     help="terrain: the closest-hit stream (default); batch: 64 scenes path traced in one wavefront; "
                          "io: flat tree files, progressive path tracing and the CLI; cache: the radiance-cache "
-                         "equal-RMSE measurement alone")
+                         "equal-RMSE measurement alone; build: the KD-tree builds on the GPU")
     ap.add_argument("--batch-scenes", type=int, default=64)
     args = ap.parse_args()
 
@@ -99,6 +99,9 @@ def main():
         return
     if args.workload == "io":
         io_workload(P, args)
+        return
+    if args.workload == "build":
+        build_workload(P, args)
         return
     if args.workload == "cache":
         ctx = P.Context(0)
@@ -540,6 +543,59 @@ def io_workload(P, args):
     finally:
         ctx.close()
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+def build_workload(P, args):
+    """The KD-tree builds on the GPU, timed on the terrain: the binned SAH
+    build and the Morton hierarchy, each five times (the median of the last
+    four; the first loads the kernels).  The CPU's binned SAH builder, which
+    makes the same tree, is timed beside them, and closest-hit traces of
+    2^20 rays over the GPU- and CPU-built SAH trees must agree bit for bit.
+    One JSON line."""
+    ctx = P.Context(0)
+    try:
+        scene = P.Scene.terrain(args.grid, args.seed)
+        abc, _, _ = scene.triangles()
+        gpu, sids = {}, {}
+        for name, method in (("binned_sah", P.BUILD_SAH), ("morton", P.BUILD_MORTON)):
+            runs = []
+            for i in range(5):
+                t0 = time.perf_counter()
+                sid, ms = ctx.build_scene(abc, method=method, leaf_size=args.leaf_size, max_depth=args.max_depth,
+                                          traversal_cost=args.traversal_cost)
+                ctx.sync()
+                runs.append((ms, (time.perf_counter() - t0) * 1e3))
+                if i < 4:
+                    ctx.free_scene(sid)
+            sids[name] = sid
+            lay = ctx.scene_layout(sid)
+            gpu[name] = {"build_ms": round(float(np.median([r[0] for r in runs[1:]])), 3),
+                         "wall_ms_with_upload": round(float(np.median([r[1] for r in runs[1:]])), 3),
+                         "first_build_ms": round(runs[0][0], 3), "depth": int(lay["max_depth"]),
+                         "nodes": len(lay["nodes"]) // 2, "references": len(lay["refs"])}
+            log(f"build: GPU {name} {gpu[name]}")
+        del abc
+        t0 = time.perf_counter()
+        tree = scene.build(2, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+        cpu_s = time.perf_counter() - t0
+        sid_c = ctx.upload(tree, shading=False)
+        rays = ctx.gen_rays(args.seed, P.scene_box_expanded(tree.box()), 1 << 20)
+        ig, tg = ctx.trace(sids["binned_sah"], rays, P.TRACE_CLOSEST, args.det_eps)
+        ic, tc = ctx.trace(sid_c, rays, P.TRACE_CLOSEST, args.det_eps)
+        same = bool(np.array_equal(ig, ic) and np.array_equal(tg.view(np.uint32), tc.view(np.uint32)))
+        log(f"build: CPU binned SAH {cpu_s:.2f} s, identical traces {same}")
+        print(json.dumps({
+            "metric": "KD-tree build time on the GPU", "n_gpus": 1, "data": "synthetic",
+            "config": {"workload": f"terrain {args.grid}x{args.grid} cells = {scene.num_tris} triangles, depth cap "
+                                   f"{args.max_depth}, leaf size {args.leaf_size}, traversal cost "
+                                   f"{args.traversal_cost:g}, 64 bins per axis"},
+            "gpu": gpu,
+            "cpu_binned_sah": {"build_s": round(cpu_s, 3), "cores": int(os.environ.get("OMP_NUM_THREADS",
+                                                                                         os.cpu_count() or 1))},
+            "identical_traces": same,
+        }), flush=True)
+    finally:
+        ctx.close()
 
 
 def cli_stage_ms(text):
