@@ -109,6 +109,17 @@ def test_gpu_build_terrain_full_size(ctx):
             assert g["max_depth"] <= 64
             tri = np.unique(g["refs"] & 0x7FFFFFFF)
             assert np.array_equal(tri, np.arange(abc.shape[0], dtype=np.uint32))
+            if method == P.BUILD_MORTON:
+                # word for word: some levels hold more inner nodes (at least
+                # 398,870 at depth 24) than one pass of the level-order
+                # kernel's 1024 x 256 threads
+                say("oracle twin of the terrain2048 Morton build")
+                o = O.lbvh_build(np.ascontiguousarray(abc, np.float32).reshape(-1, 9), 8)
+                assert g["max_depth"] == o["max_depth"]
+                assert np.array_equal(g["box"].view(np.uint32), o["box"].view(np.uint32))
+                assert np.array_equal(g["refs"], o["refs"])
+                assert first_diff(g["nodes"], o["nodes"]) is None, first_diff(g["nodes"], o["nodes"])
+                del o
             if box is None:
                 box = g["box"]
                 rays = O.gen_rays(2024, P.scene_box_expanded(box), 1 << 20)
