@@ -107,7 +107,11 @@ PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
  *   traversal_cost, intersection_cost  SAH costs (<= 0: 2 and 1)
  *   build_ms   optional: device time from the uploaded triangles to the
  *              finished layout
- * The new scene takes the default tuning. */
+ * The new scene takes the default tuning.  The context keeps the builds'
+ * device scratch, grown to the largest build so far, until pray_destroy: a
+ * scene rebuilt every frame allocates only its own arrays after the first
+ * build.  The largest part is the SAH level histograms, 1,536 bytes per node
+ * of the widest tree level. */
 #define PRAY_BUILD_MORTON 0
 #define PRAY_BUILD_SAH 1
 PRAY_API int pray_build_scene(pray_ctx *ctx, const float *tris, const float *color, const uint8_t *emit, uint32_t n,

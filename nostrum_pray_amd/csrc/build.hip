@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -725,21 +726,6 @@ static __global__ void k_sah_root(const float *box, uint32_t n, LNode *nodes, ui
 namespace prc {
 namespace {
 
-// Scratch of one build, freed on every exit.
-struct Scratch {
-    std::vector<void *> ptrs;
-    ~Scratch() {
-        for (void *p : ptrs) (void)hipFree(p);
-    }
-    template <class T> int get(pray_ctx *c, T **p, size_t count) {
-        void *q = nullptr;
-        HIPC(c, hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
-        ptrs.push_back(q);
-        *p = static_cast<T *>(q);
-        return PRAY_OK;
-    }
-};
-
 // A device buffer that grows (by at least half) and optionally keeps its
 // contents; freed on every exit.
 struct Grow {
@@ -764,6 +750,35 @@ struct Grow {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Numbered device buffers of one call site, taken in the same order on every
+// call.
+struct Pool {
+    std::deque<Grow> slots;  // a deque: taking a new slot moves no other
+    size_t next = 0;
+    template <class T> int get(pray_ctx *c, T **p, size_t count) {
+        if (next == slots.size()) slots.emplace_back();
+        Grow &g = slots[next++];
+        const int rc = g.reserve(c, std::max<size_t>(count, 1) * sizeof(T), false);
+        if (rc) return rc;
+        *p = g.as<T>();
+        return PRAY_OK;
+    }
+};
+
+// The tree builds' device buffers, kept by the context from one build to the
+// next (a scene rebuilt every frame allocates nothing after its first build)
+// and freed by pray_destroy.  Builds on one context run one at a time.
+struct BuildScratch {
+    Pool prep, morton;
+    Grow nodes[2], refs[2], rnode[2], f, s, axis, pos, hist, v, sn, tmp;  // the SAH levels
+    Grow entries, leaf_refs;  // a build's output before it is copied into the scene
+};
+
+BuildScratch &build_scratch(pray_ctx *c) {
+    if (!c->build_scratch) c->build_scratch = std::make_shared<BuildScratch>();
+    return *static_cast<BuildScratch *>(c->build_scratch.get());
+}
+
 struct Built {
     uint32_t n_entries = 0, n_refs = 0, depth = 0;
 };
@@ -772,7 +787,8 @@ struct Built {
 // refs = n leaf references.
 int build_morton(pray_ctx *c, hipStream_t st, uint32_t n, uint32_t leaf_size, const float *tlo, const float *thi,
                  const float *box, Grow &entries, Grow &refs, Built &out) {
-    Scratch S;
+    Pool &S = build_scratch(c).morton;
+    S.next = 0;
     int rc = PRAY_OK;
     uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr, *left = nullptr,
              *right = nullptr, *first = nullptr, *last = nullptr, *order = nullptr, *lcount = nullptr,
@@ -870,7 +886,9 @@ int build_sah(pray_ctx *c, hipStream_t st, uint32_t n, const prb::SahParams &P, 
     using prb::LNode;
     // f / s: reference flags (left, right) and their scan; v / sn: node values
     // (splits, child references, leaf references, 0) and their scan
-    Grow nodes[2], refs[2], rnode[2], f, s, axis, pos, hist, v, sn, tmp;
+    BuildScratch &X = build_scratch(c);
+    Grow *nodes = X.nodes, *refs = X.refs, *rnode = X.rnode;
+    Grow &f = X.f, &s = X.s, &axis = X.axis, &pos = X.pos, &hist = X.hist, &v = X.v, &sn = X.sn, &tmp = X.tmp;
     int rc = PRAY_OK;
     int cur = 0;
     // Up-front sizes from the triangle count (grown by doubling when a level
@@ -1007,7 +1025,8 @@ int pray_build_scene(pray_ctx *c, const float *tris, const float *color, const u
     for (int k = 0; k < 2; k++)
         if (!c->ievents[k]) HIPC(c, hipEventCreate(&c->ievents[k]));
     int rc = PRAY_OK;
-    Scratch S;
+    Pool &S = build_scratch(c).prep;
+    S.next = 0;
     float *abc = nullptr, *tlo = nullptr, *thi = nullptr, *partial = nullptr, *box = nullptr;
     if ((rc = S.get(c, &abc, 9 * (size_t)n)) || (rc = S.get(c, &tlo, 3 * (size_t)n)) ||
         (rc = S.get(c, &thi, 3 * (size_t)n)) || (rc = S.get(c, &partial, 6 * (size_t)prb::BOX_PARTIALS)) ||
@@ -1051,7 +1070,7 @@ int pray_build_scene(pray_ctx *c, const float *tris, const float *color, const u
                        (const float *)thi, partial);
     hipLaunchKernelGGL(prb::k_box_final, dim3(1), dim3(prb::BOX_BLOCK), 0, st, nb, (const float *)partial, box);
     HIPC(c, hipGetLastError());
-    Grow entries, refs;
+    Grow &entries = build_scratch(c).entries, &refs = build_scratch(c).leaf_refs;
     Built B;
     rc = method == PRAY_BUILD_MORTON ? build_morton(c, st, n, leaf_size, tlo, thi, box, entries, refs, B)
                                      : build_sah(c, st, n, P, tlo, thi, box, entries, refs, B);

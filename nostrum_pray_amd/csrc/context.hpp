@@ -56,6 +56,7 @@ struct pray_ctx {
     size_t spill_bytes = 0;
     std::map<std::pair<const void *, int>, int> occupancy;  // (kernel, waves per block) -> resident blocks per CU
     std::map<int, prc::Tuning> tuned;        // scene-size bucket -> autotuned tuning
+    std::shared_ptr<void> build_scratch;     // device buffers of the tree builds, kept across builds
 };
 
 namespace prc {

@@ -387,6 +387,7 @@ void pray_destroy(pray_ctx *c) {
     for (hipEvent_t e : c->kevents)
         if (e) (void)hipEventDestroy(e);
     if (c->spill) (void)hipFree(c->spill);
+    c->build_scratch.reset();
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
