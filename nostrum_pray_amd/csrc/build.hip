@@ -731,6 +731,9 @@ namespace {
 struct Grow {
     void *p = nullptr;
     size_t cap = 0;
+    Grow() = default;
+    Grow(const Grow &) = delete;  // owns p
+    Grow &operator=(const Grow &) = delete;
     ~Grow() {
         if (p) (void)hipFree(p);
     }
