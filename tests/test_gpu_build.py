@@ -153,3 +153,25 @@ def test_gpu_sah_build_equals_cpu_binned_tree(ctx, name):
         ctx.free_scene(sid)
     f = P.Scene.from_triangles(abc).build(2, max_depth=40, leaf_size=8, traversal_cost=2.0).flat()
     assert canonical(g) == canonical(f)
+
+
+def test_gpu_build_reuses_scratch(ctx):
+    """Builds on one context keep their scratch buffers: a small scene built
+    before and after a larger one (whose data stays in those buffers) gives
+    the same layout word for word, for both methods."""
+    small = np.ascontiguousarray(P.Scene.random(4, 6000).triangles()[0], np.float32).reshape(-1, 9)
+    large = np.ascontiguousarray(P.Scene.terrain(256, 5).triangles()[0], np.float32).reshape(-1, 9)
+    for method in (P.BUILD_MORTON, P.BUILD_SAH):
+        layouts = []
+        for abc in (small, large, small):
+            say(f"gpu build reuse method={method} n={abc.shape[0]}")
+            sid, _ = ctx.build_scene(abc, method=method, leaf_size=8)
+            try:
+                layouts.append(ctx.scene_layout(sid))
+            finally:
+                ctx.free_scene(sid)
+        a, b = layouts[0], layouts[2]
+        assert a["max_depth"] == b["max_depth"] and a["n_tris"] == small.shape[0]
+        assert np.array_equal(a["box"].view(np.uint32), b["box"].view(np.uint32))
+        assert np.array_equal(a["refs"], b["refs"])
+        assert first_diff(a["nodes"], b["nodes"]) is None, first_diff(a["nodes"], b["nodes"])
