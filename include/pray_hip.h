@@ -181,6 +181,26 @@ PRAY_API int pray_trace_counts_device(pray_ctx *ctx, int scene_id, const float *
                              float det_eps, const float *d_limit2, int32_t *d_hit_idx, float *d_hit_t,
                              uint32_t *d_counts, void *stream);
 
+/* pray_trace_device that allocates nothing.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory while they
+ * trace: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same arguments and results as pray_trace_device (the
+ * caller still passes rays and outputs; the scene is uploaded beforehand).
+ * The chunk counters and the stack spill area live in a static array of
+ * 64 MiB per GPU (device memory held from the library's load until the
+ * process ends) instead of the context's buffers: when the spill slots of
+ * the whole persistent grid do not fit, fewer waves trace the rays, with
+ * identical results.  Rays per call: at most 2^31-1 (PRAY_ERR_INVALID
+ * beyond); a tree whose spill does not fit one workgroup gives
+ * PRAY_ERR_UNSUPPORTED (not reachable at max_depth <= 64).  The array keeps
+ * the last call's contents and the counters are cleared by a kernel at the
+ * start of every call; all calls on one GPU share it, from every context, so
+ * calls that overlap on one GPU (other streams, other contexts or threads)
+ * race and must be serialised by the caller. */
+PRAY_API int pray_trace_device_noalloc(pray_ctx *ctx, int scene_id, const float *d_rays, uint64_t n, int mode,
+                                       float det_eps, const float *d_limit2, int32_t *d_hit_idx, float *d_hit_t,
+                                       void *stream);
+
 /* Incoherent ray stream: ray i has origin lo + span * U and a direction
  * uniform on the sphere (z = 1 - 2u, phi = 2 pi u'), normalised as the
  * reference normalises; U and u come from Philox4x32-10 keyed by `seed`

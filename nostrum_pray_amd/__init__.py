@@ -83,6 +83,7 @@ def libraries():
         "pray_trace": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp]),
         "pray_trace_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp]),
         "pray_trace_counts_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp, _vp]),
+        "pray_trace_device_noalloc": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp]),
         "pray_gen_rays_device": (C.c_int, [_vp, _u64, _vp, _u64, _vp, _vp]),
         "pray_gen_rays": (C.c_int, [_vp, _u64, _vp, _u64, _vp]),
         "pray_camera_rays": (C.c_int, [_vp, _vp, _u32, _u32, _vp]),
@@ -523,6 +524,13 @@ class Context:
     def trace_device(self, sid, d_rays, n, d_idx, d_t, mode=TRACE_CLOSEST, det_eps=REFERENCE_DET_EPS,
                      d_limit2=None, stream=None):
         self._check(self._lib.pray_trace_device(self._h, sid, d_rays, n, mode, det_eps, d_limit2, d_idx, d_t, stream))
+
+    def trace_device_noalloc(self, sid, d_rays, n, d_idx, d_t, mode=TRACE_CLOSEST, det_eps=REFERENCE_DET_EPS,
+                             d_limit2=None, stream=None):
+        """trace_device with the library's static scratch: allocates no
+        device memory (pray_trace_device_noalloc)."""
+        self._check(self._lib.pray_trace_device_noalloc(self._h, sid, d_rays, n, mode, det_eps, d_limit2, d_idx, d_t,
+                                                        stream))
 
     def trace_counts_device(self, sid, d_rays, n, d_idx, d_t, d_counts, mode=TRACE_CLOSEST,
                             det_eps=REFERENCE_DET_EPS, d_limit2=None, stream=None):

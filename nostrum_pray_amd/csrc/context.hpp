@@ -53,6 +53,7 @@ struct pray_ctx {
     std::vector<uint64_t> last_queues;  // per bounce: paths traced (pray_render_pt*)
     uint32_t *work = nullptr;   // ray counters of the persistent kernels
     uint4 *spill = nullptr;     // per-lane stack spill area
+    void *scratch = nullptr;    // device address of this device's static scratch (pray_trace_device_noalloc)
     size_t spill_bytes = 0;
     std::map<std::pair<const void *, int>, int> occupancy;  // (kernel, waves per block) -> resident blocks per CU
     std::map<int, prc::Tuning> tuned;        // scene-size bucket -> autotuned tuning

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,42 @@ using prr::Cam;
 __global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint64_t n) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+}  // namespace
+
+// Scratch of pray_trace_device_noalloc: the NCTR chunk counters of the
+// persistent trace kernels (first NOALLOC_COUNTER_BYTES), then their per-lane
+// stack spill area.  Every GPU the library runs on holds its own copy from
+// the code object's load until the process ends; all calls on one GPU share
+// it.  Host code never passes this name to a kernel or a copy: it is a
+// host-side stand-in, and the device address comes from hipGetSymbolAddress
+// (scratch_setup).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr size_t NOALLOC_COUNTER_BYTES = (size_t)prt::NCTR * prt::CTR_STRIDE * sizeof(uint32_t);
+static_assert(NOALLOC_COUNTER_BYTES % 256 == 0, "spill area stays 256-byte aligned");
+static __device__ __align__(256) uint4 scratch[NOALLOC_SCRATCH_BYTES / sizeof(uint4)];
+
+namespace {
+
+std::mutex scratch_mu;
+std::map<int, void *> scratch_addr;  // device -> address of that device's `scratch`
+
+// Looks the device address of `scratch` up once per device, with `device`
+// current (the caller has made it current).
+hipError_t scratch_setup(int device, void **out) {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    auto it = scratch_addr.find(device);
+    if (it != scratch_addr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return e != hipSuccess ? e : hipErrorInvalidSymbol;
+    scratch_addr[device] = p;
+    *out = p;
+    return hipSuccess;
 }
 
 }  // namespace
@@ -181,10 +218,21 @@ int ensure_spill(pray_ctx *c, size_t bytes) {
     return PRAY_OK;
 }
 
+// Scratch of a trace whose caller provides it instead of the context: the
+// NCTR chunk counters (NCTR * CTR_STRIDE words) and a stack spill area of
+// spill_bytes; the persistent grid is cut to the lanes whose spill slots fit.
+struct FixedScratch {
+    uint32_t *work = nullptr;
+    uint4 *spill = nullptr;
+    size_t spill_bytes = 0;
+};
+
 // Enqueues one trace over n rays (or *n_dev rays when n_dev is set, with n as
-// the launch bound).
-int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n, const uint32_t *n_dev, int mode,
-                  float eps, const float *lim2, int32_t *idx, float *t, uint32_t *counts, hipStream_t st) {
+// the launch bound), with the context's counters and spill area (grown on
+// demand), or with `fixed` scratch (nothing is allocated).
+static int enqueue_trace_ex(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n, const uint32_t *n_dev,
+                            int mode, float eps, const float *lim2, int32_t *idx, float *t, uint32_t *counts,
+                            hipStream_t st, const FixedScratch *fixed) {
     if (n == 0) return PRAY_OK;
     if (n > 0x7FFFFFFFull) return fail(c, PRAY_ERR_INVALID, "at most 2^31-1 rays per call");
     if (mode != PRAY_TRACE_CLOSEST && mode != PRAY_TRACE_ANY && mode != PRAY_TRACE_ANY_INCLUSIVE)
@@ -198,10 +246,28 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
     }
     const Tuning &tu = sc.tune;
     TraceKernel k = pick_kernel(mode != PRAY_TRACE_CLOSEST, tu.layout, tu.stack_lds, tu.leaf_ilp);
-    const Launch L = launch_shape(c, k, tu, n);
+    Launch L = launch_shape(c, k, tu, n);
     const uint32_t depth = std::max<uint32_t>(1, sc.d.max_depth);
-    int rc = ensure_spill(c, (size_t)L.lanes * depth * sizeof(uint4));
-    if (rc) return rc;
+    uint32_t *work = c->work;
+    uint4 *spill = c->spill;
+    if (fixed) {
+        // Fewer persistent waves when the spill slots of all of them do not
+        // fit: each takes more chunks, the results are the same.
+        const uint64_t lanes_fit = fixed->spill_bytes / ((uint64_t)depth * sizeof(uint4));
+        const uint64_t block_lanes = (uint64_t)L.threads;
+        if (lanes_fit < block_lanes)
+            return fail(c, PRAY_ERR_UNSUPPORTED, "the scratch holds no workgroup's stack spill at tree depth %u", depth);
+        if (L.lanes > lanes_fit) {
+            L.blocks = (unsigned)(lanes_fit / block_lanes);
+            L.lanes = (uint64_t)L.blocks * block_lanes;
+        }
+        work = fixed->work;
+        spill = fixed->spill;
+    } else {
+        int rc = ensure_spill(c, (size_t)L.lanes * depth * sizeof(uint4));
+        if (rc) return rc;
+        spill = c->spill;
+    }
     TraceArgs a{};
     a.rays = rays;
     a.n = (uint32_t)n;
@@ -212,11 +278,11 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
     a.out_idx = idx;
     a.out_t = t;
     a.counts = counts;
-    a.work = c->work;
-    a.spill = c->spill;
+    a.work = work;
+    a.spill = spill;
     a.spill_depth = depth;
     a.fetch_min = (uint32_t)tu.fetch_min;
-    hipLaunchKernelGGL(prt::k_reset_counters, dim3(1), dim3(64), 0, st, c->work);
+    hipLaunchKernelGGL(prt::k_reset_counters, dim3(1), dim3(64), 0, st, work);
     bool timed = false;
     if (c->kernel_timing) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -231,6 +297,11 @@ int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n
     }
     HIPC(c, hipGetLastError());
     return PRAY_OK;
+}
+
+int enqueue_trace(pray_ctx *c, const SceneRec &sc, const float *rays, uint64_t n, const uint32_t *n_dev, int mode,
+                  float eps, const float *lim2, int32_t *idx, float *t, uint32_t *counts, hipStream_t st) {
+    return enqueue_trace_ex(c, sc, rays, n, n_dev, mode, eps, lim2, idx, t, counts, st, nullptr);
 }
 
 int enqueue_trace_multi(pray_ctx *c, const prt::DevScene *table, const uint32_t *ray_scene, uint32_t max_depth,
@@ -364,6 +435,12 @@ int pray_init(int device, pray_ctx **out) {
         return PRAY_ERR_HIP;
     }
     c->num_cus = prop.multiProcessorCount;
+    if (scratch_setup(device, &c->scratch) != hipSuccess) {
+        (void)hipFree(c->work);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return PRAY_ERR_HIP;
+    }
     *out = c;
     return PRAY_OK;
 }
@@ -549,6 +626,23 @@ int pray_trace_counts_device(pray_ctx *c, int scene_id, const float *d_rays, uin
     HIPC(c, hipSetDevice(c->device));
     return enqueue_trace(c, *sc, d_rays, n, nullptr, mode, det_eps, d_limit2, d_hit_idx, d_hit_t, d_counts,
                          pick(c, stream));
+}
+
+int pray_trace_device_noalloc(pray_ctx *c, int scene_id, const float *d_rays, uint64_t n, int mode, float det_eps,
+                              const float *d_limit2, int32_t *d_hit_idx, float *d_hit_t, void *stream) {
+    if (!c) return PRAY_ERR_INVALID;
+    SceneRec *sc = find_scene(c, scene_id);
+    if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
+    if (n && (!d_rays || !d_hit_idx)) return fail(c, PRAY_ERR_INVALID, "null buffer");
+    if (!c->scratch) return fail(c, PRAY_ERR_HIP, "no static scratch on device %d", c->device);
+    HIPC(c, hipSetDevice(c->device));
+    // The device address looked up at pray_init, never the name `scratch`.
+    FixedScratch fx;
+    fx.work = static_cast<uint32_t *>(c->scratch);
+    fx.spill = reinterpret_cast<uint4 *>(static_cast<char *>(c->scratch) + NOALLOC_COUNTER_BYTES);
+    fx.spill_bytes = NOALLOC_SCRATCH_BYTES - NOALLOC_COUNTER_BYTES;
+    return enqueue_trace_ex(c, *sc, d_rays, n, nullptr, mode, det_eps, d_limit2, d_hit_idx, d_hit_t, nullptr,
+                            pick(c, stream), &fx);
 }
 
 int pray_trace(pray_ctx *c, int scene_id, const float *rays, uint64_t n, int mode, float det_eps,
