@@ -179,6 +179,10 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
     if (n_scenes == 0 || spp == 0) return PRAY_OK;
     if (!scene_ids || !cams16 || !widths || !heights || !seeds || !backgrounds)
         return fail(c, PRAY_ERR_INVALID, "null argument");
+    // As pray_render_pt: bounded bounce loop; the mean divides by the sample
+    // count as a float, which must stay exact.
+    if (max_depth > 64) return fail(c, PRAY_ERR_INVALID, "max_depth must be at most 64");
+    if (spp > (1u << 24)) return fail(c, PRAY_ERR_INVALID, "more than 2^24 samples");
     HIPC(c, hipSetDevice(c->device));
     std::vector<prt::DevScene> table(n_scenes);
     std::vector<prr::BatchScene> bs(n_scenes);

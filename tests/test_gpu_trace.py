@@ -36,18 +36,22 @@ def scene(name):
     return SCENES[name]
 
 
-# (layout, LDS ring entries): every traversal setting gives the same hits
-# (layout, LDS ring entries, waves per workgroup)
-SETTINGS = [(P.LAYOUT_PACKED, 8, 1), (P.LAYOUT_PACKED, 5, 2), (P.LAYOUT_INDEXED, 8, 4), (P.LAYOUT_INDEXED, 6, 1)]
+# Every traversal setting gives the same hits:
+# (layout, LDS ring entries, waves per workgroup, leaf ILP, refill threshold).
+# The first is the benchmark's autotuned setting on the 2048x2048 terrain
+# (packed, ring 8, one-wave workgroups, ILP 1, refill at 24 idle lanes).
+SETTINGS = [(P.LAYOUT_PACKED, 8, 1, 1, 24), (P.LAYOUT_PACKED, 5, 2, 2, 16), (P.LAYOUT_INDEXED, 8, 4, 1, 32),
+            (P.LAYOUT_INDEXED, 6, 1, 2, 16), (P.LAYOUT_PACKED, 6, 1, 2, 48)]
 
 
 def check_closest(ctx, name, rays, eps, bf_rows=0):
     s, t, flat = scene(name)
     sid = ctx.upload(t)
     try:
-        for layout, ring, wg in SETTINGS:
-            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg)
-            say(f"closest {name} layout={layout} stack_lds={ring} n={len(rays)} eps={eps}")
+        for layout, ring, wg, ilp, fm in SETTINGS:
+            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg, leaf_ilp=ilp, fetch_min=fm)
+            say(f"closest {name} layout={layout} stack_lds={ring} wg={wg} ilp={ilp} fetch_min={fm} n={len(rays)} "
+                f"eps={eps}")
             gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, eps)
             oi, ot = O.flat_trace(flat, rays, 0, eps)
             assert first_diff(gi, oi) is None, first_diff(gi, oi)
@@ -97,9 +101,9 @@ def test_any_hit(ctx, name, eps, mode):
     sid = ctx.upload(t)
     try:
         res = []
-        for layout, ring, wg in SETTINGS:
-            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg)
-            say(f"any {name} mode={mode} layout={layout} stack_lds={ring}")
+        for layout, ring, wg, ilp, fm in SETTINGS:
+            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg, leaf_ilp=ilp, fetch_min=fm)
+            say(f"any {name} mode={mode} layout={layout} stack_lds={ring} wg={wg} ilp={ilp} fetch_min={fm}")
             res.append(ctx.trace(sid, rays, mode, eps, limit2=lim))
     finally:
         ctx.free_scene(sid)
