@@ -5,7 +5,8 @@ path tracer against the plain path tracer on the 100k-triangle Cornell room.
 
 A step is one closest-hit launch over the whole ray stream (rays resident in
 HBM).  Prints ONE JSON line with the metric, a roofline object for the
-traversal kernel, the radiance-cache comparison and a CPU baseline (the
+traversal kernel, the radiance-cache comparison, the batched multi-scene
+render (64 scenes in one wavefront, key "batch") and a CPU baseline (the
 restated reference CPU path timed on this host).
 Run:  python bench.py [--steps K] [--warmup W]
 
@@ -17,12 +18,22 @@ comparison alone), --workload build (the KD-tree builds on the GPU).
 import argparse
 import json
 import os
-import platform
-import subprocess
-import sys
-import time
 
-import numpy as np
+# OpenMP placement of the CPU baseline (one thread per core, packed), set
+# before anything loads an OpenMP runtime: libgomp reads these once, when the
+# host library that builds the trees is loaded.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+# With a binding set, libgomp pins the loading thread to its first place, so
+# the CPUs this process may use are counted before that happens.
+USABLE_CPUS = len(os.sched_getaffinity(0))
+
+import platform  # noqa: E402
+import subprocess  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -35,15 +46,37 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_model():
+def cpu_info():
+    """The host CPU as lscpu describes it, with the CPUs this process may run
+    on: {model, vendor, sockets, cores_per_socket, threads_per_core,
+    physical_cores, logical_cpus, usable_cpus}."""
+    info = {}
+    keys = {"Model name": "model", "Vendor ID": "vendor", "Socket(s)": "sockets",
+            "Core(s) per socket": "cores_per_socket", "Thread(s) per core": "threads_per_core",
+            "CPU(s)": "logical_cpus", "CPU family": "family", "Model": "model_number"}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
         for line in out.splitlines():
-            if line.startswith("Model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in keys and keys[k] not in info:  # first occurrence: the whole machine's
+                info[keys[k]] = v.strip()
     except Exception:
         pass
-    return platform.processor() or "unknown"
+    info.setdefault("model", platform.processor() or "unknown")
+    for k in ("sockets", "cores_per_socket", "threads_per_core", "logical_cpus"):
+        try:
+            info[k] = int(info[k])
+        except (KeyError, ValueError):
+            info.pop(k, None)
+    if "sockets" in info and "cores_per_socket" in info:
+        info["physical_cores"] = info["sockets"] * info["cores_per_socket"]
+    info["usable_cpus"] = USABLE_CPUS
+    return info
+
+
+def cpu_model():
+    return cpu_info()["model"]
 
 
 def main():
@@ -85,6 +118,7 @@ def main():
                          "io: flat tree files, progressive path tracing and the CLI; cache: the radiance-cache "
                          "equal-RMSE measurement alone; build: the KD-tree builds on the GPU")
     ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg of the default run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -288,6 +322,15 @@ def main():
         sid = None
         rc_out = radiance_cache_leg(P, ctx, args)
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(P, ctx, args)
+        batch.pop("autotuned_buckets", None)
+        # per scene: [triangles, image side, rays, Mrays/s rendered alone, image equal to the batch's]
+        batch["per_scene"] = [[p["triangles"], p["image"], p["rays"], p["Mrays_per_s_alone"], p["equal_to_batch"]]
+                              for p in batch["per_scene"]]
+        batch["per_scene_fields"] = ["triangles", "image_side", "rays", "Mrays_per_s_alone", "equal_to_batch"]
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, args, box, closest_idx)
@@ -336,6 +379,7 @@ def main():
         "reference_parameter_tree": ref_tree,
         "other_trees": others,
         "radiance_cache": rc_out,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -347,25 +391,28 @@ def main():
     ctx.close()
 
 
-def batch_workload(P, args):
+def batch_leg(P, ctx, args):
     """Random icosphere/box/terrain scenes with triangle counts log-uniform in
-    [1k, 2M] and images of 128^2 .. 1024^2, 16 spp, depth 6: autotuned per
-    size bucket, path traced together in one wavefront and each alone (the
-    two must agree bit for bit); prints one JSON line with the aggregate and
-    per-scene Mrays/s."""
+    [1k, 2M] and images of 128^2 .. 1024^2, 16 spp, depth 6, trees built on the
+    GPU: autotuned per size bucket, path traced together in one wavefront and
+    each alone (the two must agree bit for bit).  Returns the aggregate and
+    per-scene figures."""
     rng = np.random.default_rng(args.seed)
     n = args.batch_scenes
     tris = np.exp(rng.uniform(np.log(1e3), np.log(2e6), n)).astype(np.int64)
     sides = rng.choice([128, 256, 512, 1024], n)
-    ctx = P.Context(0)
     t0 = time.time()
-    scenes, sids = [], []
+    scenes, sids, build_ms = [], [], []
     for k in range(n):
         s = P.Scene.random(int(args.seed * 1000 + k), int(tris[k]))
-        sids.append(gpu_sah_scene(P, ctx, s, args))
+        sid, bms = gpu_sah_scene(P, ctx, s, args)
+        sids.append(sid)
+        build_ms.append(bms)
         scenes.append(s)
-    log(f"batch: {n} scenes ({int(sum(s.num_tris for s in scenes))} triangles) built and uploaded in "
-        f"{time.time() - t0:.1f}s")
+    build_s = time.time() - t0
+    n_tris = int(sum(s.num_tris for s in scenes))
+    log(f"batch: {n} scenes ({n_tris} triangles) generated, built on the GPU and uploaded in {build_s:.1f}s "
+        f"(device build time {sum(build_ms):.1f} ms)")
     tuned = {}
     for k in range(n):
         tun = ctx.autotune(sids[k])
@@ -406,20 +453,33 @@ def batch_workload(P, args):
                     "rays": kr, "Mrays_per_s_alone": round(kr / kms / 1e3, 2), "equal_to_batch": ok})
     for sid in sids:
         ctx.free_scene(sid)
-    ctx.close()
-    out = {
-        "metric": "Mrays/s path tracing, batched scenes (16 spp, depth 6)",
-        "value": round(rays / ms / 1e3, 2), "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{n} random icosphere/box/terrain scenes, triangles log-uniform in [1k, 2M], images "
-                               f"uniform over 128^2..1024^2, {spp} spp, depth {depth}, roulette from bounce {rr}, "
-                               f"one wavefront for all scenes, SAH KD trees built on the GPU", "scenes": n},
-        "batch_ms": round(ms, 2), "rays": rays, "sum_of_single_scene_ms": round(single_ms, 2),
+    return {
+        "workload": f"{n} random icosphere/box/terrain scenes, triangles log-uniform in [1k, 2M], images uniform "
+                    f"over 128^2..1024^2, {spp} spp, depth {depth}, roulette from bounce {rr}, one wavefront for all "
+                    f"scenes, SAH KD trees built on the GPU",
+        "scenes": n, "triangles": n_tris, "spp": spp, "max_depth": depth,
+        "build_wall_s": round(build_s, 2), "gpu_build_ms_total": round(float(sum(build_ms)), 2),
+        "batch_ms": round(ms, 2), "rays": rays, "Mrays_per_s": round(rays / ms / 1e3, 2),
+        "sum_of_single_scene_ms": round(single_ms, 2),
         "batch_speedup_over_single_scene_renders": round(single_ms / ms, 3),
         "all_scenes_equal_to_single_renders": same,
         "autotuned_buckets": {str(k): v for k, v in sorted(tuned.items())},
         "batch_tuning": batch_tuning,
         "per_scene": per,
+    }
+
+
+def batch_workload(P, args):
+    """The batched multi-scene render alone, as one JSON line."""
+    ctx = P.Context(0)
+    b = batch_leg(P, ctx, args)
+    ctx.close()
+    out = {
+        "metric": "Mrays/s path tracing, batched scenes (16 spp, depth 6)",
+        "value": b["Mrays_per_s"], "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": b["workload"], "scenes": b["scenes"]},
+        **{k: v for k, v in b.items() if k not in ("workload",)},
     }
     print(json.dumps(out), flush=True)
 
@@ -684,11 +744,10 @@ def build_tree(P, ctx, scene, args, which):
 
 def gpu_sah_scene(P, ctx, scene, args):
     """Uploads a scene with its SAH KD tree built on the GPU (with shading
-    data)."""
+    data): (scene id, device build milliseconds)."""
     abc, col, emit = scene.triangles()
-    sid, _ = ctx.build_scene(abc, col, emit, method=P.BUILD_SAH, leaf_size=args.leaf_size, max_depth=args.max_depth,
-                             traversal_cost=args.traversal_cost)
-    return sid
+    return ctx.build_scene(abc, col, emit, method=P.BUILD_SAH, leaf_size=args.leaf_size, max_depth=args.max_depth,
+                           traversal_cost=args.traversal_cost)
 
 
 def P_camera(P, scene, side):
@@ -786,16 +845,17 @@ def cpu_baseline(scene, args, box, gpu_idx):
     first 4096 rays, and the reference's own compiled traversal on 2^20."""
     import oracle as O
 
-    threads = os.cpu_count() or 1
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
+    cpu = cpu_info()
+    threads = cpu["usable_cpus"]  # one OpenMP thread per CPU this process may run on
     tree = scene.build(0)
     ex = tree.export()
     fl = tree.flat()
     n = min(args.cpu_rays, args.rays)
     rays = O.gen_rays(args.seed, box, n)
     idx, t, sec = O.ref_closest_avx(ex, fl["a"], fl["ab"], fl["ac"], rays, args.det_eps, threads)
-    log(f"cpu baseline: {n} rays in {sec:.2f}s on {threads} threads ({cpu_model()})")
+    log(f"cpu baseline: {n} rays in {sec:.2f}s on {threads} threads ({cpu['model']}, "
+        f"{cpu.get('physical_cores', '?')} physical cores, {cpu.get('logical_cpus', '?')} logical CPUs, "
+        f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')})")
     nb = min(4096, n)
     t0 = time.perf_counter()
     bidx, _ = O.bf_closest(fl["a"], fl["ab"], fl["ac"], rays[:nb], args.det_eps)
@@ -829,7 +889,9 @@ def cpu_baseline(scene, args, box, gpu_idx):
         "kind": "port",
         "sample": f"first {n} rays of the same stream, same det_eps, restated reference AVX packet traversal over "
                   f"the reference's median KD tree (identical to the reference build), {threads} OpenMP threads on "
-                  f"{cpu_model()}",
+                  f"{cpu['model']}",
+        "cpu": {**cpu, "threads_used": threads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+                "omp_places": os.environ.get("OMP_PLACES")},
         "reference_code": ref_code,
         "brute_force": {"rays": nb, "seconds": round(bsec, 2), "Mrays_per_s": round(nb / bsec / 1e6, 6),
                         "same_hit_index_as_gpu": same,
