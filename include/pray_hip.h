@@ -136,9 +136,14 @@ PRAY_API int pray_scene_layout(pray_ctx *ctx, int scene_id, uint64_t *sizes, uin
  *                  the persistent grid launches: the workgroup count
  *   [5] wg_waves   waves per workgroup (1, 2 or 4): the workgroup size; the
  *                  waves of a workgroup trace independently
+ *   [6] leaf_batch 0: every iteration steps the wave's node lanes and its
+ *                  leaf lanes; T (1..64): an iteration steps only the leaf
+ *                  lanes once at least T lanes wait at a leaf (or no lane
+ *                  has node work), only the node lanes otherwise, so the
+ *                  wave issues one of the two code paths per iteration
  * pray_set_tuning: -1 keeps a value.  Calls of one context must not overlap
  * (traces share its ray counter and stack spill area). */
-#define PRAY_TUNING_FIELDS 6
+#define PRAY_TUNING_FIELDS 7
 PRAY_API int pray_set_tuning(pray_ctx *ctx, int scene_id, const int *tuning);
 PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *tuning);
 

@@ -74,9 +74,9 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
         HIPC(c, hipMalloc(&tt.p, (size_t)n_rays * 4));
         int rc = sample_rays(c, *sc, n_rays, 0x5EEDull + (uint64_t)scene_id, (float *)rays.p);
         if (rc) return rc;
-        // Grid over stack depth x leaf ILP x queue chunk at full occupancy,
-        // then the workgroup size and the occupancy of the winner (packed
-        // layout).
+        // Grid over stack depth x leaf ILP x refill threshold at full
+        // occupancy; then leaf batching (with two refill thresholds), the
+        // workgroup size and the occupancy of the winner (packed layout).
         Tuning best;
         float best_ms = 1e30f;
         auto consider = [&](Tuning cand) {
@@ -91,6 +91,15 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
                     t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm;
                     consider(t);
                 }
+        for (int ilp : {1, 2}) {
+            const Tuning base = best;
+            for (int lb : {16, 24, 32})
+                for (int fm : {16, 24}) {
+                    Tuning t = base;
+                    t.leaf_ilp = ilp, t.leaf_batch = lb, t.fetch_min = fm;
+                    consider(t);
+                }
+        }
         const Tuning grid_best = best;
         for (int wg : {2, 4}) {
             Tuning t = grid_best;
@@ -150,6 +159,15 @@ int pray_autotune_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, 
                     t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm;
                     if (!rc) rc = consider(t);
                 }
+        for (int ilp : {1, 2}) {
+            const Tuning base = best;
+            for (int lb : {16, 24, 32})
+                for (int fm : {16, 24}) {
+                    Tuning t = base;
+                    t.leaf_ilp = ilp, t.leaf_batch = lb, t.fetch_min = fm;
+                    if (!rc) rc = consider(t);
+                }
+        }
         const Tuning grid_best = best;
         for (int wg : {2, 4}) {
             Tuning t = grid_best;

@@ -21,6 +21,7 @@ struct Tuning {
     int fetch_min = 16;
     int occupancy = 100;  // percent of the resident persistent waves launched
     int wg_waves = 1;     // waves per workgroup (1, 2 or 4)
+    int leaf_batch = 0;   // 0: node and leaf steps in every iteration; T: leaf steps once T lanes wait at leaves
 };
 
 struct SceneRec {
@@ -91,7 +92,7 @@ SceneRec *find_scene(pray_ctx *c, int id);
 int device_alloc(pray_ctx *c, SceneRec &sc, void **p, size_t bytes);
 void free_scene_rec(SceneRec &sc);
 int ensure_spill(pray_ctx *c, size_t bytes);
-// {layout, stack_lds, leaf_ilp, fetch_min, occupancy, wg_waves}
+// {layout, stack_lds, leaf_ilp, fetch_min, occupancy, wg_waves, leaf_batch}
 void tuning_values(const Tuning &t, int *v);
 // Enqueues one trace over n rays (or *n_dev rays when n_dev is set, with n as
 // the launch bound).

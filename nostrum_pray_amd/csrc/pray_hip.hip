@@ -282,6 +282,7 @@ static int enqueue_trace_ex(pray_ctx *c, const SceneRec &sc, const float *rays, 
     a.spill = spill;
     a.spill_depth = depth;
     a.fetch_min = (uint32_t)tu.fetch_min;
+    a.leaf_batch = (uint32_t)tu.leaf_batch;
     hipLaunchKernelGGL(prt::k_reset_counters, dim3(1), dim3(64), 0, st, work);
     bool timed = false;
     if (c->kernel_timing) {
@@ -325,6 +326,7 @@ int enqueue_trace_multi(pray_ctx *c, const prt::DevScene *table, const uint32_t 
     a.spill = c->spill;
     a.spill_depth = depth;
     a.fetch_min = (uint32_t)tu.fetch_min;
+    a.leaf_batch = (uint32_t)tu.leaf_batch;
     a.table = table;
     a.ray_scene = ray_scene;
     hipLaunchKernelGGL(prt::k_reset_counters, dim3(1), dim3(64), 0, st, c->work);
@@ -406,7 +408,7 @@ int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n) {
 
 void tuning_values(const Tuning &t, int *v) {
     v[0] = t.layout, v[1] = t.stack_lds, v[2] = t.leaf_ilp, v[3] = t.fetch_min, v[4] = t.occupancy;
-    v[5] = t.wg_waves;
+    v[5] = t.wg_waves, v[6] = t.leaf_batch;
 }
 
 }  // namespace prc
@@ -574,7 +576,8 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
     SceneRec *sc = find_scene(c, scene_id);
     if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
     if (!v) return fail(c, PRAY_ERR_INVALID, "null tuning");
-    const int layout = v[0], stack_lds = v[1], leaf_ilp = v[2], fetch_min = v[3], occupancy = v[4], wg_waves = v[5];
+    const int layout = v[0], stack_lds = v[1], leaf_ilp = v[2], fetch_min = v[3], occupancy = v[4], wg_waves = v[5],
+              leaf_batch = v[6];
     if (layout >= 0 && layout != PRAY_LAYOUT_INDEXED && layout != PRAY_LAYOUT_PACKED)
         return fail(c, PRAY_ERR_INVALID, "unknown layout %d", layout);
     if (stack_lds >= 0 && stack_lds != 4 && stack_lds != 5 && stack_lds != 6 && stack_lds != 8 && stack_lds != 16)
@@ -586,6 +589,7 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
         return fail(c, PRAY_ERR_INVALID, "occupancy must be in 1..100");
     if (wg_waves >= 0 && wg_waves != 1 && wg_waves != 2 && wg_waves != 4)
         return fail(c, PRAY_ERR_INVALID, "wg_waves must be 1, 2 or 4");
+    if (leaf_batch > WAVE) return fail(c, PRAY_ERR_INVALID, "leaf_batch must be in 0..64");
     Tuning &t = sc->tune;
     if (layout >= 0) t.layout = layout;
     if (stack_lds >= 0) t.stack_lds = stack_lds;
@@ -593,6 +597,7 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
     if (fetch_min >= 0) t.fetch_min = fetch_min;
     if (occupancy >= 0) t.occupancy = occupancy;
     if (wg_waves >= 0) t.wg_waves = wg_waves;
+    if (leaf_batch >= 0) t.leaf_batch = leaf_batch;
     return PRAY_OK;
 }
 

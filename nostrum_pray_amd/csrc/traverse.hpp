@@ -70,6 +70,9 @@ struct TraceArgs {
     uint4 *spill;             // per lane slot: spill_depth entries
     uint32_t spill_depth;
     uint32_t fetch_min;       // refill when at least this many lanes are idle
+    uint32_t leaf_batch;      // 0: node and leaf steps in every iteration; T > 0: an iteration takes
+                              // leaf steps when at least T lanes wait at a leaf (or no lane needs a
+                              // node step), node steps otherwise
     const DevScene *table;    // multi-scene launches: scene of ray i is
     const uint32_t *ray_scene;  // table[ray_scene[i]]
 };
@@ -246,9 +249,19 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
             pool_end = (uint32_t)__builtin_amdgcn_readfirstlane((int)pool_end);
         }
 
+        // Which steps this iteration takes (wave-uniform): both, or with
+        // leaf batching the leaf steps once enough lanes wait at a leaf.
+        bool run_node = true, run_leaf = true;
+        if (A.leaf_batch) {
+            const uint64_t at_leaf = __ballot(state == ST_LEAF);
+            const uint64_t at_node = __ballot(state == ST_NODE);
+            run_leaf = at_leaf != 0ull && ((uint32_t)__popcll(at_leaf) >= A.leaf_batch || at_node == 0ull);
+            run_node = !run_leaf;
+        }
         bool need_pop = false;
+        const uint32_t st_in = state;
         // ---------------------------------------------------- node step --
-        if (state == ST_NODE) {
+        if (run_node && state == ST_NODE) {
             const uint32_t axis = w & 3u;
             const uint4 pr = pairs[w >> 3];  // pair index = (w >> 2) >> 1
             cn += 2;
@@ -294,9 +307,10 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
             }
         }
         // ---------------------------------------------------- leaf step --
-        // (also for lanes that reached their leaf in this iteration's node
-        // step: the leaf branch runs anyway while any lane is in a leaf)
-        if (!need_pop && state == ST_LEAF) {
+        // (without leaf batching also for lanes that reached their leaf in
+        // this iteration's node step: the leaf branch runs anyway while any
+        // lane is in a leaf)
+        if (run_leaf && !need_pop && state == ST_LEAF && (A.leaf_batch == 0 || st_in == ST_LEAF)) {
             uint32_t ref[ILP];
             float4 ta[ILP], tb[ILP], tc[ILP];
 #pragma unroll

@@ -66,10 +66,13 @@ def test_autotune_never_changes_hits(ctx):
         tun = ctx.autotune(sid, n_rays=1 << 18, force=True)
         assert tun["stack_lds"] in (4, 6, 8, 16) and tun["leaf_ilp"] in (1, 2) and 1 <= tun["occupancy"] <= 100
         assert tun["fetch_min"] in (16, 24, 32, 48) and tun["wg_waves"] in (1, 2, 4)
+        assert tun["leaf_batch"] in (0, 16, 24, 32)
         assert ctx.get_tuning(sid) == tun
         settings = [dict(), dict(stack_lds=4), dict(stack_lds=16, leaf_ilp=1), dict(fetch_min=16, occupancy=50),
                     dict(fetch_min=64, occupancy=75), dict(layout=P.LAYOUT_INDEXED), dict(stack_lds=6),
-                    dict(stack_lds=5, occupancy=10), dict(wg_waves=4), dict(wg_waves=2, stack_lds=16, leaf_ilp=2)]
+                    dict(stack_lds=5, occupancy=10), dict(wg_waves=4), dict(wg_waves=2, stack_lds=16, leaf_ilp=2),
+                    dict(layout=P.LAYOUT_PACKED, leaf_batch=16, stack_lds=8), dict(leaf_batch=48, occupancy=10),
+                    dict(leaf_batch=1, wg_waves=4, fetch_min=1), dict(leaf_batch=0)]
         for st in settings:
             say(f"autotune: trace with {st}")
             ctx.set_tuning(sid, **st)
@@ -77,6 +80,8 @@ def test_autotune_never_changes_hits(ctx):
             assert np.array_equal(idx, ref_idx) and np.array_equal(tt.view(np.uint32), ref_t.view(np.uint32)), st
         with pytest.raises(P.PrayError):
             ctx.set_tuning(sid, wg_waves=3)
+        with pytest.raises(P.PrayError):
+            ctx.set_tuning(sid, leaf_batch=65)
         # a second scene of the same size bucket takes the bucket's setting
         s2 = P.Scene.random(22, 45000)
         sid2 = ctx.upload(s2.build(1))

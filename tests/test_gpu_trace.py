@@ -37,21 +37,26 @@ def scene(name):
 
 
 # Every traversal setting gives the same hits:
-# (layout, LDS ring entries, waves per workgroup, leaf ILP, refill threshold).
-# The first is the benchmark's autotuned setting on the 2048x2048 terrain
-# (packed, ring 8, one-wave workgroups, ILP 1, refill at 24 idle lanes).
-SETTINGS = [(P.LAYOUT_PACKED, 8, 1, 1, 24), (P.LAYOUT_PACKED, 5, 2, 2, 16), (P.LAYOUT_INDEXED, 8, 4, 1, 32),
-            (P.LAYOUT_INDEXED, 6, 1, 2, 16), (P.LAYOUT_PACKED, 6, 1, 2, 48)]
+# (layout, LDS ring entries, waves per workgroup, leaf ILP, refill threshold,
+# leaf batch).  The first is the setting the autotuner picked for the
+# benchmark's 2048x2048 terrain before leaf batching (packed, ring 8,
+# one-wave workgroups, ILP 1, refill at 24 idle lanes), the second the one it
+# picks with it (ILP 2, refill at 16, leaf steps once 16 lanes wait).
+SETTINGS = [(P.LAYOUT_PACKED, 8, 1, 1, 24, 0), (P.LAYOUT_PACKED, 8, 1, 2, 16, 16),
+            (P.LAYOUT_PACKED, 5, 2, 2, 16, 0), (P.LAYOUT_INDEXED, 8, 4, 1, 32, 0),
+            (P.LAYOUT_INDEXED, 6, 1, 2, 16, 24), (P.LAYOUT_PACKED, 6, 1, 2, 48, 0),
+            (P.LAYOUT_PACKED, 4, 2, 2, 16, 32), (P.LAYOUT_PACKED, 16, 4, 1, 48, 64),
+            (P.LAYOUT_PACKED, 5, 1, 1, 1, 1)]
 
 
 def check_closest(ctx, name, rays, eps, bf_rows=0):
     s, t, flat = scene(name)
     sid = ctx.upload(t)
     try:
-        for layout, ring, wg, ilp, fm in SETTINGS:
-            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg, leaf_ilp=ilp, fetch_min=fm)
-            say(f"closest {name} layout={layout} stack_lds={ring} wg={wg} ilp={ilp} fetch_min={fm} n={len(rays)} "
-                f"eps={eps}")
+        for layout, ring, wg, ilp, fm, lb in SETTINGS:
+            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg, leaf_ilp=ilp, fetch_min=fm, leaf_batch=lb)
+            say(f"closest {name} layout={layout} stack_lds={ring} wg={wg} ilp={ilp} fetch_min={fm} leaf_batch={lb} "
+                f"n={len(rays)} eps={eps}")
             gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, eps)
             oi, ot = O.flat_trace(flat, rays, 0, eps)
             assert first_diff(gi, oi) is None, first_diff(gi, oi)
@@ -101,9 +106,10 @@ def test_any_hit(ctx, name, eps, mode):
     sid = ctx.upload(t)
     try:
         res = []
-        for layout, ring, wg, ilp, fm in SETTINGS:
-            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg, leaf_ilp=ilp, fetch_min=fm)
-            say(f"any {name} mode={mode} layout={layout} stack_lds={ring} wg={wg} ilp={ilp} fetch_min={fm}")
+        for layout, ring, wg, ilp, fm, lb in SETTINGS:
+            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg, leaf_ilp=ilp, fetch_min=fm, leaf_batch=lb)
+            say(f"any {name} mode={mode} layout={layout} stack_lds={ring} wg={wg} ilp={ilp} fetch_min={fm} "
+                f"leaf_batch={lb}")
             res.append(ctx.trace(sid, rays, mode, eps, limit2=lim))
     finally:
         ctx.free_scene(sid)
@@ -120,13 +126,16 @@ def test_any_hit(ctx, name, eps, mode):
         assert (tt * np.float32(tt) < lim[k]) if mode == P.TRACE_ANY else (np.float32(tt) * np.float32(tt) <= lim[k])
 
 
-def test_work_counts_match_cpu_twin(ctx):
-    """Per-ray node and triangle counts equal the CPU twin's."""
+@pytest.mark.parametrize("lb", [0, 16, 40])
+def test_work_counts_match_cpu_twin(ctx, lb):
+    """Per-ray node and triangle counts equal the CPU twin's, with and
+    without leaf batching."""
     _, t, flat = scene("terrain256-sah")
     rays = O.gen_rays(3, P.scene_box_expanded(flat["box"]), 1 << 14)
     sid = ctx.upload(t)
     try:
-        say("counts terrain256-sah")
+        ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, leaf_batch=lb)
+        say(f"counts terrain256-sah leaf_batch={lb}")
         n = rays.shape[0]
         d_rays = ctx.to_device(rays)
         d_idx = ctx.alloc(n, np.int32)
@@ -143,8 +152,8 @@ def test_work_counts_match_cpu_twin(ctx):
     assert np.array_equal(gc, oc)
 
 
-@pytest.mark.parametrize("ring", [5, 16])
-def test_launch_sizes(ctx, ring):
+@pytest.mark.parametrize("ring,lb", [(5, 0), (16, 0), (5, 32), (16, 8)])
+def test_launch_sizes(ctx, ring, lb):
     """Traces over ray counts around the wave, the 64-ray chunks and the 64
     chunk counters (partial last chunks, fewer waves than counters), one after
     another through the same counters and spill area, including a ray without
@@ -154,7 +163,8 @@ def test_launch_sizes(ctx, ring):
     _, t, flat = scene("spheres-sah")
     sid = ctx.upload(t)
     try:
-        ctx.set_tuning(sid, stack_lds=ring, fetch_min=1 if ring == 5 else 64, wg_waves=1 if ring == 5 else 4)
+        ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, stack_lds=ring, fetch_min=1 if ring == 5 else 64,
+                       wg_waves=1 if ring == 5 else 4, leaf_batch=lb)
         base = O.gen_rays(12, P.scene_box_expanded(flat["box"]), 200_000)
         base[5, 3:] = 0.0
         base[7, :3] = [1e30, -1e30, 0.0]
@@ -163,7 +173,7 @@ def test_launch_sizes(ctx, ring):
             if n == 200_000:
                 ctx.set_tuning(sid, occupancy=1)
             rays = np.ascontiguousarray(base[:n])
-            say(f"launch ring={ring} n={n}")
+            say(f"launch ring={ring} leaf_batch={lb} n={n}")
             gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, 1e-4)
             oi, ot = O.flat_trace(flat, rays, 0, 1e-4)
             k = keep[:n]

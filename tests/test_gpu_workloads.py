@@ -20,7 +20,8 @@ from conftest import first_diff, say
 
 pytestmark = pytest.mark.gpu
 
-BENCH_TUNING = dict(layout=P.LAYOUT_PACKED, stack_lds=8, leaf_ilp=1, fetch_min=24, occupancy=100, wg_waves=1)
+BENCH_TUNING = dict(layout=P.LAYOUT_PACKED, stack_lds=8, leaf_ilp=2, fetch_min=16, occupancy=100, wg_waves=1,
+                    leaf_batch=16)
 
 
 def test_terrain2048_full_stream_equals_oracle(ctx):
@@ -44,8 +45,9 @@ def test_terrain2048_full_stream_equals_oracle(ctx):
         assert np.array_equal(rays[:1 << 16].view(np.uint32), O.gen_rays(2024, box, 1 << 16).view(np.uint32))
         sample = np.concatenate([np.arange(1 << 16), np.arange(1 << 16, n, 251)])
         first = None
-        for tun in (BENCH_TUNING, dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1),
-                    dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=2, fetch_min=32, wg_waves=2)):
+        for tun in (BENCH_TUNING, dict(BENCH_TUNING, leaf_ilp=1, fetch_min=24, leaf_batch=0),
+                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0),
+                    dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=2, fetch_min=32, wg_waves=2, leaf_batch=32)):
             ctx.set_tuning(sid, **tun)
             say(f"terrain 2048: closest hit, 2^24 rays, {tun}")
             ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, P.TRACE_CLOSEST, 1e-12)
@@ -96,8 +98,9 @@ def test_cornell100k_depth8_equals_oracle(ctx, tree_kind):
     sid = ctx.upload(t)
     try:
         res = []
-        for tun in (dict(layout=P.LAYOUT_PACKED, stack_lds=4, leaf_ilp=2, fetch_min=16, wg_waves=1),
-                    dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=1, fetch_min=32, wg_waves=2)):
+        for tun in (dict(layout=P.LAYOUT_PACKED, stack_lds=4, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0),
+                    dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=1, fetch_min=32, wg_waves=2, leaf_batch=0),
+                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=16)):
             ctx.set_tuning(sid, **tun)
             say(f"cornell 100k {tree_kind} {w}x{h} {spp} spp depth {depth} rr {rr} {tun}")
             hdr, rgb = ctx.render_pt(sid, cam, w, h, spp, depth, rr, seed, bg, 1e-9)
@@ -157,8 +160,8 @@ def test_batch_size_buckets_equal_oracle(ctx):
         runs = []
         # The batch takes the setting of scene 3 (the largest image of the
         # bucket holding most pixels).
-        for tun in (dict(layout=P.LAYOUT_PACKED, stack_lds=8, leaf_ilp=2, fetch_min=24, wg_waves=1),
-                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=1, fetch_min=16, wg_waves=2)):
+        for tun in (dict(layout=P.LAYOUT_PACKED, stack_lds=8, leaf_ilp=2, fetch_min=24, wg_waves=1, leaf_batch=0),
+                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=1, fetch_min=16, wg_waves=2, leaf_batch=16)):
             ctx.set_tuning(sids[3], **tun)
             say(f"batch of {len(spec)} scenes, {spp} spp, depth {depth}, {tun}")
             imgs, ms = ctx.render_pt_batch(sids, cams, sizes, spp, depth, rr, seeds, bgs, 1e-9)
