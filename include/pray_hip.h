@@ -141,9 +141,16 @@ PRAY_API int pray_scene_layout(pray_ctx *ctx, int scene_id, uint64_t *sizes, uin
  *                  lanes once at least T lanes wait at a leaf (or no lane
  *                  has node work), only the node lanes otherwise, so the
  *                  wave issues one of the two code paths per iteration
+ *   [7] node_levels 1: a node step takes one tree level (one 16-byte
+ *                  children pair); 2: it takes two levels from one 32-byte
+ *                  record of the node and its children (one cache line per
+ *                  two levels; packed layout, and only for trees whose
+ *                  children pairs have equal clips -- the SAH trees -- whose
+ *                  records the upload or GPU build makes; PRAY_ERR_UNSUPPORTED
+ *                  for other trees)
  * pray_set_tuning: -1 keeps a value.  Calls of one context must not overlap
  * (traces share its ray counter and stack spill area). */
-#define PRAY_TUNING_FIELDS 7
+#define PRAY_TUNING_FIELDS 8
 PRAY_API int pray_set_tuning(pray_ctx *ctx, int scene_id, const int *tuning);
 PRAY_API int pray_get_tuning(pray_ctx *ctx, int scene_id, int *tuning);
 

@@ -24,7 +24,8 @@ PRAY_OK = 0
 TRACE_CLOSEST, TRACE_ANY, TRACE_ANY_INCLUSIVE = 0, 1, 2
 LAYOUT_INDEXED, LAYOUT_PACKED = 0, 1
 BUILD_MORTON, BUILD_SAH = 0, 1
-TUNING_FIELDS = ("layout", "stack_lds", "leaf_ilp", "fetch_min", "occupancy", "wg_waves", "leaf_batch")
+TUNING_FIELDS = ("layout", "stack_lds", "leaf_ilp", "fetch_min", "occupancy", "wg_waves", "leaf_batch",
+                 "node_levels")
 REFERENCE_DET_EPS = 1e-4  # geometry.h:12
 
 ERRORS = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "NOSCENE", -5: "UNSUPPORTED", -6: "NODEVICE"}
@@ -449,8 +450,8 @@ class Context:
         return float(ms.value)
 
     def set_tuning(self, sid, layout=-1, stack_lds=-1, leaf_ilp=-1, fetch_min=-1, occupancy=-1, wg_waves=-1,
-                   leaf_batch=-1):
-        v = np.array([layout, stack_lds, leaf_ilp, fetch_min, occupancy, wg_waves, leaf_batch], np.int32)
+                   leaf_batch=-1, node_levels=-1):
+        v = np.array([layout, stack_lds, leaf_ilp, fetch_min, occupancy, wg_waves, leaf_batch, node_levels], np.int32)
         self._check(self._lib.pray_set_tuning(self._h, sid, _p(v)))
 
     def get_tuning(self, sid):

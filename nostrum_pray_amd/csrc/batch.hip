@@ -74,9 +74,10 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
         HIPC(c, hipMalloc(&tt.p, (size_t)n_rays * 4));
         int rc = sample_rays(c, *sc, n_rays, 0x5EEDull + (uint64_t)scene_id, (float *)rays.p);
         if (rc) return rc;
-        // Grid over stack depth x leaf ILP x refill threshold at full
-        // occupancy; then leaf batching (with two refill thresholds), the
-        // workgroup size and the occupancy of the winner (packed layout).
+        // Grid over node levels per step x stack depth x leaf ILP x refill
+        // threshold at full occupancy; then leaf batching (with two refill
+        // thresholds), the workgroup size and the occupancy of the winner
+        // (packed layout).
         Tuning best;
         float best_ms = 1e30f;
         auto consider = [&](Tuning cand) {
@@ -84,13 +85,16 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
                                         (float *)tt.p, 2);
             if (ms < best_ms) best_ms = ms, best = cand;
         };
-        for (int K : {4, 6, 8, 16})
-            for (int ilp : {1, 2})
-                for (int fm : {16, 24, 32, 48}) {
-                    Tuning t;
-                    t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm;
-                    consider(t);
-                }
+        for (int nl : {1, 2}) {
+            if (nl == 2 && !sc->d.quads) continue;
+            for (int K : {4, 6, 8, 16})
+                for (int ilp : {1, 2})
+                    for (int fm : {16, 24, 32, 48}) {
+                        Tuning t;
+                        t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm, t.node_levels = nl;
+                        consider(t);
+                    }
+        }
         for (int ilp : {1, 2}) {
             const Tuning base = best;
             for (int lb : {16, 24, 32})
@@ -152,13 +156,16 @@ int pray_autotune_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, 
             return PRAY_OK;
         };
         int rc = PRAY_OK;
-        for (int K : {4, 6, 8, 16})
-            for (int ilp : {1, 2})
-                for (int fm : {16, 24, 32, 48}) {
-                    Tuning t;
-                    t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm;
-                    if (!rc) rc = consider(t);
-                }
+        for (int nl : {1, 2}) {
+            if (nl == 2 && !sc->d.quads) continue;
+            for (int K : {4, 6, 8, 16})
+                for (int ilp : {1, 2})
+                    for (int fm : {16, 24, 32, 48}) {
+                        Tuning t;
+                        t.stack_lds = K, t.leaf_ilp = ilp, t.fetch_min = fm, t.node_levels = nl;
+                        if (!rc) rc = consider(t);
+                    }
+        }
         for (int ilp : {1, 2}) {
             const Tuning base = best;
             for (int lb : {16, 24, 32})

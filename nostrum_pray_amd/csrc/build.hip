@@ -1097,6 +1097,17 @@ int pray_build_scene(pray_ctx *c, const float *tris, const float *color, const u
     hipLaunchKernelGGL(prb::k_pack_leaf_records, dim3(nblocks(B.n_refs, 256)), dim3(256), 0, st, B.n_refs,
                        (const uint32_t *)fr, (const float *)a, (const float *)ab, (const float *)ac, recs);
     HIPC(c, hipGetLastError());
+    d.nodes = nodes;
+    d.refs = fr;
+    d.a = a, d.ab = ab, d.ac = ac;
+    d.normal = normal, d.color = col, d.emit = em;
+    d.recs = recs;
+    d.n_nodes = B.n_entries;
+    d.n_refs = B.n_refs;
+    d.max_depth = B.depth;
+    // The SAH tree's pairs have equal clips: its two-level node records are
+    // part of its layout (the build waits for their check).
+    if (method == PRAY_BUILD_SAH && (rc = build_quads(c, *sc, st))) return rc;
     HIPC(c, hipEventRecord(c->ievents[1], st));
     // ---- end of the timed build ----
 
@@ -1104,15 +1115,7 @@ int pray_build_scene(pray_ctx *c, const float *tris, const float *color, const u
     HIPC(c, hipMemcpyAsync(box_h, box, 24, hipMemcpyDeviceToHost, st));
     HIPC(c, hipStreamSynchronize(st));
     if (build_ms) HIPC(c, hipEventElapsedTime(build_ms, c->ievents[0], c->ievents[1]));
-    d.nodes = nodes;
-    d.refs = fr;
-    d.a = a, d.ab = ab, d.ac = ac;
-    d.normal = normal, d.color = col, d.emit = em;
-    d.recs = recs;
     std::memcpy(d.box, box_h, sizeof d.box);
-    d.n_nodes = B.n_entries;
-    d.n_refs = B.n_refs;
-    d.max_depth = B.depth;
     *scene_id = c->next_id++;
     c->scenes[*scene_id] = std::move(sc);
     return PRAY_OK;

@@ -22,6 +22,7 @@ struct Tuning {
     int occupancy = 100;  // percent of the resident persistent waves launched
     int wg_waves = 1;     // waves per workgroup (1, 2 or 4)
     int leaf_batch = 0;   // 0: node and leaf steps in every iteration; T: leaf steps once T lanes wait at leaves
+    int node_levels = 1;  // tree levels per node step: 1 (children pairs) or 2 (two-level records, SAH trees)
 };
 
 struct SceneRec {
@@ -91,8 +92,13 @@ SceneRec *find_scene(pray_ctx *c, int id);
 // Device allocation owned by a scene record (freed by free_scene_rec).
 int device_alloc(pray_ctx *c, SceneRec &sc, void **p, size_t bytes);
 void free_scene_rec(SceneRec &sc);
+// Builds the scene's two-level node records (prt::k_build_quads) from its
+// uploaded children pairs, synchronising on st, and keeps them in d.quads when
+// the tree is in SAH form (every pair's two clips equal) and its indices fit;
+// otherwise d.quads stays null and only one-level node steps trace the scene.
+int build_quads(pray_ctx *c, SceneRec &sc, hipStream_t st);
 int ensure_spill(pray_ctx *c, size_t bytes);
-// {layout, stack_lds, leaf_ilp, fetch_min, occupancy, wg_waves, leaf_batch}
+// {layout, stack_lds, leaf_ilp, fetch_min, occupancy, wg_waves, leaf_batch, node_levels}
 void tuning_values(const Tuning &t, int *v);
 // Enqueues one trace over n rays (or *n_dev rays when n_dev is set, with n as
 // the launch bound).

@@ -161,7 +161,21 @@ TraceKernel pick_kernel_multi(int K, int ilp) {
     }
 }
 
-TraceKernel pick_kernel(bool any, int layout, int K, int ilp) {
+template <bool ANY>
+TraceKernel pick_kernel_q(int K, int ilp) {
+    switch (K) {
+    case 4: return ilp == 2 ? prt::k_trace_q<ANY, 4, 2> : prt::k_trace_q<ANY, 4, 1>;
+    case 5: return ilp == 2 ? prt::k_trace_q<ANY, 5, 2> : prt::k_trace_q<ANY, 5, 1>;
+    case 6: return ilp == 2 ? prt::k_trace_q<ANY, 6, 2> : prt::k_trace_q<ANY, 6, 1>;
+    case 16: return ilp == 2 ? prt::k_trace_q<ANY, 16, 2> : prt::k_trace_q<ANY, 16, 1>;
+    default: return ilp == 2 ? prt::k_trace_q<ANY, 8, 2> : prt::k_trace_q<ANY, 8, 1>;
+    }
+}
+
+// two_level: the scene has two-level records and the tuning asks for them
+// (packed layout).
+TraceKernel pick_kernel(bool any, int layout, int K, int ilp, bool two_level) {
+    if (two_level && layout == PRAY_LAYOUT_PACKED) return any ? pick_kernel_q<true>(K, ilp) : pick_kernel_q<false>(K, ilp);
     if (any)
         return layout == PRAY_LAYOUT_PACKED ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED>(K, ilp)
                                             : pick_kernel_kl<true, PRAY_LAYOUT_INDEXED>(K, ilp);
@@ -245,7 +259,8 @@ static int enqueue_trace_ex(pray_ctx *c, const SceneRec &sc, const float *rays, 
         return PRAY_OK;
     }
     const Tuning &tu = sc.tune;
-    TraceKernel k = pick_kernel(mode != PRAY_TRACE_CLOSEST, tu.layout, tu.stack_lds, tu.leaf_ilp);
+    TraceKernel k = pick_kernel(mode != PRAY_TRACE_CLOSEST, tu.layout, tu.stack_lds, tu.leaf_ilp,
+                                tu.node_levels == 2 && sc.d.quads);
     Launch L = launch_shape(c, k, tu, n);
     const uint32_t depth = std::max<uint32_t>(1, sc.d.max_depth);
     uint32_t *work = c->work;
@@ -366,6 +381,34 @@ void free_scene_rec(SceneRec &sc) {
     sc.allocs.clear();
 }
 
+int build_quads(pray_ctx *c, SceneRec &sc, hipStream_t st) {
+    prt::DevScene &d = sc.d;
+    d.quads = nullptr;
+    if (d.n_tris == 0 || d.n_nodes < 2 || d.n_refs >= (prt::Q_MASK30 >> 2)) return PRAY_OK;
+    const uint32_t n_pairs = (d.n_nodes + 1) / 2;
+    const size_t bytes = (size_t)n_pairs * 32;
+    void *p = nullptr;
+    // the records, then one word for the build's flag
+    int rc = device_alloc(c, sc, &p, bytes + 16);
+    if (rc) return rc;
+    uint32_t *flag = reinterpret_cast<uint32_t *>(static_cast<char *>(p) + bytes);
+    hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, flag, 1u);
+    hipLaunchKernelGGL(prt::k_build_quads, dim3(nblocks(n_pairs, 256)), dim3(256), 0, st,
+                       reinterpret_cast<const uint4 *>(d.nodes), n_pairs, static_cast<uint4 *>(p), flag);
+    HIPC(c, hipGetLastError());
+    uint32_t f = ~0u;
+    HIPC(c, hipMemcpyAsync(&f, flag, 4, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipStreamSynchronize(st));
+    if (f == 0) {
+        d.quads = static_cast<const uint4 *>(p);
+    } else {
+        // not a SAH-form tree (or indices too large): one-level node steps only
+        sc.allocs.pop_back();
+        HIPC(c, hipFree(p));
+    }
+    return PRAY_OK;
+}
+
 // Structural check of an uploaded tree so that no kernel can read out of
 // bounds or loop: every inner node's child pair is even-aligned and inside
 // the array, no path is longer than the declared depth (which also rules out
@@ -401,14 +444,15 @@ int validate_tree(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const ui
 }
 
 int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n) {
-    TraceKernel k = pick_kernel(false, sc.tune.layout, sc.tune.stack_lds, sc.tune.leaf_ilp);
+    TraceKernel k = pick_kernel(false, sc.tune.layout, sc.tune.stack_lds, sc.tune.leaf_ilp,
+                                sc.tune.node_levels == 2 && sc.d.quads);
     return ensure_spill(c, (size_t)launch_shape(c, k, sc.tune, n).lanes * std::max<uint32_t>(1, sc.d.max_depth) *
                                sizeof(uint4));
 }
 
 void tuning_values(const Tuning &t, int *v) {
     v[0] = t.layout, v[1] = t.stack_lds, v[2] = t.leaf_ilp, v[3] = t.fetch_min, v[4] = t.occupancy;
-    v[5] = t.wg_waves, v[6] = t.leaf_batch;
+    v[5] = t.wg_waves, v[6] = t.leaf_batch, v[7] = t.node_levels;
 }
 
 }  // namespace prc
@@ -546,6 +590,7 @@ int pray_upload_scene(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, cons
             rc = fail(c, PRAY_ERR_HIP, "record packing launch failed");
             goto bad;
         }
+        if ((rc = build_quads(c, *sc, c->stream))) goto bad;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         rc = fail(c, PRAY_ERR_HIP, "scene upload failed");
@@ -577,7 +622,7 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
     if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
     if (!v) return fail(c, PRAY_ERR_INVALID, "null tuning");
     const int layout = v[0], stack_lds = v[1], leaf_ilp = v[2], fetch_min = v[3], occupancy = v[4], wg_waves = v[5],
-              leaf_batch = v[6];
+              leaf_batch = v[6], node_levels = v[7];
     if (layout >= 0 && layout != PRAY_LAYOUT_INDEXED && layout != PRAY_LAYOUT_PACKED)
         return fail(c, PRAY_ERR_INVALID, "unknown layout %d", layout);
     if (stack_lds >= 0 && stack_lds != 4 && stack_lds != 5 && stack_lds != 6 && stack_lds != 8 && stack_lds != 16)
@@ -590,6 +635,15 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
     if (wg_waves >= 0 && wg_waves != 1 && wg_waves != 2 && wg_waves != 4)
         return fail(c, PRAY_ERR_INVALID, "wg_waves must be 1, 2 or 4");
     if (leaf_batch > WAVE) return fail(c, PRAY_ERR_INVALID, "leaf_batch must be in 0..64");
+    if (node_levels >= 0 && node_levels != 1 && node_levels != 2)
+        return fail(c, PRAY_ERR_INVALID, "node_levels must be 1 or 2");
+    if ((node_levels >= 0 ? node_levels : sc->tune.node_levels) == 2) {
+        if (!sc->d.quads)
+            return fail(c, PRAY_ERR_UNSUPPORTED, "scene %d has no two-level node records (not a SAH-form tree)",
+                        scene_id);
+        if ((layout >= 0 ? layout : sc->tune.layout) != PRAY_LAYOUT_PACKED)
+            return fail(c, PRAY_ERR_INVALID, "two-level node steps trace the packed layout");
+    }
     Tuning &t = sc->tune;
     if (layout >= 0) t.layout = layout;
     if (stack_lds >= 0) t.stack_lds = stack_lds;
@@ -598,6 +652,7 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
     if (occupancy >= 0) t.occupancy = occupancy;
     if (wg_waves >= 0) t.wg_waves = wg_waves;
     if (leaf_batch >= 0) t.leaf_batch = leaf_batch;
+    if (node_levels >= 0) t.node_levels = node_levels;
     return PRAY_OK;
 }
 

@@ -50,6 +50,7 @@ struct DevScene {
     const uint32_t *refs = nullptr;   // n_refs + 2 (padded)
     const float *a = nullptr, *ab = nullptr, *ac = nullptr;
     const float4 *recs = nullptr;     // 3 * (n_refs + 2) (padded)
+    const uint4 *quads = nullptr;     // 2 per children pair: two-level node records (SAH trees), or null
     const float *normal = nullptr, *color = nullptr;
     const uint8_t *emit = nullptr;
     float box[6] = {0, 0, 0, 0, 0, 0};
@@ -382,6 +383,380 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
             }
             if (got) {
                 if ((w & 3u) == LEAF_TAG) {
+                    state = ST_LEAF;
+                    k = w >> 2;
+                } else {
+                    state = ST_NODE;
+                }
+            } else {
+                A.out_idx[ray] = (int32_t)best_i;
+                if (A.out_t) A.out_t[ray] = best_i == NO_HIT ? FLT_MAX : best_t;
+                if (A.counts) A.counts[2 * (size_t)ray] = cn, A.counts[2 * (size_t)ray + 1] = cr;
+                state = ST_IDLE;
+            }
+        }
+    }
+}
+
+// Two-level node records (SAH trees, whose children pairs have two equal
+// clips).  For the node Z whose children pair is p (the pair a word w with
+// w >> 3 == p names), quads[2p + i] describes Z's child c_i in 16 bytes:
+//   .x  c_i's split plane (float bits), or its first leaf reference
+//   .y  bits [29:0] the word of c_i's first child (30-bit form), [31:30] c_i's axis
+//   .z  bits [29:0] the word of c_i's second child, [31:30] c_i's kind (Q_INNER,
+//       Q_LEAF, Q_EMPTY)
+//   .w  quads[2p].w: Z's split plane (float bits); quads[2p + 1].w unused
+// A word in 30-bit form is the word itself, or Q_EMPTY30 for an empty leaf
+// (indices stay below 2^28 - 1).  One 32-byte record, two loads from one cache
+// line, carries a traversal two levels down (Z, then its near child); a far
+// child pushed at the first level is a half reference (Q_HALF | p << 1 | i)
+// whose 16 bytes alone carry one level.
+constexpr uint32_t Q_INNER = 0u, Q_LEAF = 1u, Q_EMPTY = 2u;
+constexpr uint32_t Q_EMPTY30 = 0x3FFFFFFFu;
+constexpr uint32_t Q_MASK30 = 0x3FFFFFFFu;
+constexpr uint32_t Q_HALF = 0x80000000u;
+
+// Builds quads[2p], quads[2p + 1] for every pair p >= 1 (pair 0 holds the
+// padding node and the root, which is no node's child).  flag bit 0: some pair
+// has two different clips (not a SAH-form tree); bit 1: an index does not fit
+// the 30-bit word form.  Either leaves the records unusable.
+static __global__ void k_build_quads(const uint4 *pairs, uint32_t n_pairs, uint4 *quads, uint32_t *flag) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 || p >= n_pairs) return;
+    const uint4 pr = pairs[p];
+    uint32_t bad = pr.x != pr.z ? 1u : 0u;
+    uint4 h[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const uint32_t w = i == 0 ? pr.y : pr.w;
+        uint4 q = make_uint4(0u, 0u, Q_EMPTY << 30, 0u);
+        if (empty_leaf(w)) {
+        } else if ((w & 3u) == LEAF_TAG) {
+            q = make_uint4(w >> 2, 0u, Q_LEAF << 30, 0u);
+            if ((w >> 2) >= (Q_MASK30 >> 2)) bad |= 2u;
+        } else if ((w >> 3) < n_pairs) {
+            const uint4 cp = pairs[w >> 3];
+            uint32_t g[2] = {cp.y, cp.w};
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (empty_leaf(g[j]))
+                    g[j] = Q_EMPTY30;
+                else if ((g[j] >> 2) >= (Q_MASK30 >> 2))  // keeps Q_EMPTY30 free
+                    bad |= 2u;
+            }
+            q = make_uint4(cp.x, g[0] | ((w & 3u) << 30), g[1] | (Q_INNER << 30), 0u);
+        } else {
+            bad |= 2u;
+        }
+        h[i] = q;
+    }
+    h[0].w = pr.x;
+    quads[2 * (size_t)p] = h[0];
+    quads[2 * (size_t)p + 1] = h[1];
+    if (bad) atomicOr(flag, bad);
+}
+
+// The closest-hit / any-hit traversal of k_trace (packed layout, single
+// scene) over the two-level records: a node step loads one record and takes
+// the node's level and, for the near (or only) child, that child's level,
+// with the same plane arithmetic, widening, pruning and push order as two
+// k_trace node steps, so every ray visits the same nodes and triangles and
+// gets the same result and work counts.  Popping a far child of the first
+// level loads its 16-byte half and takes its one level.  Each node step
+// touches one cache line instead of one per level.
+template <bool ANY, int K, int ILP>
+static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene s, TraceArgs A) {
+    static_assert(K >= 2 && K <= 16, "LDS ring of 2..16 entries");
+    extern __shared__ uint32_t ring_lds[];
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t wave_in_block = threadIdx.x / WAVE;
+    const uint32_t gwave = blockIdx.x * (blockDim.x / WAVE) + wave_in_block;
+    uint32_t *st_w = ring_lds + wave_in_block * (3 * K * WAVE);
+    float *st_lo = reinterpret_cast<float *>(st_w + K * WAVE);
+    float *st_hi = reinterpret_cast<float *>(st_w + 2 * K * WAVE);
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint4 *spill = A.spill + ((size_t)gwave * WAVE + lane) * A.spill_depth;
+    const uint32_t nn = A.n_dev ? *A.n_dev : A.n;
+    const uint4 *pairs = reinterpret_cast<const uint4 *>(s.nodes);
+    const uint4 *quads = s.quads;
+    const float4 *recs = s.recs;
+    constexpr uint32_t ST_HALF = 4u;  // at a far child pushed by a record's first level
+
+    uint32_t state = ST_IDLE, ray = 0, w = 0, k = 0, best_i = NO_HIT, cn = 0, cr = 0;
+    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, lim2 = 0;
+    float lo = 0, hi = 0, best_t = INFINITY;
+    int sp = 0, base = 0;
+    const uint32_t nchunks = (nn + WAVE - 1) / WAVE;
+    uint32_t pool_next = 0, pool_end = 0, ctr = gwave % NCTR;
+
+    // Pushes (word, entry, exit) on the lane's stack, spilling the oldest LDS
+    // ring entry to its HBM slot when the ring is full.
+    auto push = [&](uint32_t pw, float plo, float phi) {
+        if (sp - base == K) {
+            const uint32_t at = ((uint32_t)base % K) * WAVE + lane;
+            spill[base] = make_uint4(st_w[at], __float_as_uint(st_lo[at]), __float_as_uint(st_hi[at]), 0u);
+            ++base;
+        }
+        const uint32_t at = ((uint32_t)sp % K) * WAVE + lane;
+        st_w[at] = pw;
+        st_lo[at] = plo;
+        st_hi[at] = phi;
+        ++sp;
+    };
+
+    while (true) {
+        // ------------------------------------------------------- refill --
+        const uint64_t idle = __ballot(state == ST_IDLE);
+        const uint64_t dead = __ballot(state == ST_DEAD);
+        if (dead == ~0ull) break;
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (nidle && (nidle >= A.fetch_min || (idle | dead) == ~0ull)) {
+            const uint32_t avail = pool_end - pool_next;
+            uint32_t nb = 0;
+            if (avail < nidle) {
+                const uint32_t first = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
+                uint32_t c = nchunks;
+                if (lane == first) {
+                    for (uint32_t tries = 0; tries < NCTR; ++tries) {
+                        const uint32_t own = ctr < nchunks ? (nchunks - ctr + NCTR - 1) / NCTR : 0u;
+                        uint32_t *p = A.work + ctr * CTR_STRIDE;
+                        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < own) {
+                            const uint32_t i = atomicAdd(p, 1u);
+                            if (i < own) {
+                                c = ctr + i * NCTR;
+                                break;
+                            }
+                        }
+                        ctr = ctr + 1 == NCTR ? 0u : ctr + 1;
+                    }
+                }
+                c = (uint32_t)__builtin_amdgcn_readlane((int)c, (int)first);
+                ctr = (uint32_t)__builtin_amdgcn_readlane((int)ctr, (int)first);
+                nb = c * WAVE;
+            }
+            const uint32_t r = (uint32_t)__popcll(idle & below);
+            if (state == ST_IDLE) {
+                ray = r < avail ? pool_next + r : nb + (r - avail);
+                if (ray >= nn) {
+                    state = ST_DEAD;
+                } else {
+                    const float2 r0 = reinterpret_cast<const float2 *>(A.rays)[3 * (size_t)ray];
+                    const float2 r1 = reinterpret_cast<const float2 *>(A.rays)[3 * (size_t)ray + 1];
+                    const float2 r2 = reinterpret_cast<const float2 *>(A.rays)[3 * (size_t)ray + 2];
+                    ox = r0.x, oy = r0.y, oz = r1.x, dx = r1.y, dy = r2.x, dz = r2.y;
+                    ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+                    lo = 0.f, hi = INFINITY;
+                    slab(s.box[0], s.box[3], ox, ix, lo, hi);
+                    slab(s.box[1], s.box[4], oy, iy, lo, hi);
+                    slab(s.box[2], s.box[5], oz, iz, lo, hi);
+                    best_t = INFINITY, best_i = NO_HIT, sp = 0, base = 0, cn = 0, cr = 0;
+                    bool live = true;
+                    if (ANY) {
+                        lim2 = A.limit2[ray];
+                        if (!(lim2 >= 0.f))
+                            live = false;
+                        else
+                            hi = fminf(hi, whi(sqrtf(lim2)));
+                    }
+                    if (live && lo <= hi) {
+                        cn = 1;
+                        w = pairs[0].w;  // node 1 = second word pair of the first 16 bytes
+                        if ((w & 3u) != LEAF_TAG) {
+                            state = ST_NODE;
+                        } else if (!empty_leaf(w)) {
+                            state = ST_LEAF;
+                            k = w >> 2;
+                        } else {
+                            live = false;
+                        }
+                    } else {
+                        live = false;
+                    }
+                    if (!live) {
+                        A.out_idx[ray] = -1;
+                        if (A.out_t) A.out_t[ray] = FLT_MAX;
+                        if (A.counts) A.counts[2 * (size_t)ray] = cn, A.counts[2 * (size_t)ray + 1] = 0;
+                        state = ST_IDLE;
+                    }
+                }
+            }
+            if (avail < nidle) {
+                pool_next = nb + (nidle - avail);
+                pool_end = nb + WAVE;
+            } else {
+                pool_next += nidle;
+            }
+            pool_next = (uint32_t)__builtin_amdgcn_readfirstlane((int)pool_next);
+            pool_end = (uint32_t)__builtin_amdgcn_readfirstlane((int)pool_end);
+        }
+
+        bool run_node = true, run_leaf = true;
+        if (A.leaf_batch) {
+            const uint64_t at_leaf = __ballot(state == ST_LEAF);
+            const uint64_t at_node = __ballot(state == ST_NODE || state == ST_HALF);
+            run_leaf = at_leaf != 0ull && ((uint32_t)__popcll(at_leaf) >= A.leaf_batch || at_node == 0ull);
+            run_node = !run_leaf;
+        }
+        bool need_pop = false;
+        const uint32_t st_in = state;
+        // ---------------------------------------------------- node step --
+        if (run_node && (state == ST_NODE || state == ST_HALF)) {
+            // The child whose level comes next: its half record.
+            uint4 h;
+            bool go = true;
+            if (state == ST_NODE) {
+                // first level: node w, from its record's two halves
+                const uint32_t p = w >> 3;
+                const uint4 h0 = quads[2 * (size_t)p];
+                const uint4 h1 = quads[2 * (size_t)p + 1];
+                const uint32_t axis = w & 3u;
+                cn += 2;
+                const float oa = axis == 0 ? ox : (axis == 1 ? oy : oz);
+                const float ia = axis == 0 ? ix : (axis == 1 ? iy : iz);
+                const float tz = (__uint_as_float(h0.w) - oa) * ia;
+                const bool neg = ia < 0.f;
+                const uint4 hn = neg ? h1 : h0, hf = neg ? h0 : h1;
+                const float cap = ANY ? hi : fminf(hi, best_t);
+                const float n_hi = fminf(cap, whi(tz));
+                const float f_lo = fmaxf(lo, wlo(tz));
+                const bool gn = lo <= n_hi && (hn.z >> 30) != Q_EMPTY;
+                const bool gf = f_lo <= cap && (hf.z >> 30) != Q_EMPTY;
+                h = hn;
+                if (gn && gf) {
+                    push(Q_HALF | (p << 1) | (neg ? 0u : 1u), f_lo, hi);
+                    hi = n_hi;
+                } else if (gn) {
+                    hi = n_hi;
+                } else if (gf) {
+                    h = hf;
+                    lo = f_lo;
+                } else {
+                    go = false;
+                    need_pop = true;
+                }
+            } else {
+                // a far child pushed by a first level: its half alone
+                h = quads[2 * (size_t)((w & ~Q_HALF) >> 1) + (w & 1u)];
+            }
+            if (go) {
+                const uint32_t kind = h.z >> 30;
+                if (kind == Q_LEAF) {
+                    state = ST_LEAF;
+                    k = h.x;
+                } else {
+                    // second level: the child itself
+                    const uint32_t axis = h.y >> 30;
+                    cn += 2;
+                    const float oa = axis == 0 ? ox : (axis == 1 ? oy : oz);
+                    const float ia = axis == 0 ? ix : (axis == 1 ? iy : iz);
+                    const float tc = (__uint_as_float(h.x) - oa) * ia;
+                    const bool neg = ia < 0.f;
+                    const uint32_t w0 = h.y & Q_MASK30, w1 = h.z & Q_MASK30;
+                    const uint32_t wn = neg ? w1 : w0, wf = neg ? w0 : w1;
+                    const float cap = ANY ? hi : fminf(hi, best_t);
+                    const float n_hi = fminf(cap, whi(tc));
+                    const float f_lo = fmaxf(lo, wlo(tc));
+                    const bool gn = lo <= n_hi && wn != Q_EMPTY30;
+                    const bool gf = f_lo <= cap && wf != Q_EMPTY30;
+                    uint32_t nw = wn;
+                    if (gn && gf) {
+                        push(wf, f_lo, hi);
+                        hi = n_hi;
+                    } else if (gn) {
+                        hi = n_hi;
+                    } else if (gf) {
+                        nw = wf;
+                        lo = f_lo;
+                    } else {
+                        need_pop = true;
+                    }
+                    if (!need_pop) {
+                        w = nw;
+                        if ((w & 3u) == LEAF_TAG) {
+                            state = ST_LEAF;
+                            k = w >> 2;
+                        } else {
+                            state = ST_NODE;
+                        }
+                    }
+                }
+            }
+        }
+        // ---------------------------------------------------- leaf step --
+        if (run_leaf && !need_pop && state == ST_LEAF && (A.leaf_batch == 0 || st_in == ST_LEAF)) {
+            uint32_t ref[ILP];
+            float4 ta[ILP], tb[ILP], tc[ILP];
+#pragma unroll
+            for (int j = 0; j < ILP; ++j) load_tri<1>(s, recs, k + j, ref[j], ta[j], tb[j], tc[j]);
+            bool leaf_done = false, found = false;
+#pragma unroll
+            for (int j = 0; j < ILP; ++j) {
+                if (!leaf_done && !found) {
+                    ++cr;
+                    const uint32_t tri = ref[j] & ~REF_LAST;
+                    float t;
+                    if (prd::mt_hit(ox, oy, oz, dx, dy, dz, ta[j].x, ta[j].y, ta[j].z, tb[j].x, tb[j].y, tb[j].z,
+                                    tc[j].x, tc[j].y, tc[j].z, A.eps, t)) {
+                        if (!ANY) {
+                            if (t > 0.f && (t < best_t || (t == best_t && tri < best_i))) {
+                                best_t = t;
+                                best_i = tri;
+                            }
+                        } else {
+                            const float t2 = t * t;
+                            if (t > 0.f && (A.inclusive ? (t2 <= lim2) : (t2 < lim2))) {
+                                best_t = t;
+                                best_i = tri;
+                                found = true;
+                            }
+                        }
+                    }
+                    if (ref[j] & REF_LAST) leaf_done = true;
+                }
+            }
+            if (found) {
+                sp = 0;
+                base = 0;
+                need_pop = true;  // stack is empty: the pop below finishes the ray
+            } else if (leaf_done) {
+                need_pop = true;
+            } else {
+                k += ILP;
+            }
+        }
+
+        // ----------------------------------------------------------- pop --
+        if (need_pop) {
+            bool got = false;
+            while (sp > 0) {
+                --sp;
+                uint32_t ew;
+                float elo, ehi;
+                if (sp >= base) {
+                    const uint32_t at = ((uint32_t)sp % K) * WAVE + lane;
+                    ew = st_w[at];
+                    elo = st_lo[at];
+                    ehi = st_hi[at];
+                } else {
+                    const uint4 e = spill[sp];
+                    ew = e.x;
+                    elo = __uint_as_float(e.y);
+                    ehi = __uint_as_float(e.z);
+                    base = sp;
+                }
+                const float cap = ANY ? ehi : fminf(ehi, best_t);
+                if (elo <= cap) {
+                    w = ew;
+                    lo = elo;
+                    hi = ehi;
+                    got = true;
+                    break;
+                }
+            }
+            if (got) {
+                if (w & Q_HALF) {
+                    state = ST_HALF;
+                } else if ((w & 3u) == LEAF_TAG) {
                     state = ST_LEAF;
                     k = w >> 2;
                 } else {

@@ -38,25 +38,53 @@ def scene(name):
 
 # Every traversal setting gives the same hits:
 # (layout, LDS ring entries, waves per workgroup, leaf ILP, refill threshold,
-# leaf batch).  The first is the setting the autotuner picked for the
-# benchmark's 2048x2048 terrain before leaf batching (packed, ring 8,
-# one-wave workgroups, ILP 1, refill at 24 idle lanes), the second the one it
-# picks with it (ILP 2, refill at 16, leaf steps once 16 lanes wait).
-SETTINGS = [(P.LAYOUT_PACKED, 8, 1, 1, 24, 0), (P.LAYOUT_PACKED, 8, 1, 2, 16, 16),
-            (P.LAYOUT_PACKED, 5, 2, 2, 16, 0), (P.LAYOUT_INDEXED, 8, 4, 1, 32, 0),
-            (P.LAYOUT_INDEXED, 6, 1, 2, 16, 24), (P.LAYOUT_PACKED, 6, 1, 2, 48, 0),
-            (P.LAYOUT_PACKED, 4, 2, 2, 16, 32), (P.LAYOUT_PACKED, 16, 4, 1, 48, 64),
-            (P.LAYOUT_PACKED, 5, 1, 1, 1, 1)]
+# leaf batch, tree levels per node step).  The first is the setting the
+# autotuner picked for the benchmark's 2048x2048 terrain before leaf batching
+# (packed, ring 8, one-wave workgroups, ILP 1, refill at 24 idle lanes), the
+# second with it (ILP 2, refill at 16, leaf steps once 16 lanes wait).  Two
+# levels per step need a tree whose children pairs have equal clips (every
+# SAH tree; the terrain's median tree too, whose children meet on grid lines);
+# other trees refuse them.
+SETTINGS = [(P.LAYOUT_PACKED, 8, 1, 1, 24, 0, 1), (P.LAYOUT_PACKED, 8, 1, 2, 16, 16, 1),
+            (P.LAYOUT_PACKED, 5, 2, 2, 16, 0, 1), (P.LAYOUT_INDEXED, 8, 4, 1, 32, 0, 1),
+            (P.LAYOUT_INDEXED, 6, 1, 2, 16, 24, 1), (P.LAYOUT_PACKED, 6, 1, 2, 48, 0, 1),
+            (P.LAYOUT_PACKED, 4, 2, 2, 16, 32, 1), (P.LAYOUT_PACKED, 16, 4, 1, 48, 64, 1),
+            (P.LAYOUT_PACKED, 5, 1, 1, 1, 1, 1), (P.LAYOUT_PACKED, 8, 1, 2, 16, 16, 2),
+            (P.LAYOUT_PACKED, 8, 1, 1, 24, 0, 2), (P.LAYOUT_PACKED, 4, 2, 2, 32, 32, 2),
+            (P.LAYOUT_PACKED, 16, 4, 1, 1, 0, 2)]
+
+
+def equal_clips(flat):
+    """True when every children pair after pair 0 has two bitwise-equal
+    clips, the condition for two-level node records."""
+    nodes = flat["nodes"].reshape(-1, 2)
+    if len(nodes) % 2:
+        nodes = np.concatenate([nodes, np.zeros((1, 2), np.uint32)])
+    pairs = nodes.reshape(-1, 2, 2)[1:]
+    return bool(np.array_equal(pairs[:, 0, 0], pairs[:, 1, 0]))
+
+
+def set_or_refuse(ctx, sid, name, layout, ring, wg, ilp, fm, lb, nl):
+    """Applies a setting; False when the scene's tree has no two-level
+    records, after checking that the library refuses exactly those trees."""
+    if nl == 2 and not equal_clips(scene(name)[2]):
+        with pytest.raises(P.PrayError):
+            ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, node_levels=2)
+        return False
+    ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg, leaf_ilp=ilp, fetch_min=fm, leaf_batch=lb,
+                   node_levels=nl)
+    return True
 
 
 def check_closest(ctx, name, rays, eps, bf_rows=0):
     s, t, flat = scene(name)
     sid = ctx.upload(t)
     try:
-        for layout, ring, wg, ilp, fm, lb in SETTINGS:
-            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg, leaf_ilp=ilp, fetch_min=fm, leaf_batch=lb)
+        for layout, ring, wg, ilp, fm, lb, nl in SETTINGS:
+            if not set_or_refuse(ctx, sid, name, layout, ring, wg, ilp, fm, lb, nl):
+                continue
             say(f"closest {name} layout={layout} stack_lds={ring} wg={wg} ilp={ilp} fetch_min={fm} leaf_batch={lb} "
-                f"n={len(rays)} eps={eps}")
+                f"node_levels={nl} n={len(rays)} eps={eps}")
             gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, eps)
             oi, ot = O.flat_trace(flat, rays, 0, eps)
             assert first_diff(gi, oi) is None, first_diff(gi, oi)
@@ -106,10 +134,11 @@ def test_any_hit(ctx, name, eps, mode):
     sid = ctx.upload(t)
     try:
         res = []
-        for layout, ring, wg, ilp, fm, lb in SETTINGS:
-            ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg, leaf_ilp=ilp, fetch_min=fm, leaf_batch=lb)
+        for layout, ring, wg, ilp, fm, lb, nl in SETTINGS:
+            if not set_or_refuse(ctx, sid, name, layout, ring, wg, ilp, fm, lb, nl):
+                continue
             say(f"any {name} mode={mode} layout={layout} stack_lds={ring} wg={wg} ilp={ilp} fetch_min={fm} "
-                f"leaf_batch={lb}")
+                f"leaf_batch={lb} node_levels={nl}")
             res.append(ctx.trace(sid, rays, mode, eps, limit2=lim))
     finally:
         ctx.free_scene(sid)
@@ -126,16 +155,16 @@ def test_any_hit(ctx, name, eps, mode):
         assert (tt * np.float32(tt) < lim[k]) if mode == P.TRACE_ANY else (np.float32(tt) * np.float32(tt) <= lim[k])
 
 
-@pytest.mark.parametrize("lb", [0, 16, 40])
-def test_work_counts_match_cpu_twin(ctx, lb):
+@pytest.mark.parametrize("lb,nl", [(0, 1), (16, 1), (40, 1), (0, 2), (16, 2)])
+def test_work_counts_match_cpu_twin(ctx, lb, nl):
     """Per-ray node and triangle counts equal the CPU twin's, with and
-    without leaf batching."""
+    without leaf batching, one or two tree levels per node step."""
     _, t, flat = scene("terrain256-sah")
     rays = O.gen_rays(3, P.scene_box_expanded(flat["box"]), 1 << 14)
     sid = ctx.upload(t)
     try:
-        ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, leaf_batch=lb)
-        say(f"counts terrain256-sah leaf_batch={lb}")
+        ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, leaf_batch=lb, node_levels=nl)
+        say(f"counts terrain256-sah leaf_batch={lb} node_levels={nl}")
         n = rays.shape[0]
         d_rays = ctx.to_device(rays)
         d_idx = ctx.alloc(n, np.int32)
@@ -152,8 +181,8 @@ def test_work_counts_match_cpu_twin(ctx, lb):
     assert np.array_equal(gc, oc)
 
 
-@pytest.mark.parametrize("ring,lb", [(5, 0), (16, 0), (5, 32), (16, 8)])
-def test_launch_sizes(ctx, ring, lb):
+@pytest.mark.parametrize("ring,lb,nl", [(5, 0, 1), (16, 0, 1), (5, 32, 1), (16, 8, 1), (5, 0, 2), (16, 24, 2)])
+def test_launch_sizes(ctx, ring, lb, nl):
     """Traces over ray counts around the wave, the 64-ray chunks and the 64
     chunk counters (partial last chunks, fewer waves than counters), one after
     another through the same counters and spill area, including a ray without
@@ -164,7 +193,7 @@ def test_launch_sizes(ctx, ring, lb):
     sid = ctx.upload(t)
     try:
         ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, stack_lds=ring, fetch_min=1 if ring == 5 else 64,
-                       wg_waves=1 if ring == 5 else 4, leaf_batch=lb)
+                       wg_waves=1 if ring == 5 else 4, leaf_batch=lb, node_levels=nl)
         base = O.gen_rays(12, P.scene_box_expanded(flat["box"]), 200_000)
         base[5, 3:] = 0.0
         base[7, :3] = [1e30, -1e30, 0.0]
@@ -173,7 +202,7 @@ def test_launch_sizes(ctx, ring, lb):
             if n == 200_000:
                 ctx.set_tuning(sid, occupancy=1)
             rays = np.ascontiguousarray(base[:n])
-            say(f"launch ring={ring} leaf_batch={lb} n={n}")
+            say(f"launch ring={ring} leaf_batch={lb} node_levels={nl} n={n}")
             gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, 1e-4)
             oi, ot = O.flat_trace(flat, rays, 0, 1e-4)
             k = keep[:n]

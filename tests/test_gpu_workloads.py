@@ -21,7 +21,7 @@ from conftest import first_diff, say
 pytestmark = pytest.mark.gpu
 
 BENCH_TUNING = dict(layout=P.LAYOUT_PACKED, stack_lds=8, leaf_ilp=2, fetch_min=16, occupancy=100, wg_waves=1,
-                    leaf_batch=16)
+                    leaf_batch=16, node_levels=1)
 
 
 def test_terrain2048_full_stream_equals_oracle(ctx):
@@ -47,7 +47,9 @@ def test_terrain2048_full_stream_equals_oracle(ctx):
         first = None
         for tun in (BENCH_TUNING, dict(BENCH_TUNING, leaf_ilp=1, fetch_min=24, leaf_batch=0),
                     dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0),
-                    dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=2, fetch_min=32, wg_waves=2, leaf_batch=32)):
+                    dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=2, fetch_min=32, wg_waves=2, leaf_batch=32,
+                         node_levels=1),
+                    dict(BENCH_TUNING, node_levels=2), dict(BENCH_TUNING, leaf_batch=0, leaf_ilp=1, node_levels=2)):
             ctx.set_tuning(sid, **tun)
             say(f"terrain 2048: closest hit, 2^24 rays, {tun}")
             ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, P.TRACE_CLOSEST, 1e-12)
@@ -100,7 +102,9 @@ def test_cornell100k_depth8_equals_oracle(ctx, tree_kind):
         res = []
         for tun in (dict(layout=P.LAYOUT_PACKED, stack_lds=4, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0),
                     dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=1, fetch_min=32, wg_waves=2, leaf_batch=0),
-                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=16)):
+                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=16),
+                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0,
+                         node_levels=2)):
             ctx.set_tuning(sid, **tun)
             say(f"cornell 100k {tree_kind} {w}x{h} {spp} spp depth {depth} rr {rr} {tun}")
             hdr, rgb = ctx.render_pt(sid, cam, w, h, spp, depth, rr, seed, bg, 1e-9)
