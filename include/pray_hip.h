@@ -62,7 +62,10 @@ extern "C" {
 typedef struct pray_ctx pray_ctx;
 
 /* Opens `device` and creates a context with its own stream.  Replaces the
- * implicit device setup of path_trace.cu:501-504. */
+ * implicit device setup of path_trace.cu:501-504.
+ * Test hook: with the environment variable PRAY_TEST_INIT_STATUS set to a
+ * negative PRAY_ERR_* code, pray_init returns that code without touching the
+ * device, so callers' handling of a failed setup can be tested anywhere. */
 PRAY_API int pray_init(int device, pray_ctx **out);
 PRAY_API void pray_destroy(pray_ctx *ctx);
 PRAY_API const char *pray_last_error(const pray_ctx *ctx);
@@ -70,7 +73,9 @@ PRAY_API const char *pray_last_error(const pray_ctx *ctx);
 /* Memory feasibility for the flat layout (replaces isComputableOnGPU,
  * path_trace.cu:557-583): *ok = 1 when the device has room for the scene
  * (nodes, leaf refs, triangles incl. shading data) and n_rays rays with
- * their hit records and image.  Does not allocate. */
+ * their hit records and image.  Does not allocate.
+ * Test hook: PRAY_TEST_FREE_BYTES in the environment replaces the free
+ * device memory the answer is judged by. */
 PRAY_API int pray_mem_ok(pray_ctx *ctx, uint64_t n_tris, uint64_t n_refs, uint64_t n_nodes, uint64_t n_rays, int *ok);
 
 /* Uploads a flattened tree once (replaces copyKDTree, path_trace.cu:480-492).

@@ -382,6 +382,13 @@ class Context:
         if rc != PRAY_OK:
             raise PrayError(rc, self._lib.pray_last_error(self._h).decode())
 
+    def mem_ok(self, n_tris, n_refs, n_nodes, n_rays):
+        """Whether the device has room for a scene of these sizes and n_rays
+        rays (pray_mem_ok)."""
+        ok = C.c_int(0)
+        self._check(self._lib.pray_mem_ok(self._h, n_tris, n_refs, n_nodes, n_rays, C.byref(ok)))
+        return bool(ok.value)
+
     def upload(self, tree, shading=True):
         sid = C.c_int(-1)
         self._check(self._host.ph_upload(tree._h, self._h, 1 if shading else 0, C.byref(sid)))

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -466,6 +467,11 @@ extern "C" {
 int pray_init(int device, pray_ctx **out) {
     if (!out) return PRAY_ERR_INVALID;
     *out = nullptr;
+    // test hook: the status a failed device setup would give
+    if (const char *f = std::getenv("PRAY_TEST_INIT_STATUS")) {
+        const int code = std::atoi(f);
+        if (code < 0) return code;
+    }
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return PRAY_ERR_NODEVICE;
     if (device < 0 || device >= n) return PRAY_ERR_INVALID;
@@ -522,6 +528,8 @@ int pray_mem_ok(pray_ctx *c, uint64_t n_tris, uint64_t n_refs, uint64_t n_nodes,
     HIPC(c, hipSetDevice(c->device));
     size_t fr = 0, tot = 0;
     HIPC(c, hipMemGetInfo(&fr, &tot));
+    // test hook: the free device memory to judge by, in bytes
+    if (const char *f = std::getenv("PRAY_TEST_FREE_BYTES")) fr = (size_t)std::strtoull(f, nullptr, 10);
     const uint64_t need = n_nodes * 8 + n_refs * (4 + 48) + n_tris * (36 + 24 + 1) + n_rays * (24 + 8 + 3 + 12 * 4);
     *ok = (uint64_t)fr >= need ? 1 : 0;
     return PRAY_OK;

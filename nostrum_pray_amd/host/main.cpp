@@ -99,8 +99,11 @@ int main(int argc, char *argv[]) {
         t0 = clk::now();
         pray::preprocessScene(scene);
         if (!pray::isComputableOnGPU(scene)) {
-            std::cerr << "the GPU cannot hold this scene\n";
-            return 1;
+            // The reference falls back to its CPU tracer here
+            // (scene_preprocessor.cpp:42-49); this drop-in has none.
+            std::cerr << "the GPU cannot take this scene (no usable GPU, or not enough free device memory)\n";
+            pray::shutdownGPU();
+            return 2;
         }
         t1 = clk::now();
         std::cout << "Preprocessing time: " << ms(t0, t1) << " ms" << std::endl;

@@ -39,10 +39,17 @@ int upload(const Scene &scene, const KDTreeGPU &g) {
 }
 }  // namespace
 
+// The reference's eligibility check (path_trace.cu:557-583): a bool the
+// caller acts on (scene_preprocessor.cpp:42-49), false when there is no
+// usable device or it lacks the memory -- never an exception.
 bool isComputableOnGPU(const Scene &scene) {
+    if (!g_ctx && pray_init(0, &g_ctx) != PRAY_OK) {
+        g_ctx = nullptr;
+        return false;
+    }
     int ok = 0;
     const uint64_t n = scene.triangles.size();
-    if (pray_mem_ok(ctx(), n, n + n / 2, 4 * (n / 8 + 1), scene.width * scene.height, &ok) != PRAY_OK) return false;
+    if (pray_mem_ok(g_ctx, n, n + n / 2, 4 * (n / 8 + 1), scene.width * scene.height, &ok) != PRAY_OK) return false;
     return ok != 0;
 }
 

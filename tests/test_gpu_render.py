@@ -307,3 +307,28 @@ def test_cli_progressive_accumulation_and_hdr(ctx, tmp_path):
     rc, out = run_cli(other, tmp_path / "d.bmp", "--spp", 1, "--accum", st)
     assert rc != 0 and "another scene" in out, out
     assert st.read_bytes() == before
+
+
+def test_free_memory_decides_gpu_eligibility(ctx, tmp_path):
+    """pray_mem_ok answers from the device's free memory (here overridden
+    through the library's test hook): a scene that fits is taken, the same
+    scene with 1 KB free is not, and the CLI then reports and exits with
+    status 2 instead of failing inside a render."""
+    import subprocess
+    say("mem_ok with the device's free memory and with 1 KB")
+    assert ctx.mem_ok(25_602, 40_000, 20_000, 512 * 512)
+    os.environ["PRAY_TEST_FREE_BYTES"] = "1024"
+    try:
+        assert not ctx.mem_ok(25_602, 40_000, 20_000, 512 * 512)
+    finally:
+        del os.environ["PRAY_TEST_FREE_BYTES"]
+    assert ctx.mem_ok(25_602, 40_000, 20_000, 512 * 512)
+    scene = os.path.join(GOLDEN, "scenes", "head_fov30.json")
+    say("CLI with 1 KB of free device memory")
+    out = tmp_path / "small.bmp"
+    r = subprocess.run([P.CLI, scene, str(out)], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, PRAY_TEST_FREE_BYTES="1024"))
+    assert r.returncode == 2 and "cannot take this scene" in r.stderr and not out.exists(), r.stderr
+    say("CLI with the device's free memory")
+    r = subprocess.run([P.CLI, scene, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and out.exists(), r.stderr

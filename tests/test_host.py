@@ -57,6 +57,32 @@ def test_context_without_gpu_fails_cleanly():
         assert e.code == -6
 
 
+def test_init_failure_status_reaches_the_caller(monkeypatch):
+    """A failed device setup (injected through the library's test hook)
+    comes back as its status code, on machines with and without a GPU."""
+    for code in (-2, -6, -3):
+        monkeypatch.setenv("PRAY_TEST_INIT_STATUS", str(code))
+        with pytest.raises(P.PrayError) as e:
+            P.Context(0)
+        assert e.value.code == code
+
+
+def test_cli_without_usable_gpu_reports_and_exits(tmp_path):
+    """isComputableOnGPU is a bool (path_trace.cu:557-583): without a usable
+    device the drop-in CLI says so and exits with status 2 -- no exception
+    escapes, no signal, no image (scene_preprocessor.cpp:42-49 would fall
+    back to the CPU here)."""
+    import subprocess
+    scene = os.path.join(ROOT, "tests", "golden", "scenes", "head_fov30.json")
+    for code in ("-6", "-2"):
+        out = tmp_path / f"x{code}.bmp"
+        env = dict(os.environ, PRAY_TEST_INIT_STATUS=code)
+        r = subprocess.run([P.CLI, scene, str(out)], capture_output=True, text=True, env=env, timeout=120)
+        assert r.returncode == 2, (r.returncode, r.stderr)
+        assert "cannot take this scene" in r.stderr
+        assert not out.exists()
+
+
 def compare_trees(scene, kind, depth=20):
     abc, _, _ = scene.triangles()
     t = scene.build(kind, max_depth=depth)
