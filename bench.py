@@ -119,6 +119,8 @@ def main():
                          "equal-RMSE measurement alone; build: the KD-tree builds on the GPU")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg of the default run")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="path tracers enqueue their passes kernel by kernel instead of replaying hipGraphs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -139,6 +141,7 @@ def main():
         return
     if args.workload == "cache":
         ctx = P.Context(0)
+        ctx.use_graphs(not args.no_graphs)
         try:
             out = radiance_cache_leg(P, ctx, args)
         finally:
@@ -151,6 +154,7 @@ def main():
     scene = P.Scene.terrain(args.grid, args.seed)
     n_tris = scene.num_tris
     ctx = P.Context(0)
+    ctx.use_graphs(not args.no_graphs)
     sid, tree_tag, tree_desc, tree_build, (tree_depth, tree_nodes, tree_refs), raw_box = \
         build_tree(P, ctx, scene, args, args.tree)
     log(f"scene {n_tris} tris, tree {tree_tag}: depth {tree_depth}, {tree_nodes} nodes, {tree_refs} references, "
@@ -472,6 +476,7 @@ def batch_leg(P, ctx, args):
 def batch_workload(P, args):
     """The batched multi-scene render alone, as one JSON line."""
     ctx = P.Context(0)
+    ctx.use_graphs(not args.no_graphs)
     b = batch_leg(P, ctx, args)
     ctx.close()
     out = {
@@ -502,6 +507,7 @@ def io_workload(P, args):
     import tempfile
 
     ctx = P.Context(0)
+    ctx.use_graphs(not args.no_graphs)
     tmp = tempfile.mkdtemp(prefix="pray_io_")
     try:
         scene = P.Scene.terrain(args.grid, args.seed)
@@ -613,6 +619,7 @@ def build_workload(P, args):
     2^20 rays over the GPU- and CPU-built SAH trees must agree bit for bit.
     One JSON line."""
     ctx = P.Context(0)
+    ctx.use_graphs(not args.no_graphs)
     try:
         scene = P.Scene.terrain(args.grid, args.seed)
         abc, _, _ = scene.triangles()

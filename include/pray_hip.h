@@ -269,7 +269,10 @@ PRAY_API int pray_render_pt_ex(pray_ctx *ctx, int scene_id, const float *camera1
  *          first_sample = 0); on return the sum over 0 .. first_sample+spp-1
  *   hdr / rgb  as pray_render_pt, the mean over all first_sample+spp samples
  * Renders split over several calls (passing accum on) give the image of one
- * pray_render_pt call bit for bit.  At most 2^24 samples in total. */
+ * pray_render_pt call bit for bit.  At most 2^24 samples in total.
+ * The path tracers keep their image-sized device buffers in the context
+ * (about 170 bytes per pixel for pray_render_pt*, 240 for the cache tracer)
+ * and reuse them in later calls until pray_destroy. */
 PRAY_API int pray_render_pt_progressive(pray_ctx *ctx, int scene_id, const float *camera16, uint32_t width,
                                         uint32_t height, uint32_t first_sample, uint32_t spp, uint32_t max_depth,
                                         uint32_t rr_depth, uint64_t seed, const float *background, float det_eps,
@@ -375,6 +378,11 @@ PRAY_API int pray_copy_device(pray_ctx *ctx, void *d_dst, const void *d_src, uin
 PRAY_API int pray_event_record(pray_ctx *ctx, int slot);
 /* Milliseconds between two recorded events (synchronises on the second). */
 PRAY_API int pray_event_elapsed(pray_ctx *ctx, int slot0, int slot1, float *ms);
+/* With enable = 0, the path tracers (pray_render_pt*, pray_render_pt_cache,
+ * pray_render_pt_batch) enqueue every sample pass kernel by kernel instead of
+ * replaying a hipGraph captured once per call; results are identical.  For
+ * tools that follow kernel launches one by one (profilers); the default is 1. */
+PRAY_API int pray_use_graphs(pray_ctx *ctx, int enable);
 /* Kernel timing: with enable = 1, every later trace that is not being
  * captured into a graph records events around its traversal kernel alone
  * (not the counter reset before it); pray_last_kernel_ms synchronises and

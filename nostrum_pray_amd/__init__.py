@@ -85,6 +85,7 @@ def libraries():
         "pray_trace_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp]),
         "pray_trace_counts_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp, _vp]),
         "pray_trace_device_noalloc": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp]),
+        "pray_use_graphs": (C.c_int, [_vp, C.c_int]),
         "pray_gen_rays_device": (C.c_int, [_vp, _u64, _vp, _u64, _vp, _vp]),
         "pray_gen_rays": (C.c_int, [_vp, _u64, _vp, _u64, _vp]),
         "pray_camera_rays": (C.c_int, [_vp, _vp, _u32, _u32, _vp]),
@@ -381,6 +382,11 @@ class Context:
     def _check(self, rc):
         if rc != PRAY_OK:
             raise PrayError(rc, self._lib.pray_last_error(self._h).decode())
+
+    def use_graphs(self, enable=True):
+        """Path tracers replay captured hipGraphs (default) or enqueue kernel
+        by kernel (pray_use_graphs); results are identical."""
+        self._check(self._lib.pray_use_graphs(self._h, 1 if enable else 0))
 
     def mem_ok(self, n_tris, n_refs, n_nodes, n_rays):
         """Whether the device has room for a scene of these sizes and n_rays

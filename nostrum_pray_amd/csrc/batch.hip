@@ -246,23 +246,27 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
     Tuning tu = bucket_scene[main_bucket]->tune;
     tu.layout = PRAY_LAYOUT_PACKED;
 
-    DBuf dtable, dbs, prim, rq[2], sq[2], qs[2], cnt, hit, tt, contrib, sum, outh, outb, rays_acc;
-    HIPC(c, hipMalloc(&dtable.p, n_scenes * sizeof(prt::DevScene)));
-    HIPC(c, hipMalloc(&dbs.p, n_scenes * sizeof(prr::BatchScene)));
-    HIPC(c, hipMalloc(&prim.p, total * 24));
+    // Buffers from the context's render workspace (slots 40..56), reused by
+    // later calls.
+    struct Buf {
+        void *p = nullptr;
+    } dtable, dbs, prim, rq[2], sq[2], qs[2], cnt, hit, tt, contrib, sum, outh, outb, rays_acc;
+    WS_GET(c, 40, dtable.p, n_scenes * sizeof(prt::DevScene));
+    WS_GET(c, 41, dbs.p, n_scenes * sizeof(prr::BatchScene));
+    WS_GET(c, 42, prim.p, total * 24);
     for (int q = 0; q < 2; q++) {
-        HIPC(c, hipMalloc(&rq[q].p, total * 24));
-        HIPC(c, hipMalloc(&sq[q].p, total * 16));
-        HIPC(c, hipMalloc(&qs[q].p, total * 4));
+        WS_GET(c, 43 + q, rq[q].p, total * 24);
+        WS_GET(c, 45 + q, sq[q].p, total * 16);
+        WS_GET(c, 47 + q, qs[q].p, total * 4);
     }
-    HIPC(c, hipMalloc(&cnt.p, 4 * sizeof(uint32_t)));
-    HIPC(c, hipMalloc(&hit.p, total * 4));
-    HIPC(c, hipMalloc(&tt.p, total * 4));
-    HIPC(c, hipMalloc(&contrib.p, total * 12));
-    HIPC(c, hipMalloc(&sum.p, total * 12));
-    HIPC(c, hipMalloc(&outh.p, total * 12));
-    HIPC(c, hipMalloc(&rays_acc.p, 8));
-    if (rgb) HIPC(c, hipMalloc(&outb.p, total * 3));
+    WS_GET(c, 49, cnt.p, 4 * sizeof(uint32_t));
+    WS_GET(c, 50, hit.p, total * 4);
+    WS_GET(c, 51, tt.p, total * 4);
+    WS_GET(c, 52, contrib.p, total * 12);
+    WS_GET(c, 53, sum.p, total * 12);
+    WS_GET(c, 54, outh.p, total * 12);
+    WS_GET(c, 55, rays_acc.p, 8);
+    if (rgb) WS_GET(c, 56, outb.p, total * 3);
     hipStream_t st = c->stream;
     HIPC(c, hipMemcpyAsync(dtable.p, table.data(), n_scenes * sizeof(prt::DevScene), hipMemcpyHostToDevice, st));
     HIPC(c, hipMemcpyAsync(dbs.p, bs.data(), n_scenes * sizeof(prr::BatchScene), hipMemcpyHostToDevice, st));
@@ -314,29 +318,40 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
     };
     for (int k = 2; k < 4; k++)
         if (!c->ievents[k]) HIPC(c, hipEventCreate(&c->ievents[k]));
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    HIPC(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    rc = pass();
-    hipError_t e = hipStreamEndCapture(st, &graph);
-    if (rc) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
-    }
-    if (e != hipSuccess) return fail(c, PRAY_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
+    if (c->use_graphs) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPC(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rc = pass();
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, PRAY_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(c, PRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+        }
+        e = hipEventRecord(c->ievents[2], st);
+        for (uint32_t s = 0; s < spp && e == hipSuccess; s++) e = hipGraphLaunch(exec, st);
+        if (e == hipSuccess) e = hipEventRecord(c->ievents[3], st);
+        const hipError_t e2 = hipStreamSynchronize(st);
+        (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
-        return fail(c, PRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+        if (e != hipSuccess || e2 != hipSuccess)
+            return fail(c, PRAY_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    } else {
+        // the same passes enqueued kernel by kernel
+        HIPC(c, hipEventRecord(c->ievents[2], st));
+        for (uint32_t s = 0; s < spp; s++) {
+            rc = pass();
+            if (rc) return rc;
+        }
+        HIPC(c, hipEventRecord(c->ievents[3], st));
+        HIPC(c, hipStreamSynchronize(st));
     }
-    e = hipEventRecord(c->ievents[2], st);
-    for (uint32_t s = 0; s < spp && e == hipSuccess; s++) e = hipGraphLaunch(exec, st);
-    if (e == hipSuccess) e = hipEventRecord(c->ievents[3], st);
-    const hipError_t e2 = hipStreamSynchronize(st);
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess || e2 != hipSuccess)
-        return fail(c, PRAY_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
     if (gpu_ms) HIPC(c, hipEventElapsedTime(gpu_ms, c->ievents[2], c->ievents[3]));
     HIPC(c, hipMemcpy(&c->last_rays, rays_acc.p, 8, hipMemcpyDeviceToHost));
     c->last_queues.clear();

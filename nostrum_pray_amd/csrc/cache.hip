@@ -367,28 +367,32 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
     const uint32_t stride = (uint32_t)std::max<uint64_t>(1, npix / n_train);
     prr::Cam cam;
     std::memcpy(&cam, cam16, sizeof cam);
-    DBuf prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, queries, hist, trays, tstate, thit, ttt, tq;
-    HIPC(c, hipMalloc(&prim.p, npix * 24));
+    // Buffers from the context's render workspace (slots 16..34), reused by
+    // later calls.
+    struct Buf {
+        void *p = nullptr;
+    } prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, queries, hist, trays, tstate, thit, ttt, tq;
+    WS_GET(c, 16, prim.p, npix * 24);
     for (int q = 0; q < 2; q++) {
-        HIPC(c, hipMalloc(&rq[q].p, npix * 24));
-        HIPC(c, hipMalloc(&sq[q].p, npix * 16));
+        WS_GET(c, 17 + q, rq[q].p, npix * 24);
+        WS_GET(c, 19 + q, sq[q].p, npix * 16);
     }
     // counters: [0], [1] path queues, [2] frame, [3] render queries,
     // [4] training queries
-    HIPC(c, hipMalloc(&cnt.p, 8 * sizeof(uint32_t)));
-    HIPC(c, hipMalloc(&hit.p, npix * 4));
-    HIPC(c, hipMalloc(&tt.p, npix * 4));
-    HIPC(c, hipMalloc(&contrib.p, npix * 12));
-    HIPC(c, hipMalloc(&sum.p, npix * 12));
-    HIPC(c, hipMalloc(&outh.p, npix * 12));
-    HIPC(c, hipMalloc(&queries.p, npix * 64));
-    HIPC(c, hipMalloc(&hist.p, (size_t)frames * 8));
-    HIPC(c, hipMalloc(&trays.p, (size_t)n_train * 24));
-    HIPC(c, hipMalloc(&tstate.p, (size_t)n_train * 16));
-    HIPC(c, hipMalloc(&thit.p, (size_t)n_train * 4));
-    HIPC(c, hipMalloc(&ttt.p, (size_t)n_train * 4));
-    HIPC(c, hipMalloc(&tq.p, (size_t)n_train * 64));
-    if (rgb) HIPC(c, hipMalloc(&outb.p, npix * 3));
+    WS_GET(c, 21, cnt.p, 8 * sizeof(uint32_t));
+    WS_GET(c, 22, hit.p, npix * 4);
+    WS_GET(c, 23, tt.p, npix * 4);
+    WS_GET(c, 24, contrib.p, npix * 12);
+    WS_GET(c, 25, sum.p, npix * 12);
+    WS_GET(c, 26, outh.p, npix * 12);
+    WS_GET(c, 27, queries.p, npix * 64);
+    WS_GET(c, 28, hist.p, (size_t)frames * 8);
+    WS_GET(c, 29, trays.p, (size_t)n_train * 24);
+    WS_GET(c, 30, tstate.p, (size_t)n_train * 16);
+    WS_GET(c, 31, thit.p, (size_t)n_train * 4);
+    WS_GET(c, 32, ttt.p, (size_t)n_train * 4);
+    WS_GET(c, 33, tq.p, (size_t)n_train * 64);
+    if (rgb) WS_GET(c, 34, outb.p, npix * 3);
     uint32_t *count = (uint32_t *)cnt.p;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(prr::k_camera, dim3(nblocks(npix, 256)), dim3(256), 0, st, cam, w, h, (float *)prim.p);
@@ -485,27 +489,36 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
         return PRAY_OK;
     };
 
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    HIPC(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    rc = frame();
-    hipError_t e = hipStreamEndCapture(st, &graph);
-    if (rc) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
-    }
-    if (e != hipSuccess) return fail(c, PRAY_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
+    if (c->use_graphs) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPC(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rc = frame();
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, PRAY_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(c, PRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+        }
+        for (uint32_t f = 0; f < frames && e == hipSuccess; f++) e = hipGraphLaunch(exec, st);
+        const hipError_t e2 = hipStreamSynchronize(st);
+        (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
-        return fail(c, PRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+        if (e != hipSuccess || e2 != hipSuccess)
+            return fail(c, PRAY_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    } else {
+        // the same frames enqueued kernel by kernel
+        for (uint32_t f = 0; f < frames; f++) {
+            rc = frame();
+            if (rc) return rc;
+        }
+        HIPC(c, hipStreamSynchronize(st));
     }
-    for (uint32_t f = 0; f < frames && e == hipSuccess; f++) e = hipGraphLaunch(exec, st);
-    const hipError_t e2 = hipStreamSynchronize(st);
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess || e2 != hipSuccess)
-        return fail(c, PRAY_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
     r->steps += frames;
     hipLaunchKernelGGL(prr::k_pt_finalize, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
                        (const float *)sum.p, (float)frames, (float *)outh.p, (uint8_t *)outb.p);

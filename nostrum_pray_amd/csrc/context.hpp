@@ -51,6 +51,7 @@ struct pray_ctx {
     bool kernel_timed = false;    // kevents hold a measurement
     hipStream_t side = nullptr;   // second stream for work overlapped inside graphs
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    bool use_graphs = true;       // path tracers replay captured hipGraphs (pray_use_graphs)
     uint64_t last_rays = 0;       // rays traced by the last render call
     std::vector<uint64_t> last_queues;  // per bounce: paths traced (pray_render_pt*)
     uint32_t *work = nullptr;   // ray counters of the persistent kernels
@@ -60,6 +61,11 @@ struct pray_ctx {
     std::map<std::pair<const void *, int>, int> occupancy;  // (kernel, waves per block) -> resident blocks per CU
     std::map<int, prc::Tuning> tuned;        // scene-size bucket -> autotuned tuning
     std::shared_ptr<void> build_scratch;     // device buffers of the tree builds, kept across builds
+    // Device buffers of the renderers by slot, kept across calls and grown on
+    // demand (render calls return after their work is done, so no buffer is
+    // in use when a later call reuses or replaces it).
+    std::vector<void *> ws_ptr;
+    std::vector<size_t> ws_bytes;
 };
 
 namespace prc {
@@ -98,6 +104,17 @@ void free_scene_rec(SceneRec &sc);
 // otherwise d.quads stays null and only one-level node steps trace the scene.
 int build_quads(pray_ctx *c, SceneRec &sc, hipStream_t st);
 int ensure_spill(pray_ctx *c, size_t bytes);
+// Render workspace slot `slot` with at least `bytes` bytes (kept by the
+// context until pray_destroy).  Slots: render_pt 0..15, the cache renderer
+// 16..39, the batch renderer 40..63.
+int ws_get(pray_ctx *c, unsigned slot, size_t bytes, void **out);
+#define WS_GET(ctx, slot, ptr, bytes)                                                                        \
+    do {                                                                                                      \
+        void *ws_p_ = nullptr;                                                                                \
+        const int ws_rc_ = ::prc::ws_get((ctx), (slot), (bytes), &ws_p_);                                     \
+        if (ws_rc_) return ws_rc_;                                                                            \
+        (ptr) = ws_p_;                                                                                        \
+    } while (0)
 // {layout, stack_lds, leaf_ilp, fetch_min, occupancy, wg_waves, leaf_batch, node_levels}
 void tuning_values(const Tuning &t, int *v);
 // Enqueues one trace over n rays (or *n_dev rays when n_dev is set, with n as

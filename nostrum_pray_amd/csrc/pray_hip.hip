@@ -242,6 +242,25 @@ struct FixedScratch {
     size_t spill_bytes = 0;
 };
 
+int ws_get(pray_ctx *c, unsigned slot, size_t bytes, void **out) {
+    if (slot >= c->ws_ptr.size()) {
+        c->ws_ptr.resize(slot + 1, nullptr);
+        c->ws_bytes.resize(slot + 1, 0);
+    }
+    bytes = std::max<size_t>(bytes, 256);
+    if (c->ws_bytes[slot] < bytes) {
+        if (c->ws_ptr[slot]) {
+            HIPC(c, hipFree(c->ws_ptr[slot]));
+            c->ws_ptr[slot] = nullptr;
+            c->ws_bytes[slot] = 0;
+        }
+        HIPC(c, hipMalloc(&c->ws_ptr[slot], bytes));
+        c->ws_bytes[slot] = bytes;
+    }
+    *out = c->ws_ptr[slot];
+    return PRAY_OK;
+}
+
 // Enqueues one trace over n rays (or *n_dev rays when n_dev is set, with n as
 // the launch bound), with the context's counters and spill area (grown on
 // demand), or with `fixed` scratch (nothing is allocated).
@@ -516,6 +535,8 @@ void pray_destroy(pray_ctx *c) {
     for (hipEvent_t e : c->kevents)
         if (e) (void)hipEventDestroy(e);
     if (c->spill) (void)hipFree(c->spill);
+    for (void *p : c->ws_ptr)
+        if (p) (void)hipFree(p);
     c->build_scratch.reset();
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -887,21 +908,25 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
     HIPC(c, hipSetDevice(c->device));
     Cam cam;
     std::memcpy(&cam, cam16, sizeof cam);
-    DBuf prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, qsum;
-    HIPC(c, hipMalloc(&prim.p, npix * 24));
-    for (int q = 0; q < 2; q++) {
-        HIPC(c, hipMalloc(&rq[q].p, npix * 24));
-        HIPC(c, hipMalloc(&sq[q].p, npix * 16));
-    }
-    HIPC(c, hipMalloc(&cnt.p, 6 * sizeof(uint32_t)));  // count0, count1, sample, pad, traced rays (u64)
     if (max_depth > 64) return fail(c, PRAY_ERR_INVALID, "max_depth must be at most 64");
-    HIPC(c, hipMalloc(&qsum.p, 65 * sizeof(uint64_t)));  // paths traced per bounce, all samples
-    HIPC(c, hipMalloc(&hit.p, npix * 4));
-    HIPC(c, hipMalloc(&tt.p, npix * 4));
-    HIPC(c, hipMalloc(&contrib.p, npix * 12));
-    HIPC(c, hipMalloc(&sum.p, npix * 12));
-    HIPC(c, hipMalloc(&outh.p, npix * 12));
-    if (rgb) HIPC(c, hipMalloc(&outb.p, npix * 3));
+    // Buffers from the context's render workspace (slots 0..12), reused by
+    // later calls.
+    struct Buf {
+        void *p = nullptr;
+    } prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, qsum;
+    WS_GET(c, 0, prim.p, npix * 24);
+    for (int q = 0; q < 2; q++) {
+        WS_GET(c, 1 + q, rq[q].p, npix * 24);
+        WS_GET(c, 3 + q, sq[q].p, npix * 16);
+    }
+    WS_GET(c, 5, cnt.p, 6 * sizeof(uint32_t));    // count0, count1, sample, pad, traced rays (u64)
+    WS_GET(c, 6, qsum.p, 65 * sizeof(uint64_t));  // paths traced per bounce, all samples
+    WS_GET(c, 7, hit.p, npix * 4);
+    WS_GET(c, 8, tt.p, npix * 4);
+    WS_GET(c, 9, contrib.p, npix * 12);
+    WS_GET(c, 10, sum.p, npix * 12);
+    WS_GET(c, 11, outh.p, npix * 12);
+    if (rgb) WS_GET(c, 12, outb.p, npix * 3);
     uint32_t *count = (uint32_t *)cnt.p;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(prr::k_camera, dim3(nblocks(npix, 256)), dim3(256), 0, st, cam, w, h, (float *)prim.p);
@@ -956,7 +981,7 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
         HIPC(c, hipGetLastError());
         return PRAY_OK;
     };
-    if (use_graph) {
+    if (use_graph && c->use_graphs) {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIPC(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1071,6 +1096,12 @@ int pray_event_record(pray_ctx *c, int slot) {
     HIPC(c, hipSetDevice(c->device));
     if (!c->events[slot]) HIPC(c, hipEventCreate(&c->events[slot]));
     HIPC(c, hipEventRecord(c->events[slot], c->stream));
+    return PRAY_OK;
+}
+
+int pray_use_graphs(pray_ctx *c, int enable) {
+    if (!c) return PRAY_ERR_INVALID;
+    c->use_graphs = enable != 0;
     return PRAY_OK;
 }
 

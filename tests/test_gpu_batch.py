@@ -49,6 +49,18 @@ def test_batch_equals_single_scene_renders(ctx):
             for k in range(len(scenes)):
                 assert np.array_equal(imgs[k].view(np.uint32), single[k].view(np.uint32)), \
                     (k, tun, first_diff(imgs[k].ravel(), single[k].ravel()))
+        # the passes enqueued kernel by kernel instead of replaying a graph
+        ctx.use_graphs(False)
+        try:
+            say("batch of 5 scenes without graphs")
+            imgs, ms = ctx.render_pt_batch(sids, [c for _, _, c, _, _ in scenes], [wh for _, _, _, wh, _ in scenes],
+                                           4, 6, 3, [sd for *_, sd in scenes],
+                                           [s.info()["background"] for s, *_ in scenes], 1e-9)
+        finally:
+            ctx.use_graphs(True)
+        assert ms > 0 and ctx.last_render_rays() == single_rays
+        for k in range(len(scenes)):
+            assert np.array_equal(imgs[k].view(np.uint32), single[k].view(np.uint32)), k
     finally:
         for sid in sids:
             ctx.free_scene(sid)

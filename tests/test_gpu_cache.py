@@ -169,3 +169,32 @@ def test_trained_cache_lowers_error_at_equal_samples(ctx):
     finally:
         c.close()
         ctx.free_scene(sid)
+
+
+def test_cache_frames_without_graphs_equal_graph_replays(ctx):
+    """The cache tracer's frames enqueued kernel by kernel (pray_use_graphs
+    off, as profilers see them) give bitwise the images, losses and weights of
+    the captured-graph replays, with the cache on after two steps."""
+    s, t, cam = cornell_small()
+    sid = ctx.upload(t)
+    out = []
+    try:
+        bg = s.info()["background"]
+        for graphs in (True, False):
+            ctx.use_graphs(graphs)
+            c = P.Cache(ctx, seed=5, max_batch=1 << 12)
+            try:
+                say(f"cache frames, graphs {graphs}")
+                hdr, _, stats = c.render_pt(sid, cam, 96, 72, 8, 8, 3, 11, bg, 1e-9, train_paths=1 << 12,
+                                            cache_bounce=1, min_steps=2)
+                out.append((hdr, stats, c.params()))
+            finally:
+                c.close()
+    finally:
+        ctx.use_graphs(True)
+        ctx.free_scene(sid)
+    (h0, s0, w0), (h1, s1, w1) = out
+    assert s0[:, 1].any()
+    assert np.array_equal(h0.view(np.uint32), h1.view(np.uint32))
+    assert np.array_equal(s0.view(np.uint32), s1.view(np.uint32))
+    assert np.array_equal(w0.view(np.uint32), w1.view(np.uint32))
