@@ -71,12 +71,19 @@ std::unique_ptr<KDTreeElement> KDTree::buildRecursive(std::vector<uint32_t> &tri
     std::vector<uint32_t> leftIdx(triIndices.begin(), triIndices.begin() + half);
     std::vector<uint32_t> rightIdx(triIndices.begin() + half, triIndices.end());
 
+    // Children as OpenMP tasks near the root, plain calls below (an undeferred
+    // task would run in place anyway, at the cost of a task descriptor).
     std::unique_ptr<KDTreeElement> l, r;
-#pragma omp task shared(l, leftIdx) if (depth < TASK_DEPTH)
-    l = buildRecursive(leftIdx, depth + 1);
-#pragma omp task shared(r, rightIdx) if (depth < TASK_DEPTH)
-    r = buildRecursive(rightIdx, depth + 1);
+    if (depth < TASK_DEPTH) {
+#pragma omp task shared(l, leftIdx)
+        l = buildRecursive(leftIdx, depth + 1);
+#pragma omp task shared(r, rightIdx)
+        r = buildRecursive(rightIdx, depth + 1);
 #pragma omp taskwait
+    } else {
+        l = buildRecursive(leftIdx, depth + 1);
+        r = buildRecursive(rightIdx, depth + 1);
+    }
 
     auto node = std::make_unique<KDTreeElement>();
     node->boundingBox = nodeBB;
@@ -282,11 +289,16 @@ struct SahBuilder {
         std::vector<SahEvent>().swap(br);
 
         std::unique_ptr<KDTreeElement> l, r;
-#pragma omp task shared(l, tl, le, VL, p) if (depth < KDTree::TASK_DEPTH)
-        l = rec(tl, VL, le, p, depth + 1);
-#pragma omp task shared(r, tr, re, VR, p) if (depth < KDTree::TASK_DEPTH)
-        r = rec(tr, VR, re, p, depth + 1);
+        if (depth < KDTree::TASK_DEPTH) {
+#pragma omp task shared(l, tl, le, VL, p)
+            l = rec(tl, VL, le, p, depth + 1);
+#pragma omp task shared(r, tr, re, VR, p)
+            r = rec(tr, VR, re, p, depth + 1);
 #pragma omp taskwait
+        } else {
+            l = rec(tl, VL, le, p, depth + 1);
+            r = rec(tr, VR, re, p, depth + 1);
+        }
 
         auto node = std::make_unique<KDTreeElement>();
         node->boundingBox = b;
@@ -469,12 +481,16 @@ std::unique_ptr<KDTreeElement> KDTreeBinnedSAH::rec(std::vector<uint32_t> &&refs
     lc.max[bestA] = bestP;
     rc.min[bestA] = bestP;
     std::unique_ptr<KDTreeElement> l, r;
-    const bool task = L.size() + R.size() > 4096;
-#pragma omp task shared(l, L, lc) if (task)
-    l = rec(std::move(L), lc, depth + 1);
-#pragma omp task shared(r, R, rc) if (task)
-    r = rec(std::move(R), rc, depth + 1);
+    if (L.size() + R.size() > 4096) {
+#pragma omp task shared(l, L, lc)
+        l = rec(std::move(L), lc, depth + 1);
+#pragma omp task shared(r, R, rc)
+        r = rec(std::move(R), rc, depth + 1);
 #pragma omp taskwait
+    } else {
+        l = rec(std::move(L), lc, depth + 1);
+        r = rec(std::move(R), rc, depth + 1);
+    }
     auto node = std::make_unique<KDTreeElement>();
     node->boundingBox = cell;
     node->left = std::move(l);
