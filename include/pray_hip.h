@@ -146,6 +146,7 @@ PRAY_API int pray_scene_layout(pray_ctx *ctx, int scene_id, uint64_t *sizes, uin
  *                  lanes once at least T lanes wait at a leaf (or no lane
  *                  has node work), only the node lanes otherwise, so the
  *                  wave issues one of the two code paths per iteration
+ *                  (packed layout only)
  *   [7] node_levels 1: a node step takes one tree level (one 16-byte
  *                  children pair); 2: it takes two levels from one 32-byte
  *                  record of the node and its children (one cache line per

@@ -141,14 +141,15 @@ typedef void (*TraceKernel)(DevScene, TraceArgs);
 // 768*K bytes; with 160 KB of LDS per CU, K = 6 is the largest ring that
 // leaves room for 32 resident waves (the VGPR limit of the ILP-2 kernels)
 // and K = 5 for 36 (the ILP-1 kernels).
-template <bool ANY, int LAYOUT>
+template <bool ANY, int LAYOUT, bool BATCH>
 TraceKernel pick_kernel_kl(int K, int ilp) {
     switch (K) {
-    case 4: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 4, 2> : prt::k_trace<ANY, LAYOUT, 4, 1>;
-    case 5: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 5, 2> : prt::k_trace<ANY, LAYOUT, 5, 1>;
-    case 6: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 6, 2> : prt::k_trace<ANY, LAYOUT, 6, 1>;
-    case 16: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 16, 2> : prt::k_trace<ANY, LAYOUT, 16, 1>;
-    default: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 8, 2> : prt::k_trace<ANY, LAYOUT, 8, 1>;
+    case 4: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 4, 2, false, BATCH> : prt::k_trace<ANY, LAYOUT, 4, 1, false, BATCH>;
+    case 5: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 5, 2, false, BATCH> : prt::k_trace<ANY, LAYOUT, 5, 1, false, BATCH>;
+    case 6: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 6, 2, false, BATCH> : prt::k_trace<ANY, LAYOUT, 6, 1, false, BATCH>;
+    case 16:
+        return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 16, 2, false, BATCH> : prt::k_trace<ANY, LAYOUT, 16, 1, false, BATCH>;
+    default: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 8, 2, false, BATCH> : prt::k_trace<ANY, LAYOUT, 8, 1, false, BATCH>;
     }
 }
 
@@ -162,26 +163,32 @@ TraceKernel pick_kernel_multi(int K, int ilp) {
     }
 }
 
-template <bool ANY>
+template <bool ANY, bool BATCH>
 TraceKernel pick_kernel_q(int K, int ilp) {
     switch (K) {
-    case 4: return ilp == 2 ? prt::k_trace_q<ANY, 4, 2> : prt::k_trace_q<ANY, 4, 1>;
-    case 5: return ilp == 2 ? prt::k_trace_q<ANY, 5, 2> : prt::k_trace_q<ANY, 5, 1>;
-    case 6: return ilp == 2 ? prt::k_trace_q<ANY, 6, 2> : prt::k_trace_q<ANY, 6, 1>;
-    case 16: return ilp == 2 ? prt::k_trace_q<ANY, 16, 2> : prt::k_trace_q<ANY, 16, 1>;
-    default: return ilp == 2 ? prt::k_trace_q<ANY, 8, 2> : prt::k_trace_q<ANY, 8, 1>;
+    case 4: return ilp == 2 ? prt::k_trace_q<ANY, 4, 2, BATCH> : prt::k_trace_q<ANY, 4, 1, BATCH>;
+    case 5: return ilp == 2 ? prt::k_trace_q<ANY, 5, 2, BATCH> : prt::k_trace_q<ANY, 5, 1, BATCH>;
+    case 6: return ilp == 2 ? prt::k_trace_q<ANY, 6, 2, BATCH> : prt::k_trace_q<ANY, 6, 1, BATCH>;
+    case 16: return ilp == 2 ? prt::k_trace_q<ANY, 16, 2, BATCH> : prt::k_trace_q<ANY, 16, 1, BATCH>;
+    default: return ilp == 2 ? prt::k_trace_q<ANY, 8, 2, BATCH> : prt::k_trace_q<ANY, 8, 1, BATCH>;
     }
 }
 
-// two_level: the scene has two-level records and the tuning asks for them
-// (packed layout).
-TraceKernel pick_kernel(bool any, int layout, int K, int ilp, bool two_level) {
-    if (two_level && layout == PRAY_LAYOUT_PACKED) return any ? pick_kernel_q<true>(K, ilp) : pick_kernel_q<false>(K, ilp);
-    if (any)
-        return layout == PRAY_LAYOUT_PACKED ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED>(K, ilp)
-                                            : pick_kernel_kl<true, PRAY_LAYOUT_INDEXED>(K, ilp);
-    return layout == PRAY_LAYOUT_PACKED ? pick_kernel_kl<false, PRAY_LAYOUT_PACKED>(K, ilp)
-                                        : pick_kernel_kl<false, PRAY_LAYOUT_INDEXED>(K, ilp);
+// The kernel of a tuning: two_level when the scene has two-level records and
+// the tuning asks for them; batch for leaf batching (both packed layout only).
+TraceKernel pick_kernel(bool any, int layout, int K, int ilp, bool two_level, bool batch) {
+    if (layout != PRAY_LAYOUT_PACKED)
+        return any ? pick_kernel_kl<true, PRAY_LAYOUT_INDEXED, false>(K, ilp)
+                   : pick_kernel_kl<false, PRAY_LAYOUT_INDEXED, false>(K, ilp);
+    if (two_level) {
+        if (batch) return any ? pick_kernel_q<true, true>(K, ilp) : pick_kernel_q<false, true>(K, ilp);
+        return any ? pick_kernel_q<true, false>(K, ilp) : pick_kernel_q<false, false>(K, ilp);
+    }
+    if (batch)
+        return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, true>(K, ilp)
+                   : pick_kernel_kl<false, PRAY_LAYOUT_PACKED, true>(K, ilp);
+    return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, false>(K, ilp)
+               : pick_kernel_kl<false, PRAY_LAYOUT_PACKED, false>(K, ilp);
 }
 
 // Resident blocks per CU of a trace kernel with `wpw` waves per block and
@@ -280,7 +287,7 @@ static int enqueue_trace_ex(pray_ctx *c, const SceneRec &sc, const float *rays, 
     }
     const Tuning &tu = sc.tune;
     TraceKernel k = pick_kernel(mode != PRAY_TRACE_CLOSEST, tu.layout, tu.stack_lds, tu.leaf_ilp,
-                                tu.node_levels == 2 && sc.d.quads);
+                                tu.node_levels == 2 && sc.d.quads, tu.leaf_batch > 0);
     Launch L = launch_shape(c, k, tu, n);
     const uint32_t depth = std::max<uint32_t>(1, sc.d.max_depth);
     uint32_t *work = c->work;
@@ -465,7 +472,7 @@ int validate_tree(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const ui
 
 int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n) {
     TraceKernel k = pick_kernel(false, sc.tune.layout, sc.tune.stack_lds, sc.tune.leaf_ilp,
-                                sc.tune.node_levels == 2 && sc.d.quads);
+                                sc.tune.node_levels == 2 && sc.d.quads, sc.tune.leaf_batch > 0);
     return ensure_spill(c, (size_t)launch_shape(c, k, sc.tune, n).lanes * std::max<uint32_t>(1, sc.d.max_depth) *
                                sizeof(uint4));
 }
@@ -664,6 +671,9 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
     if (wg_waves >= 0 && wg_waves != 1 && wg_waves != 2 && wg_waves != 4)
         return fail(c, PRAY_ERR_INVALID, "wg_waves must be 1, 2 or 4");
     if (leaf_batch > WAVE) return fail(c, PRAY_ERR_INVALID, "leaf_batch must be in 0..64");
+    if ((leaf_batch >= 0 ? leaf_batch : sc->tune.leaf_batch) > 0 &&
+        (layout >= 0 ? layout : sc->tune.layout) != PRAY_LAYOUT_PACKED)
+        return fail(c, PRAY_ERR_INVALID, "leaf batching traces the packed layout");
     if (node_levels >= 0 && node_levels != 1 && node_levels != 2)
         return fail(c, PRAY_ERR_INVALID, "node_levels must be 1 or 2");
     if ((node_levels >= 0 ? node_levels : sc->tune.node_levels) == 2) {

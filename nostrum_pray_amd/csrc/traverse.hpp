@@ -123,7 +123,10 @@ __device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, 
 // A workgroup holds 1..MAX_WG_WAVES waves that never synchronise with each
 // other; each has its own LDS ring in the dynamic shared memory the launch
 // sizes (ring_lds_bytes per wave).
-template <bool ANY, int LAYOUT, int K, int ILP, bool MULTI = false>
+// BATCH: leaf batching (TraceArgs::leaf_batch > 0) compiled in; without it
+// every iteration takes node and leaf steps and the loop carries no batching
+// code.
+template <bool ANY, int LAYOUT, int K, int ILP, bool MULTI = false, bool BATCH = false>
 static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s, TraceArgs A) {
     static_assert(K >= 2 && K <= 16, "LDS ring of 2..16 entries");
     static_assert(!MULTI || LAYOUT == 1, "multi-scene launches use the packed layout");
@@ -253,14 +256,13 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
         // Which steps this iteration takes (wave-uniform): both, or with
         // leaf batching the leaf steps once enough lanes wait at a leaf.
         bool run_node = true, run_leaf = true;
-        if (A.leaf_batch) {
+        if constexpr (BATCH) {
             const uint64_t at_leaf = __ballot(state == ST_LEAF);
             const uint64_t at_node = __ballot(state == ST_NODE);
             run_leaf = at_leaf != 0ull && ((uint32_t)__popcll(at_leaf) >= A.leaf_batch || at_node == 0ull);
             run_node = !run_leaf;
         }
         bool need_pop = false;
-        const uint32_t st_in = state;
         // ---------------------------------------------------- node step --
         if (run_node && state == ST_NODE) {
             const uint32_t axis = w & 3u;
@@ -311,7 +313,7 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
         // (without leaf batching also for lanes that reached their leaf in
         // this iteration's node step: the leaf branch runs anyway while any
         // lane is in a leaf)
-        if (run_leaf && !need_pop && state == ST_LEAF && (A.leaf_batch == 0 || st_in == ST_LEAF)) {
+        if (run_leaf && !need_pop && state == ST_LEAF) {
             uint32_t ref[ILP];
             float4 ta[ILP], tb[ILP], tc[ILP];
 #pragma unroll
@@ -464,7 +466,7 @@ static __global__ void k_build_quads(const uint4 *pairs, uint32_t n_pairs, uint4
 // gets the same result and work counts.  Popping a far child of the first
 // level loads its 16-byte half and takes its one level.  Each node step
 // touches one cache line instead of one per level.
-template <bool ANY, int K, int ILP>
+template <bool ANY, int K, int ILP, bool BATCH = false>
 static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene s, TraceArgs A) {
     static_assert(K >= 2 && K <= 16, "LDS ring of 2..16 entries");
     extern __shared__ uint32_t ring_lds[];
@@ -591,14 +593,13 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene
         }
 
         bool run_node = true, run_leaf = true;
-        if (A.leaf_batch) {
+        if constexpr (BATCH) {
             const uint64_t at_leaf = __ballot(state == ST_LEAF);
             const uint64_t at_node = __ballot(state == ST_NODE || state == ST_HALF);
             run_leaf = at_leaf != 0ull && ((uint32_t)__popcll(at_leaf) >= A.leaf_batch || at_node == 0ull);
             run_node = !run_leaf;
         }
         bool need_pop = false;
-        const uint32_t st_in = state;
         // ---------------------------------------------------- node step --
         if (run_node && (state == ST_NODE || state == ST_HALF)) {
             // The child whose level comes next: its half record.
@@ -683,7 +684,7 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene
             }
         }
         // ---------------------------------------------------- leaf step --
-        if (run_leaf && !need_pop && state == ST_LEAF && (A.leaf_batch == 0 || st_in == ST_LEAF)) {
+        if (run_leaf && !need_pop && state == ST_LEAF) {
             uint32_t ref[ILP];
             float4 ta[ILP], tb[ILP], tc[ILP];
 #pragma unroll

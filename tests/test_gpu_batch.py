@@ -81,7 +81,8 @@ def test_autotune_never_changes_hits(ctx):
         assert tun["leaf_batch"] in (0, 16, 24, 32)
         assert ctx.get_tuning(sid) == tun
         settings = [dict(), dict(stack_lds=4), dict(stack_lds=16, leaf_ilp=1), dict(fetch_min=16, occupancy=50),
-                    dict(fetch_min=64, occupancy=75), dict(layout=P.LAYOUT_INDEXED), dict(stack_lds=6),
+                    dict(fetch_min=64, occupancy=75), dict(layout=P.LAYOUT_INDEXED, leaf_batch=0, node_levels=1),
+                    dict(stack_lds=6),
                     dict(stack_lds=5, occupancy=10), dict(wg_waves=4), dict(wg_waves=2, stack_lds=16, leaf_ilp=2),
                     dict(layout=P.LAYOUT_PACKED, leaf_batch=16, stack_lds=8), dict(leaf_batch=48, occupancy=10),
                     dict(leaf_batch=1, wg_waves=4, fetch_min=1), dict(leaf_batch=0)]
@@ -94,6 +95,8 @@ def test_autotune_never_changes_hits(ctx):
             ctx.set_tuning(sid, wg_waves=3)
         with pytest.raises(P.PrayError):
             ctx.set_tuning(sid, leaf_batch=65)
+        with pytest.raises(P.PrayError):  # leaf batching traces the packed layout
+            ctx.set_tuning(sid, layout=P.LAYOUT_INDEXED, leaf_batch=16)
         # a second scene of the same size bucket takes the bucket's setting
         s2 = P.Scene.random(22, 45000)
         sid2 = ctx.upload(s2.build(1))

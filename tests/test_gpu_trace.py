@@ -47,7 +47,7 @@ def scene(name):
 # other trees refuse them.
 SETTINGS = [(P.LAYOUT_PACKED, 8, 1, 1, 24, 0, 1), (P.LAYOUT_PACKED, 8, 1, 2, 16, 16, 1),
             (P.LAYOUT_PACKED, 5, 2, 2, 16, 0, 1), (P.LAYOUT_INDEXED, 8, 4, 1, 32, 0, 1),
-            (P.LAYOUT_INDEXED, 6, 1, 2, 16, 24, 1), (P.LAYOUT_PACKED, 6, 1, 2, 48, 0, 1),
+            (P.LAYOUT_INDEXED, 6, 1, 2, 16, 0, 1), (P.LAYOUT_PACKED, 6, 1, 2, 48, 24, 1),
             (P.LAYOUT_PACKED, 4, 2, 2, 16, 32, 1), (P.LAYOUT_PACKED, 16, 4, 1, 48, 64, 1),
             (P.LAYOUT_PACKED, 5, 1, 1, 1, 1, 1), (P.LAYOUT_PACKED, 8, 1, 2, 16, 16, 2),
             (P.LAYOUT_PACKED, 8, 1, 1, 24, 0, 2), (P.LAYOUT_PACKED, 4, 2, 2, 32, 32, 2),

@@ -47,7 +47,7 @@ def test_terrain2048_full_stream_equals_oracle(ctx):
         first = None
         for tun in (BENCH_TUNING, dict(BENCH_TUNING, leaf_ilp=1, fetch_min=24, leaf_batch=0),
                     dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0),
-                    dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=2, fetch_min=32, wg_waves=2, leaf_batch=32,
+                    dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=2, fetch_min=32, wg_waves=2, leaf_batch=0,
                          node_levels=1),
                     dict(BENCH_TUNING, node_levels=2), dict(BENCH_TUNING, leaf_batch=0, leaf_ilp=1, node_levels=2)):
             ctx.set_tuning(sid, **tun)
