@@ -95,8 +95,9 @@ def main():
     ap.add_argument("--traversal-cost", type=float, default=2.0, help="SAH traversal cost (reference: 5)")
     ap.add_argument("--no-autotune", action="store_true", help="trace with the default tuning")
     ap.add_argument("--tuning", default="",
-                    help="explicit traversal tuning instead of the autotuner: "
-                         "layout,stack_lds,leaf_ilp,fetch_min,occupancy")
+                    help="explicit traversal tuning instead of the autotuner: comma-separated values of "
+                         "layout,stack_lds,leaf_ilp,fetch_min,occupancy,wg_waves,leaf_batch,node_levels "
+                         "(a prefix of them)")
     ap.add_argument("--no-ref-tree", action="store_true",
                     help="skip timing the same rays over the tree built with the reference's parameters")
     ap.add_argument("--tree", choices=("cpu-sah", "cpu-sah-events", "gpu-sah"), default="cpu-sah",
@@ -113,10 +114,13 @@ def main():
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-RMSE measurement")
     ap.add_argument("--cache-size", type=int, default=1024, help="image side of the radiance-cache measurement")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples per pixel of the plain reference image")
-    ap.add_argument("--workload", choices=("terrain", "batch", "io", "cache", "build"), default="terrain",
+    ap.add_argument("--workload", choices=("terrain", "batch", "io", "cache", "build", "pt"), default="terrain",
                     help="terrain: the closest-hit stream (default); batch: 64 scenes path traced in one wavefront; "
                          "io: flat tree files, progressive path tracing and the CLI; cache: the radiance-cache "
-                         "equal-RMSE measurement alone; build: the KD-tree builds on the GPU")
+                         "equal-RMSE measurement alone; build: the KD-tree builds on the GPU; pt: one plain and "
+                         "one radiance-cache render of the Cornell room (--cache-size, --pt-spp, --pt-frames)")
+    ap.add_argument("--pt-spp", type=int, default=16, help="samples per pixel of the pt workload's plain render")
+    ap.add_argument("--pt-frames", type=int, default=16, help="frames of the pt workload's cache render")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg of the default run")
     ap.add_argument("--no-graphs", action="store_true",
@@ -138,6 +142,9 @@ def main():
         return
     if args.workload == "build":
         build_workload(P, args)
+        return
+    if args.workload == "pt":
+        pt_workload(P, args)
         return
     if args.workload == "cache":
         ctx = P.Context(0)
@@ -162,8 +169,7 @@ def main():
 
     tuning = None
     if args.tuning:
-        vals = [int(x) for x in args.tuning.split(",")]
-        ctx.set_tuning(sid, **dict(zip(P.TUNING_FIELDS, vals)))
+        ctx.set_tuning(sid, **parse_tuning(P, args.tuning))
     elif args.layout >= 0:
         ctx.set_tuning(sid, layout=args.layout)
     elif not args.no_autotune:
@@ -223,19 +229,25 @@ def main():
     log(f"closest-hit {ms:.3f} ms/step  {mrays:.1f} Mrays/s  (wall {ms_wall:.3f} ms/step)")
     closest_idx, closest_t = d_idx.numpy(), d_t.numpy()
 
-    # The traversal kernel alone (HIP events around its launch, on the stream
-    # it runs on), without the counter reset in front of it.
+    # The roofline's kernel time is the timed region's own: HIP events on the
+    # kernel's stream around the K back-to-back launches, divided by K (each
+    # launch is the traversal kernel and a one-wave reset of its 64 ray
+    # counters, about 3 us, in front of it).  Beside it, events around each
+    # traversal kernel alone, enqueued back to back and read once at the end:
+    # kernels timed that way run 1-2% longer in a profiler's kernel trace than
+    # those of the timed region, so the region is the figure used.
     def kernel_time(mode, lim=None):
         ctx.kernel_timing(True)
-        kms = []
         for _ in range(args.steps):
             ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, mode, args.det_eps, d_limit2=lim)
-            kms.append(ctx.last_kernel_ms())
+        kms, cnt = ctx.timed_kernels_ms()
         ctx.kernel_timing(False)
-        return float(np.mean(kms))
+        assert cnt == min(args.steps, 64), cnt
+        return float(kms)
 
-    kernel_ms = kernel_time(P.TRACE_CLOSEST)
-    log(f"traversal kernel {kernel_ms:.3f} ms/launch (step {ms:.3f} ms)")
+    kernel_ms = ms
+    isolated_ms = kernel_time(P.TRACE_CLOSEST)
+    log(f"traversal kernel {kernel_ms:.3f} ms/launch (events around each kernel alone {isolated_ms:.3f} ms)")
     achieved = bytes_per_ray * n / (kernel_ms * 1e-3) / 1e9
 
     # Any-hit over the same stream (unlimited distance), reported beside with
@@ -246,7 +258,8 @@ def main():
     ctx.sync()
     acnt = d_cnt.numpy().astype(np.int64).sum(0) / n
     any_bytes = 24 + 8 + 8 * acnt[0] + 40 * acnt[1]
-    any_kernel_ms = kernel_time(P.TRACE_ANY, d_lim.ptr)
+    any_kernel_ms = ms_any
+    any_isolated_ms = kernel_time(P.TRACE_ANY, d_lim.ptr)
     any_achieved = any_bytes * n / (any_kernel_ms * 1e-3) / 1e9
     log(f"any-hit {ms_any:.3f} ms/step  {n / (ms_any * 1e-3) / 1e6:.1f} Mrays/s  kernel {any_kernel_ms:.3f} ms  "
         f"nodes/ray {acnt[0]:.1f}  refs/ray {acnt[1]:.1f}  {any_achieved:.0f} GB/s")
@@ -259,13 +272,9 @@ def main():
         ref_eps = {"det_eps": 1e-4, "hit_fraction": round(float((d_idx.numpy() >= 0).mean()), 4),
                    "ms_per_step": round(ms_ref_eps, 4), "Mrays_per_s": round(n / (ms_ref_eps * 1e-3) / 1e6, 2)}
         log(f"det_eps 1e-4: {ref_eps}")
-    traffic = None
-    # HBM traffic of this kernel/workload from the committed rocprofv3 PMC
-    # summary (FETCH_SIZE/WRITE_SIZE calibrated on a plain copy)
-    pmc = os.path.join(ROOT, "profiles", f"trace_closest_terrain{args.grid}_{tree_tag}_pmc.json")
-    if os.path.exists(pmc) and args.rays == 1 << 24:
-        with open(pmc) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    # HBM traffic of this kernel on this workload from the committed
+    # rocprofv3 PMC summary (FETCH_SIZE/WRITE_SIZE calibrated on a plain copy)
+    traffic = pmc_traffic(args, tree_tag, tun) if args.rays == 1 << 24 else None
 
     # The same rays over the SAH trees built the other ways (each with its
     # own autotuned setting): identical hits.
@@ -369,14 +378,17 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "k_trace (closest hit)",
+            "kernel": ("k_trace_q (closest hit, two tree levels per node step)" if tun["node_levels"] == 2
+                       else "k_trace (closest hit)"),
             "kernel_ms": round(kernel_ms, 4),
+            "kernel_ms_events_per_kernel": round(isolated_ms, 4),
             "bytes_per_ray": round(float(bytes_per_ray), 1),
             "nodes_per_ray": round(float(nodes_per_ray), 2),
             "refs_per_ray": round(float(refs_per_ray), 2),
         },
         "any_hit": {"ms_per_step": round(ms_any, 4), "Mrays_per_s": round(n / (ms_any * 1e-3) / 1e6, 2),
-                    "kernel_ms": round(any_kernel_ms, 4), "bytes_per_ray": round(float(any_bytes), 1),
+                    "kernel_ms": round(any_kernel_ms, 4), "kernel_ms_events_per_kernel": round(any_isolated_ms, 4),
+                    "bytes_per_ray": round(float(any_bytes), 1),
                     "nodes_per_ray": round(float(acnt[0]), 2), "refs_per_ray": round(float(acnt[1]), 2),
                     "achieved_GBs": round(any_achieved, 1), "frac": round(any_achieved / HBM_PEAK_GBS, 4)},
         "reference_det_eps": ref_eps,
@@ -757,6 +769,32 @@ def gpu_sah_scene(P, ctx, scene, args):
                            traversal_cost=args.traversal_cost)
 
 
+# Tuning fields that select the traversal kernel's instance (the others only
+# change its launch or refill threshold).
+KERNEL_FIELDS = ("layout", "stack_lds", "leaf_ilp", "node_levels")
+
+
+def pmc_traffic(args, tree_tag, tun):
+    """HBM bytes per launch of the traversal kernel instance `tun` selects,
+    from profiles/trace_closest_terrain<grid>_<tree>_pmc.json (one entry per
+    profiled tuning), or None when that instance was not profiled."""
+    path = os.path.join(ROOT, "profiles", f"trace_closest_terrain{args.grid}_{tree_tag}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        entries = json.load(fh).get("entries", [])
+    for e in entries:
+        et = e.get("tuning", {})
+        if all(et.get(k) == tun[k] for k in KERNEL_FIELDS) and (et.get("leaf_batch", 0) > 0) == (tun["leaf_batch"] > 0):
+            return e.get("hbm_bytes_per_launch")
+    return None
+
+
+def parse_tuning(P, text):
+    """--tuning's comma-separated values as set_tuning keywords."""
+    return dict(zip(P.TUNING_FIELDS, [int(x) for x in text.split(",")]))
+
+
 def P_camera(P, scene, side):
     """The scene's camera at a square image of `side` pixels."""
     inf = scene.info()
@@ -780,6 +818,51 @@ def equal_rmse_time(points, target):
     k = min(max(k, 1), len(pts) - 1)
     slope = (lt[k] - lt[k - 1]) / (lr[k] - lr[k - 1])
     return float(np.exp(lt[k - 1] + slope * (np.log(target) - lr[k - 1])))
+
+
+def pt_workload(P, args):
+    """The Cornell room (100k triangles, depth 8, roulette from bounce 3) at
+    --cache-size^2: one warm-up and one timed plain render of --pt-spp
+    samples per pixel, then one radiance-cache render of --pt-frames frames.
+    One JSON line with the times and rays; meant for profilers, which see
+    the kernels of each pass (with --no-graphs, launched one by one)."""
+    ctx = P.Context(0)
+    ctx.use_graphs(not args.no_graphs)
+    w = h = args.cache_size
+    scene = P.Scene.cornell(1, 100_000)
+    tree = scene.build(2, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+    sid = ctx.upload(tree)
+    cam, bg = scene.camera(), scene.info()["background"]
+    depth, rr = 8, 3
+    tun = ctx.get_tuning(sid)
+    if args.tuning:
+        ctx.set_tuning(sid, **parse_tuning(P, args.tuning))
+        tun = ctx.get_tuning(sid)
+    ctx.render_pt(sid, cam, w, h, 1, depth, rr, 7, bg, 1e-9, want_rgb=False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    img, _ = ctx.render_pt(sid, cam, w, h, args.pt_spp, depth, rr, 7, bg, 1e-9, want_rgb=False)
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    rays = ctx.last_render_rays()
+    log(f"pt: plain {args.pt_spp} spp {plain_ms:.1f} ms, {rays} rays, {rays / plain_ms / 1e3:.1f} Mrays/s")
+    c = P.Cache(ctx, seed=3, learning_rate=1e-2, max_batch=1 << 16)
+    t0 = time.perf_counter()
+    _, _, st = c.render_pt(sid, cam, w, h, args.pt_frames, depth, rr, 7, bg, 1e-9, train_paths=1 << 16,
+                           cache_bounce=1, min_steps=4)
+    cache_ms = (time.perf_counter() - t0) * 1e3
+    c.close()
+    log(f"pt: cache {args.pt_frames} frames {cache_ms:.1f} ms, {int(st[:, 1].sum())} with the cache")
+    ctx.free_scene(sid)
+    ctx.close()
+    print(json.dumps({"metric": "path tracing, Cornell room", "n_gpus": 1, "data": "synthetic",
+                      "config": {"workload": f"Cornell room, {scene.num_tris} triangles, {w}x{h}, depth {depth}, "
+                                             f"roulette from bounce {rr}"},
+                      "graphs": not args.no_graphs, "tuning": tun,
+                      "plain": {"spp": args.pt_spp, "ms": round(plain_ms, 2), "rays": rays,
+                                "Mrays_per_s": round(rays / plain_ms / 1e3, 2),
+                                "checksum": float(img.astype(np.float64).sum())},
+                      "cache": {"frames": args.pt_frames, "ms": round(cache_ms, 2),
+                                "frames_with_cache": int(st[:, 1].sum())}}), flush=True)
 
 
 def radiance_cache_leg(P, ctx, args):

@@ -386,10 +386,17 @@ PRAY_API int pray_event_elapsed(pray_ctx *ctx, int slot0, int slot1, float *ms);
 PRAY_API int pray_use_graphs(pray_ctx *ctx, int enable);
 /* Kernel timing: with enable = 1, every later trace that is not being
  * captured into a graph records events around its traversal kernel alone
- * (not the counter reset before it); pray_last_kernel_ms synchronises and
- * returns the last such kernel's duration. */
+ * (not the counter reset before it), on the context's stream, without
+ * synchronising; the events of the last 64 such kernels are kept.
+ * Switching timing on (from off) forgets earlier measurements.
+ * pray_last_kernel_ms synchronises and returns the last such kernel's
+ * duration; pray_timed_kernels_ms synchronises and returns the mean
+ * duration of the last min(64, timed) kernels and their number.  Traces
+ * enqueued back to back and read once at the end give the kernel's
+ * duration with its launches pipelined (no host round trip per launch). */
 PRAY_API int pray_kernel_timing(pray_ctx *ctx, int enable);
 PRAY_API int pray_last_kernel_ms(pray_ctx *ctx, float *ms);
+PRAY_API int pray_timed_kernels_ms(pray_ctx *ctx, float *mean_ms, uint32_t *count);
 
 #ifdef __cplusplus
 }

@@ -117,6 +117,7 @@ def libraries():
         "pray_scene_layout": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
         "pray_kernel_timing": (C.c_int, [_vp, C.c_int]),
         "pray_last_kernel_ms": (C.c_int, [_vp, C.POINTER(_f32)]),
+        "pray_timed_kernels_ms": (C.c_int, [_vp, C.POINTER(_f32), C.POINTER(_u32)]),
         "pray_event_record": (C.c_int, [_vp, C.c_int]),
         "pray_event_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_f32)]),
     }
@@ -504,6 +505,13 @@ class Context:
         ms = _f32()
         self._check(self._lib.pray_last_kernel_ms(self._h, C.byref(ms)))
         return ms.value
+
+    def timed_kernels_ms(self):
+        """(mean ms, count) over the last traversal kernels timed, at most 64
+        (pray_timed_kernels_ms)."""
+        ms, cnt = _f32(), _u32()
+        self._check(self._lib.pray_timed_kernels_ms(self._h, C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
 
     def last_render_rays(self):
         r = _u64()

@@ -46,9 +46,12 @@ struct pray_ctx {
     int next_id = 1;
     hipEvent_t events[16] = {};   // pray_event_record slots
     hipEvent_t ievents[4] = {};   // internal timing (autotuner, batch)
-    hipEvent_t kevents[2] = {};   // around the last traversal kernel (pray_kernel_timing)
+    // pray_kernel_timing: event pairs around the traversal kernels of the
+    // last KT_RING timed traces, a ring indexed by kt_total % KT_RING
+    static constexpr int KT_RING = 64;
+    hipEvent_t kevents[2 * KT_RING] = {};
     bool kernel_timing = false;
-    bool kernel_timed = false;    // kevents hold a measurement
+    uint64_t kt_total = 0;        // traversal kernels timed since timing was enabled
     hipStream_t side = nullptr;   // second stream for work overlapped inside graphs
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     bool use_graphs = true;       // path tracers replay captured hipGraphs (pray_use_graphs)

@@ -332,11 +332,12 @@ static int enqueue_trace_ex(pray_ctx *c, const SceneRec &sc, const float *rays, 
         HIPC(c, hipStreamIsCapturing(st, &cs));
         timed = cs == hipStreamCaptureStatusNone;
     }
-    if (timed) HIPC(c, hipEventRecord(c->kevents[0], st));
+    const int kslot = (int)(c->kt_total % pray_ctx::KT_RING);
+    if (timed) HIPC(c, hipEventRecord(c->kevents[2 * kslot], st));
     hipLaunchKernelGGL(k, dim3(L.blocks), dim3(L.threads), L.lds, st, sc.d, a);
     if (timed) {
-        HIPC(c, hipEventRecord(c->kevents[1], st));
-        c->kernel_timed = true;
+        HIPC(c, hipEventRecord(c->kevents[2 * kslot + 1], st));
+        c->kt_total++;
     }
     HIPC(c, hipGetLastError());
     return PRAY_OK;
@@ -1118,18 +1119,38 @@ int pray_use_graphs(pray_ctx *c, int enable) {
 int pray_kernel_timing(pray_ctx *c, int enable) {
     if (!c) return PRAY_ERR_INVALID;
     HIPC(c, hipSetDevice(c->device));
-    for (int k = 0; k < 2; k++)
-        if (!c->kevents[k]) HIPC(c, hipEventCreate(&c->kevents[k]));
+    for (hipEvent_t &e : c->kevents)
+        if (!e) HIPC(c, hipEventCreate(&e));
+    if (enable && !c->kernel_timing) c->kt_total = 0;
     c->kernel_timing = enable != 0;
     return PRAY_OK;
 }
 
 int pray_last_kernel_ms(pray_ctx *c, float *ms) {
     if (!c || !ms) return fail(c, PRAY_ERR_INVALID, "null argument");
-    if (!c->kernel_timed) return fail(c, PRAY_ERR_INVALID, "no traversal kernel has been timed");
+    if (!c->kt_total) return fail(c, PRAY_ERR_INVALID, "no traversal kernel has been timed");
     HIPC(c, hipSetDevice(c->device));
-    HIPC(c, hipEventSynchronize(c->kevents[1]));
-    HIPC(c, hipEventElapsedTime(ms, c->kevents[0], c->kevents[1]));
+    const int k = (int)((c->kt_total - 1) % pray_ctx::KT_RING);
+    HIPC(c, hipEventSynchronize(c->kevents[2 * k + 1]));
+    HIPC(c, hipEventElapsedTime(ms, c->kevents[2 * k], c->kevents[2 * k + 1]));
+    return PRAY_OK;
+}
+
+int pray_timed_kernels_ms(pray_ctx *c, float *mean_ms, uint32_t *count) {
+    if (!c || !mean_ms || !count) return fail(c, PRAY_ERR_INVALID, "null argument");
+    if (!c->kt_total) return fail(c, PRAY_ERR_INVALID, "no traversal kernel has been timed");
+    HIPC(c, hipSetDevice(c->device));
+    const uint64_t n = std::min<uint64_t>(c->kt_total, pray_ctx::KT_RING);
+    HIPC(c, hipEventSynchronize(c->kevents[2 * ((c->kt_total - 1) % pray_ctx::KT_RING) + 1]));
+    double sum = 0;
+    for (uint64_t i = c->kt_total - n; i < c->kt_total; i++) {
+        const int k = (int)(i % pray_ctx::KT_RING);
+        float ms = 0;
+        HIPC(c, hipEventElapsedTime(&ms, c->kevents[2 * k], c->kevents[2 * k + 1]));
+        sum += ms;
+    }
+    *mean_ms = (float)(sum / (double)n);
+    *count = (uint32_t)n;
     return PRAY_OK;
 }
 

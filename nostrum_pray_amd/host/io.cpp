@@ -1,6 +1,7 @@
 // Scene I/O of the drop-in host (reference src/io.cpp:27-137, main.cpp:93-98).
 // The reference links nlohmann json 2.0.7, tinyobjloader 1.0.x and
 // stb_image_write 1.02; this file implements the parts it uses.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -155,75 +156,42 @@ struct JParser {
 
 // ------------------------------------------------------------------- OBJ --
 
-// Decimal parsing with the arithmetic of tinyobjloader 1.0.x's
-// tryParseDouble (ext/tiny_obj_loader.h): integer digits accumulated as
-// m = m*10 + d, fractional digit k (1-based) added as d * 10^-k from a table
-// of 1, 0.1, ..., 1e-7 (pow(10,-k) beyond), exponent applied as
-// ldexp(m * pow(5, e), e).  The vertex bits of the reference depend on these
-// steps, so they are kept.
-bool objParseDouble(const char *s, const char *s_end, double *result) {
-    if (s >= s_end) return false;
-    double mantissa = 0.0;
-    int exponent = 0;
-    char sign = '+', exp_sign = '+';
-    const char *curr = s;
-    int read = 0;
-    bool more = false;
-    if (*curr == '+' || *curr == '-') {
-        sign = *curr;
-        curr++;
-    } else if (!(*curr >= '0' && *curr <= '9')) {
-        return false;
+// A decimal number of an OBJ token [s, e), computed in the order that
+// decides the reference's vertex bits (its loader, tinyobjloader 1.0.x,
+// ext/tiny_obj_loader.h:459-572): integer digits as m = m*10 + d; the k-th
+// fraction digit adds d * 10^-k, with 10^-k taken from the table below for
+// k < 8 and from pow(10, -k) beyond; an exponent x gives
+// ldexp(m * 5^x, x), i.e. m * 10^x with the power of two applied exactly.
+// Accepted: [+-]digits[.digits][(e|E)[+-]digits]; a token that does not start
+// that way, or whose exponent has no digit, is rejected (false, *result
+// untouched); anything after the number is ignored.
+constexpr double kNegPow10[8] = {1.0, 1e-1, 1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7};
+
+bool objParseDouble(const char *s, const char *e, double *result) {
+    const char *p = s;
+    const auto at_digit = [&p, e] { return p < e && *p >= '0' && *p <= '9'; };
+    double sign = 1.0;
+    if (p < e && (*p == '+' || *p == '-')) sign = *p++ == '-' ? -1.0 : 1.0;
+    if (!at_digit()) return false;
+    double m = 0.0;
+    for (; at_digit(); ++p) {
+        m *= 10;
+        m += *p - '0';
     }
-    more = (curr != s_end);
-    while (more && *curr >= '0' && *curr <= '9') {
-        mantissa *= 10;
-        mantissa += static_cast<int>(*curr - '0');
-        curr++;
-        read++;
-        more = (curr != s_end);
+    if (p < e && *p == '.') {
+        ++p;
+        for (int k = 1; at_digit(); ++p, ++k) m += (*p - '0') * (k < 8 ? kNegPow10[k] : std::pow(10.0, -k));
     }
-    if (read == 0) return false;
-    if (!more) goto assemble;
-    if (*curr == '.') {
-        static const double pow_lut[] = {1.0, 0.1, 0.01, 0.001, 0.0001, 0.00001, 0.000001, 0.0000001};
-        curr++;
-        read = 1;
-        more = (curr != s_end);
-        while (more && *curr >= '0' && *curr <= '9') {
-            mantissa += static_cast<int>(*curr - '0') * (read < 8 ? pow_lut[read] : std::pow(10.0, -read));
-            read++;
-            curr++;
-            more = (curr != s_end);
-        }
-    } else if (*curr == 'e' || *curr == 'E') {
-    } else {
-        goto assemble;
+    int x = 0;
+    if (p < e && (*p == 'e' || *p == 'E')) {
+        ++p;
+        bool neg = false;
+        if (p < e && (*p == '+' || *p == '-')) neg = *p++ == '-';
+        if (!at_digit()) return false;
+        for (; at_digit(); ++p) x = std::min(10 * x + (*p - '0'), 99999);  // beyond +-99999: 0 or inf either way
+        if (neg) x = -x;
     }
-    if (!more) goto assemble;
-    if (*curr == 'e' || *curr == 'E') {
-        curr++;
-        more = (curr != s_end);
-        if (more && (*curr == '+' || *curr == '-')) {
-            exp_sign = *curr;
-            curr++;
-        } else if (!(*curr >= '0' && *curr <= '9')) {
-            return false;
-        }
-        read = 0;
-        more = (curr != s_end);
-        while (more && *curr >= '0' && *curr <= '9') {
-            exponent *= 10;
-            exponent += static_cast<int>(*curr - '0');
-            curr++;
-            read++;
-            more = (curr != s_end);
-        }
-        exponent *= (exp_sign == '+' ? 1 : -1);
-        if (read == 0) return false;
-    }
-assemble:
-    *result = (sign == '+' ? 1 : -1) * (exponent ? std::ldexp(mantissa * std::pow(5.0, exponent), exponent) : mantissa);
+    *result = sign * (x ? std::ldexp(m * std::pow(5.0, x), x) : m);
     return true;
 }
 

@@ -65,3 +65,30 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+def test_pmc_traffic_matches_the_kernel_instance():
+    """The committed traffic figure is used only for the traversal kernel
+    instance it was measured on (layout, ring, ILP, levels, batching), not
+    for another instance the autotuner may pick."""
+    class Args:
+        grid = 2048
+    tag = "binsah-d40-leaf8-ct2"
+    with open(os.path.join(ROOT, "profiles", f"trace_closest_terrain2048_{tag}_pmc.json")) as fh:
+        entries = json.load(fh)["entries"]
+    assert entries and all("tuning" in e and e["hbm_bytes_per_launch"] > 0 for e in entries)
+    for e in entries:
+        tun = dict(e["tuning"])
+        assert bench.pmc_traffic(Args, tag, tun) == e["hbm_bytes_per_launch"]
+        tun["fetch_min"] = 1 if tun["fetch_min"] != 1 else 2  # refill threshold: same kernel
+        assert bench.pmc_traffic(Args, tag, tun) is not None
+    other = dict(entries[0]["tuning"], stack_lds=13, leaf_batch=7)
+    assert bench.pmc_traffic(Args, tag, other) is None
+    assert bench.pmc_traffic(Args, "no-such-tree", entries[0]["tuning"]) is None
+
+
+def test_parse_tuning():
+    import nostrum_pray_amd as P
+    assert bench.parse_tuning(P, "1,6,2,16") == {"layout": 1, "stack_lds": 6, "leaf_ilp": 2, "fetch_min": 16}
+    full = bench.parse_tuning(P, "1,6,2,16,100,1,0,2")
+    assert list(full) == list(P.TUNING_FIELDS) and full["node_levels"] == 2

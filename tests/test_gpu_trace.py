@@ -266,3 +266,38 @@ def test_axis_parallel_and_degenerate_rays(ctx):
     d[d == 0] = rng.choice([0.0, -0.0], int((d == 0).sum())).astype(np.float32)
     rays = np.concatenate([o, d], 1).astype(np.float32)
     check_closest(ctx, "terrain256-sah", rays, 1e-12, bf_rows=3000)
+
+
+def test_kernel_timing_ring(ctx):
+    """pray_timed_kernels_ms averages the traversal kernels enqueued back to
+    back since timing was switched on (the last 64 of them); the per-kernel
+    mean fits inside the events around the whole region, and timing does not
+    change the hits."""
+    _, t, flat = scene("terrain256-sah")
+    sid = ctx.upload(t, shading=False)
+    try:
+        rays = O.gen_rays(3, P.scene_box_expanded(t.box()), 1 << 16)
+        d_rays = ctx.to_device(rays)
+        d_idx, d_t = ctx.alloc(len(rays), np.int32), ctx.alloc(len(rays), np.float32)
+        ctx.trace_device(sid, d_rays.ptr, len(rays), d_idx.ptr, d_t.ptr, P.TRACE_CLOSEST, 1e-12)
+        want = d_idx.numpy()
+        with pytest.raises(P.PrayError):
+            ctx.kernel_timing(True)
+            ctx.timed_kernels_ms()
+        for launches, expect in ((3, 3), (70, 64)):
+            ctx.kernel_timing(False)
+            ctx.kernel_timing(True)
+            ctx.record(0)
+            for _ in range(launches):
+                ctx.trace_device(sid, d_rays.ptr, len(rays), d_idx.ptr, d_t.ptr, P.TRACE_CLOSEST, 1e-12)
+            ctx.record(1)
+            mean, cnt = ctx.timed_kernels_ms()
+            region = ctx.elapsed_ms(0, 1)
+            assert cnt == expect and 0 < mean and mean * cnt <= region, (launches, mean, cnt, region)
+            assert 0 < ctx.last_kernel_ms() <= region
+            assert np.array_equal(d_idx.numpy(), want)
+        ctx.kernel_timing(False)
+        for b in (d_rays, d_idx, d_t):
+            b.free()
+    finally:
+        ctx.free_scene(sid)

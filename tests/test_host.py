@@ -211,6 +211,61 @@ def test_scene_loader_equals_reference(name):
         assert inf[k] == r[k], k
 
 
+def number_spellings(rng, n):
+    """OBJ number tokens in many spellings: fixed, exponent, upper-case E,
+    explicit '+', trailing '.', long fractions, signed zeros, tiny and large
+    magnitudes; and tokens the OBJ number grammar rejects (read as 0)."""
+    fixed = ["-0", "-0.0", "+0", "0.", "7.", "+7.5", "1e3", "1E-3", "2.5e+2", "1.e2", "123456789012.5",
+             "0.1234567890123456789", "3.14159265358979323846", "1e-38", "1e-45", "4e38", "-1.17549435e-38",
+             "9999999.999999999", "1e", "1e+", ".5", "-.25", "x1", "+", "1ex", "5e-x", "12abc", "0x10"]
+    out = list(fixed)
+    for _ in range(n):
+        v = float(rng.normal() * 10.0 ** rng.uniform(-9, 9))
+        f = rng.integers(0, 7)
+        out.append([f"{v:.17g}", f"{v:.9f}", f"{v:e}", f"{v:.3E}", ("+" if v >= 0 else "") + f"{v:.6f}",
+                    f"{v:.0f}.", f"{v:.12e}"][f])
+    # Exact decimal spellings of the midpoints between adjacent floats: the
+    # float a parser lands on then depends on the rounding of every step of
+    # its double arithmetic, so these tell apart parsers that agree elsewhere.
+    f32 = (rng.normal(size=n) * 10.0 ** rng.uniform(-6, 6, n)).astype(np.float32)
+    up = np.nextafter(f32, np.float32(np.inf))
+    mids = (f32.astype(np.float64) + up.astype(np.float64)) / 2
+    for k, m in enumerate(mids):
+        digits = int(rng.integers(8, 30))
+        out.append(f"{m:.{digits}g}" if k % 2 else f"{m:.{digits}e}")
+    return out
+
+
+@needs_ref
+def test_obj_numbers_equal_reference_loader(tmp_path):
+    """Vertex coordinates and material colours written in every number
+    spelling load to the same float bits as in the reference's loader
+    (compiled from its sources), rejected spellings included."""
+    rng = np.random.default_rng(11)
+    toks = number_spellings(rng, 3000)
+    toks += ["0"] * (-len(toks) % 9)
+    lines = ["mtllib t.mtl", "usemtl a"]
+    nv = len(toks) // 3
+    for k in range(nv):
+        lines.append("v " + " ".join(toks[3 * k: 3 * k + 3]))
+    for k in range(nv // 3):
+        lines.append(f"f {3 * k + 1} {3 * k + 2} {3 * k + 3}")
+    (tmp_path / "t.obj").write_text("\n".join(lines) + "\n")
+    kd = number_spellings(np.random.default_rng(12), 0)[:18]
+    (tmp_path / "t.mtl").write_text("newmtl a\nKd " + " ".join(kd[:3]) + "\nKe " + " ".join(kd[3:6]) + "\n")
+    js = {"obj_file": "t.obj", "resolution_x": 8, "resolution_y": 8, "camera_position": [0, 0, -5],
+          "camera_look": [0, 0, 1], "fov": 60, "lights": [], "background": [0, 0, 0]}
+    import json
+    (tmp_path / "t.json").write_text(json.dumps(js))
+    r = O.ref_load_scene(str(tmp_path / "t.json"))
+    s = P.Scene.load_json(str(tmp_path / "t.json"))
+    a, c, e = s.triangles()
+    assert len(a) == nv // 3
+    assert np.array_equal(a.view(np.uint32), r["tris"].view(np.uint32))
+    assert np.array_equal(c.view(np.uint32), r["color"].view(np.uint32))
+    assert np.array_equal(e, r["emit"])
+
+
 def test_scene_json_round_trip(tmp_path):
     s = P.Scene.cornell(4, 2000)
     path = str(tmp_path / "c.json")
