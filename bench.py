@@ -123,6 +123,9 @@ def main():
     ap.add_argument("--pt-frames", type=int, default=16, help="frames of the pt workload's cache render")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg of the default run")
+    ap.add_argument("--path-budget", type=int, default=-1,
+                    help="paths the plain path tracer keeps in flight (0: one sample per pass; default: the "
+                         "library's, 2^24)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="path tracers enqueue their passes kernel by kernel instead of replaying hipGraphs")
     args = ap.parse_args()
@@ -148,7 +151,7 @@ def main():
         return
     if args.workload == "cache":
         ctx = P.Context(0)
-        ctx.use_graphs(not args.no_graphs)
+        configure(ctx, args)
         try:
             out = radiance_cache_leg(P, ctx, args)
         finally:
@@ -161,7 +164,7 @@ def main():
     scene = P.Scene.terrain(args.grid, args.seed)
     n_tris = scene.num_tris
     ctx = P.Context(0)
-    ctx.use_graphs(not args.no_graphs)
+    configure(ctx, args)
     sid, tree_tag, tree_desc, tree_build, (tree_depth, tree_nodes, tree_refs), raw_box = \
         build_tree(P, ctx, scene, args, args.tree)
     log(f"scene {n_tris} tris, tree {tree_tag}: depth {tree_depth}, {tree_nodes} nodes, {tree_refs} references, "
@@ -488,7 +491,7 @@ def batch_leg(P, ctx, args):
 def batch_workload(P, args):
     """The batched multi-scene render alone, as one JSON line."""
     ctx = P.Context(0)
-    ctx.use_graphs(not args.no_graphs)
+    configure(ctx, args)
     b = batch_leg(P, ctx, args)
     ctx.close()
     out = {
@@ -519,7 +522,7 @@ def io_workload(P, args):
     import tempfile
 
     ctx = P.Context(0)
-    ctx.use_graphs(not args.no_graphs)
+    configure(ctx, args)
     tmp = tempfile.mkdtemp(prefix="pray_io_")
     try:
         scene = P.Scene.terrain(args.grid, args.seed)
@@ -631,7 +634,7 @@ def build_workload(P, args):
     2^20 rays over the GPU- and CPU-built SAH trees must agree bit for bit.
     One JSON line."""
     ctx = P.Context(0)
-    ctx.use_graphs(not args.no_graphs)
+    configure(ctx, args)
     try:
         scene = P.Scene.terrain(args.grid, args.seed)
         abc, _, _ = scene.triangles()
@@ -769,6 +772,13 @@ def gpu_sah_scene(P, ctx, scene, args):
                            traversal_cost=args.traversal_cost)
 
 
+def configure(ctx, args):
+    """Applies --no-graphs and --path-budget to a new context."""
+    ctx.use_graphs(not args.no_graphs)
+    if args.path_budget >= 0:
+        ctx.set_path_budget(args.path_budget)
+
+
 # Tuning fields that select the traversal kernel's instance (the others only
 # change its launch or refill threshold).
 KERNEL_FIELDS = ("layout", "stack_lds", "leaf_ilp", "node_levels")
@@ -827,7 +837,7 @@ def pt_workload(P, args):
     One JSON line with the times and rays; meant for profilers, which see
     the kernels of each pass (with --no-graphs, launched one by one)."""
     ctx = P.Context(0)
-    ctx.use_graphs(not args.no_graphs)
+    configure(ctx, args)
     w = h = args.cache_size
     scene = P.Scene.cornell(1, 100_000)
     tree = scene.build(2, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
@@ -857,7 +867,7 @@ def pt_workload(P, args):
     print(json.dumps({"metric": "path tracing, Cornell room", "n_gpus": 1, "data": "synthetic",
                       "config": {"workload": f"Cornell room, {scene.num_tris} triangles, {w}x{h}, depth {depth}, "
                                              f"roulette from bounce {rr}"},
-                      "graphs": not args.no_graphs, "tuning": tun,
+                      "graphs": not args.no_graphs, "path_budget": args.path_budget, "tuning": tun,
                       "plain": {"spp": args.pt_spp, "ms": round(plain_ms, 2), "rays": rays,
                                 "Mrays_per_s": round(rays / plain_ms / 1e3, 2),
                                 "checksum": float(img.astype(np.float64).sum())},

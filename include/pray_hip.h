@@ -384,6 +384,16 @@ PRAY_API int pray_event_elapsed(pray_ctx *ctx, int slot0, int slot1, float *ms);
  * replaying a hipGraph captured once per call; results are identical.  For
  * tools that follow kernel launches one by one (profilers); the default is 1. */
 PRAY_API int pray_use_graphs(pray_ctx *ctx, int enable);
+/* Paths the plain path tracers (pray_render_pt, pray_render_pt_ex,
+ * pray_render_pt_progressive) keep in flight: a call traces
+ * S = min(spp, max(1, max_paths / pixels)) samples of every pixel together,
+ * so each bounce's trace launch holds up to S x pixels paths, in passes of S
+ * samples (the last pass takes the rest).  Images, running sums and ray
+ * counts do not depend on S.  Device memory of a call grows by about
+ * 100 bytes per path in flight; the default is 2^24 paths (1.6 GB), and 0
+ * traces one sample per pass.  No counterpart in the reference, whose
+ * kernel traces one sample per pixel launch. */
+PRAY_API int pray_set_path_budget(pray_ctx *ctx, uint64_t max_paths);
 /* Kernel timing: with enable = 1, every later trace that is not being
  * captured into a graph records events around its traversal kernel alone
  * (not the counter reset before it), on the context's stream, without

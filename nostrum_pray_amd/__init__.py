@@ -86,6 +86,7 @@ def libraries():
         "pray_trace_counts_device": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp, _vp]),
         "pray_trace_device_noalloc": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp, _vp]),
         "pray_use_graphs": (C.c_int, [_vp, C.c_int]),
+        "pray_set_path_budget": (C.c_int, [_vp, _u64]),
         "pray_gen_rays_device": (C.c_int, [_vp, _u64, _vp, _u64, _vp, _vp]),
         "pray_gen_rays": (C.c_int, [_vp, _u64, _vp, _u64, _vp]),
         "pray_camera_rays": (C.c_int, [_vp, _vp, _u32, _u32, _vp]),
@@ -383,6 +384,11 @@ class Context:
     def _check(self, rc):
         if rc != PRAY_OK:
             raise PrayError(rc, self._lib.pray_last_error(self._h).decode())
+
+    def set_path_budget(self, max_paths):
+        """Paths the plain path tracers keep in flight (pray_set_path_budget);
+        0: one sample per pass."""
+        self._check(self._lib.pray_set_path_budget(self._h, int(max_paths)))
 
     def use_graphs(self, enable=True):
         """Path tracers replay captured hipGraphs (default) or enqueue kernel

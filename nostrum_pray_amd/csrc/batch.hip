@@ -293,6 +293,7 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
     if (rc) return rc;
 
     auto pass = [&]() -> int {
+        Range rp("batch pass");
         hipLaunchKernelGGL(prr::k_pt_generate_batch, dim3(nblocks(total, 256)), dim3(256), 0, st,
                            (const float *)prim.p, (uint32_t)total, (const prr::BatchScene *)dbs.p, n_scenes,
                            (float *)rq[0].p, (float4 *)sq[0].p, (uint32_t *)qs[0].p, count + 0);
@@ -301,9 +302,15 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
             hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + nxt, 1u);
             hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, (unsigned long long *)rays_acc.p,
                                (const uint32_t *)(count + cur));
-            int e = enqueue_trace_multi(c, dtab, (const uint32_t *)qs[cur].p, tree_depth, tu, (const float *)rq[cur].p,
-                                        total, count + cur, det_eps, (int32_t *)hit.p, (float *)tt.p, st);
+            int e;
+            {
+                Range r("batch extend");
+                e = enqueue_trace_multi(c, dtab, (const uint32_t *)qs[cur].p, tree_depth, tu,
+                                        (const float *)rq[cur].p, total, count + cur, det_eps, (int32_t *)hit.p,
+                                        (float *)tt.p, st);
+            }
             if (e) return e;
+            Range rs("batch shade");
             prr::BatchArgs B{dtab, (const prr::BatchScene *)dbs.p, (const uint32_t *)qs[cur].p, (uint32_t *)qs[nxt].p};
             hipLaunchKernelGGL((prr::k_pt_shade<prr::PT_PLAIN, true>), dim3(nblocks(total, 256)), dim3(256), 0, st,
                                prt::DevScene{}, P, prr::CacheArgs{}, b, (const float *)rq[cur].p,
@@ -311,7 +318,7 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
                                (const float *)tt.p, (float *)rq[nxt].p, (float4 *)sq[nxt].p, count + nxt,
                                (float *)contrib.p, B);
         }
-        hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * total, 256)), dim3(256), 0, st, (uint32_t)total,
+        hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * total, 256)), dim3(256), 0, st, (uint32_t)total, 1u,
                            (const float *)contrib.p, (float *)sum.p, count + 2);
         HIPC(c, hipGetLastError());
         return PRAY_OK;

@@ -439,18 +439,24 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
     // cache predicts the pixels' and the training samples' remaining
     // radiance; one training step; accumulation.
     auto frame = [&]() -> int {
+        Range rf("cache frame");
         hipLaunchKernelGGL(prr::k_fill_f32, dim3(nblocks(4 * (uint64_t)n_train, 256)), dim3(256), 0, st,
                            (float *)r->targets, 4u * n_train, 0.f);
         hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, r->ctr, 1u);
         hipLaunchKernelGGL(prr::k_pt_generate, dim3(nblocks(npix, 256)), dim3(256), 0, st, (const float *)prim.p,
-                           (uint32_t)npix, (float *)rq[0].p, (float4 *)sq[0].p, count + 0);
+                           (uint32_t)npix, (uint32_t)npix, (float *)rq[0].p, (float4 *)sq[0].p, count + 0);
         hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + 3, 2u);
         for (uint32_t b = 0; b <= max_depth; b++) {
             const int cur = b & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + nxt, 1u);
-            int e = enqueue_trace(c, *sc, (const float *)rq[cur].p, npix, count + cur, PRAY_TRACE_CLOSEST, det_eps,
+            int e;
+            {
+                Range r("cache extend");
+                e = enqueue_trace(c, *sc, (const float *)rq[cur].p, npix, count + cur, PRAY_TRACE_CLOSEST, det_eps,
                                   nullptr, (int32_t *)hit.p, (float *)tt.p, nullptr, st);
+            }
             if (e) return e;
+            Range r("cache shade");
             hipLaunchKernelGGL(prr::k_pt_shade<prr::PT_CACHE>, dim3(nblocks(npix, 256)), dim3(256), 0, st, sc->d, P,
                                C, b, (const float *)rq[cur].p, (const float4 *)sq[cur].p,
                                (const uint32_t *)(count + cur), (const int32_t *)hit.p, (const float *)tt.p,
@@ -458,32 +464,39 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
         }
         // The training rays (few, latency bound) and their targets run on the
         // side stream while the pixels' inference runs on the main one.
-        HIPC(c, hipEventRecord(c->fork_ev, st));
-        HIPC(c, hipStreamWaitEvent(side, c->fork_ev, 0));
-        int e = enqueue_trace(c, *sc, (const float *)trays.p, n_train, r->ctr, PRAY_TRACE_CLOSEST, det_eps, nullptr,
+        int e;
+        {
+            Range ri("cache training targets and inference");
+            HIPC(c, hipEventRecord(c->fork_ev, st));
+            HIPC(c, hipStreamWaitEvent(side, c->fork_ev, 0));
+            e = enqueue_trace(c, *sc, (const float *)trays.p, n_train, r->ctr, PRAY_TRACE_CLOSEST, det_eps, nullptr,
                               (int32_t *)thit.p, (float *)ttt.p, nullptr, side);
+            if (e) return e;
+            hipLaunchKernelGGL(prr::k_pt_train_targets, dim3(nblocks(n_train, 256)), dim3(256), 0, side, sc->d, P, C,
+                               (const float *)trays.p, (const float4 *)tstate.p, (const uint32_t *)r->ctr,
+                               (const int32_t *)thit.p, (const float *)ttt.p, r->targets, (float *)tq.p, count + 4);
+            prm::InferArgs it{};
+            it.queries = (const float *)tq.p;
+            it.n = n_train;
+            it.n_dev = count + 4;
+            it.targets = r->targets;
+            hipLaunchKernelGGL(prm::k_mlp_infer, dim3((unsigned)std::min<uint64_t>((n_train + 63) / 64, infer_grid)),
+                               dim3(256), 0, side, (const prm::bf16x8 *)r->ffrag, it);
+            HIPC(c, hipEventRecord(c->join_ev, side));
+            prm::InferArgs ia{};
+            ia.queries = (const float *)queries.p;
+            ia.n = (uint32_t)npix;
+            ia.n_dev = count + 3;
+            ia.contrib = (float *)contrib.p;
+            hipLaunchKernelGGL(prm::k_mlp_infer, dim3(infer_grid), dim3(256), 0, st, (const prm::bf16x8 *)r->ffrag, ia);
+            HIPC(c, hipStreamWaitEvent(st, c->join_ev, 0));
+        }
+        {
+            Range rt("cache train");
+            e = enqueue_train(c, *r, n_train, true, max_loss, min_steps, (float *)hist.p, step_base, frames, st);
+        }
         if (e) return e;
-        hipLaunchKernelGGL(prr::k_pt_train_targets, dim3(nblocks(n_train, 256)), dim3(256), 0, side, sc->d, P, C,
-                           (const float *)trays.p, (const float4 *)tstate.p, (const uint32_t *)r->ctr,
-                           (const int32_t *)thit.p, (const float *)ttt.p, r->targets, (float *)tq.p, count + 4);
-        prm::InferArgs it{};
-        it.queries = (const float *)tq.p;
-        it.n = n_train;
-        it.n_dev = count + 4;
-        it.targets = r->targets;
-        hipLaunchKernelGGL(prm::k_mlp_infer, dim3((unsigned)std::min<uint64_t>((n_train + 63) / 64, infer_grid)),
-                           dim3(256), 0, side, (const prm::bf16x8 *)r->ffrag, it);
-        HIPC(c, hipEventRecord(c->join_ev, side));
-        prm::InferArgs ia{};
-        ia.queries = (const float *)queries.p;
-        ia.n = (uint32_t)npix;
-        ia.n_dev = count + 3;
-        ia.contrib = (float *)contrib.p;
-        hipLaunchKernelGGL(prm::k_mlp_infer, dim3(infer_grid), dim3(256), 0, st, (const prm::bf16x8 *)r->ffrag, ia);
-        HIPC(c, hipStreamWaitEvent(st, c->join_ev, 0));
-        e = enqueue_train(c, *r, n_train, true, max_loss, min_steps, (float *)hist.p, step_base, frames, st);
-        if (e) return e;
-        hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
+        hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix, 1u,
                            (const float *)contrib.p, (float *)sum.p, count + 2);
         HIPC(c, hipGetLastError());
         return PRAY_OK;

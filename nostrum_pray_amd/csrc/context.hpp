@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <map>
 #include <memory>
@@ -11,6 +12,17 @@
 
 #include "../../include/pray_hip.h"
 #include "traverse.hpp"
+
+// A roctx range for the lifetime of a scope: tools that record roctx
+// markers (rocprofv3 --marker-trace) show when the host enqueued each stage
+// of a render (the kernels run later on the stream, or inside a graph
+// replay).  Without such a tool a range costs a call into an empty stub.
+struct Range {
+    explicit Range(const char *name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range &) = delete;
+    Range &operator=(const Range &) = delete;
+};
 
 namespace prc {
 
@@ -55,6 +67,7 @@ struct pray_ctx {
     hipStream_t side = nullptr;   // second stream for work overlapped inside graphs
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     bool use_graphs = true;       // path tracers replay captured hipGraphs (pray_use_graphs)
+    uint64_t path_budget = 1ull << 24;  // paths in flight in pray_render_pt* (pray_set_path_budget)
     uint64_t last_rays = 0;       // rays traced by the last render call
     std::vector<uint64_t> last_queues;  // per bounce: paths traced (pray_render_pt*)
     uint32_t *work = nullptr;   // ray counters of the persistent kernels

@@ -828,6 +828,7 @@ int pray_render_rt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, ui
     if (sc->d.n_tris && (!sc->d.normal || !sc->d.color))
         return fail(c, PRAY_ERR_INVALID, "scene was uploaded without shading data");
     HIPC(c, hipSetDevice(c->device));
+    Range whole("pray_render_rt");
     prr::RtGrid g{};
     g.w = w, g.h = h;
     if (w >= 4 && h >= 2) {
@@ -853,10 +854,18 @@ int pray_render_rt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, ui
         HIPC(c, hipMalloc(&so.p, nsh * 4));
         HIPC(c, hipMemcpyAsync(lt.p, lights, (size_t)nl * 24, hipMemcpyHostToDevice, c->stream));
     }
-    hipLaunchKernelGGL(prr::k_camera, dim3(nblocks(npix, 256)), dim3(256), 0, c->stream, cam, w, h, (float *)rays.p);
-    HIPC(c, hipGetLastError());
-    int rc = enqueue_trace(c, *sc, (const float *)rays.p, npix, nullptr, PRAY_TRACE_CLOSEST, det_eps, nullptr,
+    {
+        Range r("rt generate");
+        hipLaunchKernelGGL(prr::k_camera, dim3(nblocks(npix, 256)), dim3(256), 0, c->stream, cam, w, h,
+                           (float *)rays.p);
+        HIPC(c, hipGetLastError());
+    }
+    int rc;
+    {
+        Range r("rt extend");
+        rc = enqueue_trace(c, *sc, (const float *)rays.p, npix, nullptr, PRAY_TRACE_CLOSEST, det_eps, nullptr,
                            (int32_t *)hit.p, (float *)tt.p, nullptr, c->stream);
+    }
     if (rc) return rc;
     prr::RtArgs a;
     a.g = g;
@@ -871,6 +880,7 @@ int pray_render_rt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, ui
     a.socc = (const int32_t *)so.p;
     a.rgb = (uint8_t *)out.p;
     if (nl && sc->d.n_tris) {
+        Range r("rt shadow");
         hipLaunchKernelGGL(prr::k_rt_shadow_setup, dim3(nblocks(lanes, 256)), dim3(256), 0, c->stream, sc->d, a);
         HIPC(c, hipGetLastError());
         rc = enqueue_trace(c, *sc, (const float *)sr.p, nsh, nullptr, PRAY_TRACE_ANY_INCLUSIVE, det_eps,
@@ -880,8 +890,11 @@ int pray_render_rt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, ui
         hipLaunchKernelGGL(k_fill_miss, dim3(nblocks(nsh, 256)), dim3(256), 0, c->stream, (uint32_t)nsh,
                            (int32_t *)so.p, (float *)nullptr, (uint32_t *)nullptr);
     }
-    hipLaunchKernelGGL(prr::k_rt_shade, dim3(nblocks(lanes, WAVE)), dim3(WAVE), 0, c->stream, sc->d, a);
-    HIPC(c, hipGetLastError());
+    {
+        Range r("rt shade");
+        hipLaunchKernelGGL(prr::k_rt_shade, dim3(nblocks(lanes, WAVE)), dim3(WAVE), 0, c->stream, sc->d, a);
+        HIPC(c, hipGetLastError());
+    }
     HIPC(c, hipMemcpyAsync(rgb, out.p, npix * 3, hipMemcpyDeviceToHost, c->stream));
     if (hit_idx) HIPC(c, hipMemcpyAsync(hit_idx, hit.p, npix * 4, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -920,6 +933,14 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
     Cam cam;
     std::memcpy(&cam, cam16, sizeof cam);
     if (max_depth > 64) return fail(c, PRAY_ERR_INVALID, "max_depth must be at most 64");
+    Range whole("pray_render_pt");
+    // Samples per pass: as many as keep the paths in flight within the
+    // context's path budget (at least one), so that every bounce's trace
+    // launch holds samples x pixels paths.  Path p of a pass is sample
+    // p / npix of pixel p % npix; results do not depend on the split.
+    const uint64_t per_pass = std::max<uint64_t>(1, std::min<uint64_t>(c->path_budget, 1ull << 31) / npix);
+    const uint32_t S = (uint32_t)std::min<uint64_t>(spp, per_pass);
+    const uint64_t npath = npix * S;
     // Buffers from the context's render workspace (slots 0..12), reused by
     // later calls.
     struct Buf {
@@ -927,14 +948,14 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
     } prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, qsum;
     WS_GET(c, 0, prim.p, npix * 24);
     for (int q = 0; q < 2; q++) {
-        WS_GET(c, 1 + q, rq[q].p, npix * 24);
-        WS_GET(c, 3 + q, sq[q].p, npix * 16);
+        WS_GET(c, 1 + q, rq[q].p, npath * 24);
+        WS_GET(c, 3 + q, sq[q].p, npath * 16);
     }
     WS_GET(c, 5, cnt.p, 6 * sizeof(uint32_t));    // count0, count1, sample, pad, traced rays (u64)
     WS_GET(c, 6, qsum.p, 65 * sizeof(uint64_t));  // paths traced per bounce, all samples
-    WS_GET(c, 7, hit.p, npix * 4);
-    WS_GET(c, 8, tt.p, npix * 4);
-    WS_GET(c, 9, contrib.p, npix * 12);
+    WS_GET(c, 7, hit.p, npath * 4);
+    WS_GET(c, 8, tt.p, npath * 4);
+    WS_GET(c, 9, contrib.p, npath * 12);
     WS_GET(c, 10, sum.p, npix * 12);
     WS_GET(c, 11, outh.p, npix * 12);
     if (rgb) WS_GET(c, 12, outb.p, npix * 3);
@@ -960,14 +981,20 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
     P.sample = count + 2;
     // The persistent trace kernel needs its spill area before any capture.
     {
-        int rc = prepare_trace(c, *sc, npix);
+        int rc = prepare_trace(c, *sc, npath);
         if (rc) return rc;
     }
-    // One sample pass: generate, (zero next count, extend, shade) per bounce,
-    // accumulate.
-    auto pass = [&]() -> int {
-        hipLaunchKernelGGL(prr::k_pt_generate, dim3(nblocks(npix, 256)), dim3(256), 0, st, (const float *)prim.p,
-                           (uint32_t)npix, (float *)rq[0].p, (float4 *)sq[0].p, count + 0);
+    // One pass of `ns` samples: generate, (zero next count, extend, shade)
+    // per bounce, accumulate.
+    auto pass = [&](uint32_t ns) -> int {
+        const uint64_t np = npix * ns;
+        P.samples = ns;
+        Range rp("pt pass");
+        {
+            Range rg("pt generate");
+            hipLaunchKernelGGL(prr::k_pt_generate, dim3(nblocks(np, 256)), dim3(256), 0, st, (const float *)prim.p,
+                               (uint32_t)npix, (uint32_t)np, (float *)rq[0].p, (float4 *)sq[0].p, count + 0);
+        }
         for (uint32_t b = 0; b <= max_depth; b++) {
             const int cur = b & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + nxt, 1u);
@@ -975,28 +1002,32 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
             hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, (unsigned long long *)qsum.p + b,
                                (const uint32_t *)(count + cur));
             if (sc->d.n_tris) {
-                int rc = enqueue_trace(c, *sc, (const float *)rq[cur].p, npix, count + cur, PRAY_TRACE_CLOSEST,
+                Range r("pt extend");
+                int rc = enqueue_trace(c, *sc, (const float *)rq[cur].p, np, count + cur, PRAY_TRACE_CLOSEST,
                                        det_eps, nullptr, (int32_t *)hit.p, (float *)tt.p, nullptr, st);
                 if (rc) return rc;
             } else {
-                hipLaunchKernelGGL(k_fill_miss, dim3(nblocks(npix, 256)), dim3(256), 0, st, (uint32_t)npix,
+                hipLaunchKernelGGL(k_fill_miss, dim3(nblocks(np, 256)), dim3(256), 0, st, (uint32_t)np,
                                    (int32_t *)hit.p, (float *)tt.p, (uint32_t *)nullptr);
             }
-            hipLaunchKernelGGL(prr::k_pt_shade<prr::PT_PLAIN>, dim3(nblocks(npix, 256)), dim3(256), 0, st, sc->d, P,
+            Range r("pt shade");
+            hipLaunchKernelGGL(prr::k_pt_shade<prr::PT_PLAIN>, dim3(nblocks(np, 256)), dim3(256), 0, st, sc->d, P,
                                prr::CacheArgs{}, b, (const float *)rq[cur].p, (const float4 *)sq[cur].p, (const uint32_t *)(count + cur),
                                (const int32_t *)hit.p, (const float *)tt.p, (float *)rq[nxt].p, (float4 *)sq[nxt].p,
                                count + nxt, (float *)contrib.p, prr::BatchArgs{});
         }
-        hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
+        Range ra("pt accumulate");
+        hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix, ns,
                            (const float *)contrib.p, (float *)sum.p, count + 2);
         HIPC(c, hipGetLastError());
         return PRAY_OK;
     };
-    if (use_graph && c->use_graphs) {
+    const uint32_t full = spp / S, rest = spp % S;
+    if (use_graph && c->use_graphs && full > 1) {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIPC(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        int rc = pass();
+        int rc = pass(S);
         hipError_t e = hipStreamEndCapture(st, &graph);
         if (rc) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -1008,17 +1039,21 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
             (void)hipGraphDestroy(graph);
             return fail(c, PRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
         }
-        for (uint32_t s = 0; s < spp && e == hipSuccess; s++) e = hipGraphLaunch(exec, st);
+        for (uint32_t s = 0; s < full && e == hipSuccess; s++) e = hipGraphLaunch(exec, st);
         const hipError_t e2 = hipStreamSynchronize(st);
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess || e2 != hipSuccess)
             return fail(c, PRAY_ERR_HIP, "graph launch failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
     } else {
-        for (uint32_t s = 0; s < spp; s++) {
-            int rc = pass();
+        for (uint32_t s = 0; s < full; s++) {
+            int rc = pass(S);
             if (rc) return rc;
         }
+    }
+    if (rest) {  // the last samples, fewer than a pass holds
+        int rc = pass(rest);
+        if (rc) return rc;
     }
     hipLaunchKernelGGL(prr::k_pt_finalize, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
                        (const float *)sum.p, (float)(first + spp), (float *)outh.p, (uint8_t *)outb.p);
@@ -1107,6 +1142,12 @@ int pray_event_record(pray_ctx *c, int slot) {
     HIPC(c, hipSetDevice(c->device));
     if (!c->events[slot]) HIPC(c, hipEventCreate(&c->events[slot]));
     HIPC(c, hipEventRecord(c->events[slot], c->stream));
+    return PRAY_OK;
+}
+
+int pray_set_path_budget(pray_ctx *c, uint64_t max_paths) {
+    if (!c) return PRAY_ERR_INVALID;
+    c->path_budget = max_paths;
     return PRAY_OK;
 }
 

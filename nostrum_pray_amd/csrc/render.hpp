@@ -240,15 +240,21 @@ static __global__ void k_camera(Cam c, uint32_t w, uint32_t h, float *rays) {
 struct PtArgs {
     uint32_t n_pix, max_depth, rr_depth, k0, k1;
     float bg[3];
-    const uint32_t *sample;   // current sample index (device)
+    const uint32_t *sample;   // first sample index of the pass (device)
+    // Samples of the plain path tracer in flight together: path p of a pass
+    // is sample *sample + p / n_pix of pixel p % n_pix (1 for the cache and
+    // batch tracers, whose paths are pixels).
+    uint32_t samples = 1;
 };
 
-// Queue entries: ray (6 floats) + state float4 {throughput rgb, pixel bits}.
-static __global__ void k_pt_generate(const float *prim, uint32_t n_pix, float *rays, float4 *state, uint32_t *count) {
+// Queue entries: ray (6 floats) + state float4 {throughput rgb, path bits}.
+// Paths p < n_paths: the camera ray of pixel p % n_pix.
+static __global__ void k_pt_generate(const float *prim, uint32_t n_pix, uint32_t n_paths, float *rays, float4 *state,
+                                     uint32_t *count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *count = n_pix;
-    if (i >= n_pix) return;
-    const float *r = prim + 6 * (size_t)i;
+    if (i == 0) *count = n_paths;
+    if (i >= n_paths) return;
+    const float *r = prim + 6 * (size_t)(i % n_pix);
     float *o = rays + 6 * (size_t)i;
     o[0] = r[0], o[1] = r[1], o[2] = r[2], o[3] = r[3], o[4] = r[4], o[5] = r[5];
     state[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(i));
@@ -375,7 +381,12 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
         const uint8_t *emit = s.emit;
         const float *color = s.color, *normal = s.normal;
         float bg0 = P.bg[0], bg1 = P.bg[1], bg2 = P.bg[2];
-        uint32_t k0 = P.k0, k1 = P.k1, lpix = pix;
+        uint32_t k0 = P.k0, k1 = P.k1, lpix = pix, smp = *P.sample;
+        if (!MULTI && P.samples > 1) {
+            const uint32_t j = pix / P.n_pix;
+            lpix = pix - j * P.n_pix;
+            smp += j;
+        }
         if (MULTI) {
             scn = B.scene_in[i];
             const DevScene &S = B.table[scn];
@@ -414,7 +425,7 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
                 query = true;
                 term = false;
             } else {
-                const uint4 u = prd::philox4x32_10(make_uint4(lpix, *P.sample, bounce, 0u), k0, k1);
+                const uint4 u = prd::philox4x32_10(make_uint4(lpix, smp, bounce, 0u), k0, k1);
                 prd::sample_hemisphere(nx, ny, nz, prd::u01(u.x), prd::u01(u.y), dx, dy, dz);
                 tr = st.x * ar, tg = st.y * ag, tb = st.z * ab;
                 bool live = true;
@@ -525,10 +536,18 @@ static __global__ __launch_bounds__(256) void k_pt_train_targets(DevScene s, PtA
 }
 
 // sum += contribution of this pass (sample order), then advance the sample.
-static __global__ void k_pt_accumulate(uint32_t n_pix, const float *contrib, float *sum, uint32_t *sample) {
+// Adds the pass's `samples` contributions per pixel to the running sum, in
+// sample order (the order the one-sample passes would add them in), and
+// advances the sample index past them.
+static __global__ void k_pt_accumulate(uint32_t n_pix, uint32_t samples, const float *contrib, float *sum,
+                                       uint32_t *sample) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 3 * n_pix) sum[i] = sum[i] + contrib[i];
-    if (i == 0) *sample = *sample + 1;
+    if (i < 3 * n_pix) {
+        float v = sum[i];
+        for (uint32_t j = 0; j < samples; ++j) v = v + contrib[(size_t)j * 3 * n_pix + i];
+        sum[i] = v;
+    }
+    if (i == 0) *sample = *sample + samples;
 }
 
 // Adds a queue count to a traced-ray counter (inside graphs: a kernel, not

@@ -92,8 +92,12 @@ def test_rt_multi_light_negative_lambert_quirk(ctx):
         ctx.free_scene(sid)
 
 
-@pytest.mark.parametrize("use_graph", [1, 0])
-def test_pt_cornell_equal_oracle(ctx, use_graph):
+# (hipGraph replays, path budget): all 8 samples in one pass (the default
+# budget), passes of 3, 3 and 2 samples, and one sample per pass; with and
+# without graphs.
+@pytest.mark.parametrize("use_graph,budget", [(1, None), (0, None), (1, 3 * 64 * 48), (0, 3 * 64 * 48), (1, 0),
+                                              (0, 0)])
+def test_pt_cornell_equal_oracle(ctx, use_graph, budget):
     s = P.Scene.cornell(3, 20000)
     t = s.build(1)
     f = t.flat()
@@ -101,8 +105,10 @@ def test_pt_cornell_equal_oracle(ctx, use_graph):
     w, h, spp, depth, rr = 64, 48, 8, 6, 2
     cam = O.camera([0, 1, 3.8], [0, 0, -1], 40, w, h)
     sid = ctx.upload(t)
+    if budget is not None:
+        ctx.set_path_budget(budget)
     try:
-        say(f"pt cornell graph={use_graph}")
+        say(f"pt cornell graph={use_graph} budget={budget}")
         hdr = np.empty((h, w, 3), np.float32)
         rgb = np.empty((h, w, 3), np.uint8)
         lib = P.libraries()[0]
@@ -113,6 +119,7 @@ def test_pt_cornell_equal_oracle(ctx, use_graph):
         ctx._check(lib.pray_render_pt_ex(ctx.handle, sid, P._p(cam), w, h, spp, depth, rr, 77, P._p(bg), 1e-9,
                                          P._p(hdr), P._p(rgb), use_graph))
     finally:
+        ctx.set_path_budget(1 << 24)
         ctx.free_scene(sid)
     queues = ctx.last_render_queues()
     oh, orgb, oq = O.render_pt(f, cam, w, h, spp, depth, rr, 77, bg, 1e-9, queues=True)

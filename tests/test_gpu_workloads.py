@@ -129,6 +129,37 @@ def test_cornell100k_depth8_equals_oracle(ctx, tree_kind):
     assert 0.05 < oh.mean() < 10
 
 
+def test_cornell100k_full_size_split_invariance(ctx):
+    """The Cornell room at the workload's own image size (1024^2, depth 8,
+    roulette from bounce 3): samples traced all in one pass (6 x 1024^2 paths
+    in flight), in passes of 2 (graph replays) and one by one give the same
+    image bit for bit, the same traced rays and per-bounce queues, whose
+    first entry is every path of every sample."""
+    s = P.Scene.cornell(1, 100_000)
+    t = s.build(2, max_depth=40, leaf_size=8, traversal_cost=2.0)
+    inf = s.info()
+    w = h = 1024
+    spp, depth, rr, seed = 6, 8, 3, 7
+    s.set_render(w, h, spp, depth, rr)
+    cam = s.camera()
+    sid = ctx.upload(t)
+    out = []
+    try:
+        for budget in (1 << 24, 2 * w * h, 0):
+            ctx.set_path_budget(budget)
+            say(f"cornell 100k {w}x{h} {spp} spp, path budget {budget}")
+            hdr, _ = ctx.render_pt(sid, cam, w, h, spp, depth, rr, seed, inf["background"], 1e-9, want_rgb=False)
+            out.append((hdr, ctx.last_render_rays(), ctx.last_render_queues()))
+    finally:
+        ctx.set_path_budget(1 << 24)
+        ctx.free_scene(sid)
+    hdr0, rays0, q0 = out[0]
+    assert q0[0] == w * h * spp and rays0 == int(q0.sum()) and hdr0.mean() > 0.05
+    for hdr, rays, q in out[1:]:
+        assert np.array_equal(hdr.view(np.uint32), hdr0.view(np.uint32))
+        assert rays == rays0 and np.array_equal(q, q0)
+
+
 def gpu_flat(ctx, sid, scene):
     """The oracle's view of a scene whose tree was built on the GPU: the
     device layout plus the host's per-triangle data."""
