@@ -121,6 +121,8 @@ def main():
                          "one radiance-cache render of the Cornell room (--cache-size, --pt-spp, --pt-frames)")
     ap.add_argument("--pt-spp", type=int, default=16, help="samples per pixel of the pt workload's plain render")
     ap.add_argument("--pt-frames", type=int, default=16, help="frames of the pt workload's cache render")
+    ap.add_argument("--cache-spf", type=int, default=8,
+                    help="samples per pixel in each radiance-cache frame (one training step per frame)")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg of the default run")
     ap.add_argument("--path-budget", type=int, default=-1,
@@ -858,10 +860,11 @@ def pt_workload(P, args):
     c = P.Cache(ctx, seed=3, learning_rate=1e-2, max_batch=1 << 16)
     t0 = time.perf_counter()
     _, _, st = c.render_pt(sid, cam, w, h, args.pt_frames, depth, rr, 7, bg, 1e-9, train_paths=1 << 16,
-                           cache_bounce=1, min_steps=4)
+                           cache_bounce=1, min_steps=4, samples_per_frame=args.cache_spf)
     cache_ms = (time.perf_counter() - t0) * 1e3
     c.close()
-    log(f"pt: cache {args.pt_frames} frames {cache_ms:.1f} ms, {int(st[:, 1].sum())} with the cache")
+    log(f"pt: cache {args.pt_frames} frames of {args.cache_spf} samples {cache_ms:.1f} ms, "
+        f"{int(st[:, 1].sum())} with the cache")
     ctx.free_scene(sid)
     ctx.close()
     print(json.dumps({"metric": "path tracing, Cornell room", "n_gpus": 1, "data": "synthetic",
@@ -871,7 +874,8 @@ def pt_workload(P, args):
                       "plain": {"spp": args.pt_spp, "ms": round(plain_ms, 2), "rays": rays,
                                 "Mrays_per_s": round(rays / plain_ms / 1e3, 2),
                                 "checksum": float(img.astype(np.float64).sum())},
-                      "cache": {"frames": args.pt_frames, "ms": round(cache_ms, 2),
+                      "cache": {"frames": args.pt_frames, "samples_per_frame": args.cache_spf,
+                                "ms": round(cache_ms, 2),
                                 "frames_with_cache": int(st[:, 1].sum())}}), flush=True)
 
 
@@ -899,25 +903,27 @@ def radiance_cache_leg(P, ctx, args):
         return float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
 
     plain = []
-    for spp in (16, 32, 64, 128, 256, 512):
+    for spp in (16, 32, 64, 128, 256, 512, 1024):
         t0 = time.perf_counter()
         img, _ = ctx.render_pt(sid, cam, w, h, spp, depth, rr, 7, bg, 1e-9, want_rgb=False)
         ms = (time.perf_counter() - t0) * 1e3
         plain.append({"spp": spp, "ms": round(ms, 2), "rmse": round(rmse(img), 6)})
         log(f"cache leg: plain {spp} spp {ms:.1f} ms rmse {plain[-1]['rmse']:.5f}")
     cache = []
-    # plain frames before the cache takes over; training samples per frame (2^16, the workload's)
-    warm, train = 4, 1 << 16
-    for frames in (16, 32, 64, 128, 256):
+    # plain frames before the cache takes over; training samples per frame (2^16, the workload's);
+    # samples per pixel in a frame (one training step each)
+    warm, train, spf = 4, 1 << 16, args.cache_spf
+    for total in (32, 64, 128, 256, 512):
+        frames = max(1, total // spf)
         c = P.Cache(ctx, seed=3, learning_rate=1e-2, max_batch=1 << 16)
         t0 = time.perf_counter()
         img, _, st = c.render_pt(sid, cam, w, h, frames, depth, rr, 7, bg, 1e-9, train_paths=train,
-                                 cache_bounce=1, min_steps=warm)
+                                 cache_bounce=1, min_steps=warm, samples_per_frame=spf)
         ms = (time.perf_counter() - t0) * 1e3
         c.close()
-        cache.append({"frames": frames, "ms": round(ms, 2), "rmse": round(rmse(img), 6),
+        cache.append({"samples": frames * spf, "frames": frames, "ms": round(ms, 2), "rmse": round(rmse(img), 6),
                       "frames_with_cache": int(st[:, 1].sum()), "final_loss": round(float(st[-1, 0]), 4)})
-        log(f"cache leg: cache {frames} frames {ms:.1f} ms rmse {cache[-1]['rmse']:.5f}")
+        log(f"cache leg: cache {frames} frames x {spf} samples {ms:.1f} ms rmse {cache[-1]['rmse']:.5f}")
     pp = [(p["ms"], p["rmse"]) for p in plain]
     for cpt in cache:
         cpt["plain_ms_at_equal_rmse"] = round(equal_rmse_time(pp, cpt["rmse"]), 2)
@@ -929,7 +935,8 @@ def radiance_cache_leg(P, ctx, args):
                  f"SAH KD tree built on the CPU (64 bins per axis, leaf size {args.leaf_size})",
         "reference_spp": args.ref_spp, "tuning": tun,
         "cache": {"network": "192 (172 used) -> 5 x 64 ReLU -> 3, BF16 matrix cores, FP32 Adam lr 1e-2",
-                  "training_samples_per_frame": train, "terminates_at_hit": 2, "warmup_frames": warm},
+                  "training_samples_per_frame": train, "samples_per_frame": spf, "terminates_at_hit": 2,
+                  "warmup_frames": warm},
         "plain_runs": plain,
         "cache_runs": cache,
         "equal_rmse_speedup": best["speedup"],

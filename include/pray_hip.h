@@ -361,6 +361,18 @@ PRAY_API int pray_render_pt_cache(pray_ctx *ctx, int scene_id, int cache_id, con
                                   uint64_t seed, const float *background, float det_eps, uint32_t train_paths,
                                   uint32_t cache_bounce, float max_loss, uint32_t min_steps, float *hdr, uint8_t *rgb,
                                   float *frame_stats);
+/* As pray_render_pt_cache with `samples_per_frame` paths per pixel in each
+ * frame (frames x samples_per_frame samples in all, traced together): sample
+ * s of a frame is sample frame * samples_per_frame + s of pray_render_pt's
+ * random streams; training pixels come from each frame's first sample, so a
+ * frame still takes one Adam step on at most `train_paths` samples.  With
+ * max_loss = 0 the image is pray_render_pt's at frames x samples_per_frame
+ * samples per pixel.  samples_per_frame = 1 is pray_render_pt_cache. */
+PRAY_API int pray_render_pt_cache_ex(pray_ctx *ctx, int scene_id, int cache_id, const float *camera16,
+                                     uint32_t width, uint32_t height, uint32_t frames, uint32_t samples_per_frame,
+                                     uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *background,
+                                     float det_eps, uint32_t train_paths, uint32_t cache_bounce, float max_loss,
+                                     uint32_t min_steps, float *hdr, uint8_t *rgb, float *frame_stats);
 
 /* Device-memory and timing utilities (no reference counterpart: the
  * reference allocates and copies inside every call).  They let a caller keep

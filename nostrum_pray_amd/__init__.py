@@ -111,6 +111,8 @@ def libraries():
                                        C.POINTER(C.c_int)]),
         "pray_render_pt_cache": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u64, _vp, _f32,
                                            _u32, _u32, _f32, _u32, _vp, _vp, _vp]),
+        "pray_render_pt_cache_ex": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u64,
+                                              _vp, _f32, _u32, _u32, _f32, _u32, _vp, _vp, _vp]),
         "pray_render_pt_progressive": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u64, _vp,
                                                  _f32, _vp, _vp, _vp]),
         "pray_build_scene": (C.c_int, [_vp, _vp, _vp, _vp, _u32, C.c_int, _u32, _u32, _f32, _f32,
@@ -726,17 +728,25 @@ class Cache:
         return {"steps": st.value, "loss": ll.value, "loss_running": lr.value, "cache_on": bool(on.value)}
 
     def render_pt(self, sid, cam16, width, height, frames, max_depth, rr_depth, seed, background, det_eps,
-                  train_paths=1 << 16, cache_bounce=2, max_loss=1e30, min_steps=1, want_rgb=False):
-        """Path tracing with this cache (pray_render_pt_cache); returns
+                  train_paths=1 << 16, cache_bounce=2, max_loss=1e30, min_steps=1, want_rgb=False,
+                  samples_per_frame=1):
+        """Path tracing with this cache (pray_render_pt_cache, or
+        pray_render_pt_cache_ex with samples_per_frame > 1); returns
         (hdr, rgb or None, per-frame {loss, cache on} array)."""
         cam16 = np.ascontiguousarray(cam16, np.float32)
         bg = np.ascontiguousarray(background, np.float32).reshape(3)
         hdr = np.empty((height, width, 3), np.float32)
         rgb = np.empty((height, width, 3), np.uint8) if want_rgb else None
         stats = np.empty((frames, 2), np.float32)
-        self.ctx._check(self.ctx._lib.pray_render_pt_cache(
-            self.ctx._h, sid, self.id, _p(cam16), width, height, frames, max_depth, rr_depth, seed, _p(bg), det_eps,
-            train_paths, cache_bounce, max_loss, min_steps, _p(hdr), _p(rgb), _p(stats)))
+        if samples_per_frame == 1:
+            self.ctx._check(self.ctx._lib.pray_render_pt_cache(
+                self.ctx._h, sid, self.id, _p(cam16), width, height, frames, max_depth, rr_depth, seed, _p(bg),
+                det_eps, train_paths, cache_bounce, max_loss, min_steps, _p(hdr), _p(rgb), _p(stats)))
+        else:
+            self.ctx._check(self.ctx._lib.pray_render_pt_cache_ex(
+                self.ctx._h, sid, self.id, _p(cam16), width, height, frames, samples_per_frame, max_depth, rr_depth,
+                seed, _p(bg), det_eps, train_paths, cache_bounce, max_loss, min_steps, _p(hdr), _p(rgb),
+                _p(stats)))
         return hdr, rgb, stats
 
 

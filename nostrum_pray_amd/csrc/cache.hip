@@ -346,6 +346,14 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
                          uint32_t frames, uint32_t max_depth, uint32_t rr_depth, uint64_t seed, const float *bg,
                          float det_eps, uint32_t train_paths, uint32_t cache_bounce, float max_loss,
                          uint32_t min_steps, float *hdr, uint8_t *rgb, float *frame_stats) {
+    return pray_render_pt_cache_ex(c, scene_id, cache_id, cam16, w, h, frames, 1, max_depth, rr_depth, seed, bg,
+                                   det_eps, train_paths, cache_bounce, max_loss, min_steps, hdr, rgb, frame_stats);
+}
+
+int pray_render_pt_cache_ex(pray_ctx *c, int scene_id, int cache_id, const float *cam16, uint32_t w, uint32_t h,
+                            uint32_t frames, uint32_t spf, uint32_t max_depth, uint32_t rr_depth, uint64_t seed,
+                            const float *bg, float det_eps, uint32_t train_paths, uint32_t cache_bounce,
+                            float max_loss, uint32_t min_steps, float *hdr, uint8_t *rgb, float *frame_stats) {
     if (!c) return PRAY_ERR_INVALID;
     SceneRec *sc = find_scene(c, scene_id);
     if (!sc) return fail(c, PRAY_ERR_NOSCENE, "unknown scene %d", scene_id);
@@ -355,6 +363,9 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
     const uint64_t npix = (uint64_t)w * h;
     if (npix == 0 || frames == 0) return PRAY_OK;
     if (npix > (1u << 28)) return fail(c, PRAY_ERR_INVALID, "image too large");
+    if (spf == 0 || npix * spf > (1ull << 31))
+        return fail(c, PRAY_ERR_INVALID, "samples_per_frame x pixels must be in [1, 2^31]");
+    if ((uint64_t)frames * spf > (1u << 24)) return fail(c, PRAY_ERR_INVALID, "more than 2^24 samples");
     if (train_paths == 0 || train_paths > r->max_batch)
         return fail(c, PRAY_ERR_INVALID, "train_paths must be in [1, %u]", r->max_batch);
     if (cache_bounce >= max_depth) return fail(c, PRAY_ERR_INVALID, "cache_bounce must be below max_depth");
@@ -372,20 +383,21 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
     struct Buf {
         void *p = nullptr;
     } prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, queries, hist, trays, tstate, thit, ttt, tq;
+    const uint64_t npath = npix * spf;  // paths of a frame: spf samples of every pixel
     WS_GET(c, 16, prim.p, npix * 24);
     for (int q = 0; q < 2; q++) {
-        WS_GET(c, 17 + q, rq[q].p, npix * 24);
-        WS_GET(c, 19 + q, sq[q].p, npix * 16);
+        WS_GET(c, 17 + q, rq[q].p, npath * 24);
+        WS_GET(c, 19 + q, sq[q].p, npath * 16);
     }
-    // counters: [0], [1] path queues, [2] frame, [3] render queries,
-    // [4] training queries
+    // counters: [0], [1] path queues, [2] first sample of the frame,
+    // [3] render queries, [4] training queries
     WS_GET(c, 21, cnt.p, 8 * sizeof(uint32_t));
-    WS_GET(c, 22, hit.p, npix * 4);
-    WS_GET(c, 23, tt.p, npix * 4);
-    WS_GET(c, 24, contrib.p, npix * 12);
+    WS_GET(c, 22, hit.p, npath * 4);
+    WS_GET(c, 23, tt.p, npath * 4);
+    WS_GET(c, 24, contrib.p, npath * 12);
     WS_GET(c, 25, sum.p, npix * 12);
     WS_GET(c, 26, outh.p, npix * 12);
-    WS_GET(c, 27, queries.p, npix * 64);
+    WS_GET(c, 27, queries.p, npath * 64);
     WS_GET(c, 28, hist.p, (size_t)frames * 8);
     WS_GET(c, 29, trays.p, (size_t)n_train * 24);
     WS_GET(c, 30, tstate.p, (size_t)n_train * 16);
@@ -409,6 +421,7 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
     P.k0 = (uint32_t)seed, P.k1 = (uint32_t)(seed >> 32);
     P.bg[0] = bg[0], P.bg[1] = bg[1], P.bg[2] = bg[2];
     P.sample = count + 2;
+    P.samples = spf;
     prr::CacheArgs C{};
     C.cache_bounce = cache_bounce;
     C.cache_on = r->ctr + 2;
@@ -425,14 +438,14 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
         C.lo[k] = lo;
         C.inv[k] = ext > 0.f ? 1.f / ext : 0.f;
     }
-    int rc = prepare_trace(c, *sc, std::max<uint64_t>(npix, n_train));
+    int rc = prepare_trace(c, *sc, std::max<uint64_t>(npath, n_train));
     if (rc) return rc;
     if (!c->side) HIPC(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     if (!c->fork_ev) HIPC(c, hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
     if (!c->join_ev) HIPC(c, hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
     hipStream_t side = c->side;
     const uint32_t step_base = r->steps;
-    const unsigned infer_grid = (unsigned)std::min<uint64_t>((npix + 63) / 64, c->num_cus * 8ull);
+    const unsigned infer_grid = (unsigned)std::min<uint64_t>((npath + 63) / 64, c->num_cus * 8ull);
 
     // One frame (one sample per pixel): paths end in the cache at bounce
     // cache_bounce while it is on; training pixels leave training rays; the
@@ -443,8 +456,8 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
         hipLaunchKernelGGL(prr::k_fill_f32, dim3(nblocks(4 * (uint64_t)n_train, 256)), dim3(256), 0, st,
                            (float *)r->targets, 4u * n_train, 0.f);
         hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, r->ctr, 1u);
-        hipLaunchKernelGGL(prr::k_pt_generate, dim3(nblocks(npix, 256)), dim3(256), 0, st, (const float *)prim.p,
-                           (uint32_t)npix, (uint32_t)npix, (float *)rq[0].p, (float4 *)sq[0].p, count + 0);
+        hipLaunchKernelGGL(prr::k_pt_generate, dim3(nblocks(npath, 256)), dim3(256), 0, st, (const float *)prim.p,
+                           (uint32_t)npix, (uint32_t)npath, (float *)rq[0].p, (float4 *)sq[0].p, count + 0);
         hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + 3, 2u);
         for (uint32_t b = 0; b <= max_depth; b++) {
             const int cur = b & 1, nxt = cur ^ 1;
@@ -452,12 +465,12 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
             int e;
             {
                 Range r("cache extend");
-                e = enqueue_trace(c, *sc, (const float *)rq[cur].p, npix, count + cur, PRAY_TRACE_CLOSEST, det_eps,
+                e = enqueue_trace(c, *sc, (const float *)rq[cur].p, npath, count + cur, PRAY_TRACE_CLOSEST, det_eps,
                                   nullptr, (int32_t *)hit.p, (float *)tt.p, nullptr, st);
             }
             if (e) return e;
             Range r("cache shade");
-            hipLaunchKernelGGL(prr::k_pt_shade<prr::PT_CACHE>, dim3(nblocks(npix, 256)), dim3(256), 0, st, sc->d, P,
+            hipLaunchKernelGGL(prr::k_pt_shade<prr::PT_CACHE>, dim3(nblocks(npath, 256)), dim3(256), 0, st, sc->d, P,
                                C, b, (const float *)rq[cur].p, (const float4 *)sq[cur].p,
                                (const uint32_t *)(count + cur), (const int32_t *)hit.p, (const float *)tt.p,
                                (float *)rq[nxt].p, (float4 *)sq[nxt].p, count + nxt, (float *)contrib.p, prr::BatchArgs{});
@@ -485,7 +498,7 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
             HIPC(c, hipEventRecord(c->join_ev, side));
             prm::InferArgs ia{};
             ia.queries = (const float *)queries.p;
-            ia.n = (uint32_t)npix;
+            ia.n = (uint32_t)npath;
             ia.n_dev = count + 3;
             ia.contrib = (float *)contrib.p;
             hipLaunchKernelGGL(prm::k_mlp_infer, dim3(infer_grid), dim3(256), 0, st, (const prm::bf16x8 *)r->ffrag, ia);
@@ -496,7 +509,7 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
             e = enqueue_train(c, *r, n_train, true, max_loss, min_steps, (float *)hist.p, step_base, frames, st);
         }
         if (e) return e;
-        hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix, 1u,
+        hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix, spf,
                            (const float *)contrib.p, (float *)sum.p, count + 2);
         HIPC(c, hipGetLastError());
         return PRAY_OK;
@@ -534,7 +547,7 @@ int pray_render_pt_cache(pray_ctx *c, int scene_id, int cache_id, const float *c
     }
     r->steps += frames;
     hipLaunchKernelGGL(prr::k_pt_finalize, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
-                       (const float *)sum.p, (float)frames, (float *)outh.p, (uint8_t *)outb.p);
+                       (const float *)sum.p, (float)((uint64_t)frames * spf), (float *)outh.p, (uint8_t *)outb.p);
     HIPC(c, hipGetLastError());
     if (hdr) HIPC(c, hipMemcpyAsync(hdr, outh.p, npix * 12, hipMemcpyDeviceToHost, st));
     if (rgb) HIPC(c, hipMemcpyAsync(rgb, outb.p, npix * 3, hipMemcpyDeviceToHost, st));

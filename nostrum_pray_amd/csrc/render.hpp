@@ -241,9 +241,9 @@ struct PtArgs {
     uint32_t n_pix, max_depth, rr_depth, k0, k1;
     float bg[3];
     const uint32_t *sample;   // first sample index of the pass (device)
-    // Samples of the plain path tracer in flight together: path p of a pass
-    // is sample *sample + p / n_pix of pixel p % n_pix (1 for the cache and
-    // batch tracers, whose paths are pixels).
+    // Samples in flight together: path p of a pass (a frame of the cache
+    // tracer) is sample *sample + p / n_pix of pixel p % n_pix (1 for the
+    // batch tracer, whose paths are pixels).
     uint32_t samples = 1;
 };
 
@@ -409,13 +409,13 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
             const float ar = color[3 * (size_t)h], ag = color[3 * (size_t)h + 1], ab = color[3 * (size_t)h + 2];
             const float nx = normal[3 * (size_t)h], ny = normal[3 * (size_t)h + 1], nz = normal[3 * (size_t)h + 2];
             const bool at_cache = MODE == PT_CACHE && bounce == C.cache_bounce;
-            if (at_cache) {
-                const uint32_t d = (pix + P.n_pix - train_offset(*P.sample, P.n_pix, P.k0, P.k1)) % P.n_pix;
+            if (at_cache && smp == *P.sample) {  // training pixels: from the frame's first sample
+                const uint32_t d = (lpix + P.n_pix - train_offset(smp, P.n_pix, P.k0, P.k1)) % P.n_pix;
                 if (d % C.stride == 0u && d / C.stride < C.n_train) {
                     tk = d / C.stride;
                     train = true;
                     cache_features(C, ox, oy, oz, r[3], r[4], r[5], nx, ny, nz, ar, ag, ab, C.feats + 10 * (size_t)tk);
-                    const uint4 u = prd::philox4x32_10(make_uint4(pix, *P.sample, bounce, 1u), P.k0, P.k1);
+                    const uint4 u = prd::philox4x32_10(make_uint4(lpix, smp, bounce, 1u), P.k0, P.k1);
                     prd::sample_hemisphere(nx, ny, nz, prd::u01(u.x), prd::u01(u.y), tdx, tdy, tdz);
                 }
             }

@@ -119,30 +119,36 @@ def cornell_small():
     return s, t, cam
 
 
-def test_fallback_equals_plain_path_tracer(ctx):
+@pytest.mark.parametrize("frames,spf", [(6, 1), (2, 3), (1, 6)])
+def test_fallback_equals_plain_path_tracer(ctx, frames, spf):
     """With the cache never switched on, every frame is plain path tracing
-    with the plain tracer's random numbers: bitwise the same image."""
+    with the plain tracer's random numbers: bitwise the same image, whether
+    the 6 samples come as 6 frames of one sample, 2 of 3 or 1 of 6 (one
+    training step per frame)."""
     s, t, cam = cornell_small()
     sid = ctx.upload(t)
     c = P.Cache(ctx, seed=1, max_batch=4096)
     try:
-        say("cache fallback vs plain")
+        say(f"cache fallback vs plain, {frames} frames x {spf} samples")
         bg = s.info()["background"]
-        hdr_c, _, stats = c.render_pt(sid, cam, 96, 72, 6, 8, 3, 41, bg, 1e-9, train_paths=4096, cache_bounce=2,
-                                      max_loss=0.0, min_steps=1)
+        hdr_c, _, stats = c.render_pt(sid, cam, 96, 72, frames, 8, 3, 41, bg, 1e-9, train_paths=4096,
+                                      cache_bounce=2, max_loss=0.0, min_steps=1, samples_per_frame=spf)
         hdr_p, _ = ctx.render_pt(sid, cam, 96, 72, 6, 8, 3, 41, bg, 1e-9, want_rgb=False)
         assert np.array_equal(hdr_c.view(np.uint32), hdr_p.view(np.uint32))
         assert np.all(stats[:, 1] == 0) and np.all(stats[:, 0] > 0)
-        assert c.stats()["steps"] == 6
+        assert c.stats()["steps"] == frames
+        with pytest.raises(P.PrayError):
+            c.render_pt(sid, cam, 96, 72, 1, 8, 3, 41, bg, 1e-9, train_paths=4096, samples_per_frame=0)
     finally:
         c.close()
         ctx.free_scene(sid)
 
 
-def test_trained_cache_lowers_error_at_equal_samples(ctx):
-    """Against a 2048-spp plain reference, 32 frames whose paths end in the
-    cache at their second hit have a lower RMSE than 32 plain samples per
-    pixel."""
+@pytest.mark.parametrize("frames,spf", [(32, 1), (8, 4)])
+def test_trained_cache_lowers_error_at_equal_samples(ctx, frames, spf):
+    """Against a 2048-spp plain reference, 32 samples per pixel whose paths
+    end in the cache at their second hit (32 frames of one sample, or 8 of
+    four) have a lower RMSE than 32 plain samples per pixel."""
     s, t, cam = cornell_small()
     sid = ctx.upload(t)
     c = P.Cache(ctx, seed=2, max_batch=1 << 14)
@@ -157,8 +163,8 @@ def test_trained_cache_lowers_error_at_equal_samples(ctx):
         # 16 steps were taken
         warm, _, _ = c.render_pt(sid, cam, 96, 72, 64, 8, 3, 99, bg, 1e-9, train_paths=1 << 12, cache_bounce=1,
                                  min_steps=16)
-        hdr, _, stats = c.render_pt(sid, cam, 96, 72, 32, 8, 3, 7, bg, 1e-9, train_paths=1 << 12, cache_bounce=1,
-                                    min_steps=16)
+        hdr, _, stats = c.render_pt(sid, cam, 96, 72, frames, 8, 3, 7, bg, 1e-9, train_paths=1 << 12,
+                                    cache_bounce=1, min_steps=16, samples_per_frame=spf)
         assert np.all(stats[:, 1] == 1)
         rmse = lambda a: float(np.sqrt(np.mean((a - ref) ** 2)))  # noqa: E731
         say(f"rmse plain {rmse(plain):.4f} cache {rmse(hdr):.4f}")
@@ -171,7 +177,8 @@ def test_trained_cache_lowers_error_at_equal_samples(ctx):
         ctx.free_scene(sid)
 
 
-def test_cache_frames_without_graphs_equal_graph_replays(ctx):
+@pytest.mark.parametrize("spf", [1, 4])
+def test_cache_frames_without_graphs_equal_graph_replays(ctx, spf):
     """The cache tracer's frames enqueued kernel by kernel (pray_use_graphs
     off, as profilers see them) give bitwise the images, losses and weights of
     the captured-graph replays, with the cache on after two steps."""
@@ -186,7 +193,7 @@ def test_cache_frames_without_graphs_equal_graph_replays(ctx):
             try:
                 say(f"cache frames, graphs {graphs}")
                 hdr, _, stats = c.render_pt(sid, cam, 96, 72, 8, 8, 3, 11, bg, 1e-9, train_paths=1 << 12,
-                                            cache_bounce=1, min_steps=2)
+                                            cache_bounce=1, min_steps=2, samples_per_frame=spf)
                 out.append((hdr, stats, c.params()))
             finally:
                 c.close()
