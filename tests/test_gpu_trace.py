@@ -39,13 +39,15 @@ def scene(name):
 # Every traversal setting gives the same hits:
 # (layout, LDS ring entries, waves per workgroup, leaf ILP, refill threshold,
 # leaf batch, tree levels per node step).  The first is the setting the
-# autotuner picked for the benchmark's 2048x2048 terrain before leaf batching
-# (packed, ring 8, one-wave workgroups, ILP 1, refill at 24 idle lanes), the
-# second with it (ILP 2, refill at 16, leaf steps once 16 lanes wait).  Two
-# levels per step need a tree whose children pairs have equal clips (every
-# SAH tree; the terrain's median tree too, whose children meet on grid lines);
-# other trees refuse them.
-SETTINGS = [(P.LAYOUT_PACKED, 8, 1, 1, 24, 0, 1), (P.LAYOUT_PACKED, 8, 1, 2, 16, 16, 1),
+# autotuner picks for the benchmark's 2048x2048 terrain now (two levels per
+# node step, ring 6, ILP 2, refill at 16 idle lanes); the next two are the
+# ones it picked before two-level records (ring 8, ILP 1, refill at 24) and
+# before that with leaf batching (ILP 2, refill at 16, leaf steps once 16
+# lanes wait).  Two levels per step need a tree whose children pairs have
+# equal clips (every SAH tree; the terrain's median tree too, whose children
+# meet on grid lines); other trees refuse them.
+SETTINGS = [(P.LAYOUT_PACKED, 6, 1, 2, 16, 0, 2),
+            (P.LAYOUT_PACKED, 8, 1, 1, 24, 0, 1), (P.LAYOUT_PACKED, 8, 1, 2, 16, 16, 1),
             (P.LAYOUT_PACKED, 5, 2, 2, 16, 0, 1), (P.LAYOUT_INDEXED, 8, 4, 1, 32, 0, 1),
             (P.LAYOUT_INDEXED, 6, 1, 2, 16, 0, 1), (P.LAYOUT_PACKED, 6, 1, 2, 48, 24, 1),
             (P.LAYOUT_PACKED, 4, 2, 2, 16, 32, 1), (P.LAYOUT_PACKED, 16, 4, 1, 48, 64, 1),

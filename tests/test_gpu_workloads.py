@@ -20,8 +20,11 @@ from conftest import first_diff, say
 
 pytestmark = pytest.mark.gpu
 
-BENCH_TUNING = dict(layout=P.LAYOUT_PACKED, stack_lds=8, leaf_ilp=2, fetch_min=16, occupancy=100, wg_waves=1,
-                    leaf_batch=16, node_levels=1)
+# The setting the autotuner picks for the benchmark's stream (two tree levels
+# per node step, ring of 6, ILP 2, refill at 16), compared with the oracle
+# first; the others must give the same hits.
+BENCH_TUNING = dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, occupancy=100, wg_waves=1,
+                    leaf_batch=0, node_levels=2)
 
 
 def test_terrain2048_full_stream_equals_oracle(ctx):
@@ -45,11 +48,11 @@ def test_terrain2048_full_stream_equals_oracle(ctx):
         assert np.array_equal(rays[:1 << 16].view(np.uint32), O.gen_rays(2024, box, 1 << 16).view(np.uint32))
         sample = np.concatenate([np.arange(1 << 16), np.arange(1 << 16, n, 251)])
         first = None
-        for tun in (BENCH_TUNING, dict(BENCH_TUNING, leaf_ilp=1, fetch_min=24, leaf_batch=0),
-                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0),
+        for tun in (BENCH_TUNING, dict(BENCH_TUNING, stack_lds=8, leaf_ilp=1, fetch_min=24, node_levels=1),
+                    dict(BENCH_TUNING, stack_lds=8, leaf_batch=16, node_levels=1),
                     dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=2, fetch_min=32, wg_waves=2, leaf_batch=0,
                          node_levels=1),
-                    dict(BENCH_TUNING, node_levels=2), dict(BENCH_TUNING, leaf_batch=0, leaf_ilp=1, node_levels=2)):
+                    dict(BENCH_TUNING, stack_lds=8, leaf_batch=16), dict(BENCH_TUNING, leaf_ilp=1)):
             ctx.set_tuning(sid, **tun)
             say(f"terrain 2048: closest hit, 2^24 rays, {tun}")
             ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, P.TRACE_CLOSEST, 1e-12)
