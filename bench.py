@@ -903,7 +903,7 @@ def radiance_cache_leg(P, ctx, args):
         return float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
 
     plain = []
-    for spp in (16, 32, 64, 128, 256, 512, 1024):
+    for spp in (16, 32, 64, 128, 256, 512, 1024, 2048):
         t0 = time.perf_counter()
         img, _ = ctx.render_pt(sid, cam, w, h, spp, depth, rr, 7, bg, 1e-9, want_rgb=False)
         ms = (time.perf_counter() - t0) * 1e3
@@ -913,7 +913,7 @@ def radiance_cache_leg(P, ctx, args):
     # plain frames before the cache takes over; training samples per frame (2^16, the workload's);
     # samples per pixel in a frame (one training step each)
     warm, train, spf = 4, 1 << 16, args.cache_spf
-    for total in (32, 64, 128, 256, 512):
+    for total in (32, 64, 128, 256, 512, 1024):
         frames = max(1, total // spf)
         c = P.Cache(ctx, seed=3, learning_rate=1e-2, max_batch=1 << 16)
         t0 = time.perf_counter()

@@ -8,10 +8,12 @@
 //
 // Features per sample: position in the scene box mapped to [0,1]^3, ray
 // direction and surface normal as (theta/pi, phi/2pi + 1/2), albedo rgb.
-// Encoding (192 input channels; see enc_value): with channel c held in slot
+// Encoding (192 input channels; see enc_next): with channel c held in slot
 // s of lane group g of reduction block m (c = kchan(m, g, s)) and
 // q = 8m + s, e = 4(q % 6) + g:
 //   q < 42   sin (e even) or cos (e odd) of 2 pi 2^(e/2 - 1) x_f, f = q / 6
+//            (the lowest frequency of a feature evaluated, the others by
+//            angle doubling)
 //   q == 42  albedo r, g, b for g = 0, 1, 2 and the constant 1 for g = 3
 //   q > 42   0
 // so the feature read by a slot does not depend on the lane.
@@ -94,23 +96,38 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f3
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Encoded input held in slot s of lane group g of reduction block m (m and s
-// are compile-time constants where this is inlined, so f is indexed
-// statically).
-__device__ __forceinline__ float enc_value(const float *f, int m, int g, int s) {
+// Encoded inputs in slot order.  Slot q = 8m + s of lane group g holds, for
+// q < 42, sin (g even) or cos (g odd) of 2 pi 2^(2j + h - 1) x_f with
+// f = q / 6, j = q % 6, h = g / 2; q == 42 the albedo (g < 3) or the constant
+// 1; q > 42 zero.  The six frequencies of a feature in one lane grow by 4x
+// from slot to slot, so only the first is evaluated (sincos2pi of the
+// fraction of 2^(h-1) x_f); each next one comes from the previous by two
+// angle doublings, sin 2a = 2 sin a cos a and cos 2a = (cos a - sin a)(cos a
+// + sin a), in FP32 (within 1e-4 of the exact value at the highest
+// frequency).  Calls must come in increasing q (m outer, s inner, both
+// compile-time constants where this is inlined); `sn`, `cs` carry the
+// current feature's sine and cosine from one call to the next.
+__device__ __forceinline__ void enc_double(float &sn, float &cs) {
+    const float s2 = 2.0f * (sn * cs);
+    const float c2 = (cs - sn) * (cs + sn);
+    sn = s2, cs = c2;
+}
+
+__device__ __forceinline__ float enc_next(const float *f, int m, int g, int s, float &sn, float &cs) {
     const int q = 8 * m + s;
     if (q < 42) {
-        const int e = 4 * (q % 6) + g;
-        const float x = ldexpf(f[q / 6], (e >> 1) - 1);
-        const float u = x - floorf(x);
-        float sn, cs;
-        prd::sincos2pi(u, sn, cs);
-        return (e & 1) ? cs : sn;
+        if (q % 6 == 0) {
+            const float x = ldexpf(f[q / 6], (g >> 1) - 1);
+            prd::sincos2pi(x - floorf(x), sn, cs);
+        } else {
+            enc_double(sn, cs);
+            enc_double(sn, cs);
+        }
+        return (g & 1) ? cs : sn;
     }
     if (q == 42) return g == 0 ? f[7] : (g == 1 ? f[8] : (g == 2 ? f[9] : 1.f));
     return 0.f;
 }
-
 
 // Fragment positions of master parameter p (forward and, for layers >= 1,
 // backward / transposed).  Returns the layer.
@@ -179,10 +196,11 @@ __device__ __forceinline__ f32x4 forward16(const bf16x8 *__restrict__ Wf, const 
 }
 
 __device__ __forceinline__ void encode_frags(const float *f, uint32_t g, Frag (&X)[6]) {
+    float sn = 0.f, cs = 0.f;
 #pragma unroll
     for (int m = 0; m < 6; ++m)
 #pragma unroll
-        for (int s = 0; s < 8; ++s) X[m].h[s] = bf16_bits(enc_value(f, m, (int)g, s));
+        for (int s = 0; s < 8; ++s) X[m].h[s] = bf16_bits(enc_next(f, m, (int)g, s, sn, cs));
 }
 
 // Queries for inference: 16 floats each {features[10], weight rgb, slot bits,
@@ -226,13 +244,14 @@ static __global__ __launch_bounds__(256) void k_mlp_infer(const bf16x8 *__restri
         for (int ob = 0; ob < 4; ++ob)
 #pragma unroll
             for (int q = 0; q < INF_NB; ++q) acc[ob][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+        float sn[INF_NB] = {}, cs[INF_NB] = {};
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
             Frag X[INF_NB];
 #pragma unroll
             for (int q = 0; q < INF_NB; ++q)
 #pragma unroll
-                for (int s = 0; s < 8; ++s) X[q].h[s] = bf16_bits(enc_value(f[q], m, (int)g, s));
+                for (int s = 0; s < 8; ++s) X[q].h[s] = bf16_bits(enc_next(f[q], m, (int)g, s, sn[q], cs[q]));
 #pragma unroll
             for (int ob = 0; ob < 4; ++ob) {
                 const bf16x8 w = Wf[(ob * 6 + m) * WAVE + lane];

@@ -964,20 +964,26 @@ inline float bf16r(float x) {
 
 // Channel c sits in slot s of lane group g of the 32-channel block m
 // (c = 32m + 4g + s for s < 4, 32m + 16 + 4g + s - 4 otherwise); with
-// q = 8m + s and e = 4(q % 6) + g it holds sin (e even) / cos (e odd) of
-// 2 pi 2^(e/2 - 1) x_f, f = q / 6, for q < 42; albedo r, g, b and the
-// constant 1 for q == 42; zero beyond.
+// q = 8m + s, j = q % 6, h = g / 2 it holds sin (g even) / cos (g odd) of
+// 2 pi 2^(2j + h - 1) x_f, f = q / 6, for q < 42; albedo r, g, b and the
+// constant 1 for q == 42; zero beyond.  The FP32 value is sincos2pi of the
+// fraction of 2^(h-1) x_f followed by 2j angle doublings
+// (sin 2a = 2 (sin a cos a), cos 2a = (cos a - sin a)(cos a + sin a)).
 float mlp_encode1(const float *f, int c) {
     const int m = c >> 5, cc = c & 31;
     const int g = (cc & 15) >> 2, s = (cc & 3) + (cc >= 16 ? 4 : 0);
     const int q = 8 * m + s;
     if (q < 42) {
-        const int e = 4 * (q % 6) + g;
-        const float x = std::ldexp(f[q / 6], (e >> 1) - 1);
+        const float x = std::ldexp(f[q / 6], (g >> 1) - 1);
         const float u = x - std::floor(x);
         float sn, cs;
         sincos2pi(u, sn, cs);
-        return (e & 1) ? cs : sn;
+        for (int k = 0; k < 2 * (q % 6); k++) {
+            const float s2 = 2.0f * (sn * cs);
+            const float c2 = (cs - sn) * (cs + sn);
+            sn = s2, cs = c2;
+        }
+        return (g & 1) ? cs : sn;
     }
     if (q == 42) return g < 3 ? f[7 + g] : 1.f;
     return 0.f;
