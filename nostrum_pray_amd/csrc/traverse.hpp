@@ -99,6 +99,13 @@ __device__ __forceinline__ void slab(float bmin, float bmax, float o, float inv,
     hi = fminf(hi, whi(t1));
 }
 
+// The values must be in registers at this point: placed after a leaf step's
+// record loads, it keeps the compiler from sinking a load below the branch
+// that first uses it (a triangle's first vertex is only read after its
+// determinant test, which would cost the step a second memory round trip),
+// so all loads of the step are in flight together.
+__device__ __forceinline__ void hold(const float4 &v) { asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z)); }
+
 template <int LAYOUT>
 __device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, uint32_t k, uint32_t &ref, float4 &A,
                                          float4 &B, float4 &Cc) {
@@ -318,6 +325,8 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
             float4 ta[ILP], tb[ILP], tc[ILP];
 #pragma unroll
             for (int j = 0; j < ILP; ++j) load_tri<LAYOUT>(s, recs, k + j, ref[j], ta[j], tb[j], tc[j]);
+#pragma unroll
+            for (int j = 0; j < ILP; ++j) hold(ta[j]), hold(tb[j]), hold(tc[j]);
             bool leaf_done = false, found = false;
 #pragma unroll
             for (int j = 0; j < ILP; ++j) {
@@ -689,6 +698,8 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene
             float4 ta[ILP], tb[ILP], tc[ILP];
 #pragma unroll
             for (int j = 0; j < ILP; ++j) load_tri<1>(s, recs, k + j, ref[j], ta[j], tb[j], tc[j]);
+#pragma unroll
+            for (int j = 0; j < ILP; ++j) hold(ta[j]), hold(tb[j]), hold(tc[j]);
             bool leaf_done = false, found = false;
 #pragma unroll
             for (int j = 0; j < ILP; ++j) {
