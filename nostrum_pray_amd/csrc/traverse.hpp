@@ -105,6 +105,7 @@ __device__ __forceinline__ void slab(float bmin, float bmax, float o, float inv,
 // determinant test, which would cost the step a second memory round trip),
 // so all loads of the step are in flight together.
 __device__ __forceinline__ void hold(const float4 &v) { asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z)); }
+__device__ __forceinline__ void hold(const uint4 &v) { asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w)); }
 
 template <int LAYOUT>
 __device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, uint32_t k, uint32_t &ref, float4 &A,
@@ -149,6 +150,9 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
     const uint32_t nn = A.n_dev ? *A.n_dev : A.n;
     const uint4 *pairs = reinterpret_cast<const uint4 *>(s.nodes);
     const float4 *recs = s.recs;
+    // The root's word (node 1, the second word pair of the first 16 bytes),
+    // read once per wave, not once per ray (single-scene launches).
+    const uint32_t root_w = MULTI ? 0u : (uint32_t)__builtin_amdgcn_readfirstlane((int)pairs[0].w);
 
     uint32_t state = ST_IDLE, ray = 0, w = 0, k = 0, best_i = NO_HIT, cn = 0, cr = 0;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, lim2 = 0;
@@ -230,7 +234,7 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
                     }
                     if (live && lo <= hi) {
                         cn = 1;
-                        w = pairs[0].w;  // node 1 = second word pair of the first 16 bytes
+                        w = MULTI ? pairs[0].w : root_w;  // node 1: the root
                         if ((w & 3u) != LEAF_TAG) {
                             state = ST_NODE;
                         } else if (!empty_leaf(w)) {
@@ -491,6 +495,8 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene
     const uint4 *pairs = reinterpret_cast<const uint4 *>(s.nodes);
     const uint4 *quads = s.quads;
     const float4 *recs = s.recs;
+    // The root's word (node 1), read once per wave, not once per ray.
+    const uint32_t root_w = (uint32_t)__builtin_amdgcn_readfirstlane((int)pairs[0].w);
     constexpr uint32_t ST_HALF = 4u;  // at a far child pushed by a record's first level
 
     uint32_t state = ST_IDLE, ray = 0, w = 0, k = 0, best_i = NO_HIT, cn = 0, cr = 0;
@@ -571,7 +577,7 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene
                     }
                     if (live && lo <= hi) {
                         cn = 1;
-                        w = pairs[0].w;  // node 1 = second word pair of the first 16 bytes
+                        w = root_w;
                         if ((w & 3u) != LEAF_TAG) {
                             state = ST_NODE;
                         } else if (!empty_leaf(w)) {
@@ -611,14 +617,18 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene
         bool need_pop = false;
         // ---------------------------------------------------- node step --
         if (run_node && (state == ST_NODE || state == ST_HALF)) {
-            // The child whose level comes next: its half record.
+            // The child whose level comes next: its half record.  Both
+            // halves of the record are loaded for every lane in the step (a
+            // far child's lane needs one of them), so that lanes at a node
+            // and lanes at a far child wait for one round trip together.
+            const uint32_t p = state == ST_NODE ? w >> 3 : (w & ~Q_HALF) >> 1;
+            const uint4 h0 = quads[2 * (size_t)p];
+            const uint4 h1 = quads[2 * (size_t)p + 1];
+            hold(h0), hold(h1);
             uint4 h;
             bool go = true;
             if (state == ST_NODE) {
                 // first level: node w, from its record's two halves
-                const uint32_t p = w >> 3;
-                const uint4 h0 = quads[2 * (size_t)p];
-                const uint4 h1 = quads[2 * (size_t)p + 1];
                 const uint32_t axis = w & 3u;
                 cn += 2;
                 const float oa = axis == 0 ? ox : (axis == 1 ? oy : oz);
@@ -646,7 +656,7 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene
                 }
             } else {
                 // a far child pushed by a first level: its half alone
-                h = quads[2 * (size_t)((w & ~Q_HALF) >> 1) + (w & 1u)];
+                h = (w & 1u) ? h1 : h0;
             }
             if (go) {
                 const uint32_t kind = h.z >> 30;
