@@ -107,51 +107,6 @@ __device__ __forceinline__ void slab(float bmin, float bmax, float o, float inv,
 __device__ __forceinline__ void hold(const float4 &v) { asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z)); }
 __device__ __forceinline__ void hold(const uint4 &v) { asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w)); }
 
-// A wave's source of ray chunks: the counter it draws from and a spare chunk.
-// A claim takes two chunks from the current counter with one atomic (the
-// spare serves the next refill without a memory access); a counter found
-// drained sends the wave on to the next ones, each checked with a relaxed
-// read before its atomic.
-struct ChunkSource {
-    uint32_t ctr;    // wave-uniform: the counter to draw from
-    uint32_t spare;  // wave-uniform: a claimed chunk not yet taken, or nchunks
-};
-
-__device__ __forceinline__ uint32_t chunk_count(uint32_t ctr, uint32_t nchunks) {
-    return ctr < nchunks ? (nchunks - ctr + NCTR - 1) / NCTR : 0u;
-}
-
-// The next chunk for the wave (nchunks when none is left); called by every
-// lane with the first idle lane's index.
-__device__ __forceinline__ uint32_t take_chunk(ChunkSource &cs, uint32_t *work, uint32_t nchunks, uint32_t lane,
-                                               uint32_t first) {
-    if (cs.spare < nchunks) {
-        const uint32_t c = cs.spare;
-        cs.spare = nchunks;
-        return c;
-    }
-    uint32_t c = nchunks, spare = nchunks, ctr = cs.ctr;
-    if (lane == first) {
-        for (uint32_t tries = 0; tries < NCTR; ++tries) {
-            const uint32_t own = chunk_count(ctr, nchunks);
-            uint32_t *p = work + ctr * CTR_STRIDE;
-            if (tries == 0 || __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < own) {
-                const uint32_t i = atomicAdd(p, 2u);
-                if (i < own) {
-                    c = ctr + i * NCTR;
-                    if (i + 1 < own) spare = c + NCTR;
-                    break;
-                }
-            }
-            ctr = ctr + 1 == NCTR ? 0u : ctr + 1;
-        }
-    }
-    c = (uint32_t)__builtin_amdgcn_readlane((int)c, (int)first);
-    cs.spare = (uint32_t)__builtin_amdgcn_readlane((int)spare, (int)first);
-    cs.ctr = (uint32_t)__builtin_amdgcn_readlane((int)ctr, (int)first);
-    return c;
-}
-
 template <int LAYOUT>
 __device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, uint32_t k, uint32_t &ref, float4 &A,
                                          float4 &B, float4 &Cc) {
@@ -203,11 +158,10 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, lim2 = 0;
     float lo = 0, hi = 0, best_t = INFINITY;
     int sp = 0, base = 0;
-    // Wave-uniform pool of ray indices [pool_next, pool_end) and the source
-    // of the wave's next chunks.
+    // Wave-uniform pool of ray indices [pool_next, pool_end) and the counter
+    // the wave takes its next chunk from.
     const uint32_t nchunks = (nn + WAVE - 1) / WAVE;
-    uint32_t pool_next = 0, pool_end = 0;
-    ChunkSource cs{gwave % NCTR, nchunks};
+    uint32_t pool_next = 0, pool_end = 0, ctr = gwave % NCTR;
 
     while (true) {
         // ------------------------------------------------------- refill --
@@ -221,9 +175,29 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
             const uint32_t avail = pool_end - pool_next;
             uint32_t nb = 0;
             if (avail < nidle) {
-                // none left: nb points past the last ray
+                // One lane walks the counters from the current one until a
+                // chunk is left (a relaxed read skips drained counters without
+                // an atomic); none left: nb points past the last ray.
                 const uint32_t first = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
-                nb = take_chunk(cs, A.work, nchunks, lane, first) * WAVE;
+                uint32_t c = nchunks;
+                if (lane == first) {
+                    for (uint32_t tries = 0; tries < NCTR; ++tries) {
+                        const uint32_t own = ctr < nchunks ? (nchunks - ctr + NCTR - 1) / NCTR : 0u;
+                        uint32_t *p = A.work + ctr * CTR_STRIDE;
+                        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < own) {
+                            const uint32_t i = atomicAdd(p, 1u);
+                            if (i < own) {
+                                c = ctr + i * NCTR;
+                                break;
+                            }
+                        }
+                        ctr = ctr + 1 == NCTR ? 0u : ctr + 1;
+                    }
+                }
+                // wave-uniform from here: scalar registers
+                c = (uint32_t)__builtin_amdgcn_readlane((int)c, (int)first);
+                ctr = (uint32_t)__builtin_amdgcn_readlane((int)ctr, (int)first);
+                nb = c * WAVE;
             }
             const uint32_t r = (uint32_t)__popcll(idle & below);
             if (state == ST_IDLE) {
@@ -530,8 +504,7 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene
     float lo = 0, hi = 0, best_t = INFINITY;
     int sp = 0, base = 0;
     const uint32_t nchunks = (nn + WAVE - 1) / WAVE;
-    uint32_t pool_next = 0, pool_end = 0;
-    ChunkSource cs{gwave % NCTR, nchunks};
+    uint32_t pool_next = 0, pool_end = 0, ctr = gwave % NCTR;
 
     // Pushes (word, entry, exit) on the lane's stack, spilling the oldest LDS
     // ring entry to its HBM slot when the ring is full.
@@ -559,7 +532,24 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace_q(DevScene
             uint32_t nb = 0;
             if (avail < nidle) {
                 const uint32_t first = (uint32_t)(__ffsll((unsigned long long)idle) - 1);
-                nb = take_chunk(cs, A.work, nchunks, lane, first) * WAVE;
+                uint32_t c = nchunks;
+                if (lane == first) {
+                    for (uint32_t tries = 0; tries < NCTR; ++tries) {
+                        const uint32_t own = ctr < nchunks ? (nchunks - ctr + NCTR - 1) / NCTR : 0u;
+                        uint32_t *p = A.work + ctr * CTR_STRIDE;
+                        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < own) {
+                            const uint32_t i = atomicAdd(p, 1u);
+                            if (i < own) {
+                                c = ctr + i * NCTR;
+                                break;
+                            }
+                        }
+                        ctr = ctr + 1 == NCTR ? 0u : ctr + 1;
+                    }
+                }
+                c = (uint32_t)__builtin_amdgcn_readlane((int)c, (int)first);
+                ctr = (uint32_t)__builtin_amdgcn_readlane((int)ctr, (int)first);
+                nb = c * WAVE;
             }
             const uint32_t r = (uint32_t)__popcll(idle & below);
             if (state == ST_IDLE) {
