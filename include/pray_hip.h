@@ -403,8 +403,9 @@ PRAY_API int pray_use_graphs(pray_ctx *ctx, int enable);
  * samples (the last pass takes the rest).  Images, running sums and ray
  * counts do not depend on S.  Device memory of a call grows by about
  * 100 bytes per path in flight; the default is 2^24 paths (1.6 GB), and 0
- * traces one sample per pass.  No counterpart in the reference, whose
- * kernel traces one sample per pixel launch. */
+ * traces one sample per pass.  S is also capped so that the path buffers
+ * stay within half the device's free memory at the call.  No counterpart in
+ * the reference, whose kernel traces one sample per pixel launch. */
 PRAY_API int pray_set_path_budget(pray_ctx *ctx, uint64_t max_paths);
 /* Kernel timing: with enable = 1, every later trace that is not being
  * captured into a graph records events around its traversal kernel alone

@@ -552,15 +552,23 @@ void pray_destroy(pray_ctx *c) {
 
 const char *pray_last_error(const pray_ctx *c) { return c ? c->err.c_str() : "no context"; }
 
+// Free device memory of the context's device, in bytes (the test hook
+// PRAY_TEST_FREE_BYTES replaces the device's figure).
+static int free_device_bytes(pray_ctx *c, uint64_t *bytes) {
+    size_t fr = 0, tot = 0;
+    HIPC(c, hipMemGetInfo(&fr, &tot));
+    if (const char *f = std::getenv("PRAY_TEST_FREE_BYTES")) fr = (size_t)std::strtoull(f, nullptr, 10);
+    *bytes = fr;
+    return PRAY_OK;
+}
+
 int pray_mem_ok(pray_ctx *c, uint64_t n_tris, uint64_t n_refs, uint64_t n_nodes, uint64_t n_rays, int *ok) {
     if (!c || !ok) return fail(c, PRAY_ERR_INVALID, "null argument");
     HIPC(c, hipSetDevice(c->device));
-    size_t fr = 0, tot = 0;
-    HIPC(c, hipMemGetInfo(&fr, &tot));
-    // test hook: the free device memory to judge by, in bytes
-    if (const char *f = std::getenv("PRAY_TEST_FREE_BYTES")) fr = (size_t)std::strtoull(f, nullptr, 10);
+    uint64_t fr = 0;
+    if (int rc = free_device_bytes(c, &fr)) return rc;
     const uint64_t need = n_nodes * 8 + n_refs * (4 + 48) + n_tris * (36 + 24 + 1) + n_rays * (24 + 8 + 3 + 12 * 4);
-    *ok = (uint64_t)fr >= need ? 1 : 0;
+    *ok = fr >= need ? 1 : 0;
     return PRAY_OK;
 }
 
@@ -938,7 +946,17 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
     // context's path budget (at least one), so that every bounce's trace
     // launch holds samples x pixels paths.  Path p of a pass is sample
     // p / npix of pixel p % npix; results do not depend on the split.
-    const uint64_t per_pass = std::max<uint64_t>(1, std::min<uint64_t>(c->path_budget, 1ull << 31) / npix);
+    // The path buffers (100 bytes per path: two queues of rays and states,
+    // hit index and distance, contribution) also stay within half the free
+    // device memory, counting what the workspace already holds for them.
+    uint64_t per_pass = std::max<uint64_t>(1, std::min<uint64_t>(c->path_budget, 1ull << 31) / npix);
+    {
+        uint64_t fr = 0;
+        if (int rc = free_device_bytes(c, &fr)) return rc;
+        uint64_t held = 0;
+        for (size_t k = 1; k <= 9 && k < c->ws_bytes.size(); ++k) held += c->ws_bytes[k];
+        per_pass = std::min<uint64_t>(per_pass, std::max<uint64_t>(1, (fr / 2 + held) / 100 / npix));
+    }
     const uint32_t S = (uint32_t)std::min<uint64_t>(spp, per_pass);
     const uint64_t npath = npix * S;
     // Buffers from the context's render workspace (slots 0..12), reused by

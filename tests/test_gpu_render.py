@@ -132,6 +132,30 @@ def test_pt_cornell_equal_oracle(ctx, use_graph, budget):
     assert queues[0] == w * h * spp and (np.diff(queues.astype(np.int64)) <= 0).all() and queues[-1] > 0
 
 
+@pytest.mark.parametrize("free_paths", [3, 0])
+def test_pt_samples_in_flight_follow_free_memory(monkeypatch, free_paths):
+    """With little free device memory (forced through the test hook) the
+    path tracer keeps fewer samples in flight (3 of 8, or 1) and renders the
+    oracle's image all the same."""
+    s = P.Scene.cornell(3, 20000)
+    t = s.build(1)
+    w, h, spp, depth, rr = 64, 48, 8, 6, 2
+    cam = O.camera([0, 1, 3.8], [0, 0, -1], 40, w, h)
+    bg = np.asarray(s.info()["background"], np.float32)
+    monkeypatch.setenv("PRAY_TEST_FREE_BYTES", str(2 * free_paths * w * h * 100))
+    c = P.Context(0)  # a fresh workspace: nothing held to count as free
+    try:
+        sid = c.upload(t)
+        say(f"pt with free memory for {free_paths} samples in flight")
+        hdr, rgb = c.render_pt(sid, cam, w, h, spp, depth, rr, 77, bg, 1e-9)
+        queues = c.last_render_queues()
+    finally:
+        c.close()
+    oh, orgb, oq = O.render_pt(t.flat(), cam, w, h, spp, depth, rr, 77, bg, 1e-9, queues=True)
+    assert np.array_equal(hdr.view(np.uint32), oh.view(np.uint32)) and np.array_equal(rgb, orgb)
+    assert np.array_equal(queues, oq)
+
+
 def test_pt_reference_cornell_example_equal_oracle(ctx):
     s, t, f = load("pt_cornell.json")
     inf = s.info()
