@@ -114,11 +114,12 @@ def main():
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-RMSE measurement")
     ap.add_argument("--cache-size", type=int, default=1024, help="image side of the radiance-cache measurement")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples per pixel of the plain reference image")
-    ap.add_argument("--workload", choices=("terrain", "batch", "io", "cache", "build", "pt"), default="terrain",
+    ap.add_argument("--workload", choices=("terrain", "batch", "io", "cache", "build", "pt", "rt"), default="terrain",
                     help="terrain: the closest-hit stream (default); batch: 64 scenes path traced in one wavefront; "
                          "io: flat tree files, progressive path tracing and the CLI; cache: the radiance-cache "
                          "equal-RMSE measurement alone; build: the KD-tree builds on the GPU; pt: one plain and "
-                         "one radiance-cache render of the Cornell room (--cache-size, --pt-spp, --pt-frames)")
+                         "one radiance-cache render of the Cornell room (--cache-size, --pt-spp, --pt-frames); rt: "
+                         "the ray tracer on the 25,602-triangle spheres scene at 512x512 with both CLIs")
     ap.add_argument("--pt-spp", type=int, default=16, help="samples per pixel of the pt workload's plain render")
     ap.add_argument("--pt-frames", type=int, default=16, help="frames of the pt workload's cache render")
     ap.add_argument("--cache-spf", type=int, default=8,
@@ -150,6 +151,9 @@ def main():
         return
     if args.workload == "pt":
         pt_workload(P, args)
+        return
+    if args.workload == "rt":
+        rt_workload(P, args)
         return
     if args.workload == "cache":
         ctx = P.Context(0)
@@ -383,8 +387,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": ("k_trace_q (closest hit, two tree levels per node step)" if tun["node_levels"] == 2
-                       else "k_trace (closest hit)"),
+            "kernel": {2: "k_trace_q (closest hit, two tree levels per node step)",
+                       3: "k_trace (closest hit, boxed children pairs)"}.get(tun["node_levels"], "k_trace (closest hit)"),
             "kernel_ms": round(kernel_ms, 4),
             "kernel_ms_events_per_kernel": round(isolated_ms, 4),
             "bytes_per_ray": round(float(bytes_per_ray), 1),
@@ -830,6 +834,88 @@ def equal_rmse_time(points, target):
     k = min(max(k, 1), len(pts) - 1)
     slope = (lt[k] - lt[k - 1]) / (lr[k] - lr[k - 1])
     return float(np.exp(lt[k - 1] + slope * (np.log(target) - lr[k - 1])))
+
+
+def rt_workload(P, args):
+    """The ray tracer (pray_render_rt) on the spheres scene: a level-4
+    icosphere, 16 level-3 icospheres and a ground quad (25,602 triangles),
+    median KD tree, 512x512, fov 60, one point light.  --steps calls are
+    timed after --warmup (each call returns the 8-bit image to the host);
+    rays are the primary rays plus one shadow ray per lit hit.  The scene is
+    also written as JSON + OBJ and rendered by the drop-in CLI and, as the
+    CPU baseline, by the reference's own CLI compiled here; their images are
+    compared byte for byte.  One JSON line."""
+    import shutil
+    import tempfile
+
+    ctx = P.Context(0)
+    configure(ctx, args)
+    scene = P.Scene.spheres(1)
+    tree = scene.build(0)
+    sid = ctx.upload(tree)
+    tun = ctx.get_tuning(sid)
+    inf = scene.info()
+    w, h = inf["width"], inf["height"]
+    cam, lights, bg, eps = scene.camera(), scene.lights(), inf["background"], float(inf["det_eps"])
+    for _ in range(max(1, args.warmup)):
+        rgb, hits = ctx.render_rt(sid, cam, w, h, lights, bg, eps, want_hits=True)
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        rgb, hits = ctx.render_rt(sid, cam, w, h, lights, bg, eps, want_hits=True)
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(times))
+    rays = w * h + int((hits >= 0).sum()) * len(lights)
+    ctx.free_scene(sid)
+    ctx.close()
+    log(f"rt: {scene.num_tris} triangles {w}x{h}: {ms:.3f} ms per frame (median of {args.steps}), "
+        f"{rays} rays, {rays / ms / 1e3:.1f} Mrays/s")
+    tmp = tempfile.mkdtemp(prefix="pray_rt_")
+    try:
+        path = os.path.join(tmp, "spheres.json")
+        scene.write_json(path)
+        out = os.path.join(tmp, "gpu.bmp")
+        t0 = time.perf_counter()
+        r = subprocess.run([P.CLI, path, out], capture_output=True, text=True, timeout=300)
+        cli = {"exit": r.returncode, "wall_ms": round((time.perf_counter() - t0) * 1e3, 1), **cli_stage_ms(r.stdout)}
+        log(f"rt: drop-in CLI {cli}")
+        cpu = None
+        if not args.no_cpu_baseline:
+            cpu = rt_cpu_baseline(path, out if r.returncode == 0 else None, tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    print(json.dumps({
+        "metric": "Mrays/s ray tracing (primary + shadow rays), spheres scene", "value": round(rays / ms / 1e3, 2),
+        "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"ray tracer, {scene.num_tris} triangles (1 level-4 and 16 level-3 icospheres, a "
+                               f"ground quad), median KD tree, {w}x{h}, fov {inf['fov']}, {len(lights)} light"},
+        "ms_per_frame": round(ms, 4), "rays_per_frame": rays, "tuning": tun, "cli": cli, "cpu_baseline": cpu}),
+        flush=True)
+
+
+def rt_cpu_baseline(scene_json, gpu_bmp, tmp):
+    """The reference's own CLI, compiled from its sources into the oracle's
+    build folder, on the same scene file: its times and whether its image
+    equals the drop-in CLI's byte for byte.  None when that binary was not
+    built."""
+    import oracle as O
+
+    if not os.path.exists(O.REF_CLI):
+        return None
+    out = os.path.join(tmp, "cpu.bmp")
+    t0 = time.perf_counter()
+    r = subprocess.run([O.REF_CLI, scene_json, out], capture_output=True, text=True, timeout=600)
+    wall = (time.perf_counter() - t0) * 1e3
+    same = None
+    if r.returncode == 0 and gpu_bmp:
+        same = open(out, "rb").read() == open(gpu_bmp, "rb").read()
+    st = cli_stage_ms(r.stdout)
+    log(f"rt: reference CLI (CPU) exit {r.returncode}, {wall:.0f} ms {st}, image equal to the drop-in CLI's: {same}")
+    return {"kind": "reference", "cores": USABLE_CPUS, "exit": r.returncode, "wall_ms": round(wall, 1), **st,
+            "equal_to_gpu_cli_bmp": same,
+            "sample": f"the reference CLI built here without FMA contraction on the same scene file (it ignores "
+                      f"the file's tree key and builds its SAH tree, as its policy gives for 20k-1M triangles), "
+                      f"OpenMP on {cpu_model()}"}
 
 
 def pt_workload(P, args):

@@ -153,7 +153,12 @@ PRAY_API int pray_scene_layout(pray_ctx *ctx, int scene_id, uint64_t *sizes, uin
  *                  two levels; packed layout, and only for trees whose
  *                  children pairs have equal clips -- the SAH trees -- whose
  *                  records the upload or GPU build makes; PRAY_ERR_UNSUPPORTED
- *                  for other trees)
+ *                  for other trees); 3: it takes one level from a 64-byte
+ *                  record of both children's full boxes (packed layout, and
+ *                  only for trees with a children pair whose clips differ --
+ *                  median trees, whose children's boxes overlap -- whose
+ *                  records the upload makes and whose default this is;
+ *                  PRAY_ERR_UNSUPPORTED for other trees)
  * pray_set_tuning: -1 keeps a value.  Calls of one context must not overlap
  * (traces share its ray counter and stack spill area). */
 #define PRAY_TUNING_FIELDS 8

@@ -64,6 +64,9 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
     auto it = c->tuned.find(bucket);
     if (!force && it != c->tuned.end()) {
         sc->tune = it->second;
+        // the bucket's node form where this tree has it, else its own
+        const int nl = sc->tune.node_levels;
+        if ((nl == 2 && !sc->d.quads) || (nl == 3 && !sc->d.boxes)) sc->tune.node_levels = sc->d.boxes ? 3 : 1;
     } else {
         if (n_rays == 0) n_rays = 1u << 20;
         for (int k = 0; k < 2; k++)
@@ -74,7 +77,7 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
         HIPC(c, hipMalloc(&tt.p, (size_t)n_rays * 4));
         int rc = sample_rays(c, *sc, n_rays, 0x5EEDull + (uint64_t)scene_id, (float *)rays.p);
         if (rc) return rc;
-        // Grid over node levels per step x stack depth x leaf ILP x refill
+        // Grid over node records x stack depth x leaf ILP x refill
         // threshold at full occupancy; then leaf batching (with two refill
         // thresholds), the workgroup size and the occupancy of the winner
         // (packed layout).
@@ -85,8 +88,8 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
                                         (float *)tt.p, 2);
             if (ms < best_ms) best_ms = ms, best = cand;
         };
-        for (int nl : {1, 2}) {
-            if (nl == 2 && !sc->d.quads) continue;
+        for (int nl : {1, 2, 3}) {
+            if ((nl == 2 && !sc->d.quads) || (nl == 3 && !sc->d.boxes)) continue;
             for (int K : {4, 6, 8, 16})
                 for (int ilp : {1, 2})
                     for (int fm : {16, 24, 32, 48}) {
@@ -156,8 +159,8 @@ int pray_autotune_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, 
             return PRAY_OK;
         };
         int rc = PRAY_OK;
-        for (int nl : {1, 2}) {
-            if (nl == 2 && !sc->d.quads) continue;
+        for (int nl : {1, 2, 3}) {
+            if ((nl == 2 && !sc->d.quads) || (nl == 3 && !sc->d.boxes)) continue;
             for (int K : {4, 6, 8, 16})
                 for (int ilp : {1, 2})
                     for (int fm : {16, 24, 32, 48}) {

@@ -34,7 +34,8 @@ struct Tuning {
     int occupancy = 100;  // percent of the resident persistent waves launched
     int wg_waves = 1;     // waves per workgroup (1, 2 or 4)
     int leaf_batch = 0;   // 0: node and leaf steps in every iteration; T: leaf steps once T lanes wait at leaves
-    int node_levels = 1;  // tree levels per node step: 1 (children pairs) or 2 (two-level records, SAH trees)
+    int node_levels = 1;  // node records: 1 children pairs, 2 two-level records (SAH trees), 3 boxed children
+                          // pairs (median trees)
 };
 
 struct SceneRec {
@@ -122,7 +123,7 @@ int build_quads(pray_ctx *c, SceneRec &sc, hipStream_t st);
 int ensure_spill(pray_ctx *c, size_t bytes);
 // Render workspace slot `slot` with at least `bytes` bytes (kept by the
 // context until pray_destroy).  Slots: render_pt 0..15, the cache renderer
-// 16..39, the batch renderer 40..63.
+// 16..39, the batch renderer 40..63, render_rt 64..71.
 int ws_get(pray_ctx *c, unsigned slot, size_t bytes, void **out);
 #define WS_GET(ctx, slot, ptr, bytes)                                                                        \
     do {                                                                                                      \

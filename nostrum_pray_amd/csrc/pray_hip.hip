@@ -141,15 +141,16 @@ typedef void (*TraceKernel)(DevScene, TraceArgs);
 // 768*K bytes; with 160 KB of LDS per CU, K = 6 is the largest ring that
 // leaves room for 32 resident waves (the VGPR limit of the ILP-2 kernels)
 // and K = 5 for 36 (the ILP-1 kernels).
-template <bool ANY, int LAYOUT, bool BATCH>
+template <bool ANY, int LAYOUT, bool BATCH, bool BOX = false>
 TraceKernel pick_kernel_kl(int K, int ilp) {
+    using namespace prt;
     switch (K) {
-    case 4: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 4, 2, false, BATCH> : prt::k_trace<ANY, LAYOUT, 4, 1, false, BATCH>;
-    case 5: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 5, 2, false, BATCH> : prt::k_trace<ANY, LAYOUT, 5, 1, false, BATCH>;
-    case 6: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 6, 2, false, BATCH> : prt::k_trace<ANY, LAYOUT, 6, 1, false, BATCH>;
+    case 4: return ilp == 2 ? k_trace<ANY, LAYOUT, 4, 2, false, BATCH, BOX> : k_trace<ANY, LAYOUT, 4, 1, false, BATCH, BOX>;
+    case 5: return ilp == 2 ? k_trace<ANY, LAYOUT, 5, 2, false, BATCH, BOX> : k_trace<ANY, LAYOUT, 5, 1, false, BATCH, BOX>;
+    case 6: return ilp == 2 ? k_trace<ANY, LAYOUT, 6, 2, false, BATCH, BOX> : k_trace<ANY, LAYOUT, 6, 1, false, BATCH, BOX>;
     case 16:
-        return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 16, 2, false, BATCH> : prt::k_trace<ANY, LAYOUT, 16, 1, false, BATCH>;
-    default: return ilp == 2 ? prt::k_trace<ANY, LAYOUT, 8, 2, false, BATCH> : prt::k_trace<ANY, LAYOUT, 8, 1, false, BATCH>;
+        return ilp == 2 ? k_trace<ANY, LAYOUT, 16, 2, false, BATCH, BOX> : k_trace<ANY, LAYOUT, 16, 1, false, BATCH, BOX>;
+    default: return ilp == 2 ? k_trace<ANY, LAYOUT, 8, 2, false, BATCH, BOX> : k_trace<ANY, LAYOUT, 8, 1, false, BATCH, BOX>;
     }
 }
 
@@ -174,15 +175,30 @@ TraceKernel pick_kernel_q(int K, int ilp) {
     }
 }
 
-// The kernel of a tuning: two_level when the scene has two-level records and
-// the tuning asks for them; batch for leaf batching (both packed layout only).
-TraceKernel pick_kernel(bool any, int layout, int K, int ilp, bool two_level, bool batch) {
+int node_form(const SceneRec &sc, const Tuning &tu) {
+    if (tu.layout != PRAY_LAYOUT_PACKED) return 1;
+    if (tu.node_levels == 2 && sc.d.quads) return 2;
+    if (tu.node_levels == 3 && sc.d.boxes) return 3;
+    return 1;
+}
+
+// The kernel of a tuning: form 2 (two-level records) or 3 (boxed children
+// pairs) as node_form resolves them; batch for leaf batching (packed layout
+// only).
+TraceKernel pick_kernel(bool any, int layout, int K, int ilp, int form, bool batch) {
     if (layout != PRAY_LAYOUT_PACKED)
         return any ? pick_kernel_kl<true, PRAY_LAYOUT_INDEXED, false>(K, ilp)
                    : pick_kernel_kl<false, PRAY_LAYOUT_INDEXED, false>(K, ilp);
-    if (two_level) {
+    if (form == 2) {
         if (batch) return any ? pick_kernel_q<true, true>(K, ilp) : pick_kernel_q<false, true>(K, ilp);
         return any ? pick_kernel_q<true, false>(K, ilp) : pick_kernel_q<false, false>(K, ilp);
+    }
+    if (form == 3) {
+        if (batch)
+            return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, true, true>(K, ilp)
+                       : pick_kernel_kl<false, PRAY_LAYOUT_PACKED, true, true>(K, ilp);
+        return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, false, true>(K, ilp)
+                   : pick_kernel_kl<false, PRAY_LAYOUT_PACKED, false, true>(K, ilp);
     }
     if (batch)
         return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, true>(K, ilp)
@@ -286,8 +302,8 @@ static int enqueue_trace_ex(pray_ctx *c, const SceneRec &sc, const float *rays, 
         return PRAY_OK;
     }
     const Tuning &tu = sc.tune;
-    TraceKernel k = pick_kernel(mode != PRAY_TRACE_CLOSEST, tu.layout, tu.stack_lds, tu.leaf_ilp,
-                                tu.node_levels == 2 && sc.d.quads, tu.leaf_batch > 0);
+    TraceKernel k = pick_kernel(mode != PRAY_TRACE_CLOSEST, tu.layout, tu.stack_lds, tu.leaf_ilp, node_form(sc, tu),
+                                tu.leaf_batch > 0);
     Launch L = launch_shape(c, k, tu, n);
     const uint32_t depth = std::max<uint32_t>(1, sc.d.max_depth);
     uint32_t *work = c->work;
@@ -437,6 +453,92 @@ int build_quads(pray_ctx *c, SceneRec &sc, hipStream_t st) {
     return PRAY_OK;
 }
 
+// Boxed children pairs (DevScene::boxes) for a tree with a children pair
+// whose two clips differ (median trees, whose children's triangle boxes
+// overlap); other trees keep boxes null.  Each node's box is the union of the
+// boxes of the triangles its subtree references, with the vertices a, a + ab
+// and a + ac summed in double and the bounds rounded outward to float, so
+// that it holds the triangle the intersection test sees; an empty subtree
+// gets {+inf, -inf}, which no ray enters.  Built on the host from the
+// uploaded arrays (one pass over nodes and leaf references).
+int build_boxes(pray_ctx *c, SceneRec &sc, const uint32_t *nodes, uint32_t n_nodes, const uint32_t *refs,
+                const float *A, const float *AB, const float *AC, hipStream_t st) {
+    prt::DevScene &d = sc.d;
+    d.boxes = nullptr;
+    if (d.n_tris == 0 || n_nodes < 4) return PRAY_OK;
+    bool unequal = false;
+    for (uint32_t p = 1; 2 * p + 1 < n_nodes && !unequal; ++p) unequal = nodes[4 * p] != nodes[4 * p + 2];
+    if (!unequal) return PRAY_OK;
+    auto down = [](double x) {
+        const float f = (float)x;
+        return (double)f > x ? std::nextafter(f, -INFINITY) : f;
+    };
+    auto up = [](double x) {
+        const float f = (float)x;
+        return (double)f < x ? std::nextafter(f, INFINITY) : f;
+    };
+    std::vector<float> box((size_t)n_nodes * 6);
+    for (size_t i = 0; i < n_nodes; ++i)
+        for (int k = 0; k < 3; ++k) box[6 * i + k] = INFINITY, box[6 * i + 3 + k] = -INFINITY;
+    // post order (the structure was validated: no cycles, indices in range)
+    std::vector<std::pair<uint32_t, bool>> stack{{1u, false}};
+    while (!stack.empty()) {
+        const auto [i, done] = stack.back();
+        stack.pop_back();
+        const uint32_t w = nodes[2 * (size_t)i + 1];
+        float *b = &box[6 * (size_t)i];
+        if ((w & 3u) == prt::LEAF_TAG) {
+            uint32_t k = w >> 2;
+            if (k == prt::LEAF_EMPTY) continue;
+            while (true) {
+                const size_t t = 3 * (size_t)(refs[k] & ~prt::REF_LAST);
+                for (int a = 0; a < 3; ++a) {
+                    const double v0 = A[t + a], v1 = v0 + AB[t + a], v2 = v0 + AC[t + a];
+                    b[a] = std::min(b[a], down(std::min(v0, std::min(v1, v2))));
+                    b[3 + a] = std::max(b[3 + a], up(std::max(v0, std::max(v1, v2))));
+                }
+                if (refs[k] & prt::REF_LAST) break;
+                ++k;
+            }
+        } else if (!done) {
+            stack.push_back({i, true});
+            stack.push_back({w >> 2, false});
+            stack.push_back({(w >> 2) + 1, false});
+        } else {
+            for (int ch = 0; ch < 2; ++ch) {
+                const float *cb = &box[6 * (size_t)((w >> 2) + ch)];
+                for (int a = 0; a < 3; ++a) b[a] = std::min(b[a], cb[a]), b[3 + a] = std::max(b[3 + a], cb[3 + a]);
+            }
+        }
+    }
+    const uint32_t n_pairs = (n_nodes + 1) / 2;
+    std::vector<uint32_t> rec((size_t)n_pairs * 16, 0u);
+    auto bits = [](float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    };
+    for (uint32_t p = 1; p < n_pairs; ++p) {
+        uint32_t *r = &rec[16 * (size_t)p];
+        for (int ch = 0; ch < 2; ++ch) {
+            const uint32_t i = 2 * p + ch;
+            const bool there = i < n_nodes;
+            for (int a = 0; a < 3; ++a) {
+                r[4 * a + ch] = bits(there ? box[6 * (size_t)i + a] : INFINITY);
+                r[4 * a + 2 + ch] = bits(there ? box[6 * (size_t)i + 3 + a] : -INFINITY);
+            }
+            r[12 + ch] = there ? nodes[2 * (size_t)i + 1] : ((prt::LEAF_EMPTY << 2) | prt::LEAF_TAG);
+        }
+    }
+    void *p = nullptr;
+    int rc = device_alloc(c, sc, &p, rec.size() * 4);
+    if (rc) return rc;
+    HIPC(c, hipMemcpyAsync(p, rec.data(), rec.size() * 4, hipMemcpyHostToDevice, st));
+    HIPC(c, hipStreamSynchronize(st));  // rec is freed on return
+    d.boxes = static_cast<const uint4 *>(p);
+    return PRAY_OK;
+}
+
 // Structural check of an uploaded tree so that no kernel can read out of
 // bounds or loop: every inner node's child pair is even-aligned and inside
 // the array, no path is longer than the declared depth (which also rules out
@@ -472,8 +574,8 @@ int validate_tree(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const ui
 }
 
 int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n) {
-    TraceKernel k = pick_kernel(false, sc.tune.layout, sc.tune.stack_lds, sc.tune.leaf_ilp,
-                                sc.tune.node_levels == 2 && sc.d.quads, sc.tune.leaf_batch > 0);
+    TraceKernel k = pick_kernel(false, sc.tune.layout, sc.tune.stack_lds, sc.tune.leaf_ilp, node_form(sc, sc.tune),
+                                sc.tune.leaf_batch > 0);
     return ensure_spill(c, (size_t)launch_shape(c, k, sc.tune, n).lanes * std::max<uint32_t>(1, sc.d.max_depth) *
                                sizeof(uint4));
 }
@@ -636,6 +738,9 @@ int pray_upload_scene(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, cons
             goto bad;
         }
         if ((rc = build_quads(c, *sc, c->stream))) goto bad;
+        if ((rc = build_boxes(c, *sc, nodes, n_nodes, leaf_refs, tri_a, tri_ab, tri_ac, c->stream))) goto bad;
+        // boxed node steps by default where the tree has them
+        if (d.boxes) sc->tune.node_levels = 3;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         rc = fail(c, PRAY_ERR_HIP, "scene upload failed");
@@ -683,15 +788,16 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
     if ((leaf_batch >= 0 ? leaf_batch : sc->tune.leaf_batch) > 0 &&
         (layout >= 0 ? layout : sc->tune.layout) != PRAY_LAYOUT_PACKED)
         return fail(c, PRAY_ERR_INVALID, "leaf batching traces the packed layout");
-    if (node_levels >= 0 && node_levels != 1 && node_levels != 2)
-        return fail(c, PRAY_ERR_INVALID, "node_levels must be 1 or 2");
-    if ((node_levels >= 0 ? node_levels : sc->tune.node_levels) == 2) {
-        if (!sc->d.quads)
-            return fail(c, PRAY_ERR_UNSUPPORTED, "scene %d has no two-level node records (not a SAH-form tree)",
-                        scene_id);
-        if ((layout >= 0 ? layout : sc->tune.layout) != PRAY_LAYOUT_PACKED)
-            return fail(c, PRAY_ERR_INVALID, "two-level node steps trace the packed layout");
-    }
+    if (node_levels >= 0 && node_levels != 1 && node_levels != 2 && node_levels != 3)
+        return fail(c, PRAY_ERR_INVALID, "node_levels must be 1, 2 or 3");
+    const int nl = node_levels >= 0 ? node_levels : sc->tune.node_levels;
+    if (nl == 2 && !sc->d.quads)
+        return fail(c, PRAY_ERR_UNSUPPORTED, "scene %d has no two-level node records (not a SAH-form tree)", scene_id);
+    if (nl == 3 && !sc->d.boxes)
+        return fail(c, PRAY_ERR_UNSUPPORTED, "scene %d has no boxed children pairs (its children pairs have equal clips)",
+                    scene_id);
+    if (nl != 1 && (layout >= 0 ? layout : sc->tune.layout) != PRAY_LAYOUT_PACKED)
+        return fail(c, PRAY_ERR_INVALID, "two-level and boxed node steps trace the packed layout");
     Tuning &t = sc->tune;
     if (layout >= 0) t.layout = layout;
     if (stack_lds >= 0) t.stack_lds = stack_lds;
@@ -850,16 +956,20 @@ int pray_render_rt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, ui
     const uint64_t nsh = lanes * nl;
     Cam cam;
     std::memcpy(&cam, cam16, sizeof cam);
-    DBuf rays, hit, tt, lt, sr, sl, so, out;
-    HIPC(c, hipMalloc(&rays.p, npix * 24));
-    HIPC(c, hipMalloc(&hit.p, npix * 4));
-    HIPC(c, hipMalloc(&tt.p, npix * 4));
-    HIPC(c, hipMalloc(&out.p, npix * 3));
+    // Buffers come from the context's workspace (slots 64..71), which keeps
+    // them across frames: a frame allocates only when it outgrows them.
+    struct {
+        void *p = nullptr;
+    } rays, hit, tt, lt, sr, sl, so, out;
+    WS_GET(c, 64, rays.p, npix * 24);
+    WS_GET(c, 65, hit.p, npix * 4);
+    WS_GET(c, 66, tt.p, npix * 4);
+    WS_GET(c, 67, out.p, npix * 3);
     if (nl) {
-        HIPC(c, hipMalloc(&lt.p, (size_t)nl * 24));
-        HIPC(c, hipMalloc(&sr.p, nsh * 24));
-        HIPC(c, hipMalloc(&sl.p, nsh * 4));
-        HIPC(c, hipMalloc(&so.p, nsh * 4));
+        WS_GET(c, 68, lt.p, (size_t)nl * 24);
+        WS_GET(c, 69, sr.p, nsh * 24);
+        WS_GET(c, 70, sl.p, nsh * 4);
+        WS_GET(c, 71, so.p, nsh * 4);
         HIPC(c, hipMemcpyAsync(lt.p, lights, (size_t)nl * 24, hipMemcpyHostToDevice, c->stream));
     }
     {

@@ -51,6 +51,7 @@ struct DevScene {
     const float *a = nullptr, *ab = nullptr, *ac = nullptr;
     const float4 *recs = nullptr;     // 3 * (n_refs + 2) (padded)
     const uint4 *quads = nullptr;     // 2 per children pair: two-level node records (SAH trees), or null
+    const uint4 *boxes = nullptr;     // 4 per children pair: boxed children pairs (median trees), or null
     const float *normal = nullptr, *color = nullptr;
     const uint8_t *emit = nullptr;
     float box[6] = {0, 0, 0, 0, 0, 0};
@@ -134,10 +135,21 @@ __device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, 
 // BATCH: leaf batching (TraceArgs::leaf_batch > 0) compiled in; without it
 // every iteration takes node and leaf steps and the loop carries no batching
 // code.
-template <bool ANY, int LAYOUT, int K, int ILP, bool MULTI = false, bool BATCH = false>
+// BOX: node steps over boxed children pairs (DevScene::boxes) instead of the
+// 8-byte nodes' clip planes.  Per children pair p, boxes[4p + a] for axis
+// a = 0, 1, 2 holds {child 0's lower bound, child 1's lower bound, child 0's
+// upper bound, child 1's upper bound} along that axis (float bits) and
+// boxes[4p + 3] the children's words.  A step intersects the ray's interval,
+// capped at the best t, with both children's boxes, enters the child with
+// the smaller entry distance (child 0 on a tie) and stacks the other.  For
+// trees whose children overlap (median trees: the children's boxes are
+// those of their triangles) this prunes on all three axes where the clip
+// planes bound one.
+template <bool ANY, int LAYOUT, int K, int ILP, bool MULTI = false, bool BATCH = false, bool BOX = false>
 static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s, TraceArgs A) {
     static_assert(K >= 2 && K <= 16, "LDS ring of 2..16 entries");
     static_assert(!MULTI || LAYOUT == 1, "multi-scene launches use the packed layout");
+    static_assert(!BOX || (LAYOUT == 1 && !MULTI), "boxed node steps trace the packed layout of one scene");
     extern __shared__ uint32_t ring_lds[];
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t wave_in_block = threadIdx.x / WAVE;
@@ -274,8 +286,51 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
             run_node = !run_leaf;
         }
         bool need_pop = false;
+        // ---------------------------------------------- boxed node step --
+        if constexpr (BOX) {
+            if (run_node && state == ST_NODE) {
+                const uint4 *bp = s.boxes + 4 * (size_t)(w >> 3);
+                const uint4 qx = bp[0], qy = bp[1], qz = bp[2], qw = bp[3];
+                cn += 2;
+                const float cap = ANY ? hi : fminf(hi, best_t);
+                float l0 = lo, h0 = cap, l1 = lo, h1 = cap;
+                slab(__uint_as_float(qx.x), __uint_as_float(qx.z), ox, ix, l0, h0);
+                slab(__uint_as_float(qx.y), __uint_as_float(qx.w), ox, ix, l1, h1);
+                slab(__uint_as_float(qy.x), __uint_as_float(qy.z), oy, iy, l0, h0);
+                slab(__uint_as_float(qy.y), __uint_as_float(qy.w), oy, iy, l1, h1);
+                slab(__uint_as_float(qz.x), __uint_as_float(qz.z), oz, iz, l0, h0);
+                slab(__uint_as_float(qz.y), __uint_as_float(qz.w), oz, iz, l1, h1);
+                const bool g0 = l0 <= h0 && !empty_leaf(qw.x);
+                const bool g1 = l1 <= h1 && !empty_leaf(qw.y);
+                const bool n1 = g1 && (!g0 || l1 < l0);
+                if (g0 && g1) {
+                    if (sp - base == K) {  // ring full: spill the oldest entry
+                        const uint32_t at = ((uint32_t)base % K) * WAVE + lane;
+                        spill[base] =
+                            make_uint4(st_w[at], __float_as_uint(st_lo[at]), __float_as_uint(st_hi[at]), 0u);
+                        ++base;
+                    }
+                    const uint32_t at = ((uint32_t)sp % K) * WAVE + lane;
+                    st_w[at] = n1 ? qw.x : qw.y;
+                    st_lo[at] = n1 ? l0 : l1;
+                    st_hi[at] = n1 ? h0 : h1;
+                    ++sp;
+                }
+                if (g0 || g1) {
+                    w = n1 ? qw.y : qw.x;
+                    lo = n1 ? l1 : l0;
+                    hi = n1 ? h1 : h0;
+                    if ((w & 3u) == LEAF_TAG) {
+                        state = ST_LEAF;
+                        k = w >> 2;
+                    }
+                } else {
+                    need_pop = true;
+                }
+            }
+        }
         // ---------------------------------------------------- node step --
-        if (run_node && state == ST_NODE) {
+        if (!BOX && run_node && state == ST_NODE) {
             const uint32_t axis = w & 3u;
             const uint4 pr = pairs[w >> 3];  // pair index = (w >> 2) >> 1
             cn += 2;

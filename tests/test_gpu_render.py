@@ -81,8 +81,10 @@ def test_rt_multi_light_negative_lambert_quirk(ctx):
     sid = ctx.upload(t)
     try:
         # widths below 4 are undefined in the reference (pray.cpp:388 loops
-        # to width - 4 in uint64_t); the smallest packet image is 4 x 2
-        for w, h in [(512, 512), (101, 77), (6, 3), (4, 2)]:
+        # to width - 4 in uint64_t); the smallest packet image is 4 x 2.
+        # Frames of shrinking, then growing sizes reuse the context's
+        # workspace buffers.
+        for w, h in [(512, 512), (101, 77), (6, 3), (4, 2), (203, 101)]:
             say(f"rt multi-light {w}x{h}")
             cam = O.camera([0, 3, 8], [0, -0.35, -1], 60, w, h)
             img = ctx.render_rt(sid, cam, w, h, lights, inf["background"], 1e-4)

@@ -30,6 +30,9 @@ def scene(name):
         elif name == "cornell20k-sah":
             s = P.Scene.cornell(3, 20000)
             t = s.build(1)
+        elif name == "random-median":
+            s = P.Scene.random(5, 6000)
+            t = s.build(0)
         else:
             raise KeyError(name)
         SCENES[name] = (s, t, t.flat())
@@ -45,8 +48,12 @@ def scene(name):
 # before that with leaf batching (ILP 2, refill at 16, leaf steps once 16
 # lanes wait).  Two levels per step need a tree whose children pairs have
 # equal clips (every SAH tree; the terrain's median tree too, whose children
-# meet on grid lines); other trees refuse them.
+# meet on grid lines); other trees refuse them.  Node records 3 (boxed
+# children pairs) exist for exactly those other trees (median trees whose
+# children overlap) and are refused for the rest.
 SETTINGS = [(P.LAYOUT_PACKED, 6, 1, 2, 16, 0, 2),
+            (P.LAYOUT_PACKED, 6, 1, 2, 16, 0, 3), (P.LAYOUT_PACKED, 8, 4, 1, 24, 16, 3),
+            (P.LAYOUT_PACKED, 5, 1, 1, 1, 1, 3), (P.LAYOUT_PACKED, 16, 2, 2, 48, 0, 3),
             (P.LAYOUT_PACKED, 8, 1, 1, 24, 0, 1), (P.LAYOUT_PACKED, 8, 1, 2, 16, 16, 1),
             (P.LAYOUT_PACKED, 5, 2, 2, 16, 0, 1), (P.LAYOUT_INDEXED, 8, 4, 1, 32, 0, 1),
             (P.LAYOUT_INDEXED, 6, 1, 2, 16, 0, 1), (P.LAYOUT_PACKED, 6, 1, 2, 48, 24, 1),
@@ -68,10 +75,12 @@ def equal_clips(flat):
 
 def set_or_refuse(ctx, sid, name, layout, ring, wg, ilp, fm, lb, nl):
     """Applies a setting; False when the scene's tree has no two-level
-    records, after checking that the library refuses exactly those trees."""
-    if nl == 2 and not equal_clips(scene(name)[2]):
+    records (nl 2) or no boxed pairs (nl 3), after checking that the library
+    refuses exactly those trees."""
+    eq = equal_clips(scene(name)[2])
+    if (nl == 2 and not eq) or (nl == 3 and eq):
         with pytest.raises(P.PrayError):
-            ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, node_levels=2)
+            ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, node_levels=nl)
         return False
     ctx.set_tuning(sid, layout=layout, stack_lds=ring, wg_waves=wg, leaf_ilp=ilp, fetch_min=fm, leaf_batch=lb,
                    node_levels=nl)
@@ -183,15 +192,69 @@ def test_work_counts_match_cpu_twin(ctx, lb, nl):
     assert np.array_equal(gc, oc)
 
 
-@pytest.mark.parametrize("ring,lb,nl", [(5, 0, 1), (16, 0, 1), (5, 32, 1), (16, 8, 1), (5, 0, 2), (16, 24, 2)])
-def test_launch_sizes(ctx, ring, lb, nl):
+@pytest.mark.parametrize("lb", [0, 16])
+@pytest.mark.parametrize("name,rays_of", [("spheres-median", "camera"), ("spheres-median", "incoherent"),
+                                          ("random-median", "incoherent")])
+def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb):
+    """Boxed node steps (node records 3, the default for median trees whose
+    children overlap): per-ray node and triangle counts and hits equal the
+    CPU twin's with the same boxes, with and without leaf batching; the
+    twin's hits equal the plane-form traversal's and brute force."""
+    s, t, flat = scene(name)
+    rays = s.prim_rays() if rays_of == "camera" else O.gen_rays(4, P.scene_box_expanded(flat["box"]), 1 << 15)
+    sid = ctx.upload(t)
+    try:
+        assert ctx.get_tuning(sid)["node_levels"] == 3
+        ctx.set_tuning(sid, leaf_batch=lb)
+        say(f"counts boxed {name} {rays_of} leaf_batch={lb}")
+        n = rays.shape[0]
+        d_rays = ctx.to_device(rays)
+        d_idx = ctx.alloc(n, np.int32)
+        d_t = ctx.alloc(n, np.float32)
+        d_c = ctx.alloc((n, 2), np.uint32)
+        ctx.trace_counts_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, d_c.ptr, P.TRACE_CLOSEST, 1e-4)
+        ctx.sync()
+        gc, gi, gt = d_c.numpy(), d_idx.numpy(), d_t.numpy()
+    finally:
+        ctx.free_scene(sid)
+    oi, ot, oc = O.flat_trace(flat, rays, 0, 1e-4, counts=True, boxed=True)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
+    assert np.array_equal(gc, oc)
+    pi, pt, pc = O.flat_trace(flat, rays, 0, 1e-4, counts=True)
+    assert np.array_equal(oi, pi) and np.array_equal(ot.view(np.uint32), pt.view(np.uint32))
+    # the boxes prune: fewer triangle tests than the clip planes
+    assert oc[:, 1].sum() < pc[:, 1].sum()
+    bi, bt = O.bf_closest(flat["a"], flat["ab"], flat["ac"], rays[:2048], 1e-4)
+    assert first_diff(oi[:2048], bi) is None
+
+
+def test_node_records_by_tree(ctx):
+    """The upload's default node records: boxed pairs for the median tree
+    whose children overlap, children pairs for SAH trees and for the
+    terrain's median tree (equal clips)."""
+    for name, want in [("spheres-median", 3), ("spheres-sah", 1), ("terrain256-median", 1)]:
+        _, t, flat = scene(name)
+        sid = ctx.upload(t)
+        try:
+            assert ctx.get_tuning(sid)["node_levels"] == want, name
+            assert (want == 3) == (not equal_clips(flat))
+        finally:
+            ctx.free_scene(sid)
+
+
+@pytest.mark.parametrize("name,ring,lb,nl", [("spheres-sah", 5, 0, 1), ("spheres-sah", 16, 0, 1),
+                                             ("spheres-sah", 5, 32, 1), ("spheres-sah", 16, 8, 1),
+                                             ("spheres-sah", 5, 0, 2), ("spheres-sah", 16, 24, 2),
+                                             ("spheres-median", 5, 0, 3), ("spheres-median", 16, 24, 3)])
+def test_launch_sizes(ctx, name, ring, lb, nl):
     """Traces over ray counts around the wave, the 64-ray chunks and the 64
     chunk counters (partial last chunks, fewer waves than counters), one after
     another through the same counters and spill area, including a ray without
     direction (its hit is not compared) and one far outside the scene box.
     The last launch has a grid of a few hundred waves for 3125 chunks, so the
     waves walk the counters as they run dry."""
-    _, t, flat = scene("spheres-sah")
+    _, t, flat = scene(name)
     sid = ctx.upload(t)
     try:
         ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, stack_lds=ring, fetch_min=1 if ring == 5 else 64,
