@@ -182,6 +182,12 @@ int node_form(const SceneRec &sc, const Tuning &tu) {
     return 1;
 }
 
+// Stack entries a lane may need (the tree's depth; the boxed layout's is
+// deeper where it splits leaves).
+uint32_t trace_depth(const SceneRec &sc, const Tuning &tu) {
+    return std::max<uint32_t>(1, node_form(sc, tu) == 3 ? sc.d.box_depth : sc.d.max_depth);
+}
+
 // The kernel of a tuning: form 2 (two-level records) or 3 (boxed children
 // pairs) as node_form resolves them; batch for leaf batching (packed layout
 // only).
@@ -305,7 +311,7 @@ static int enqueue_trace_ex(pray_ctx *c, const SceneRec &sc, const float *rays, 
     TraceKernel k = pick_kernel(mode != PRAY_TRACE_CLOSEST, tu.layout, tu.stack_lds, tu.leaf_ilp, node_form(sc, tu),
                                 tu.leaf_batch > 0);
     Launch L = launch_shape(c, k, tu, n);
-    const uint32_t depth = std::max<uint32_t>(1, sc.d.max_depth);
+    const uint32_t depth = trace_depth(sc, tu);
     uint32_t *work = c->work;
     uint4 *spill = c->spill;
     if (fixed) {
@@ -453,89 +459,196 @@ int build_quads(pray_ctx *c, SceneRec &sc, hipStream_t st) {
     return PRAY_OK;
 }
 
-// Boxed children pairs (DevScene::boxes) for a tree with a children pair
-// whose two clips differ (median trees, whose children's triangle boxes
-// overlap); other trees keep boxes null.  Each node's box is the union of the
-// boxes of the triangles its subtree references, with the vertices a, a + ab
-// and a + ac summed in double and the bounds rounded outward to float, so
-// that it holds the triangle the intersection test sees; an empty subtree
-// gets {+inf, -inf}, which no ray enters.  Built on the host from the
-// uploaded arrays (one pass over nodes and leaf references).
+// ---------------------------------------------------------- boxed layout --
+// The layout of boxed node steps (DevScene::boxes, DevScene::brecs), made
+// for trees with a children pair whose two clips differ (median trees: an
+// object partition whose children's boxes overlap).  It has its own pair and
+// leaf-reference index spaces:
+// - Every inner node of the uploaded tree becomes a pair record of its two
+//   children (16 words: per axis {child 0 lower, child 1 lower, child 0
+//   upper, child 1 upper}, then the children's words); pair 0 holds an empty
+//   child 0 and the root as child 1.
+// - Every leaf of more than BOX_LEAF references is split further: its
+//   triangles are sorted by centroid along the longest axis of their
+//   centroids' bounds (ties by triangle index) and halved until no leaf holds
+//   more, each split a pair record of its own.  The leaves' runs are written
+//   in depth-first order to a reference array of their own, with the last
+//   entry of each run flagged.
+// - A child's box is the union of the boxes of the triangles its subtree
+//   references, with the vertices a, a + ab, a + ac summed in double and the
+//   bounds rounded outward to float, so that it holds the triangle the
+//   intersection test sees; an empty subtree gets {+inf, -inf}.
+// The traversal's closest hit does not depend on the tree (it is the
+// lexicographic minimum of (t, index) over all triangles), so this finer
+// tree gives the same hits as the uploaded one.
+namespace {
+
+constexpr uint32_t BOX_LEAF = 4;
+
+struct Bounds {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const Bounds &o) {
+        for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], o.lo[a]), hi[a] = std::max(hi[a], o.hi[a]);
+    }
+};
+
+struct BoxedBuilder {
+    const uint32_t *nodes, *refs;
+    const float *A, *AB, *AC;
+    std::vector<uint32_t> rec;    // 16 words per pair
+    std::vector<uint32_t> brefs;  // leaf runs
+    uint32_t depth = 0;
+
+    static float down(double x) {
+        const float f = (float)x;
+        return (double)f > x ? std::nextafter(f, -INFINITY) : f;
+    }
+    static float up(double x) {
+        const float f = (float)x;
+        return (double)f < x ? std::nextafter(f, INFINITY) : f;
+    }
+    Bounds tri_box(uint32_t t) const {
+        Bounds b;
+        for (int a = 0; a < 3; ++a) {
+            const double v0 = A[3 * (size_t)t + a], v1 = v0 + AB[3 * (size_t)t + a], v2 = v0 + AC[3 * (size_t)t + a];
+            b.lo[a] = down(std::min(v0, std::min(v1, v2)));
+            b.hi[a] = up(std::max(v0, std::max(v1, v2)));
+        }
+        return b;
+    }
+    double centroid(uint32_t t, int a) const {
+        return (double)A[3 * (size_t)t + a] + ((double)AB[3 * (size_t)t + a] + (double)AC[3 * (size_t)t + a]) / 3.0;
+    }
+    uint32_t new_pair() {
+        rec.resize(rec.size() + 16, 0u);
+        return (uint32_t)(rec.size() / 16 - 1);
+    }
+    void set_child(uint32_t p, int ch, uint32_t w, const Bounds &b) {
+        uint32_t *r = &rec[16 * (size_t)p];
+        for (int a = 0; a < 3; ++a) {
+            std::memcpy(&r[4 * a + ch], &b.lo[a], 4);
+            std::memcpy(&r[4 * a + 2 + ch], &b.hi[a], 4);
+        }
+        r[12 + ch] = w;
+    }
+    // A leaf of the n triangles at t, split while over BOX_LEAF: its word
+    // and box.
+    uint32_t leaf(uint32_t *t, uint32_t n, uint32_t d, Bounds &box) {
+        box = Bounds{};
+        if (n <= BOX_LEAF) {
+            depth = std::max(depth, d);
+            const uint32_t k = (uint32_t)brefs.size();
+            for (uint32_t j = 0; j < n; ++j) {
+                brefs.push_back(t[j] | (j + 1 == n ? prt::REF_LAST : 0u));
+                box.grow(tri_box(t[j]));
+            }
+            return (k << 2) | prt::LEAF_TAG;
+        }
+        double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t j = 0; j < n; ++j)
+            for (int a = 0; a < 3; ++a) {
+                const double c = centroid(t[j], a);
+                clo[a] = std::min(clo[a], c), chi[a] = std::max(chi[a], c);
+            }
+        int ax = 0;
+        for (int a = 1; a < 3; ++a)
+            if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+        std::sort(t, t + n, [&](uint32_t x, uint32_t y) {
+            const double cx = centroid(x, ax), cy = centroid(y, ax);
+            return cx < cy || (cx == cy && x < y);
+        });
+        const uint32_t p = new_pair();
+        Bounds b0, b1;
+        const uint32_t w0 = leaf(t, n / 2, d + 1, b0);
+        const uint32_t w1 = leaf(t + n / 2, n - n / 2, d + 1, b1);
+        set_child(p, 0, w0, b0);
+        set_child(p, 1, w1, b1);
+        box = b0;
+        box.grow(b1);
+        return (2 * p) << 2;
+    }
+    // Node i of the uploaded tree: its word in this layout and its box.
+    uint32_t node(uint32_t i, uint32_t d, Bounds &box) {
+        const uint32_t w = nodes[2 * (size_t)i + 1];
+        if ((w & 3u) == prt::LEAF_TAG) {
+            uint32_t k = w >> 2;
+            if (k == prt::LEAF_EMPTY) {
+                box = Bounds{};
+                return w;
+            }
+            std::vector<uint32_t> run;
+            while (true) {
+                run.push_back(refs[k] & ~prt::REF_LAST);
+                if (refs[k] & prt::REF_LAST) break;
+                ++k;
+            }
+            return leaf(run.data(), (uint32_t)run.size(), d, box);
+        }
+        const uint32_t p = new_pair();
+        Bounds b0, b1;
+        const uint32_t w0 = node(w >> 2, d + 1, b0);
+        const uint32_t w1 = node((w >> 2) + 1, d + 1, b1);
+        set_child(p, 0, w0, b0);
+        set_child(p, 1, w1, b1);
+        box = b0;
+        box.grow(b1);
+        return (2 * p) << 2;
+    }
+};
+
+// Whether the uploaded tree takes the boxed layout: some children pair after
+// pair 0 has two different clips.
+bool unequal_clips(const uint32_t *nodes, uint32_t n_nodes) {
+    for (uint32_t p = 1; 2 * p + 1 < n_nodes; ++p)
+        if (nodes[4 * (size_t)p] != nodes[4 * (size_t)p + 2]) return true;
+    return false;
+}
+
+// Builds the layout of a validated tree (the caller checked unequal_clips).
+int make_boxed_layout(pray_ctx *c, const uint32_t *nodes, const uint32_t *refs, const float *A, const float *AB,
+                      const float *AC, BoxedBuilder &B) {
+    B.nodes = nodes, B.refs = refs, B.A = A, B.AB = AB, B.AC = AC;
+    const uint32_t p0 = B.new_pair();
+    Bounds root;
+    const uint32_t rw = B.node(1, 0, root);
+    B.set_child(p0, 0, (prt::LEAF_EMPTY << 2) | prt::LEAF_TAG, Bounds{});
+    B.set_child(p0, 1, rw, root);
+    if (B.rec.size() / 16 >= (1u << 29) || B.brefs.size() >= prt::LEAF_EMPTY)
+        return fail(c, PRAY_ERR_UNSUPPORTED, "tree too large for the boxed layout");
+    if (B.depth > MAX_BOXED_DEPTH) return fail(c, PRAY_ERR_UNSUPPORTED, "boxed layout deeper than %u", MAX_BOXED_DEPTH);
+    return PRAY_OK;
+}
+
+}  // namespace
+
+// Uploads the boxed layout of a tree that takes one (else leaves boxes null).
 int build_boxes(pray_ctx *c, SceneRec &sc, const uint32_t *nodes, uint32_t n_nodes, const uint32_t *refs,
                 const float *A, const float *AB, const float *AC, hipStream_t st) {
     prt::DevScene &d = sc.d;
     d.boxes = nullptr;
-    if (d.n_tris == 0 || n_nodes < 4) return PRAY_OK;
-    bool unequal = false;
-    for (uint32_t p = 1; 2 * p + 1 < n_nodes && !unequal; ++p) unequal = nodes[4 * p] != nodes[4 * p + 2];
-    if (!unequal) return PRAY_OK;
-    auto down = [](double x) {
-        const float f = (float)x;
-        return (double)f > x ? std::nextafter(f, -INFINITY) : f;
-    };
-    auto up = [](double x) {
-        const float f = (float)x;
-        return (double)f < x ? std::nextafter(f, INFINITY) : f;
-    };
-    std::vector<float> box((size_t)n_nodes * 6);
-    for (size_t i = 0; i < n_nodes; ++i)
-        for (int k = 0; k < 3; ++k) box[6 * i + k] = INFINITY, box[6 * i + 3 + k] = -INFINITY;
-    // post order (the structure was validated: no cycles, indices in range)
-    std::vector<std::pair<uint32_t, bool>> stack{{1u, false}};
-    while (!stack.empty()) {
-        const auto [i, done] = stack.back();
-        stack.pop_back();
-        const uint32_t w = nodes[2 * (size_t)i + 1];
-        float *b = &box[6 * (size_t)i];
-        if ((w & 3u) == prt::LEAF_TAG) {
-            uint32_t k = w >> 2;
-            if (k == prt::LEAF_EMPTY) continue;
-            while (true) {
-                const size_t t = 3 * (size_t)(refs[k] & ~prt::REF_LAST);
-                for (int a = 0; a < 3; ++a) {
-                    const double v0 = A[t + a], v1 = v0 + AB[t + a], v2 = v0 + AC[t + a];
-                    b[a] = std::min(b[a], down(std::min(v0, std::min(v1, v2))));
-                    b[3 + a] = std::max(b[3 + a], up(std::max(v0, std::max(v1, v2))));
-                }
-                if (refs[k] & prt::REF_LAST) break;
-                ++k;
-            }
-        } else if (!done) {
-            stack.push_back({i, true});
-            stack.push_back({w >> 2, false});
-            stack.push_back({(w >> 2) + 1, false});
-        } else {
-            for (int ch = 0; ch < 2; ++ch) {
-                const float *cb = &box[6 * (size_t)((w >> 2) + ch)];
-                for (int a = 0; a < 3; ++a) b[a] = std::min(b[a], cb[a]), b[3 + a] = std::max(b[3 + a], cb[3 + a]);
-            }
-        }
-    }
-    const uint32_t n_pairs = (n_nodes + 1) / 2;
-    std::vector<uint32_t> rec((size_t)n_pairs * 16, 0u);
-    auto bits = [](float f) {
-        uint32_t u;
-        std::memcpy(&u, &f, 4);
-        return u;
-    };
-    for (uint32_t p = 1; p < n_pairs; ++p) {
-        uint32_t *r = &rec[16 * (size_t)p];
-        for (int ch = 0; ch < 2; ++ch) {
-            const uint32_t i = 2 * p + ch;
-            const bool there = i < n_nodes;
-            for (int a = 0; a < 3; ++a) {
-                r[4 * a + ch] = bits(there ? box[6 * (size_t)i + a] : INFINITY);
-                r[4 * a + 2 + ch] = bits(there ? box[6 * (size_t)i + 3 + a] : -INFINITY);
-            }
-            r[12 + ch] = there ? nodes[2 * (size_t)i + 1] : ((prt::LEAF_EMPTY << 2) | prt::LEAF_TAG);
-        }
-    }
-    void *p = nullptr;
-    int rc = device_alloc(c, sc, &p, rec.size() * 4);
+    d.brecs = nullptr;
+    d.box_depth = 0;
+    if (d.n_tris == 0 || n_nodes < 4 || !unequal_clips(nodes, n_nodes)) return PRAY_OK;
+    BoxedBuilder B;
+    if (int rc = make_boxed_layout(c, nodes, refs, A, AB, AC, B)) return rc;
+    const size_t n_brefs = B.brefs.size();
+    void *rp = nullptr, *fp = nullptr, *bp = nullptr;
+    int rc = device_alloc(c, sc, &rp, B.rec.size() * 4);
+    if (!rc) rc = device_alloc(c, sc, &fp, (n_brefs + 2) * 4);
+    if (!rc) rc = device_alloc(c, sc, &bp, (n_brefs + 2) * 48);  // padded for the ILP look-ahead
     if (rc) return rc;
-    HIPC(c, hipMemcpyAsync(p, rec.data(), rec.size() * 4, hipMemcpyHostToDevice, st));
-    HIPC(c, hipStreamSynchronize(st));  // rec is freed on return
-    d.boxes = static_cast<const uint4 *>(p);
+    HIPC(c, hipMemcpyAsync(rp, B.rec.data(), B.rec.size() * 4, hipMemcpyHostToDevice, st));
+    HIPC(c, hipMemsetAsync(fp, 0, (n_brefs + 2) * 4, st));
+    HIPC(c, hipMemcpyAsync(fp, B.brefs.data(), n_brefs * 4, hipMemcpyHostToDevice, st));
+    HIPC(c, hipMemsetAsync(bp, 0, (n_brefs + 2) * 48, st));
+    hipLaunchKernelGGL(k_pack_records, dim3(nblocks(n_brefs, 256)), dim3(256), 0, st, (const uint32_t *)fp,
+                       (uint32_t)n_brefs, d.a, d.ab, d.ac, (float4 *)bp);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(st));  // B's vectors are freed on return
+    d.boxes = static_cast<const uint4 *>(rp);
+    d.brecs = static_cast<const float4 *>(bp);
+    d.box_depth = B.depth;
     return PRAY_OK;
 }
 
@@ -576,8 +689,7 @@ int validate_tree(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, const ui
 int prepare_trace(pray_ctx *c, const SceneRec &sc, uint64_t n) {
     TraceKernel k = pick_kernel(false, sc.tune.layout, sc.tune.stack_lds, sc.tune.leaf_ilp, node_form(sc, sc.tune),
                                 sc.tune.leaf_batch > 0);
-    return ensure_spill(c, (size_t)launch_shape(c, k, sc.tune, n).lanes * std::max<uint32_t>(1, sc.d.max_depth) *
-                               sizeof(uint4));
+    return ensure_spill(c, (size_t)launch_shape(c, k, sc.tune, n).lanes * trace_depth(sc, sc.tune) * sizeof(uint4));
 }
 
 void tuning_values(const Tuning &t, int *v) {
@@ -763,6 +875,26 @@ int pray_free_scene(pray_ctx *c, int scene_id) {
     HIPC(c, hipStreamSynchronize(c->stream));
     free_scene_rec(*it->second);
     c->scenes.erase(it);
+    return PRAY_OK;
+}
+
+int pray_boxed_layout(const uint32_t *nodes, uint32_t n_nodes, const uint32_t *leaf_refs, uint32_t n_refs,
+                      const float *tri_a, const float *tri_ab, const float *tri_ac, uint32_t n_tris,
+                      uint32_t max_depth, uint32_t *n_pairs, uint32_t *n_brefs, uint32_t *depth, uint32_t *rec,
+                      uint32_t *brefs) {
+    if (!nodes || !leaf_refs || !tri_a || !tri_ab || !tri_ac || !n_pairs || !n_brefs || !depth || n_tris == 0 ||
+        n_refs == 0)
+        return PRAY_ERR_INVALID;
+    if (max_depth > MAX_TREE_DEPTH) return PRAY_ERR_UNSUPPORTED;
+    if (int v = validate_tree(nullptr, nodes, n_nodes, leaf_refs, n_refs, n_tris, max_depth)) return v;
+    if (n_nodes < 4 || !unequal_clips(nodes, n_nodes)) return PRAY_ERR_UNSUPPORTED;
+    BoxedBuilder B;
+    if (int rc = make_boxed_layout(nullptr, nodes, leaf_refs, tri_a, tri_ab, tri_ac, B)) return rc;
+    *n_pairs = (uint32_t)(B.rec.size() / 16);
+    *n_brefs = (uint32_t)B.brefs.size();
+    *depth = B.depth;
+    if (rec) std::memcpy(rec, B.rec.data(), B.rec.size() * 4);
+    if (brefs) std::memcpy(brefs, B.brefs.data(), B.brefs.size() * 4);
     return PRAY_OK;
 }
 

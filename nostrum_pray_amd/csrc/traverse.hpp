@@ -51,11 +51,13 @@ struct DevScene {
     const float *a = nullptr, *ab = nullptr, *ac = nullptr;
     const float4 *recs = nullptr;     // 3 * (n_refs + 2) (padded)
     const uint4 *quads = nullptr;     // 2 per children pair: two-level node records (SAH trees), or null
-    const uint4 *boxes = nullptr;     // 4 per children pair: boxed children pairs (median trees), or null
+    const uint4 *boxes = nullptr;     // 4 per pair of the boxed layout (median trees), or null
+    const float4 *brecs = nullptr;    // packed records of the boxed layout's leaf runs (padded)
     const float *normal = nullptr, *color = nullptr;
     const uint8_t *emit = nullptr;
     float box[6] = {0, 0, 0, 0, 0, 0};
     uint32_t n_tris = 0, n_nodes = 0, n_refs = 0, max_depth = 0;
+    uint32_t box_depth = 0;           // depth of the boxed layout
 };
 
 struct TraceArgs {
@@ -135,16 +137,18 @@ __device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, 
 // BATCH: leaf batching (TraceArgs::leaf_batch > 0) compiled in; without it
 // every iteration takes node and leaf steps and the loop carries no batching
 // code.
-// BOX: node steps over boxed children pairs (DevScene::boxes) instead of the
-// 8-byte nodes' clip planes.  Per children pair p, boxes[4p + a] for axis
-// a = 0, 1, 2 holds {child 0's lower bound, child 1's lower bound, child 0's
-// upper bound, child 1's upper bound} along that axis (float bits) and
-// boxes[4p + 3] the children's words.  A step intersects the ray's interval,
-// capped at the best t, with both children's boxes, enters the child with
-// the smaller entry distance (child 0 on a tie) and stacks the other.  For
-// trees whose children overlap (median trees: the children's boxes are
-// those of their triangles) this prunes on all three axes where the clip
-// planes bound one.
+// BOX: node steps over the boxed layout (DevScene::boxes, with the records
+// of its leaf runs in DevScene::brecs) instead of the 8-byte nodes' clip
+// planes.  Per
+// pair p, boxes[4p + a] for axis a = 0, 1, 2 holds {child 0's lower bound,
+// child 1's lower bound, child 0's upper bound, child 1's upper bound} along
+// that axis (float bits) and boxes[4p + 3] the children's words (an inner
+// child's word names its pair as w >> 3; pair 0's child 1 is the root).  A
+// step intersects the ray's interval, capped at the best t, with both
+// children's boxes, enters the child with the smaller entry distance (child
+// 0 on a tie) and stacks the other.  For trees whose children overlap
+// (median trees: the children's boxes are those of their triangles) this
+// prunes on all three axes where the clip planes bound one.
 template <bool ANY, int LAYOUT, int K, int ILP, bool MULTI = false, bool BATCH = false, bool BOX = false>
 static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s, TraceArgs A) {
     static_assert(K >= 2 && K <= 16, "LDS ring of 2..16 entries");
@@ -161,10 +165,12 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
     uint4 *spill = A.spill + ((size_t)gwave * WAVE + lane) * A.spill_depth;
     const uint32_t nn = A.n_dev ? *A.n_dev : A.n;
     const uint4 *pairs = reinterpret_cast<const uint4 *>(s.nodes);
-    const float4 *recs = s.recs;
-    // The root's word (node 1, the second word pair of the first 16 bytes),
-    // read once per wave, not once per ray (single-scene launches).
-    const uint32_t root_w = MULTI ? 0u : (uint32_t)__builtin_amdgcn_readfirstlane((int)pairs[0].w);
+    const float4 *recs = BOX ? s.brecs : s.recs;
+    // The root's word (node 1, the second word pair of the first 16 bytes;
+    // child 1 of the boxed layout's pair 0), read once per wave, not once per
+    // ray (single-scene launches).
+    const uint32_t root_w =
+        MULTI ? 0u : (uint32_t)__builtin_amdgcn_readfirstlane((int)(BOX ? s.boxes[3].y : pairs[0].w));
 
     uint32_t state = ST_IDLE, ray = 0, w = 0, k = 0, best_i = NO_HIT, cn = 0, cr = 0;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, lim2 = 0;

@@ -51,9 +51,8 @@ def lib():
                                                  _vp]),
             "orc_flat_trace": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp,
                                       _vp]),
-            "orc_flat_trace_boxed": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _u64, C.c_int, _f32, _vp,
-                                            _vp, _vp, _vp]),
-            "orc_node_boxes": (None, [_vp, _u64, _vp, _vp, _vp, _vp, _vp]),
+            "orc_trace_boxed": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp,
+                                       _vp]),
             "orc_camera": (None, [_vp, _vp, _u64, _u64, _u64, _vp]),
             "orc_camera_rays": (None, [_vp, _u64, _u64, _vp]),
             "orc_mlp_encode": (None, [_vp, _u64, _vp]),
@@ -228,30 +227,20 @@ def ref_closest_avx(export, a, ab, ac, rays, eps, threads):
     return idx, t, sec
 
 
-def node_boxes(flat):
-    """Per-node boxes (n_nodes x 6) of a flat tree: the union of the
-    subtree's triangle boxes, rounded outward."""
-    nodes = np.ascontiguousarray(flat["nodes"], np.uint32)
-    n_nodes = nodes.size // 2
-    out = np.empty((n_nodes, 6), np.float32)
-    lib().orc_node_boxes(_p(nodes), n_nodes, _p(flat["refs"]), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]),
-                         _p(out))
-    return out
-
-
-def flat_trace(flat, rays, mode=0, eps=1e-4, limit2=None, counts=False, boxed=False):
-    """The GPU traversal's twin; boxed: with boxed node steps (node_levels
-    3), the boxes from node_boxes."""
+def flat_trace(flat, rays, mode=0, eps=1e-4, limit2=None, counts=False, boxed=None):
+    """The GPU traversal's twin over the flat layout, or with boxed node steps
+    over `boxed`, a boxed layout {"rec": pairs x 16 uint32, "refs": leaf
+    runs} (node_levels 3)."""
     rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
     n = rays.shape[0]
     idx, t = np.empty(n, np.int32), np.empty(n, np.float32)
     cnt = np.empty((n, 2), np.uint32) if counts else None
     lim = None if limit2 is None else np.ascontiguousarray(limit2, np.float32)
-    if boxed:
-        nb = node_boxes(flat)
-        lib().orc_flat_trace_boxed(_p(flat["nodes"]), _p(nb), _p(flat["refs"]), _p(flat["a"]), _p(flat["ab"]),
-                                   _p(flat["ac"]), _p(flat["box"]), flat["a"].shape[0], _p(rays), n, mode, eps,
-                                   _p(lim), _p(idx), _p(t), _p(cnt))
+    if boxed is not None:
+        rec = np.ascontiguousarray(boxed["rec"], np.uint32)
+        brefs = np.ascontiguousarray(boxed["refs"], np.uint32)
+        lib().orc_trace_boxed(_p(rec), _p(brefs), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]), _p(flat["box"]),
+                              flat["a"].shape[0], _p(rays), n, mode, eps, _p(lim), _p(idx), _p(t), _p(cnt))
     else:
         lib().orc_flat_trace(_p(flat["nodes"]), _p(flat["refs"]), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]),
                              _p(flat["box"]), flat["a"].shape[0], _p(rays), n, mode, eps, _p(lim), _p(idx), _p(t),

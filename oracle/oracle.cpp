@@ -322,16 +322,17 @@ uint32_t flatTrace(const Flat &f, V o, V d, int mode, float lim2, float eps, flo
     return best_i;
 }
 
-// Twin of the boxed node step (the GPU traversal with boxed children pairs,
-// node_levels 3): each node step tests both children's full boxes (nb, 6
-// floats per node) against the ray's interval capped at the best t, enters
-// the child with the smaller entry distance (child 0 on a tie) and stacks
-// the other; leaves, stack and counters as flatTrace.  The median tree
-// (kd_tree.cpp:47-84) is an object partition whose nodes carry the box of
-// their triangles (the reference tests that box per node, kd_tree.cpp:142-
-// 279); its 8-byte nodes keep only the bound along the parent's split axis.
-uint32_t flatTraceBoxed(const Flat &f, const float *nb, V o, V d, int mode, float lim2, float eps, float &tout,
-                        uint32_t &cn, uint32_t &cr) {
+// Twin of the boxed node step over the boxed layout (pray_boxed_layout,
+// node_levels 3): 16 words per pair, per axis {child 0 lower, child 1 lower,
+// child 0 upper, child 1 upper}, then the children's words; pair 0's child 1
+// is the root.  Each node step tests both children's boxes against the
+// ray's interval capped at the best t, enters the child with the smaller
+// entry distance (child 0 on a tie) and stacks the other; leaves, stack and
+// counters as flatTrace.  The median tree (kd_tree.cpp:47-84) is an object
+// partition whose nodes carry the box of their triangles, and the reference
+// tests that box per node (kd_tree.cpp:142-279).
+uint32_t boxedTrace(const uint32_t *rec, const Flat &f, V o, V d, int mode, float lim2, float eps, float &tout,
+                    uint32_t &cn, uint32_t &cr) {
     const V inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     float lo = 0.f, hi = INFINITY;
     slabW(f.box[0], f.box[3], o.x, inv.x, lo, hi);
@@ -348,31 +349,33 @@ uint32_t flatTraceBoxed(const Flat &f, const float *nb, V o, V d, int mode, floa
         uint32_t w;
         float lo, hi;
     };
-    E st[64];
+    E st[128];
     int sp = 0;
-    uint32_t w = f.nodes[3];
+    uint32_t w = rec[13];
     cn += 1;
+    auto fl = [](uint32_t u) {
+        float x;
+        std::memcpy(&x, &u, 4);
+        return x;
+    };
     while (true) {
         while ((w & 3u) != LEAF_TAG) {
-            const uint32_t lc = w >> 2;
+            const uint32_t *r = rec + 16 * (size_t)(w >> 3);
             cn += 2;
             const float cap = mode == 0 ? std::fmin(hi, best_t) : hi;
             float l[2], h[2];
             bool g[2];
-            uint32_t cw[2];
             for (int c = 0; c < 2; c++) {
-                const float *b = nb + 6 * (size_t)(lc + c);
                 l[c] = lo, h[c] = cap;
-                slabW(b[0], b[3], o.x, inv.x, l[c], h[c]);
-                slabW(b[1], b[4], o.y, inv.y, l[c], h[c]);
-                slabW(b[2], b[5], o.z, inv.z, l[c], h[c]);
-                cw[c] = f.nodes[2 * (size_t)(lc + c) + 1];
-                g[c] = l[c] <= h[c] && cw[c] != ((LEAF_EMPTY << 2) | LEAF_TAG);
+                slabW(fl(r[c]), fl(r[2 + c]), o.x, inv.x, l[c], h[c]);
+                slabW(fl(r[4 + c]), fl(r[6 + c]), o.y, inv.y, l[c], h[c]);
+                slabW(fl(r[8 + c]), fl(r[10 + c]), o.z, inv.z, l[c], h[c]);
+                g[c] = l[c] <= h[c] && r[12 + c] != ((LEAF_EMPTY << 2) | LEAF_TAG);
             }
             const int n = (g[1] && (!g[0] || l[1] < l[0])) ? 1 : 0;
-            if (g[0] && g[1]) st[sp++] = {cw[1 - n], l[1 - n], h[1 - n]};
+            if (g[0] && g[1]) st[sp++] = {r[12 + 1 - n], l[1 - n], h[1 - n]};
             if (g[n]) {
-                w = cw[n], lo = l[n], hi = h[n];
+                w = r[12 + n], lo = l[n], hi = h[n];
             } else {
                 w = (LEAF_EMPTY << 2) | LEAF_TAG;
             }
@@ -414,16 +417,6 @@ uint32_t flatTraceBoxed(const Flat &f, const float *nb, V o, V d, int mode, floa
     }
     tout = best_t;
     return best_i;
-}
-
-// A float at or below (down) / at or above (up) a double.
-inline float floatDown(double x) {
-    const float f = (float)x;
-    return (double)f > x ? std::nextafter(f, -INFINITY) : f;
-}
-inline float floatUp(double x) {
-    const float f = (float)x;
-    return (double)f < x ? std::nextafter(f, INFINITY) : f;
 }
 
 // ----------------------------------------------------- reference KD tree --
@@ -728,65 +721,20 @@ ORC_API void orc_flat_trace(const uint32_t *nodes, const uint32_t *refs, const f
     }
 }
 
-// Per-node boxes of a flat tree (n_nodes x 6: min xyz, max xyz): the union
-// over the node's subtree of its referenced triangles' boxes, with the
-// vertices a, a + ab, a + ac summed in double and the bounds rounded outward
-// to float, so that each box holds the triangle Moeller-Trumbore tests;
-// {+inf, -inf} for an empty subtree and for node 0.
-ORC_API void orc_node_boxes(const uint32_t *nodes, uint64_t n_nodes, const uint32_t *refs, const float *A,
-                            const float *AB, const float *AC, float *boxes) {
-    for (uint64_t i = 0; i < n_nodes; i++)
-        for (int k = 0; k < 3; k++) boxes[6 * i + k] = INFINITY, boxes[6 * i + 3 + k] = -INFINITY;
-    if (n_nodes < 2) return;
-    // post order: a node's box after both children's
-    std::vector<std::pair<uint32_t, bool>> stack{{1u, false}};
-    while (!stack.empty()) {
-        const auto [i, done] = stack.back();
-        stack.pop_back();
-        const uint32_t w = nodes[2 * (size_t)i + 1];
-        float *b = boxes + 6 * (size_t)i;
-        if ((w & 3u) == LEAF_TAG) {
-            uint32_t k = w >> 2;
-            if (k == LEAF_EMPTY) continue;
-            while (true) {
-                const uint32_t tri = refs[k] & ~REF_LAST;
-                for (int c = 0; c < 3; c++) {
-                    const double a = A[3 * (size_t)tri + c], p = a + AB[3 * (size_t)tri + c],
-                                 q = a + AC[3 * (size_t)tri + c];
-                    b[c] = std::fmin(b[c], floatDown(std::fmin(a, std::fmin(p, q))));
-                    b[3 + c] = std::fmax(b[3 + c], floatUp(std::fmax(a, std::fmax(p, q))));
-                }
-                if (refs[k] & REF_LAST) break;
-                ++k;
-            }
-        } else if (!done) {
-            stack.push_back({i, true});
-            stack.push_back({w >> 2, false});
-            stack.push_back({(w >> 2) + 1, false});
-        } else {
-            for (int c = 0; c < 2; c++) {
-                const float *cb = boxes + 6 * (size_t)((w >> 2) + c);
-                for (int k = 0; k < 3; k++) b[k] = std::fmin(b[k], cb[k]), b[3 + k] = std::fmax(b[3 + k], cb[3 + k]);
-            }
-        }
-    }
-}
-
-// flat_trace with boxed node steps (flatTraceBoxed), boxes from
-// orc_node_boxes.
-ORC_API void orc_flat_trace_boxed(const uint32_t *nodes, const float *node_boxes, const uint32_t *refs, const float *A,
-                                  const float *AB, const float *AC, const float *box6, uint64_t n_tris,
-                                  const float *rays, uint64_t n, int mode, float eps, const float *lim2,
-                                  int32_t *out_idx, float *out_t, uint32_t *counts) {
-    const Flat f{nodes, refs, A, AB, AC, box6};
+// flat_trace over the boxed layout (boxedTrace): rec 16 words per pair,
+// brefs its leaf runs (the last entry of each run flagged).
+ORC_API void orc_trace_boxed(const uint32_t *rec, const uint32_t *brefs, const float *A, const float *AB,
+                             const float *AC, const float *box6, uint64_t n_tris, const float *rays, uint64_t n,
+                             int mode, float eps, const float *lim2, int32_t *out_idx, float *out_t,
+                             uint32_t *counts) {
+    const Flat f{nullptr, brefs, A, AB, AC, box6};
 #pragma omp parallel for schedule(dynamic, 64)
     for (uint64_t i = 0; i < n; i++) {
         uint32_t cn = 0, cr = 0;
         float t = INFINITY;
         uint32_t h = NO_HIT;
         if (n_tris)
-            h = flatTraceBoxed(f, node_boxes, ld(rays + 6 * i), ld(rays + 6 * i + 3), mode, lim2 ? lim2[i] : 0.f, eps,
-                               t, cn, cr);
+            h = boxedTrace(rec, f, ld(rays + 6 * i), ld(rays + 6 * i + 3), mode, lim2 ? lim2[i] : 0.f, eps, t, cn, cr);
         out_idx[i] = (int32_t)h;
         out_t[i] = h == NO_HIT ? FLT_MAX : t;
         if (counts) counts[2 * i] = cn, counts[2 * i + 1] = cr;

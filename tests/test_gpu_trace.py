@@ -198,8 +198,8 @@ def test_work_counts_match_cpu_twin(ctx, lb, nl):
 def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb):
     """Boxed node steps (node records 3, the default for median trees whose
     children overlap): per-ray node and triangle counts and hits equal the
-    CPU twin's with the same boxes, with and without leaf batching; the
-    twin's hits equal the plane-form traversal's and brute force."""
+    CPU twin's over the same boxed layout, with and without leaf batching;
+    the twin's hits equal the plane-form traversal's and brute force."""
     s, t, flat = scene(name)
     rays = s.prim_rays() if rays_of == "camera" else O.gen_rays(4, P.scene_box_expanded(flat["box"]), 1 << 15)
     sid = ctx.upload(t)
@@ -217,7 +217,7 @@ def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb):
         gc, gi, gt = d_c.numpy(), d_idx.numpy(), d_t.numpy()
     finally:
         ctx.free_scene(sid)
-    oi, ot, oc = O.flat_trace(flat, rays, 0, 1e-4, counts=True, boxed=True)
+    oi, ot, oc = O.flat_trace(flat, rays, 0, 1e-4, counts=True, boxed=P.boxed_layout(flat))
     assert np.array_equal(gi, oi)
     assert np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
     assert np.array_equal(gc, oc)

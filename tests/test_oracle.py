@@ -196,75 +196,91 @@ def test_flat_twin_equals_bruteforce():
             assert np.array_equal(ft[fi >= 0], bt[bi >= 0])
 
 
-def node_vertex_bounds(f):
-    """Per node, the float64 min and max of the vertices a, a + ab, a + ac of
-    the triangles its subtree references (NaN for an empty subtree), by a
-    recursive walk of the flat layout."""
-    nodes = f["nodes"].reshape(-1, 2)
-    a, ab, ac = (f[k].astype(np.float64).reshape(-1, 3) for k in ("a", "ab", "ac"))
-    v = np.stack([a, a + ab, a + ac], 1)
-    tmin, tmax = v.min(1), v.max(1)
-    lo = np.full((len(nodes), 3), np.nan)
-    hi = np.full((len(nodes), 3), np.nan)
+def leaf_runs(refs, k):
+    """Triangle indices of the leaf run starting at k."""
+    out = []
+    while True:
+        r = int(refs[k])
+        out.append(r & 0x7FFFFFFF)
+        if r & 0x80000000:
+            return out
+        k += 1
 
-    def walk(i):
-        w = int(nodes[i, 1])
+
+def boxed_subtrees(L):
+    """Per pair of a boxed layout, for each child: the triangles its subtree
+    references (by walking the records), and the leaves' run lengths."""
+    rec, refs = L["rec"], L["refs"]
+    runs = []
+
+    def tris(w):
+        if w == 0xFFFFFFFF:
+            return []
         if w & 3 == 3:
-            k = w >> 2
-            if k == 0x3FFFFFFF:
-                return
-            tris = []
-            while True:
-                r = int(f["refs"][k])
-                tris.append(r & 0x7FFFFFFF)
-                if r & 0x80000000:
-                    break
-                k += 1
-            lo[i], hi[i] = tmin[tris].min(0), tmax[tris].max(0)
-            return
-        for c in (w >> 2, (w >> 2) + 1):
-            walk(c)
-        kids = [c for c in (w >> 2, (w >> 2) + 1) if not np.isnan(lo[c, 0])]
-        if kids:
-            lo[i], hi[i] = lo[kids].min(0), hi[kids].max(0)
+            r = leaf_runs(refs, w >> 2)
+            runs.append(len(r))
+            return r
+        p = w >> 3
+        return tris(int(rec[p, 12])) + tris(int(rec[p, 13]))
 
-    walk(1)
-    return lo, hi
+    return tris, runs
 
 
-def test_node_boxes_round_outward():
-    """Each node box is the union of its subtree's triangle boxes with the
-    vertices a + ab, a + ac summed exactly (float64) and rounded outward: the
-    largest float at or below the exact minimum and the smallest at or above
-    the exact maximum; empty subtrees and node 0 get {+inf, -inf}."""
+def test_boxed_layout_structure():
+    """The library's boxed layout (pray_boxed_layout, host code) of a median
+    tree: it references every leaf reference of the tree once (a partition of
+    each leaf), its leaves hold at most 4, every child box is the union of its
+    triangles' boxes with the vertices a + ab, a + ac summed exactly (float64)
+    and rounded outward -- the largest float at or below the exact minimum and
+    the smallest at or above the exact maximum -- and SAH trees (equal clips)
+    get none."""
     import nostrum_pray_amd as P
     s = P.Scene.spheres(3)
     f = s.build(0).flat()
-    nb = O.node_boxes(f).astype(np.float64)
-    lo, hi = node_vertex_bounds(f)
-    full = ~np.isnan(lo[:, 0])
-    assert full.sum() > 100 and not full[0]
-    assert (nb[~full, :3] == np.inf).all() and (nb[~full, 3:] == -np.inf).all()
-    bmin, bmax = nb[full, :3].astype(np.float32), nb[full, 3:].astype(np.float32)
-    assert (bmin <= lo[full]).all() and (np.nextafter(bmin, np.float32(np.inf)) > lo[full]).all()
-    assert (bmax >= hi[full]).all() and (np.nextafter(bmax, np.float32(-np.inf)) < hi[full]).all()
+    L = P.boxed_layout(f)
+    rec, refs = L["rec"], L["refs"]
+    assert np.array_equal(np.sort(refs & 0x7FFFFFFF), np.sort(f["refs"] & 0x7FFFFFFF))
+    assert rec[0, 12] == 0xFFFFFFFF and L["depth"] <= f["max_depth"] + 5
+    a, ab, ac = (f[k].astype(np.float64).reshape(-1, 3) for k in ("a", "ab", "ac"))
+    v = np.stack([a, a + ab, a + ac], 1)
+    tmin, tmax = v.min(1), v.max(1)
+    tris, runs = boxed_subtrees(L)
+    fb = rec[:, :12].copy().view(np.float32)
+    checked = 0
+    for p in range(len(rec)):
+        for ch in (0, 1):
+            t = tris(int(rec[p, 12 + ch]))
+            lo = fb[p, [ch, 4 + ch, 8 + ch]].astype(np.float32)
+            hi = fb[p, [2 + ch, 6 + ch, 10 + ch]].astype(np.float32)
+            if not t:
+                assert (lo == np.inf).all() and (hi == -np.inf).all()
+                continue
+            elo, ehi = tmin[t].min(0), tmax[t].max(0)
+            assert (lo <= elo).all() and (np.nextafter(lo, np.float32(np.inf)) > elo).all(), (p, ch)
+            assert (hi >= ehi).all() and (np.nextafter(hi, np.float32(-np.inf)) < ehi).all(), (p, ch)
+            checked += 1
+    assert checked > 1000
+    assert max(runs) <= 4 and sum(runs) >= len(refs)
     # the root box is the scene's box up to that rounding
-    assert np.allclose(nb[1], f["box"], rtol=1e-6, atol=1e-6)
+    assert np.allclose(fb[0, [1, 5, 9, 3, 7, 11]], f["box"], rtol=1e-6, atol=1e-6)
+    with pytest.raises(P.PrayError):
+        P.boxed_layout(s.build(1).flat())
 
 
 @pytest.mark.parametrize("which", ["spheres", "random", "cornell"])
 def test_boxed_twin_equals_bruteforce(which):
-    """The boxed node-step twin on median trees (children overlap) gives the
-    brute-force closest hit on every ray, camera and incoherent, and any-hit
-    answers that threshold it, with fewer triangle tests than the plane
-    form."""
+    """The boxed node-step twin over the library's boxed layout of median
+    trees gives the brute-force closest hit on every ray, camera and
+    incoherent, and any-hit answers that threshold it, with fewer triangle
+    tests than the plane form."""
     import nostrum_pray_amd as P
     s = {"spheres": lambda: P.Scene.spheres(1), "random": lambda: P.Scene.random(3, 5000),
          "cornell": lambda: P.Scene.cornell(7, 8000)}[which]()
     f = s.build(0).flat()
+    L = P.boxed_layout(f)
     rays = np.concatenate([s.prim_rays()[::41], O.gen_rays(13, P.scene_box_expanded(f["box"]), 6000)])
     for eps in (1e-4, 1e-9):
-        fi, ft, fc = O.flat_trace(f, rays, 0, eps, counts=True, boxed=True)
+        fi, ft, fc = O.flat_trace(f, rays, 0, eps, counts=True, boxed=L)
         bi, bt = O.bf_closest(f["a"], f["ab"], f["ac"], rays, eps)
         assert np.array_equal(fi, bi)
         assert np.array_equal(ft[fi >= 0], bt[bi >= 0])
@@ -273,13 +289,13 @@ def test_boxed_twin_equals_bruteforce(which):
     hit = fi >= 0
     lim = np.ones_like(ft)
     lim[hit] = ft[hit] * ft[hit]
-    ai, _ = O.flat_trace(f, rays, 1, 1e-9, limit2=lim, boxed=True)
-    ii, _ = O.flat_trace(f, rays, 2, 1e-9, limit2=lim, boxed=True)
+    ai, _ = O.flat_trace(f, rays, 1, 1e-9, limit2=lim, boxed=L)
+    ii, _ = O.flat_trace(f, rays, 2, 1e-9, limit2=lim, boxed=L)
     assert not (ai >= 0).any()
     assert np.array_equal(ii >= 0, hit)
     lim2 = np.where(hit, lim * np.float32(4.0), np.float32(1.0)).astype(np.float32)
-    ai2, at2 = O.flat_trace(f, rays, 1, 1e-9, limit2=lim2, boxed=True)
-    assert np.array_equal(ai2 >= 0, hit & (ft > 0))
+    ai2, _ = O.flat_trace(f, rays, 1, 1e-9, limit2=lim2, boxed=L)
+    assert np.array_equal(ai2 >= 0, hit)
 
 
 def test_flat_twin_any_hit_is_threshold_of_closest():
