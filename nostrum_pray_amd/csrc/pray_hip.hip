@@ -554,10 +554,10 @@ struct BoxedBuilder {
         int ax = 0;
         for (int a = 1; a < 3; ++a)
             if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
-        std::sort(t, t + n, [&](uint32_t x, uint32_t y) {
-            const double cx = centroid(x, ax), cy = centroid(y, ax);
-            return cx < cy || (cx == cy && x < y);
-        });
+        std::vector<std::pair<double, uint32_t>> key(n);
+        for (uint32_t j = 0; j < n; ++j) key[j] = {centroid(t[j], ax), t[j]};
+        std::sort(key.begin(), key.end());  // by centroid, then index
+        for (uint32_t j = 0; j < n; ++j) t[j] = key[j].second;
         const uint32_t p = new_pair();
         Bounds b0, b1;
         const uint32_t w0 = leaf(t, n / 2, d + 1, b0);
