@@ -6,14 +6,16 @@ path tracer against the plain path tracer on the 100k-triangle Cornell room.
 A step is one closest-hit launch over the whole ray stream (rays resident in
 HBM).  Prints ONE JSON line with the metric, a roofline object for the
 traversal kernel, the radiance-cache comparison, the batched multi-scene
-render (64 scenes in one wavefront, key "batch") and a CPU baseline (the
+render (64 scenes in one wavefront, key "batch"), the ray tracer on the
+25,602-triangle spheres scene (key "ray_tracer") and a CPU baseline (the
 restated reference CPU path timed on this host).
 Run:  python bench.py [--steps K] [--warmup W]
 
 Other workloads, one JSON line each: --workload batch (64 scenes path traced
 in one wavefront), --workload io (flat tree files, progressive path tracing,
 the drop-in CLI on the example scenes), --workload cache (the radiance-cache
-comparison alone), --workload build (the KD-tree builds on the GPU).
+comparison alone), --workload build (the KD-tree builds on the GPU),
+--workload rt (the ray tracer on the spheres scene).
 """
 import argparse
 import json
@@ -26,7 +28,9 @@ os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
 # With a binding set, libgomp pins the loading thread to its first place, so
 # the CPUs this process may use are counted before that happens.
-USABLE_CPUS = len(os.sched_getaffinity(0))
+ALL_CPUS = os.sched_getaffinity(0)
+USABLE_CPUS = len(ALL_CPUS)
+OMP_MASTER_CPUS = ALL_CPUS  # the loading thread's place, once libgomp has pinned it
 
 import platform  # noqa: E402
 import subprocess  # noqa: E402
@@ -79,6 +83,29 @@ def cpu_model():
     return cpu_info()["model"]
 
 
+def load_libraries(P):
+    """Loads the native libraries.  libgomp, loaded with the host library,
+    pins this thread to its first place (OMP_PROC_BIND); the thread's full
+    CPU set is then restored, so that the threads the GPU runtime starts later
+    and the processes this run spawns (the CLIs, which inherit it) are not
+    confined to that one core.  The CPU baseline re-pins the thread to its
+    place while it runs (omp_master_pinned)."""
+    global OMP_MASTER_CPUS
+    P.libraries()
+    OMP_MASTER_CPUS = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, ALL_CPUS)
+
+
+class omp_master_pinned:
+    """This thread on its OpenMP place (the first core) for the duration."""
+
+    def __enter__(self):
+        os.sched_setaffinity(0, OMP_MASTER_CPUS)
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, ALL_CPUS)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +153,7 @@ def main():
                     help="samples per pixel in each radiance-cache frame (one training step per frame)")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg of the default run")
+    ap.add_argument("--no-rt", action="store_true", help="skip the ray-tracer leg (spheres scene) of the default run")
     ap.add_argument("--path-budget", type=int, default=-1,
                     help="paths the plain path tracer keeps in flight (0: one sample per pass; default: the "
                          "library's, 2^24)")
@@ -140,6 +168,7 @@ def main():
 
     import nostrum_pray_amd as P
 
+    load_libraries(P)
     if args.workload == "batch":
         batch_workload(P, args)
         return
@@ -335,6 +364,17 @@ def main():
                     "tuning": rtun}
         log(f"reference-parameter tree: {ms_ref:.3f} ms/step, identical hits {same}")
 
+    # The ray tracer before the path-tracing legs: frames measured after both
+    # of them in the same process took 0.18 ms longer in the host-side copies
+    # (kernel times unchanged), which a ray-tracing process does not see.
+    rt = None
+    if not args.no_rt:
+        # a context of its own, as a ray-tracing application would hold
+        rctx = P.Context(0)
+        configure(rctx, args)
+        rt = ray_tracer_leg(P, rctx, args)
+        rctx.close()
+
     rc_out = None
     if not args.no_cache:
         for b in (d_rays, d_idx, d_t, d_lim, d_cnt2):
@@ -352,6 +392,7 @@ def main():
         batch["per_scene"] = [[p["triangles"], p["image"], p["rays"], p["Mrays_per_s_alone"], p["equal_to_batch"]]
                               for p in batch["per_scene"]]
         batch["per_scene_fields"] = ["triangles", "image_side", "rays", "Mrays_per_s_alone", "equal_to_batch"]
+
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -405,6 +446,7 @@ def main():
         "other_trees": others,
         "radiance_cache": rc_out,
         "batch": batch,
+        "ray_tracer": rt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -837,6 +879,18 @@ def equal_rmse_time(points, target):
 
 
 def rt_workload(P, args):
+    """--workload rt: ray_tracer_leg as one JSON line of its own."""
+    ctx = P.Context(0)
+    configure(ctx, args)
+    r = ray_tracer_leg(P, ctx, args)
+    ctx.close()
+    print(json.dumps({"metric": "Mrays/s ray tracing (primary + shadow rays), spheres scene",
+                      "value": r.pop("Mrays_per_s"), "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True,
+                      "dtype": "f32", "data": "synthetic", "config": {"workload": r.pop("workload")}, **r}),
+          flush=True)
+
+
+def ray_tracer_leg(P, ctx, args):
     """The ray tracer (pray_render_rt) on the spheres scene: a level-4
     icosphere, 16 level-3 icospheres and a ground quad (25,602 triangles),
     median KD tree, 512x512, fov 60, one point light.  --steps calls are
@@ -844,12 +898,10 @@ def rt_workload(P, args):
     rays are the primary rays plus one shadow ray per lit hit.  The scene is
     also written as JSON + OBJ and rendered by the drop-in CLI and, as the
     CPU baseline, by the reference's own CLI compiled here; their images are
-    compared byte for byte.  One JSON line."""
+    compared byte for byte."""
     import shutil
     import tempfile
 
-    ctx = P.Context(0)
-    configure(ctx, args)
     scene = P.Scene.spheres(1)
     tree = scene.build(0)
     sid = ctx.upload(tree)
@@ -892,7 +944,6 @@ def rt_workload(P, args):
         b.free()
     achieved = bytes_per_ray * npix / (kernel_ms * 1e-3) / 1e9
     ctx.free_scene(sid)
-    ctx.close()
     log(f"rt: {scene.num_tris} triangles {w}x{h}: {ms:.3f} ms per frame (median of {args.steps}), "
         f"{rays} rays, {rays / ms / 1e3:.1f} Mrays/s; closest-hit launch {kernel_ms:.4f} ms, "
         f"nodes/ray {nodes_per_ray:.1f} refs/ray {refs_per_ray:.1f}")
@@ -910,11 +961,10 @@ def rt_workload(P, args):
             cpu = rt_cpu_baseline(path, out if r.returncode == 0 else None, tmp)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
-    print(json.dumps({
-        "metric": "Mrays/s ray tracing (primary + shadow rays), spheres scene", "value": round(rays / ms / 1e3, 2),
-        "unit": "Mrays/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"ray tracer, {scene.num_tris} triangles (1 level-4 and 16 level-3 icospheres, a "
-                               f"ground quad), median KD tree, {w}x{h}, fov {inf['fov']}, {len(lights)} light"},
+    return {
+        "Mrays_per_s": round(rays / ms / 1e3, 2),
+        "workload": f"ray tracer, {scene.num_tris} triangles (1 level-4 and 16 level-3 icospheres, a ground quad), "
+                    f"median KD tree, {w}x{h}, fov {inf['fov']}, {len(lights)} light",
         "ms_per_frame": round(ms, 4), "rays_per_frame": rays, "tuning": tun,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -925,7 +975,7 @@ def rt_workload(P, args):
                      "nodes_per_ray": round(float(nodes_per_ray), 2), "refs_per_ray": round(float(refs_per_ray), 2),
                      "note": "the scene's records (1.3 MB) stay in L2; the launch of 262,144 rays is bound by "
                              "its longest rays' chains of dependent loads, not by HBM"},
-        "cli": cli, "cpu_baseline": cpu}), flush=True)
+        "cli": cli, "cpu_baseline": cpu}
 
 
 def rt_cpu_baseline(scene_json, gpu_bmp, tmp):
@@ -1080,7 +1130,8 @@ def cpu_baseline(scene, args, box, gpu_idx):
     fl = tree.flat()
     n = min(args.cpu_rays, args.rays)
     rays = O.gen_rays(args.seed, box, n)
-    idx, t, sec = O.ref_closest_avx(ex, fl["a"], fl["ab"], fl["ac"], rays, args.det_eps, threads)
+    with omp_master_pinned():
+        idx, t, sec = O.ref_closest_avx(ex, fl["a"], fl["ab"], fl["ac"], rays, args.det_eps, threads)
     log(f"cpu baseline: {n} rays in {sec:.2f}s on {threads} threads ({cpu['model']}, "
         f"{cpu.get('physical_cores', '?')} physical cores, {cpu.get('logical_cpus', '?')} logical CPUs, "
         f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} OMP_PLACES={os.environ.get('OMP_PLACES')})")
@@ -1097,12 +1148,13 @@ def cpu_baseline(scene, args, box, gpu_idx):
     ref_code = None
     if os.path.exists(O.REF_LIB):
         abc, _, _ = scene.triangles()
-        t0 = time.perf_counter()
-        db = O.RefDB(abc, 0)
-        rbuild = time.perf_counter() - t0
-        del abc
-        nr = min(1 << 20, n)
-        ridx, _, rsec = db.intersect_avx(rays[:nr], threads)
+        with omp_master_pinned():
+            t0 = time.perf_counter()
+            db = O.RefDB(abc, 0)
+            rbuild = time.perf_counter() - t0
+            del abc
+            nr = min(1 << 20, n)
+            ridx, _, rsec = db.intersect_avx(rays[:nr], threads)
         db.close()
         ref_code = {"sample": f"first {nr} rays", "det_eps": 1e-4, "hits": int((ridx >= 0).sum()),
                     "Mrays_per_s": round(nr / rsec / 1e6, 4), "build_s": round(rbuild, 2), "cores": threads,
