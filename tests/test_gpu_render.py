@@ -97,20 +97,24 @@ def test_rt_multi_light_negative_lambert_quirk(ctx):
 # (hipGraph replays, path budget): all 8 samples in one pass (the default
 # budget), passes of 3, 3 and 2 samples, and one sample per pass; with and
 # without graphs.
-@pytest.mark.parametrize("use_graph,budget", [(1, None), (0, None), (1, 3 * 64 * 48), (0, 3 * 64 * 48), (1, 0),
-                                              (0, 0)])
-def test_pt_cornell_equal_oracle(ctx, use_graph, budget):
+# The last two run the room's median tree, which the upload gives the boxed
+# layout: the path tracer's bounces then take boxed node steps.
+@pytest.mark.parametrize("use_graph,budget,kind", [(1, None, 1), (0, None, 1), (1, 3 * 64 * 48, 1),
+                                                   (0, 3 * 64 * 48, 1), (1, 0, 1), (0, 0, 1), (1, None, 0),
+                                                   (0, 3 * 64 * 48, 0)])
+def test_pt_cornell_equal_oracle(ctx, use_graph, budget, kind):
     s = P.Scene.cornell(3, 20000)
-    t = s.build(1)
+    t = s.build(kind)
     f = t.flat()
     inf = s.info()
     w, h, spp, depth, rr = 64, 48, 8, 6, 2
     cam = O.camera([0, 1, 3.8], [0, 0, -1], 40, w, h)
     sid = ctx.upload(t)
+    assert ctx.get_tuning(sid)["node_levels"] == (3 if kind == 0 else 1)
     if budget is not None:
         ctx.set_path_budget(budget)
     try:
-        say(f"pt cornell graph={use_graph} budget={budget}")
+        say(f"pt cornell graph={use_graph} budget={budget} tree={kind}")
         hdr = np.empty((h, w, 3), np.float32)
         rgb = np.empty((h, w, 3), np.uint8)
         lib = P.libraries()[0]
