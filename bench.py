@@ -389,9 +389,10 @@ def main():
         batch = batch_leg(P, ctx, args)
         batch.pop("autotuned_buckets", None)
         # per scene: [triangles, image side, rays, Mrays/s rendered alone, image equal to the batch's]
-        batch["per_scene"] = [[p["triangles"], p["image"], p["rays"], p["Mrays_per_s_alone"], p["equal_to_batch"]]
-                              for p in batch["per_scene"]]
-        batch["per_scene_fields"] = ["triangles", "image_side", "rays", "Mrays_per_s_alone", "equal_to_batch"]
+        batch["per_scene"] = [[p["triangles"], p["image"], p["rays"], p["Mrays_per_s_alone"],
+                               p["Msegments_per_s_alone"], p["equal_to_batch"]] for p in batch["per_scene"]]
+        batch["per_scene_fields"] = ["triangles", "image_side", "rays", "Mrays_per_s_alone", "Msegments_per_s_alone",
+                                     "equal_to_batch"]
 
 
     cpu = None
@@ -504,7 +505,13 @@ def batch_leg(P, ctx, args):
     imgs, _ = ctx.render_pt_batch(sids, cams, sizes, 1, depth, rr, seeds, bgs, 1e-9)  # warm-up
     imgs, ms = ctx.render_pt_batch(sids, cams, sizes, spp, depth, rr, seeds, bgs, 1e-9)
     rays = ctx.last_render_rays()
-    log(f"batch: {ms:.1f} ms, {rays} rays, {rays / ms / 1e3:.1f} Mrays/s")
+    # Path segments: the rays a tracer that traces every sample's camera ray
+    # itself would trace (the renderers trace each pixel's camera ray once
+    # per call and share its hit among the samples).
+    n_pix = sum(w * h for w, h in sizes)
+    segs = rays + (spp - 1) * n_pix
+    log(f"batch: {ms:.1f} ms, {rays} rays traced ({rays / ms / 1e3:.1f} Mrays/s), {segs} path segments "
+        f"({segs / ms / 1e3:.1f} M/s)")
     per, same, single_ms = [], True, 0.0
     for k in range(n):
         ctx.record(0)
@@ -516,8 +523,10 @@ def batch_leg(P, ctx, args):
         ok = bool(np.array_equal(hdr.view(np.uint32), imgs[k].view(np.uint32)))
         same = same and ok
         single_ms += kms
+        ks = kr + (spp - 1) * sizes[k][0] * sizes[k][1]
         per.append({"triangles": scenes[k].num_tris, "image": sizes[k][0], "ms_alone": round(kms, 3),
-                    "rays": kr, "Mrays_per_s_alone": round(kr / kms / 1e3, 2), "equal_to_batch": ok})
+                    "rays": kr, "Mrays_per_s_alone": round(kr / kms / 1e3, 2),
+                    "Msegments_per_s_alone": round(ks / kms / 1e3, 2), "equal_to_batch": ok})
     for sid in sids:
         ctx.free_scene(sid)
     return {
@@ -527,6 +536,9 @@ def batch_leg(P, ctx, args):
         "scenes": n, "triangles": n_tris, "spp": spp, "max_depth": depth,
         "build_wall_s": round(build_s, 2), "gpu_build_ms_total": round(float(sum(build_ms)), 2),
         "batch_ms": round(ms, 2), "rays": rays, "Mrays_per_s": round(rays / ms / 1e3, 2),
+        "path_segments": segs, "Msegments_per_s": round(segs / ms / 1e3, 2),
+        "rays_note": "rays traced, each pixel's camera ray once per render call (its hit is shared by the pixel's "
+                     "samples); path segments also count the camera ray of every sample",
         "sum_of_single_scene_ms": round(single_ms, 2),
         "batch_speedup_over_single_scene_renders": round(single_ms / ms, 3),
         "all_scenes_equal_to_single_renders": same,
@@ -1027,7 +1039,9 @@ def pt_workload(P, args):
     img, _ = ctx.render_pt(sid, cam, w, h, args.pt_spp, depth, rr, 7, bg, 1e-9, want_rgb=False)
     plain_ms = (time.perf_counter() - t0) * 1e3
     rays = ctx.last_render_rays()
-    log(f"pt: plain {args.pt_spp} spp {plain_ms:.1f} ms, {rays} rays, {rays / plain_ms / 1e3:.1f} Mrays/s")
+    segs = rays + (args.pt_spp - 1) * w * h  # every sample's camera ray counted
+    log(f"pt: plain {args.pt_spp} spp {plain_ms:.1f} ms, {rays} rays traced ({rays / plain_ms / 1e3:.1f} Mrays/s), "
+        f"{segs} path segments ({segs / plain_ms / 1e3:.1f} M/s)")
     c = P.Cache(ctx, seed=3, learning_rate=1e-2, max_batch=1 << 16)
     t0 = time.perf_counter()
     _, _, st = c.render_pt(sid, cam, w, h, args.pt_frames, depth, rr, 7, bg, 1e-9, train_paths=1 << 16,
@@ -1043,7 +1057,8 @@ def pt_workload(P, args):
                                              f"roulette from bounce {rr}"},
                       "graphs": not args.no_graphs, "path_budget": args.path_budget, "tuning": tun,
                       "plain": {"spp": args.pt_spp, "ms": round(plain_ms, 2), "rays": rays,
-                                "Mrays_per_s": round(rays / plain_ms / 1e3, 2),
+                                "Mrays_per_s": round(rays / plain_ms / 1e3, 2), "path_segments": segs,
+                                "Msegments_per_s": round(segs / plain_ms / 1e3, 2),
                                 "checksum": float(img.astype(np.float64).sum())},
                       "cache": {"frames": args.pt_frames, "samples_per_frame": args.cache_spf,
                                 "ms": round(cache_ms, 2),

@@ -328,10 +328,12 @@ PRAY_API int pray_render_pt_batch(pray_ctx *ctx, uint32_t n_scenes, const int *s
                                   uint32_t rr_depth, const uint64_t *seeds, const float *backgrounds, float det_eps,
                                   float *const *hdr, uint8_t *const *rgb, float *gpu_ms);
 /* Rays traced (all bounces, all samples) by the last pray_render_pt* or
- * pray_render_pt_batch call of the context. */
+ * pray_render_pt_batch call of the context.  Every sample of a pixel starts
+ * with the pixel's camera ray, which a call traces once and whose hit its
+ * samples share: the camera rays count once per call. */
 PRAY_API int pray_last_render_rays(pray_ctx *ctx, uint64_t *rays);
-/* Per bounce b (0 = camera rays), the paths traced at that bounce summed
- * over all samples of the last pray_render_pt / _ex / _progressive call:
+/* Per bounce b (0 = camera rays), the paths at that bounce summed over all
+ * samples of the last pray_render_pt / _ex / _progressive call:
  * the sizes of the wavefront queues.  Fills min(n, *n_bounces) counts;
  * *n_bounces = max_depth + 1 (0 after other calls). */
 PRAY_API int pray_last_render_queues(pray_ctx *ctx, uint64_t *counts, uint32_t n, uint32_t *n_bounces);

@@ -249,11 +249,11 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
     Tuning tu = bucket_scene[main_bucket]->tune;
     tu.layout = PRAY_LAYOUT_PACKED;
 
-    // Buffers from the context's render workspace (slots 40..56), reused by
+    // Buffers from the context's render workspace (slots 40..58), reused by
     // later calls.
     struct Buf {
         void *p = nullptr;
-    } dtable, dbs, prim, rq[2], sq[2], qs[2], cnt, hit, tt, contrib, sum, outh, outb, rays_acc;
+    } dtable, dbs, prim, rq[2], sq[2], qs[2], cnt, hit, tt, contrib, sum, outh, outb, rays_acc, phit, pt;
     WS_GET(c, 40, dtable.p, n_scenes * sizeof(prt::DevScene));
     WS_GET(c, 41, dbs.p, n_scenes * sizeof(prr::BatchScene));
     WS_GET(c, 42, prim.p, total * 24);
@@ -270,6 +270,8 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
     WS_GET(c, 54, outh.p, total * 12);
     WS_GET(c, 55, rays_acc.p, 8);
     if (rgb) WS_GET(c, 56, outb.p, total * 3);
+    WS_GET(c, 57, phit.p, total * 4);  // the camera rays' hits, shared by all samples
+    WS_GET(c, 58, pt.p, total * 4);
     hipStream_t st = c->stream;
     HIPC(c, hipMemcpyAsync(dtable.p, table.data(), n_scenes * sizeof(prt::DevScene), hipMemcpyHostToDevice, st));
     HIPC(c, hipMemcpyAsync(dbs.p, bs.data(), n_scenes * sizeof(prr::BatchScene), hipMemcpyHostToDevice, st));
@@ -294,32 +296,43 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
     const prt::DevScene *dtab = (const prt::DevScene *)dtable.p;
     int rc = prepare_trace_multi(c, tu, tree_depth, total);
     if (rc) return rc;
+    // Every sample of a pixel starts with the pixel's camera ray: its hit is
+    // traced once for the whole call (and counted once among the rays), with
+    // the pixels' scenes from a first generate.
+    hipLaunchKernelGGL(prr::k_pt_generate_batch, dim3(nblocks(total, 256)), dim3(256), 0, st, (uint32_t)total,
+                       (const prr::BatchScene *)dbs.p, n_scenes, (float4 *)sq[0].p, (uint32_t *)qs[0].p, count + 0);
+    hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, (unsigned long long *)rays_acc.p,
+                       (const uint32_t *)(count + 0));
+    rc = enqueue_trace_multi(c, dtab, (const uint32_t *)qs[0].p, tree_depth, tu, (const float *)prim.p, total,
+                             nullptr, det_eps, (int32_t *)phit.p, (float *)pt.p, st);
+    if (rc) return rc;
 
     auto pass = [&]() -> int {
         Range rp("batch pass");
-        hipLaunchKernelGGL(prr::k_pt_generate_batch, dim3(nblocks(total, 256)), dim3(256), 0, st,
-                           (const float *)prim.p, (uint32_t)total, (const prr::BatchScene *)dbs.p, n_scenes,
-                           (float *)rq[0].p, (float4 *)sq[0].p, (uint32_t *)qs[0].p, count + 0);
+        hipLaunchKernelGGL(prr::k_pt_generate_batch, dim3(nblocks(total, 256)), dim3(256), 0, st, (uint32_t)total,
+                           (const prr::BatchScene *)dbs.p, n_scenes, (float4 *)sq[0].p, (uint32_t *)qs[0].p,
+                           count + 0);
         for (uint32_t b = 0; b <= max_depth; b++) {
             const int cur = b & 1, nxt = cur ^ 1;
+            const bool first_bounce = b == 0;  // hits of the camera rays, traced once
             hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + nxt, 1u);
-            hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, (unsigned long long *)rays_acc.p,
-                               (const uint32_t *)(count + cur));
-            int e;
-            {
+            if (!first_bounce) {
+                hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, (unsigned long long *)rays_acc.p,
+                                   (const uint32_t *)(count + cur));
                 Range r("batch extend");
-                e = enqueue_trace_multi(c, dtab, (const uint32_t *)qs[cur].p, tree_depth, tu,
-                                        (const float *)rq[cur].p, total, count + cur, det_eps, (int32_t *)hit.p,
-                                        (float *)tt.p, st);
+                const int e = enqueue_trace_multi(c, dtab, (const uint32_t *)qs[cur].p, tree_depth, tu,
+                                                  (const float *)rq[cur].p, total, count + cur, det_eps,
+                                                  (int32_t *)hit.p, (float *)tt.p, st);
+                if (e) return e;
             }
-            if (e) return e;
             Range rs("batch shade");
             prr::BatchArgs B{dtab, (const prr::BatchScene *)dbs.p, (const uint32_t *)qs[cur].p, (uint32_t *)qs[nxt].p};
             hipLaunchKernelGGL((prr::k_pt_shade<prr::PT_PLAIN, true>), dim3(nblocks(total, 256)), dim3(256), 0, st,
-                               prt::DevScene{}, P, prr::CacheArgs{}, b, (const float *)rq[cur].p,
-                               (const float4 *)sq[cur].p, (const uint32_t *)(count + cur), (const int32_t *)hit.p,
-                               (const float *)tt.p, (float *)rq[nxt].p, (float4 *)sq[nxt].p, count + nxt,
-                               (float *)contrib.p, B);
+                               prt::DevScene{}, P, prr::CacheArgs{}, b, first_bounce ? 1 : 0,
+                               (const float *)(first_bounce ? prim.p : rq[cur].p), (const float4 *)sq[cur].p,
+                               (const uint32_t *)(count + cur), (const int32_t *)(first_bounce ? phit.p : hit.p),
+                               (const float *)(first_bounce ? pt.p : tt.p), (float *)rq[nxt].p, (float4 *)sq[nxt].p,
+                               count + nxt, (float *)contrib.p, B);
         }
         hipLaunchKernelGGL(prr::k_pt_accumulate, dim3(nblocks(3 * total, 256)), dim3(256), 0, st, (uint32_t)total, 1u,
                            (const float *)contrib.p, (float *)sum.p, count + 2);

@@ -378,11 +378,12 @@ int pray_render_pt_cache_ex(pray_ctx *c, int scene_id, int cache_id, const float
     const uint32_t stride = (uint32_t)std::max<uint64_t>(1, npix / n_train);
     prr::Cam cam;
     std::memcpy(&cam, cam16, sizeof cam);
-    // Buffers from the context's render workspace (slots 16..34), reused by
+    // Buffers from the context's render workspace (slots 16..36), reused by
     // later calls.
     struct Buf {
         void *p = nullptr;
-    } prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, queries, hist, trays, tstate, thit, ttt, tq;
+    } prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, queries, hist, trays, tstate, thit, ttt, tq, phit,
+        pt;
     const uint64_t npath = npix * spf;  // paths of a frame: spf samples of every pixel
     WS_GET(c, 16, prim.p, npix * 24);
     for (int q = 0; q < 2; q++) {
@@ -405,6 +406,8 @@ int pray_render_pt_cache_ex(pray_ctx *c, int scene_id, int cache_id, const float
     WS_GET(c, 32, ttt.p, (size_t)n_train * 4);
     WS_GET(c, 33, tq.p, (size_t)n_train * 64);
     if (rgb) WS_GET(c, 34, outb.p, npix * 3);
+    WS_GET(c, 35, phit.p, npix * 4);  // the camera rays' hits, shared by all samples
+    WS_GET(c, 36, pt.p, npix * 4);
     uint32_t *count = (uint32_t *)cnt.p;
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(prr::k_camera, dim3(nblocks(npix, 256)), dim3(256), 0, st, cam, w, h, (float *)prim.p);
@@ -438,7 +441,12 @@ int pray_render_pt_cache_ex(pray_ctx *c, int scene_id, int cache_id, const float
         C.lo[k] = lo;
         C.inv[k] = ext > 0.f ? 1.f / ext : 0.f;
     }
-    int rc = prepare_trace(c, *sc, std::max<uint64_t>(npath, n_train));
+    // Every sample of a pixel starts with the pixel's camera ray: its hit is
+    // traced once for all frames.
+    int rc = enqueue_trace(c, *sc, (const float *)prim.p, npix, nullptr, PRAY_TRACE_CLOSEST, det_eps, nullptr,
+                           (int32_t *)phit.p, (float *)pt.p, nullptr, st);
+    if (rc) return rc;
+    rc = prepare_trace(c, *sc, std::max<uint64_t>(npath, n_train));
     if (rc) return rc;
     if (!c->side) HIPC(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     if (!c->fork_ev) HIPC(c, hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
@@ -456,24 +464,26 @@ int pray_render_pt_cache_ex(pray_ctx *c, int scene_id, int cache_id, const float
         hipLaunchKernelGGL(prr::k_fill_f32, dim3(nblocks(4 * (uint64_t)n_train, 256)), dim3(256), 0, st,
                            (float *)r->targets, 4u * n_train, 0.f);
         hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, r->ctr, 1u);
-        hipLaunchKernelGGL(prr::k_pt_generate, dim3(nblocks(npath, 256)), dim3(256), 0, st, (const float *)prim.p,
-                           (uint32_t)npix, (uint32_t)npath, (float *)rq[0].p, (float4 *)sq[0].p, count + 0);
+        hipLaunchKernelGGL(prr::k_pt_generate, dim3(nblocks(npath, 256)), dim3(256), 0, st, (uint32_t)npath,
+                           (float4 *)sq[0].p, count + 0);
         hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + 3, 2u);
         for (uint32_t b = 0; b <= max_depth; b++) {
             const int cur = b & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + nxt, 1u);
-            int e;
-            {
+            const bool first_bounce = b == 0;  // hits of the camera rays, traced once
+            if (!first_bounce) {
                 Range r("cache extend");
-                e = enqueue_trace(c, *sc, (const float *)rq[cur].p, npath, count + cur, PRAY_TRACE_CLOSEST, det_eps,
-                                  nullptr, (int32_t *)hit.p, (float *)tt.p, nullptr, st);
+                const int e = enqueue_trace(c, *sc, (const float *)rq[cur].p, npath, count + cur, PRAY_TRACE_CLOSEST,
+                                            det_eps, nullptr, (int32_t *)hit.p, (float *)tt.p, nullptr, st);
+                if (e) return e;
             }
-            if (e) return e;
             Range r("cache shade");
             hipLaunchKernelGGL(prr::k_pt_shade<prr::PT_CACHE>, dim3(nblocks(npath, 256)), dim3(256), 0, st, sc->d, P,
-                               C, b, (const float *)rq[cur].p, (const float4 *)sq[cur].p,
-                               (const uint32_t *)(count + cur), (const int32_t *)hit.p, (const float *)tt.p,
-                               (float *)rq[nxt].p, (float4 *)sq[nxt].p, count + nxt, (float *)contrib.p, prr::BatchArgs{});
+                               C, b, first_bounce ? 1 : 0, (const float *)(first_bounce ? prim.p : rq[cur].p),
+                               (const float4 *)sq[cur].p, (const uint32_t *)(count + cur),
+                               (const int32_t *)(first_bounce ? phit.p : hit.p),
+                               (const float *)(first_bounce ? pt.p : tt.p), (float *)rq[nxt].p, (float4 *)sq[nxt].p,
+                               count + nxt, (float *)contrib.p, prr::BatchArgs{});
         }
         // The training rays (few, latency bound) and their targets run on the
         // side stream while the pixels' inference runs on the main one.

@@ -1201,12 +1201,14 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
     }
     const uint32_t S = (uint32_t)std::min<uint64_t>(spp, per_pass);
     const uint64_t npath = npix * S;
-    // Buffers from the context's render workspace (slots 0..12), reused by
+    // Buffers from the context's render workspace (slots 0..14), reused by
     // later calls.
     struct Buf {
         void *p = nullptr;
-    } prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, qsum;
+    } prim, rq[2], sq[2], cnt, hit, tt, contrib, sum, outh, outb, qsum, phit, pt;
     WS_GET(c, 0, prim.p, npix * 24);
+    WS_GET(c, 13, phit.p, npix * 4);  // the camera rays' hits, shared by all samples
+    WS_GET(c, 14, pt.p, npix * 4);
     for (int q = 0; q < 2; q++) {
         WS_GET(c, 1 + q, rq[q].p, npath * 24);
         WS_GET(c, 3 + q, sq[q].p, npath * 16);
@@ -1239,41 +1241,61 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
     P.k0 = (uint32_t)seed, P.k1 = (uint32_t)(seed >> 32);
     P.bg[0] = bg[0], P.bg[1] = bg[1], P.bg[2] = bg[2];
     P.sample = count + 2;
+    // Every sample of a pixel starts with the pixel's camera ray: its hit is
+    // traced once for the whole call (and counted once among the rays).
+    {
+        Range r("pt camera hits");
+        if (sc->d.n_tris) {
+            int rc = enqueue_trace(c, *sc, (const float *)prim.p, npix, nullptr, PRAY_TRACE_CLOSEST, det_eps, nullptr,
+                                   (int32_t *)phit.p, (float *)pt.p, nullptr, st);
+            if (rc) return rc;
+        } else {
+            hipLaunchKernelGGL(k_fill_miss, dim3(nblocks(npix, 256)), dim3(256), 0, st, (uint32_t)npix,
+                               (int32_t *)phit.p, (float *)pt.p, (uint32_t *)nullptr);
+        }
+        hipLaunchKernelGGL(prt::k_set_u32, dim3(1), dim3(64), 0, st, count + 3, (uint32_t)npix);
+        hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, traced, (const uint32_t *)(count + 3));
+    }
     // The persistent trace kernel needs its spill area before any capture.
     {
         int rc = prepare_trace(c, *sc, npath);
         if (rc) return rc;
     }
     // One pass of `ns` samples: generate, (zero next count, extend, shade)
-    // per bounce, accumulate.
+    // per bounce (the first bounce's hits are the camera rays'), accumulate.
     auto pass = [&](uint32_t ns) -> int {
         const uint64_t np = npix * ns;
         P.samples = ns;
         Range rp("pt pass");
         {
             Range rg("pt generate");
-            hipLaunchKernelGGL(prr::k_pt_generate, dim3(nblocks(np, 256)), dim3(256), 0, st, (const float *)prim.p,
-                               (uint32_t)npix, (uint32_t)np, (float *)rq[0].p, (float4 *)sq[0].p, count + 0);
+            hipLaunchKernelGGL(prr::k_pt_generate, dim3(nblocks(np, 256)), dim3(256), 0, st, (uint32_t)np,
+                               (float4 *)sq[0].p, count + 0);
         }
         for (uint32_t b = 0; b <= max_depth; b++) {
             const int cur = b & 1, nxt = cur ^ 1;
             hipLaunchKernelGGL(prt::k_zero_u32, dim3(1), dim3(64), 0, st, count + nxt, 1u);
-            hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, traced, (const uint32_t *)(count + cur));
             hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, (unsigned long long *)qsum.p + b,
                                (const uint32_t *)(count + cur));
-            if (sc->d.n_tris) {
-                Range r("pt extend");
-                int rc = enqueue_trace(c, *sc, (const float *)rq[cur].p, np, count + cur, PRAY_TRACE_CLOSEST,
-                                       det_eps, nullptr, (int32_t *)hit.p, (float *)tt.p, nullptr, st);
-                if (rc) return rc;
-            } else {
-                hipLaunchKernelGGL(k_fill_miss, dim3(nblocks(np, 256)), dim3(256), 0, st, (uint32_t)np,
-                                   (int32_t *)hit.p, (float *)tt.p, (uint32_t *)nullptr);
+            if (b > 0) {
+                hipLaunchKernelGGL(prr::k_acc_rays, dim3(1), dim3(64), 0, st, traced, (const uint32_t *)(count + cur));
+                if (sc->d.n_tris) {
+                    Range r("pt extend");
+                    int rc = enqueue_trace(c, *sc, (const float *)rq[cur].p, np, count + cur, PRAY_TRACE_CLOSEST,
+                                           det_eps, nullptr, (int32_t *)hit.p, (float *)tt.p, nullptr, st);
+                    if (rc) return rc;
+                } else {
+                    hipLaunchKernelGGL(k_fill_miss, dim3(nblocks(np, 256)), dim3(256), 0, st, (uint32_t)np,
+                                       (int32_t *)hit.p, (float *)tt.p, (uint32_t *)nullptr);
+                }
             }
             Range r("pt shade");
+            const bool first_bounce = b == 0;
             hipLaunchKernelGGL(prr::k_pt_shade<prr::PT_PLAIN>, dim3(nblocks(np, 256)), dim3(256), 0, st, sc->d, P,
-                               prr::CacheArgs{}, b, (const float *)rq[cur].p, (const float4 *)sq[cur].p, (const uint32_t *)(count + cur),
-                               (const int32_t *)hit.p, (const float *)tt.p, (float *)rq[nxt].p, (float4 *)sq[nxt].p,
+                               prr::CacheArgs{}, b, first_bounce ? 1 : 0,
+                               (const float *)(first_bounce ? prim.p : rq[cur].p), (const float4 *)sq[cur].p,
+                               (const uint32_t *)(count + cur), (const int32_t *)(first_bounce ? phit.p : hit.p),
+                               (const float *)(first_bounce ? pt.p : tt.p), (float *)rq[nxt].p, (float4 *)sq[nxt].p,
                                count + nxt, (float *)contrib.p, prr::BatchArgs{});
         }
         Range ra("pt accumulate");

@@ -248,15 +248,15 @@ struct PtArgs {
 };
 
 // Queue entries: ray (6 floats) + state float4 {throughput rgb, path bits}.
-// Paths p < n_paths: the camera ray of pixel p % n_pix.
-static __global__ void k_pt_generate(const float *prim, uint32_t n_pix, uint32_t n_paths, float *rays, float4 *state,
-                                     uint32_t *count) {
+// The first bounce's queue holds only states, paths p < n_paths: path p
+// starts with the camera ray of pixel p % n_pix, whose hit the renderer
+// traces once per call for all samples (k_pt_shade's `prim`), since every
+// sample of a pixel starts with the same ray (the reference's primRays,
+// pray.cpp:368-372).
+static __global__ void k_pt_generate(uint32_t n_paths, float4 *state, uint32_t *count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *count = n_paths;
     if (i >= n_paths) return;
-    const float *r = prim + 6 * (size_t)(i % n_pix);
-    float *o = rays + 6 * (size_t)i;
-    o[0] = r[0], o[1] = r[1], o[2] = r[2], o[3] = r[3], o[4] = r[4], o[5] = r[5];
     state[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(i));
 }
 
@@ -361,8 +361,11 @@ struct BatchArgs {
     uint32_t *scene_out;
 };
 
+// prim: the first bounce, whose ray, hit and distance are the camera ray's,
+// read per pixel from rays_in, hit and tt (pixel-indexed: the scene-local
+// pixel, or the batch's global one) instead of per queue entry.
 template <int MODE, bool MULTI = false>
-static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, CacheArgs C, uint32_t bounce,
+static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, CacheArgs C, uint32_t bounce, int prim,
                                                          const float *rays_in, const float4 *state_in,
                                                          const uint32_t *count_in, const int32_t *hit,
                                                          const float *tt, float *rays_out, float4 *state_out,
@@ -395,7 +398,8 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
             bg0 = Q.bg[0], bg1 = Q.bg[1], bg2 = Q.bg[2];
             k0 = Q.k0, k1 = Q.k1, lpix = pix - Q.pix_off;
         }
-        const int32_t h = hit[i];
+        const uint32_t at = prim ? (MULTI ? pix : lpix) : i;  // entry of rays_in, hit, tt
+        const int32_t h = hit[at];
         float cr = 0.f, cg = 0.f, cb = 0.f;
         bool term = true;
         if (h < 0) {
@@ -403,8 +407,8 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
         } else if (emit[h]) {
             cr = st.x * color[3 * (size_t)h], cg = st.y * color[3 * (size_t)h + 1], cb = st.z * color[3 * (size_t)h + 2];
         } else if (bounce < P.max_depth) {
-            const float *r = rays_in + 6 * (size_t)i;
-            const float t = tt[i];
+            const float *r = rays_in + 6 * (size_t)at;
+            const float t = tt[at];
             ox = r[0] + r[3] * t, oy = r[1] + r[4] * t, oz = r[2] + r[5] * t;
             const float ar = color[3 * (size_t)h], ag = color[3 * (size_t)h + 1], ab = color[3 * (size_t)h + 2];
             const float nx = normal[3 * (size_t)h], ny = normal[3 * (size_t)h + 1], nz = normal[3 * (size_t)h + 2];
@@ -473,11 +477,11 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
     }
 }
 
-// Camera rays of all scenes of a batch, from their concatenated primary rays:
-// entry g belongs to the scene whose pixel range holds g.
-static __global__ void k_pt_generate_batch(const float *prim, uint32_t n_total, const BatchScene *scenes,
-                                           uint32_t n_scenes, float *rays, float4 *state, uint32_t *scene,
-                                           uint32_t *count) {
+// First-bounce states of all scenes of a batch, one per pixel of their
+// concatenated images: entry g (camera ray g of the concatenated primary
+// rays) belongs to the scene whose pixel range holds g.
+static __global__ void k_pt_generate_batch(uint32_t n_total, const BatchScene *scenes, uint32_t n_scenes,
+                                           float4 *state, uint32_t *scene, uint32_t *count) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g == 0) *count = n_total;
     if (g >= n_total) return;
@@ -489,9 +493,6 @@ static __global__ void k_pt_generate_batch(const float *prim, uint32_t n_total, 
         else
             hi = mid - 1;
     }
-    const float *r = prim + 6 * (size_t)g;
-    float *o = rays + 6 * (size_t)g;
-    o[0] = r[0], o[1] = r[1], o[2] = r[2], o[3] = r[3], o[4] = r[4], o[5] = r[5];
     state[g] = make_float4(1.f, 1.f, 1.f, __uint_as_float(g));
     scene[g] = lo;
 }

@@ -157,7 +157,9 @@ def test_cornell100k_full_size_split_invariance(ctx):
         ctx.set_path_budget(1 << 24)
         ctx.free_scene(sid)
     hdr0, rays0, q0 = out[0]
-    assert q0[0] == w * h * spp and rays0 == int(q0.sum()) and hdr0.mean() > 0.05
+    # rays traced: the paths of every bounce after the first, and the camera
+    # rays once (every sample of a pixel starts with its camera ray)
+    assert q0[0] == w * h * spp and rays0 == int(q0.sum()) - int(q0[0]) + w * h and hdr0.mean() > 0.05
     for hdr, rays, q in out[1:]:
         assert np.array_equal(hdr.view(np.uint32), hdr0.view(np.uint32))
         assert rays == rays0 and np.array_equal(q, q0)
@@ -210,7 +212,7 @@ def test_batch_size_buckets_equal_oracle(ctx):
             f = gpu_flat(ctx, sids[k], s)
             oh, _, oq = O.render_pt(f, cams[k], sizes[k][0], sizes[k][1], spp, depth, rr, seeds[k], bgs[k], 1e-9,
                                     queues=True)
-            total += int(oq.sum())
+            total += int(oq.sum()) - int(oq[0]) + sizes[k][0] * sizes[k][1]  # camera rays traced once
             assert oh.mean() > 0
             for imgs, _ in runs:
                 assert first_diff(imgs[k].view(np.uint32).ravel(), oh.view(np.uint32).ravel()) is None, \
