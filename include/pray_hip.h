@@ -279,7 +279,12 @@ PRAY_API int pray_render_rt(pray_ctx *ctx, int scene_id, const float *camera16, 
  * path_trace.cu:494-555, NORMAL mode estimator pray.cpp:342-378):
  * per sample pass, generate -> (extend -> shade)* over HBM ray queues,
  * Philox4x32-10 keyed by (seed, pixel, sample, bounce); Russian roulette
- * from bounce `rr_depth` on (rr_depth >= max_depth disables it).
+ * from bounce `rr_depth` on (rr_depth >= max_depth disables it).  At
+ * max_depth <= 1 the reference's DIFFUSE estimator instead (pray.cpp:315-340,
+ * chosen there by scene_preprocessor.cpp:34-40, also for pray_render_pt_batch):
+ * a camera-ray miss gives the background and an emissive hit its emission,
+ * exactly; otherwise the samples' arriving radiance is summed and multiplied
+ * by the hit's albedo once, (sum * albedo) / samples; no roulette.
  *   hdr   width*height*3 floats: mean radiance before clamping, may be NULL
  *   rgb   width*height*3 bytes: clamp(mean)*255+0.5 truncated, may be NULL */
 PRAY_API int pray_render_pt(pray_ctx *ctx, int scene_id, const float *camera16, uint32_t width, uint32_t height,

@@ -293,6 +293,7 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
     P.max_depth = max_depth;
     P.rr_depth = rr_depth;
     P.sample = count + 2;
+    P.diffuse = max_depth <= 1 ? 1u : 0u;  // the reference's estimator choice (scene_preprocessor.cpp:34-40)
     const prt::DevScene *dtab = (const prt::DevScene *)dtable.p;
     int rc = prepare_trace_multi(c, tu, tree_depth, total);
     if (rc) return rc;
@@ -357,7 +358,11 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
             (void)hipGraphDestroy(graph);
             return fail(c, PRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
         }
-        e = hipEventRecord(c->ievents[2], st);
+        // The work enqueued before the capture (camera rays and their hits)
+        // completes before the first replay: the replays are not left to
+        // order themselves after it.
+        e = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = hipEventRecord(c->ievents[2], st);
         for (uint32_t s = 0; s < spp && e == hipSuccess; s++) e = hipGraphLaunch(exec, st);
         if (e == hipSuccess) e = hipEventRecord(c->ievents[3], st);
         const hipError_t e2 = hipStreamSynchronize(st);
@@ -378,8 +383,19 @@ int pray_render_pt_batch(pray_ctx *c, uint32_t n_scenes, const int *scene_ids, c
     if (gpu_ms) HIPC(c, hipEventElapsedTime(gpu_ms, c->ievents[2], c->ievents[3]));
     HIPC(c, hipMemcpy(&c->last_rays, rays_acc.p, 8, hipMemcpyDeviceToHost));
     c->last_queues.clear();
-    hipLaunchKernelGGL(prr::k_pt_finalize, dim3(nblocks(3 * total, 256)), dim3(256), 0, st, (uint32_t)total,
-                       (const float *)sum.p, (float)spp, (float *)outh.p, (uint8_t *)outb.p);
+    if (P.diffuse) {
+        for (uint32_t k = 0; k < n_scenes; k++) {
+            const size_t off = bs[k].pix_off, np = bs[k].n_pix;
+            hipLaunchKernelGGL(prr::k_pt_finalize_diffuse, dim3(nblocks(3 * np, 256)), dim3(256), 0, st,
+                               (uint32_t)np, (const float *)sum.p + 3 * off, (float)spp,
+                               (const int32_t *)phit.p + off, table[k].emit, table[k].color, bs[k].bg[0],
+                               bs[k].bg[1], bs[k].bg[2], max_depth, (float *)outh.p + 3 * off,
+                               outb.p ? (uint8_t *)outb.p + 3 * off : nullptr);
+        }
+    } else {
+        hipLaunchKernelGGL(prr::k_pt_finalize, dim3(nblocks(3 * total, 256)), dim3(256), 0, st, (uint32_t)total,
+                           (const float *)sum.p, (float)spp, (float *)outh.p, (uint8_t *)outb.p);
+    }
     HIPC(c, hipGetLastError());
     for (uint32_t k = 0; k < n_scenes; k++) {
         const size_t off = bs[k].pix_off, np = bs[k].n_pix;

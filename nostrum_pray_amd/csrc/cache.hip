@@ -541,6 +541,10 @@ int pray_render_pt_cache_ex(pray_ctx *c, int scene_id, int cache_id, const float
             (void)hipGraphDestroy(graph);
             return fail(c, PRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
         }
+        // The work enqueued before the capture (camera rays and their hits)
+        // completes before the first replay: the replays, whose frames fork
+        // onto a second stream, are not left to order themselves after it.
+        e = hipStreamSynchronize(st);
         for (uint32_t f = 0; f < frames && e == hipSuccess; f++) e = hipGraphLaunch(exec, st);
         const hipError_t e2 = hipStreamSynchronize(st);
         (void)hipGraphExecDestroy(exec);

@@ -1241,6 +1241,9 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
     P.k0 = (uint32_t)seed, P.k1 = (uint32_t)(seed >> 32);
     P.bg[0] = bg[0], P.bg[1] = bg[1], P.bg[2] = bg[2];
     P.sample = count + 2;
+    // the reference's estimator choice: diffuse at a depth limit of at most 1
+    // (scene_preprocessor.cpp:34-40)
+    P.diffuse = max_depth <= 1 ? 1u : 0u;
     // Every sample of a pixel starts with the pixel's camera ray: its hit is
     // traced once for the whole call (and counted once among the rays).
     {
@@ -1321,6 +1324,10 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
             (void)hipGraphDestroy(graph);
             return fail(c, PRAY_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
         }
+        // The work enqueued before the capture (camera rays and their hits)
+        // completes before the first replay: the replays are not left to
+        // order themselves after it.
+        e = hipStreamSynchronize(st);
         for (uint32_t s = 0; s < full && e == hipSuccess; s++) e = hipGraphLaunch(exec, st);
         const hipError_t e2 = hipStreamSynchronize(st);
         (void)hipGraphExecDestroy(exec);
@@ -1337,8 +1344,14 @@ int render_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, uint32_
         int rc = pass(rest);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(prr::k_pt_finalize, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
-                       (const float *)sum.p, (float)(first + spp), (float *)outh.p, (uint8_t *)outb.p);
+    if (P.diffuse)
+        hipLaunchKernelGGL(prr::k_pt_finalize_diffuse, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st,
+                           (uint32_t)npix, (const float *)sum.p, (float)(first + spp), (const int32_t *)phit.p,
+                           sc->d.emit, sc->d.color, bg[0], bg[1], bg[2], max_depth, (float *)outh.p,
+                           (uint8_t *)outb.p);
+    else
+        hipLaunchKernelGGL(prr::k_pt_finalize, dim3(nblocks(3 * npix, 256)), dim3(256), 0, st, (uint32_t)npix,
+                           (const float *)sum.p, (float)(first + spp), (float *)outh.p, (uint8_t *)outb.p);
     HIPC(c, hipGetLastError());
     if (hdr) HIPC(c, hipMemcpyAsync(hdr, outh.p, npix * 12, hipMemcpyDeviceToHost, st));
     if (rgb) HIPC(c, hipMemcpyAsync(rgb, outb.p, npix * 3, hipMemcpyDeviceToHost, st));

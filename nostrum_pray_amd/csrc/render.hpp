@@ -245,6 +245,12 @@ struct PtArgs {
     // tracer) is sample *sample + p / n_pix of pixel p % n_pix (1 for the
     // batch tracer, whose paths are pixels).
     uint32_t samples = 1;
+    // The reference's diffuse estimator (pray.cpp:315-340, chosen at
+    // max_depth <= 1, scene_preprocessor.cpp:34-40): the first bounce keeps
+    // throughput 1 and no roulette, so the samples' sum is the sum of the
+    // radiance arriving at the primary hit, which k_pt_finalize_diffuse
+    // multiplies by the hit's albedo once.
+    uint32_t diffuse = 0;
 };
 
 // Queue entries: ray (6 floats) + state float4 {throughput rgb, path bits}.
@@ -431,9 +437,12 @@ static __global__ __launch_bounds__(256) void k_pt_shade(DevScene s, PtArgs P, C
             } else {
                 const uint4 u = prd::philox4x32_10(make_uint4(lpix, smp, bounce, 0u), k0, k1);
                 prd::sample_hemisphere(nx, ny, nz, prd::u01(u.x), prd::u01(u.y), dx, dy, dz);
-                tr = st.x * ar, tg = st.y * ag, tb = st.z * ab;
+                if (P.diffuse)
+                    tr = st.x, tg = st.y, tb = st.z;
+                else
+                    tr = st.x * ar, tg = st.y * ag, tb = st.z * ab;
                 bool live = true;
-                if (bounce >= P.rr_depth) {
+                if (!P.diffuse && bounce >= P.rr_depth) {
                     const float q = fmaxf(fmaxf(ar, ag), ab);
                     if (q < 1.f) {
                         if (!(prd::u01(u.z) < q))
@@ -561,6 +570,32 @@ static __global__ void k_pt_finalize(uint32_t n_pix, const float *sum, float spp
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 3 * n_pix) return;
     const float m = sum[i] / spp;
+    if (hdr) hdr[i] = m;
+    if (rgb) rgb[i] = to_byte(clamp_std(m));
+}
+
+// The diffuse estimator's pixels (pray.cpp:315-340 with the depth limit of
+// at most 1 that selects it): the camera ray's miss gives the background and
+// an emissive hit its emission, both once, not averaged; a non-emissive hit
+// gives black at depth limit 0 and otherwise (sum of the samples' arriving
+// radiance * albedo) / samples, multiplied and divided in that order.
+// Pixels [0, n_pix) of one scene; hit is that scene's camera-ray hits.
+static __global__ void k_pt_finalize_diffuse(uint32_t n_pix, const float *sum, float spp, const int32_t *hit,
+                                             const uint8_t *emit, const float *color, float bg0, float bg1, float bg2,
+                                             uint32_t max_depth, float *hdr, uint8_t *rgb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n_pix) return;
+    const uint32_t c = i % 3;
+    const int32_t h = hit[i / 3];
+    float m;
+    if (h < 0)
+        m = c == 0 ? bg0 : (c == 1 ? bg1 : bg2);
+    else if (emit[h])
+        m = color[3 * (size_t)h + c];
+    else if (max_depth == 0)
+        m = 0.f;
+    else
+        m = (sum[i] * color[3 * (size_t)h + c]) / spp;
     if (hdr) hdr[i] = m;
     if (rgb) rgb[i] = to_byte(clamp_std(m));
 }

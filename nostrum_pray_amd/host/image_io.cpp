@@ -105,7 +105,10 @@ uint64_t renderFingerprint(const Scene &s) {
     cameraParams(s, cam);
     f.add(cam, sizeof cam);
     f.val((uint64_t)s.width), f.val((uint64_t)s.height), f.val((uint64_t)s.maxDepth);
-    f.val((uint64_t)s.rrDepth), f.val(s.seed), f.val((uint32_t)s.ptMode);
+    // the estimator as preprocessing chooses it (from the mode and depth),
+    // so that a scene fingerprints the same before and after preprocessing
+    const PtMode pm = (s.mode == PATHTRACING && s.maxDepth > MAX_DEPTH_DIFFUSE) ? NORMAL : DIFFUSE;
+    f.val((uint64_t)s.rrDepth), f.val(s.seed), f.val((uint32_t)pm);
     const float bg[4] = {s.backgroundColor.x, s.backgroundColor.y, s.backgroundColor.z, s.detEps};
     f.add(bg, sizeof bg);
     return f.h;

@@ -81,10 +81,12 @@ void preprocessScene(Scene &scene, bool verbose) {
     auto t1 = clk::now();
     if (verbose) std::cout << "\tInitialising triangles... Done: " << ms(t0, t1) << " ms" << std::endl;
 
-    // Mode selection (scene_preprocessor.cpp:34-40).  The reference leaves
-    // ptMode unset for max_depth <= 1; here it is NORMAL in that case, which
-    // is what an all-zero Scene gives.
-    if (scene.mode == PATHTRACING && scene.maxDepth > MAX_DEPTH_DIFFUSE) scene.ptMode = NORMAL;
+    // Estimator selection (scene_preprocessor.cpp:34-40): NORMAL above a
+    // depth limit of 1, DIFFUSE otherwise.  The reference only assigns NORMAL
+    // and leaves the member uninitialised in the other case, where it prints
+    // that it uses the diffuse estimator; here that case is DIFFUSE, which
+    // the path tracer applies at the same depth limits (pray_render_pt).
+    scene.ptMode = (scene.mode == PATHTRACING && scene.maxDepth > MAX_DEPTH_DIFFUSE) ? NORMAL : DIFFUSE;
 
     const int kind = chooseTree(scene);
     if (kind == 1)

@@ -912,6 +912,14 @@ ORC_API void orc_render_rt(const uint32_t *nodes, const uint32_t *refs, const fl
 // u1, u2 (hemisphere) and u3 (Russian roulette from bounce rr_depth on:
 // survive with q = max albedo component, throughput divided by q).
 // Samples are summed in order; hdr = sum / spp; rgb = clamp -> byte.
+// At max_depth <= 1 the reference takes its DIFFUSE estimator instead
+// (scene_preprocessor.cpp:34-40, pray.cpp:315-340, evaluatePath :366-376):
+// the camera ray is traced once; a miss gives the background and an
+// emissive hit its emission; a non-emissive hit gives black at max_depth 0
+// and otherwise, for each sample in order, the radiance arriving along a
+// hemisphere ray of the same Philox counter (background, emission, or black
+// at the depth limit) is summed, and the pixel is (sum * albedo) / spp.  No
+// roulette.
 ORC_API void orc_render_pt(const uint32_t *nodes, const uint32_t *refs, const float *A, const float *AB,
                            const float *AC, const float *box6, const float *normal, const float *color,
                            const uint8_t *emit, const float *cam16, uint64_t w, uint64_t h, uint32_t spp,
@@ -926,6 +934,59 @@ ORC_API void orc_render_pt(const uint32_t *nodes, const uint32_t *refs, const fl
     orc_camera_rays(cam16, w, h, rays.data());
     const V bg = ld(bg3);
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    auto out = [&](uint64_t p, V m) {
+        if (hdr) hdr[3 * p] = m.x, hdr[3 * p + 1] = m.y, hdr[3 * p + 2] = m.z;
+        if (rgb) {
+            rgb[3 * p] = toByte(clampStd(0.f, 1.f, m.x));
+            rgb[3 * p + 1] = toByte(clampStd(0.f, 1.f, m.y));
+            rgb[3 * p + 2] = toByte(clampStd(0.f, 1.f, m.z));
+        }
+    };
+    if (max_depth <= 1) {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (uint64_t p = 0; p < w * h; p++) {
+            const V o = ld(&rays[6 * p]), d = ld(&rays[6 * p + 3]);
+            if (queues) {
+#pragma omp atomic
+                queues[0] += spp;
+            }
+            float t;
+            const uint32_t tri = closest(S, o, d, t);
+            if (tri == NO_HIT) {
+                out(p, bg);
+                continue;
+            }
+            const V alb = ld(color + 3 * (size_t)tri);
+            if (emit[tri]) {
+                out(p, alb);
+                continue;
+            }
+            if (max_depth == 0) {
+                out(p, {0.f, 0.f, 0.f});
+                continue;
+            }
+            if (queues) {
+#pragma omp atomic
+                queues[1] += spp;
+            }
+            const V pos = add(o, mul(d, t));
+            const V n = ld(normal + 3 * (size_t)tri);
+            V sum = {0.f, 0.f, 0.f};
+            for (uint32_t s = 0; s < spp; s++) {
+                uint32_t c[4] = {(uint32_t)p, s, 0u, 0u};
+                philox(c, k0, k1);
+                const V nd = sampleHemisphere(n, u01(c[0]), u01(c[1]));
+                float t2;
+                const uint32_t tri2 = closest(S, pos, nd, t2);
+                // depth 1 == max_depth: only the background or an emitter add
+                const V L = tri2 == NO_HIT ? bg : (emit[tri2] ? ld(color + 3 * (size_t)tri2) : V{0.f, 0.f, 0.f});
+                sum = add(sum, L);
+            }
+            const float fs = (float)spp;
+            out(p, {(sum.x * alb.x) / fs, (sum.y * alb.y) / fs, (sum.z * alb.z) / fs});
+        }
+        return;
+    }
 #pragma omp parallel for schedule(dynamic, 16)
     for (uint64_t p = 0; p < w * h; p++) {
         V sum = {0.f, 0.f, 0.f};
@@ -968,13 +1029,7 @@ ORC_API void orc_render_pt(const uint32_t *nodes, const uint32_t *refs, const fl
             sum = add(sum, contrib);
         }
         const float fs = (float)spp;
-        const V m = {sum.x / fs, sum.y / fs, sum.z / fs};
-        if (hdr) hdr[3 * p] = m.x, hdr[3 * p + 1] = m.y, hdr[3 * p + 2] = m.z;
-        if (rgb) {
-            rgb[3 * p] = toByte(clampStd(0.f, 1.f, m.x));
-            rgb[3 * p + 1] = toByte(clampStd(0.f, 1.f, m.y));
-            rgb[3 * p + 2] = toByte(clampStd(0.f, 1.f, m.z));
-        }
+        out(p, {sum.x / fs, sum.y / fs, sum.z / fs});
     }
 }
 

@@ -66,6 +66,27 @@ def test_batch_equals_single_scene_renders(ctx):
             ctx.free_scene(sid)
 
 
+@pytest.mark.parametrize("max_depth", [1, 0])
+def test_batch_diffuse_estimator_equal_oracle(ctx, max_depth):
+    """At a depth limit of at most 1 the batch takes the reference's diffuse
+    estimator (pray.cpp:315-340) per scene, like the single-scene renderer:
+    every scene's image equals the oracle's bit for bit."""
+    scenes = scene_set()[:3]
+    sids = [ctx.upload(t) for _, t, _, _, _ in scenes]
+    try:
+        say(f"batch of 3 scenes, diffuse estimator, max_depth {max_depth}")
+        imgs, ms = ctx.render_pt_batch(sids, [c for _, _, c, _, _ in scenes], [wh for _, _, _, wh, _ in scenes],
+                                       4, max_depth, 0, [sd for *_, sd in scenes],
+                                       [s.info()["background"] for s, *_ in scenes], 1e-9)
+    finally:
+        for sid in sids:
+            ctx.free_scene(sid)
+    for k, (s, t, cam, (w, h), seed) in enumerate(scenes):
+        oh, _ = O.render_pt(t.flat(), cam, w, h, 4, max_depth, 0, seed, s.info()["background"], 1e-9)
+        assert np.array_equal(imgs[k].view(np.uint32), oh.view(np.uint32)), (k, first_diff(imgs[k].ravel(),
+                                                                                           oh.ravel()))
+
+
 def test_autotune_never_changes_hits(ctx):
     s = P.Scene.random(21, 40000)
     t = s.build(1)

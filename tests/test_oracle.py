@@ -348,6 +348,36 @@ def test_pt_oracle_closed_forms():
     assert list(q0) == [w * h * 3]
 
 
+def test_pt_oracle_diffuse_estimator_closed_forms():
+    """The DIFFUSE estimator the reference takes at a depth limit of 1
+    (pray.cpp:315-340): in the emissive room with an inner box every box
+    pixel is (Le + Le + Le) * rho / 3 in that order of float operations --
+    which for these values differs from NORMAL's sum of rho * Le / 3 -- and
+    every wall pixel is Le itself; no roulette even from bounce 0; every
+    camera path is counted at bounce 0 and the box pixels' at bounce 1."""
+    import nostrum_pray_amd as P
+    from test_gpu_render import box_room
+    f32 = np.float32
+    le, rho, n = f32(0.9), f32(0.61), 3
+    w = h = 24
+    cam = O.camera([0, 0, 1.8], [0, 0, -1], 90, w, h)
+    abc, col, emit = box_room(le, 0.5, rho)
+    f = P.Scene.from_triangles(abc, col, emit).build(1).flat()
+    hdr, _, q = O.render_pt(f, cam, w, h, n, 1, 0, 1, [0, 0, 0], 1e-9, queues=True)
+    s = f32(0)
+    for _ in range(n):
+        s = f32(s + le)
+    diffuse = f32(f32(s * rho) / f32(n))
+    t = f32(0)
+    for _ in range(n):
+        t = f32(t + f32(rho * le))
+    normal = f32(t / f32(n))
+    assert diffuse != normal  # the check tells the two orders apart
+    box = hdr[..., 0] != le
+    assert box.any() and (hdr[box] == diffuse).all() and (hdr[~box] == le).all()
+    assert list(q) == [w * h * n, n * int(box.sum())]
+
+
 def test_sample_hemisphere_properties():
     rng = np.random.default_rng(1)
     n = rng.normal(size=(50000, 3)).astype(np.float32)
