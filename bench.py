@@ -154,6 +154,7 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene leg of the default run")
     ap.add_argument("--no-rt", action="store_true", help="skip the ray-tracer leg (spheres scene) of the default run")
+    ap.add_argument("--rt-tuning", default="", help="traversal tuning of the ray-tracer leg (as --tuning)")
     ap.add_argument("--path-budget", type=int, default=-1,
                     help="paths the plain path tracer keeps in flight (0: one sample per pass; default: the "
                          "library's, 2^24)")
@@ -917,6 +918,8 @@ def ray_tracer_leg(P, ctx, args):
     scene = P.Scene.spheres(1)
     tree = scene.build(0)
     sid = ctx.upload(tree)
+    if args.rt_tuning:
+        ctx.set_tuning(sid, **parse_tuning(P, args.rt_tuning))
     tun = ctx.get_tuning(sid)
     inf = scene.info()
     w, h = inf["width"], inf["height"]

@@ -89,7 +89,8 @@ PRAY_API int pray_mem_ok(pray_ctx *ctx, uint64_t n_tris, uint64_t n_refs, uint64
  *   max_depth  root-to-leaf depth of the tree (<= 64)
  * n_tris == 0 is a valid empty scene (every ray misses).  For a tree with a
  * children pair whose two clips differ (a median tree) the upload also makes
- * the boxed layout of pray_boxed_layout and selects it (node_levels 3). */
+ * the boxed layouts of pray_boxed_layout and selects the 4-wide one
+ * (node_levels 4). */
 PRAY_API int pray_upload_scene(pray_ctx *ctx, const uint32_t *nodes, uint32_t n_nodes, const uint32_t *leaf_refs,
                       uint32_t n_refs, const float *tri_a, const float *tri_ab, const float *tri_ac,
                       const float *tri_normal, const float *tri_color, const uint8_t *tri_emit, uint32_t n_tris,
@@ -103,22 +104,31 @@ PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
  * boxes, every leaf of more than 4 references is split further (triangles
  * sorted by centroid along the longest axis of their centroids' bounds, ties
  * by index, halved until no leaf holds more), and the leaves' runs go to a
- * reference array of their own.  A record is 16 uint32: per axis x, y, z
- * {child 0 lower, child 1 lower, child 0 upper, child 1 upper} (float bits;
- * a box is the union of its triangles' boxes, vertices a, a + ab, a + ac
- * summed in double and rounded outward; {+inf, -inf} when empty), then the
- * two children's words (inner: pair index << 3; leaf: first entry of its run
- * in brefs << 2 | 3; 0xFFFFFFFF an empty leaf) and two zero words.  Pair 0's
- * child 0 is empty and its child 1 is the root.  *n_pairs, *n_brefs and
- * *depth (root-to-leaf edges) receive the sizes; rec (16 * n_pairs words) and
- * brefs (n_brefs words, the last entry of each run | 0x80000000) are filled
- * when not NULL.  PRAY_ERR_UNSUPPORTED for a tree whose children pairs all
- * have equal clips (it keeps its planes), PRAY_ERR_INVALID for a malformed
- * one. */
+ * reference array of their own (brefs: n_brefs words, the last entry of each
+ * run | 0x80000000).  A box is the union of its triangles' boxes, vertices a,
+ * a + ab, a + ac summed in double and rounded outward; {+inf, -inf} when
+ * empty.  A leaf word is (first entry of its run in brefs) << 2 | 3;
+ * 0xFFFFFFFF is an empty leaf; an inner word is (record index) << 3.
+ *   width 2  (node_levels 3) records of 16 uint32: per axis x, y, z {child 0
+ *            lower, child 1 lower, child 0 upper, child 1 upper} (float
+ *            bits), the two children's words, two zero words.  Record 0's
+ *            child 0 is empty and its child 1 is the root (*root receives its
+ *            word); *depth receives the root-to-leaf edges.
+ *   width 4  (node_levels 4) records of 32 uint32, each a binary node
+ *            collapsed with its children: up to four slots (a leaf child, or
+ *            the two children of an inner child), {lower x of slots 0..3},
+ *            {upper x}, {lower y}, {upper y}, {lower z}, {upper z}, the
+ *            slots' words, four zero words; unused slots are empty.  *root
+ *            receives the root word and *depth the stack entries a lane may
+ *            need (three per record level).
+ * *n_records receives the record count; rec (16 or 32 * n_records words) and
+ * brefs are filled when not NULL.  PRAY_ERR_UNSUPPORTED for a tree whose
+ * children pairs all have equal clips (it keeps its planes) or whose layout
+ * would be too deep, PRAY_ERR_INVALID for a malformed one. */
 PRAY_API int pray_boxed_layout(const uint32_t *nodes, uint32_t n_nodes, const uint32_t *leaf_refs, uint32_t n_refs,
                                const float *tri_a, const float *tri_ab, const float *tri_ac, uint32_t n_tris,
-                               uint32_t max_depth, uint32_t *n_pairs, uint32_t *n_brefs, uint32_t *depth,
-                               uint32_t *rec, uint32_t *brefs);
+                               uint32_t max_depth, uint32_t width, uint32_t *n_records, uint32_t *n_brefs,
+                               uint32_t *depth, uint32_t *root, uint32_t *rec, uint32_t *brefs);
 
 /* Tree build on the GPU (the reference builds its trees on the CPU,
  * kd_tree.cpp:25-84, kd_tree_sah.cpp:32-296), built and kept on the device
@@ -184,8 +194,10 @@ PRAY_API int pray_scene_layout(pray_ctx *ctx, int scene_id, uint64_t *sizes, uin
  *                  of pray_boxed_layout (packed layout, and only for trees
  *                  with a children pair whose clips differ -- median trees,
  *                  whose children's boxes overlap -- whose layout the upload
- *                  makes and whose default this is; PRAY_ERR_UNSUPPORTED for
- *                  other trees)
+ *                  makes; PRAY_ERR_UNSUPPORTED for other trees); 4: it takes
+ *                  two binary levels from a 128-byte record of up to four
+ *                  slots' boxes, over the layout's 4-wide form (the same
+ *                  trees; the default for them)
  * pray_set_tuning: -1 keeps a value.  Calls of one context must not overlap
  * (traces share its ray counter and stack spill area). */
 #define PRAY_TUNING_FIELDS 8

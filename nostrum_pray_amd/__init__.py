@@ -118,8 +118,8 @@ def libraries():
         "pray_build_scene": (C.c_int, [_vp, _vp, _vp, _vp, _u32, C.c_int, _u32, _u32, _f32, _f32,
                                        C.POINTER(C.c_int), C.POINTER(_f32)]),
         "pray_scene_layout": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
-        "pray_boxed_layout": (C.c_int, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp,
-                                        _vp]),
+        "pray_boxed_layout": (C.c_int, [_vp, _u32, _vp, _u32, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp,
+                                        _vp, _vp]),
         "pray_kernel_timing": (C.c_int, [_vp, C.c_int]),
         "pray_last_kernel_ms": (C.c_int, [_vp, C.POINTER(_f32)]),
         "pray_timed_kernels_ms": (C.c_int, [_vp, C.POINTER(_f32), C.POINTER(_u32)]),
@@ -782,28 +782,30 @@ class DeviceBuffer:
     __del__ = free
 
 
-def boxed_layout(flat):
+def boxed_layout(flat, width=2):
     """The boxed layout the upload makes for a median tree's boxed node steps
-    (pray_boxed_layout; host code, no device): {"rec": pairs x 16 uint32,
-    "refs": leaf runs, "depth": root-to-leaf edges}.  PrayError
-    (UNSUPPORTED) for a tree whose children pairs all have equal clips."""
+    (pray_boxed_layout; host code, no device): {"rec": records x 16 (width 2)
+    or x 32 (width 4) uint32, "refs": leaf runs, "root": root word,
+    "depth": root-to-leaf edges (width 2) or stack entries (width 4),
+    "width"}.  PrayError (UNSUPPORTED) for a tree whose children pairs all
+    have equal clips."""
     L = libraries()[0]
     nodes = np.ascontiguousarray(flat["nodes"], np.uint32).reshape(-1)
     refs = np.ascontiguousarray(flat["refs"], np.uint32)
     a, ab, ac = (np.ascontiguousarray(flat[k], np.float32) for k in ("a", "ab", "ac"))
     n_tris = a.size // 3
-    n_pairs, n_brefs, depth = _u32(), _u32(), _u32()
+    n_rec, n_brefs, depth, root = _u32(), _u32(), _u32(), _u32()
     args = (_p(nodes), nodes.size // 2, _p(refs), refs.size, _p(a), _p(ab), _p(ac), n_tris, int(flat["max_depth"]),
-            C.byref(n_pairs), C.byref(n_brefs), C.byref(depth))
+            width, C.byref(n_rec), C.byref(n_brefs), C.byref(depth), C.byref(root))
     rc = L.pray_boxed_layout(*args, None, None)
     if rc != PRAY_OK:
         raise PrayError(rc, "no boxed layout for this tree")
-    rec = np.empty((n_pairs.value, 16), np.uint32)
+    rec = np.empty((n_rec.value, 8 * width), np.uint32)
     brefs = np.empty(n_brefs.value, np.uint32)
     rc = L.pray_boxed_layout(*args, _p(rec), _p(brefs))
     if rc != PRAY_OK:
         raise PrayError(rc, "no boxed layout for this tree")
-    return {"rec": rec, "refs": brefs, "depth": depth.value}
+    return {"rec": rec, "refs": brefs, "depth": depth.value, "root": root.value, "width": width}
 
 
 def scene_box_expanded(box6, frac=0.1):

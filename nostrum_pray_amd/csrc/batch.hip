@@ -66,7 +66,8 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
         sc->tune = it->second;
         // the bucket's node form where this tree has it, else its own
         const int nl = sc->tune.node_levels;
-        if ((nl == 2 && !sc->d.quads) || (nl == 3 && !sc->d.boxes)) sc->tune.node_levels = sc->d.boxes ? 3 : 1;
+        if ((nl == 2 && !sc->d.quads) || (nl == 3 && !sc->d.boxes) || (nl == 4 && !sc->d.boxes4))
+            sc->tune.node_levels = sc->d.boxes4 ? 4 : (sc->d.boxes ? 3 : 1);
     } else {
         if (n_rays == 0) n_rays = 1u << 20;
         for (int k = 0; k < 2; k++)
@@ -88,8 +89,8 @@ int pray_autotune(pray_ctx *c, int scene_id, uint32_t n_rays, int force, int *tu
                                         (float *)tt.p, 2);
             if (ms < best_ms) best_ms = ms, best = cand;
         };
-        for (int nl : {1, 2, 3}) {
-            if ((nl == 2 && !sc->d.quads) || (nl == 3 && !sc->d.boxes)) continue;
+        for (int nl : {1, 2, 3, 4}) {
+            if ((nl == 2 && !sc->d.quads) || (nl == 3 && !sc->d.boxes) || (nl == 4 && !sc->d.boxes4)) continue;
             for (int K : {4, 6, 8, 16})
                 for (int ilp : {1, 2})
                     for (int fm : {16, 24, 32, 48}) {
@@ -159,8 +160,8 @@ int pray_autotune_pt(pray_ctx *c, int scene_id, const float *cam16, uint32_t w, 
             return PRAY_OK;
         };
         int rc = PRAY_OK;
-        for (int nl : {1, 2, 3}) {
-            if ((nl == 2 && !sc->d.quads) || (nl == 3 && !sc->d.boxes)) continue;
+        for (int nl : {1, 2, 3, 4}) {
+            if ((nl == 2 && !sc->d.quads) || (nl == 3 && !sc->d.boxes) || (nl == 4 && !sc->d.boxes4)) continue;
             for (int K : {4, 6, 8, 16})
                 for (int ilp : {1, 2})
                     for (int fm : {16, 24, 32, 48}) {

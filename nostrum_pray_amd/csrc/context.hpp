@@ -110,8 +110,10 @@ struct DBuf {
 
 // Deepest tree the traversal's spill area is sized for.
 constexpr uint32_t MAX_TREE_DEPTH = 64;
-// Deepest boxed layout (the tree's depth plus the splits of its leaves).
-constexpr uint32_t MAX_BOXED_DEPTH = 96;
+// Most stack entries of boxed node steps: the binary layout's depth (the
+// tree's plus the splits of its leaves), or three per level of its 4-wide
+// form.
+constexpr uint32_t MAX_BOXED_DEPTH = 192;
 
 SceneRec *find_scene(pray_ctx *c, int id);
 // Device allocation owned by a scene record (freed by free_scene_rec).

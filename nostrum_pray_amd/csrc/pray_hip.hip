@@ -141,7 +141,7 @@ typedef void (*TraceKernel)(DevScene, TraceArgs);
 // 768*K bytes; with 160 KB of LDS per CU, K = 6 is the largest ring that
 // leaves room for 32 resident waves (the VGPR limit of the ILP-2 kernels)
 // and K = 5 for 36 (the ILP-1 kernels).
-template <bool ANY, int LAYOUT, bool BATCH, bool BOX = false>
+template <bool ANY, int LAYOUT, bool BATCH, int BOX = 0>
 TraceKernel pick_kernel_kl(int K, int ilp) {
     using namespace prt;
     switch (K) {
@@ -179,13 +179,15 @@ int node_form(const SceneRec &sc, const Tuning &tu) {
     if (tu.layout != PRAY_LAYOUT_PACKED) return 1;
     if (tu.node_levels == 2 && sc.d.quads) return 2;
     if (tu.node_levels == 3 && sc.d.boxes) return 3;
+    if (tu.node_levels == 4 && sc.d.boxes4) return 4;
     return 1;
 }
 
 // Stack entries a lane may need (the tree's depth; the boxed layout's is
 // deeper where it splits leaves).
 uint32_t trace_depth(const SceneRec &sc, const Tuning &tu) {
-    return std::max<uint32_t>(1, node_form(sc, tu) == 3 ? sc.d.box_depth : sc.d.max_depth);
+    const int f = node_form(sc, tu);
+    return std::max<uint32_t>(1, f == 3 ? sc.d.box_depth : (f == 4 ? sc.d.box4_depth : sc.d.max_depth));
 }
 
 // The kernel of a tuning: form 2 (two-level records) or 3 (boxed children
@@ -201,10 +203,17 @@ TraceKernel pick_kernel(bool any, int layout, int K, int ilp, int form, bool bat
     }
     if (form == 3) {
         if (batch)
-            return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, true, true>(K, ilp)
-                       : pick_kernel_kl<false, PRAY_LAYOUT_PACKED, true, true>(K, ilp);
-        return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, false, true>(K, ilp)
-                   : pick_kernel_kl<false, PRAY_LAYOUT_PACKED, false, true>(K, ilp);
+            return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, true, 1>(K, ilp)
+                       : pick_kernel_kl<false, PRAY_LAYOUT_PACKED, true, 1>(K, ilp);
+        return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, false, 1>(K, ilp)
+                   : pick_kernel_kl<false, PRAY_LAYOUT_PACKED, false, 1>(K, ilp);
+    }
+    if (form == 4) {
+        if (batch)
+            return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, true, 2>(K, ilp)
+                       : pick_kernel_kl<false, PRAY_LAYOUT_PACKED, true, 2>(K, ilp);
+        return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, false, 2>(K, ilp)
+                   : pick_kernel_kl<false, PRAY_LAYOUT_PACKED, false, 2>(K, ilp);
     }
     if (batch)
         return any ? pick_kernel_kl<true, PRAY_LAYOUT_PACKED, true>(K, ilp)
@@ -620,6 +629,80 @@ int make_boxed_layout(pray_ctx *c, const uint32_t *nodes, const uint32_t *refs, 
     return PRAY_OK;
 }
 
+// The 4-wide form of a boxed layout (node_levels 4): each record collapses a
+// binary node with its children, so that a node step tests up to four boxes
+// two binary levels down.  32 words per record: lower x of slots 0..3, upper
+// x, lower y, upper y, lower z, upper z (float bits), the slots' words, four
+// zero words.  A slot is a child of the binary node that is a leaf (or
+// empty), or a grandchild through a child that is inner; an inner slot's
+// word names its record as w >> 3; unused slots are empty leaves with boxes
+// {+inf, -inf}.  Leaf words and the leaf runs are the binary layout's.
+struct WideBuilder {
+    const std::vector<uint32_t> &rec2;
+    std::vector<uint32_t> rec4;
+    uint32_t depth = 0;  // records on the longest root-to-leaf path, minus one
+    explicit WideBuilder(const std::vector<uint32_t> &r) : rec2(r) {}
+    struct Slot {
+        uint32_t w, lo[3], hi[3];
+    };
+    Slot child(uint32_t p, int ch) const {
+        const uint32_t *r = &rec2[16 * (size_t)p];
+        return Slot{r[12 + ch], {r[ch], r[4 + ch], r[8 + ch]}, {r[2 + ch], r[6 + ch], r[10 + ch]}};
+    }
+    static bool inner(uint32_t w) { return (w & 3u) != prt::LEAF_TAG; }
+    // The record of binary inner node word w2, at wide depth d: its word.
+    uint32_t node(uint32_t w2, uint32_t d) {
+        depth = std::max(depth, d);
+        const uint32_t p = w2 >> 3;
+        Slot slot[4];
+        int n = 0;
+        for (int ch = 0; ch < 2; ++ch) {
+            const Slot c = child(p, ch);
+            if (c.w != ((prt::LEAF_EMPTY << 2) | prt::LEAF_TAG) && inner(c.w)) {
+                slot[n++] = child(c.w >> 3, 0);
+                slot[n++] = child(c.w >> 3, 1);
+            } else {
+                slot[n++] = c;
+            }
+        }
+        const uint32_t k = (uint32_t)(rec4.size() / 32);
+        rec4.resize(rec4.size() + 32, 0u);
+        uint32_t words[4];
+        for (int j = 0; j < 4; ++j) {
+            if (j >= n) {
+                words[j] = (prt::LEAF_EMPTY << 2) | prt::LEAF_TAG;
+                continue;
+            }
+            const uint32_t w = slot[j].w;
+            words[j] = (w != ((prt::LEAF_EMPTY << 2) | prt::LEAF_TAG) && inner(w)) ? node(w, d + 1) : w;
+        }
+        uint32_t *r = &rec4[32 * (size_t)k];
+        const uint32_t pinf = 0x7F800000u, ninf = 0xFF800000u;
+        for (int j = 0; j < 4; ++j)
+            for (int a = 0; a < 3; ++a) {
+                r[8 * a + j] = j < n ? slot[j].lo[a] : pinf;
+                r[8 * a + 4 + j] = j < n ? slot[j].hi[a] : ninf;
+            }
+        for (int j = 0; j < 4; ++j) r[24 + j] = words[j];
+        return k << 3;
+    }
+};
+
+// The 4-wide records of a binary boxed layout, their root word and the stack
+// entries a lane may need (each step stacks at most three slots).
+int make_wide_layout(pray_ctx *c, const BoxedBuilder &B, std::vector<uint32_t> &rec4, uint32_t &root,
+                     uint32_t &stack_depth) {
+    WideBuilder W(B.rec);
+    const uint32_t rw = B.rec[13];  // the binary layout's root
+    root = (rw != ((prt::LEAF_EMPTY << 2) | prt::LEAF_TAG) && WideBuilder::inner(rw)) ? W.node(rw, 0) : rw;
+    if (W.rec4.empty()) W.rec4.assign(32, 0u);
+    stack_depth = 3 * (W.depth + 1);
+    if (stack_depth > MAX_BOXED_DEPTH)
+        return fail(c, PRAY_ERR_UNSUPPORTED, "4-wide boxed layout needs %u stack entries", stack_depth);
+    rec4.swap(W.rec4);
+    return PRAY_OK;
+}
+
 }  // namespace
 
 // Uploads the boxed layout of a tree that takes one (else leaves boxes null).
@@ -627,28 +710,39 @@ int build_boxes(pray_ctx *c, SceneRec &sc, const uint32_t *nodes, uint32_t n_nod
                 const float *A, const float *AB, const float *AC, hipStream_t st) {
     prt::DevScene &d = sc.d;
     d.boxes = nullptr;
+    d.boxes4 = nullptr;
     d.brecs = nullptr;
-    d.box_depth = 0;
+    d.box_depth = d.box4_depth = d.box4_root = 0;
     if (d.n_tris == 0 || n_nodes < 4 || !unequal_clips(nodes, n_nodes)) return PRAY_OK;
     BoxedBuilder B;
     if (int rc = make_boxed_layout(c, nodes, refs, A, AB, AC, B)) return rc;
+    // the 4-wide form too, unless its stack would be too deep (then boxed
+    // node steps stay binary)
+    std::vector<uint32_t> rec4;
+    uint32_t root4 = 0, depth4 = 0;
+    const bool wide = make_wide_layout(nullptr, B, rec4, root4, depth4) == PRAY_OK;
     const size_t n_brefs = B.brefs.size();
-    void *rp = nullptr, *fp = nullptr, *bp = nullptr;
+    void *rp = nullptr, *fp = nullptr, *bp = nullptr, *wp = nullptr;
     int rc = device_alloc(c, sc, &rp, B.rec.size() * 4);
+    if (!rc && wide) rc = device_alloc(c, sc, &wp, rec4.size() * 4);
     if (!rc) rc = device_alloc(c, sc, &fp, (n_brefs + 2) * 4);
     if (!rc) rc = device_alloc(c, sc, &bp, (n_brefs + 2) * 48);  // padded for the ILP look-ahead
     if (rc) return rc;
     HIPC(c, hipMemcpyAsync(rp, B.rec.data(), B.rec.size() * 4, hipMemcpyHostToDevice, st));
+    if (wide) HIPC(c, hipMemcpyAsync(wp, rec4.data(), rec4.size() * 4, hipMemcpyHostToDevice, st));
     HIPC(c, hipMemsetAsync(fp, 0, (n_brefs + 2) * 4, st));
     HIPC(c, hipMemcpyAsync(fp, B.brefs.data(), n_brefs * 4, hipMemcpyHostToDevice, st));
     HIPC(c, hipMemsetAsync(bp, 0, (n_brefs + 2) * 48, st));
     hipLaunchKernelGGL(k_pack_records, dim3(nblocks(n_brefs, 256)), dim3(256), 0, st, (const uint32_t *)fp,
                        (uint32_t)n_brefs, d.a, d.ab, d.ac, (float4 *)bp);
     HIPC(c, hipGetLastError());
-    HIPC(c, hipStreamSynchronize(st));  // B's vectors are freed on return
+    HIPC(c, hipStreamSynchronize(st));  // the host vectors are freed on return
     d.boxes = static_cast<const uint4 *>(rp);
+    d.boxes4 = static_cast<const uint4 *>(wp);
     d.brecs = static_cast<const float4 *>(bp);
     d.box_depth = B.depth;
+    d.box4_depth = depth4;
+    d.box4_root = root4;
     return PRAY_OK;
 }
 
@@ -851,8 +945,9 @@ int pray_upload_scene(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, cons
         }
         if ((rc = build_quads(c, *sc, c->stream))) goto bad;
         if ((rc = build_boxes(c, *sc, nodes, n_nodes, leaf_refs, tri_a, tri_ab, tri_ac, c->stream))) goto bad;
-        // boxed node steps by default where the tree has them
-        if (d.boxes) sc->tune.node_levels = 3;
+        // boxed node steps by default where the tree has them, 4-wide where
+        // that form exists
+        if (d.boxes) sc->tune.node_levels = d.boxes4 ? 4 : 3;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         rc = fail(c, PRAY_ERR_HIP, "scene upload failed");
@@ -880,21 +975,29 @@ int pray_free_scene(pray_ctx *c, int scene_id) {
 
 int pray_boxed_layout(const uint32_t *nodes, uint32_t n_nodes, const uint32_t *leaf_refs, uint32_t n_refs,
                       const float *tri_a, const float *tri_ab, const float *tri_ac, uint32_t n_tris,
-                      uint32_t max_depth, uint32_t *n_pairs, uint32_t *n_brefs, uint32_t *depth, uint32_t *rec,
-                      uint32_t *brefs) {
-    if (!nodes || !leaf_refs || !tri_a || !tri_ab || !tri_ac || !n_pairs || !n_brefs || !depth || n_tris == 0 ||
-        n_refs == 0)
+                      uint32_t max_depth, uint32_t width, uint32_t *n_records, uint32_t *n_brefs, uint32_t *depth,
+                      uint32_t *root, uint32_t *rec, uint32_t *brefs) {
+    if (!nodes || !leaf_refs || !tri_a || !tri_ab || !tri_ac || !n_records || !n_brefs || !depth || !root ||
+        n_tris == 0 || n_refs == 0 || (width != 2 && width != 4))
         return PRAY_ERR_INVALID;
     if (max_depth > MAX_TREE_DEPTH) return PRAY_ERR_UNSUPPORTED;
     if (int v = validate_tree(nullptr, nodes, n_nodes, leaf_refs, n_refs, n_tris, max_depth)) return v;
     if (n_nodes < 4 || !unequal_clips(nodes, n_nodes)) return PRAY_ERR_UNSUPPORTED;
     BoxedBuilder B;
     if (int rc = make_boxed_layout(nullptr, nodes, leaf_refs, tri_a, tri_ab, tri_ac, B)) return rc;
-    *n_pairs = (uint32_t)(B.rec.size() / 16);
     *n_brefs = (uint32_t)B.brefs.size();
-    *depth = B.depth;
-    if (rec) std::memcpy(rec, B.rec.data(), B.rec.size() * 4);
     if (brefs) std::memcpy(brefs, B.brefs.data(), B.brefs.size() * 4);
+    if (width == 2) {
+        *n_records = (uint32_t)(B.rec.size() / 16);
+        *depth = B.depth;
+        *root = B.rec[13];
+        if (rec) std::memcpy(rec, B.rec.data(), B.rec.size() * 4);
+        return PRAY_OK;
+    }
+    std::vector<uint32_t> rec4;
+    if (int rc = make_wide_layout(nullptr, B, rec4, *root, *depth)) return rc;
+    *n_records = (uint32_t)(rec4.size() / 32);
+    if (rec) std::memcpy(rec, rec4.data(), rec4.size() * 4);
     return PRAY_OK;
 }
 
@@ -920,14 +1023,16 @@ int pray_set_tuning(pray_ctx *c, int scene_id, const int *v) {
     if ((leaf_batch >= 0 ? leaf_batch : sc->tune.leaf_batch) > 0 &&
         (layout >= 0 ? layout : sc->tune.layout) != PRAY_LAYOUT_PACKED)
         return fail(c, PRAY_ERR_INVALID, "leaf batching traces the packed layout");
-    if (node_levels >= 0 && node_levels != 1 && node_levels != 2 && node_levels != 3)
-        return fail(c, PRAY_ERR_INVALID, "node_levels must be 1, 2 or 3");
+    if (node_levels >= 0 && (node_levels < 1 || node_levels > 4))
+        return fail(c, PRAY_ERR_INVALID, "node_levels must be 1, 2, 3 or 4");
     const int nl = node_levels >= 0 ? node_levels : sc->tune.node_levels;
     if (nl == 2 && !sc->d.quads)
         return fail(c, PRAY_ERR_UNSUPPORTED, "scene %d has no two-level node records (not a SAH-form tree)", scene_id);
     if (nl == 3 && !sc->d.boxes)
         return fail(c, PRAY_ERR_UNSUPPORTED, "scene %d has no boxed children pairs (its children pairs have equal clips)",
                     scene_id);
+    if (nl == 4 && !sc->d.boxes4)
+        return fail(c, PRAY_ERR_UNSUPPORTED, "scene %d has no 4-wide boxed layout", scene_id);
     if (nl != 1 && (layout >= 0 ? layout : sc->tune.layout) != PRAY_LAYOUT_PACKED)
         return fail(c, PRAY_ERR_INVALID, "two-level and boxed node steps trace the packed layout");
     Tuning &t = sc->tune;

@@ -53,11 +53,13 @@ struct DevScene {
     const uint4 *quads = nullptr;     // 2 per children pair: two-level node records (SAH trees), or null
     const uint4 *boxes = nullptr;     // 4 per pair of the boxed layout (median trees), or null
     const float4 *brecs = nullptr;    // packed records of the boxed layout's leaf runs (padded)
+    const uint4 *boxes4 = nullptr;    // 8 per record of the 4-wide boxed layout, or null
     const float *normal = nullptr, *color = nullptr;
     const uint8_t *emit = nullptr;
     float box[6] = {0, 0, 0, 0, 0, 0};
     uint32_t n_tris = 0, n_nodes = 0, n_refs = 0, max_depth = 0;
     uint32_t box_depth = 0;           // depth of the boxed layout
+    uint32_t box4_depth = 0, box4_root = 0;  // stack entries and root word of the 4-wide layout
 };
 
 struct TraceArgs {
@@ -137,7 +139,7 @@ __device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, 
 // BATCH: leaf batching (TraceArgs::leaf_batch > 0) compiled in; without it
 // every iteration takes node and leaf steps and the loop carries no batching
 // code.
-// BOX: node steps over the boxed layout (DevScene::boxes, with the records
+// BOX == 1: node steps over the boxed layout (DevScene::boxes, with the records
 // of its leaf runs in DevScene::brecs) instead of the 8-byte nodes' clip
 // planes.  Per
 // pair p, boxes[4p + a] for axis a = 0, 1, 2 holds {child 0's lower bound,
@@ -149,11 +151,18 @@ __device__ __forceinline__ void load_tri(const DevScene &s, const float4 *recs, 
 // 0 on a tie) and stacks the other.  For trees whose children overlap
 // (median trees: the children's boxes are those of their triangles) this
 // prunes on all three axes where the clip planes bound one.
-template <bool ANY, int LAYOUT, int K, int ILP, bool MULTI = false, bool BATCH = false, bool BOX = false>
+// BOX == 2: the same over the 4-wide form (DevScene::boxes4, root word
+// DevScene::box4_root): record q is boxes4[8q .. 8q + 7] = {lower x of slots
+// 0..3}, {upper x}, {lower y}, {upper y}, {lower z}, {upper z}, {the slots'
+// words}, unused.  A step intersects the ray's interval with the four
+// slots' boxes, orders the live ones by entry distance (lower slot first on
+// a tie), enters the nearest and stacks the others farthest first, so that
+// pops return them nearest first.
+template <bool ANY, int LAYOUT, int K, int ILP, bool MULTI = false, bool BATCH = false, int BOX = 0>
 static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s, TraceArgs A) {
     static_assert(K >= 2 && K <= 16, "LDS ring of 2..16 entries");
     static_assert(!MULTI || LAYOUT == 1, "multi-scene launches use the packed layout");
-    static_assert(!BOX || (LAYOUT == 1 && !MULTI), "boxed node steps trace the packed layout of one scene");
+    static_assert(BOX == 0 || (LAYOUT == 1 && !MULTI), "boxed node steps trace the packed layout of one scene");
     extern __shared__ uint32_t ring_lds[];
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t wave_in_block = threadIdx.x / WAVE;
@@ -167,10 +176,12 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
     const uint4 *pairs = reinterpret_cast<const uint4 *>(s.nodes);
     const float4 *recs = BOX ? s.brecs : s.recs;
     // The root's word (node 1, the second word pair of the first 16 bytes;
-    // child 1 of the boxed layout's pair 0), read once per wave, not once per
-    // ray (single-scene launches).
+    // child 1 of the boxed layout's pair 0; the 4-wide layout's root word),
+    // read once per wave, not once per ray (single-scene launches).
     const uint32_t root_w =
-        MULTI ? 0u : (uint32_t)__builtin_amdgcn_readfirstlane((int)(BOX ? s.boxes[3].y : pairs[0].w));
+        MULTI ? 0u
+              : (BOX == 2 ? s.box4_root
+                          : (uint32_t)__builtin_amdgcn_readfirstlane((int)(BOX ? s.boxes[3].y : pairs[0].w)));
 
     uint32_t state = ST_IDLE, ray = 0, w = 0, k = 0, best_i = NO_HIT, cn = 0, cr = 0;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, ix = 0, iy = 0, iz = 0, lim2 = 0;
@@ -292,8 +303,78 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
             run_node = !run_leaf;
         }
         bool need_pop = false;
+        // --------------------------------------- 4-wide boxed node step --
+        if constexpr (BOX == 2) {
+            if (run_node && state == ST_NODE) {
+                const uint4 *bp = s.boxes4 + 8 * (size_t)(w >> 3);
+                const uint4 qlx = bp[0], qhx = bp[1], qly = bp[2], qhy = bp[3], qlz = bp[4], qhz = bp[5];
+                const uint4 qw = bp[6];
+                cn += 4;
+                const float cap = ANY ? hi : fminf(hi, best_t);
+                float l[4], h[4];
+                uint32_t cw[4];
+                const uint32_t lxa[4] = {qlx.x, qlx.y, qlx.z, qlx.w}, hxa[4] = {qhx.x, qhx.y, qhx.z, qhx.w};
+                const uint32_t lya[4] = {qly.x, qly.y, qly.z, qly.w}, hya[4] = {qhy.x, qhy.y, qhy.z, qhy.w};
+                const uint32_t lza[4] = {qlz.x, qlz.y, qlz.z, qlz.w}, hza[4] = {qhz.x, qhz.y, qhz.z, qhz.w};
+                cw[0] = qw.x, cw[1] = qw.y, cw[2] = qw.z, cw[3] = qw.w;
+                // key: entry distance, +inf for a slot not entered
+                float key[4];
+                uint32_t sl[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    l[j] = lo, h[j] = cap;
+                    slab(__uint_as_float(lxa[j]), __uint_as_float(hxa[j]), ox, ix, l[j], h[j]);
+                    slab(__uint_as_float(lya[j]), __uint_as_float(hya[j]), oy, iy, l[j], h[j]);
+                    slab(__uint_as_float(lza[j]), __uint_as_float(hza[j]), oz, iz, l[j], h[j]);
+                    key[j] = (l[j] <= h[j] && !empty_leaf(cw[j])) ? l[j] : INFINITY;
+                    sl[j] = (uint32_t)j;
+                }
+                // sorting network over (key, slot), ties by slot
+                auto cx = [&](int a, int b) {
+                    const bool sw = key[b] < key[a] || (key[b] == key[a] && sl[b] < sl[a]);
+                    const float ka = key[a], kb = key[b];
+                    const uint32_t sa = sl[a], sb = sl[b];
+                    key[a] = sw ? kb : ka, key[b] = sw ? ka : kb;
+                    sl[a] = sw ? sb : sa, sl[b] = sw ? sa : sb;
+                };
+                cx(0, 1), cx(2, 3), cx(0, 2), cx(1, 3), cx(1, 2);
+                auto pick = [&](uint32_t j, uint32_t &pw, float &pl, float &ph) {
+                    pw = j == 0 ? cw[0] : (j == 1 ? cw[1] : (j == 2 ? cw[2] : cw[3]));
+                    pl = j == 0 ? l[0] : (j == 1 ? l[1] : (j == 2 ? l[2] : l[3]));
+                    ph = j == 0 ? h[0] : (j == 1 ? h[1] : (j == 2 ? h[2] : h[3]));
+                };
+#pragma unroll
+                for (int q = 3; q >= 1; --q) {
+                    if (key[q] != INFINITY) {
+                        uint32_t pw;
+                        float pl, ph;
+                        pick(sl[q], pw, pl, ph);
+                        if (sp - base == K) {  // ring full: spill the oldest entry
+                            const uint32_t at = ((uint32_t)base % K) * WAVE + lane;
+                            spill[base] =
+                                make_uint4(st_w[at], __float_as_uint(st_lo[at]), __float_as_uint(st_hi[at]), 0u);
+                            ++base;
+                        }
+                        const uint32_t at = ((uint32_t)sp % K) * WAVE + lane;
+                        st_w[at] = pw;
+                        st_lo[at] = pl;
+                        st_hi[at] = ph;
+                        ++sp;
+                    }
+                }
+                if (key[0] != INFINITY) {
+                    pick(sl[0], w, lo, hi);
+                    if ((w & 3u) == LEAF_TAG) {
+                        state = ST_LEAF;
+                        k = w >> 2;
+                    }
+                } else {
+                    need_pop = true;
+                }
+            }
+        }
         // ---------------------------------------------- boxed node step --
-        if constexpr (BOX) {
+        if constexpr (BOX == 1) {
             if (run_node && state == ST_NODE) {
                 const uint4 *bp = s.boxes + 4 * (size_t)(w >> 3);
                 const uint4 qx = bp[0], qy = bp[1], qz = bp[2], qw = bp[3];
@@ -336,7 +417,7 @@ static __global__ __launch_bounds__(MAX_WG_WAVES * WAVE) void k_trace(DevScene s
             }
         }
         // ---------------------------------------------------- node step --
-        if (!BOX && run_node && state == ST_NODE) {
+        if (BOX == 0 && run_node && state == ST_NODE) {
             const uint32_t axis = w & 3u;
             const uint4 pr = pairs[w >> 3];  // pair index = (w >> 2) >> 1
             cn += 2;

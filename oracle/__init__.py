@@ -53,6 +53,8 @@ def lib():
                                       _vp]),
             "orc_trace_boxed": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _u64, C.c_int, _f32, _vp, _vp, _vp,
                                        _vp]),
+            "orc_trace_boxed4": (None, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _u64, C.c_int, _f32, _vp, _vp,
+                                        _vp, _vp]),
             "orc_camera": (None, [_vp, _vp, _u64, _u64, _u64, _vp]),
             "orc_camera_rays": (None, [_vp, _u64, _u64, _vp]),
             "orc_mlp_encode": (None, [_vp, _u64, _vp]),
@@ -229,8 +231,8 @@ def ref_closest_avx(export, a, ab, ac, rays, eps, threads):
 
 def flat_trace(flat, rays, mode=0, eps=1e-4, limit2=None, counts=False, boxed=None):
     """The GPU traversal's twin over the flat layout, or with boxed node steps
-    over `boxed`, a boxed layout {"rec": pairs x 16 uint32, "refs": leaf
-    runs} (node_levels 3)."""
+    over `boxed`, a boxed layout from nostrum_pray_amd.boxed_layout (width 2:
+    node_levels 3; width 4: node_levels 4)."""
     rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
     n = rays.shape[0]
     idx, t = np.empty(n, np.int32), np.empty(n, np.float32)
@@ -239,8 +241,13 @@ def flat_trace(flat, rays, mode=0, eps=1e-4, limit2=None, counts=False, boxed=No
     if boxed is not None:
         rec = np.ascontiguousarray(boxed["rec"], np.uint32)
         brefs = np.ascontiguousarray(boxed["refs"], np.uint32)
-        lib().orc_trace_boxed(_p(rec), _p(brefs), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]), _p(flat["box"]),
-                              flat["a"].shape[0], _p(rays), n, mode, eps, _p(lim), _p(idx), _p(t), _p(cnt))
+        if boxed.get("width", 2) == 4:
+            lib().orc_trace_boxed4(_p(rec), boxed["root"], _p(brefs), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]),
+                                   _p(flat["box"]), flat["a"].shape[0], _p(rays), n, mode, eps, _p(lim), _p(idx),
+                                   _p(t), _p(cnt))
+        else:
+            lib().orc_trace_boxed(_p(rec), _p(brefs), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]), _p(flat["box"]),
+                                  flat["a"].shape[0], _p(rays), n, mode, eps, _p(lim), _p(idx), _p(t), _p(cnt))
     else:
         lib().orc_flat_trace(_p(flat["nodes"]), _p(flat["refs"]), _p(flat["a"]), _p(flat["ab"]), _p(flat["ac"]),
                              _p(flat["box"]), flat["a"].shape[0], _p(rays), n, mode, eps, _p(lim), _p(idx), _p(t),

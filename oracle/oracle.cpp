@@ -419,6 +419,102 @@ uint32_t boxedTrace(const uint32_t *rec, const Flat &f, V o, V d, int mode, floa
     return best_i;
 }
 
+// Twin of the 4-wide boxed node step (node_levels 4): record q is 32 words,
+// {lower x of slots 0..3}, {upper x}, {lower y}, {upper y}, {lower z},
+// {upper z}, {slot words}; a step intersects the ray's interval, capped at
+// the best t, with the four slots' boxes, orders the entered ones by entry
+// distance (lower slot first on a tie), enters the nearest and stacks the
+// others farthest first; leaves, stack and counters as boxedTrace, with 4
+// counted per step.
+uint32_t boxedTrace4(const uint32_t *rec, uint32_t root, const Flat &f, V o, V d, int mode, float lim2, float eps,
+                     float &tout, uint32_t &cn, uint32_t &cr) {
+    const V inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    float lo = 0.f, hi = INFINITY;
+    slabW(f.box[0], f.box[3], o.x, inv.x, lo, hi);
+    slabW(f.box[1], f.box[4], o.y, inv.y, lo, hi);
+    slabW(f.box[2], f.box[5], o.z, inv.z, lo, hi);
+    if (mode != 0) {
+        if (!(lim2 >= 0.f)) return NO_HIT;
+        hi = std::fmin(hi, whi(std::sqrt(lim2)));
+    }
+    float best_t = INFINITY;
+    uint32_t best_i = NO_HIT;
+    if (!(lo <= hi)) return best_i;
+    struct E {
+        uint32_t w;
+        float lo, hi;
+    };
+    std::vector<E> st;
+    uint32_t w = root;
+    cn += 1;
+    auto fl = [](uint32_t u) {
+        float x;
+        std::memcpy(&x, &u, 4);
+        return x;
+    };
+    const uint32_t EMPTY = (LEAF_EMPTY << 2) | LEAF_TAG;
+    while (true) {
+        while ((w & 3u) != LEAF_TAG) {
+            const uint32_t *r = rec + 32 * (size_t)(w >> 3);
+            cn += 4;
+            const float cap = mode == 0 ? std::fmin(hi, best_t) : hi;
+            float l[4], h[4];
+            int order[4], n = 0;
+            for (int j = 0; j < 4; j++) {
+                l[j] = lo, h[j] = cap;
+                slabW(fl(r[j]), fl(r[4 + j]), o.x, inv.x, l[j], h[j]);
+                slabW(fl(r[8 + j]), fl(r[12 + j]), o.y, inv.y, l[j], h[j]);
+                slabW(fl(r[16 + j]), fl(r[20 + j]), o.z, inv.z, l[j], h[j]);
+                if (l[j] <= h[j] && r[24 + j] != EMPTY) order[n++] = j;
+            }
+            std::stable_sort(order, order + n, [&](int a, int b) { return l[a] < l[b]; });
+            for (int q = n - 1; q >= 1; q--) st.push_back({r[24 + order[q]], l[order[q]], h[order[q]]});
+            if (n) {
+                w = r[24 + order[0]], lo = l[order[0]], hi = h[order[0]];
+            } else {
+                w = EMPTY;
+            }
+        }
+        uint32_t k = w >> 2;
+        if (k != LEAF_EMPTY) {
+            while (true) {
+                const uint32_t ref = f.refs[k];
+                const uint32_t tri = ref & ~REF_LAST;
+                cr += 1;
+                float t;
+                if (mt(o, d, ld(f.a + 3 * (size_t)tri), ld(f.ab + 3 * (size_t)tri), ld(f.ac + 3 * (size_t)tri), eps,
+                       t)) {
+                    if (mode == 0) {
+                        if (t > 0.f && (t < best_t || (t == best_t && tri < best_i))) best_t = t, best_i = tri;
+                    } else {
+                        const bool occ = mode == 1 ? (t * t < lim2) : (t * t <= lim2);
+                        if (t > 0.f && occ) {
+                            tout = t;
+                            return tri;
+                        }
+                    }
+                }
+                if (ref & REF_LAST) break;
+                ++k;
+            }
+        }
+        bool got = false;
+        while (!st.empty()) {
+            const E e = st.back();
+            st.pop_back();
+            const float cap = mode == 0 ? std::fmin(e.hi, best_t) : e.hi;
+            if (e.lo <= cap) {
+                w = e.w, lo = e.lo, hi = e.hi;
+                got = true;
+                break;
+            }
+        }
+        if (!got) break;
+    }
+    tout = best_t;
+    return best_i;
+}
+
 // ----------------------------------------------------- reference KD tree --
 // A tree in depth-first preorder (left subtree before right): per node
 // is_leaf, first/count into the leaf triangle list, box[6].
@@ -715,6 +811,26 @@ ORC_API void orc_flat_trace(const uint32_t *nodes, const uint32_t *refs, const f
         float t = INFINITY;
         uint32_t h = NO_HIT;
         if (n_tris) h = flatTrace(f, ld(rays + 6 * i), ld(rays + 6 * i + 3), mode, lim2 ? lim2[i] : 0.f, eps, t, cn, cr);
+        out_idx[i] = (int32_t)h;
+        out_t[i] = h == NO_HIT ? FLT_MAX : t;
+        if (counts) counts[2 * i] = cn, counts[2 * i + 1] = cr;
+    }
+}
+
+// flat_trace over the 4-wide boxed layout (boxedTrace4).
+ORC_API void orc_trace_boxed4(const uint32_t *rec, uint32_t root, const uint32_t *brefs, const float *A,
+                              const float *AB, const float *AC, const float *box6, uint64_t n_tris, const float *rays,
+                              uint64_t n, int mode, float eps, const float *lim2, int32_t *out_idx, float *out_t,
+                              uint32_t *counts) {
+    const Flat f{nullptr, brefs, A, AB, AC, box6};
+#pragma omp parallel for schedule(dynamic, 64)
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t cn = 0, cr = 0;
+        float t = INFINITY;
+        uint32_t h = NO_HIT;
+        if (n_tris)
+            h = boxedTrace4(rec, root, f, ld(rays + 6 * i), ld(rays + 6 * i + 3), mode, lim2 ? lim2[i] : 0.f, eps, t,
+                            cn, cr);
         out_idx[i] = (int32_t)h;
         out_t[i] = h == NO_HIT ? FLT_MAX : t;
         if (counts) counts[2 * i] = cn, counts[2 * i + 1] = cr;

@@ -110,7 +110,7 @@ def test_pt_cornell_equal_oracle(ctx, use_graph, budget, kind):
     w, h, spp, depth, rr = 64, 48, 8, 6, 2
     cam = O.camera([0, 1, 3.8], [0, 0, -1], 40, w, h)
     sid = ctx.upload(t)
-    assert ctx.get_tuning(sid)["node_levels"] == (3 if kind == 0 else 1)
+    assert ctx.get_tuning(sid)["node_levels"] == (4 if kind == 0 else 1)
     if budget is not None:
         ctx.set_path_budget(budget)
     try:

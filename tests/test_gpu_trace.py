@@ -49,11 +49,13 @@ def scene(name):
 # lanes wait).  Two levels per step need a tree whose children pairs have
 # equal clips (every SAH tree; the terrain's median tree too, whose children
 # meet on grid lines); other trees refuse them.  Node records 3 (boxed
-# children pairs) exist for exactly those other trees (median trees whose
-# children overlap) and are refused for the rest.
+# children pairs) and 4 (their 4-wide form) exist for exactly those other
+# trees (median trees whose children overlap) and are refused for the rest.
 SETTINGS = [(P.LAYOUT_PACKED, 6, 1, 2, 16, 0, 2),
             (P.LAYOUT_PACKED, 6, 1, 2, 16, 0, 3), (P.LAYOUT_PACKED, 8, 4, 1, 24, 16, 3),
             (P.LAYOUT_PACKED, 5, 1, 1, 1, 1, 3), (P.LAYOUT_PACKED, 16, 2, 2, 48, 0, 3),
+            (P.LAYOUT_PACKED, 6, 1, 2, 16, 0, 4), (P.LAYOUT_PACKED, 4, 4, 1, 24, 16, 4),
+            (P.LAYOUT_PACKED, 5, 1, 1, 1, 1, 4), (P.LAYOUT_PACKED, 16, 2, 2, 48, 0, 4),
             (P.LAYOUT_PACKED, 8, 1, 1, 24, 0, 1), (P.LAYOUT_PACKED, 8, 1, 2, 16, 16, 1),
             (P.LAYOUT_PACKED, 5, 2, 2, 16, 0, 1), (P.LAYOUT_INDEXED, 8, 4, 1, 32, 0, 1),
             (P.LAYOUT_INDEXED, 6, 1, 2, 16, 0, 1), (P.LAYOUT_PACKED, 6, 1, 2, 48, 24, 1),
@@ -78,7 +80,7 @@ def set_or_refuse(ctx, sid, name, layout, ring, wg, ilp, fm, lb, nl):
     records (nl 2) or no boxed pairs (nl 3), after checking that the library
     refuses exactly those trees."""
     eq = equal_clips(scene(name)[2])
-    if (nl == 2 and not eq) or (nl == 3 and eq):
+    if (nl == 2 and not eq) or (nl >= 3 and eq):
         with pytest.raises(P.PrayError):
             ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, node_levels=nl)
         return False
@@ -192,21 +194,23 @@ def test_work_counts_match_cpu_twin(ctx, lb, nl):
     assert np.array_equal(gc, oc)
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("lb", [0, 16])
 @pytest.mark.parametrize("name,rays_of", [("spheres-median", "camera"), ("spheres-median", "incoherent"),
                                           ("random-median", "incoherent")])
-def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb):
-    """Boxed node steps (node records 3, the default for median trees whose
-    children overlap): per-ray node and triangle counts and hits equal the
-    CPU twin's over the same boxed layout, with and without leaf batching;
-    the twin's hits equal the plane-form traversal's and brute force."""
+def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb, width):
+    """Boxed node steps (node records 3, and 4 over the 4-wide form, the
+    default for median trees whose children overlap): per-ray node and
+    triangle counts and hits equal the CPU twin's over the same boxed layout,
+    with and without leaf batching; the twin's hits equal the plane-form
+    traversal's and brute force."""
     s, t, flat = scene(name)
     rays = s.prim_rays() if rays_of == "camera" else O.gen_rays(4, P.scene_box_expanded(flat["box"]), 1 << 15)
     sid = ctx.upload(t)
     try:
-        assert ctx.get_tuning(sid)["node_levels"] == 3
-        ctx.set_tuning(sid, leaf_batch=lb)
-        say(f"counts boxed {name} {rays_of} leaf_batch={lb}")
+        assert ctx.get_tuning(sid)["node_levels"] == 4
+        ctx.set_tuning(sid, leaf_batch=lb, node_levels=3 if width == 2 else 4)
+        say(f"counts boxed {name} {rays_of} leaf_batch={lb} width={width}")
         n = rays.shape[0]
         d_rays = ctx.to_device(rays)
         d_idx = ctx.alloc(n, np.int32)
@@ -217,7 +221,7 @@ def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb):
         gc, gi, gt = d_c.numpy(), d_idx.numpy(), d_t.numpy()
     finally:
         ctx.free_scene(sid)
-    oi, ot, oc = O.flat_trace(flat, rays, 0, 1e-4, counts=True, boxed=P.boxed_layout(flat))
+    oi, ot, oc = O.flat_trace(flat, rays, 0, 1e-4, counts=True, boxed=P.boxed_layout(flat, width))
     assert np.array_equal(gi, oi)
     assert np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
     assert np.array_equal(gc, oc)
@@ -230,15 +234,15 @@ def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb):
 
 
 def test_node_records_by_tree(ctx):
-    """The upload's default node records: boxed pairs for the median tree
-    whose children overlap, children pairs for SAH trees and for the
-    terrain's median tree (equal clips)."""
-    for name, want in [("spheres-median", 3), ("spheres-sah", 1), ("terrain256-median", 1)]:
+    """The upload's default node records: the 4-wide boxed layout for the
+    median tree whose children overlap, children pairs for SAH trees and for
+    the terrain's median tree (equal clips)."""
+    for name, want in [("spheres-median", 4), ("spheres-sah", 1), ("terrain256-median", 1)]:
         _, t, flat = scene(name)
         sid = ctx.upload(t)
         try:
             assert ctx.get_tuning(sid)["node_levels"] == want, name
-            assert (want == 3) == (not equal_clips(flat))
+            assert (want == 4) == (not equal_clips(flat))
         finally:
             ctx.free_scene(sid)
 
@@ -246,7 +250,8 @@ def test_node_records_by_tree(ctx):
 @pytest.mark.parametrize("name,ring,lb,nl", [("spheres-sah", 5, 0, 1), ("spheres-sah", 16, 0, 1),
                                              ("spheres-sah", 5, 32, 1), ("spheres-sah", 16, 8, 1),
                                              ("spheres-sah", 5, 0, 2), ("spheres-sah", 16, 24, 2),
-                                             ("spheres-median", 5, 0, 3), ("spheres-median", 16, 24, 3)])
+                                             ("spheres-median", 5, 0, 3), ("spheres-median", 16, 24, 3),
+                                             ("spheres-median", 4, 0, 4), ("spheres-median", 16, 24, 4)])
 def test_launch_sizes(ctx, name, ring, lb, nl):
     """Traces over ray counts around the wave, the 64-ray chunks and the 64
     chunk counters (partial last chunks, fewer waves than counters), one after

@@ -267,8 +267,74 @@ def test_boxed_layout_structure():
         P.boxed_layout(s.build(1).flat())
 
 
+def test_boxed_layout_4wide_structure():
+    """The 4-wide form of the boxed layout: every record's live slots are
+    exactly the binary layout's children and grandchildren two levels down
+    (a leaf child, or both children of an inner child), with their boxes and
+    leaf words, so that both forms reach the same leaves; unused slots are
+    empty; the stack bound is three entries per record level."""
+    import nostrum_pray_amd as P
+    f = P.Scene.spheres(3).build(0).flat()
+    L2, L4 = P.boxed_layout(f, 2), P.boxed_layout(f, 4)
+    assert np.array_equal(L2["refs"], L4["refs"])
+    r2, r4 = L2["rec"], L4["rec"]
+    E = 0xFFFFFFFF
+
+    def slots2(w):  # the binary layout's slots of inner binary word w
+        out = []
+        p = w >> 3
+        for ch in (0, 1):
+            cw = int(r2[p, 12 + ch])
+            box = r2[p, [ch, 2 + ch, 4 + ch, 6 + ch, 8 + ch, 10 + ch]]
+            if cw != E and cw & 3 != 3:
+                q = cw >> 3
+                out += [(int(r2[q, 12 + c]), r2[q, [c, 2 + c, 4 + c, 6 + c, 8 + c, 10 + c]]) for c in (0, 1)]
+            else:
+                out.append((cw, box))
+        return out
+
+    levels, leaves2, leaves4 = [0], [], []
+
+    def walk(w4, w2, d):
+        levels[0] = max(levels[0], d)
+        q = w4 >> 3
+        s = slots2(w2)
+        assert 2 <= len(s) <= 4
+        for j in range(4):
+            w = int(r4[q, 24 + j])
+            box4 = r4[q, [j, 4 + j, 8 + j, 12 + j, 16 + j, 20 + j]]
+            if j >= len(s):
+                assert w == E and (box4[[0, 2, 4]] == 0x7F800000).all() and (box4[[1, 3, 5]] == 0xFF800000).all()
+                continue
+            sw, sbox = s[j]
+            assert np.array_equal(box4, sbox)
+            if sw == E or sw & 3 == 3:
+                assert w == sw
+                if sw != E:
+                    leaves4.append(w)
+            else:
+                walk(w, sw, d + 1)
+
+    root2 = int(r2[0, 13])
+    walk(L4["root"], root2, 0)
+    assert L4["depth"] == 3 * (levels[0] + 1)
+
+    def leaves_of2(w):
+        if w == E:
+            return
+        if w & 3 == 3:
+            leaves2.append(w)
+            return
+        for c in (0, 1):
+            leaves_of2(int(r2[w >> 3, 12 + c]))
+
+    leaves_of2(root2)
+    assert sorted(leaves2) == sorted(leaves4) and len(leaves4) > 1000
+
+
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("which", ["spheres", "random", "cornell"])
-def test_boxed_twin_equals_bruteforce(which):
+def test_boxed_twin_equals_bruteforce(which, width):
     """The boxed node-step twin over the library's boxed layout of median
     trees gives the brute-force closest hit on every ray, camera and
     incoherent, and any-hit answers that threshold it, with fewer triangle
@@ -277,7 +343,7 @@ def test_boxed_twin_equals_bruteforce(which):
     s = {"spheres": lambda: P.Scene.spheres(1), "random": lambda: P.Scene.random(3, 5000),
          "cornell": lambda: P.Scene.cornell(7, 8000)}[which]()
     f = s.build(0).flat()
-    L = P.boxed_layout(f)
+    L = P.boxed_layout(f, width)
     rays = np.concatenate([s.prim_rays()[::41], O.gen_rays(13, P.scene_box_expanded(f["box"]), 6000)])
     for eps in (1e-4, 1e-9):
         fi, ft, fc = O.flat_trace(f, rays, 0, eps, counts=True, boxed=L)

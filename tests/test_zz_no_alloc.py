@@ -67,3 +67,43 @@ def test_noalloc_trace_equals_trace_device(ctx):
         for b in bufs:
             b.free()
         ctx.free_scene(sid)
+
+
+def test_noalloc_boxed_layout_equals_trace_device(ctx):
+    """The same over a median tree's boxed layout (boxed node steps, whose
+    stack depth is the boxed layout's, deeper than the uploaded tree's):
+    closest hits equal pray_trace_device's on two calls in a row, the second
+    with fewer rays."""
+    s = P.Scene.random(21, 60000)
+    t = s.build(0)
+    flat = t.flat()
+    sid = ctx.upload(t, shading=False)
+    bufs = []
+    try:
+        assert ctx.get_tuning(sid)["node_levels"] == 4
+        n1, n2 = 1 << 18, 5000
+        d_rays = ctx.alloc((n1, 6), np.float32)
+        ctx.gen_rays_device(77, P.scene_box_expanded(t.box()), n1, d_rays.ptr)
+        d_ri, d_rt = ctx.alloc(n1, np.int32), ctx.alloc(n1, np.float32)
+        d_ni, d_nt = ctx.alloc(n1, np.int32), ctx.alloc(n1, np.float32)
+        bufs = [d_rays, d_ri, d_rt, d_ni, d_nt]
+        say(f"no-alloc boxed: trace_device n={n1}")
+        ctx.trace_device(sid, d_rays.ptr, n1, d_ri.ptr, d_rt.ptr, P.TRACE_CLOSEST, 1e-9)
+        ctx.sync()
+        ri, rt = d_ri.numpy(), d_rt.numpy()
+        bi, _ = O.bf_closest(flat["a"], flat["ab"], flat["ac"], d_rays.numpy()[:2048], 1e-9)
+        assert first_diff(ri[:2048], bi) is None
+        for n in (n1, n2):
+            d_ni.upload(np.full(n1, -7, np.int32))
+            say(f"no-alloc boxed: trace_device_noalloc n={n}")
+            ctx.trace_device_noalloc(sid, d_rays.ptr, n, d_ni.ptr, d_nt.ptr, P.TRACE_CLOSEST, 1e-9)
+            ctx.sync()
+            gi, gt = d_ni.numpy(), d_nt.numpy()
+            assert first_diff(gi[:n], ri[:n]) is None, (n, first_diff(gi[:n], ri[:n]))
+            assert np.array_equal(gt[:n].view(np.uint32), rt[:n].view(np.uint32))
+            assert (gi[n:] == -7).all()
+        assert (ri >= 0).any() and (ri < 0).any()
+    finally:
+        for b in bufs:
+            b.free()
+        ctx.free_scene(sid)
