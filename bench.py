@@ -234,20 +234,38 @@ def main():
         src.free()
         dst.free()
 
-    # Work counts (untimed): nodes and leaf references per ray.
-    ctx.trace_counts_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, d_cnt.ptr, P.TRACE_CLOSEST, args.det_eps)
-    ctx.sync()
-    cnt = d_cnt.numpy().astype(np.int64).sum(0)
-    hits = int((d_idx.numpy() >= 0).sum())
-    d_cnt2 = d_cnt  # reused for the other tree's counts
+    # Work counts (untimed): nodes and leaf references per ray, of the
+    # uploaded KD tree's traversal (plane-form node steps) and of the node
+    # form the tuning runs.
     tun = ctx.get_tuning(sid)
-    # Algorithmic bytes of the traversal kernel: ray 24 + result 8 + 8 per
-    # node visited + 40 per triangle reference tested (4-byte index + a, ab,
-    # ac); the packed layout actually moves 48 bytes per reference.
-    nodes_per_ray = cnt[0] / n
-    refs_per_ray = cnt[1] / n
-    bytes_per_ray = 24 + 8 + 8 * nodes_per_ray + 40 * refs_per_ray
-    log(f"hits {hits}/{n}  nodes/ray {nodes_per_ray:.1f}  refs/ray {refs_per_ray:.1f}  B/ray {bytes_per_ray:.0f}")
+
+    def work_counts(mode, lim=None, plane_form=False):
+        if plane_form:
+            ctx.set_tuning(sid, node_levels=1)
+        ctx.trace_counts_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, d_cnt.ptr, mode, args.det_eps,
+                                d_limit2=lim)
+        ctx.sync()
+        if plane_form:
+            ctx.set_tuning(sid, **tun)
+        return d_cnt.numpy().astype(np.int64).sum(0) / n
+
+    kd_cnt = work_counts(P.TRACE_CLOSEST, plane_form=True)
+    hits = int((d_idx.numpy() >= 0).sum())
+    own_cnt = work_counts(P.TRACE_CLOSEST)
+    d_cnt2 = d_cnt  # reused for the other tree's counts
+    # Algorithmic bytes per ray of the traversal that runs: ray 24 + result
+    # 8 + the bytes of each node visited + 40 per triangle reference tested
+    # (4-byte index + a, ab, ac; the packed layout moves 48 bytes per
+    # reference).  A KD node is 8 bytes; the boxed node forms (node records 3
+    # and 4) run over a finer tree of their own whose children cost 28 bytes
+    # each (box and word).  The uploaded KD tree's own work per ray is
+    # reported beside.
+    boxed = tun["node_levels"] in (3, 4)
+    nodes_per_ray, refs_per_ray = own_cnt
+    bytes_per_ray = boxed_bytes(own_cnt) if boxed else kd_bytes(own_cnt)
+    log(f"hits {hits}/{n}  {'children' if boxed else 'nodes'}/ray {nodes_per_ray:.1f}  refs/ray {refs_per_ray:.1f}  "
+        f"B/ray {bytes_per_ray:.0f}  (KD tree: nodes/ray {kd_cnt[0]:.1f} refs/ray {kd_cnt[1]:.1f} "
+        f"B/ray {kd_bytes(kd_cnt):.0f})")
 
     def timed(mode, steps, warmup, lim=None, scene_id=sid, eps=args.det_eps):
         for _ in range(warmup):
@@ -292,11 +310,11 @@ def main():
     # Any-hit over the same stream (unlimited distance), reported beside with
     # its own work counts and kernel time.
     ms_any, _ = timed(P.TRACE_ANY, args.steps, 2, lim=d_lim.ptr)
-    ctx.trace_counts_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, d_cnt.ptr, P.TRACE_ANY, args.det_eps,
-                            d_limit2=d_lim.ptr)
-    ctx.sync()
-    acnt = d_cnt.numpy().astype(np.int64).sum(0) / n
-    any_bytes = 24 + 8 + 8 * acnt[0] + 40 * acnt[1]
+    acnt = work_counts(P.TRACE_ANY, d_lim.ptr, plane_form=True)
+    any_kd_bytes = kd_bytes(acnt)
+    if boxed:
+        acnt = work_counts(P.TRACE_ANY, d_lim.ptr)
+    any_bytes = boxed_bytes(acnt) if boxed else kd_bytes(acnt)
     any_kernel_ms = ms_any
     any_isolated_ms = kernel_time(P.TRACE_ANY, d_lim.ptr)
     any_achieved = any_bytes * n / (any_kernel_ms * 1e-3) / 1e9
@@ -430,19 +448,30 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            **({"traffic_GBs": round(traffic / (kernel_ms * 1e-3) / 1e9, 1),
+                "traffic_frac": round(traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)} if traffic else {}),
             "kernel": {2: "k_trace_q (closest hit, two tree levels per node step)",
-                       3: "k_trace (closest hit, boxed children pairs)"}.get(tun["node_levels"], "k_trace (closest hit)"),
+                       3: "k_trace (closest hit, boxed children pairs)",
+                       4: "k_trace (closest hit, 4-wide boxed records)"}.get(tun["node_levels"],
+                                                                            "k_trace (closest hit)"),
             "kernel_ms": round(kernel_ms, 4),
             "kernel_ms_events_per_kernel": round(isolated_ms, 4),
             "bytes_per_ray": round(float(bytes_per_ray), 1),
-            "nodes_per_ray": round(float(nodes_per_ray), 2),
+            "bytes_basis": ("24 + 8 + 28 per boxed child tested + 40 per reference" if boxed else
+                            "24 + 8 + 8 per node + 40 per reference"),
+            ("children_per_ray" if boxed else "nodes_per_ray"): round(float(nodes_per_ray), 2),
             "refs_per_ray": round(float(refs_per_ray), 2),
+            "kd_tree_traversal": {"nodes_per_ray": round(float(kd_cnt[0]), 2),
+                                  "refs_per_ray": round(float(kd_cnt[1]), 2),
+                                  "bytes_per_ray": round(kd_bytes(kd_cnt), 1)},
         },
         "any_hit": {"ms_per_step": round(ms_any, 4), "Mrays_per_s": round(n / (ms_any * 1e-3) / 1e6, 2),
                     "kernel_ms": round(any_kernel_ms, 4), "kernel_ms_events_per_kernel": round(any_isolated_ms, 4),
                     "bytes_per_ray": round(float(any_bytes), 1),
-                    "nodes_per_ray": round(float(acnt[0]), 2), "refs_per_ray": round(float(acnt[1]), 2),
-                    "achieved_GBs": round(any_achieved, 1), "frac": round(any_achieved / HBM_PEAK_GBS, 4)},
+                    ("children_per_ray" if boxed else "nodes_per_ray"): round(float(acnt[0]), 2),
+                    "refs_per_ray": round(float(acnt[1]), 2),
+                    "achieved_GBs": round(any_achieved, 1), "frac": round(any_achieved / HBM_PEAK_GBS, 4),
+                    "kd_tree_bytes_per_ray": round(any_kd_bytes, 1)},
         "reference_det_eps": ref_eps,
         "reference_parameter_tree": ref_tree,
         "other_trees": others,
@@ -861,6 +890,19 @@ def pmc_traffic(args, tree_tag, tun):
     return None
 
 
+def kd_bytes(cnt):
+    """Algorithmic bytes per ray of a KD traversal with (nodes, references)
+    per ray `cnt`: ray 24 + result 8 + 8 per node + 40 per reference."""
+    return 24 + 8 + 8 * float(cnt[0]) + 40 * float(cnt[1])
+
+
+def boxed_bytes(cnt):
+    """Bytes per ray of a boxed traversal with (children tested, references)
+    per ray `cnt`: ray 24 + result 8 + 28 per child (its box and word) + 40
+    per reference."""
+    return 24 + 8 + 28 * float(cnt[0]) + 40 * float(cnt[1])
+
+
 def parse_tuning(P, text):
     """--tuning's comma-separated values as set_tuning keywords."""
     return dict(zip(P.TUNING_FIELDS, [int(x) for x in text.split(",")]))
@@ -924,12 +966,14 @@ def ray_tracer_leg(P, ctx, args):
     inf = scene.info()
     w, h = inf["width"], inf["height"]
     cam, lights, bg, eps = scene.camera(), scene.lights(), inf["background"], float(inf["det_eps"])
+    # the frame loop fills the same host image and hit buffers every frame
+    frame_out = (np.empty((h, w, 3), np.uint8), np.empty(w * h, np.int32))
     for _ in range(max(1, args.warmup)):
-        rgb, hits = ctx.render_rt(sid, cam, w, h, lights, bg, eps, want_hits=True)
+        rgb, hits = ctx.render_rt(sid, cam, w, h, lights, bg, eps, want_hits=True, out=frame_out)
     times = []
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        rgb, hits = ctx.render_rt(sid, cam, w, h, lights, bg, eps, want_hits=True)
+        rgb, hits = ctx.render_rt(sid, cam, w, h, lights, bg, eps, want_hits=True, out=frame_out)
         times.append((time.perf_counter() - t0) * 1e3)
     ms = float(np.median(times))
     rays = w * h + int((hits >= 0).sum()) * len(lights)
@@ -945,7 +989,11 @@ def ray_tracer_leg(P, ctx, args):
     cnt = d_cnt.numpy().astype(np.int64).sum(0)
     assert np.array_equal(d_idx.numpy(), hits)
     nodes_per_ray, refs_per_ray = cnt[0] / npix, cnt[1] / npix
-    bytes_per_ray = 24 + 8 + 8 * nodes_per_ray + 40 * refs_per_ray
+    # the median tree's own plane-form traversal tests ~150 nodes and ~740
+    # triangles per ray; the boxed forms (its default) are priced by their
+    # own records: 28 bytes per child tested
+    boxed = tun["node_levels"] in (3, 4)
+    bytes_per_ray = (boxed_bytes if boxed else kd_bytes)(cnt / npix)
     for _ in range(max(1, args.warmup)):
         ctx.trace_device(sid, d_rays.ptr, npix, d_idx.ptr, d_t.ptr, P.TRACE_CLOSEST, eps)
     ctx.sync()
@@ -983,11 +1031,15 @@ def ray_tracer_leg(P, ctx, args):
         "ms_per_frame": round(ms, 4), "rays_per_frame": rays, "tuning": tun,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": {3: "k_trace (closest hit, boxed children pairs)",
+                     "kernel": {4: "k_trace (closest hit, 4-wide boxed records)",
+                                3: "k_trace (closest hit, boxed children pairs)",
                                 2: "k_trace_q (closest hit, two tree levels per node step)"}.get(
                                     tun["node_levels"], "k_trace (closest hit)"),
                      "kernel_ms": round(kernel_ms, 4), "bytes_per_ray": round(float(bytes_per_ray), 1),
-                     "nodes_per_ray": round(float(nodes_per_ray), 2), "refs_per_ray": round(float(refs_per_ray), 2),
+                     "bytes_basis": ("24 + 8 + 28 per boxed child tested + 40 per reference" if boxed else
+                                     "24 + 8 + 8 per node + 40 per reference"),
+                     ("children_per_ray" if boxed else "nodes_per_ray"): round(float(nodes_per_ray), 2),
+                     "refs_per_ray": round(float(refs_per_ray), 2),
                      "note": "the scene's records (1.3 MB) stay in L2; the launch of 262,144 rays is bound by "
                              "its longest rays' chains of dependent loads, not by HBM"},
         "cli": cli, "cpu_baseline": cpu}

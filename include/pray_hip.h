@@ -87,9 +87,9 @@ PRAY_API int pray_mem_ok(pray_ctx *ctx, uint64_t n_tris, uint64_t n_refs, uint64
  *   tri_emit   n_tris bytes (1 = emissive), may be NULL
  *   root_box   6 floats: min xyz, max xyz of the root cell
  *   max_depth  root-to-leaf depth of the tree (<= 64)
- * n_tris == 0 is a valid empty scene (every ray misses).  For a tree with a
- * children pair whose two clips differ (a median tree) the upload also makes
- * the boxed layouts of pray_boxed_layout and selects the 4-wide one
+ * n_tris == 0 is a valid empty scene (every ray misses).  The upload also
+ * makes the boxed layouts of pray_boxed_layout; for a tree with a children
+ * pair whose two clips differ (a median tree) it selects the 4-wide one
  * (node_levels 4). */
 PRAY_API int pray_upload_scene(pray_ctx *ctx, const uint32_t *nodes, uint32_t n_nodes, const uint32_t *leaf_refs,
                       uint32_t n_refs, const float *tri_a, const float *tri_ab, const float *tri_ac,
@@ -99,15 +99,18 @@ PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
 
 /* The boxed layout the upload makes for boxed node steps (no reference
  * counterpart; host code only, no device or context needed).  For a valid
- * tree (the arrays of pray_upload_scene) with a children pair whose two clips
- * differ: every inner node becomes a pair record of its two children's full
- * boxes, every leaf of more than 4 references is split further (triangles
+ * tree (the arrays of pray_upload_scene): every inner node becomes a pair
+ * record of its two children's boxes, every leaf of more than 4 references
+ * is split further (triangles
  * sorted by centroid along the longest axis of their centroids' bounds, ties
  * by index, halved until no leaf holds more), and the leaves' runs go to a
  * reference array of their own (brefs: n_brefs words, the last entry of each
  * run | 0x80000000).  A box is the union of its triangles' boxes, vertices a,
- * a + ab, a + ac summed in double and rounded outward; {+inf, -inf} when
- * empty.  A leaf word is (first entry of its run in brefs) << 2 | 3;
+ * a + ab, a + ac summed in double and rounded outward, cut to the node's
+ * cell: where a children pair's two clips are equal (a KD split plane), child
+ * 0's cell ends at that plane on the parent's axis and child 1's begins
+ * there; a pair with two different clips (a median tree's) cuts nothing.
+ * {+inf, -inf} when empty.  A leaf word is (first entry of its run in brefs) << 2 | 3;
  * 0xFFFFFFFF is an empty leaf; an inner word is (record index) << 3.
  *   width 2  (node_levels 3) records of 16 uint32: per axis x, y, z {child 0
  *            lower, child 1 lower, child 0 upper, child 1 upper} (float
@@ -122,9 +125,9 @@ PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
  *            receives the root word and *depth the stack entries a lane may
  *            need (three per record level).
  * *n_records receives the record count; rec (16 or 32 * n_records words) and
- * brefs are filled when not NULL.  PRAY_ERR_UNSUPPORTED for a tree whose
- * children pairs all have equal clips (it keeps its planes) or whose layout
- * would be too deep, PRAY_ERR_INVALID for a malformed one. */
+ * brefs are filled when not NULL.  PRAY_ERR_UNSUPPORTED for a tree of fewer
+ * than two levels or whose layout would be too deep, PRAY_ERR_INVALID for a
+ * malformed one. */
 PRAY_API int pray_boxed_layout(const uint32_t *nodes, uint32_t n_nodes, const uint32_t *leaf_refs, uint32_t n_refs,
                                const float *tri_a, const float *tri_ab, const float *tri_ac, uint32_t n_tris,
                                uint32_t max_depth, uint32_t width, uint32_t *n_records, uint32_t *n_brefs,
@@ -190,14 +193,14 @@ PRAY_API int pray_scene_layout(pray_ctx *ctx, int scene_id, uint64_t *sizes, uin
  *                  children pairs have equal clips -- the SAH trees -- whose
  *                  records the upload or GPU build makes; PRAY_ERR_UNSUPPORTED
  *                  for other trees); 3: it takes one level from a 64-byte
- *                  record of both children's full boxes, over the boxed layout
- *                  of pray_boxed_layout (packed layout, and only for trees
- *                  with a children pair whose clips differ -- median trees,
- *                  whose children's boxes overlap -- whose layout the upload
- *                  makes; PRAY_ERR_UNSUPPORTED for other trees); 4: it takes
- *                  two binary levels from a 128-byte record of up to four
- *                  slots' boxes, over the layout's 4-wide form (the same
- *                  trees; the default for them)
+ *                  record of both children's boxes, over the boxed layout of
+ *                  pray_boxed_layout (packed layout; every tree the upload
+ *                  takes, PRAY_ERR_UNSUPPORTED for trees built on the
+ *                  device); 4: it takes two binary levels from a 128-byte
+ *                  record of up to four slots' boxes, over the layout's
+ *                  4-wide form (the same trees; the default for trees with
+ *                  a children pair whose clips differ -- median trees, whose
+ *                  children's boxes overlap)
  * pray_set_tuning: -1 keeps a value.  Calls of one context must not overlap
  * (traces share its ray counter and stack spill area). */
 #define PRAY_TUNING_FIELDS 8

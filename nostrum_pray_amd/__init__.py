@@ -586,12 +586,26 @@ class Context:
         self._check(self._lib.pray_camera_rays(self._h, _p(cam16), width, height, _p(out)))
         return out
 
-    def render_rt(self, sid, cam16, width, height, lights, background, det_eps=REFERENCE_DET_EPS, want_hits=False):
+    def render_rt(self, sid, cam16, width, height, lights, background, det_eps=REFERENCE_DET_EPS, want_hits=False,
+                  out=None):
+        """One ray-traced frame (pray_render_rt): the 8-bit image, and with
+        want_hits the primary rays' hit indices.  `out` = (rgb, hits) arrays
+        to fill instead of new ones, as a frame loop reuses its buffers."""
         cam16 = np.ascontiguousarray(cam16, np.float32)
         lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 6)
         bg = np.ascontiguousarray(background, np.float32).reshape(3)
-        rgb = np.empty((height, width, 3), np.uint8)
-        hits = np.empty(width * height, np.int32) if want_hits else None
+        if out is None:
+            rgb = np.empty((height, width, 3), np.uint8)
+            hits = np.empty(width * height, np.int32) if want_hits else None
+        else:
+            rgb, hits = out
+            if not (rgb.dtype == np.uint8 and rgb.size == 3 * width * height and rgb.flags.c_contiguous):
+                raise ValueError("out rgb must be a contiguous uint8 array of height x width x 3")
+            if want_hits and not (hits is not None and hits.dtype == np.int32 and hits.size == width * height
+                                  and hits.flags.c_contiguous):
+                raise ValueError("out hits must be a contiguous int32 array of width x height")
+            if not want_hits:
+                hits = None
         self._check(self._lib.pray_render_rt(self._h, sid, _p(cam16), width, height, _p(lights), lights.shape[0],
                                              _p(bg), det_eps, _p(rgb), _p(hits)))
         return (rgb, hits) if want_hits else rgb
@@ -783,12 +797,12 @@ class DeviceBuffer:
 
 
 def boxed_layout(flat, width=2):
-    """The boxed layout the upload makes for a median tree's boxed node steps
+    """The boxed layout the upload makes for boxed node steps
     (pray_boxed_layout; host code, no device): {"rec": records x 16 (width 2)
     or x 32 (width 4) uint32, "refs": leaf runs, "root": root word,
     "depth": root-to-leaf edges (width 2) or stack entries (width 4),
-    "width"}.  PrayError (UNSUPPORTED) for a tree whose children pairs all
-    have equal clips."""
+    "width"}.  PrayError (UNSUPPORTED) for a tree of fewer than two levels
+    or whose layout would be too deep."""
     L = libraries()[0]
     nodes = np.ascontiguousarray(flat["nodes"], np.uint32).reshape(-1)
     refs = np.ascontiguousarray(flat["refs"], np.uint32)

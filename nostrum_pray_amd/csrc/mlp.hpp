@@ -343,7 +343,12 @@ static __global__ __launch_bounds__(256) void k_mlp_train(const bf16x8 *__restri
     const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
     if (b * 16 >= A.n_pad) return;
     const uint32_t i = b * 16 + j;
-    const bool ok = i < A.n;
+    // A row without a target (w == 0: a training pixel whose path did not
+    // reach the cache this frame) holds whatever an earlier frame or call
+    // left in the feature buffer.  It is trained on zero features: its
+    // output gradient is zero, but stale activations that are not finite
+    // would still turn its zero terms of the weight gradient into NaN.
+    const bool ok = i < A.n && A.targets[i].w != 0.f;
     float f[N_FEAT];
 #pragma unroll
     for (int k = 0; k < N_FEAT; ++k) f[k] = ok ? A.feats[N_FEAT * (size_t)i + k] : 0.f;

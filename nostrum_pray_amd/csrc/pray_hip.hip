@@ -470,9 +470,9 @@ int build_quads(pray_ctx *c, SceneRec &sc, hipStream_t st) {
 
 // ---------------------------------------------------------- boxed layout --
 // The layout of boxed node steps (DevScene::boxes, DevScene::brecs), made
-// for trees with a children pair whose two clips differ (median trees: an
-// object partition whose children's boxes overlap).  It has its own pair and
-// leaf-reference index spaces:
+// for every uploaded tree (the default node steps of median trees, an
+// object partition whose children's boxes overlap; selectable for SAH
+// trees).  It has its own pair and leaf-reference index spaces:
 // - Every inner node of the uploaded tree becomes a pair record of its two
 //   children (16 words: per axis {child 0 lower, child 1 lower, child 0
 //   upper, child 1 upper}, then the children's words); pair 0 holds an empty
@@ -486,7 +486,17 @@ int build_quads(pray_ctx *c, SceneRec &sc, hipStream_t st) {
 // - A child's box is the union of the boxes of the triangles its subtree
 //   references, with the vertices a, a + ab, a + ac summed in double and the
 //   bounds rounded outward to float, so that it holds the triangle the
-//   intersection test sees; an empty subtree gets {+inf, -inf}.
+//   intersection test sees, cut to the child's cell: the region the split
+//   planes on the path from the root bound (where a children pair's two
+//   clips are equal, child 0 lies at or below that plane on its parent's
+//   axis and child 1 at or above it).  A KD tree references a triangle from
+//   the leaves on both sides of every plane it crosses, and the two sides
+//   are cut at the same float, so each point of the triangle lies in the
+//   cut box of a leaf that references it; without the cut, a leaf's box
+//   would reach as far as its largest triangle beyond the cell.  A pair
+//   with two different clips (a median tree's object partition, each
+//   triangle referenced once) cuts nothing.  An empty subtree (or cut) gets
+//   {+inf, -inf}.
 // The traversal's closest hit does not depend on the tree (it is the
 // lexicographic minimum of (t, index) over all triangles), so this finer
 // tree gives the same hits as the uploaded one.
@@ -499,6 +509,20 @@ struct Bounds {
     float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     void grow(const Bounds &o) {
         for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], o.lo[a]), hi[a] = std::max(hi[a], o.hi[a]);
+    }
+    // Cuts this box to `cell`; a box left empty becomes {+inf, -inf}.
+    void cut(const Bounds &cell) {
+        bool empty = false;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::max(lo[a], cell.lo[a]), hi[a] = std::min(hi[a], cell.hi[a]);
+            empty = empty || !(lo[a] <= hi[a]);
+        }
+        if (empty) *this = Bounds{};
+    }
+    static Bounds everywhere() {
+        Bounds b;
+        for (int a = 0; a < 3; ++a) b.lo[a] = -INFINITY, b.hi[a] = INFINITY;
+        return b;
     }
 };
 
@@ -541,9 +565,9 @@ struct BoxedBuilder {
         }
         r[12 + ch] = w;
     }
-    // A leaf of the n triangles at t, split while over BOX_LEAF: its word
-    // and box.
-    uint32_t leaf(uint32_t *t, uint32_t n, uint32_t d, Bounds &box) {
+    // A leaf of the n triangles at t in `cell`, split while over BOX_LEAF:
+    // its word and box.
+    uint32_t leaf(uint32_t *t, uint32_t n, uint32_t d, const Bounds &cell, Bounds &box) {
         box = Bounds{};
         if (n <= BOX_LEAF) {
             depth = std::max(depth, d);
@@ -552,6 +576,7 @@ struct BoxedBuilder {
                 brefs.push_back(t[j] | (j + 1 == n ? prt::REF_LAST : 0u));
                 box.grow(tri_box(t[j]));
             }
+            box.cut(cell);
             return (k << 2) | prt::LEAF_TAG;
         }
         double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -569,16 +594,17 @@ struct BoxedBuilder {
         for (uint32_t j = 0; j < n; ++j) t[j] = key[j].second;
         const uint32_t p = new_pair();
         Bounds b0, b1;
-        const uint32_t w0 = leaf(t, n / 2, d + 1, b0);
-        const uint32_t w1 = leaf(t + n / 2, n - n / 2, d + 1, b1);
+        const uint32_t w0 = leaf(t, n / 2, d + 1, cell, b0);
+        const uint32_t w1 = leaf(t + n / 2, n - n / 2, d + 1, cell, b1);
         set_child(p, 0, w0, b0);
         set_child(p, 1, w1, b1);
         box = b0;
         box.grow(b1);
         return (2 * p) << 2;
     }
-    // Node i of the uploaded tree: its word in this layout and its box.
-    uint32_t node(uint32_t i, uint32_t d, Bounds &box) {
+    // Node i of the uploaded tree, whose cell is `cell`: its word in this
+    // layout and its box.
+    uint32_t node(uint32_t i, uint32_t d, const Bounds &cell, Bounds &box) {
         const uint32_t w = nodes[2 * (size_t)i + 1];
         if ((w & 3u) == prt::LEAF_TAG) {
             uint32_t k = w >> 2;
@@ -592,12 +618,19 @@ struct BoxedBuilder {
                 if (refs[k] & prt::REF_LAST) break;
                 ++k;
             }
-            return leaf(run.data(), (uint32_t)run.size(), d, box);
+            return leaf(run.data(), (uint32_t)run.size(), d, cell, box);
         }
-        const uint32_t p = new_pair();
+        const uint32_t p = new_pair(), axis = w & 3u, lc = w >> 2;
+        Bounds c0 = cell, c1 = cell;
+        if (nodes[2 * (size_t)lc] == nodes[2 * (size_t)lc + 2]) {
+            float plane;
+            std::memcpy(&plane, &nodes[2 * (size_t)lc], 4);
+            c0.hi[axis] = std::min(c0.hi[axis], plane);
+            c1.lo[axis] = std::max(c1.lo[axis], plane);
+        }
         Bounds b0, b1;
-        const uint32_t w0 = node(w >> 2, d + 1, b0);
-        const uint32_t w1 = node((w >> 2) + 1, d + 1, b1);
+        const uint32_t w0 = node(lc, d + 1, c0, b0);
+        const uint32_t w1 = node(lc + 1, d + 1, c1, b1);
         set_child(p, 0, w0, b0);
         set_child(p, 1, w1, b1);
         box = b0;
@@ -606,21 +639,21 @@ struct BoxedBuilder {
     }
 };
 
-// Whether the uploaded tree takes the boxed layout: some children pair after
-// pair 0 has two different clips.
+// Whether the uploaded tree is an object partition (a median tree): some
+// children pair after pair 0 has two different clips.
 bool unequal_clips(const uint32_t *nodes, uint32_t n_nodes) {
     for (uint32_t p = 1; 2 * p + 1 < n_nodes; ++p)
         if (nodes[4 * (size_t)p] != nodes[4 * (size_t)p + 2]) return true;
     return false;
 }
 
-// Builds the layout of a validated tree (the caller checked unequal_clips).
+// Builds the layout of a validated tree.
 int make_boxed_layout(pray_ctx *c, const uint32_t *nodes, const uint32_t *refs, const float *A, const float *AB,
                       const float *AC, BoxedBuilder &B) {
     B.nodes = nodes, B.refs = refs, B.A = A, B.AB = AB, B.AC = AC;
     const uint32_t p0 = B.new_pair();
     Bounds root;
-    const uint32_t rw = B.node(1, 0, root);
+    const uint32_t rw = B.node(1, 0, Bounds::everywhere(), root);
     B.set_child(p0, 0, (prt::LEAF_EMPTY << 2) | prt::LEAF_TAG, Bounds{});
     B.set_child(p0, 1, rw, root);
     if (B.rec.size() / 16 >= (1u << 29) || B.brefs.size() >= prt::LEAF_EMPTY)
@@ -713,7 +746,7 @@ int build_boxes(pray_ctx *c, SceneRec &sc, const uint32_t *nodes, uint32_t n_nod
     d.boxes4 = nullptr;
     d.brecs = nullptr;
     d.box_depth = d.box4_depth = d.box4_root = 0;
-    if (d.n_tris == 0 || n_nodes < 4 || !unequal_clips(nodes, n_nodes)) return PRAY_OK;
+    if (d.n_tris == 0 || n_nodes < 4) return PRAY_OK;
     BoxedBuilder B;
     if (int rc = make_boxed_layout(c, nodes, refs, A, AB, AC, B)) return rc;
     // the 4-wide form too, unless its stack would be too deep (then boxed
@@ -945,9 +978,9 @@ int pray_upload_scene(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, cons
         }
         if ((rc = build_quads(c, *sc, c->stream))) goto bad;
         if ((rc = build_boxes(c, *sc, nodes, n_nodes, leaf_refs, tri_a, tri_ab, tri_ac, c->stream))) goto bad;
-        // boxed node steps by default where the tree has them, 4-wide where
-        // that form exists
-        if (d.boxes) sc->tune.node_levels = d.boxes4 ? 4 : 3;
+        // boxed node steps by default for median trees, 4-wide where that
+        // form exists
+        if (d.boxes && unequal_clips(nodes, n_nodes)) sc->tune.node_levels = d.boxes4 ? 4 : 3;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         rc = fail(c, PRAY_ERR_HIP, "scene upload failed");
@@ -982,7 +1015,7 @@ int pray_boxed_layout(const uint32_t *nodes, uint32_t n_nodes, const uint32_t *l
         return PRAY_ERR_INVALID;
     if (max_depth > MAX_TREE_DEPTH) return PRAY_ERR_UNSUPPORTED;
     if (int v = validate_tree(nullptr, nodes, n_nodes, leaf_refs, n_refs, n_tris, max_depth)) return v;
-    if (n_nodes < 4 || !unequal_clips(nodes, n_nodes)) return PRAY_ERR_UNSUPPORTED;
+    if (n_nodes < 4) return PRAY_ERR_UNSUPPORTED;
     BoxedBuilder B;
     if (int rc = make_boxed_layout(nullptr, nodes, leaf_refs, tri_a, tri_ab, tri_ac, B)) return rc;
     *n_brefs = (uint32_t)B.brefs.size();

@@ -31,6 +31,11 @@ def test_gpu_build_equals_oracle(ctx, method, name, leaf_size):
         assert np.array_equal(g["box"].view(np.uint32), o["box"].view(np.uint32))
         assert np.array_equal(g["refs"], o["refs"])
         assert first_diff(g["nodes"], o["nodes"]) is None, first_diff(g["nodes"], o["nodes"])
+        # the boxed layout is made on the host at upload, so a tree built on
+        # the device has none
+        for nl in (3, 4):
+            with pytest.raises(P.PrayError):
+                ctx.set_tuning(sid, node_levels=nl)
         rays = O.gen_rays(23, P.scene_box_expanded(o["box"]), 5000)
         say(f"gpu build trace {name}")
         gi, gt = ctx.trace(sid, rays, P.TRACE_CLOSEST, 1e-9)

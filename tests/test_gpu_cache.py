@@ -209,3 +209,39 @@ def test_cache_frames_without_graphs_equal_graph_replays(ctx, spf):
     assert np.array_equal(s0.view(np.uint32), s1.view(np.uint32)), detail
     assert np.array_equal(w0.view(np.uint32), w1.view(np.uint32)), detail
     assert np.array_equal(h0.view(np.uint32), h1.view(np.uint32)), detail
+
+
+def test_cache_frames_ignore_stale_training_rows(ctx):
+    """A frame trains on its training pixels' rows of the feature buffer;
+    the rows of pixels whose paths did not reach the cache keep whatever an
+    earlier call left there and must not reach the weights.  Two caches of
+    one seed, whose buffers first hold a NaN batch and a finite batch (a
+    gradient evaluation loads the batch and leaves the weights alone), then
+    render bitwise alike, with finite weights, graphs and no graphs."""
+    s, t, cam = cornell_small()
+    sid = ctx.upload(t)
+    bg = s.info()["background"]
+    n = 1 << 12
+    try:
+        for graphs in (True, False):
+            ctx.use_graphs(graphs)
+            out = []
+            for fill in (np.nan, 0.25):
+                c = P.Cache(ctx, seed=5, max_batch=n)
+                try:
+                    say(f"stale rows {fill}, graphs {graphs}")
+                    c.gradient(np.full((n, P.CACHE_FEATURES), fill, np.float32), np.zeros((n, 3), np.float32))
+                    hdr, _, stats = c.render_pt(sid, cam, 96, 72, 4, 8, 3, 11, bg, 1e-9, train_paths=n,
+                                                cache_bounce=1, min_steps=2, samples_per_frame=2)
+                    out.append((hdr, stats, c.params()))
+                finally:
+                    c.close()
+            (h0, s0, w0), (h1, s1, w1) = out
+            assert np.isfinite(w0).all() and np.isfinite(s0).all(), (graphs, s0.tolist())
+            assert s0[:, 1].any()
+            assert np.array_equal(w0.view(np.uint32), w1.view(np.uint32)), (graphs, int(np.count_nonzero(w0 != w1)))
+            assert np.array_equal(s0.view(np.uint32), s1.view(np.uint32)), (graphs, s0.tolist(), s1.tolist())
+            assert np.array_equal(h0.view(np.uint32), h1.view(np.uint32)), graphs
+    finally:
+        ctx.use_graphs(True)
+        ctx.free_scene(sid)

@@ -63,11 +63,25 @@ def test_rt_spheres_equal_oracle(ctx, seed):
         say(f"rt spheres {seed}")
         img = ctx.render_rt(sid, s.camera(), inf["width"], inf["height"], s.lights(), inf["background"],
                             P.REFERENCE_DET_EPS)
+        # a frame loop's reused buffers: filled over whatever they held
+        out = (np.full((inf["height"], inf["width"], 3), 7, np.uint8),
+               np.full(inf["width"] * inf["height"], 5, np.int32))
+        for _ in range(2):
+            img2, hits2 = ctx.render_rt(sid, s.camera(), inf["width"], inf["height"], s.lights(), inf["background"],
+                                        P.REFERENCE_DET_EPS, want_hits=True, out=out)
+            assert img2 is out[0] and hits2 is out[1]
+            assert np.array_equal(img2, img)
+        with pytest.raises(ValueError):
+            ctx.render_rt(sid, s.camera(), inf["width"], inf["height"], s.lights(), inf["background"],
+                          P.REFERENCE_DET_EPS, want_hits=True, out=(out[0][:-1], out[1]))
     finally:
         ctx.free_scene(sid)
     ref = O.render_rt(f, s.camera(), inf["width"], inf["height"], s.lights(), inf["background"], 1e-4)
     assert np.array_equal(img, ref), f"{np.count_nonzero(np.any(img != ref, axis=2))} pixels differ"
     assert img.std() > 10  # not a blank image
+    rays = s.prim_rays()
+    bi, _ = O.bf_closest(f["a"], f["ab"], f["ac"], rays[::97], 1e-4)
+    assert np.array_equal(hits2[::97], bi)
 
 
 def test_rt_multi_light_negative_lambert_quirk(ctx):

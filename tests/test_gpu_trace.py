@@ -49,8 +49,9 @@ def scene(name):
 # lanes wait).  Two levels per step need a tree whose children pairs have
 # equal clips (every SAH tree; the terrain's median tree too, whose children
 # meet on grid lines); other trees refuse them.  Node records 3 (boxed
-# children pairs) and 4 (their 4-wide form) exist for exactly those other
-# trees (median trees whose children overlap) and are refused for the rest.
+# children pairs) and 4 (their 4-wide form) run over the boxed layout the
+# upload makes for every tree (the default for median trees whose children
+# overlap; for SAH trees with the boxes cut to the KD cells).
 SETTINGS = [(P.LAYOUT_PACKED, 6, 1, 2, 16, 0, 2),
             (P.LAYOUT_PACKED, 6, 1, 2, 16, 0, 3), (P.LAYOUT_PACKED, 8, 4, 1, 24, 16, 3),
             (P.LAYOUT_PACKED, 5, 1, 1, 1, 1, 3), (P.LAYOUT_PACKED, 16, 2, 2, 48, 0, 3),
@@ -77,10 +78,10 @@ def equal_clips(flat):
 
 def set_or_refuse(ctx, sid, name, layout, ring, wg, ilp, fm, lb, nl):
     """Applies a setting; False when the scene's tree has no two-level
-    records (nl 2) or no boxed pairs (nl 3), after checking that the library
-    refuses exactly those trees."""
+    records (nl 2), after checking that the library refuses exactly those
+    trees."""
     eq = equal_clips(scene(name)[2])
-    if (nl == 2 and not eq) or (nl >= 3 and eq):
+    if nl == 2 and not eq:
         with pytest.raises(P.PrayError):
             ctx.set_tuning(sid, layout=P.LAYOUT_PACKED, node_levels=nl)
         return False
@@ -197,18 +198,20 @@ def test_work_counts_match_cpu_twin(ctx, lb, nl):
 @pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("lb", [0, 16])
 @pytest.mark.parametrize("name,rays_of", [("spheres-median", "camera"), ("spheres-median", "incoherent"),
-                                          ("random-median", "incoherent")])
+                                          ("random-median", "incoherent"), ("spheres-sah", "camera"),
+                                          ("terrain256-sah", "incoherent"), ("cornell20k-sah", "incoherent")])
 def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb, width):
     """Boxed node steps (node records 3, and 4 over the 4-wide form, the
-    default for median trees whose children overlap): per-ray node and
-    triangle counts and hits equal the CPU twin's over the same boxed layout,
-    with and without leaf batching; the twin's hits equal the plane-form
-    traversal's and brute force."""
+    default for median trees whose children overlap; over SAH trees with the
+    boxes cut to the KD cells): per-ray node and triangle counts and hits
+    equal the CPU twin's over the same boxed layout, with and without leaf
+    batching; the twin's hits equal the plane-form traversal's and brute
+    force."""
     s, t, flat = scene(name)
     rays = s.prim_rays() if rays_of == "camera" else O.gen_rays(4, P.scene_box_expanded(flat["box"]), 1 << 15)
     sid = ctx.upload(t)
     try:
-        assert ctx.get_tuning(sid)["node_levels"] == 4
+        assert ctx.get_tuning(sid)["node_levels"] == (1 if equal_clips(flat) else 4)
         ctx.set_tuning(sid, leaf_batch=lb, node_levels=3 if width == 2 else 4)
         say(f"counts boxed {name} {rays_of} leaf_batch={lb} width={width}")
         n = rays.shape[0]
@@ -236,13 +239,17 @@ def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb, width):
 def test_node_records_by_tree(ctx):
     """The upload's default node records: the 4-wide boxed layout for the
     median tree whose children overlap, children pairs for SAH trees and for
-    the terrain's median tree (equal clips)."""
+    the terrain's median tree (equal clips); every uploaded tree takes the
+    boxed forms when asked."""
     for name, want in [("spheres-median", 4), ("spheres-sah", 1), ("terrain256-median", 1)]:
         _, t, flat = scene(name)
         sid = ctx.upload(t)
         try:
             assert ctx.get_tuning(sid)["node_levels"] == want, name
             assert (want == 4) == (not equal_clips(flat))
+            for nl in (3, 4, 1):
+                ctx.set_tuning(sid, node_levels=nl)
+                assert ctx.get_tuning(sid)["node_levels"] == nl
         finally:
             ctx.free_scene(sid)
 
@@ -251,7 +258,8 @@ def test_node_records_by_tree(ctx):
                                              ("spheres-sah", 5, 32, 1), ("spheres-sah", 16, 8, 1),
                                              ("spheres-sah", 5, 0, 2), ("spheres-sah", 16, 24, 2),
                                              ("spheres-median", 5, 0, 3), ("spheres-median", 16, 24, 3),
-                                             ("spheres-median", 4, 0, 4), ("spheres-median", 16, 24, 4)])
+                                             ("spheres-median", 4, 0, 4), ("spheres-median", 16, 24, 4),
+                                             ("spheres-sah", 8, 0, 4), ("spheres-sah", 5, 16, 3)])
 def test_launch_sizes(ctx, name, ring, lb, nl):
     """Traces over ray counts around the wave, the 64-ray chunks and the 64
     chunk counters (partial last chunks, fewer waves than counters), one after

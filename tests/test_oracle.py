@@ -232,8 +232,8 @@ def test_boxed_layout_structure():
     each leaf), its leaves hold at most 4, every child box is the union of its
     triangles' boxes with the vertices a + ab, a + ac summed exactly (float64)
     and rounded outward -- the largest float at or below the exact minimum and
-    the smallest at or above the exact maximum -- and SAH trees (equal clips)
-    get none."""
+    the smallest at or above the exact maximum (a median tree's children
+    pairs have two clips, so nothing is cut to a cell)."""
     import nostrum_pray_amd as P
     s = P.Scene.spheres(3)
     f = s.build(0).flat()
@@ -263,8 +263,92 @@ def test_boxed_layout_structure():
     assert max(runs) <= 4 and sum(runs) >= len(refs)
     # the root box is the scene's box up to that rounding
     assert np.allclose(fb[0, [1, 5, 9, 3, 7, 11]], f["box"], rtol=1e-6, atol=1e-6)
-    with pytest.raises(P.PrayError):
-        P.boxed_layout(s.build(1).flat())
+
+
+def rounded_box(tmin, tmax):
+    """Float bounds of exact (float64) bounds: the largest float at or below
+    each minimum and the smallest at or above each maximum."""
+    lo, hi = tmin.astype(np.float32), tmax.astype(np.float32)
+    lo = np.where(lo.astype(np.float64) > tmin, np.nextafter(lo, np.float32(-np.inf)), lo)
+    hi = np.where(hi.astype(np.float64) < tmax, np.nextafter(hi, np.float32(np.inf)), hi)
+    return lo, hi
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_boxed_layout_cut_to_kd_cells(kind):
+    """The boxed layout of a SAH KD tree (equal clips: split planes): walking
+    the uploaded tree from the root with each node's cell (child 0 at or
+    below its pair's plane on the parent's axis, child 1 at or above it),
+    every pair of the layout made for an inner node holds its children's
+    boxes as the recursion over the uploaded tree gives them: a leaf's box is
+    the rounded box of its triangles cut to its cell (empty -> {+inf, -inf}),
+    an inner node's the union of its children's.  Cuts happen (the boxes are
+    smaller than their triangles'), and never cut across a plane the wrong
+    way (a child 0 box never reaches above its plane)."""
+    import nostrum_pray_amd as P
+    s = P.Scene.cornell(2, 6000)
+    f = s.build(kind).flat()
+    L = P.boxed_layout(f)
+    rec = L["rec"]
+    fb = rec[:, :12].copy().view(np.float32)
+    nodes = np.asarray(f["nodes"], np.uint32).reshape(-1)
+    refs = np.asarray(f["refs"], np.uint32)
+    a, ab, ac = (f[k].astype(np.float64).reshape(-1, 3) for k in ("a", "ab", "ac"))
+    v = np.stack([a, a + ab, a + ac], 1)
+    tlo, thi = rounded_box(v.min(1), v.max(1))
+    E = 0xFFFFFFFF
+    empty = (np.full(3, np.inf, np.float32), np.full(3, -np.inf, np.float32))
+    stats = {"pairs": 0, "cut": 0, "planes": 0}
+
+    def leaf_tris(w):
+        k, out = int(w) >> 2, []
+        if k == 0x3FFFFFFF:
+            return out
+        while True:
+            out.append(int(refs[k]) & 0x7FFFFFFF)
+            if refs[k] & 0x80000000:
+                return out
+            k += 1
+
+    def box_of(i, clo, chi, bw):
+        """Expected box of uploaded node i in cell [clo, chi]; bw is its word
+        in the layout, whose pairs are checked on the way."""
+        w = int(nodes[2 * i + 1])
+        if w & 3 == 3:
+            t = leaf_tris(w)
+            if not t:
+                return empty
+            lo = np.maximum(tlo[t].min(0), clo)
+            hi = np.minimum(thi[t].max(0), chi)
+            if (tlo[t].min(0) < clo).any() or (thi[t].max(0) > chi).any():
+                stats["cut"] += 1
+            return (lo, hi) if (lo <= hi).all() else empty
+        axis, lc = w & 3, w >> 2
+        p = bw >> 3
+        c0lo, c0hi, c1lo, c1hi = clo.copy(), chi.copy(), clo.copy(), chi.copy()
+        assert nodes[2 * lc] == nodes[2 * lc + 2]  # a SAH tree's split plane
+        plane = np.float32(nodes[2 * lc : 2 * lc + 1].view(np.float32)[0])
+        c0hi[axis] = min(c0hi[axis], plane)
+        c1lo[axis] = max(c1lo[axis], plane)
+        stats["planes"] += 1
+        b0 = box_of(lc, c0lo, c0hi, int(rec[p, 12]))
+        b1 = box_of(lc + 1, c1lo, c1hi, int(rec[p, 13]))
+        for ch, (lo, hi) in ((0, b0), (1, b1)):
+            got_lo = fb[p, [ch, 4 + ch, 8 + ch]]
+            got_hi = fb[p, [2 + ch, 6 + ch, 10 + ch]]
+            assert np.array_equal(got_lo, lo) and np.array_equal(got_hi, hi), (i, ch, got_lo, lo, got_hi, hi)
+        if np.isfinite(b0[1][axis]):
+            assert b0[1][axis] <= plane
+        if np.isfinite(b1[0][axis]):
+            assert b1[0][axis] >= plane
+        stats["pairs"] += 1
+        return np.minimum(b0[0], b1[0]), np.maximum(b0[1], b1[1])
+
+    inf = np.full(3, np.inf, np.float32)
+    root = int(rec[0, 13])
+    assert root != E and root & 3 != 3
+    box_of(1, -inf, inf, root)
+    assert stats["pairs"] > 200 and stats["cut"] > 100, stats
 
 
 def test_boxed_layout_4wide_structure():
@@ -333,16 +417,18 @@ def test_boxed_layout_4wide_structure():
 
 
 @pytest.mark.parametrize("width", [2, 4])
-@pytest.mark.parametrize("which", ["spheres", "random", "cornell"])
+@pytest.mark.parametrize("which", ["spheres", "random", "cornell", "cornell-sah", "terrain-sah"])
 def test_boxed_twin_equals_bruteforce(which, width):
     """The boxed node-step twin over the library's boxed layout of median
-    trees gives the brute-force closest hit on every ray, camera and
-    incoherent, and any-hit answers that threshold it, with fewer triangle
-    tests than the plane form."""
+    trees and of SAH KD trees (boxes cut to their cells) gives the
+    brute-force closest hit on every ray, camera and incoherent, and any-hit
+    answers that threshold it, with fewer triangle tests than the plane
+    form."""
     import nostrum_pray_amd as P
     s = {"spheres": lambda: P.Scene.spheres(1), "random": lambda: P.Scene.random(3, 5000),
-         "cornell": lambda: P.Scene.cornell(7, 8000)}[which]()
-    f = s.build(0).flat()
+         "cornell": lambda: P.Scene.cornell(7, 8000), "cornell-sah": lambda: P.Scene.cornell(7, 8000),
+         "terrain-sah": lambda: P.Scene.terrain(64, 9)}[which]()
+    f = (s.build(2, max_depth=40, leaf_size=8, traversal_cost=2.0) if which.endswith("-sah") else s.build(0)).flat()
     L = P.boxed_layout(f, width)
     rays = np.concatenate([s.prim_rays()[::41], O.gen_rays(13, P.scene_box_expanded(f["box"]), 6000)])
     for eps in (1e-4, 1e-9):
