@@ -117,7 +117,8 @@ def main():
     ap.add_argument("--det-eps", type=float, default=1e-12,
                     help="determinant cull; the reference's 1e-4 culls every terrain triangle")
     ap.add_argument("--layout", type=int, default=-1)
-    ap.add_argument("--max-depth", type=int, default=40, help="KD-tree depth cap (reference: 20)")
+    ap.add_argument("--max-depth", type=int, default=40,
+                    help="KD-tree depth cap of the trees built with this build's parameters (reference: 20)")
     ap.add_argument("--leaf-size", type=int, default=8, help="nodes this small are not split (reference: 32)")
     ap.add_argument("--traversal-cost", type=float, default=2.0, help="SAH traversal cost (reference: 5)")
     ap.add_argument("--no-autotune", action="store_true", help="trace with the default tuning")
@@ -126,10 +127,13 @@ def main():
                          "layout,stack_lds,leaf_ilp,fetch_min,occupancy,wg_waves,leaf_batch,node_levels "
                          "(a prefix of them)")
     ap.add_argument("--no-ref-tree", action="store_true",
-                    help="skip timing the same rays over the tree built with the reference's parameters")
-    ap.add_argument("--tree", choices=("cpu-sah", "cpu-sah-events", "gpu-sah"), default="cpu-sah",
-                    help="the timed tree: SAH KD tree built on the CPU with 64 bins per axis (default) or with the "
-                         "reference's event sweep, and uploaded (the workload's definition), or built on the GPU")
+                    help="skip timing the same rays over the tree built with the reference's parameters (when the "
+                         "timed tree is another)")
+    ap.add_argument("--tree", choices=("ref-sah", "cpu-sah", "cpu-sah-events", "gpu-sah"), default="ref-sah",
+                    help="the timed tree: the reference's SAH KD tree (its event sweep and parameters: depth cap 20, "
+                         "leaf size 32, costs 5 / 1.5; default), or one built with --max-depth, --leaf-size and "
+                         "--traversal-cost on the CPU with 64 bins per axis or the event sweep, all uploaded (the "
+                         "workload's definition), or on the GPU")
     ap.add_argument("--no-other-tree", action="store_true",
                     help="skip timing the same rays over the trees built the other ways")
     ap.add_argument("--no-ref-eps", action="store_true",
@@ -195,6 +199,18 @@ def main():
         print(json.dumps({"metric": "equal-RMSE time, radiance cache vs plain path tracing", "n_gpus": 1,
                           "data": "synthetic", "radiance_cache": out}), flush=True)
         return
+
+    # The ray tracer first, in a context of its own and before this process
+    # holds anything else, as a ray-tracing application's process would:
+    # measured after the terrain leg (its context, 8.4M-triangle scene and
+    # 2^24-ray buffers still held), the same frames took 0.19 ms longer on
+    # the host side (kernel times unchanged).
+    rt = None
+    if not args.no_rt:
+        rctx = P.Context(0)
+        configure(rctx, args)
+        rt = ray_tracer_leg(P, rctx, args)
+        rctx.close()
 
     t0 = time.time()
     scene = P.Scene.terrain(args.grid, args.seed)
@@ -338,7 +354,7 @@ def main():
     others = {}
     if not args.no_other_tree:
         for which in TREES:
-            if which == args.tree:
+            if which in (args.tree, "ref-sah"):  # the reference's tree has a leg of its own
                 continue
             osid, _, odesc, obuild, (od, on, orf), _ = build_tree(P, ctx, scene, args, which)
             if tuning is not None:
@@ -361,12 +377,13 @@ def main():
                 f"identical hits {same_o}")
 
     # The same rays over the tree built with the reference's parameters
-    # (depth cap 20, leaf size 32, SAH costs 5 / 1.5): identical hits, more
-    # triangle tests per ray.
+    # (depth cap 20, leaf size 32, SAH costs 5 / 1.5), when the timed tree is
+    # another: identical hits.
     ref_tree = None
-    if not args.no_ref_tree:
+    if not args.no_ref_tree and args.tree != "ref-sah":
         t1 = time.time()
         rtree = scene.build(1)
+        rdesc = TREES["ref-sah"]
         rsid = ctx.upload(rtree, shading=False)
         if tuning is not None:  # its own setting: its 32-triangle leaves favour other refill thresholds
             ctx.autotune(rsid, n_rays=n_rays_for_tuning(args), force=True)
@@ -379,20 +396,8 @@ def main():
         rtun = ctx.get_tuning(rsid)
         ctx.free_scene(rsid)
         ref_tree = {"ms_per_step": round(ms_ref, 4), "Mrays_per_s": round(n / (ms_ref * 1e-3) / 1e6, 2),
-                    "identical_hits": same, "tree": "SAH depth cap 20, leaf size 32, costs 5 / 1.5",
-                    "tuning": rtun}
+                    "identical_hits": same, "tree": rdesc, "tuning": rtun}
         log(f"reference-parameter tree: {ms_ref:.3f} ms/step, identical hits {same}")
-
-    # The ray tracer before the path-tracing legs: frames measured after both
-    # of them in the same process took 0.18 ms longer in the host-side copies
-    # (kernel times unchanged), which a ray-tracing process does not see.
-    rt = None
-    if not args.no_rt:
-        # a context of its own, as a ray-tracing application would hold
-        rctx = P.Context(0)
-        configure(rctx, args)
-        rt = ray_tracer_leg(P, rctx, args)
-        rctx.close()
 
     rc_out = None
     if not args.no_cache:
@@ -595,6 +600,8 @@ def batch_workload(P, args):
 
 
 TREES = {
+    "ref-sah": "the reference's SAH KD tree built on the CPU (its event sweep and parameters: depth cap 20, leaf "
+               "size 32, costs 5 / 1.5)",
     "cpu-sah": "SAH KD tree built on the CPU (64 bins per axis)",
     "cpu-sah-events": "SAH KD tree built on the CPU (the reference's event sweep)",
     "gpu-sah": "SAH KD tree built on the GPU (64 bins per axis)",
@@ -842,15 +849,21 @@ def build_tree(P, ctx, scene, args, which):
         return (sid, f"gpusah-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}",
                 f"{TREES[which]}, {params}", info,
                 (lay["max_depth"], len(lay["nodes"]) // 2, len(lay["refs"])), lay["box"])
-    kind, tag, method = (2, "binsah", "binned SAH, 64 bins per axis") if which == "cpu-sah" else \
-        (1, "sah", "exact-event SAH (the reference's sweep)")
     t1 = time.time()
-    tree = scene.build(kind, max_depth=args.max_depth, leaf_size=args.leaf_size, traversal_cost=args.traversal_cost)
+    if which == "ref-sah":
+        tree = scene.build(1)
+        tag, desc, method = "sah-d20-leaf32-ct5", TREES[which], "exact-event SAH (the reference's sweep)"
+    else:
+        kind, tag, method = (2, "binsah", "binned SAH, 64 bins per axis") if which == "cpu-sah" else \
+            (1, "sah", "exact-event SAH (the reference's sweep)")
+        tree = scene.build(kind, max_depth=args.max_depth, leaf_size=args.leaf_size,
+                           traversal_cost=args.traversal_cost)
+        tag = f"{tag}-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}"
+        desc = f"{TREES[which]}, {params}"
     build_s = time.time() - t1
     st = tree.stats()
     sid = ctx.upload(tree, shading=False)
-    return (sid, f"{tag}-d{args.max_depth}-leaf{args.leaf_size}-ct{args.traversal_cost:g}",
-            f"{TREES[which]}, {params}", {"where": "CPU", "method": method, "build_s": round(build_s, 2)},
+    return (sid, tag, desc, {"where": "CPU", "method": method, "build_s": round(build_s, 2)},
             (st["maxdepth"], st["flat_nodes"], st["flat_refs"]), tree.box())
 
 
@@ -966,8 +979,9 @@ def ray_tracer_leg(P, ctx, args):
     inf = scene.info()
     w, h = inf["width"], inf["height"]
     cam, lights, bg, eps = scene.camera(), scene.lights(), inf["background"], float(inf["det_eps"])
-    # the frame loop fills the same host image and hit buffers every frame
-    frame_out = (np.empty((h, w, 3), np.uint8), np.empty(w * h, np.int32))
+    # the frame loop fills the same page-locked host image and hit buffers
+    # every frame
+    frame_out = (ctx.host_array((h, w, 3), np.uint8), ctx.host_array(w * h, np.int32))
     for _ in range(max(1, args.warmup)):
         rgb, hits = ctx.render_rt(sid, cam, w, h, lights, bg, eps, want_hits=True, out=frame_out)
     times = []

@@ -434,6 +434,13 @@ PRAY_API int pray_render_pt_cache_ex(pray_ctx *ctx, int scene_id, int cache_id, 
  * context's stream with HIP events. */
 PRAY_API int pray_device_alloc(pray_ctx *ctx, uint64_t bytes, void **d_ptr);
 PRAY_API int pray_device_free(pray_ctx *ctx, void *d_ptr);
+/* Page-locked host memory (no reference counterpart) for results that every
+ * call writes back, such as a frame loop's image and hit buffers: the device
+ * copies into it directly, while a copy into pageable memory passes through
+ * the runtime's staging buffers.  pray_host_free takes no context, so memory
+ * may outlive the context that allocated it. */
+PRAY_API int pray_host_alloc(pray_ctx *ctx, uint64_t bytes, void **ptr);
+PRAY_API int pray_host_free(void *ptr);
 PRAY_API int pray_copy_to_device(pray_ctx *ctx, void *d_dst, const void *src, uint64_t bytes);
 PRAY_API int pray_copy_to_host(pray_ctx *ctx, void *dst, const void *d_src, uint64_t bytes);
 PRAY_API int pray_sync(pray_ctx *ctx);

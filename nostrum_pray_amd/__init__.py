@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -94,6 +95,8 @@ def libraries():
         "pray_render_pt": (C.c_int, [_vp, C.c_int, _vp, _u32, _u32, _u32, _u32, _u32, _u64, _vp, _f32, _vp, _vp]),
         "pray_device_alloc": (C.c_int, [_vp, _u64, C.POINTER(_vp)]),
         "pray_device_free": (C.c_int, [_vp, _vp]),
+        "pray_host_alloc": (C.c_int, [_vp, _u64, C.POINTER(_vp)]),
+        "pray_host_free": (C.c_int, [_vp]),
         "pray_copy_to_device": (C.c_int, [_vp, _vp, _vp, _u64]),
         "pray_copy_to_host": (C.c_int, [_vp, _vp, _vp, _u64]),
         "pray_sync": (C.c_int, [_vp]),
@@ -585,6 +588,19 @@ class Context:
         out = np.empty((width * height, 6), np.float32)
         self._check(self._lib.pray_camera_rays(self._h, _p(cam16), width, height, _p(out)))
         return out
+
+    def host_array(self, shape, dtype):
+        """A numpy array in page-locked host memory (pray_host_alloc), which
+        the device copies into directly; freed once the array and every view
+        of it are gone."""
+        dtype = np.dtype(dtype)
+        count = int(np.prod(shape))
+        nbytes = max(1, count * dtype.itemsize)
+        p = _vp()
+        self._check(self._lib.pray_host_alloc(self._h, nbytes, C.byref(p)))
+        buf = (C.c_uint8 * nbytes).from_address(p.value)
+        weakref.finalize(buf, self._lib.pray_host_free, p.value)
+        return np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
 
     def render_rt(self, sid, cam16, width, height, lights, background, det_eps=REFERENCE_DET_EPS, want_hits=False,
                   out=None):

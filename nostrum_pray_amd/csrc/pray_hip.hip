@@ -1534,6 +1534,19 @@ int pray_device_free(pray_ctx *c, void *p) {
     return PRAY_OK;
 }
 
+int pray_host_alloc(pray_ctx *c, uint64_t bytes, void **p) {
+    if (!c || !p) return fail(c, PRAY_ERR_INVALID, "null argument");
+    *p = nullptr;
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault));
+    return PRAY_OK;
+}
+
+int pray_host_free(void *p) {
+    if (!p) return PRAY_OK;
+    return hipHostFree(p) == hipSuccess ? PRAY_OK : PRAY_ERR_HIP;
+}
+
 int pray_copy_to_device(pray_ctx *c, void *dst, const void *src, uint64_t bytes) {
     if (!c || (bytes && (!dst || !src))) return fail(c, PRAY_ERR_INVALID, "null argument");
     if (!bytes) return PRAY_OK;

@@ -63,9 +63,12 @@ def test_rt_spheres_equal_oracle(ctx, seed):
         say(f"rt spheres {seed}")
         img = ctx.render_rt(sid, s.camera(), inf["width"], inf["height"], s.lights(), inf["background"],
                             P.REFERENCE_DET_EPS)
-        # a frame loop's reused buffers: filled over whatever they held
-        out = (np.full((inf["height"], inf["width"], 3), 7, np.uint8),
-               np.full(inf["width"] * inf["height"], 5, np.int32))
+        # a frame loop's reused buffers (page-locked): filled over whatever
+        # they held
+        out = (ctx.host_array((inf["height"], inf["width"], 3), np.uint8),
+               ctx.host_array(inf["width"] * inf["height"], np.int32))
+        out[0][:] = 7
+        out[1][:] = 5
         for _ in range(2):
             img2, hits2 = ctx.render_rt(sid, s.camera(), inf["width"], inf["height"], s.lights(), inf["background"],
                                         P.REFERENCE_DET_EPS, want_hits=True, out=out)
