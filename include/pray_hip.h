@@ -88,8 +88,7 @@ PRAY_API int pray_mem_ok(pray_ctx *ctx, uint64_t n_tris, uint64_t n_refs, uint64
  *   root_box   6 floats: min xyz, max xyz of the root cell
  *   max_depth  root-to-leaf depth of the tree (<= 64)
  * n_tris == 0 is a valid empty scene (every ray misses).  The upload also
- * makes the boxed layouts of pray_boxed_layout; for a tree with a children
- * pair whose two clips differ (a median tree) it selects the 4-wide one
+ * makes the boxed layouts of pray_boxed_layout and selects the 4-wide one
  * (node_levels 4). */
 PRAY_API int pray_upload_scene(pray_ctx *ctx, const uint32_t *nodes, uint32_t n_nodes, const uint32_t *leaf_refs,
                       uint32_t n_refs, const float *tri_a, const float *tri_ab, const float *tri_ac,
@@ -198,9 +197,7 @@ PRAY_API int pray_scene_layout(pray_ctx *ctx, int scene_id, uint64_t *sizes, uin
  *                  takes, PRAY_ERR_UNSUPPORTED for trees built on the
  *                  device); 4: it takes two binary levels from a 128-byte
  *                  record of up to four slots' boxes, over the layout's
- *                  4-wide form (the same trees; the default for trees with
- *                  a children pair whose clips differ -- median trees, whose
- *                  children's boxes overlap)
+ *                  4-wide form (the same trees; the upload's default)
  * pray_set_tuning: -1 keeps a value.  Calls of one context must not overlap
  * (traces share its ray counter and stack spill area). */
 #define PRAY_TUNING_FIELDS 8

@@ -639,14 +639,6 @@ struct BoxedBuilder {
     }
 };
 
-// Whether the uploaded tree is an object partition (a median tree): some
-// children pair after pair 0 has two different clips.
-bool unequal_clips(const uint32_t *nodes, uint32_t n_nodes) {
-    for (uint32_t p = 1; 2 * p + 1 < n_nodes; ++p)
-        if (nodes[4 * (size_t)p] != nodes[4 * (size_t)p + 2]) return true;
-    return false;
-}
-
 // Builds the layout of a validated tree.
 int make_boxed_layout(pray_ctx *c, const uint32_t *nodes, const uint32_t *refs, const float *A, const float *AB,
                       const float *AC, BoxedBuilder &B) {
@@ -978,9 +970,11 @@ int pray_upload_scene(pray_ctx *c, const uint32_t *nodes, uint32_t n_nodes, cons
         }
         if ((rc = build_quads(c, *sc, c->stream))) goto bad;
         if ((rc = build_boxes(c, *sc, nodes, n_nodes, leaf_refs, tri_a, tri_ab, tri_ac, c->stream))) goto bad;
-        // boxed node steps by default for median trees, 4-wide where that
-        // form exists
-        if (d.boxes && unequal_clips(nodes, n_nodes)) sc->tune.node_levels = d.boxes4 ? 4 : 3;
+        // boxed node steps by default, 4-wide where that form exists: over
+        // median trees far ahead of the clip planes; over SAH trees ahead or
+        // within a tenth of the KD records, and far ahead where leaves are
+        // large (the reference's 32-reference leaves)
+        if (d.boxes) sc->tune.node_levels = d.boxes4 ? 4 : 3;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         rc = fail(c, PRAY_ERR_HIP, "scene upload failed");

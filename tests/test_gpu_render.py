@@ -114,8 +114,9 @@ def test_rt_multi_light_negative_lambert_quirk(ctx):
 # (hipGraph replays, path budget): all 8 samples in one pass (the default
 # budget), passes of 3, 3 and 2 samples, and one sample per pass; with and
 # without graphs.
-# The last two run the room's median tree, which the upload gives the boxed
-# layout: the path tracer's bounces then take boxed node steps.
+# The bounces take the upload's default 4-wide boxed node steps, over the
+# room's SAH tree and (the last two) its median tree; the SAH cases without
+# graphs take the KD tree's plane-form node steps instead.
 @pytest.mark.parametrize("use_graph,budget,kind", [(1, None, 1), (0, None, 1), (1, 3 * 64 * 48, 1),
                                                    (0, 3 * 64 * 48, 1), (1, 0, 1), (0, 0, 1), (1, None, 0),
                                                    (0, 3 * 64 * 48, 0)])
@@ -127,7 +128,9 @@ def test_pt_cornell_equal_oracle(ctx, use_graph, budget, kind):
     w, h, spp, depth, rr = 64, 48, 8, 6, 2
     cam = O.camera([0, 1, 3.8], [0, 0, -1], 40, w, h)
     sid = ctx.upload(t)
-    assert ctx.get_tuning(sid)["node_levels"] == (4 if kind == 0 else 1)
+    assert ctx.get_tuning(sid)["node_levels"] == 4
+    if kind == 1 and not use_graph:
+        ctx.set_tuning(sid, node_levels=1)
     if budget is not None:
         ctx.set_path_budget(budget)
     try:

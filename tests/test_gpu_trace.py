@@ -211,7 +211,7 @@ def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb, width):
     rays = s.prim_rays() if rays_of == "camera" else O.gen_rays(4, P.scene_box_expanded(flat["box"]), 1 << 15)
     sid = ctx.upload(t)
     try:
-        assert ctx.get_tuning(sid)["node_levels"] == (1 if equal_clips(flat) else 4)
+        assert ctx.get_tuning(sid)["node_levels"] == 4
         ctx.set_tuning(sid, leaf_batch=lb, node_levels=3 if width == 2 else 4)
         say(f"counts boxed {name} {rays_of} leaf_batch={lb} width={width}")
         n = rays.shape[0]
@@ -237,19 +237,21 @@ def test_work_counts_boxed_match_cpu_twin(ctx, name, rays_of, lb, width):
 
 
 def test_node_records_by_tree(ctx):
-    """The upload's default node records: the 4-wide boxed layout for the
-    median tree whose children overlap, children pairs for SAH trees and for
-    the terrain's median tree (equal clips); every uploaded tree takes the
-    boxed forms when asked."""
-    for name, want in [("spheres-median", 4), ("spheres-sah", 1), ("terrain256-median", 1)]:
+    """The upload's default node records: the 4-wide boxed layout for every
+    tree (the median tree whose children overlap, SAH trees, the terrain's
+    median tree with equal clips); every uploaded tree takes the other
+    forms when asked, two-level records only where clips are equal."""
+    for name in ("spheres-median", "spheres-sah", "terrain256-median"):
         _, t, flat = scene(name)
         sid = ctx.upload(t)
         try:
-            assert ctx.get_tuning(sid)["node_levels"] == want, name
-            assert (want == 4) == (not equal_clips(flat))
-            for nl in (3, 4, 1):
+            assert ctx.get_tuning(sid)["node_levels"] == 4, name
+            for nl in (3, 4, 1) + ((2,) if equal_clips(flat) else ()):
                 ctx.set_tuning(sid, node_levels=nl)
                 assert ctx.get_tuning(sid)["node_levels"] == nl
+            if not equal_clips(flat):
+                with pytest.raises(P.PrayError):
+                    ctx.set_tuning(sid, node_levels=2)
         finally:
             ctx.free_scene(sid)
 

@@ -2,8 +2,9 @@
 the oracle:
   - closest and any hit of the 2^24-ray incoherent stream over the
     2048x2048-cell terrain (8,388,608 triangles) with the benchmark's tree
-    (binned SAH built on the CPU, depth cap 40, leaf 8, traversal cost 2) and
-    its autotuned setting: a prefix and a stride of the stream against the
+    (the reference's SAH KD tree built on the CPU: its event sweep, depth cap
+    20, leaf 32, costs 5 / 1.5) and its autotuned setting (4-wide boxed node
+    steps): a prefix and a stride of the stream against the
     CPU twin of the traversal, 4096 rays against brute force
     (bf_search.cpp:26-40), and the whole stream equal across settings;
   - the 100,248-triangle Cornell room path traced at depth 8 with roulette
@@ -20,17 +21,17 @@ from conftest import first_diff, say
 
 pytestmark = pytest.mark.gpu
 
-# The setting the autotuner picks for the benchmark's stream (two tree levels
-# per node step, ring of 6, ILP 2, refill at 16), compared with the oracle
-# first; the others must give the same hits.
-BENCH_TUNING = dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, occupancy=100, wg_waves=1,
-                    leaf_batch=0, node_levels=2)
+# The setting the autotuner picks for the benchmark's stream (4-wide boxed
+# node steps, ring of 8, ILP 2, refill at 24, leaf steps once 16 lanes
+# wait), compared with the oracle first; the others must give the same hits.
+BENCH_TUNING = dict(layout=P.LAYOUT_PACKED, stack_lds=8, leaf_ilp=2, fetch_min=24, occupancy=100, wg_waves=1,
+                    leaf_batch=16, node_levels=4)
 
 
 def test_terrain2048_full_stream_equals_oracle(ctx):
-    say("terrain 2048: scene and CPU binned-SAH tree")
+    say("terrain 2048: scene and the reference's SAH tree")
     scene = P.Scene.terrain(2048, 2024)
-    tree = scene.build(2, max_depth=40, leaf_size=8, traversal_cost=2.0)
+    tree = scene.build(1)
     flat = tree.flat()
     assert flat["a"].shape[0] == 8_388_608
     sid = ctx.upload(tree, shading=False)
@@ -52,7 +53,8 @@ def test_terrain2048_full_stream_equals_oracle(ctx):
                     dict(BENCH_TUNING, stack_lds=8, leaf_batch=16, node_levels=1),
                     dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=2, fetch_min=32, wg_waves=2, leaf_batch=0,
                          node_levels=1),
-                    dict(BENCH_TUNING, stack_lds=8, leaf_batch=16), dict(BENCH_TUNING, leaf_ilp=1)):
+                    dict(BENCH_TUNING, stack_lds=6, leaf_batch=0, node_levels=2), dict(BENCH_TUNING, leaf_ilp=1),
+                    dict(BENCH_TUNING, leaf_batch=0, node_levels=3)):
             ctx.set_tuning(sid, **tun)
             say(f"terrain 2048: closest hit, 2^24 rays, {tun}")
             ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, P.TRACE_CLOSEST, 1e-12)
@@ -77,11 +79,16 @@ def test_terrain2048_full_stream_equals_oracle(ctx):
         say("terrain 2048: any hit, 2^24 rays")
         ctx.trace_device(sid, d_rays.ptr, n, d_idx.ptr, d_t.ptr, P.TRACE_ANY, 1e-12, d_limit2=d_lim.ptr)
         ctx.sync()
-        ai = d_idx.numpy()
+        ai, at = d_idx.numpy(), d_t.numpy()
         assert np.array_equal(ai >= 0, first[0] >= 0)
         lim = np.full(len(sample), np.finfo(np.float32).max, np.float32)
         oa, _ = O.flat_trace(flat, rays[sample], 1, 1e-12, limit2=lim)
-        assert np.array_equal(ai[sample], oa)
+        assert np.array_equal(ai[sample] >= 0, oa >= 0)
+        # the boxed node steps meet the occluders in another order than the
+        # KD twin: every reported one is a genuine hit at the reported t
+        for k in np.flatnonzero(ai[:4096] >= 0)[:512]:
+            hit, tt = O.mt(rays[k], flat["a"][ai[k]], flat["ab"][ai[k]], flat["ac"][ai[k]], 1e-12)
+            assert hit and tt > 0 and tt == at[k], k
     finally:
         for b in bufs:
             b.free()
@@ -103,11 +110,16 @@ def test_cornell100k_depth8_equals_oracle(ctx, tree_kind):
     sid = ctx.upload(t)
     try:
         res = []
-        for tun in (dict(layout=P.LAYOUT_PACKED, stack_lds=4, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0),
-                    dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=1, fetch_min=32, wg_waves=2, leaf_batch=0),
-                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=16),
+        for tun in (dict(layout=P.LAYOUT_PACKED, stack_lds=4, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0,
+                         node_levels=1),
+                    dict(layout=P.LAYOUT_INDEXED, stack_lds=8, leaf_ilp=1, fetch_min=32, wg_waves=2, leaf_batch=0,
+                         node_levels=1),
+                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=16,
+                         node_levels=1),
                     dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0,
-                         node_levels=2)):
+                         node_levels=2),
+                    dict(layout=P.LAYOUT_PACKED, stack_lds=6, leaf_ilp=2, fetch_min=16, wg_waves=1, leaf_batch=0,
+                         node_levels=4)):
             ctx.set_tuning(sid, **tun)
             say(f"cornell 100k {tree_kind} {w}x{h} {spp} spp depth {depth} rr {rr} {tun}")
             hdr, rgb = ctx.render_pt(sid, cam, w, h, spp, depth, rr, seed, bg, 1e-9)
