@@ -99,7 +99,7 @@ PRAY_API int pray_free_scene(pray_ctx *ctx, int scene_id);
 /* The boxed layout the upload makes for boxed node steps (no reference
  * counterpart; host code only, no device or context needed).  For a valid
  * tree (the arrays of pray_upload_scene): every inner node becomes a pair
- * record of its two children's boxes, every leaf of more than 4 references
+ * record of its two children's boxes, every leaf of more than 2 references
  * is split further (triangles
  * sorted by centroid along the longest axis of their centroids' bounds, ties
  * by index, halved until no leaf holds more), and the leaves' runs go to a

@@ -229,7 +229,7 @@ def boxed_subtrees(L):
 def test_boxed_layout_structure():
     """The library's boxed layout (pray_boxed_layout, host code) of a median
     tree: it references every leaf reference of the tree once (a partition of
-    each leaf), its leaves hold at most 4, every child box is the union of its
+    each leaf), its leaves hold at most 2, every child box is the union of its
     triangles' boxes with the vertices a + ab, a + ac summed exactly (float64)
     and rounded outward -- the largest float at or below the exact minimum and
     the smallest at or above the exact maximum (a median tree's children
@@ -260,7 +260,7 @@ def test_boxed_layout_structure():
             assert (hi >= ehi).all() and (np.nextafter(hi, np.float32(-np.inf)) < ehi).all(), (p, ch)
             checked += 1
     assert checked > 1000
-    assert max(runs) <= 4 and sum(runs) >= len(refs)
+    assert max(runs) <= 2 and sum(runs) >= len(refs)
     # the root box is the scene's box up to that rounding
     assert np.allclose(fb[0, [1, 5, 9, 3, 7, 11]], f["box"], rtol=1e-6, atol=1e-6)
 
